@@ -1,0 +1,1149 @@
+// Host side of the MI355X SingleR annotation engine: the C-ABI declared in
+// include/cusingler_amd.h, device memory plans and kernel launches.
+//
+// Score step (get_label, src/cusingler.cu:986-1136): the reference rows are
+// ranked once; query rows are processed in chunks of Qc rows: rank, contract
+// against every reference row, per-label quantile select, then one label pass
+// over all cells.  Fine-tune (cufinetune, :1297-1333): level-synchronous over
+// all cells; each round groups the active cells by candidate set on the host,
+// and the device ranks, correlates and filters every group in batches.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cusingler_amd.h"
+#include "csg_ft_kernels.h"
+#include "csg_kernels.h"
+
+using namespace csg;
+
+namespace {
+
+struct PhaseEvents
+{
+    int        phase;
+    hipEvent_t a, b;
+    int        launches;
+};
+
+size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct csg_engine
+{
+    int             device = 0;
+    hipStream_t     stream = nullptr;
+    std::string     err;
+    hipDeviceProp_t prop{};
+
+    // ---- score step ----
+    bool      score_loaded = false;
+    uint32_t  Nr = 0, Nq = 0, G = 0, Kp = 0, L = 0;
+    int       nbins = 0;
+    std::vector<uint32_t> ctidx;
+    uint16_t *d_rdata = nullptr, *d_qdata = nullptr;
+    int *d_rind = nullptr, *d_rptr = nullptr, *d_qind = nullptr, *d_qptr = nullptr;
+    uint32_t* d_lstart = nullptr;
+    uint32_t* d_llen   = nullptr;
+    PctParam* d_pct    = nullptr;
+    float     pct_quantile = -1.f;
+    int16_t*  d_Xr = nullptr;
+    float*    d_sqr_f = nullptr;
+    long long* d_sqr_i = nullptr;
+    bool      ref_ranked = false;
+    uint32_t  Qc = 0;
+    int16_t*  d_Xq = nullptr;
+    float*    d_sqq_f = nullptr;
+    long long* d_sqq_i = nullptr;
+    float*    d_S = nullptr;
+    float*    d_P = nullptr;
+    int32_t*  d_first = nullptr;
+    float*    d_mask = nullptr;
+    bool      mask_valid = false;
+    uint64_t* d_nan_list = nullptr;
+    unsigned int* d_nan_count = nullptr;
+    int*      d_status = nullptr;
+
+    // ---- fine-tune ----
+    bool      ft_loaded = false;
+    uint32_t  fNr = 0, fNq = 0, Gf = 0, fL = 0;
+    int       fnbins = 0;
+    std::vector<uint32_t> fctidx, ctdidx, ctdiff;
+    std::vector<float>    host_mask;  // empty: use the device mask
+    uint16_t* d_fref_codes = nullptr;
+    uint16_t *d_fqdata = nullptr;
+    int *d_fqind = nullptr, *d_fqptr = nullptr;
+    PctParam* d_fpct = nullptr;
+    float     fpct_quantile = -1.f;
+
+    // ---- timing / counters ----
+    double  phase_ms[CSG_N_PHASES]  = {};
+    int64_t phase_cnt[CSG_N_PHASES] = {};
+    std::vector<PhaseEvents> pending;
+    std::vector<hipEvent_t>  event_pool;
+    int64_t counters[CSG_N_COUNTERS] = {};
+
+    int fail(int code, const std::string& m)
+    {
+        err = m;
+        return code;
+    }
+};
+
+#define HIPCHK(call)                                                                           \
+    do                                                                                         \
+    {                                                                                          \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return eng->fail(CSG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHK()                                                                            \
+    do                                                                                         \
+    {                                                                                          \
+        hipError_t e_ = hipGetLastError();                                                     \
+        if (e_ != hipSuccess)                                                                  \
+            return eng->fail(CSG_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+template <class T> void dfree(T*& p)
+{
+    if (p)
+        (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template <class T> int dalloc(csg_engine* eng, T** p, size_t count)
+{
+    dfree(*p);
+    if (count == 0)
+        count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess)
+    {
+        *p = nullptr;
+        return eng->fail(CSG_ERR_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                             " bytes: " + hipGetErrorString(e));
+    }
+    return CSG_OK;
+}
+
+template <class T> int upload(csg_engine* eng, T** p, const T* src, size_t count)
+{
+    int rc = dalloc(eng, p, count);
+    if (rc)
+        return rc;
+    if (count)
+        HIPCHK(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, eng->stream));
+    return CSG_OK;
+}
+
+hipEvent_t take_event(csg_engine* eng)
+{
+    if (!eng->event_pool.empty())
+    {
+        hipEvent_t e = eng->event_pool.back();
+        eng->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct PhaseScope
+{
+    csg_engine* eng;
+    PhaseEvents pe;
+    PhaseScope(csg_engine* e, int phase, int launches) : eng(e)
+    {
+        pe.phase    = phase;
+        pe.a        = take_event(e);
+        pe.b        = take_event(e);
+        pe.launches = launches;
+        (void)hipEventRecord(pe.a, e->stream);
+    }
+    ~PhaseScope()
+    {
+        (void)hipEventRecord(pe.b, eng->stream);
+        eng->pending.push_back(pe);
+    }
+};
+
+// Folds finished phase events into the totals (after a stream sync).
+void collect_phases(csg_engine* eng)
+{
+    for (auto& pe : eng->pending)
+    {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess)
+        {
+            eng->phase_ms[pe.phase] += ms;
+            eng->phase_cnt[pe.phase] += pe.launches;
+        }
+        eng->event_pool.push_back(pe.a);
+        eng->event_pool.push_back(pe.b);
+    }
+    eng->pending.clear();
+}
+
+int validate_csr(csg_engine* eng, const char* what, uint32_t rows, uint32_t width,
+                 const uint16_t* data, const int32_t* ind, const int32_t* ptr, uint64_t nbins)
+{
+    if (ptr[0] != 0)
+        return eng->fail(CSG_ERR_ARGUMENT, std::string(what) + ": indptr[0] != 0");
+    for (uint32_t r = 0; r < rows; ++r)
+        if (ptr[r + 1] < ptr[r])
+            return eng->fail(CSG_ERR_ARGUMENT, std::string(what) + ": indptr not monotone");
+    const int64_t nnz = ptr[rows];
+    for (int64_t i = 0; i < nnz; ++i)
+    {
+        if (ind[i] < 0 || (uint32_t)ind[i] >= width)
+            return eng->fail(CSG_ERR_ARGUMENT, std::string(what) + ": column index out of range");
+        if (data[i] == 0 || data[i] >= nbins)
+            return eng->fail(CSG_ERR_ARGUMENT,
+                             std::string(what) + ": value code outside [1, max_uniq_gene)");
+    }
+    return CSG_OK;
+}
+
+int check_ctidx(csg_engine* eng, uint32_t ref_height, uint32_t L, const uint32_t* ctidx)
+{
+    if (L == 0 || L > 256)
+        return eng->fail(CSG_ERR_ARGUMENT, "ct_num must be in [1, 256] (labels are uint8 codes)");
+    uint64_t pos = 0;
+    for (uint32_t l = 0; l < L; ++l)
+    {
+        if (ctidx[2 * l] != pos)
+            return eng->fail(CSG_ERR_ARGUMENT, "ctidx ranges are not contiguous");
+        if (ctidx[2 * l + 1] == 0)
+            return eng->fail(CSG_ERR_UNDEFINED,
+                             "label " + std::to_string(l) +
+                                 " has no reference cell (undefined in the reference)");
+        pos += ctidx[2 * l + 1];
+    }
+    if (pos != ref_height)
+        return eng->fail(CSG_ERR_ARGUMENT, "ctidx does not cover the reference rows");
+    return CSG_OK;
+}
+
+int set_pct(csg_engine* eng, PctParam** d, float* cached, const std::vector<uint32_t>& ctidx,
+            uint32_t L, float q)
+{
+    if (*d && *cached == q)
+        return CSG_OK;
+    std::vector<PctParam> pp(L);
+    for (uint32_t l = 0; l < L; ++l)
+        pp[l] = make_pct_param((long)ctidx[2 * l + 1], q);
+    int rc = upload(eng, d, pp.data(), pp.size());
+    if (rc)
+        return rc;
+    *cached = q;
+    return CSG_OK;
+}
+
+size_t rank_smem(int nbins, uint32_t Kp) { return (size_t)((nbins + 3) & ~3) * 4 + (size_t)Kp * 2; }
+
+// Runs the libstdc++ sort emulation on the NaN-carrying segments found by a
+// k_select launch.
+template <class Src>
+int resolve_nan_segments(csg_engine* eng, Src src, const std::vector<uint32_t>& seg_len_of_param,
+                         const float* scores, const PctParam* params, float p, float* out,
+                         const std::vector<uint64_t>* host_seg_len)
+{
+    unsigned int cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, eng->d_nan_count, sizeof(cnt), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    if (cnt == 0)
+        return CSG_OK;
+    eng->counters[5] += cnt;
+    std::vector<uint64_t> items(cnt);
+    HIPCHK(hipMemcpy(items.data(), eng->d_nan_list, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    (void)seg_len_of_param;
+    std::vector<uint64_t> off(cnt);
+    uint64_t              total = 0;
+    for (unsigned int i = 0; i < cnt; ++i)
+    {
+        off[i] = total;
+        total += (*host_seg_len)[items[i]];
+    }
+    float*    d_work = nullptr;
+    uint64_t* d_off  = nullptr;
+    int       rc     = dalloc(eng, &d_work, total);
+    if (rc)
+        return rc;
+    rc = upload(eng, &d_off, off.data(), off.size());
+    if (rc)
+    {
+        dfree(d_work);
+        return rc;
+    }
+    HIPCHK(hipMemsetAsync(eng->d_status, 0, sizeof(int), eng->stream));
+    k_select_nan<Src><<<(cnt + 63) / 64, 64, 0, eng->stream>>>(src, eng->d_nan_list, cnt, scores,
+                                                               params, p, out, d_work, d_off,
+                                                               eng->d_status);
+    LAUNCHCHK();
+    int status = 0;
+    HIPCHK(hipMemcpyAsync(&status, eng->d_status, sizeof(int), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    dfree(d_work);
+    dfree(d_off);
+    if (status)
+        return eng->fail(CSG_ERR_UNDEFINED,
+                         "a NaN-carrying score segment makes the reference's std::sort read out "
+                         "of bounds; its percentile is undefined");
+    return CSG_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C-ABI
+
+extern "C" {
+
+int csg_init_gpu(int gpuid, csg_engine** out)
+{
+    if (!out)
+        return CSG_ERR_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || gpuid < 0 || gpuid >= n)
+        return CSG_ERR_DEVICE;
+    csg_engine* eng = new csg_engine();
+    eng->device     = gpuid;
+    if (hipSetDevice(gpuid) != hipSuccess || hipGetDeviceProperties(&eng->prop, gpuid) != hipSuccess ||
+        hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
+    {
+        delete eng;
+        return CSG_ERR_DEVICE;
+    }
+    if (dalloc(eng, &eng->d_nan_count, 1) || dalloc(eng, &eng->d_status, 1))
+    {
+        delete eng;
+        return CSG_ERR_MEMORY;
+    }
+    *out = eng;
+    return CSG_OK;
+}
+
+const char* csg_last_error(const csg_engine* eng) { return eng ? eng->err.c_str() : "no engine"; }
+
+int csg_destroy_score(csg_engine* eng)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    (void)hipSetDevice(eng->device);
+    (void)hipStreamSynchronize(eng->stream);
+    dfree(eng->d_rdata);
+    dfree(eng->d_qdata);
+    dfree(eng->d_rind);
+    dfree(eng->d_rptr);
+    dfree(eng->d_qind);
+    dfree(eng->d_qptr);
+    dfree(eng->d_lstart);
+    dfree(eng->d_llen);
+    dfree(eng->d_pct);
+    dfree(eng->d_Xr);
+    dfree(eng->d_sqr_f);
+    dfree(eng->d_sqr_i);
+    dfree(eng->d_Xq);
+    dfree(eng->d_sqq_f);
+    dfree(eng->d_sqq_i);
+    dfree(eng->d_S);
+    dfree(eng->d_nan_list);
+    eng->pct_quantile = -1.f;
+    eng->score_loaded = false;
+    eng->ref_ranked   = false;
+    return CSG_OK;
+}
+
+int csg_destroy(csg_engine* eng)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    (void)hipSetDevice(eng->device);
+    (void)hipStreamSynchronize(eng->stream);
+    dfree(eng->d_fref_codes);
+    dfree(eng->d_fqdata);
+    dfree(eng->d_fqind);
+    dfree(eng->d_fqptr);
+    dfree(eng->d_fpct);
+    eng->fpct_quantile = -1.f;
+    eng->ft_loaded     = false;
+    eng->host_mask.clear();
+    return CSG_OK;
+}
+
+void csg_release(csg_engine* eng)
+{
+    if (!eng)
+        return;
+    csg_destroy_score(eng);
+    csg_destroy(eng);
+    dfree(eng->d_P);
+    dfree(eng->d_first);
+    dfree(eng->d_mask);
+    dfree(eng->d_nan_count);
+    dfree(eng->d_status);
+    collect_phases(eng);
+    for (auto e : eng->event_pool)
+        (void)hipEventDestroy(e);
+    if (eng->stream)
+        (void)hipStreamDestroy(eng->stream);
+    delete eng;
+}
+
+int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint16_t* ref_data,
+                     const int32_t* ref_indices, const int32_t* ref_indptr, uint32_t qry_height,
+                     const uint16_t* qry_data, const int32_t* qry_indices,
+                     const int32_t* qry_indptr, uint32_t ct_num, const uint32_t* ctidx,
+                     uint64_t max_uniq_gene)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    HIPCHK(hipSetDevice(eng->device));
+    csg_destroy_score(eng);
+    if (width == 0 || width > 32767)
+        return eng->fail(CSG_ERR_ARGUMENT, "score gene set size must be in [1, 32767]");
+    if (ref_height == 0 || qry_height == 0)
+        return eng->fail(CSG_ERR_ARGUMENT, "empty reference or query matrix");
+    if (max_uniq_gene < 1 || max_uniq_gene > 16384)
+        return eng->fail(CSG_ERR_ARGUMENT, "max_uniq_gene must be in [1, 16384]");
+    int rc = check_ctidx(eng, ref_height, ct_num, ctidx);
+    if (rc)
+        return rc;
+    if ((rc = validate_csr(eng, "reference", ref_height, width, ref_data, ref_indices, ref_indptr,
+                           max_uniq_gene)) ||
+        (rc = validate_csr(eng, "query", qry_height, width, qry_data, qry_indices, qry_indptr,
+                           max_uniq_gene)))
+        return rc;
+    eng->Nr    = ref_height;
+    eng->Nq    = qry_height;
+    eng->G     = width;
+    eng->Kp    = (uint32_t)round_up(width, 64);
+    eng->L     = ct_num;
+    eng->nbins = (int)max_uniq_gene;
+    if (rank_smem(eng->nbins, eng->Kp) > 65536)
+        return eng->fail(CSG_ERR_ARGUMENT, "gene set and code range exceed the rank kernel's LDS");
+    eng->ctidx.assign(ctidx, ctidx + 2 * ct_num);
+    const size_t rnnz = (size_t)ref_indptr[ref_height], qnnz = (size_t)qry_indptr[qry_height];
+    if ((rc = upload(eng, &eng->d_rdata, ref_data, rnnz)) ||
+        (rc = upload(eng, &eng->d_rind, ref_indices, rnnz)) ||
+        (rc = upload(eng, &eng->d_rptr, ref_indptr, (size_t)ref_height + 1)) ||
+        (rc = upload(eng, &eng->d_qdata, qry_data, qnnz)) ||
+        (rc = upload(eng, &eng->d_qind, qry_indices, qnnz)) ||
+        (rc = upload(eng, &eng->d_qptr, qry_indptr, (size_t)qry_height + 1)))
+        return rc;
+    std::vector<uint32_t> ls(ct_num), ll(ct_num);
+    for (uint32_t l = 0; l < ct_num; ++l)
+    {
+        ls[l] = ctidx[2 * l];
+        ll[l] = ctidx[2 * l + 1];
+    }
+    if ((rc = upload(eng, &eng->d_lstart, ls.data(), ls.size())) ||
+        (rc = upload(eng, &eng->d_llen, ll.data(), ll.size())))
+        return rc;
+    // reference ranks (rows padded to the 128-row tile)
+    const size_t nr_pad = round_up(ref_height, 128);
+    if ((rc = dalloc(eng, &eng->d_Xr, nr_pad * eng->Kp)) ||
+        (rc = dalloc(eng, &eng->d_sqr_f, nr_pad)) || (rc = dalloc(eng, &eng->d_sqr_i, nr_pad)))
+        return rc;
+    HIPCHK(hipMemsetAsync(eng->d_Xr, 0, nr_pad * eng->Kp * sizeof(int16_t), eng->stream));
+    // query chunk: the score chunk holds at most ~2 GiB
+    const size_t budget = (size_t)2 << 30;
+    size_t       qc     = budget / ((size_t)ref_height * sizeof(float));
+    qc                  = std::max<size_t>(128, qc / 128 * 128);
+    qc                  = std::min<size_t>(qc, round_up(qry_height, 128));
+    eng->Qc             = (uint32_t)qc;
+    if ((rc = dalloc(eng, &eng->d_Xq, qc * eng->Kp)) || (rc = dalloc(eng, &eng->d_sqq_f, qc)) ||
+        (rc = dalloc(eng, &eng->d_sqq_i, qc)) || (rc = dalloc(eng, &eng->d_S, qc * ref_height)) ||
+        (rc = dalloc(eng, &eng->d_nan_list, qc * ct_num)) ||
+        (rc = dalloc(eng, &eng->d_P, (size_t)qry_height * ct_num)) ||
+        (rc = dalloc(eng, &eng->d_first, qry_height)) ||
+        (rc = dalloc(eng, &eng->d_mask, (size_t)qry_height * ct_num)))
+        return rc;
+    HIPCHK(hipMemsetAsync(eng->d_Xq, 0, qc * eng->Kp * sizeof(int16_t), eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    eng->mask_valid   = false;
+    eng->score_loaded = true;
+    return CSG_OK;
+}
+
+namespace {
+
+int rank_rows(csg_engine* eng, const uint16_t* data, const int* ind, const int* ptr, uint32_t row0,
+              uint32_t nrows, int16_t* X, float* sq_f, long long* sq_i)
+{
+    const size_t smem = rank_smem(eng->nbins, eng->Kp);
+    k_rank_csr<256><<<nrows, 256, smem, eng->stream>>>(data, ind, ptr, row0, nrows, eng->G, eng->Kp,
+                                                       eng->nbins, X, sq_f, sq_i);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+int score_chunk_plain(csg_engine* eng, uint32_t q0, uint32_t nq)
+{
+    dim3 grid((eng->Nr + PG_BN - 1) / PG_BN, (nq + PG_BM - 1) / PG_BM);
+    k_gemm_plain<<<grid, 256, 0, eng->stream>>>(eng->d_Xq, eng->d_Xr, (int)eng->Kp, eng->d_sqq_f,
+                                                eng->d_sqr_f, (int)nq, (int)eng->Nr, eng->d_S,
+                                                (size_t)eng->Nr);
+    LAUNCHCHK();
+    (void)q0;
+    return CSG_OK;
+}
+
+}  // namespace
+
+int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path,
+                  int32_t* first_labels, float* labels_mask, float* scores)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->score_loaded)
+        return eng->fail(CSG_ERR_STATE, "csg_get_label before csg_copyin_score");
+    if (path != CSG_PATH_PLAIN)
+        return eng->fail(CSG_ERR_ARGUMENT, "correlation path not available in this build");
+    HIPCHK(hipSetDevice(eng->device));
+    int rc = set_pct(eng, &eng->d_pct, &eng->pct_quantile, eng->ctidx, eng->L, quantile);
+    if (rc)
+        return rc;
+    std::memset(eng->counters, 0, sizeof(eng->counters));
+    eng->counters[0] = eng->Nq;
+    eng->counters[1] = eng->G;
+    eng->counters[2] = eng->Nr;
+    if (!eng->ref_ranked)
+    {
+        PhaseScope ps(eng, 0, 1);
+        if ((rc = rank_rows(eng, eng->d_rdata, eng->d_rind, eng->d_rptr, 0, eng->Nr, eng->d_Xr,
+                            eng->d_sqr_f, eng->d_sqr_i)))
+            return rc;
+        eng->ref_ranked = true;
+    }
+    // host copy of segment lengths per score item is only needed for NaN work sizes
+    std::vector<uint64_t> seg_len;
+    for (uint32_t q0 = 0; q0 < eng->Nq; q0 += eng->Qc)
+    {
+        const uint32_t nq = std::min(eng->Qc, eng->Nq - q0);
+        {
+            PhaseScope ps(eng, 0, 1);
+            if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
+                                eng->d_sqq_f, eng->d_sqq_i)))
+                return rc;
+        }
+        {
+            PhaseScope ps(eng, 1, 1);
+            if ((rc = score_chunk_plain(eng, q0, nq)))
+                return rc;
+        }
+        HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
+        ScoreSegments src{ eng->d_lstart, eng->d_llen, eng->L, (uint64_t)eng->Nr };
+        const uint64_t n_items = (uint64_t)nq * eng->L;
+        {
+            PhaseScope ps(eng, 2, 1);
+            k_select<ScoreSegments><<<(unsigned)((n_items + 3) / 4), 256, 0, eng->stream>>>(
+                src, n_items, eng->d_S, eng->d_pct, quantile, eng->d_P + (size_t)q0 * eng->L,
+                eng->d_nan_list, eng->d_nan_count);
+            LAUNCHCHK();
+        }
+        if (seg_len.size() != n_items)
+        {
+            seg_len.resize(n_items);
+            for (uint64_t i = 0; i < n_items; ++i)
+                seg_len[i] = eng->ctidx[2 * (i % eng->L) + 1];
+        }
+        if ((rc = resolve_nan_segments(eng, src, {}, eng->d_S, eng->d_pct, quantile,
+                                       eng->d_P + (size_t)q0 * eng->L, &seg_len)))
+            return rc;
+    }
+    {
+        PhaseScope ps(eng, 3, 1);
+        k_label<<<(eng->Nq + 255) / 256, 256, 0, eng->stream>>>(eng->d_P, eng->Nq, eng->L,
+                                                                finetune_thre, eng->d_first,
+                                                                eng->d_mask);
+        LAUNCHCHK();
+    }
+    const size_t nql = (size_t)eng->Nq * eng->L;
+    if (first_labels)
+        HIPCHK(hipMemcpyAsync(first_labels, eng->d_first, eng->Nq * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, eng->stream));
+    if (labels_mask)
+        HIPCHK(hipMemcpyAsync(labels_mask, eng->d_mask, nql * sizeof(float), hipMemcpyDeviceToHost,
+                              eng->stream));
+    if (scores)
+        HIPCHK(hipMemcpyAsync(scores, eng->d_P, nql * sizeof(float), hipMemcpyDeviceToHost,
+                              eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    collect_phases(eng);
+    eng->mask_valid = true;
+    return CSG_OK;
+}
+
+// ================================================================ fine-tune
+
+namespace {
+
+__global__ void k_densify(const uint16_t* __restrict__ data, const int* __restrict__ ind,
+                          const int* __restrict__ ptr, uint32_t nrows, uint32_t width,
+                          uint16_t* __restrict__ out)
+{
+    const uint32_t r = blockIdx.x;
+    if (r >= nrows)
+        return;
+    uint16_t* o = out + (size_t)r * width;
+    for (uint32_t g = threadIdx.x; g < width; g += blockDim.x)
+        o[g] = 0;
+    __syncthreads();
+    for (int i = ptr[r] + (int)threadIdx.x; i < ptr[r + 1]; i += blockDim.x)
+        o[ind[i]] = data[i];
+}
+
+}  // namespace
+
+int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint16_t* ref_data,
+               const int32_t* ref_indices, const int32_t* ref_indptr, uint32_t qry_height,
+               const uint16_t* qry_data, const int32_t* qry_indices, const int32_t* qry_indptr,
+               uint32_t ct_num, const uint32_t* ctidx, const uint32_t* ctdidx,
+               const uint32_t* ctdiff, uint64_t n_ctdiff, uint64_t max_uniq_gene,
+               const float* labels_mask)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    HIPCHK(hipSetDevice(eng->device));
+    csg_destroy(eng);
+    if (width > 32767)
+        return eng->fail(CSG_ERR_ARGUMENT, "fine-tune gene set size must be at most 32767");
+    if (ref_height == 0 || qry_height == 0)
+        return eng->fail(CSG_ERR_ARGUMENT, "empty reference or query matrix");
+    if (max_uniq_gene < 1 || max_uniq_gene > 16384)
+        return eng->fail(CSG_ERR_ARGUMENT, "max_uniq_gene must be in [1, 16384]");
+    int rc = check_ctidx(eng, ref_height, ct_num, ctidx);
+    if (rc)
+        return rc;
+    const uint32_t w = std::max<uint32_t>(width, 1);
+    if ((rc = validate_csr(eng, "reference", ref_height, w, ref_data, ref_indices, ref_indptr,
+                           max_uniq_gene)) ||
+        (rc = validate_csr(eng, "query", qry_height, w, qry_data, qry_indices, qry_indptr,
+                           max_uniq_gene)))
+        return rc;
+    for (uint64_t i = 0; i < (uint64_t)ct_num * ct_num; ++i)
+        if ((uint64_t)ctdidx[2 * i] + ctdidx[2 * i + 1] > n_ctdiff)
+            return eng->fail(CSG_ERR_ARGUMENT, "ctdidx window outside ctdiff");
+    for (uint64_t i = 0; i < n_ctdiff; ++i)
+        if (ctdiff[i] >= w)
+            return eng->fail(CSG_ERR_ARGUMENT, "ctdiff gene index outside the fine-tune gene set");
+    eng->fNr    = ref_height;
+    eng->fNq    = qry_height;
+    eng->Gf     = width;
+    eng->fL     = ct_num;
+    eng->fnbins = (int)max_uniq_gene;
+    eng->fctidx.assign(ctidx, ctidx + 2 * ct_num);
+    eng->ctdidx.assign(ctdidx, ctdidx + 2 * (size_t)ct_num * ct_num);
+    eng->ctdiff.assign(ctdiff, ctdiff + n_ctdiff);
+    if (labels_mask)
+        eng->host_mask.assign(labels_mask, labels_mask + (size_t)qry_height * ct_num);
+    const size_t rnnz = (size_t)ref_indptr[ref_height], qnnz = (size_t)qry_indptr[qry_height];
+    uint16_t* d_rd = nullptr;
+    int *     d_ri = nullptr, *d_rp = nullptr;
+    if ((rc = upload(eng, &d_rd, ref_data, rnnz)) || (rc = upload(eng, &d_ri, ref_indices, rnnz)) ||
+        (rc = upload(eng, &d_rp, ref_indptr, (size_t)ref_height + 1)) ||
+        (rc = dalloc(eng, &eng->d_fref_codes, (size_t)ref_height * w)) ||
+        (rc = upload(eng, &eng->d_fqdata, qry_data, qnnz)) ||
+        (rc = upload(eng, &eng->d_fqind, qry_indices, qnnz)) ||
+        (rc = upload(eng, &eng->d_fqptr, qry_indptr, (size_t)qry_height + 1)))
+    {
+        dfree(d_rd);
+        dfree(d_ri);
+        dfree(d_rp);
+        return rc;
+    }
+    k_densify<<<ref_height, 256, 0, eng->stream>>>(d_rd, d_ri, d_rp, ref_height, w,
+                                                   eng->d_fref_codes);
+    hipError_t le = hipGetLastError();
+    hipError_t se = hipStreamSynchronize(eng->stream);
+    dfree(d_rd);
+    dfree(d_ri);
+    dfree(d_rp);
+    if (le != hipSuccess || se != hipSuccess)
+        return eng->fail(CSG_ERR_DEVICE, std::string("densify: ") +
+                                             hipGetErrorString(le != hipSuccess ? le : se));
+    eng->ft_loaded = true;
+    return CSG_OK;
+}
+
+namespace {
+
+struct FtBatch
+{
+    std::vector<FtGroup>    groups;
+    std::vector<uint32_t>   genes;
+    std::vector<uint32_t>   labels;
+    std::vector<FtRankItem> ref_items, cell_items;
+    std::vector<FtCorrTile> tiles;
+    std::vector<FtCell>     cells;
+    std::vector<Segment>    segs;
+    std::vector<uint64_t>   seg_len;
+    std::vector<uint8_t>    in_lists;
+    std::vector<uint32_t>   in_counts;
+    std::vector<uint32_t>   cell_ids;  // query row per cell item
+    uint64_t ref_elems = 0, cell_elems = 0, n_scores = 0;
+    void clear()
+    {
+        groups.clear();
+        genes.clear();
+        labels.clear();
+        ref_items.clear();
+        cell_items.clear();
+        tiles.clear();
+        cells.clear();
+        segs.clear();
+        seg_len.clear();
+        in_lists.clear();
+        in_counts.clear();
+        cell_ids.clear();
+        ref_elems = cell_elems = n_scores = 0;
+    }
+};
+
+constexpr int FT_TC = 16, FT_TR = 16;
+
+// Uploads a batch, runs rank / correlate / select / filter, and writes the
+// new candidate lists of the batch's cells into `lists` / `counts`.
+int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
+                 std::vector<uint8_t>& lists, std::vector<uint32_t>& counts)
+{
+    const uint32_t L = eng->fL;
+    FtGroup*    d_groups = nullptr;
+    uint32_t*   d_genes = nullptr, *d_labels = nullptr, *d_in_counts = nullptr, *d_out_counts = nullptr;
+    FtRankItem *d_ritems = nullptr, *d_citems = nullptr;
+    FtCorrTile* d_tiles = nullptr;
+    FtCell*     d_cells = nullptr;
+    Segment*    d_segs = nullptr;
+    uint8_t *   d_in_lists = nullptr, *d_out_lists = nullptr;
+    int16_t *   d_rr = nullptr, *d_cr = nullptr;
+    float *     d_rsq = nullptr, *d_csq = nullptr, *d_scores = nullptr, *d_P = nullptr;
+    uint64_t*   d_nan_list = nullptr;
+    int         rc = CSG_OK;
+    auto cleanup = [&]() {
+        dfree(d_groups);
+        dfree(d_genes);
+        dfree(d_labels);
+        dfree(d_in_counts);
+        dfree(d_out_counts);
+        dfree(d_ritems);
+        dfree(d_citems);
+        dfree(d_tiles);
+        dfree(d_cells);
+        dfree(d_segs);
+        dfree(d_in_lists);
+        dfree(d_out_lists);
+        dfree(d_rr);
+        dfree(d_cr);
+        dfree(d_rsq);
+        dfree(d_csq);
+        dfree(d_scores);
+        dfree(d_P);
+        dfree(d_nan_list);
+    };
+    const uint32_t ncell = (uint32_t)b.cells.size();
+    if ((rc = upload(eng, &d_groups, b.groups.data(), b.groups.size())) ||
+        (rc = upload(eng, &d_genes, b.genes.data(), b.genes.size())) ||
+        (rc = upload(eng, &d_labels, b.labels.data(), b.labels.size())) ||
+        (rc = upload(eng, &d_ritems, b.ref_items.data(), b.ref_items.size())) ||
+        (rc = upload(eng, &d_citems, b.cell_items.data(), b.cell_items.size())) ||
+        (rc = upload(eng, &d_tiles, b.tiles.data(), b.tiles.size())) ||
+        (rc = upload(eng, &d_cells, b.cells.data(), b.cells.size())) ||
+        (rc = upload(eng, &d_segs, b.segs.data(), b.segs.size())) ||
+        (rc = upload(eng, &d_in_lists, b.in_lists.data(), b.in_lists.size())) ||
+        (rc = upload(eng, &d_in_counts, b.in_counts.data(), b.in_counts.size())) ||
+        (rc = dalloc(eng, &d_out_lists, (size_t)ncell * L)) ||
+        (rc = dalloc(eng, &d_out_counts, ncell)) || (rc = dalloc(eng, &d_rr, b.ref_elems)) ||
+        (rc = dalloc(eng, &d_cr, b.cell_elems)) || (rc = dalloc(eng, &d_rsq, b.ref_items.size())) ||
+        (rc = dalloc(eng, &d_csq, b.cell_items.size())) ||
+        (rc = dalloc(eng, &d_scores, b.n_scores)) || (rc = dalloc(eng, &d_P, b.segs.size())) ||
+        (rc = dalloc(eng, &d_nan_list, b.segs.size())))
+    {
+        cleanup();
+        return rc;
+    }
+    uint32_t max_gs = 0;
+    for (auto& g : b.groups)
+        max_gs = std::max(max_gs, g.gs);
+    const size_t stride_max = round_up(max_gs, 8);
+    {
+        PhaseScope ps(eng, 7, 2);
+        const size_t smem_r = (size_t)((eng->fnbins + 3) & ~3) * 4 + stride_max * 4;
+        k_ft_rank_ref<<<(unsigned)b.ref_items.size(), 128, smem_r, eng->stream>>>(
+            d_ritems, d_groups, d_genes, eng->d_fref_codes, std::max<uint32_t>(eng->Gf, 1),
+            eng->fnbins, d_rr, d_rsq);
+        hipError_t e1 = hipGetLastError();
+        const size_t smem_q = (size_t)((eng->fnbins + 3) & ~3) * 4 +
+                              round_up(std::max<uint32_t>(eng->Gf, 1), 8) * 2 + stride_max * 4;
+        k_ft_rank_qry<<<(unsigned)b.cell_items.size(), 128, smem_q, eng->stream>>>(
+            d_citems, d_groups, d_genes, eng->d_fqdata, eng->d_fqind, eng->d_fqptr,
+            std::max<uint32_t>(eng->Gf, 1), eng->fnbins, d_cr, d_csq);
+        hipError_t e2 = hipGetLastError();
+        if (e1 != hipSuccess || e2 != hipSuccess)
+        {
+            cleanup();
+            return eng->fail(CSG_ERR_DEVICE, std::string("fine-tune rank launch: ") +
+                                                 hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+        }
+    }
+    {
+        PhaseScope ps(eng, 6, 1);
+        const size_t smem = (size_t)(FT_TC + FT_TR) * stride_max * 2;
+        k_ft_corr<FT_TC, FT_TR><<<(unsigned)b.tiles.size(), FT_TC * FT_TR, smem, eng->stream>>>(
+            d_tiles, d_groups, d_cr, d_csq, d_rr, d_rsq, d_scores);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+        {
+            cleanup();
+            return eng->fail(CSG_ERR_DEVICE, std::string("fine-tune corr launch: ") +
+                                                 hipGetErrorString(e));
+        }
+    }
+    {
+        PhaseScope ps(eng, 8, 1);
+        (void)hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream);
+        TableSegments src{ d_segs };
+        const uint64_t n = b.segs.size();
+        k_select<TableSegments><<<(unsigned)((n + 3) / 4), 256, 0, eng->stream>>>(
+            src, n, d_scores, eng->d_fpct, quantile, d_P, d_nan_list, eng->d_nan_count);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+        {
+            cleanup();
+            return eng->fail(CSG_ERR_DEVICE, std::string("fine-tune select launch: ") +
+                                                 hipGetErrorString(e));
+        }
+    }
+    {
+        std::swap(eng->d_nan_list, d_nan_list);
+        rc = resolve_nan_segments(eng, TableSegments{ d_segs }, {}, d_scores, eng->d_fpct, quantile,
+                                  d_P, &b.seg_len);
+        std::swap(eng->d_nan_list, d_nan_list);
+        if (rc)
+        {
+            cleanup();
+            return rc;
+        }
+    }
+    {
+        PhaseScope ps(eng, 8, 1);
+        k_ft_filter<<<(ncell + 127) / 128, 128, 0, eng->stream>>>(
+            d_cells, ncell, d_groups, d_labels, d_P, d_in_lists, d_in_counts, L, thre, d_out_lists,
+            d_out_counts);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+        {
+            cleanup();
+            return eng->fail(CSG_ERR_DEVICE, std::string("fine-tune filter launch: ") +
+                                                 hipGetErrorString(e));
+        }
+    }
+    std::vector<uint8_t>  ol((size_t)ncell * L);
+    std::vector<uint32_t> oc(ncell);
+    hipError_t e1 = hipMemcpyAsync(ol.data(), d_out_lists, ol.size(), hipMemcpyDeviceToHost, eng->stream);
+    hipError_t e2 = hipMemcpyAsync(oc.data(), d_out_counts, oc.size() * 4, hipMemcpyDeviceToHost,
+                                   eng->stream);
+    hipError_t e3 = hipStreamSynchronize(eng->stream);
+    cleanup();
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        return eng->fail(CSG_ERR_DEVICE, "fine-tune result copy failed");
+    for (uint32_t i = 0; i < ncell; ++i)
+    {
+        const uint32_t c = b.cell_ids[i];
+        counts[c]        = oc[i];
+        std::memcpy(&lists[(size_t)c * L], &ol[(size_t)i * L], L);
+    }
+    return CSG_OK;
+}
+
+}  // namespace
+
+int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
+                 uint32_t* final_labels, int32_t* rounds_out)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->ft_loaded)
+        return eng->fail(CSG_ERR_STATE, "csg_finetune before csg_copyin");
+    HIPCHK(hipSetDevice(eng->device));
+    const uint32_t L  = eng->fL;
+    const uint32_t Nq = eng->fNq;
+    int            rc = set_pct(eng, &eng->d_fpct, &eng->fpct_quantile, eng->fctidx, L, quantile);
+    if (rc)
+        return rc;
+    for (int i = 6; i < CSG_N_COUNTERS; ++i)
+        eng->counters[i] = 0;
+    PhaseScope total(eng, 5, 0);
+    // candidate lists from the mask (ascending label order, src/cusingler.cu:1307-1312)
+    std::vector<float> maskbuf;
+    const float*       mask = nullptr;
+    if (!eng->host_mask.empty())
+        mask = eng->host_mask.data();
+    else
+    {
+        if (!eng->mask_valid || eng->Nq != Nq || eng->L != L)
+            return eng->fail(CSG_ERR_STATE, "no candidate mask: pass one to csg_copyin or run "
+                                            "csg_get_label on the same cells first");
+        maskbuf.resize((size_t)Nq * L);
+        HIPCHK(hipMemcpyAsync(maskbuf.data(), eng->d_mask, maskbuf.size() * 4,
+                              hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        mask = maskbuf.data();
+    }
+    std::vector<uint8_t>  lists((size_t)Nq * L);
+    std::vector<uint32_t> counts(Nq, 0);
+    std::vector<int32_t>  nrounds(Nq, 0);
+    for (uint32_t c = 0; c < Nq; ++c)
+        for (uint32_t l = 0; l < L; ++l)
+            if (mask[(size_t)c * L + l] != 0)
+                lists[(size_t)c * L + counts[c]++] = (uint8_t)l;
+
+    const size_t           words = (L + 63) / 64;
+    std::vector<uint32_t>  mark(std::max<uint32_t>(eng->Gf, 1), 0);
+    uint32_t               stamp = 0;
+    const uint64_t         budget_elems = (uint64_t)1 << 28;  // per batch (scores + ranks)
+    FtBatch                b;
+
+    while (true)
+    {
+        // active cells grouped by candidate set
+        std::unordered_map<std::string, uint32_t> gid;
+        std::vector<std::vector<uint32_t>>        gcells;
+        std::vector<std::string>                  gkey;
+        for (uint32_t c = 0; c < Nq; ++c)
+        {
+            if (counts[c] <= 1)
+                continue;
+            if (finetune_times != 0 && nrounds[c] >= finetune_times)
+                continue;
+            std::string key(words * 8, '\0');
+            uint64_t*   kw = (uint64_t*)&key[0];
+            for (uint32_t t = 0; t < counts[c]; ++t)
+            {
+                const uint32_t l = lists[(size_t)c * L + t];
+                kw[l / 64] |= 1ull << (l % 64);
+            }
+            auto it = gid.find(key);
+            if (it == gid.end())
+            {
+                it = gid.emplace(key, (uint32_t)gcells.size()).first;
+                gcells.emplace_back();
+                gkey.push_back(key);
+            }
+            gcells[it->second].push_back(c);
+        }
+        if (gcells.empty())
+            break;
+        eng->counters[6]++;
+        b.clear();
+        auto flush = [&]() -> int {
+            if (b.groups.empty())
+                return CSG_OK;
+            int r = run_ft_batch(eng, b, quantile, finetune_thre, lists, counts);
+            b.clear();
+            return r;
+        };
+        for (size_t gi = 0; gi < gcells.size(); ++gi)
+        {
+            const uint64_t* kw = (const uint64_t*)gkey[gi].data();
+            std::vector<uint32_t> labs;
+            for (uint32_t l = 0; l < L; ++l)
+                if (kw[l / 64] >> (l % 64) & 1)
+                    labs.push_back(l);
+            const int k = (int)labs.size();
+            // G_sub: union of each ordered pair's first min(len, gene_thre)
+            // genes (src/cusingler.cu:1140-1156)
+            const int gene_thre = (int)std::round(500 * std::pow((2 / 3.0), std::log2((double)k)));
+            ++stamp;
+            std::vector<uint32_t> gsub;
+            for (uint32_t i : labs)
+                for (uint32_t j : labs)
+                {
+                    if (i == j)
+                        continue;
+                    const uint32_t pos = eng->ctdidx[((size_t)i * L + j) * 2];
+                    int            len = (int)eng->ctdidx[((size_t)i * L + j) * 2 + 1];
+                    if (len > gene_thre)
+                        len = gene_thre;
+                    for (int t = 0; t < len; ++t)
+                    {
+                        const uint32_t gg = eng->ctdiff[pos + t];
+                        if (mark[gg] != stamp)
+                        {
+                            mark[gg] = stamp;
+                            gsub.push_back(gg);
+                        }
+                    }
+                }
+            auto& cells = gcells[gi];
+            eng->counters[7] += cells.size();
+            eng->counters[8] += 1;
+            for (uint32_t c : cells)
+                nrounds[c]++;
+            if (gsub.size() < 20)
+            {
+                // fewer than 20 genes: the round returns {front} (:1158-1159)
+                for (uint32_t c : cells)
+                    counts[c] = 1;
+                continue;
+            }
+            std::sort(gsub.begin(), gsub.end());
+            uint32_t nref = 0;
+            for (uint32_t l : labs)
+                nref += eng->fctidx[2 * l + 1];
+            const uint32_t stride = (uint32_t)round_up(gsub.size(), 8);
+            // split very large groups by cells so that one batch stays bounded
+            size_t cpos = 0;
+            while (cpos < cells.size())
+            {
+                const uint64_t fixed = (uint64_t)nref * stride;
+                uint64_t       per   = (uint64_t)stride + nref;
+                uint64_t       room  = budget_elems > b.ref_elems + b.cell_elems + b.n_scores + fixed
+                                           ? budget_elems - (b.ref_elems + b.cell_elems + b.n_scores + fixed)
+                                           : 0;
+                size_t take = (size_t)std::min<uint64_t>(cells.size() - cpos, room / per);
+                if (take == 0)
+                {
+                    if (!b.groups.empty())
+                    {
+                        if ((rc = flush()))
+                            return rc;
+                        continue;
+                    }
+                    take = 1;
+                }
+                FtGroup g{};
+                g.gene_off   = (uint32_t)b.genes.size();
+                g.gs         = (uint32_t)gsub.size();
+                g.stride     = stride;
+                g.nref       = nref;
+                g.ncell      = (uint32_t)take;
+                g.ref_item0  = (uint32_t)b.ref_items.size();
+                g.cell_item0 = (uint32_t)b.cell_items.size();
+                g.label_off  = (uint32_t)b.labels.size();
+                g.nlabel     = (uint32_t)labs.size();
+                g.ref_dst0   = b.ref_elems;
+                g.cell_dst0  = b.cell_elems;
+                g.score0     = b.n_scores;
+                const uint32_t gidx = (uint32_t)b.groups.size();
+                b.genes.insert(b.genes.end(), gsub.begin(), gsub.end());
+                b.labels.insert(b.labels.end(), labs.begin(), labs.end());
+                for (uint32_t l : labs)
+                    for (uint32_t r = 0; r < eng->fctidx[2 * l + 1]; ++r)
+                    {
+                        b.ref_items.push_back(
+                            FtRankItem{ eng->fctidx[2 * l] + r, gidx, b.ref_elems });
+                        b.ref_elems += stride;
+                    }
+                for (size_t t = 0; t < take; ++t)
+                {
+                    const uint32_t c = cells[cpos + t];
+                    b.cell_items.push_back(FtRankItem{ c, gidx, b.cell_elems });
+                    b.cell_elems += stride;
+                    FtCell fc{ c, gidx, (uint64_t)b.segs.size() };
+                    b.cells.push_back(fc);
+                    b.cell_ids.push_back(c);
+                    uint32_t off = 0;
+                    for (uint32_t l : labs)
+                    {
+                        const uint32_t n = eng->fctidx[2 * l + 1];
+                        b.segs.push_back(Segment{ b.n_scores + (uint64_t)t * nref + off, n, l });
+                        b.seg_len.push_back(n);
+                        off += n;
+                    }
+                    b.in_lists.insert(b.in_lists.end(), &lists[(size_t)c * L],
+                                      &lists[(size_t)c * L] + L);
+                    b.in_counts.push_back(counts[c]);
+                }
+                for (uint32_t c0 = 0; c0 < take; c0 += FT_TC)
+                    for (uint32_t r0 = 0; r0 < nref; r0 += FT_TR)
+                        b.tiles.push_back(FtCorrTile{ gidx, c0, r0, 0 });
+                eng->counters[9] += (int64_t)take * nref;
+                eng->counters[10] += (int64_t)take * nref * gsub.size();
+                b.n_scores += (uint64_t)take * nref;
+                b.groups.push_back(g);
+                cpos += take;
+                if (b.ref_elems + b.cell_elems + b.n_scores >= budget_elems)
+                    if ((rc = flush()))
+                        return rc;
+            }
+        }
+        if ((rc = flush()))
+            return rc;
+    }
+    for (uint32_t c = 0; c < Nq; ++c)
+    {
+        final_labels[c] = counts[c] == 0 ? 0u : (uint32_t)lists[(size_t)c * L];
+        if (rounds_out)
+            rounds_out[c] = nrounds[c];
+    }
+    return CSG_OK;
+}
+
+int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    (void)hipStreamSynchronize(eng->stream);
+    collect_phases(eng);
+    for (int i = 0; i < CSG_N_PHASES; ++i)
+    {
+        if (ms)
+            ms[i] = eng->phase_ms[i];
+        if (counts)
+            counts[i] = eng->phase_cnt[i];
+        if (reset)
+        {
+            eng->phase_ms[i]  = 0;
+            eng->phase_cnt[i] = 0;
+        }
+    }
+    return CSG_OK;
+}
+
+int csg_counters(csg_engine* eng, int64_t* out)
+{
+    if (!eng || !out)
+        return CSG_ERR_ARGUMENT;
+    std::memcpy(out, eng->counters, sizeof(eng->counters));
+    return CSG_OK;
+}
+
+int csg_host_percentile(const float* arr, int64_t len, float quantile, float* out)
+{
+    if (len <= 0 || !arr || !out)
+        return CSG_ERR_ARGUMENT;
+    const PctParam     pp = make_pct_param((long)len, quantile);
+    std::vector<float> a(arr, arr + len);
+    if (pp.kind == 0)
+    {
+        *out = a[0];
+        return CSG_OK;
+    }
+    if (!std_sort(a.data(), (long)len))
+        return CSG_ERR_UNDEFINED;
+    if (pp.kind == 1)
+        *out = a[0];
+    else if (pp.kind == 2)
+        *out = a[len - 1];
+    else
+        *out = pct_interp(quantile, pp.prev_index, pp.step, a[pp.k - 1], a[pp.k]);
+    return CSG_OK;
+}
+
+void csg_host_minmax(const float* arr, int n, int* imin, int* imax) { std_minmax(arr, n, imin, imax); }
+
+}  // extern "C"

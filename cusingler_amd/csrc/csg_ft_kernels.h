@@ -1,0 +1,380 @@
+// Device kernels of the fine-tune step (src/cusingler.cu:1138-1333).
+//
+// The reference refines one query cell at a time: each round it builds the
+// gene subset G_sub of the cell's candidate labels, ranks the cell and every
+// reference cell of those labels over G_sub, correlates them with a
+// 128-chunk + tree FP32 reduction (spearman_reduce, :804-856), takes each
+// label's quantile and filters the labels.  Here a round processes all cells
+// at once: cells with the same candidate set share G_sub and the reference
+// ranks, so the host groups them and these kernels run over flat work tables.
+#pragma once
+
+#include "csg_kernels.h"
+
+namespace csg {
+
+// FP32 chunk + tree sum of spearman_reduce: thread t of 128 sums genes
+// [t*step, (t+1)*step) of `f(i)` by fma in order, then partial[t] +=
+// partial[t + stride] for stride 1, 2, ..., 64 (src/cusingler.cu:823-845).
+// Called by all 128 threads of a block; `part` is 128 floats of LDS.
+template <class F> __device__ inline float chunk_tree_sum128(int n, F f, float* part)
+{
+    const int t    = threadIdx.x;
+    const int step = (n + 127) / 128;
+    float     acc  = 0.f;
+    const int lo   = t * step;
+    const int hi   = min(lo + step, n);
+    for (int i = lo; i < hi; ++i)
+    {
+        const float v = f(i);
+        acc           = __builtin_fmaf(v, v, acc);
+    }
+    part[t] = acc;
+    __syncthreads();
+    for (int stride = 1; stride < 128; stride *= 2)
+    {
+        if ((t % (2 * stride)) == 0)
+            part[t] += part[t + stride];
+        __syncthreads();
+    }
+    const float r = part[0];
+    __syncthreads();
+    return r;
+}
+
+// A row to rank over a group's gene subset.
+struct FtRankItem
+{
+    uint32_t src_row;  // reference row (dense codes) or query row (CSR)
+    uint32_t group;
+    uint64_t dst;      // element offset of the output row in the rank buffer
+};
+
+struct FtGroup
+{
+    uint32_t gene_off;    // into the gene table
+    uint32_t gs;          // |G_sub|
+    uint32_t stride;      // row stride of the rank buffers (gs rounded up to 8)
+    uint32_t nref;        // reference rows of the group's labels
+    uint32_t ncell;       // cells of the group
+    uint32_t ref_item0;   // first reference rank item (row) of the group
+    uint32_t cell_item0;  // first cell item of the group
+    uint32_t label_off;   // into the label table (ascending labels)
+    uint32_t nlabel;
+    uint32_t pad_;
+    uint64_t ref_dst0;    // element offset of the first reference rank row
+    uint64_t cell_dst0;   // element offset of the first cell rank row
+    uint64_t score0;      // first score of the group (cell-major, nref per cell)
+};
+
+// Doubled centred ranks of codes[0..gs) (code 0 = implicit zero, counted like
+// any other value) into row[], over nbins code bins held in LDS.
+template <int NT>
+__device__ inline void rank_codes(const uint16_t* codes, int gs, int nbins, int* bins, int16_t* row,
+                                  int* tmp)
+{
+    for (int i = threadIdx.x; i < nbins; i += NT)
+        bins[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < gs; i += NT)
+        atomicAdd(&bins[codes[i]], 1);
+    __syncthreads();
+    bins_to_ranks<NT>(bins, nbins, 0, gs, tmp);
+    for (int i = threadIdx.x; i < gs; i += NT)
+        row[i] = (int16_t)bins[codes[i]];
+    __syncthreads();
+}
+
+// Reference rows: gather the dense fine-tune codes at G_sub
+// (get_device_ref_lines, src/cusingler.cu:566-577), rank them
+// (rankdata_bin3, :745-801) and form the chunk-tree sum of squares.
+__global__ __launch_bounds__(128) void k_ft_rank_ref(const FtRankItem* __restrict__ items,
+                                                     const FtGroup* __restrict__ groups,
+                                                     const uint32_t* __restrict__ genes,
+                                                     const uint16_t* __restrict__ ref_codes,
+                                                     uint32_t Gf, int nbins,
+                                                     int16_t* __restrict__ out,
+                                                     float* __restrict__ out_sq)
+{
+    extern __shared__ int smem[];
+    __shared__ float part[128];
+    __shared__ int   tmp[2];
+    const FtRankItem it = items[blockIdx.x];
+    const FtGroup    g  = groups[it.group];
+    int*             bins  = smem;
+    uint16_t*        codes = (uint16_t*)(smem + ((nbins + 3) & ~3));
+    int16_t*         row   = (int16_t*)(codes + ((g.gs + 7) & ~7u));
+    const uint16_t*  src   = ref_codes + (size_t)it.src_row * Gf;
+    const uint32_t*  gl    = genes + g.gene_off;
+    for (uint32_t i = threadIdx.x; i < g.gs; i += 128)
+        codes[i] = src[gl[i]];
+    __syncthreads();
+    rank_codes<128>(codes, (int)g.gs, nbins, bins, row, tmp);
+    int16_t* dst = out + it.dst;
+    for (uint32_t i = threadIdx.x; i < g.stride; i += 128)
+        dst[i] = i < g.gs ? row[i] : (int16_t)0;
+    const float sq = chunk_tree_sum128((int)g.gs, [&](int i) { return (float)row[i]; }, part);
+    if (threadIdx.x == 0)
+        out_sq[blockIdx.x] = sq;
+}
+
+// Query rows: densify the CSR row over the fine-tune genes, gather G_sub
+// (csr2dense single-row form, src/cusingler.cu:241-264), rank (rankdata,
+// :159-179) and form the chunk-tree sum of squares.
+__global__ __launch_bounds__(128) void k_ft_rank_qry(const FtRankItem* __restrict__ items,
+                                                     const FtGroup* __restrict__ groups,
+                                                     const uint32_t* __restrict__ genes,
+                                                     const uint16_t* __restrict__ qdata,
+                                                     const int* __restrict__ qind,
+                                                     const int* __restrict__ qptr, uint32_t Gf,
+                                                     int nbins, int16_t* __restrict__ out,
+                                                     float* __restrict__ out_sq)
+{
+    extern __shared__ int smem[];
+    __shared__ float part[128];
+    __shared__ int   tmp[2];
+    const FtRankItem it = items[blockIdx.x];
+    const FtGroup    g  = groups[it.group];
+    int*             bins  = smem;
+    uint16_t*        dense = (uint16_t*)(smem + ((nbins + 3) & ~3));
+    uint16_t*        codes = dense + ((Gf + 7) & ~7u);
+    int16_t*         row   = (int16_t*)(codes + ((g.gs + 7) & ~7u));
+    for (uint32_t i = threadIdx.x; i < Gf; i += 128)
+        dense[i] = 0;
+    __syncthreads();
+    const int beg = qptr[it.src_row], end = qptr[it.src_row + 1];
+    for (int i = beg + (int)threadIdx.x; i < end; i += 128)
+        dense[qind[i]] = qdata[i];
+    __syncthreads();
+    const uint32_t* gl = genes + g.gene_off;
+    for (uint32_t i = threadIdx.x; i < g.gs; i += 128)
+        codes[i] = dense[gl[i]];
+    __syncthreads();
+    rank_codes<128>(codes, (int)g.gs, nbins, bins, row, tmp);
+    int16_t* dst = out + it.dst;
+    for (uint32_t i = threadIdx.x; i < g.stride; i += 128)
+        dst[i] = i < g.gs ? row[i] : (int16_t)0;
+    const float sq = chunk_tree_sum128((int)g.gs, [&](int i) { return (float)row[i]; }, part);
+    if (threadIdx.x == 0)
+        out_sq[blockIdx.x] = sq;
+}
+
+// Correlation tile of one group: TC cells x TR reference rows, one pair per
+// thread, rows staged in LDS as int16.  sxy follows spearman_reduce: 128
+// chunks of ceil(gs/128) genes summed by fma in order, folded by the
+// stride-1..64 tree, which for chunk sums c_0..c_127 is the balanced binary
+// tree ((c0+c1)+(c2+c3))+...; here the tree is built left to right with a
+// 7-level stack (a completed node is added to its left sibling).
+struct FtCorrTile
+{
+    uint32_t group;
+    uint32_t cell0;  // first cell of the tile within the group
+    uint32_t ref0;   // first reference row of the tile within the group
+    uint32_t pad_;
+};
+
+template <int TC, int TR>
+__global__ __launch_bounds__(TC * TR) void k_ft_corr(const FtCorrTile* __restrict__ tiles,
+                                                     const FtGroup* __restrict__ groups,
+                                                     const int16_t* __restrict__ cell_rank,
+                                                     const float* __restrict__ cell_sq,
+                                                     const int16_t* __restrict__ ref_rank,
+                                                     const float* __restrict__ ref_sq,
+                                                     float* __restrict__ scores)
+{
+    extern __shared__ int16_t srow[];
+    const FtCorrTile tl    = tiles[blockIdx.x];
+    const FtGroup    g     = groups[tl.group];
+    const int        gs    = (int)g.gs;
+    const int        st    = (int)g.stride;
+    const int        ncell = min(TC, (int)g.ncell - (int)tl.cell0);
+    const int        nref  = min(TR, (int)g.nref - (int)tl.ref0);
+    int16_t*         cs    = srow;           // TC x stride
+    int16_t*         rs    = srow + TC * st; // TR x stride
+    constexpr int    NT    = TC * TR;
+    const int        v8    = st / 8;
+    const int16_t*   cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
+    const int16_t*   rbase = ref_rank + g.ref_dst0 + (uint64_t)tl.ref0 * st;
+    for (int v = threadIdx.x; v < ncell * v8; v += NT)
+        *(int4*)(cs + v * 8) = *(const int4*)(cbase + (uint64_t)v * 8);
+    for (int v = threadIdx.x; v < nref * v8; v += NT)
+        *(int4*)(rs + v * 8) = *(const int4*)(rbase + (uint64_t)v * 8);
+    __syncthreads();
+    const int c = threadIdx.x / TR;
+    const int r = threadIdx.x % TR;
+    if (c >= ncell || r >= nref)
+        return;
+    const int16_t* x    = cs + c * st;
+    const int16_t* y    = rs + r * st;
+    const int      step = (gs + 127) / 128;
+    float          stk[7];
+#pragma unroll
+    for (int l = 0; l < 7; ++l)
+        stk[l] = 0.f;
+    float root = 0.f;
+    for (int ch = 0; ch < 128; ++ch)
+    {
+        const int lo  = ch * step;
+        const int hi  = min(lo + step, gs);
+        float     acc = 0.f;
+        for (int i = lo; i < hi; ++i)
+            acc = __builtin_fmaf((float)x[i], (float)y[i], acc);
+        // merge: while the low bits of ch are ones the node closes a pair
+        float v  = acc;
+        int   cc = ch;
+#pragma unroll
+        for (int l = 0; l < 7; ++l)
+        {
+            if (cc & 1)
+            {
+                v = stk[l] + v;
+                cc >>= 1;
+            }
+            else
+            {
+                stk[l] = v;
+                cc     = -1;
+                break;
+            }
+        }
+        if (cc != -1)
+            root = v;  // ch == 127: the whole tree
+    }
+    const uint32_t ci = g.cell_item0 + tl.cell0 + c;
+    const uint32_t ri = g.ref_item0 + tl.ref0 + r;
+    scores[g.score0 + (uint64_t)(tl.cell0 + c) * g.nref + tl.ref0 + r] =
+        corr_from_sums(root, cell_sq[ci], ref_sq[ri]);
+}
+
+// New candidate list of one cell from its per-label scores, following
+// finetune_round's tail (src/cusingler.cu:1270-1295): NaN maximum -> {front};
+// keep labels with score > min and score >= max - thre, labels equal to the
+// maximum moved to the front (each inserted before the previous ones);
+// nothing kept -> {front}.
+struct FtCell
+{
+    uint32_t cell;     // query row
+    uint32_t group;
+    uint64_t seg0;     // first per-label score (ascending label order)
+};
+
+__global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
+                            const FtGroup* __restrict__ groups,
+                            const uint32_t* __restrict__ labels, const float* __restrict__ P,
+                            const uint8_t* __restrict__ in_lists,
+                            const uint32_t* __restrict__ in_counts, uint32_t L, float thre,
+                            uint8_t* __restrict__ out_lists, uint32_t* __restrict__ out_counts)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncells)
+        return;
+    const FtCell   fc = cells[i];
+    const FtGroup  g  = groups[fc.group];
+    const uint8_t* in = in_lists + (size_t)i * L;
+    const uint32_t k  = in_counts[i];
+    const uint32_t* gl = labels + g.label_off;
+    auto score_of = [&](uint32_t pos) {
+        const uint32_t lab = in[pos];
+        uint32_t       j   = 0;
+        while (j < g.nlabel && gl[j] != lab)
+            ++j;
+        return P[fc.seg0 + j];
+    };
+    // minmax_element over the scores in list order
+    int mn = 0, mx = 0;
+    if (k >= 2)
+    {
+        float s0 = score_of(0), s1 = score_of(1);
+        if (s1 < s0)
+        {
+            mn = 1;
+            mx = 0;
+        }
+        else
+        {
+            mn = 0;
+            mx = 1;
+        }
+        float smn = s1 < s0 ? s1 : s0, smx = s1 < s0 ? s0 : s1;
+        uint32_t t = 2;
+        while (t < k)
+        {
+            const float a = score_of(t);
+            if (t + 1 == k)
+            {
+                if (a < smn)
+                {
+                    mn  = t;
+                    smn = a;
+                }
+                else if (!(a < smx))
+                {
+                    mx  = t;
+                    smx = a;
+                }
+                break;
+            }
+            const float b = score_of(t + 1);
+            if (b < a)
+            {
+                if (b < smn)
+                {
+                    mn  = t + 1;
+                    smn = b;
+                }
+                if (!(a < smx))
+                {
+                    mx  = t;
+                    smx = a;
+                }
+            }
+            else
+            {
+                if (a < smn)
+                {
+                    mn  = t;
+                    smn = a;
+                }
+                if (!(b < smx))
+                {
+                    mx  = t + 1;
+                    smx = b;
+                }
+            }
+            t += 2;
+        }
+    }
+    const float vmin = score_of(mn);
+    const float vmax = score_of(mx);
+    uint8_t*    out  = out_lists + (size_t)i * L;
+    uint32_t    n    = 0;
+    if (__builtin_isnan(vmax))
+    {
+        out[0]        = in[0];
+        out_counts[i] = 1;
+        return;
+    }
+    const float th = vmax - thre;
+    // labels equal to the maximum, latest first
+    for (int t = (int)k - 1; t >= 0; --t)
+    {
+        const float s = score_of(t);
+        if (s <= vmin || s < th)
+            continue;
+        if (s == vmax)
+            out[n++] = in[t];
+    }
+    for (uint32_t t = 0; t < k; ++t)
+    {
+        const float s = score_of(t);
+        if (s <= vmin || s < th)
+            continue;
+        if (!(s == vmax))
+            out[n++] = in[t];
+    }
+    if (n == 0)
+        out[n++] = in[0];
+    out_counts[i] = n;
+}
+
+}  // namespace csg

@@ -1,0 +1,660 @@
+// Device kernels of the SingleR annotation engine (gfx950 / MI355X).
+//
+// Value convention.  Ranks are kept as doubled, centred integers
+//     X = 2 * rank - (G + 1)
+// in int16, where rank is the 1-based tie-averaged rank of the value among the
+// G genes of the active gene set (src/cusingler.cu:745-801 rankdata_bin3,
+// :159-179 rankdata and :601-625 rankdata_batch all produce it).  X is exact;
+// the reference's centred float rank x = rank - (G+1)/2 equals X/2.  Every
+// FP32 sum below is formed from X instead of x: scaling all inputs by a power
+// of two scales every rounded sum, product, square root and quotient exactly,
+// so the resulting correlation has the same bits as the reference's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "csg_stdalgo.h"
+
+namespace csg {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------- utilities ---
+
+__device__ inline int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1)
+    {
+        int t = __shfl_up(v, d, kWave);
+        if (lane >= d)
+            v += t;
+    }
+    return v;
+}
+
+__device__ inline int wave_sum(int v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1)
+        v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+__device__ inline long long wave_sum64(long long v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1)
+        v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+__device__ inline float wave_min(float v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1)
+        v = fminf(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+
+__device__ inline float wave_max(float v)
+{
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1)
+        v = fmaxf(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+
+// Correlation from the three FP32 sums, in the reference's operation order:
+// divisor = sqrt(sxx * syy); divisor != 0 ? sxy / divisor : NaN
+// (src/cusingler.cu:849-854, :940-944).  sqrtf and '/' are correctly rounded
+// in HIP device code (IEEE single, as the reference's nvcc defaults).
+__device__ inline float corr_from_sums(float sxy, float sxx, float syy)
+{
+    const float prod = sxx * syy;
+    const float div  = sqrtf(prod);
+    return div != 0.f ? sxy / div : __builtin_nanf("");
+}
+
+// Block-wide exclusive scan of `v` for a block of NT threads (NT a multiple of
+// 64, at most 1024); `tmp` holds NT/64 ints of LDS.
+template <int NT> __device__ inline int block_excl_scan(int v, int* tmp, int* total)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid  = threadIdx.x / kWave;
+    int       inc  = wave_incl_scan(v);
+    if (lane == kWave - 1)
+        tmp[wid] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NT / kWave; ++w)
+    {
+        const int t = tmp[w];
+        if (w < wid)
+            off += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+// Turns per-code counts in bins[0..nbins) into doubled centred ranks in place:
+// X(c) = 2*start(c) + cnt(c) + 1 - (G + 1), start(c) = sum of counts below c.
+// `base` is added to every start (the implicit-zero count when code 0 is not
+// in the bins).
+template <int NT> __device__ inline void bins_to_ranks(int* bins, int nbins, int base, int G, int* tmp)
+{
+    const int per = (nbins + NT - 1) / NT;
+    const int b0  = threadIdx.x * per;
+    const int b1  = min(b0 + per, nbins);
+    int       s   = 0;
+    for (int i = b0; i < b1; ++i)
+        s += bins[i];
+    int total;
+    int start = block_excl_scan<NT>(s, tmp, &total) + base;
+    for (int i = b0; i < b1; ++i)
+    {
+        const int c = bins[i];
+        bins[i]     = 2 * start + c - G;
+        start += c;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------- score-step rank rows ---
+
+// One block per row of a coded CSR matrix over G genes.  Writes the dense
+// doubled centred ranks X[row, 0..Kp) (zero past G), the FP32 sum of X^2 in
+// gene order (the reference's `sumxx += x*x` loop of src/cusingler.cu:931-938,
+// contracted to fma) and the exact int64 sum of X^2.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ data,
+                                                 const int* __restrict__ indices,
+                                                 const int* __restrict__ indptr, uint32_t row0,
+                                                 uint32_t nrows, uint32_t G, uint32_t Kp,
+                                                 int nbins, int16_t* __restrict__ X,
+                                                 float* __restrict__ sq_f32,
+                                                 long long* __restrict__ sq_i64)
+{
+    extern __shared__ int smem[];
+    int*     bins = smem;                          // nbins
+    int16_t* row  = (int16_t*)(smem + ((nbins + 3) & ~3));  // Kp
+    __shared__ int       tmp[NT / kWave];
+    __shared__ long long red[NT / kWave];
+
+    const uint32_t r = blockIdx.x;
+    if (r >= nrows)
+        return;
+    const int beg = indptr[row0 + r];
+    const int end = indptr[row0 + r + 1];
+    const int nnz = end - beg;
+    for (int i = threadIdx.x; i < nbins; i += NT)
+        bins[i] = 0;
+    __syncthreads();
+    for (int i = beg + threadIdx.x; i < end; i += NT)
+        atomicAdd(&bins[data[i]], 1);
+    __syncthreads();
+    // code 0 (implicit zeros) is not stored; its count is G - nnz
+    if (threadIdx.x == 0)
+        bins[0] = 0;
+    __syncthreads();
+    bins_to_ranks<NT>(bins, nbins, (int)G - nnz, (int)G, tmp);
+    const int16_t x0 = (int16_t)(-nnz);  // 2*0 + (G-nnz) - G
+    for (uint32_t g = threadIdx.x; g < Kp; g += NT)
+        row[g] = g < G ? x0 : (int16_t)0;
+    __syncthreads();
+    for (int i = beg + threadIdx.x; i < end; i += NT)
+        row[indices[i]] = (int16_t)bins[data[i]];
+    __syncthreads();
+    int16_t* out = X + (size_t)r * Kp;
+    for (uint32_t g = threadIdx.x * 8; g < Kp; g += NT * 8)
+        *(int4*)(out + g) = *(const int4*)(row + g);
+    long long s = 0;
+    for (uint32_t g = threadIdx.x; g < G; g += NT)
+    {
+        const long long v = row[g];
+        s += v * v;
+    }
+    s = wave_sum64(s);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        long long t = 0;
+        for (int w = 0; w < NT / kWave; ++w)
+            t += red[w];
+        sq_i64[r] = t;
+        float acc = 0.f;
+        uint32_t g = 0;
+        for (; g + 8 <= G; g += 8)
+        {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                v[u] = (float)row[g + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                acc = __builtin_fmaf(v[u], v[u], acc);
+        }
+        for (; g < G; ++g)
+        {
+            const float v = (float)row[g];
+            acc           = __builtin_fmaf(v, v, acc);
+        }
+        sq_f32[r] = acc;
+    }
+}
+
+// ------------------------------------------- plain FP32 score contraction ---
+
+// S[m, n] = corr(Xq[m], Xr[n]) with sum_k Xq[m,k]*Xr[n,k] accumulated by fma in
+// k order 0..Kp-1 (zero padding past G adds exact zeros), i.e. the reference's
+// sequential loop (src/cusingler.cu:931-938).  128x128 tile per 256-thread
+// block, 8x8 outputs per thread, K staged through LDS 16 at a time.
+constexpr int PG_BM = 128, PG_BN = 128, PG_BK = 16;
+
+__global__ __launch_bounds__(256) void k_gemm_plain(const int16_t* __restrict__ Xq,
+                                                    const int16_t* __restrict__ Xr, int Kp,
+                                                    const float* __restrict__ sq_q,
+                                                    const float* __restrict__ sq_r, int nq, int nr,
+                                                    float* __restrict__ S, size_t ldS)
+{
+    __shared__ float As[2][PG_BK][PG_BM + 4];
+    __shared__ float Bs[2][PG_BK][PG_BN + 4];
+    const int tid = threadIdx.x;
+    const int tx  = tid & 15;
+    const int ty  = tid >> 4;
+    const int m0  = blockIdx.y * PG_BM;
+    const int n0  = blockIdx.x * PG_BN;
+
+    float acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            acc[i][j] = 0.f;
+
+    const int lr = tid >> 1;        // tile row loaded by this thread
+    const int lk = (tid & 1) * 8;   // first of 8 k values
+    const int16_t* qa = Xq + (size_t)(m0 + lr) * Kp + lk;
+    const int16_t* rb = Xr + (size_t)(n0 + lr) * Kp + lk;
+
+    auto stage = [&](int buf, int4 av, int4 bv) {
+        const int16_t* a = (const int16_t*)&av;
+        const int16_t* b = (const int16_t*)&bv;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            As[buf][lk + u][lr] = (float)a[u];
+            Bs[buf][lk + u][lr] = (float)b[u];
+        }
+    };
+
+    int4 av = *(const int4*)qa;
+    int4 bv = *(const int4*)rb;
+    stage(0, av, bv);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < Kp; k0 += PG_BK)
+    {
+        const bool more = k0 + PG_BK < Kp;
+        if (more)
+        {
+            av = *(const int4*)(qa + k0 + PG_BK);
+            bv = *(const int4*)(rb + k0 + PG_BK);
+        }
+#pragma unroll
+        for (int kk = 0; kk < PG_BK; ++kk)
+        {
+            const float4 a0 = *(const float4*)&As[buf][kk][ty * 4];
+            const float4 a1 = *(const float4*)&As[buf][kk][64 + ty * 4];
+            const float4 b0 = *(const float4*)&Bs[buf][kk][tx * 4];
+            const float4 b1 = *(const float4*)&Bs[buf][kk][64 + tx * 4];
+            const float  a[8] = { a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w };
+            const float  b[8] = { b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w };
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    acc[i][j] = __builtin_fmaf(a[i], b[j], acc[i][j]);
+        }
+        if (more)
+        {
+            stage(buf ^ 1, av, bv);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+    {
+        const int m = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4));
+        if (m >= nq)
+            continue;
+        const float sq = sq_q[m];
+        float*      out = S + (size_t)m * ldS;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+        {
+            const int n = n0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4));
+            if (n < nr)
+                out[n] = corr_from_sums(acc[i][j], sq, sq_r[n]);
+        }
+    }
+}
+
+// ------------------------------------------------ per-label quantile select ---
+
+// A segment of scores whose per-label quantile is wanted.
+struct Segment
+{
+    uint64_t off;    // first score
+    uint32_t n;      // number of scores
+    uint32_t param;  // index into the PctParam table
+};
+
+// Segment source for the score step: item = q * L + l reads
+// S[q * ldS + lstart[l] .. + llen[l]].
+struct ScoreSegments
+{
+    const uint32_t* lstart;
+    const uint32_t* llen;
+    uint32_t        L;
+    uint64_t        ldS;
+    __device__ Segment get(uint64_t item) const
+    {
+        const uint32_t q = (uint32_t)(item / L);
+        const uint32_t l = (uint32_t)(item % L);
+        return Segment{ (uint64_t)q * ldS + lstart[l], llen[l], l };
+    }
+};
+
+struct TableSegments
+{
+    const Segment* seg;
+    __device__ Segment get(uint64_t item) const { return seg[item]; }
+};
+
+// Order statistics a_(k0) and a_(k1) (0-based, ascending, k0 <= k1) of a NaN
+// free segment, found by one wave: min/max, then a 256-bucket histogram whose
+// bucket index is monotone in the value, then exact ranking of the values in
+// the buckets that hold the two ranks.  While those buckets hold more than
+// kCand values the search narrows to their value range and repeats; the last
+// level ranks the remaining candidates against the whole segment.
+constexpr int kBuckets  = 256;
+constexpr int kCand     = 256;
+constexpr int kMaxLevel = 5;
+
+struct WaveSelectSmem
+{
+    int   hist[kBuckets];
+    float cand[kCand];
+    int   ncand;
+};
+
+__device__ inline int bucket_of(float x, float lo, double dscale)
+{
+    int b = (int)(((double)x - (double)lo) * dscale);
+    return b < 0 ? 0 : (b >= kBuckets ? kBuckets - 1 : b);
+}
+
+__device__ inline void wave_select2(const float* __restrict__ v, uint32_t n, uint32_t k0,
+                                    uint32_t k1, WaveSelectSmem& sm, float* out0, float* out1)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    float     lo   = __builtin_inff();
+    float     hi   = -__builtin_inff();
+    for (uint32_t i = lane; i < n; i += kWave)
+    {
+        const float x = v[i];
+        lo            = fminf(lo, x);
+        hi            = fmaxf(hi, x);
+    }
+    lo             = wave_min(lo);
+    hi             = wave_max(hi);
+    uint32_t below = 0;  // number of segment values below lo
+    float    r0 = lo, r1 = lo;
+    for (int level = 0;; ++level)
+    {
+        if (lo == hi)
+        {
+            r0 = r1 = lo;
+            break;
+        }
+        const double dscale = (double)kBuckets / ((double)hi - (double)lo);
+        for (int b = lane; b < kBuckets; b += kWave)
+            sm.hist[b] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i = lane; i < n; i += kWave)
+        {
+            const float x = v[i];
+            if (x >= lo && x <= hi)
+                atomicAdd(&sm.hist[bucket_of(x, lo, dscale)], 1);
+        }
+        __builtin_amdgcn_wave_barrier();
+        int c[4], s = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            c[u] = sm.hist[lane * 4 + u];
+            s += c[u];
+        }
+        int ex = wave_incl_scan(s) - s + (int)below;
+        int b0 = -1, b1 = -1;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            if ((uint32_t)ex <= k0 && k0 < (uint32_t)(ex + c[u]))
+                b0 = lane * 4 + u;
+            if ((uint32_t)ex <= k1 && k1 < (uint32_t)(ex + c[u]))
+                b1 = lane * 4 + u;
+            ex += c[u];
+        }
+        const unsigned long long m0 = __ballot(b0 >= 0);
+        const unsigned long long m1 = __ballot(b1 >= 0);
+        const int blo = __shfl(b0, m0 ? __builtin_ctzll(m0) : 0, kWave);
+        const int bhi = __shfl(b1, m1 ? __builtin_ctzll(m1) : 0, kWave);
+        int cb = 0, cc = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            const int b = lane * 4 + u;
+            if (b < blo)
+                cb += c[u];
+            else if (b <= bhi)
+                cc += c[u];
+        }
+        cb = wave_sum(cb);
+        cc = wave_sum(cc);
+        const bool last = cc <= kCand || level >= kMaxLevel;
+        if (last)
+        {
+            bool  hit0 = false, hit1 = false;
+            float v0 = 0.f, v1 = 0.f;
+            if (cc <= kCand)
+            {
+                if (lane == 0)
+                    sm.ncand = 0;
+                __builtin_amdgcn_wave_barrier();
+                for (uint32_t i = lane; i < n; i += kWave)
+                {
+                    const float x = v[i];
+                    if (x >= lo && x <= hi)
+                    {
+                        const int b = bucket_of(x, lo, dscale);
+                        if (b >= blo && b <= bhi)
+                            sm.cand[atomicAdd(&sm.ncand, 1)] = x;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t base = below + (uint32_t)cb;
+                for (int i = lane; i < cc; i += kWave)
+                {
+                    const float x    = sm.cand[i];
+                    uint32_t    less = 0, eq = 0;
+                    for (int j = 0; j < cc; ++j)
+                    {
+                        const float y = sm.cand[j];
+                        less += y < x;
+                        eq += y == x;
+                    }
+                    const uint32_t p0 = base + less, p1 = p0 + eq;
+                    if (p0 <= k0 && k0 < p1)
+                    {
+                        hit0 = true;
+                        v0   = x;
+                    }
+                    if (p0 <= k1 && k1 < p1)
+                    {
+                        hit1 = true;
+                        v1   = x;
+                    }
+                }
+            }
+            else
+            {
+                // many candidates after kMaxLevel narrowings: rank each one
+                // against the whole segment
+                for (uint32_t i = lane; i < n; i += kWave)
+                {
+                    const float x = v[i];
+                    if (!(x >= lo && x <= hi))
+                        continue;
+                    const int b = bucket_of(x, lo, dscale);
+                    if (b < blo || b > bhi)
+                        continue;
+                    uint32_t less = 0, eq = 0;
+                    for (uint32_t j = 0; j < n; ++j)
+                    {
+                        const float y = v[j];
+                        less += y < x;
+                        eq += y == x;
+                    }
+                    const uint32_t p0 = less, p1 = less + eq;
+                    if (p0 <= k0 && k0 < p1)
+                    {
+                        hit0 = true;
+                        v0   = x;
+                    }
+                    if (p0 <= k1 && k1 < p1)
+                    {
+                        hit1 = true;
+                        v1   = x;
+                    }
+                }
+            }
+            const unsigned long long f0 = __ballot(hit0);
+            const unsigned long long f1 = __ballot(hit1);
+            r0 = __shfl(v0, f0 ? __builtin_ctzll(f0) : 0, kWave);
+            r1 = __shfl(v1, f1 ? __builtin_ctzll(f1) : 0, kWave);
+            break;
+        }
+        float nlo = __builtin_inff(), nhi = -__builtin_inff();
+        for (uint32_t i = lane; i < n; i += kWave)
+        {
+            const float x = v[i];
+            if (x >= lo && x <= hi)
+            {
+                const int b = bucket_of(x, lo, dscale);
+                if (b >= blo && b <= bhi)
+                {
+                    nlo = fminf(nlo, x);
+                    nhi = fmaxf(nhi, x);
+                }
+            }
+        }
+        below += (uint32_t)cb;
+        lo = wave_min(nlo);
+        hi = wave_max(nhi);
+        __builtin_amdgcn_wave_barrier();
+    }
+    *out0 = r0;
+    *out1 = r1;
+}
+
+// Reference percentile of a NaN-free segment from its order statistics.
+__device__ inline float pct_from_stats(const PctParam& pp, float p, float a_km1, float a_k,
+                                       float vmin, float vmax)
+{
+    if (pp.kind == 1)
+        return vmin;
+    if (pp.kind == 2)
+        return vmax;
+    return pct_interp(p, pp.prev_index, pp.step, a_km1, a_k);
+}
+
+// One wave per segment: the reference's percentile (src/cusingler.cu:947-978)
+// of the segment's scores.  Segments holding a NaN are appended to `nan_list`
+// (the libstdc++ sort emulation runs on them in k_select_nan).
+template <class Src>
+__global__ __launch_bounds__(256) void k_select(Src src, uint64_t n_items,
+                                                const float* __restrict__ scores,
+                                                const PctParam* __restrict__ params, float p,
+                                                float* __restrict__ out,
+                                                uint64_t* __restrict__ nan_list,
+                                                unsigned int* __restrict__ nan_count)
+{
+    __shared__ WaveSelectSmem sm[256 / kWave];
+    const int      wid  = threadIdx.x / kWave;
+    const int      lane = threadIdx.x & (kWave - 1);
+    const uint64_t item = (uint64_t)blockIdx.x * (256 / kWave) + wid;
+    if (item >= n_items)
+        return;
+    const Segment  sg = src.get(item);
+    const float*   v  = scores + sg.off;
+    const PctParam pp = params[sg.param];
+    if (pp.kind == 0)
+    {
+        // len <= 1: the reference returns arr.front() unsorted
+        if (lane == 0)
+            out[item] = sg.n > 0 ? v[0] : __builtin_nanf("");
+        return;
+    }
+    bool nan = false;
+    for (uint32_t i = lane; i < sg.n; i += kWave)
+        nan |= __builtin_isnan(v[i]);
+    if (__ballot(nan))
+    {
+        if (lane == 0)
+        {
+            const unsigned int s = atomicAdd(nan_count, 1u);
+            nan_list[s]          = item;
+            out[item]            = __builtin_nanf("");
+        }
+        return;
+    }
+    float a0 = 0.f, a1 = 0.f, vmin = 0.f, vmax = 0.f;
+    if (pp.kind == 3)
+        wave_select2(v, sg.n, (uint32_t)(pp.k - 1), (uint32_t)pp.k, sm[wid], &a0, &a1);
+    else
+    {
+        float lo = __builtin_inff(), hi = -__builtin_inff();
+        for (uint32_t i = lane; i < sg.n; i += kWave)
+        {
+            lo = fminf(lo, v[i]);
+            hi = fmaxf(hi, v[i]);
+        }
+        vmin = wave_min(lo);
+        vmax = wave_max(hi);
+    }
+    if (lane == 0)
+        out[item] = pct_from_stats(pp, p, a0, a1, vmin, vmax);
+}
+
+// NaN-carrying segments: one lane copies the segment to `work` and runs the
+// libstdc++ introsort emulation, then reads the reference's positions.
+template <class Src>
+__global__ void k_select_nan(Src src, const uint64_t* __restrict__ nan_list, unsigned int count,
+                             const float* __restrict__ scores, const PctParam* __restrict__ params,
+                             float p, float* __restrict__ out, float* __restrict__ work,
+                             const uint64_t* __restrict__ work_off, int* __restrict__ status)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    const uint64_t item = nan_list[i];
+    const Segment  sg   = src.get(item);
+    const PctParam pp   = params[sg.param];
+    float*         a    = work + work_off[i];
+    for (uint32_t t = 0; t < sg.n; ++t)
+        a[t] = scores[sg.off + t];
+    if (!std_sort(a, (long)sg.n))
+    {
+        atomicExch(status, 1);
+        out[item] = __builtin_nanf("");
+        return;
+    }
+    float r;
+    if (pp.kind == 1)
+        r = a[0];
+    else if (pp.kind == 2)
+        r = a[sg.n - 1];
+    else
+        r = pct_interp(p, pp.prev_index, pp.step, a[pp.k - 1], a[pp.k]);
+    out[item] = r;
+}
+
+// First label (last maximum of libstdc++ minmax_element) and candidate mask
+// (score >= max - thre, in float) per query cell (src/cusingler.cu:1119-1135).
+__global__ void k_label(const float* __restrict__ P, uint32_t nq, uint32_t L, float thre,
+                        int32_t* __restrict__ first, float* __restrict__ mask)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq)
+        return;
+    const float* s = P + (size_t)q * L;
+    int          imin, imax;
+    std_minmax(s, (int)L, &imin, &imax);
+    first[q]       = imax;
+    const float th = s[imax] - thre;
+    for (uint32_t j = 0; j < L; ++j)
+        mask[(size_t)q * L + j] = s[j] >= th ? 1.f : 0.f;
+}
+
+}  // namespace csg

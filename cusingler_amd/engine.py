@@ -1,0 +1,305 @@
+"""ctypes bindings of ``libcusingler_amd.so`` (include/cusingler_amd.h and
+include/cusingler_prep.h) for step-by-step use: the host preprocessing
+(``Prep``), the device engine (``Engine``) and a convenience ``annotate`` that
+runs train -> score -> finetune like the reference's pipeline
+(src/pipeline.cpp:128-164) and returns label indices and scores.
+
+There is no Python fallback: a missing library raises, and the engine
+functions need a GPU.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+
+PATHS = {"plain": 0, "exact_int8": 1, "fp32_mfma": 2, "fp16": 3}
+PHASES = ["rank", "corr", "select", "label", "recheck", "finetune", "ft_corr", "ft_rank",
+          "ft_select"]
+COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells", "nan_segments",
+            "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes"]
+N_PHASES = 9
+N_COUNTERS = 11
+
+_V = C.c_void_p
+_U32 = C.c_uint32
+_I32 = C.c_int32
+_U64 = C.c_uint64
+_I64 = C.c_int64
+_F = C.c_float
+
+
+class CsgError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+class CodedStruct(C.Structure):
+    _fields_ = [("ref_height", _U32), ("qry_height", _U32), ("width", _U32), ("ct_num", _U32),
+                ("ref_data", _V), ("ref_indices", _V), ("ref_indptr", _V),
+                ("qry_data", _V), ("qry_indices", _V), ("qry_indptr", _V),
+                ("max_uniq_gene", _U64), ("ctidx", _V), ("ctdidx", _V), ("ctdiff", _V),
+                ("n_ctdiff", _U64), ("genes", _V)]
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    L = _lib.load("libcusingler_amd.so")
+    P = C.POINTER
+    L.csg_init_gpu.argtypes = [C.c_int, P(_V)]
+    L.csg_last_error.restype = C.c_char_p
+    L.csg_last_error.argtypes = [_V]
+    L.csg_release.argtypes = [_V]
+    L.csg_copyin_score.argtypes = [_V, _U32, _U32, _V, _V, _V, _U32, _V, _V, _V, _U32, _V, _U64]
+    L.csg_get_label.argtypes = [_V, _F, _F, C.c_int, _V, _V, _V]
+    L.csg_destroy_score.argtypes = [_V]
+    L.csg_copyin.argtypes = [_V, _U32, _U32, _V, _V, _V, _U32, _V, _V, _V, _U32, _V, _V, _V, _U64,
+                             _U64, _V]
+    L.csg_finetune.argtypes = [_V, _F, _F, C.c_int, _V, _V]
+    L.csg_destroy.argtypes = [_V]
+    L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
+    L.csg_counters.argtypes = [_V, _V]
+    L.csg_host_percentile.argtypes = [_V, _I64, _F, P(_F)]
+    L.csg_host_minmax.argtypes = [_V, C.c_int, P(C.c_int), P(C.c_int)]
+    L.csg_prep_train.argtypes = [C.c_int, _U32, _U32, _V, _V, _V, _U32, _U32, _V, _V, _V, _U32, _V,
+                                 _V, _U32, _V, _U32, P(_V), C.c_char_p, C.c_size_t]
+    L.csg_prep_generate.argtypes = [_V, C.c_int, P(CodedStruct)]
+    L.csg_prep_train_table.restype = _U64
+    L.csg_prep_train_table.argtypes = [_V, C.c_int, _V]
+    L.csg_prep_filtered.argtypes = [_V]
+    L.csg_prep_release.argtypes = [_V]
+    _LIB = L
+    return L
+
+
+def _p(a: np.ndarray) -> _V:
+    return _V(a.ctypes.data)
+
+
+def _np(ptr, n, dtype) -> np.ndarray:
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    buf = (C.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dtype).copy()
+
+
+def _names(lst: Sequence[str]):
+    enc = [s.encode() for s in lst]
+    arr = (C.c_char_p * len(enc))(*enc)
+    return arr, enc
+
+
+@dataclass
+class Coded:
+    """Value-coded CSR matrices of one step (generateDenseMatrix output)."""
+    ref_height: int
+    qry_height: int
+    width: int
+    ct_num: int
+    ref_data: np.ndarray
+    ref_indices: np.ndarray
+    ref_indptr: np.ndarray
+    qry_data: np.ndarray
+    qry_indices: np.ndarray
+    qry_indptr: np.ndarray
+    max_uniq_gene: int
+    ctidx: np.ndarray
+    ctdidx: np.ndarray
+    ctdiff: np.ndarray
+    genes: np.ndarray
+
+
+class Prep:
+    """Host preprocessing (DataParser, src/io.cpp)."""
+
+    def __init__(self, ds, cores: int = 8):
+        L = lib()
+        self._h = _V()
+        rip = np.ascontiguousarray(ds.ref_indptr, dtype=np.int32)
+        qip = np.ascontiguousarray(ds.qry_indptr, dtype=np.int32)
+        rd = np.ascontiguousarray(ds.ref_data, dtype=np.float32)
+        ri = np.ascontiguousarray(ds.ref_indices, dtype=np.int32)
+        qd = np.ascontiguousarray(ds.qry_data, dtype=np.float32)
+        qi = np.ascontiguousarray(ds.qry_indices, dtype=np.int32)
+        ct = np.ascontiguousarray(ds.celltype, dtype=np.int32)
+        rg, _k1 = _names(ds.ref_genes)
+        qg, _k2 = _names(ds.qry_genes)
+        err = C.create_string_buffer(512)
+        t0 = time.perf_counter()
+        rc = L.csg_prep_train(cores, ds.ref_height, ds.ref_width, _p(rd), _p(ri), _p(rip),
+                              ds.qry_height, ds.qry_width, _p(qd), _p(qi), _p(qip), len(ds.codes),
+                              _p(ct), C.cast(rg, _V), len(ds.ref_genes), C.cast(qg, _V),
+                              len(ds.qry_genes), C.byref(self._h), err, 512)
+        self.train_seconds = time.perf_counter() - t0
+        if rc != 0:
+            raise CsgError(rc, err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().csg_prep_release(self._h)
+            self._h = None
+
+    def generate(self, step: int) -> Coded:
+        c = CodedStruct()
+        rc = lib().csg_prep_generate(self._h, step, C.byref(c))
+        if rc != 0:
+            raise CsgError(rc, f"generateDenseMatrix({step}) failed")
+        rn = _np(c.ref_indptr, c.ref_height + 1, np.int32)
+        qn = _np(c.qry_indptr, c.qry_height + 1, np.int32)
+        L = c.ct_num
+        nd = 2 * L * L
+        return Coded(c.ref_height, c.qry_height, c.width, c.ct_num,
+                     _np(c.ref_data, int(rn[-1]), np.uint16), _np(c.ref_indices, int(rn[-1]), np.int32),
+                     rn, _np(c.qry_data, int(qn[-1]), np.uint16),
+                     _np(c.qry_indices, int(qn[-1]), np.int32), qn, int(c.max_uniq_gene),
+                     _np(c.ctidx, 2 * L, np.uint32), _np(c.ctdidx, nd, np.uint32),
+                     _np(c.ctdiff, int(c.n_ctdiff), np.uint32), _np(c.genes, c.width, np.uint32))
+
+    def table(self, which: int) -> np.ndarray:
+        n = lib().csg_prep_train_table(self._h, which, None)
+        out = np.zeros(n, dtype=np.uint32)
+        lib().csg_prep_train_table(self._h, which, _p(out))
+        return out
+
+    @property
+    def filtered(self) -> bool:
+        return bool(lib().csg_prep_filtered(self._h))
+
+
+class Engine:
+    """Device engine (include/cusingler_amd.h)."""
+
+    def __init__(self, gpuid: int = 0):
+        self._h = _V()
+        rc = lib().csg_init_gpu(gpuid, C.byref(self._h))
+        if rc != 0:
+            raise CsgError(rc, f"initGPU({gpuid}) failed")
+        self._keep: List[np.ndarray] = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().csg_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise CsgError(rc, f"{what}: {lib().csg_last_error(self._h).decode()}")
+
+    def copyin_score(self, c: Coded):
+        self._check(lib().csg_copyin_score(
+            self._h, c.ref_height, c.width, _p(c.ref_data), _p(c.ref_indices), _p(c.ref_indptr),
+            c.qry_height, _p(c.qry_data), _p(c.qry_indices), _p(c.qry_indptr), c.ct_num,
+            _p(c.ctidx), c.max_uniq_gene), "copyin_score")
+        self.Nq, self.L = c.qry_height, c.ct_num
+
+    def get_label(self, quantile: float = 0.8, thre: float = 0.05, path: str = "plain",
+                  outputs: bool = True):
+        first = np.zeros(self.Nq, np.int32) if outputs else None
+        mask = np.zeros((self.Nq, self.L), np.float32) if outputs else None
+        scores = np.zeros((self.Nq, self.L), np.float32) if outputs else None
+        self._check(lib().csg_get_label(
+            self._h, quantile, thre, PATHS[path], _p(first) if outputs else None,
+            _p(mask) if outputs else None, _p(scores) if outputs else None), "get_label")
+        return first, mask, scores
+
+    def destroy_score(self):
+        self._check(lib().csg_destroy_score(self._h), "destroy_score")
+
+    def copyin(self, c: Coded, mask: Optional[np.ndarray] = None):
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        self._keep = [m] if m is not None else []
+        self._check(lib().csg_copyin(
+            self._h, c.ref_height, c.width, _p(c.ref_data), _p(c.ref_indices), _p(c.ref_indptr),
+            c.qry_height, _p(c.qry_data), _p(c.qry_indices), _p(c.qry_indptr), c.ct_num,
+            _p(c.ctidx), _p(c.ctdidx), _p(c.ctdiff), len(c.ctdiff), c.max_uniq_gene,
+            _p(m) if m is not None else None), "copyin")
+        self.fNq = c.qry_height
+
+    def finetune(self, quantile: float = 0.8, thre: float = 0.05, times: int = 0):
+        final = np.zeros(self.fNq, np.uint32)
+        rounds = np.zeros(self.fNq, np.int32)
+        self._check(lib().csg_finetune(self._h, quantile, thre, times, _p(final), _p(rounds)),
+                    "finetune")
+        return final, rounds
+
+    def destroy(self):
+        self._check(lib().csg_destroy(self._h), "destroy")
+
+    def phase_times(self, reset: bool = False) -> Dict[str, tuple]:
+        ms = np.zeros(N_PHASES, np.float64)
+        cnt = np.zeros(N_PHASES, np.int64)
+        self._check(lib().csg_phase_times(self._h, _p(ms), _p(cnt), int(reset)), "phase_times")
+        return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(N_PHASES)}
+
+    def counters(self) -> Dict[str, int]:
+        out = np.zeros(N_COUNTERS, np.int64)
+        self._check(lib().csg_counters(self._h, _p(out)), "counters")
+        return {COUNTERS[i]: int(out[i]) for i in range(N_COUNTERS)}
+
+
+def host_percentile(arr: np.ndarray, quantile: float) -> float:
+    a = np.ascontiguousarray(arr, dtype=np.float32)
+    out = C.c_float()
+    rc = lib().csg_host_percentile(_p(a), a.shape[0], quantile, C.byref(out))
+    if rc != 0:
+        raise CsgError(rc, "percentile undefined for this input")
+    return out.value
+
+
+def host_minmax(arr: np.ndarray):
+    a = np.ascontiguousarray(arr, dtype=np.float32)
+    mn, mx = C.c_int(), C.c_int()
+    lib().csg_host_minmax(_p(a), a.shape[0], C.byref(mn), C.byref(mx))
+    return mn.value, mx.value
+
+
+@dataclass
+class Annotation:
+    first: np.ndarray
+    final: np.ndarray
+    mask: np.ndarray
+    scores: np.ndarray
+    rounds: np.ndarray
+    timings: Dict[str, float] = field(default_factory=dict)
+    counters: Dict[str, int] = field(default_factory=dict)
+
+
+def annotate(ds, path: str = "plain", quantile: float = 0.8, thre: float = 0.05, times: int = 0,
+             cores: int = 8, gpuid: int = 0, prep: Optional[Prep] = None,
+             engine: Optional[Engine] = None) -> Annotation:
+    """train -> score -> finetune on one dataset; label indices are returned."""
+    prep = prep or Prep(ds, cores)
+    eng = engine or Engine(gpuid)
+    c0 = prep.generate(0)
+    eng.copyin_score(c0)
+    t0 = time.perf_counter()
+    first, mask, scores = eng.get_label(quantile, thre, path)
+    t1 = time.perf_counter()
+    cnt = eng.counters()
+    eng.destroy_score()
+    c1 = prep.generate(1)
+    eng.copyin(c1)
+    t2 = time.perf_counter()
+    final, rounds = eng.finetune(quantile, thre, times)
+    t3 = time.perf_counter()
+    cnt2 = eng.counters()
+    cnt.update({k: v for k, v in cnt2.items() if k.startswith("ft_")})
+    eng.destroy()
+    return Annotation(first, final, mask, scores, rounds,
+                      {"score_s": t1 - t0, "finetune_s": t3 - t2, "train_s": prep.train_seconds},
+                      cnt)

@@ -1,0 +1,139 @@
+/* C-ABI of the MI355X SingleR annotation engine (libcusingler_amd.so).
+ *
+ * Each entry point replaces one function of the reference's device API
+ * (src/cusingler.cuh:13-22) and keeps its meaning: the same inputs (the coded
+ * CSR matrices, label ranges and pair gene windows the host preprocessing
+ * produces, src/types.h:21-42), the same outputs (first-label indices, the
+ * 0/1 candidate mask, final-label indices).  Differences at the boundary:
+ *   - state lives in an opaque engine handle instead of file-scope globals
+ *     (src/cusingler.cu:26-46), so several engines may coexist;
+ *   - every function returns a status code (0 = success) instead of calling
+ *     exit() (src/cusingler.cu:48-57); csg_last_error() gives the message;
+ *   - plain pointers and sizes only, no C++ types.
+ *
+ * Index conventions: reference rows are grouped by label, label l owning rows
+ * [ctidx[2l], ctidx[2l] + ctidx[2l+1]) (src/io.cpp:616-684); value codes are
+ * the per-row dense ranks of distinct stored values, starting at 1, with 0
+ * for implicit zeros (src/io.cpp:470-523).
+ */
+#ifndef CUSINGLER_AMD_H
+#define CUSINGLER_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct csg_engine csg_engine;
+
+enum csg_status
+{
+    CSG_OK            = 0,
+    CSG_ERR_ARGUMENT  = 1, /* invalid sizes, codes or indices */
+    CSG_ERR_DEVICE    = 2, /* HIP runtime error (the message names the call) */
+    CSG_ERR_MEMORY    = 3, /* not enough device memory for the requested plan */
+    CSG_ERR_STATE     = 4, /* entry points called out of order */
+    CSG_ERR_UNDEFINED = 5  /* the reference's own behaviour is undefined here */
+};
+
+/* Correlation forms of the score step (csg_get_label `path`). */
+enum csg_path
+{
+    CSG_PATH_PLAIN      = 0, /* FP32, reference summation order: bit-exact scores */
+    CSG_PATH_EXACT_INT8 = 1, /* exact rank dot products on INT8-limb MFMA, FP64
+                                normalisation, certified labels with FP32 recheck */
+    CSG_PATH_FP32_MFMA  = 2, /* FP32-in/FP32-accumulate MFMA (approximate) */
+    CSG_PATH_FP16_MFMA  = 3  /* FP16-in/FP32-accumulate MFMA (approximate) */
+};
+
+/* initGPU (src/cusingler.cuh:15, src/cusingler.cu:74-90): selects device
+ * `gpuid` and creates an engine bound to it. */
+int csg_init_gpu(int gpuid, csg_engine** out);
+
+/* Message of the last failing call on this engine ("" if none). */
+const char* csg_last_error(const csg_engine* eng);
+
+/* Releases every device buffer and the engine itself. */
+void csg_release(csg_engine* eng);
+
+/* copyin_score (src/cusingler.cuh:18, src/cusingler.cu:283-413): uploads the
+ * score-step inputs, coded over the score gene set (width = |common_genes|).
+ * max_uniq_gene is the preprocessing's bound on codes + 1
+ * (src/io.cpp:502,520).  ctidx holds 2*ct_num entries. */
+int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width,
+                     const uint16_t* ref_data, const int32_t* ref_indices,
+                     const int32_t* ref_indptr, uint32_t qry_height, const uint16_t* qry_data,
+                     const int32_t* qry_indices, const int32_t* qry_indptr, uint32_t ct_num,
+                     const uint32_t* ctidx, uint64_t max_uniq_gene);
+
+/* get_label (src/cusingler.cuh:19, src/cusingler.cu:986-1136): Spearman
+ * correlation of every query cell with every reference cell, the per-label
+ * `quantile` (a fraction, 0.8 by default) of those scores, the first label
+ * (last maximum) and the candidate mask (score >= max - finetune_thre).
+ * Outputs may be NULL; results stay on the device for csg_finetune either way.
+ *   first_labels : int32 [qry_height]
+ *   labels_mask  : float [qry_height * ct_num] (0/1, the reference's `labels`)
+ *   scores       : float [qry_height * ct_num] per-label quantile scores */
+int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path,
+                  int32_t* first_labels, float* labels_mask, float* scores);
+
+/* destroy_score (src/cusingler.cuh:20): frees the score-step buffers. */
+int csg_destroy_score(csg_engine* eng);
+
+/* copyin (src/cusingler.cuh:24, src/cusingler.cu:415-521): uploads the
+ * fine-tune inputs, coded over the fine-tune gene set (width = |thre_genes|),
+ * and the per ordered label pair gene windows: ctdidx holds [pos, len] at
+ * (i*ct_num + j)*2 into ctdiff (gene indices in the fine-tune space).
+ * labels_mask (qry_height*ct_num floats) may be NULL to use the mask the last
+ * csg_get_label left on the device. */
+int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint16_t* ref_data,
+               const int32_t* ref_indices, const int32_t* ref_indptr, uint32_t qry_height,
+               const uint16_t* qry_data, const int32_t* qry_indices, const int32_t* qry_indptr,
+               uint32_t ct_num, const uint32_t* ctidx, const uint32_t* ctdidx,
+               const uint32_t* ctdiff, uint64_t n_ctdiff, uint64_t max_uniq_gene,
+               const float* labels_mask);
+
+/* cufinetune (src/cusingler.cuh:26, src/cusingler.cu:1297-1333): iterative
+ * refinement of every cell's candidate labels; finetune_times == 0 runs until
+ * one label is left.  final_labels: uint32 [qry_height]; rounds (optional):
+ * int32 [qry_height] rounds run per cell. */
+int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
+                 uint32_t* final_labels, int32_t* rounds);
+
+/* destroy (src/cusingler.cuh:25): frees the fine-tune buffers. */
+int csg_destroy(csg_engine* eng);
+
+/* Per-phase device time of the calls since the last reset, from HIP events on
+ * the engine's stream, in milliseconds: [0] rank, [1] correlation,
+ * [2] quantile select, [3] label/mask, [4] certification recheck,
+ * [5] fine-tune total, [6] fine-tune correlation, [7] fine-tune rank,
+ * [8] fine-tune select+filter.  counts[i] is the number of launches timed in
+ * phase i.  Either pointer may be NULL. */
+#define CSG_N_PHASES 9
+int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
+
+/* Counters of the last csg_get_label / csg_finetune call:
+ * [0] query cells, [1] score-step gene count G_s, [2] reference cells,
+ * [3] (cell, label) pairs re-scored in FP32 by the exact path,
+ * [4] cells the exact path could not certify, [5] NaN-carrying segments,
+ * [6] fine-tune rounds, [7] fine-tune (cell, round) items,
+ * [8] fine-tune candidate-set groups, [9] fine-tune correlation pairs,
+ * [10] fine-tune sum over items of gene-subset size * reference rows. */
+#define CSG_N_COUNTERS 11
+int csg_counters(csg_engine* eng, int64_t* out);
+
+/* Host-side evaluation of the reference's `percentile` (src/cusingler.cu:
+ * 947-978) with the same libstdc++ sort emulation the device uses; returns
+ * CSG_ERR_UNDEFINED if the library's unguarded scans would leave the array. */
+int csg_host_percentile(const float* arr, int64_t len, float quantile, float* out);
+
+/* Host-side libstdc++ minmax_element emulation (first min, last max). */
+void csg_host_minmax(const float* arr, int n, int* imin, int* imax);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUSINGLER_AMD_H */
