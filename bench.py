@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the SingleR annotation path on one MI355X.
+
+A step is one pass of the hot path over the whole query batch: the score step
+(rank, correlate every query cell with every reference cell, per-label 0.8
+quantile, first label and candidate mask) followed by the fine-tune loop, with
+the coded inputs already resident in HBM.  `value` is annotated query cells
+per second over the timed steps.
+
+Default workload: the 200,000-query x 20,000-reference, 50-label atlas
+(synthetic, seeded).  Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_PEAK = {"hbm_GBps": 8000.0, "fp32_tflops": 157.3, "fp16_tflops": 2500.0,
+               "int8_tops": 5000.0}
+
+
+def _hip_sync():
+    try:
+        ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def cpu_baseline(ds, n_sample: int, cores: int):
+    """The oracle (a CPU restatement of the reference's score + fine-tune steps)
+    on the first n_sample query cells."""
+    from oracle import oracle as O
+    import numpy as np
+    orc = O.Oracle(ds, cores)
+    t0 = time.perf_counter()
+    _, mask, _ = orc.score(0, n_sample, 0.8, 0.05, O.VAR_SEQ)
+    orc.finetune(0, n_sample, mask, 0.8, 0.05, 0, O.VAR_SEQ)
+    dt = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "cells/s", "cores": cores, "kind": "port",
+            "sample": f"first {n_sample} of {ds.qry_height} query cells, score + fine-tune, "
+                      f"{dt:.1f} s on {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="atlas")
+    ap.add_argument("--nq", type=int, default=None, help="override the query cell count")
+    ap.add_argument("--path", default="plain", choices=["plain", "exact_int8", "fp32_mfma", "fp16"])
+    ap.add_argument("--cpu-sample", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cores", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the annotation path shards query cells across GPUs "
+              f"by design but this build measures one GPU's share (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from cusingler_amd import engine, synth
+
+    t0 = time.perf_counter()
+    ds = synth.generate(args.config, n_qry=args.nq)
+    t_gen = time.perf_counter() - t0
+    print(f"[bench] generated {args.config}: {ds.qry_height} query x {ds.ref_height} reference "
+          f"cells in {t_gen:.1f} s", file=sys.stderr, flush=True)
+    prep = engine.Prep(ds, args.cores)
+    c0 = prep.generate(0)
+    c1 = prep.generate(1)
+    print(f"[bench] preprocessing {prep.train_seconds:.1f} s, G_s={c0.width} G_f={c1.width}",
+          file=sys.stderr, flush=True)
+
+    eng = engine.Engine(0)
+    eng.copyin_score(c0)
+    eng.copyin(c1)  # fine-tune inputs; the candidate mask comes from the device
+
+    def step():
+        eng.get_label(0.8, 0.05, args.path, outputs=False)
+        return eng.finetune(0.8, 0.05, 0)
+
+    for i in range(args.warmup):
+        step()
+        print(f"[bench] warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    eng.phase_times(reset=True)
+    _hip_sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        print(f"[bench] step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    _hip_sync()
+    dt = time.perf_counter() - t0
+    ph = eng.phase_times()
+    cnt = eng.counters()
+    Nq, Nr, Gs = ds.qry_height, ds.ref_height, c0.width
+    value = Nq * args.steps / dt
+
+    corr_ms, corr_n = ph["corr"]
+    flops = 2.0 * Nq * Nr * Gs * args.steps          # one logical MAC = 2 ops
+    achieved = flops / (corr_ms * 1e-3) / 1e12 if corr_ms > 0 else 0.0
+    if args.path == "exact_int8":
+        peak, unit = MI355X_PEAK["int8_tops"], "TOP/s"
+    elif args.path == "fp16":
+        peak, unit = MI355X_PEAK["fp16_tflops"], "TFLOP/s"
+    else:
+        peak, unit = MI355X_PEAK["fp32_tflops"], "TFLOP/s"
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 5), "traffic": None,
+                "kernel": "k_gemm_plain" if args.path == "plain" else "score contraction",
+                "launches": corr_n, "avg_launch_ms": round(corr_ms / max(corr_n, 1), 4)}
+
+    out = {
+        "metric": "annotated query cells/sec on 1 MI355X (score step + fine-tune, inputs "
+                  "resident in HBM)",
+        "value": round(value, 1), "unit": "cells/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.path in ("plain", "fp32_mfma") else
+                 ("int8" if args.path == "exact_int8" else "f16"),
+        "data": "synthetic (seeded NB counts, log2(1+x), float32 CSR)",
+        "config": {"workload": f"{args.config}: {Nq} query x {Nr} reference cells, "
+                               f"{len(ds.codes)} labels, G_s={Gs}, G_f={c1.width}",
+                   "path": args.path, "query_cells": Nq, "reference_cells": Nr,
+                   "labels": len(ds.codes), "score_genes": Gs, "finetune_genes": c1.width},
+        "roofline": roofline,
+        "phases_ms": {k: round(v[0], 3) for k, v in ph.items()},
+        "counters": cnt,
+    }
+    eng.close()
+    if not args.no_cpu:
+        n = min(args.cpu_sample, Nq)
+        print(f"[bench] CPU baseline on {n} cells, {args.cores} threads", file=sys.stderr,
+              flush=True)
+        out["cpu_baseline"] = cpu_baseline(ds, n, args.cores)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

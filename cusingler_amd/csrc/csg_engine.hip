@@ -779,6 +779,16 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
     for (auto& g : b.groups)
         max_gs = std::max(max_gs, g.gs);
     const size_t stride_max = round_up(max_gs, 8);
+    const size_t lds_rank_q = (size_t)((eng->fnbins + 3) & ~3) * 4 +
+                              round_up(std::max<uint32_t>(eng->Gf, 1), 8) * 2 + stride_max * 4;
+    const size_t lds_corr   = (size_t)(FT_TC + FT_TR) * stride_max * 2;
+    if (lds_rank_q > 65536 || lds_corr > 65536 || b.tiles.empty() || b.ref_items.empty() ||
+        b.cell_items.empty())
+    {
+        cleanup();
+        return eng->fail(CSG_ERR_ARGUMENT, "fine-tune gene subset of " + std::to_string(max_gs) +
+                                               " genes exceeds the kernels' LDS plan");
+    }
     {
         PhaseScope ps(eng, 7, 2);
         const size_t smem_r = (size_t)((eng->fnbins + 3) & ~3) * 4 + stride_max * 4;

@@ -278,7 +278,7 @@ __global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
         uint32_t       j   = 0;
         while (j < g.nlabel && gl[j] != lab)
             ++j;
-        return P[fc.seg0 + j];
+        return j < g.nlabel ? P[fc.seg0 + j] : __builtin_nanf("");
     };
     // minmax_element over the scores in list order
     int mn = 0, mx = 0;

@@ -9,9 +9,13 @@ size 5 (dispersion 0.2); the mean of gene g in cell c is
 
 with base ~ LogNormal(0, 1), lib ~ LogNormal(8, 0.3) normalised to mean 1, and
 s calibrated so that the expected nonzero fraction hits the configured target.
-Labels up-regulate marker genes; near-duplicate labels share a fraction of
-their markers (families / parent-child hierarchies), which is what makes the
-fine-tune step see several candidate labels per cell.
+Labels up-regulate marker genes drawn from the 5% of genes with the highest
+baseline (markers are well-expressed genes, so they are detected at 4-8%
+density); near-duplicate labels share a fraction of their markers (families /
+parent-child hierarchies), which is what makes the fine-tune step see several
+candidate labels per cell: on the atlas the candidate sets are Poisson-skewed
+over 1-14 labels (median 2), on the streaming hierarchy they are the ~10
+children of one parent.
 
 The per-entry sampling runs in ``libcsg_synth.so`` (OpenMP C++), keyed by
 (seed, stream, row, gene) so the data does not depend on the thread count.
@@ -54,6 +58,7 @@ class SynthConfig:
     n_parents: int = 0
     seed: int = 1234
     n_train: int = 0         # extra query-like cells (learned proposal training set)
+    marker_pool: float = 0.05  # markers are drawn from this top fraction of genes by baseline
 
 
 CONFIGS: Dict[str, SynthConfig] = {
@@ -154,7 +159,7 @@ def _label_families(cfg: SynthConfig, rng: np.random.Generator) -> List[List[int
 
 
 def _fold_matrix(cfg: SynthConfig, rng: np.random.Generator, common_pos: np.ndarray,
-                 n_universe: int) -> np.ndarray:
+                 n_universe: int, base: np.ndarray) -> np.ndarray:
     """Per-label fold over the gene universe; markers are drawn from shared genes."""
     L = cfg.n_labels
     fold = np.ones((L, n_universe), dtype=np.float32)
@@ -164,6 +169,11 @@ def _fold_matrix(cfg: SynthConfig, rng: np.random.Generator, common_pos: np.ndar
     # candidates exclude the first 32 shared genes so the low-index genes stay
     # un-marked (they fill the zero-difference slots of the gene sets)
     pool = common_pos[32:] if common_pos.shape[0] > 64 + cfg.markers else common_pos
+    if cfg.marker_pool < 1.0:
+        # markers are genes that are well expressed at baseline (as real markers are)
+        n_keep = max(4 * cfg.markers, int(round(pool.shape[0] * cfg.marker_pool)))
+        order = np.argsort(-base[pool], kind="stable")
+        pool = np.sort(pool[order[:n_keep]])
     for fam in fams:
         shared = rng.choice(pool, size=n_shared, replace=False) if n_shared else np.zeros(0, np.int64)
         shared_fold = rng.uniform(cfg.fold_lo, cfg.fold_hi, size=n_shared).astype(np.float32)
@@ -230,7 +240,7 @@ def generate(cfg: SynthConfig | str, n_qry: Optional[int] = None, n_ref: Optiona
     qry_order = interleave(qry_only)
 
     base = rng.lognormal(0.0, 1.0, size=n_uni).astype(np.float32)
-    fold = _fold_matrix(cfg, rng, common, n_uni)
+    fold = _fold_matrix(cfg, rng, common, n_uni, base)
     mu_uni = base[None, :] * fold  # [L, universe]
 
     # reference labels (grouped sizes), shuffled cell order
