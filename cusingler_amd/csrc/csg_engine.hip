@@ -19,8 +19,10 @@
 #include <vector>
 
 #include "../../include/cusingler_amd.h"
+#include "csg_exact.h"
 #include "csg_ft_kernels.h"
 #include "csg_kernels.h"
+#include "csg_mfma.h"
 
 using namespace csg;
 
@@ -71,6 +73,17 @@ struct csg_engine
     uint64_t* d_nan_list = nullptr;
     unsigned int* d_nan_count = nullptr;
     int*      d_status = nullptr;
+    // exact / approximate MFMA paths
+    int8_t *  d_rh = nullptr, *d_rl = nullptr, *d_qh = nullptr, *d_ql = nullptr;
+    _Float16 *d_rf16 = nullptr, *d_qf16 = nullptr;
+    int       ref_prepared_path = -1;
+    uint8_t*  d_cstatus = nullptr;
+    uint8_t*  d_exact = nullptr;
+    RecheckItem* d_items = nullptr;
+    unsigned int* d_nitems = nullptr;
+    float*    d_Pref = nullptr;
+    uint64_t* d_item_off = nullptr;
+    Segment*  d_item_seg = nullptr;
 
     // ---- fine-tune ----
     bool      ft_loaded = false;
@@ -361,6 +374,20 @@ int csg_destroy_score(csg_engine* eng)
     dfree(eng->d_sqq_i);
     dfree(eng->d_S);
     dfree(eng->d_nan_list);
+    dfree(eng->d_rh);
+    dfree(eng->d_rl);
+    dfree(eng->d_qh);
+    dfree(eng->d_ql);
+    dfree(eng->d_rf16);
+    dfree(eng->d_qf16);
+    dfree(eng->d_cstatus);
+    dfree(eng->d_exact);
+    dfree(eng->d_items);
+    dfree(eng->d_nitems);
+    dfree(eng->d_Pref);
+    dfree(eng->d_item_off);
+    dfree(eng->d_item_seg);
+    eng->ref_prepared_path = -1;
     eng->pct_quantile = -1.f;
     eng->score_loaded = false;
     eng->ref_ranked   = false;
@@ -491,14 +518,140 @@ int rank_rows(csg_engine* eng, const uint16_t* data, const int* ind, const int* 
     return CSG_OK;
 }
 
-int score_chunk_plain(csg_engine* eng, uint32_t q0, uint32_t nq)
+int score_chunk(csg_engine* eng, int path, uint32_t nq)
 {
-    dim3 grid((eng->Nr + PG_BN - 1) / PG_BN, (nq + PG_BM - 1) / PG_BM);
-    k_gemm_plain<<<grid, 256, 0, eng->stream>>>(eng->d_Xq, eng->d_Xr, (int)eng->Kp, eng->d_sqq_f,
-                                                eng->d_sqr_f, (int)nq, (int)eng->Nr, eng->d_S,
-                                                (size_t)eng->Nr);
+    if (path == CSG_PATH_PLAIN)
+    {
+        dim3 grid((eng->Nr + PG_BN - 1) / PG_BN, (nq + PG_BM - 1) / PG_BM);
+        k_gemm_plain<<<grid, 256, 0, eng->stream>>>(eng->d_Xq, eng->d_Xr, (int)eng->Kp,
+                                                    eng->d_sqq_f, eng->d_sqr_f, (int)nq,
+                                                    (int)eng->Nr, eng->d_S, (size_t)eng->Nr);
+    }
+    else if (path == CSG_PATH_EXACT_INT8)
+    {
+        dim3 grid((eng->Nr + I8_BN - 1) / I8_BN, (nq + I8_BM - 1) / I8_BM);
+        k_gemm_int8<<<grid, 256, 0, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                                                   (int)eng->Kp, eng->d_sqq_f, eng->d_sqr_f,
+                                                   (int)nq, (int)eng->Nr, eng->d_S,
+                                                   (size_t)eng->Nr);
+    }
+    else if (path == CSG_PATH_FP16_MFMA)
+    {
+        dim3 grid((eng->Nr + H_BN - 1) / H_BN, (nq + H_BM - 1) / H_BM);
+        k_gemm_f16<<<grid, 256, 0, eng->stream>>>(eng->d_qf16, eng->d_rf16, (int)eng->Kp,
+                                                  eng->d_sqq_f, eng->d_sqr_f, (int)nq,
+                                                  (int)eng->Nr, eng->d_S, (size_t)eng->Nr);
+    }
+    else
+    {
+        dim3 grid((eng->Nr + F_BN - 1) / F_BN, (nq + F_BM - 1) / F_BM);
+        k_gemm_f32mfma<<<grid, 256, 0, eng->stream>>>(eng->d_Xq, eng->d_Xr, (int)eng->Kp,
+                                                      eng->d_sqq_f, eng->d_sqr_f, (int)nq,
+                                                      (int)eng->Nr, eng->d_S, (size_t)eng->Nr);
+    }
     LAUNCHCHK();
-    (void)q0;
+    return CSG_OK;
+}
+
+// Operand planes of `rows` rows of X for the MFMA paths.
+int prepare_operands(csg_engine* eng, int path, const int16_t* X, size_t rows, int8_t* h,
+                     int8_t* l, _Float16* f)
+{
+    const size_t n = rows * eng->Kp;
+    const unsigned blocks = (unsigned)((n / 8 + 255) / 256);
+    if (path == CSG_PATH_EXACT_INT8)
+        k_split_limbs<<<blocks, 256, 0, eng->stream>>>(X, n, h, l);
+    else if (path == CSG_PATH_FP16_MFMA)
+        k_to_half<<<blocks, 256, 0, eng->stream>>>(X, n, f);
+    else
+        return CSG_OK;
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// Half-width of the reference's per-label quantile around the exact path's
+// (see csg_exact.h): gamma_{G-1} / (1 - gamma - 2u) + 3u per score, plus 16u
+// for the two float evaluations of the quantile formula.
+double certify_width(uint32_t G)
+{
+    const double u     = 5.9604644775390625e-08;
+    const double n     = G > 1 ? (double)(G - 1) : 0.0;
+    const double gamma = n * u / (1.0 - n * u);
+    const double wpair = gamma / (1.0 - gamma - 2 * u) + 3 * u;
+    return wpair * (1.0 + 1e-6) + 16 * u;
+}
+
+int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, float thre)
+{
+    const uint32_t L  = eng->L;
+    float*         P  = eng->d_P + (size_t)q0 * L;
+    int32_t*       F  = eng->d_first + q0;
+    float*         M  = eng->d_mask + (size_t)q0 * L;
+    const double   wp = certify_width(eng->G);
+    HIPCHK(hipMemsetAsync(eng->d_nitems, 0, sizeof(unsigned int), eng->stream));
+    HIPCHK(hipMemsetAsync(eng->d_exact, 0, (size_t)nq * L, eng->stream));
+    {
+        PhaseScope ps(eng, 3, 1);
+        k_label_certify<<<(nq + 255) / 256, 256, 0, eng->stream>>>(
+            P, nq, L, wp, thre, F, M, eng->d_cstatus, eng->d_items, eng->d_nitems);
+        LAUNCHCHK();
+    }
+    unsigned int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, eng->d_nitems, sizeof(n), hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    if (n == 0)
+        return CSG_OK;
+    std::vector<RecheckItem> items(n);
+    HIPCHK(hipMemcpy(items.data(), eng->d_items, n * sizeof(RecheckItem), hipMemcpyDeviceToHost));
+    std::vector<uint64_t> off(n), seg_len(n);
+    std::vector<Segment>  segs(n);
+    std::vector<uint8_t>  cell_seen(nq, 0);
+    uint64_t              tot = 0;
+    for (unsigned int i = 0; i < n; ++i)
+    {
+        const uint32_t len = eng->ctidx[2 * items[i].l + 1];
+        off[i]             = tot;
+        segs[i]            = Segment{ tot, len, items[i].l };
+        seg_len[i]         = len;
+        tot += len;
+        if (!cell_seen[items[i].q])
+        {
+            cell_seen[items[i].q] = 1;
+            eng->counters[4]++;
+        }
+    }
+    eng->counters[3] += n;
+    if (tot > (uint64_t)eng->Qc * eng->Nr)
+        return eng->fail(CSG_ERR_MEMORY, "re-scored pairs exceed the score buffer");
+    int rc;
+    if ((rc = upload(eng, &eng->d_item_off, off.data(), n)) ||
+        (rc = upload(eng, &eng->d_item_seg, segs.data(), n)))
+        return rc;
+    {
+        PhaseScope ps(eng, 4, 3);
+        k_rescore<<<n, 64, 0, eng->stream>>>(eng->d_items, eng->d_item_off, eng->d_Xq, eng->d_Xr,
+                                             (int)eng->Kp, eng->d_sqq_f, eng->d_sqr_f,
+                                             eng->d_lstart, eng->d_llen, eng->d_S);
+        LAUNCHCHK();
+        HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
+        TableSegments src{ eng->d_item_seg };
+        k_select<TableSegments><<<(n + 3) / 4, 256, 0, eng->stream>>>(
+            src, n, eng->d_S, eng->d_pct, quantile, eng->d_Pref, eng->d_nan_list,
+            eng->d_nan_count);
+        LAUNCHCHK();
+    }
+    if ((rc = resolve_nan_segments(eng, TableSegments{ eng->d_item_seg }, {}, eng->d_S, eng->d_pct,
+                                   quantile, eng->d_Pref, &seg_len)))
+        return rc;
+    {
+        PhaseScope ps(eng, 4, 2);
+        k_apply_recheck<<<(n + 255) / 256, 256, 0, eng->stream>>>(eng->d_items, n, eng->d_Pref, P,
+                                                                  L, eng->d_exact);
+        LAUNCHCHK();
+        k_label_final<<<(nq + 255) / 256, 256, 0, eng->stream>>>(P, eng->d_cstatus, eng->d_exact,
+                                                                 nq, L, thre, F, M);
+        LAUNCHCHK();
+    }
     return CSG_OK;
 }
 
@@ -511,8 +664,8 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
         return CSG_ERR_ARGUMENT;
     if (!eng->score_loaded)
         return eng->fail(CSG_ERR_STATE, "csg_get_label before csg_copyin_score");
-    if (path != CSG_PATH_PLAIN)
-        return eng->fail(CSG_ERR_ARGUMENT, "correlation path not available in this build");
+    if (path < CSG_PATH_PLAIN || path > CSG_PATH_FP16_MFMA)
+        return eng->fail(CSG_ERR_ARGUMENT, "unknown correlation path");
     HIPCHK(hipSetDevice(eng->device));
     int rc = set_pct(eng, &eng->d_pct, &eng->pct_quantile, eng->ctidx, eng->L, quantile);
     if (rc)
@@ -521,28 +674,58 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     eng->counters[0] = eng->Nq;
     eng->counters[1] = eng->G;
     eng->counters[2] = eng->Nr;
+    const size_t nr_pad = round_up(eng->Nr, 128);
+    const size_t qc     = eng->Qc;
+    if (path == CSG_PATH_EXACT_INT8 && !eng->d_rh)
+    {
+        if ((rc = dalloc(eng, &eng->d_rh, nr_pad * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_rl, nr_pad * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_qh, qc * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_ql, qc * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_cstatus, qc)) ||
+            (rc = dalloc(eng, &eng->d_exact, qc * eng->L)) ||
+            (rc = dalloc(eng, &eng->d_items, qc * eng->L)) ||
+            (rc = dalloc(eng, &eng->d_nitems, 1)) || (rc = dalloc(eng, &eng->d_Pref, qc * eng->L)))
+            return rc;
+    }
+    if (path == CSG_PATH_FP16_MFMA && !eng->d_rf16)
+    {
+        if ((rc = dalloc(eng, &eng->d_rf16, nr_pad * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_qf16, qc * eng->Kp)))
+            return rc;
+    }
     if (!eng->ref_ranked)
     {
         PhaseScope ps(eng, 0, 1);
         if ((rc = rank_rows(eng, eng->d_rdata, eng->d_rind, eng->d_rptr, 0, eng->Nr, eng->d_Xr,
                             eng->d_sqr_f, eng->d_sqr_i)))
             return rc;
-        eng->ref_ranked = true;
+        eng->ref_ranked        = true;
+        eng->ref_prepared_path = -1;
     }
-    // host copy of segment lengths per score item is only needed for NaN work sizes
+    if ((path == CSG_PATH_EXACT_INT8 || path == CSG_PATH_FP16_MFMA) && eng->ref_prepared_path != path)
+    {
+        PhaseScope ps(eng, 0, 1);
+        if ((rc = prepare_operands(eng, path, eng->d_Xr, nr_pad, eng->d_rh, eng->d_rl, eng->d_rf16)))
+            return rc;
+        eng->ref_prepared_path = path;
+    }
     std::vector<uint64_t> seg_len;
     for (uint32_t q0 = 0; q0 < eng->Nq; q0 += eng->Qc)
     {
         const uint32_t nq = std::min(eng->Qc, eng->Nq - q0);
         {
-            PhaseScope ps(eng, 0, 1);
+            PhaseScope ps(eng, 0, path == CSG_PATH_PLAIN || path == CSG_PATH_FP32_MFMA ? 1 : 2);
             if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
                                 eng->d_sqq_f, eng->d_sqq_i)))
+                return rc;
+            if ((rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh,
+                                       eng->d_ql, eng->d_qf16)))
                 return rc;
         }
         {
             PhaseScope ps(eng, 1, 1);
-            if ((rc = score_chunk_plain(eng, q0, nq)))
+            if ((rc = score_chunk(eng, path, nq)))
                 return rc;
         }
         HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
@@ -555,6 +738,13 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
                 eng->d_nan_list, eng->d_nan_count);
             LAUNCHCHK();
         }
+        if (path == CSG_PATH_EXACT_INT8)
+        {
+            // NaN quantiles stay NaN here; the certification re-scores those cells
+            if ((rc = certify_chunk(eng, q0, nq, quantile, finetune_thre)))
+                return rc;
+            continue;
+        }
         if (seg_len.size() != n_items)
         {
             seg_len.resize(n_items);
@@ -565,6 +755,7 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
                                        eng->d_P + (size_t)q0 * eng->L, &seg_len)))
             return rc;
     }
+    if (path != CSG_PATH_EXACT_INT8)
     {
         PhaseScope ps(eng, 3, 1);
         k_label<<<(eng->Nq + 255) / 256, 256, 0, eng->stream>>>(eng->d_P, eng->Nq, eng->L,
@@ -1129,6 +1320,99 @@ int csg_counters(csg_engine* eng, int64_t* out)
     if (!eng || !out)
         return CSG_ERR_ARGUMENT;
     std::memcpy(out, eng->counters, sizeof(eng->counters));
+    return CSG_OK;
+}
+
+int csg_selftest(csg_engine* eng, int which, double* max_err)
+{
+    if (!eng || !max_err || which < 0 || which > 3)
+        return CSG_ERR_ARGUMENT;
+    HIPCHK(hipSetDevice(eng->device));
+    const int nq = 256, nr = 256, Kp = 192, G = 1500;
+    std::vector<int16_t> xq((size_t)nq * Kp), xr((size_t)nr * Kp);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto rnd = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return (int)(st % (2 * G + 1)) - G;
+    };
+    for (auto& v : xq)
+        v = (int16_t)rnd();
+    for (auto& v : xr)
+        v = (int16_t)rnd();
+    std::vector<float> ones(std::max(nq, nr), 1.f);
+    int16_t *dq = nullptr, *dr = nullptr;
+    int8_t *qh = nullptr, *ql = nullptr, *rh = nullptr, *rl = nullptr;
+    _Float16 *qf = nullptr, *rf = nullptr;
+    float *d1 = nullptr, *dS = nullptr;
+    int    rc = CSG_OK;
+    auto   cleanup = [&]() {
+        dfree(dq); dfree(dr); dfree(qh); dfree(ql); dfree(rh); dfree(rl); dfree(qf); dfree(rf);
+        dfree(d1); dfree(dS);
+    };
+    if ((rc = upload(eng, &dq, xq.data(), xq.size())) || (rc = upload(eng, &dr, xr.data(), xr.size())) ||
+        (rc = upload(eng, &d1, ones.data(), ones.size())) || (rc = dalloc(eng, &dS, (size_t)nq * nr)) ||
+        (rc = dalloc(eng, &qh, xq.size())) || (rc = dalloc(eng, &ql, xq.size())) ||
+        (rc = dalloc(eng, &rh, xr.size())) || (rc = dalloc(eng, &rl, xr.size())) ||
+        (rc = dalloc(eng, &qf, xq.size())) || (rc = dalloc(eng, &rf, xr.size())))
+    {
+        cleanup();
+        return rc;
+    }
+    const unsigned bq = (unsigned)((xq.size() / 8 + 255) / 256), br = (unsigned)((xr.size() / 8 + 255) / 256);
+    if (which == 0)
+    {
+        k_split_limbs<<<bq, 256, 0, eng->stream>>>(dq, xq.size(), qh, ql);
+        k_split_limbs<<<br, 256, 0, eng->stream>>>(dr, xr.size(), rh, rl);
+        k_gemm_int8<<<dim3(nr / I8_BN, nq / I8_BM), 256, 0, eng->stream>>>(qh, ql, rh, rl, Kp, d1, d1,
+                                                                          nq, nr, dS, nr);
+    }
+    else if (which == 1)
+    {
+        k_to_half<<<bq, 256, 0, eng->stream>>>(dq, xq.size(), qf);
+        k_to_half<<<br, 256, 0, eng->stream>>>(dr, xr.size(), rf);
+        k_gemm_f16<<<dim3(nr / H_BN, nq / H_BM), 256, 0, eng->stream>>>(qf, rf, Kp, d1, d1, nq, nr,
+                                                                        dS, nr);
+    }
+    else if (which == 2)
+        k_gemm_f32mfma<<<dim3(nr / F_BN, nq / F_BM), 256, 0, eng->stream>>>(dq, dr, Kp, d1, d1, nq,
+                                                                            nr, dS, nr);
+    else
+        k_gemm_plain<<<dim3(nr / PG_BN, nq / PG_BM), 256, 0, eng->stream>>>(dq, dr, Kp, d1, d1, nq,
+                                                                            nr, dS, nr);
+    hipError_t le = hipGetLastError();
+    std::vector<float> out((size_t)nq * nr);
+    hipError_t ce = hipMemcpyAsync(out.data(), dS, out.size() * 4, hipMemcpyDeviceToHost, eng->stream);
+    hipError_t se = hipStreamSynchronize(eng->stream);
+    cleanup();
+    if (le != hipSuccess || ce != hipSuccess || se != hipSuccess)
+        return eng->fail(CSG_ERR_DEVICE, "selftest kernel failed");
+    double worst = 0;
+    for (int m = 0; m < nq; ++m)
+        for (int n = 0; n < nr; ++n)
+        {
+            long long s4  = 0, sxx = 0, syy = 0;
+            float     seq = 0.f;
+            for (int k = 0; k < Kp; ++k)
+            {
+                const long long a = xq[(size_t)m * Kp + k], b = xr[(size_t)n * Kp + k];
+                s4 += a * b;
+                sxx += a * a;
+                syy += b * b;
+                seq = std::fma((float)a, (float)b, seq);
+            }
+            const float  dev = out[(size_t)m * nr + n];
+            double       err;
+            if (which == 0)
+                err = dev == (float)(double)s4 ? 0.0 : std::fabs((double)dev - (double)s4) + 1.0;
+            else if (which == 3)
+                err = dev == seq ? 0.0 : std::fabs((double)dev - (double)seq) + 1.0;
+            else
+                err = std::fabs((double)dev - (double)s4) / std::sqrt((double)sxx * (double)syy);
+            worst = std::max(worst, err);
+        }
+    *max_err = worst;
     return CSG_OK;
 }
 

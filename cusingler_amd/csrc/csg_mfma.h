@@ -1,0 +1,390 @@
+// MFMA score contractions (gfx950): the exact INT8-limb path and the two
+// approximate paths (FP16 in / FP32 accumulate, FP32 in / FP32 accumulate).
+//
+// Operand layout.  A 16x16 MFMA tile takes, in lane l of the wave, row
+// (l % 16) of A and column (l % 16) of B together with one slice of the K
+// block chosen by the lane group l / 16; the same slice index is used for A
+// and B, so the products are summed over the whole K block whatever order the
+// hardware walks the slice in.  The 16x16 result sits in lane l as rows
+// 4*(l/16) + i, i = 0..3, of column l % 16.
+#pragma once
+
+#include "csg_kernels.h"
+
+namespace csg {
+
+typedef int      v4i __attribute__((ext_vector_type(4)));
+typedef float    v4f __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------ limb split ---
+
+// X = 256*h + l with l = signed low byte, h = (X - l) / 256; both in int8 for
+// |X| <= 32639.  Writes the two planes of a [rows x Kp] int16 matrix.
+__global__ void k_split_limbs(const int16_t* __restrict__ X, size_t n, int8_t* __restrict__ hi,
+                              int8_t* __restrict__ lo)
+{
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    if (i >= n)
+        return;
+    const int4 v = *(const int4*)(X + i);
+    const int16_t* x = (const int16_t*)&v;
+    int8_t hh[8], ll[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+    {
+        const int l = (int)(int8_t)(x[u] & 0xff);
+        ll[u]       = (int8_t)l;
+        hh[u]       = (int8_t)((x[u] - l) >> 8);
+    }
+    *(int2*)(hi + i) = *(const int2*)hh;
+    *(int2*)(lo + i) = *(const int2*)ll;
+}
+
+// FP16 copy of a [rows x Kp] int16 matrix (exact for |X| <= 2048).
+__global__ void k_to_half(const int16_t* __restrict__ X, size_t n, _Float16* __restrict__ out)
+{
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    if (i >= n)
+        return;
+    const int4 v = *(const int4*)(X + i);
+    const int16_t* x = (const int16_t*)&v;
+    v8h h;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        h[u] = (_Float16)(float)x[u];
+    *(v8h*)(out + i) = h;
+}
+
+// ------------------------------------------------- exact INT8-limb GEMM ---
+
+// Block tile 64 query rows x 128 reference rows, 4 waves in a 2 x 2 grid of
+// 32 x 64 wave tiles, K staged 64 at a time through double-buffered LDS
+// (rows padded to 80 bytes).  Every output keeps three int32 partials
+// (hi*hi, hi*lo + lo*hi, lo*lo) recombined in int64, so
+//     S4 = sum_k Xq[m,k] * Xr[n,k]
+// is exact.  The epilogue forms the score with the reference's own FP32
+// divisor  D = sqrtf(sxx_f * syy_f)  (per-row FP32 sequential sums, computed
+// as the reference does) and one correctly rounded division:
+//     score = (float)((double)S4 / D),  NaN if D == 0.
+constexpr int I8_BM = 64, I8_BN = 128, I8_BK = 64, I8_LD = 80;
+
+__global__ __launch_bounds__(256) void k_gemm_int8(
+    const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
+    const int8_t* __restrict__ rl, int Kp, const float* __restrict__ sq_q,
+    const float* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sm[2][(2 * I8_BM + 2 * I8_BN) * I8_LD];
+    const int tid  = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid  = tid >> 6;
+    const int wm   = wid >> 1;  // 0..1: 32-row half
+    const int wn   = wid & 1;   // 0..1: 64-col half
+    const int m0   = blockIdx.y * I8_BM;
+    const int n0   = blockIdx.x * I8_BN;
+
+    // global -> LDS staging: A planes 64 rows x 64 B (2 x 256 x 16 B), B planes
+    // 128 rows x 64 B (2 x 512 x 16 B); thread t moves 16-byte chunks
+    auto load_stage = [&](int buf, int k0, int4 (&ra)[2], int4 (&rb)[4]) {
+        (void)buf;
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            const int c   = tid + u * 256;   // 0..511 over (plane, row, chunk)
+            const int pl  = c >> 8;          // 0: hi, 1: lo
+            const int row = (c >> 2) & 63;
+            const int ch  = c & 3;
+            const int8_t* src = (pl ? ql : qh) + (size_t)(m0 + row) * Kp + k0 + ch * 16;
+            ra[u] = *(const int4*)src;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            const int c   = tid + u * 256;   // 0..1023
+            const int pl  = c >> 9;
+            const int row = (c >> 2) & 127;
+            const int ch  = c & 3;
+            const int8_t* src = (pl ? rl : rh) + (size_t)(n0 + row) * Kp + k0 + ch * 16;
+            rb[u] = *(const int4*)src;
+        }
+    };
+    auto store_stage = [&](int buf, const int4 (&ra)[2], const int4 (&rb)[4]) {
+        int8_t* base = sm[buf];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            const int c = tid + u * 256, pl = c >> 8, row = (c >> 2) & 63, ch = c & 3;
+            *(int4*)(base + (pl * I8_BM + row) * I8_LD + ch * 16) = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            const int c = tid + u * 1024 / 4, pl = c >> 9, row = (c >> 2) & 127, ch = c & 3;
+            *(int4*)(base + (2 * I8_BM + pl * I8_BN + row) * I8_LD + ch * 16) = rb[u];
+        }
+    };
+
+    v4i acc_hh[2][4], acc_x[2][4], acc_ll[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            acc_hh[i][j] = v4i{ 0, 0, 0, 0 };
+            acc_x[i][j]  = v4i{ 0, 0, 0, 0 };
+            acc_ll[i][j] = v4i{ 0, 0, 0, 0 };
+        }
+
+    int4 ra[2], rb[4];
+    load_stage(0, 0, ra, rb);
+    store_stage(0, ra, rb);
+    __syncthreads();
+    const int r16  = lane & 15;
+    const int kofs = (lane >> 4) * 16;
+    int       buf  = 0;
+    for (int k0 = 0; k0 < Kp; k0 += I8_BK)
+    {
+        const bool more = k0 + I8_BK < Kp;
+        if (more)
+            load_stage(buf ^ 1, k0 + I8_BK, ra, rb);
+        const int8_t* base = sm[buf];
+        v4i ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+        {
+            const int row = wm * 32 + i * 16 + r16;
+            ah[i] = *(const v4i*)(base + (row)*I8_LD + kofs);
+            al[i] = *(const v4i*)(base + (I8_BM + row) * I8_LD + kofs);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int row = wn * 64 + j * 16 + r16;
+            bh[j] = *(const v4i*)(base + (2 * I8_BM + row) * I8_LD + kofs);
+            bl[j] = *(const v4i*)(base + (2 * I8_BM + I8_BN + row) * I8_LD + kofs);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                acc_hh[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bh[j], acc_hh[i][j], 0, 0, 0);
+                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bl[j], acc_x[i][j], 0, 0, 0);
+                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bh[j], acc_x[i][j], 0, 0, 0);
+                acc_ll[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bl[j], acc_ll[i][j], 0, 0, 0);
+            }
+        if (more)
+        {
+            __syncthreads();  // every wave is done reading `buf`
+            store_stage(buf ^ 1, ra, rb);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+
+    // epilogue
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + e;
+            if (m >= nq)
+                continue;
+            const float sq  = sq_q[m];
+            float*      out = S + (size_t)m * ldS;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int n = n0 + wn * 64 + j * 16 + r16;
+                if (n >= nr)
+                    continue;
+                const long long s4 = ((long long)acc_hh[i][j][e] << 16) +
+                                     ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
+                const float prod = sq * sq_r[n];
+                const float d    = sqrtf(prod);
+                out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
+            }
+        }
+}
+
+// ------------------------------------- approximate MFMA contractions ---
+
+// FP16 operands, FP32 accumulation (v_mfma_f32_16x16x32_f16).  Same tiling as
+// the INT8 kernel with one plane of 2-byte values: LDS rows of 64 halves.
+constexpr int H_BM = 64, H_BN = 128, H_BK = 64, H_LD = 72;  // halves per LDS row
+
+__global__ __launch_bounds__(256) void k_gemm_f16(const _Float16* __restrict__ qa,
+                                                  const _Float16* __restrict__ rb_, int Kp,
+                                                  const float* __restrict__ sq_q,
+                                                  const float* __restrict__ sq_r, int nq, int nr,
+                                                  float* __restrict__ S, size_t ldS)
+{
+    __shared__ __attribute__((aligned(16))) _Float16 sm[2][(H_BM + H_BN) * H_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int m0 = blockIdx.y * H_BM, n0 = blockIdx.x * H_BN;
+    auto load = [&](int k0, int4 (&ra)[2], int4 (&rb)[4]) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            const int c = tid + u * 256, row = c >> 3, ch = c & 7;  // 64 rows x 8 chunks
+            ra[u] = *(const int4*)(qa + (size_t)(m0 + row) * Kp + k0 + ch * 8);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            const int c = tid + u * 256, row = c >> 3, ch = c & 7;  // 128 rows
+            rb[u] = *(const int4*)(rb_ + (size_t)(n0 + row) * Kp + k0 + ch * 8);
+        }
+    };
+    auto store = [&](int buf, const int4 (&ra)[2], const int4 (&rb)[4]) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            const int c = tid + u * 256, row = c >> 3, ch = c & 7;
+            *(int4*)(&sm[buf][row * H_LD + ch * 8]) = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            const int c = tid + u * 256, row = c >> 3, ch = c & 7;
+            *(int4*)(&sm[buf][(H_BM + row) * H_LD + ch * 8]) = rb[u];
+        }
+    };
+    v4f acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            acc[i][j] = v4f{ 0.f, 0.f, 0.f, 0.f };
+    int4 ra[2], rb[4];
+    load(0, ra, rb);
+    store(0, ra, rb);
+    __syncthreads();
+    const int r16 = lane & 15, kofs = (lane >> 4) * 8;
+    int       buf = 0;
+    for (int k0 = 0; k0 < Kp; k0 += H_BK)
+    {
+        const bool more = k0 + H_BK < Kp;
+        if (more)
+            load(k0 + H_BK, ra, rb);
+#pragma unroll
+        for (int kk = 0; kk < H_BK; kk += 32)
+        {
+            v8h a[2], b[4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *(const v8h*)(&sm[buf][(wm * 32 + i * 16 + r16) * H_LD + kk + kofs]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                b[j] = *(const v8h*)(&sm[buf][(H_BM + wn * 64 + j * 16 + r16) * H_LD + kk + kofs]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more)
+        {
+            __syncthreads();
+            store(buf ^ 1, ra, rb);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + e;
+            if (m >= nq)
+                continue;
+            const float sq = sq_q[m];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int n = n0 + wn * 64 + j * 16 + r16;
+                if (n < nr)
+                    S[(size_t)m * ldS + n] = corr_from_sums(acc[i][j][e], sq, sq_r[n]);
+            }
+        }
+}
+
+// FP32 operands, FP32 accumulation (v_mfma_f32_16x16x4_f32): lane l holds
+// one float of row (l % 16) at k = k0 + l / 16.  Operands are converted from
+// the int16 ranks while staging.
+constexpr int F_BM = 64, F_BN = 128, F_BK = 32, F_LD = 36;  // floats per LDS row
+
+__global__ __launch_bounds__(256) void k_gemm_f32mfma(const int16_t* __restrict__ Xq,
+                                                      const int16_t* __restrict__ Xr, int Kp,
+                                                      const float* __restrict__ sq_q,
+                                                      const float* __restrict__ sq_r, int nq,
+                                                      int nr, float* __restrict__ S, size_t ldS)
+{
+    __shared__ __attribute__((aligned(16))) float sm[(F_BM + F_BN) * F_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int m0 = blockIdx.y * F_BM, n0 = blockIdx.x * F_BN;
+    v4f acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            acc[i][j] = v4f{ 0.f, 0.f, 0.f, 0.f };
+    const int r16 = lane & 15, kq = lane >> 4;
+    for (int k0 = 0; k0 < Kp; k0 += F_BK)
+    {
+        // 192 rows x 32 k int16 = 6144 values: 256 threads x 24 (3 x 8)
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+        {
+            const int c = tid + u * 256, row = c >> 2, ch = c & 3;  // 192 rows x 4 chunks of 8
+            const int16_t* src = row < F_BM ? Xq + (size_t)(m0 + row) * Kp
+                                            : Xr + (size_t)(n0 + row - F_BM) * Kp;
+            const int4 v = *(const int4*)(src + k0 + ch * 8);
+            const int16_t* x = (const int16_t*)&v;
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                sm[row * F_LD + ch * 8 + t] = (float)x[t];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < F_BK; kk += 4)
+        {
+            float a[2], b[4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = sm[(wm * 32 + i * 16 + r16) * F_LD + kk + kq];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                b[j] = sm[(F_BM + wn * 64 + j * 16 + r16) * F_LD + kk + kq];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + e;
+            if (m >= nq)
+                continue;
+            const float sq = sq_q[m];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int n = n0 + wn * 64 + j * 16 + r16;
+                if (n < nr)
+                    S[(size_t)m * ldS + n] = corr_from_sums(acc[i][j][e], sq, sq_r[n]);
+            }
+        }
+}
+
+}  // namespace csg

@@ -71,6 +71,7 @@ def lib():
     L.csg_destroy.argtypes = [_V]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
+    L.csg_selftest.argtypes = [_V, C.c_int, P(C.c_double)]
     L.csg_host_percentile.argtypes = [_V, _I64, _F, P(_F)]
     L.csg_host_minmax.argtypes = [_V, C.c_int, P(C.c_int), P(C.c_int)]
     L.csg_prep_train.argtypes = [C.c_int, _U32, _U32, _V, _V, _V, _U32, _U32, _V, _V, _V, _U32, _V,
@@ -246,10 +247,25 @@ class Engine:
         self._check(lib().csg_phase_times(self._h, _p(ms), _p(cnt), int(reset)), "phase_times")
         return {PHASES[i]: (float(ms[i]), int(cnt[i])) for i in range(N_PHASES)}
 
+    def selftest(self, which: int) -> float:
+        out = C.c_double()
+        self._check(lib().csg_selftest(self._h, which, C.byref(out)), "selftest")
+        return out.value
+
     def counters(self) -> Dict[str, int]:
         out = np.zeros(N_COUNTERS, np.int64)
         self._check(lib().csg_counters(self._h, _p(out)), "counters")
         return {COUNTERS[i]: int(out[i]) for i in range(N_COUNTERS)}
+
+
+def certify_width(G: int) -> float:
+    """Half-width within which the exact path's per-label quantiles bound the
+    reference's FP32 ones (the engine's certification width)."""
+    u = 2.0 ** -24
+    n = max(G - 1, 0)
+    gamma = n * u / (1 - n * u)
+    wpair = gamma / (1 - gamma - 2 * u) + 3 * u
+    return wpair * (1 + 1e-6) + 16 * u
 
 
 def host_percentile(arr: np.ndarray, quantile: float) -> float:

@@ -124,6 +124,15 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
 #define CSG_N_COUNTERS 11
 int csg_counters(csg_engine* eng, int64_t* out);
 
+/* Diagnostic of the contraction kernels on seeded random ranks (|X| <= 1500,
+ * 256 x 256 rows, K = 192): which = 0 INT8 limbs, 1 FP16, 2 FP32 MFMA,
+ * 3 plain FP32.  *max_err receives 0 when the device results equal the host's
+ * exact (0) / sequential-fma (3) values, else the largest difference (+1); for
+ * 1 and 2 the largest error of the dot product relative to sqrt(sxx * syy),
+ * i.e. in correlation units.  No
+ * counterpart in the reference: it checks the MFMA operand layouts. */
+int csg_selftest(csg_engine* eng, int which, double* max_err);
+
 /* Host-side evaluation of the reference's `percentile` (src/cusingler.cu:
  * 947-978) with the same libstdc++ sort emulation the device uses; returns
  * CSG_ERR_UNDEFINED if the library's unguarded scans would leave the array. */

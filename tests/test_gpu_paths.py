@@ -1,0 +1,92 @@
+"""The MFMA score paths on the GPU: operand layouts, the exact INT8-limb path
+(labels bit-exact to the reference order, scores within the certified width)
+and the approximate FP16 / FP32-MFMA paths (error against the FP64 oracle)."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from cusingler_amd import engine
+    e = engine.Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("which,name", [(0, "int8"), (3, "plain"), (1, "fp16"), (2, "fp32_mfma")])
+def test_contraction_selftest(eng, which, name):
+    _say("selftest", name)
+    err = eng.selftest(which)
+    if which in (0, 3):
+        assert err == 0.0, f"{name} contraction differs from the host ({err})"
+    else:
+        assert err < 1e-5, f"{name} relative error {err}"
+
+
+def _data(name):
+    from cusingler_amd import synth
+    return {
+        "tiny": lambda: synth.tiny(),
+        "tiny_dirichlet": lambda: synth.tiny(n_ref_per_label=9, n_labels=4, n_genes=400, n_qry=40,
+                                             props="dirichlet", seed=5),
+        "small": lambda: synth.generate("small"),
+        "pancreas_2k": lambda: synth.generate("pancreas", n_qry=2000),
+        "atlas_1k": lambda: synth.generate("atlas", n_qry=1000),
+    }[name]()
+
+
+@pytest.mark.parametrize("name", ["tiny", "tiny_dirichlet", "small", "pancreas_2k", "atlas_1k"])
+def test_exact_path_labels_match_reference_order(eng, name):
+    from cusingler_amd import engine
+    from oracle import oracle as O
+    ds = _data(name)
+    prep = engine.Prep(ds, 8)
+    c0 = prep.generate(0)
+    _say("exact_int8", name, ds.qry_height, ds.ref_height, c0.width)
+    eng.copyin_score(c0)
+    first, mask, scores = eng.get_label(0.8, 0.05, "exact_int8")
+    cnt = eng.counters()
+    eng.destroy_score()
+    orc = O.Oracle(ds, 8)
+    of, om, osc = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
+    assert np.array_equal(first, of), f"{(first != of).sum()} first labels differ"
+    assert np.array_equal(mask, om), f"{(mask != om).sum()} mask entries differ"
+    w = engine.certify_width(c0.width)
+    err = np.abs(scores.astype(np.float64) - osc.astype(np.float64))
+    assert np.nanmax(err) <= w, f"score error {np.nanmax(err)} above the certified width {w}"
+    print(f"[exact] {name}: rechecked pairs {cnt['rechecked_pairs']}, uncertified cells "
+          f"{cnt['uncertified_cells']} of {ds.qry_height}, max |dP| {np.nanmax(err):.3g} "
+          f"(width {w:.3g})", flush=True)
+
+
+@pytest.mark.parametrize("path", ["fp16", "fp32_mfma"])
+def test_approximate_paths_error(eng, path):
+    from cusingler_amd import engine
+    from oracle import oracle as O
+    ds = _data("pancreas_2k")
+    prep = engine.Prep(ds, 8)
+    c0 = prep.generate(0)
+    _say(path, "pancreas_2k")
+    eng.copyin_score(c0)
+    first, mask, scores = eng.get_label(0.8, 0.05, path)
+    eng.destroy_score()
+    orc = O.Oracle(ds, 8)
+    of, om, osc = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
+    err = np.nanmax(np.abs(scores.astype(np.float64) - osc.astype(np.float64)))
+    agree = (first == of).mean()
+    print(f"[{path}] max |dP| vs reference order {err:.3g}, first-label agreement {agree:.5f}",
+          flush=True)
+    assert err < 1e-4
+    assert agree > 0.99
