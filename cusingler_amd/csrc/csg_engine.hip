@@ -37,12 +37,19 @@ struct PhaseEvents
 
 size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
+// fine-tune correlation tile: 32 cells x 32 reference rows, 2 x 2 per thread
+constexpr int FT_TC = 32, FT_TR = 32, FT_RC = 2, FT_RR = 2;
+constexpr int FT_NT = (FT_TC / FT_RC) * (FT_TR / FT_RR);
+// groups of at most FT_THIN cells use the thin tile (FT_THIN cells x 256 rows)
+constexpr int FT_THIN = 8;
+
 }  // namespace
 
 struct csg_engine
 {
     int             device = 0;
     hipStream_t     stream = nullptr;
+    size_t          lds_limit = 65536;  // dynamic LDS a block may use
     std::string     err;
     hipDeviceProp_t prop{};
 
@@ -92,10 +99,24 @@ struct csg_engine
     std::vector<uint32_t> fctidx, ctdidx, ctdiff;
     std::vector<float>    host_mask;  // empty: use the device mask
     uint16_t* d_fref_codes = nullptr;
+    uint32_t *d_flstart = nullptr, *d_fllen = nullptr;
     uint16_t *d_fqdata = nullptr;
     int *d_fqind = nullptr, *d_fqptr = nullptr;
     PctParam* d_fpct = nullptr;
     float     fpct_quantile = -1.f;
+    // fine-tune work buffers (grow-only)
+    uint8_t*  d_ft_tables = nullptr;
+    size_t    d_ft_tables_cap = 0;
+    int16_t * d_ft_rr = nullptr, *d_ft_cr = nullptr;
+    size_t    d_ft_rr_cap = 0, d_ft_cr_cap = 0;
+    float *   d_ft_rsq = nullptr, *d_ft_csq = nullptr, *d_ft_scores = nullptr, *d_ft_P = nullptr;
+    size_t    d_ft_rsq_cap = 0, d_ft_csq_cap = 0, d_ft_scores_cap = 0, d_ft_P_cap = 0;
+    uint64_t* d_ft_nan = nullptr;
+    size_t    d_ft_nan_cap = 0;
+    uint8_t*  d_ft_out = nullptr;
+    size_t    d_ft_out_cap = 0;
+    uint8_t * h_ft_stage = nullptr, *h_ft_out = nullptr;
+    size_t    h_ft_stage_cap = 0, h_ft_out_cap = 0;
 
     // ---- timing / counters ----
     double  phase_ms[CSG_N_PHASES]  = {};
@@ -267,12 +288,21 @@ int set_pct(csg_engine* eng, PctParam** d, float* cached, const std::vector<uint
 
 size_t rank_smem(int nbins, uint32_t Kp) { return (size_t)((nbins + 3) & ~3) * 4 + (size_t)Kp * 2; }
 
-// Runs the libstdc++ sort emulation on the NaN-carrying segments found by a
-// k_select launch.
+// Lengths of listed segments (for sizing the NaN work buffer).
 template <class Src>
-int resolve_nan_segments(csg_engine* eng, Src src, const std::vector<uint32_t>& seg_len_of_param,
-                         const float* scores, const PctParam* params, float p, float* out,
-                         const std::vector<uint64_t>* host_seg_len)
+__global__ void k_seg_len(Src src, const uint64_t* __restrict__ items, unsigned int n,
+                          uint32_t* __restrict__ out)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = src.get(items[i]).n;
+}
+
+// Runs the libstdc++ sort emulation on the NaN-carrying segments found by a
+// k_select launch (listed in eng->d_nan_list, counted in eng->d_nan_count).
+template <class Src>
+int resolve_nan_segments(csg_engine* eng, Src src, const float* scores, const PctParam* params,
+                         float p, float* out)
 {
     unsigned int cnt = 0;
     HIPCHK(hipMemcpyAsync(&cnt, eng->d_nan_count, sizeof(cnt), hipMemcpyDeviceToHost, eng->stream));
@@ -280,23 +310,28 @@ int resolve_nan_segments(csg_engine* eng, Src src, const std::vector<uint32_t>& 
     if (cnt == 0)
         return CSG_OK;
     eng->counters[5] += cnt;
-    std::vector<uint64_t> items(cnt);
-    HIPCHK(hipMemcpy(items.data(), eng->d_nan_list, cnt * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    (void)seg_len_of_param;
+    uint32_t* d_len = nullptr;
+    int       rc    = dalloc(eng, &d_len, cnt);
+    if (rc)
+        return rc;
+    k_seg_len<Src><<<(cnt + 255) / 256, 256, 0, eng->stream>>>(src, eng->d_nan_list, cnt, d_len);
+    std::vector<uint32_t> len(cnt);
+    hipError_t e1 = hipGetLastError();
+    hipError_t e2 = hipMemcpyAsync(len.data(), d_len, cnt * 4, hipMemcpyDeviceToHost, eng->stream);
+    hipError_t e3 = hipStreamSynchronize(eng->stream);
+    dfree(d_len);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        return eng->fail(CSG_ERR_DEVICE, "NaN segment sizing failed");
     std::vector<uint64_t> off(cnt);
     uint64_t              total = 0;
     for (unsigned int i = 0; i < cnt; ++i)
     {
         off[i] = total;
-        total += (*host_seg_len)[items[i]];
+        total += len[i];
     }
     float*    d_work = nullptr;
     uint64_t* d_off  = nullptr;
-    int       rc     = dalloc(eng, &d_work, total);
-    if (rc)
-        return rc;
-    rc = upload(eng, &d_off, off.data(), off.size());
-    if (rc)
+    if ((rc = dalloc(eng, &d_work, total)) || (rc = upload(eng, &d_off, off.data(), off.size())))
     {
         dfree(d_work);
         return rc;
@@ -339,6 +374,22 @@ int csg_init_gpu(int gpuid, csg_engine** out)
     {
         delete eng;
         return CSG_ERR_DEVICE;
+    }
+    // larger dynamic LDS for the fine-tune kernels where the device allows it
+    // (static LDS of the kernels stays below 2 KiB)
+    eng->lds_limit = 65536;
+    if ((size_t)eng->prop.sharedMemPerBlock > 65536 + 2048)
+    {
+        const int  want = (int)eng->prop.sharedMemPerBlock - 2048;
+        const bool ok1  = hipFuncSetAttribute((const void*)k_ft_corr<FT_TC, FT_TR, FT_RC, FT_RR>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              want) == hipSuccess;
+        const bool ok2  = hipFuncSetAttribute((const void*)k_ft_rank_qry,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              want) == hipSuccess;
+        if (ok1 && ok2)
+            eng->lds_limit = (size_t)want;
+        (void)hipGetLastError();  // a refused attribute must not surface at the next launch
     }
     if (dalloc(eng, &eng->d_nan_count, 1) || dalloc(eng, &eng->d_status, 1))
     {
@@ -401,10 +452,30 @@ int csg_destroy(csg_engine* eng)
     (void)hipSetDevice(eng->device);
     (void)hipStreamSynchronize(eng->stream);
     dfree(eng->d_fref_codes);
+    dfree(eng->d_flstart);
+    dfree(eng->d_fllen);
     dfree(eng->d_fqdata);
     dfree(eng->d_fqind);
     dfree(eng->d_fqptr);
     dfree(eng->d_fpct);
+    dfree(eng->d_ft_tables);
+    dfree(eng->d_ft_rr);
+    dfree(eng->d_ft_cr);
+    dfree(eng->d_ft_rsq);
+    dfree(eng->d_ft_csq);
+    dfree(eng->d_ft_scores);
+    dfree(eng->d_ft_P);
+    dfree(eng->d_ft_nan);
+    dfree(eng->d_ft_out);
+    eng->d_ft_tables_cap = eng->d_ft_rr_cap = eng->d_ft_cr_cap = eng->d_ft_rsq_cap = 0;
+    eng->d_ft_csq_cap = eng->d_ft_scores_cap = eng->d_ft_P_cap = eng->d_ft_nan_cap = 0;
+    eng->d_ft_out_cap = 0;
+    if (eng->h_ft_stage)
+        (void)hipHostFree(eng->h_ft_stage);
+    if (eng->h_ft_out)
+        (void)hipHostFree(eng->h_ft_out);
+    eng->h_ft_stage = eng->h_ft_out = nullptr;
+    eng->h_ft_stage_cap = eng->h_ft_out_cap = 0;
     eng->fpct_quantile = -1.f;
     eng->ft_loaded     = false;
     eng->host_mask.clear();
@@ -481,7 +552,7 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
         (rc = upload(eng, &eng->d_llen, ll.data(), ll.size())))
         return rc;
     // reference ranks (rows padded to the 128-row tile)
-    const size_t nr_pad = round_up(ref_height, 128);
+    const size_t nr_pad = round_up(ref_height, 256);
     if ((rc = dalloc(eng, &eng->d_Xr, nr_pad * eng->Kp)) ||
         (rc = dalloc(eng, &eng->d_sqr_f, nr_pad)) || (rc = dalloc(eng, &eng->d_sqr_i, nr_pad)))
         return rc;
@@ -526,6 +597,14 @@ int score_chunk(csg_engine* eng, int path, uint32_t nq)
         k_gemm_plain<<<grid, 256, 0, eng->stream>>>(eng->d_Xq, eng->d_Xr, (int)eng->Kp,
                                                     eng->d_sqq_f, eng->d_sqr_f, (int)nq,
                                                     (int)eng->Nr, eng->d_S, (size_t)eng->Nr);
+    }
+    else if (path == CSG_PATH_EXACT_INT8 && eng->G <= 1920)
+    {
+        dim3 grid((eng->Nr + W8_BN - 1) / W8_BN, (nq + W8_BM - 1) / W8_BM);
+        k_gemm_int8w<<<grid, 512, 0, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                                                    (int)eng->Kp, eng->d_sqq_f, eng->d_sqr_f,
+                                                    (int)nq, (int)eng->Nr, eng->d_S,
+                                                    (size_t)eng->Nr);
     }
     else if (path == CSG_PATH_EXACT_INT8)
     {
@@ -603,7 +682,7 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
         return CSG_OK;
     std::vector<RecheckItem> items(n);
     HIPCHK(hipMemcpy(items.data(), eng->d_items, n * sizeof(RecheckItem), hipMemcpyDeviceToHost));
-    std::vector<uint64_t> off(n), seg_len(n);
+    std::vector<uint64_t> off(n);
     std::vector<Segment>  segs(n);
     std::vector<uint8_t>  cell_seen(nq, 0);
     uint64_t              tot = 0;
@@ -612,7 +691,6 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
         const uint32_t len = eng->ctidx[2 * items[i].l + 1];
         off[i]             = tot;
         segs[i]            = Segment{ tot, len, items[i].l };
-        seg_len[i]         = len;
         tot += len;
         if (!cell_seen[items[i].q])
         {
@@ -640,8 +718,8 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
             eng->d_nan_count);
         LAUNCHCHK();
     }
-    if ((rc = resolve_nan_segments(eng, TableSegments{ eng->d_item_seg }, {}, eng->d_S, eng->d_pct,
-                                   quantile, eng->d_Pref, &seg_len)))
+    if ((rc = resolve_nan_segments(eng, TableSegments{ eng->d_item_seg }, eng->d_S, eng->d_pct,
+                                   quantile, eng->d_Pref)))
         return rc;
     {
         PhaseScope ps(eng, 4, 2);
@@ -674,7 +752,7 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     eng->counters[0] = eng->Nq;
     eng->counters[1] = eng->G;
     eng->counters[2] = eng->Nr;
-    const size_t nr_pad = round_up(eng->Nr, 128);
+    const size_t nr_pad = round_up(eng->Nr, 256);
     const size_t qc     = eng->Qc;
     if (path == CSG_PATH_EXACT_INT8 && !eng->d_rh)
     {
@@ -710,7 +788,6 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
             return rc;
         eng->ref_prepared_path = path;
     }
-    std::vector<uint64_t> seg_len;
     for (uint32_t q0 = 0; q0 < eng->Nq; q0 += eng->Qc)
     {
         const uint32_t nq = std::min(eng->Qc, eng->Nq - q0);
@@ -745,14 +822,8 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
                 return rc;
             continue;
         }
-        if (seg_len.size() != n_items)
-        {
-            seg_len.resize(n_items);
-            for (uint64_t i = 0; i < n_items; ++i)
-                seg_len[i] = eng->ctidx[2 * (i % eng->L) + 1];
-        }
-        if ((rc = resolve_nan_segments(eng, src, {}, eng->d_S, eng->d_pct, quantile,
-                                       eng->d_P + (size_t)q0 * eng->L, &seg_len)))
+        if ((rc = resolve_nan_segments(eng, src, eng->d_S, eng->d_pct, quantile,
+                                       eng->d_P + (size_t)q0 * eng->L)))
             return rc;
     }
     if (path != CSG_PATH_EXACT_INT8)
@@ -842,6 +913,17 @@ int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint1
     eng->ctdiff.assign(ctdiff, ctdiff + n_ctdiff);
     if (labels_mask)
         eng->host_mask.assign(labels_mask, labels_mask + (size_t)qry_height * ct_num);
+    {
+        std::vector<uint32_t> ls(ct_num), ll(ct_num);
+        for (uint32_t l = 0; l < ct_num; ++l)
+        {
+            ls[l] = ctidx[2 * l];
+            ll[l] = ctidx[2 * l + 1];
+        }
+        if ((rc = upload(eng, &eng->d_flstart, ls.data(), ls.size())) ||
+            (rc = upload(eng, &eng->d_fllen, ll.data(), ll.size())))
+            return rc;
+    }
     const size_t rnnz = (size_t)ref_indptr[ref_height], qnnz = (size_t)qry_indptr[qry_height];
     uint16_t* d_rd = nullptr;
     int *     d_ri = nullptr, *d_rp = nullptr;
@@ -878,11 +960,12 @@ struct FtBatch
     std::vector<FtGroup>    groups;
     std::vector<uint32_t>   genes;
     std::vector<uint32_t>   labels;
-    std::vector<FtRankItem> ref_items, cell_items;
-    std::vector<FtCorrTile> tiles;
+    std::vector<FtRankItem> cell_items;
+    std::vector<uint32_t>   label_pos;  // per group label: offset within the group's rows
+    uint64_t                n_ref_items = 0;
+    std::vector<FtCorrTile> tiles, thin_tiles;
     std::vector<FtCell>     cells;
-    std::vector<Segment>    segs;
-    std::vector<uint64_t>   seg_len;
+    uint64_t                n_segs = 0;
     std::vector<uint8_t>    in_lists;
     std::vector<uint32_t>   in_counts;
     std::vector<uint32_t>   cell_ids;  // query row per cell item
@@ -892,12 +975,13 @@ struct FtBatch
         groups.clear();
         genes.clear();
         labels.clear();
-        ref_items.clear();
         cell_items.clear();
+        label_pos.clear();
+        n_ref_items = 0;
         tiles.clear();
+        thin_tiles.clear();
         cells.clear();
-        segs.clear();
-        seg_len.clear();
+        n_segs = 0;
         in_lists.clear();
         in_counts.clear();
         cell_ids.clear();
@@ -905,162 +989,178 @@ struct FtBatch
     }
 };
 
-constexpr int FT_TC = 16, FT_TR = 16;
 
-// Uploads a batch, runs rank / correlate / select / filter, and writes the
-// new candidate lists of the batch's cells into `lists` / `counts`.
+// Grow-only device allocation.
+extern "C++" {
+template <class T> int ensure(csg_engine* eng, T** p, size_t* cap, size_t n)
+{
+    if (n <= *cap && *p)
+        return CSG_OK;
+    const size_t want = std::max(n, *cap + *cap / 2);
+    int          rc   = dalloc(eng, p, want);
+    *cap              = rc ? 0 : want;
+    return rc;
+}
+}
+
+// Grow-only pinned host allocation.
+int ensure_host(csg_engine* eng, uint8_t** p, size_t* cap, size_t n)
+{
+    if (n <= *cap && *p)
+        return CSG_OK;
+    if (*p)
+        (void)hipHostFree(*p);
+    *p              = nullptr;
+    const size_t want = std::max(n, *cap + *cap / 2);
+    HIPCHK(hipHostMalloc((void**)p, want, hipHostMallocDefault));
+    *cap = want;
+    return CSG_OK;
+}
+
+size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+// Uploads a batch (one packed copy), runs rank / correlate / select / filter,
+// and writes the new candidate lists of the batch's cells into `lists` /
+// `counts`.
 int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
                  std::vector<uint8_t>& lists, std::vector<uint32_t>& counts)
 {
-    const uint32_t L = eng->fL;
-    FtGroup*    d_groups = nullptr;
-    uint32_t*   d_genes = nullptr, *d_labels = nullptr, *d_in_counts = nullptr, *d_out_counts = nullptr;
-    FtRankItem *d_ritems = nullptr, *d_citems = nullptr;
-    FtCorrTile* d_tiles = nullptr;
-    FtCell*     d_cells = nullptr;
-    Segment*    d_segs = nullptr;
-    uint8_t *   d_in_lists = nullptr, *d_out_lists = nullptr;
-    int16_t *   d_rr = nullptr, *d_cr = nullptr;
-    float *     d_rsq = nullptr, *d_csq = nullptr, *d_scores = nullptr, *d_P = nullptr;
-    uint64_t*   d_nan_list = nullptr;
-    int         rc = CSG_OK;
-    auto cleanup = [&]() {
-        dfree(d_groups);
-        dfree(d_genes);
-        dfree(d_labels);
-        dfree(d_in_counts);
-        dfree(d_out_counts);
-        dfree(d_ritems);
-        dfree(d_citems);
-        dfree(d_tiles);
-        dfree(d_cells);
-        dfree(d_segs);
-        dfree(d_in_lists);
-        dfree(d_out_lists);
-        dfree(d_rr);
-        dfree(d_cr);
-        dfree(d_rsq);
-        dfree(d_csq);
-        dfree(d_scores);
-        dfree(d_P);
-        dfree(d_nan_list);
-    };
+    const uint32_t L     = eng->fL;
     const uint32_t ncell = (uint32_t)b.cells.size();
-    if ((rc = upload(eng, &d_groups, b.groups.data(), b.groups.size())) ||
-        (rc = upload(eng, &d_genes, b.genes.data(), b.genes.size())) ||
-        (rc = upload(eng, &d_labels, b.labels.data(), b.labels.size())) ||
-        (rc = upload(eng, &d_ritems, b.ref_items.data(), b.ref_items.size())) ||
-        (rc = upload(eng, &d_citems, b.cell_items.data(), b.cell_items.size())) ||
-        (rc = upload(eng, &d_tiles, b.tiles.data(), b.tiles.size())) ||
-        (rc = upload(eng, &d_cells, b.cells.data(), b.cells.size())) ||
-        (rc = upload(eng, &d_segs, b.segs.data(), b.segs.size())) ||
-        (rc = upload(eng, &d_in_lists, b.in_lists.data(), b.in_lists.size())) ||
-        (rc = upload(eng, &d_in_counts, b.in_counts.data(), b.in_counts.size())) ||
-        (rc = dalloc(eng, &d_out_lists, (size_t)ncell * L)) ||
-        (rc = dalloc(eng, &d_out_counts, ncell)) || (rc = dalloc(eng, &d_rr, b.ref_elems)) ||
-        (rc = dalloc(eng, &d_cr, b.cell_elems)) || (rc = dalloc(eng, &d_rsq, b.ref_items.size())) ||
-        (rc = dalloc(eng, &d_csq, b.cell_items.size())) ||
-        (rc = dalloc(eng, &d_scores, b.n_scores)) || (rc = dalloc(eng, &d_P, b.segs.size())) ||
-        (rc = dalloc(eng, &d_nan_list, b.segs.size())))
-    {
-        cleanup();
+    // table layout in the packed upload
+    size_t off = 0;
+    auto   place = [&](size_t bytes) {
+        const size_t o = off;
+        off            = align16(off + bytes);
+        return o;
+    };
+    const size_t o_groups = place(b.groups.size() * sizeof(FtGroup));
+    const size_t o_genes  = place(b.genes.size() * 4);
+    const size_t o_labels = place(b.labels.size() * 4);
+    const size_t o_lpos   = place(b.label_pos.size() * 4);
+    const size_t o_citems = place(b.cell_items.size() * sizeof(FtRankItem));
+    const size_t o_tiles  = place(b.tiles.size() * sizeof(FtCorrTile));
+    const size_t o_thin   = place(b.thin_tiles.size() * sizeof(FtCorrTile));
+    const size_t o_cells  = place(b.cells.size() * sizeof(FtCell));
+    const size_t o_inl    = place(b.in_lists.size());
+    const size_t o_inc    = place(b.in_counts.size() * 4);
+    const size_t total    = off;
+    int          rc;
+    if ((rc = ensure_host(eng, &eng->h_ft_stage, &eng->h_ft_stage_cap, total)) ||
+        (rc = ensure(eng, &eng->d_ft_tables, &eng->d_ft_tables_cap, total)))
         return rc;
-    }
+    uint8_t* h = eng->h_ft_stage;
+    auto     put = [&](size_t o, const void* src, size_t bytes) {
+        if (bytes)
+            std::memcpy(h + o, src, bytes);
+    };
+    put(o_groups, b.groups.data(), b.groups.size() * sizeof(FtGroup));
+    put(o_genes, b.genes.data(), b.genes.size() * 4);
+    put(o_labels, b.labels.data(), b.labels.size() * 4);
+    put(o_lpos, b.label_pos.data(), b.label_pos.size() * 4);
+    put(o_citems, b.cell_items.data(), b.cell_items.size() * sizeof(FtRankItem));
+    put(o_tiles, b.tiles.data(), b.tiles.size() * sizeof(FtCorrTile));
+    put(o_thin, b.thin_tiles.data(), b.thin_tiles.size() * sizeof(FtCorrTile));
+    put(o_cells, b.cells.data(), b.cells.size() * sizeof(FtCell));
+    put(o_inl, b.in_lists.data(), b.in_lists.size());
+    put(o_inc, b.in_counts.data(), b.in_counts.size() * 4);
+    HIPCHK(hipMemcpyAsync(eng->d_ft_tables, h, total, hipMemcpyHostToDevice, eng->stream));
+    uint8_t*          d          = eng->d_ft_tables;
+    const FtGroup*    d_groups   = (const FtGroup*)(d + o_groups);
+    const uint32_t*   d_genes    = (const uint32_t*)(d + o_genes);
+    const uint32_t*   d_labels   = (const uint32_t*)(d + o_labels);
+    const uint32_t*   d_lpos     = (const uint32_t*)(d + o_lpos);
+    const FtRankItem* d_citems   = (const FtRankItem*)(d + o_citems);
+    const FtCorrTile* d_tiles    = (const FtCorrTile*)(d + o_tiles);
+    const FtCorrTile* d_thin     = (const FtCorrTile*)(d + o_thin);
+    const FtCell*     d_cells    = (const FtCell*)(d + o_cells);
+    const uint8_t*    d_in_lists = (const uint8_t*)(d + o_inl);
+    const uint32_t*   d_in_cnt   = (const uint32_t*)(d + o_inc);
+    // work buffers
+    const size_t nseg = b.n_segs;
+    if ((rc = ensure(eng, &eng->d_ft_rr, &eng->d_ft_rr_cap, b.ref_elems)) ||
+        (rc = ensure(eng, &eng->d_ft_cr, &eng->d_ft_cr_cap, b.cell_elems)) ||
+        (rc = ensure(eng, &eng->d_ft_rsq, &eng->d_ft_rsq_cap, b.n_ref_items)) ||
+        (rc = ensure(eng, &eng->d_ft_csq, &eng->d_ft_csq_cap, b.cell_items.size())) ||
+        (rc = ensure(eng, &eng->d_ft_scores, &eng->d_ft_scores_cap, b.n_scores)) ||
+        (rc = ensure(eng, &eng->d_ft_P, &eng->d_ft_P_cap, nseg)) ||
+        (rc = ensure(eng, &eng->d_ft_nan, &eng->d_ft_nan_cap, nseg)) ||
+        (rc = ensure(eng, &eng->d_ft_out, &eng->d_ft_out_cap, (size_t)ncell * (L + 4))) ||
+        (rc = ensure_host(eng, &eng->h_ft_out, &eng->h_ft_out_cap, (size_t)ncell * (L + 4))))
+        return rc;
+    uint8_t*  d_out_lists  = eng->d_ft_out;
+    uint32_t* d_out_counts = (uint32_t*)(eng->d_ft_out + (size_t)ncell * L);
+
     uint32_t max_gs = 0;
     for (auto& g : b.groups)
         max_gs = std::max(max_gs, g.gs);
     const size_t stride_max = round_up(max_gs, 8);
     const size_t lds_rank_q = (size_t)((eng->fnbins + 3) & ~3) * 4 +
                               round_up(std::max<uint32_t>(eng->Gf, 1), 8) * 2 + stride_max * 4;
-    const size_t lds_corr   = (size_t)(FT_TC + FT_TR) * stride_max * 2;
-    if (lds_rank_q > 65536 || lds_corr > 65536 || b.tiles.empty() || b.ref_items.empty() ||
-        b.cell_items.empty())
-    {
-        cleanup();
+    const size_t lds_corr   = (size_t)(FT_TC + FT_TR) * (stride_max + 2) * 2;
+    const size_t lds_thin   = (size_t)FT_THIN * stride_max * 2;
+    if (lds_rank_q > eng->lds_limit || lds_corr > eng->lds_limit || lds_thin > 65536 ||
+        (b.tiles.empty() && b.thin_tiles.empty()) || b.n_ref_items == 0 || b.cell_items.empty())
         return eng->fail(CSG_ERR_ARGUMENT, "fine-tune gene subset of " + std::to_string(max_gs) +
                                                " genes exceeds the kernels' LDS plan");
-    }
     {
-        PhaseScope ps(eng, 7, 2);
+        PhaseScope   ps(eng, 7, 2);
         const size_t smem_r = (size_t)((eng->fnbins + 3) & ~3) * 4 + stride_max * 4;
-        k_ft_rank_ref<<<(unsigned)b.ref_items.size(), 128, smem_r, eng->stream>>>(
-            d_ritems, d_groups, d_genes, eng->d_fref_codes, std::max<uint32_t>(eng->Gf, 1),
-            eng->fnbins, d_rr, d_rsq);
-        hipError_t e1 = hipGetLastError();
-        const size_t smem_q = (size_t)((eng->fnbins + 3) & ~3) * 4 +
-                              round_up(std::max<uint32_t>(eng->Gf, 1), 8) * 2 + stride_max * 4;
-        k_ft_rank_qry<<<(unsigned)b.cell_items.size(), 128, smem_q, eng->stream>>>(
+        k_ft_rank_ref<<<(unsigned)b.n_ref_items, 128, smem_r, eng->stream>>>(
+            d_groups, (uint32_t)b.groups.size(), d_labels, eng->d_flstart, eng->d_fllen, d_genes,
+            eng->d_fref_codes, std::max<uint32_t>(eng->Gf, 1), eng->fnbins, eng->d_ft_rr,
+            eng->d_ft_rsq);
+        LAUNCHCHK();
+        k_ft_rank_qry<<<(unsigned)b.cell_items.size(), 128, lds_rank_q, eng->stream>>>(
             d_citems, d_groups, d_genes, eng->d_fqdata, eng->d_fqind, eng->d_fqptr,
-            std::max<uint32_t>(eng->Gf, 1), eng->fnbins, d_cr, d_csq);
-        hipError_t e2 = hipGetLastError();
-        if (e1 != hipSuccess || e2 != hipSuccess)
-        {
-            cleanup();
-            return eng->fail(CSG_ERR_DEVICE, std::string("fine-tune rank launch: ") +
-                                                 hipGetErrorString(e1 != hipSuccess ? e1 : e2));
-        }
+            std::max<uint32_t>(eng->Gf, 1), eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
+        LAUNCHCHK();
     }
     {
-        PhaseScope ps(eng, 6, 1);
-        const size_t smem = (size_t)(FT_TC + FT_TR) * stride_max * 2;
-        k_ft_corr<FT_TC, FT_TR><<<(unsigned)b.tiles.size(), FT_TC * FT_TR, smem, eng->stream>>>(
-            d_tiles, d_groups, d_cr, d_csq, d_rr, d_rsq, d_scores);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess)
+        PhaseScope ps(eng, 6, (int)!b.tiles.empty() + (int)!b.thin_tiles.empty());
+        if (!b.tiles.empty())
         {
-            cleanup();
-            return eng->fail(CSG_ERR_DEVICE, std::string("fine-tune corr launch: ") +
-                                                 hipGetErrorString(e));
+            k_ft_corr<FT_TC, FT_TR, FT_RC, FT_RR><<<(unsigned)b.tiles.size(), FT_NT, lds_corr,
+                                                    eng->stream>>>(
+                d_tiles, d_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr, eng->d_ft_rsq,
+                eng->d_ft_scores);
+            LAUNCHCHK();
+        }
+        if (!b.thin_tiles.empty())
+        {
+            k_ft_corr_thin<FT_THIN><<<(unsigned)b.thin_tiles.size(), 256, lds_thin, eng->stream>>>(
+                d_thin, d_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr, eng->d_ft_rsq,
+                eng->d_ft_scores);
+            LAUNCHCHK();
         }
     }
     {
         PhaseScope ps(eng, 8, 1);
-        (void)hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream);
-        TableSegments src{ d_segs };
-        const uint64_t n = b.segs.size();
-        k_select<TableSegments><<<(unsigned)((n + 3) / 4), 256, 0, eng->stream>>>(
-            src, n, d_scores, eng->d_fpct, quantile, d_P, d_nan_list, eng->d_nan_count);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess)
-        {
-            cleanup();
-            return eng->fail(CSG_ERR_DEVICE, std::string("fine-tune select launch: ") +
-                                                 hipGetErrorString(e));
-        }
+        HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
+        FtSegments src{ d_cells, ncell, d_groups, d_labels, d_lpos, eng->d_fllen };
+        k_select<FtSegments><<<(unsigned)((nseg + 3) / 4), 256, 0, eng->stream>>>(
+            src, nseg, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P, eng->d_ft_nan,
+            eng->d_nan_count);
+        LAUNCHCHK();
     }
-    {
-        std::swap(eng->d_nan_list, d_nan_list);
-        rc = resolve_nan_segments(eng, TableSegments{ d_segs }, {}, d_scores, eng->d_fpct, quantile,
-                                  d_P, &b.seg_len);
-        std::swap(eng->d_nan_list, d_nan_list);
-        if (rc)
-        {
-            cleanup();
-            return rc;
-        }
-    }
+    std::swap(eng->d_nan_list, eng->d_ft_nan);
+    rc = resolve_nan_segments(eng, FtSegments{ d_cells, ncell, d_groups, d_labels, d_lpos, eng->d_fllen },
+                              eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P);
+    std::swap(eng->d_nan_list, eng->d_ft_nan);
+    if (rc)
+        return rc;
     {
         PhaseScope ps(eng, 8, 1);
         k_ft_filter<<<(ncell + 127) / 128, 128, 0, eng->stream>>>(
-            d_cells, ncell, d_groups, d_labels, d_P, d_in_lists, d_in_counts, L, thre, d_out_lists,
-            d_out_counts);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess)
-        {
-            cleanup();
-            return eng->fail(CSG_ERR_DEVICE, std::string("fine-tune filter launch: ") +
-                                                 hipGetErrorString(e));
-        }
+            d_cells, ncell, d_groups, d_labels, eng->d_ft_P, d_in_lists, d_in_cnt, L, thre,
+            d_out_lists, d_out_counts);
+        LAUNCHCHK();
     }
-    std::vector<uint8_t>  ol((size_t)ncell * L);
-    std::vector<uint32_t> oc(ncell);
-    hipError_t e1 = hipMemcpyAsync(ol.data(), d_out_lists, ol.size(), hipMemcpyDeviceToHost, eng->stream);
-    hipError_t e2 = hipMemcpyAsync(oc.data(), d_out_counts, oc.size() * 4, hipMemcpyDeviceToHost,
-                                   eng->stream);
-    hipError_t e3 = hipStreamSynchronize(eng->stream);
-    cleanup();
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
-        return eng->fail(CSG_ERR_DEVICE, "fine-tune result copy failed");
+    HIPCHK(hipMemcpyAsync(eng->h_ft_out, eng->d_ft_out, (size_t)ncell * (L + 4),
+                          hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    const uint8_t*  ol = eng->h_ft_out;
+    const uint32_t* oc = (const uint32_t*)(eng->h_ft_out + (size_t)ncell * L);
     for (uint32_t i = 0; i < ncell; ++i)
     {
         const uint32_t c = b.cell_ids[i];
@@ -1069,6 +1169,47 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
     }
     return CSG_OK;
 }
+
+// Mask rows packed into 64-bit words (bit l of word l/64), so the host reads
+// Nq * ceil(L/64) words instead of Nq * L floats.
+__global__ void k_pack_mask(const float* __restrict__ mask, uint32_t nq, uint32_t L,
+                            uint64_t* __restrict__ out)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq)
+        return;
+    const uint32_t W = (L + 63) / 64;
+    for (uint32_t w = 0; w < W; ++w)
+    {
+        uint64_t bits = 0;
+        for (uint32_t j = w * 64; j < min(L, w * 64 + 64); ++j)
+            if (mask[(size_t)q * L + j] != 0.f)
+                bits |= 1ull << (j - w * 64);
+        out[(size_t)q * W + w] = bits;
+    }
+}
+
+struct SetKey
+{
+    uint64_t w[4];
+    bool     operator==(const SetKey& o) const
+    {
+        return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3];
+    }
+};
+struct SetKeyHash
+{
+    size_t operator()(const SetKey& k) const
+    {
+        uint64_t h = 0x9E3779B97F4A7C15ull;
+        for (int i = 0; i < 4; ++i)
+        {
+            h ^= k.w[i] + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+            h *= 0xBF58476D1CE4E5B9ull;
+        }
+        return (size_t)(h ^ (h >> 31));
+    }
+};
 
 }  // namespace
 
@@ -1088,54 +1229,76 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
     for (int i = 6; i < CSG_N_COUNTERS; ++i)
         eng->counters[i] = 0;
     PhaseScope total(eng, 5, 0);
-    // candidate lists from the mask (ascending label order, src/cusingler.cu:1307-1312)
-    std::vector<float> maskbuf;
-    const float*       mask = nullptr;
+    // candidate lists from the mask, ascending label order (src/cusingler.cu:1307-1312)
+    std::vector<uint8_t>  lists((size_t)Nq * L);
+    std::vector<uint32_t> counts(Nq, 0);
+    std::vector<int32_t>  nrounds(Nq, 0);
     if (!eng->host_mask.empty())
-        mask = eng->host_mask.data();
+    {
+        const float* mask = eng->host_mask.data();
+        for (uint32_t c = 0; c < Nq; ++c)
+            for (uint32_t l = 0; l < L; ++l)
+                if (mask[(size_t)c * L + l] != 0)
+                    lists[(size_t)c * L + counts[c]++] = (uint8_t)l;
+    }
     else
     {
         if (!eng->mask_valid || eng->Nq != Nq || eng->L != L)
             return eng->fail(CSG_ERR_STATE, "no candidate mask: pass one to csg_copyin or run "
                                             "csg_get_label on the same cells first");
-        maskbuf.resize((size_t)Nq * L);
-        HIPCHK(hipMemcpyAsync(maskbuf.data(), eng->d_mask, maskbuf.size() * 4,
-                              hipMemcpyDeviceToHost, eng->stream));
-        HIPCHK(hipStreamSynchronize(eng->stream));
-        mask = maskbuf.data();
+        const uint32_t W = (L + 63) / 64;
+        uint64_t*      d_bits = nullptr;
+        if ((rc = dalloc(eng, &d_bits, (size_t)Nq * W)))
+            return rc;
+        std::vector<uint64_t> bits((size_t)Nq * W);
+        k_pack_mask<<<(Nq + 255) / 256, 256, 0, eng->stream>>>(eng->d_mask, Nq, L, d_bits);
+        hipError_t e1 = hipGetLastError();
+        hipError_t e2 = hipMemcpyAsync(bits.data(), d_bits, bits.size() * 8, hipMemcpyDeviceToHost,
+                                       eng->stream);
+        hipError_t e3 = hipStreamSynchronize(eng->stream);
+        dfree(d_bits);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+            return eng->fail(CSG_ERR_DEVICE, "mask copy failed");
+        for (uint32_t c = 0; c < Nq; ++c)
+            for (uint32_t w = 0; w < W; ++w)
+            {
+                uint64_t v = bits[(size_t)c * W + w];
+                while (v)
+                {
+                    const int t = __builtin_ctzll(v);
+                    v &= v - 1;
+                    lists[(size_t)c * L + counts[c]++] = (uint8_t)(w * 64 + t);
+                }
+            }
     }
-    std::vector<uint8_t>  lists((size_t)Nq * L);
-    std::vector<uint32_t> counts(Nq, 0);
-    std::vector<int32_t>  nrounds(Nq, 0);
+
+    std::vector<uint32_t> mark(std::max<uint32_t>(eng->Gf, 1), 0);
+    uint32_t              stamp        = 0;
+    const uint64_t        budget_elems = (uint64_t)1 << 28;  // per batch (scores + ranks)
+    FtBatch               b;
+    std::vector<uint32_t> active;
+    active.reserve(Nq);
     for (uint32_t c = 0; c < Nq; ++c)
-        for (uint32_t l = 0; l < L; ++l)
-            if (mask[(size_t)c * L + l] != 0)
-                lists[(size_t)c * L + counts[c]++] = (uint8_t)l;
+        if (counts[c] > 1 && (finetune_times == 0 || nrounds[c] < finetune_times))
+            active.push_back(c);
 
-    const size_t           words = (L + 63) / 64;
-    std::vector<uint32_t>  mark(std::max<uint32_t>(eng->Gf, 1), 0);
-    uint32_t               stamp = 0;
-    const uint64_t         budget_elems = (uint64_t)1 << 28;  // per batch (scores + ranks)
-    FtBatch                b;
-
-    while (true)
+    std::unordered_map<SetKey, uint32_t, SetKeyHash> gid;
+    std::vector<std::vector<uint32_t>>              gcells;
+    std::vector<SetKey>                             gkey;
+    while (!active.empty())
     {
         // active cells grouped by candidate set
-        std::unordered_map<std::string, uint32_t> gid;
-        std::vector<std::vector<uint32_t>>        gcells;
-        std::vector<std::string>                  gkey;
-        for (uint32_t c = 0; c < Nq; ++c)
+        gid.clear();
+        gcells.clear();
+        gkey.clear();
+        gid.reserve(active.size() / 4 + 16);
+        for (uint32_t c : active)
         {
-            if (counts[c] <= 1)
-                continue;
-            if (finetune_times != 0 && nrounds[c] >= finetune_times)
-                continue;
-            std::string key(words * 8, '\0');
-            uint64_t*   kw = (uint64_t*)&key[0];
+            SetKey key{ { 0, 0, 0, 0 } };
             for (uint32_t t = 0; t < counts[c]; ++t)
             {
                 const uint32_t l = lists[(size_t)c * L + t];
-                kw[l / 64] |= 1ull << (l % 64);
+                key.w[l / 64] |= 1ull << (l % 64);
             }
             auto it = gid.find(key);
             if (it == gid.end())
@@ -1146,8 +1309,6 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
             }
             gcells[it->second].push_back(c);
         }
-        if (gcells.empty())
-            break;
         eng->counters[6]++;
         b.clear();
         auto flush = [&]() -> int {
@@ -1159,10 +1320,9 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         };
         for (size_t gi = 0; gi < gcells.size(); ++gi)
         {
-            const uint64_t* kw = (const uint64_t*)gkey[gi].data();
             std::vector<uint32_t> labs;
             for (uint32_t l = 0; l < L; ++l)
-                if (kw[l / 64] >> (l % 64) & 1)
+                if (gkey[gi].w[l / 64] >> (l % 64) & 1)
                     labs.push_back(l);
             const int k = (int)labs.size();
             // G_sub: union of each ordered pair's first min(len, gene_thre)
@@ -1211,10 +1371,9 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
             while (cpos < cells.size())
             {
                 const uint64_t fixed = (uint64_t)nref * stride;
-                uint64_t       per   = (uint64_t)stride + nref;
-                uint64_t       room  = budget_elems > b.ref_elems + b.cell_elems + b.n_scores + fixed
-                                           ? budget_elems - (b.ref_elems + b.cell_elems + b.n_scores + fixed)
-                                           : 0;
+                const uint64_t per   = (uint64_t)stride + nref;
+                const uint64_t used  = b.ref_elems + b.cell_elems + b.n_scores + fixed;
+                const uint64_t room  = budget_elems > used ? budget_elems - used : 0;
                 size_t take = (size_t)std::min<uint64_t>(cells.size() - cpos, room / per);
                 if (take == 0)
                 {
@@ -1232,7 +1391,7 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
                 g.stride     = stride;
                 g.nref       = nref;
                 g.ncell      = (uint32_t)take;
-                g.ref_item0  = (uint32_t)b.ref_items.size();
+                g.ref_item0  = (uint32_t)b.n_ref_items;
                 g.cell_item0 = (uint32_t)b.cell_items.size();
                 g.label_off  = (uint32_t)b.labels.size();
                 g.nlabel     = (uint32_t)labs.size();
@@ -1242,36 +1401,35 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
                 const uint32_t gidx = (uint32_t)b.groups.size();
                 b.genes.insert(b.genes.end(), gsub.begin(), gsub.end());
                 b.labels.insert(b.labels.end(), labs.begin(), labs.end());
-                for (uint32_t l : labs)
-                    for (uint32_t r = 0; r < eng->fctidx[2 * l + 1]; ++r)
+                {
+                    uint32_t o = 0;
+                    for (uint32_t l : labs)
                     {
-                        b.ref_items.push_back(
-                            FtRankItem{ eng->fctidx[2 * l] + r, gidx, b.ref_elems });
-                        b.ref_elems += stride;
+                        b.label_pos.push_back(o);
+                        o += eng->fctidx[2 * l + 1];
                     }
+                }
+                b.n_ref_items += nref;
+                b.ref_elems += (uint64_t)nref * stride;
                 for (size_t t = 0; t < take; ++t)
                 {
                     const uint32_t c = cells[cpos + t];
                     b.cell_items.push_back(FtRankItem{ c, gidx, b.cell_elems });
                     b.cell_elems += stride;
-                    FtCell fc{ c, gidx, (uint64_t)b.segs.size() };
-                    b.cells.push_back(fc);
+                    b.cells.push_back(FtCell{ c, gidx, b.n_segs });
                     b.cell_ids.push_back(c);
-                    uint32_t off = 0;
-                    for (uint32_t l : labs)
-                    {
-                        const uint32_t n = eng->fctidx[2 * l + 1];
-                        b.segs.push_back(Segment{ b.n_scores + (uint64_t)t * nref + off, n, l });
-                        b.seg_len.push_back(n);
-                        off += n;
-                    }
+                    b.n_segs += labs.size();
                     b.in_lists.insert(b.in_lists.end(), &lists[(size_t)c * L],
                                       &lists[(size_t)c * L] + L);
                     b.in_counts.push_back(counts[c]);
                 }
-                for (uint32_t c0 = 0; c0 < take; c0 += FT_TC)
-                    for (uint32_t r0 = 0; r0 < nref; r0 += FT_TR)
-                        b.tiles.push_back(FtCorrTile{ gidx, c0, r0, 0 });
+                if (take <= FT_THIN)
+                    for (uint32_t r0 = 0; r0 < nref; r0 += 256)
+                        b.thin_tiles.push_back(FtCorrTile{ gidx, 0, r0, 0 });
+                else
+                    for (uint32_t c0 = 0; c0 < take; c0 += FT_TC)
+                        for (uint32_t r0 = 0; r0 < nref; r0 += FT_TR)
+                            b.tiles.push_back(FtCorrTile{ gidx, c0, r0, 0 });
                 eng->counters[9] += (int64_t)take * nref;
                 eng->counters[10] += (int64_t)take * nref * gsub.size();
                 b.n_scores += (uint64_t)take * nref;
@@ -1284,6 +1442,11 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         }
         if ((rc = flush()))
             return rc;
+        size_t keep = 0;
+        for (uint32_t c : active)
+            if (counts[c] > 1 && (finetune_times == 0 || nrounds[c] < finetune_times))
+                active[keep++] = c;
+        active.resize(keep);
     }
     for (uint32_t c = 0; c < Nq; ++c)
     {
@@ -1325,7 +1488,7 @@ int csg_counters(csg_engine* eng, int64_t* out)
 
 int csg_selftest(csg_engine* eng, int which, double* max_err)
 {
-    if (!eng || !max_err || which < 0 || which > 3)
+    if (!eng || !max_err || which < 0 || which > 4)
         return CSG_ERR_ARGUMENT;
     HIPCHK(hipSetDevice(eng->device));
     const int nq = 256, nr = 256, Kp = 192, G = 1500;
@@ -1375,6 +1538,13 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
         k_gemm_f16<<<dim3(nr / H_BN, nq / H_BM), 256, 0, eng->stream>>>(qf, rf, Kp, d1, d1, nq, nr,
                                                                         dS, nr);
     }
+    else if (which == 4)
+    {
+        k_split_limbs<<<bq, 256, 0, eng->stream>>>(dq, xq.size(), qh, ql);
+        k_split_limbs<<<br, 256, 0, eng->stream>>>(dr, xr.size(), rh, rl);
+        k_gemm_int8w<<<dim3(nr / W8_BN, nq / W8_BM), 512, 0, eng->stream>>>(qh, ql, rh, rl, Kp, d1,
+                                                                            d1, nq, nr, dS, nr);
+    }
     else if (which == 2)
         k_gemm_f32mfma<<<dim3(nr / F_BN, nq / F_BM), 256, 0, eng->stream>>>(dq, dr, Kp, d1, d1, nq,
                                                                             nr, dS, nr);
@@ -1404,7 +1574,7 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
             }
             const float  dev = out[(size_t)m * nr + n];
             double       err;
-            if (which == 0)
+            if (which == 0 || which == 4)
                 err = dev == (float)(double)s4 ? 0.0 : std::fabs((double)dev - (double)s4) + 1.0;
             else if (which == 3)
                 err = dev == seq ? 0.0 : std::fabs((double)dev - (double)seq) + 1.0;
@@ -1413,6 +1583,115 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
             worst = std::max(worst, err);
         }
     *max_err = worst;
+    return CSG_OK;
+}
+
+int csg_selftest_ft(csg_engine* eng, int gs, int64_t* mismatches)
+{
+    if (!eng || !mismatches || gs < 1 || gs > 4096)
+        return CSG_ERR_ARGUMENT;
+    HIPCHK(hipSetDevice(eng->device));
+    const int      ncell = 40, nref = 70;
+    const uint32_t st    = (uint32_t)round_up(gs, 8);
+    std::vector<int16_t> cr((size_t)ncell * st, 0), rr((size_t)nref * st, 0);
+    uint64_t             sd = 0x2545F4914F6CDD1Dull ^ (uint64_t)gs;
+    auto rnd = [&]() {
+        sd ^= sd << 13;
+        sd ^= sd >> 7;
+        sd ^= sd << 17;
+        return (int)(sd % (2 * gs + 1)) - gs;
+    };
+    for (int c = 0; c < ncell; ++c)
+        for (int i = 0; i < gs; ++i)
+            cr[(size_t)c * st + i] = (int16_t)rnd();
+    for (int r = 0; r < nref; ++r)
+        for (int i = 0; i < gs; ++i)
+            rr[(size_t)r * st + i] = (int16_t)rnd();
+    FtGroup g{};
+    g.gs = (uint32_t)gs;
+    g.stride = st;
+    g.nref   = nref;
+    g.ncell  = ncell;
+    std::vector<FtCorrTile> tiles;
+    for (uint32_t c0 = 0; c0 < (uint32_t)ncell; c0 += FT_TC)
+        for (uint32_t r0 = 0; r0 < (uint32_t)nref; r0 += FT_TR)
+            tiles.push_back(FtCorrTile{ 0, c0, r0, 0 });
+    std::vector<float> ones(std::max(ncell, nref), 1.f);
+    FtGroup*    dg = nullptr;
+    FtCorrTile* dt = nullptr;
+    int16_t *   dc = nullptr, *dr = nullptr;
+    float *     d1 = nullptr, *ds = nullptr;
+    int         rc = CSG_OK;
+    auto        cleanup = [&]() {
+        dfree(dg); dfree(dt); dfree(dc); dfree(dr); dfree(d1); dfree(ds);
+    };
+    if ((rc = upload(eng, &dg, &g, 1)) || (rc = upload(eng, &dt, tiles.data(), tiles.size())) ||
+        (rc = upload(eng, &dc, cr.data(), cr.size())) || (rc = upload(eng, &dr, rr.data(), rr.size())) ||
+        (rc = upload(eng, &d1, ones.data(), ones.size())) ||
+        (rc = dalloc(eng, &ds, (size_t)ncell * nref)))
+    {
+        cleanup();
+        return rc;
+    }
+    const size_t lds = (size_t)(FT_TC + FT_TR) * (st + 2) * 2;
+    if (lds > eng->lds_limit)
+    {
+        cleanup();
+        return eng->fail(CSG_ERR_ARGUMENT, "gene subset too large for the LDS plan");
+    }
+    k_ft_corr<FT_TC, FT_TR, FT_RC, FT_RR><<<(unsigned)tiles.size(), FT_NT, lds, eng->stream>>>(
+        dt, dg, dc, d1, dr, d1, ds);
+    hipError_t         le = hipGetLastError();
+    std::vector<float> out((size_t)ncell * nref), out_thin((size_t)ncell * nref);
+    hipError_t ce = hipMemcpyAsync(out.data(), ds, out.size() * 4, hipMemcpyDeviceToHost, eng->stream);
+    hipError_t se = hipStreamSynchronize(eng->stream);
+    // the thin-tile kernel on the same pairs
+    std::vector<FtCorrTile> thin;
+    for (uint32_t c0 = 0; c0 < (uint32_t)ncell; c0 += FT_THIN)
+        for (uint32_t r0 = 0; r0 < (uint32_t)nref; r0 += 256)
+            thin.push_back(FtCorrTile{ 0, c0, r0, 0 });
+    FtCorrTile* dth = nullptr;
+    hipError_t  te  = hipSuccess;
+    if (le == hipSuccess && se == hipSuccess && upload(eng, &dth, thin.data(), thin.size()) == CSG_OK)
+    {
+        k_ft_corr_thin<FT_THIN><<<(unsigned)thin.size(), 256, (size_t)FT_THIN * st * 2, eng->stream>>>(
+            dth, dg, dc, d1, dr, d1, ds);
+        te = hipGetLastError();
+        if (te == hipSuccess)
+            te = hipMemcpyAsync(out_thin.data(), ds, out_thin.size() * 4, hipMemcpyDeviceToHost,
+                                eng->stream);
+        if (te == hipSuccess)
+            te = hipStreamSynchronize(eng->stream);
+    }
+    else
+        te = hipErrorUnknown;
+    dfree(dth);
+    cleanup();
+    if (le != hipSuccess || ce != hipSuccess || se != hipSuccess || te != hipSuccess)
+        return eng->fail(CSG_ERR_DEVICE, "ft selftest kernel failed");
+    // host: 128 chunks of ceil(gs/128) genes summed by fma, stride 1..64 tree
+    int64_t   bad  = 0;
+    const int step = (gs + 127) / 128;
+    for (int c = 0; c < ncell; ++c)
+        for (int r = 0; r < nref; ++r)
+        {
+            float part[128];
+            for (int t = 0; t < 128; ++t)
+            {
+                float acc = 0.f;
+                for (int i = t * step; i < std::min((t + 1) * step, gs); ++i)
+                    acc = std::fma((float)cr[(size_t)c * st + i], (float)rr[(size_t)r * st + i], acc);
+                part[t] = acc;
+            }
+            for (int stride = 1; stride < 128; stride *= 2)
+                for (int t = 0; t < 128; t += 2 * stride)
+                    part[t] = part[t] + part[t + stride];
+            if (!(out[(size_t)c * nref + r] == part[0]))
+                ++bad;
+            if (!(out_thin[(size_t)c * nref + r] == part[0]))
+                ++bad;
+        }
+    *mismatches = bad;
     return CSG_OK;
 }
 

@@ -85,11 +85,33 @@ __device__ inline void rank_codes(const uint16_t* codes, int gs, int nbins, int*
     __syncthreads();
 }
 
-// Reference rows: gather the dense fine-tune codes at G_sub
-// (get_device_ref_lines, src/cusingler.cu:566-577), rank them
-// (rankdata_bin3, :745-801) and form the chunk-tree sum of squares.
-__global__ __launch_bounds__(128) void k_ft_rank_ref(const FtRankItem* __restrict__ items,
-                                                     const FtGroup* __restrict__ groups,
+// Group holding batch-wide item `i` given the groups' first items (sorted).
+__device__ inline uint32_t find_group(const FtGroup* __restrict__ groups, uint32_t ngroups,
+                                      uint32_t i, bool by_ref)
+{
+    uint32_t lo = 0, hi = ngroups;  // last group with first item <= i
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) / 2;
+        const uint32_t f   = by_ref ? groups[mid].ref_item0 : groups[mid].cell_item0;
+        if (f <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Reference rows: one block per reference row of every group (the group and
+// the row follow from the block index and the group's label list); gather the
+// dense fine-tune codes at G_sub (get_device_ref_lines, src/cusingler.cu:
+// 566-577), rank them (rankdata_bin3, :745-801) and form the chunk-tree sum of
+// squares.
+__global__ __launch_bounds__(128) void k_ft_rank_ref(const FtGroup* __restrict__ groups,
+                                                     uint32_t ngroups,
+                                                     const uint32_t* __restrict__ labels,
+                                                     const uint32_t* __restrict__ lstart,
+                                                     const uint32_t* __restrict__ llen,
                                                      const uint32_t* __restrict__ genes,
                                                      const uint16_t* __restrict__ ref_codes,
                                                      uint32_t Gf, int nbins,
@@ -99,23 +121,37 @@ __global__ __launch_bounds__(128) void k_ft_rank_ref(const FtRankItem* __restric
     extern __shared__ int smem[];
     __shared__ float part[128];
     __shared__ int   tmp[2];
-    const FtRankItem it = items[blockIdx.x];
-    const FtGroup    g  = groups[it.group];
-    int*             bins  = smem;
-    uint16_t*        codes = (uint16_t*)(smem + ((nbins + 3) & ~3));
-    int16_t*         row   = (int16_t*)(codes + ((g.gs + 7) & ~7u));
-    const uint16_t*  src   = ref_codes + (size_t)it.src_row * Gf;
-    const uint32_t*  gl    = genes + g.gene_off;
+    const uint32_t   item = blockIdx.x;
+    const uint32_t   gi   = find_group(groups, ngroups, item, true);
+    const FtGroup    g    = groups[gi];
+    uint32_t         j    = item - g.ref_item0;
+    uint32_t         src_row = 0;
+    for (uint32_t t = 0; t < g.nlabel; ++t)
+    {
+        const uint32_t l = labels[g.label_off + t];
+        if (j < llen[l])
+        {
+            src_row = lstart[l] + j;
+            break;
+        }
+        j -= llen[l];
+    }
+    const uint32_t  local = item - g.ref_item0;
+    int*            bins  = smem;
+    uint16_t*       codes = (uint16_t*)(smem + ((nbins + 3) & ~3));
+    int16_t*        row   = (int16_t*)(codes + ((g.gs + 7) & ~7u));
+    const uint16_t* src   = ref_codes + (size_t)src_row * Gf;
+    const uint32_t* gl    = genes + g.gene_off;
     for (uint32_t i = threadIdx.x; i < g.gs; i += 128)
         codes[i] = src[gl[i]];
     __syncthreads();
     rank_codes<128>(codes, (int)g.gs, nbins, bins, row, tmp);
-    int16_t* dst = out + it.dst;
+    int16_t* dst = out + g.ref_dst0 + (uint64_t)local * g.stride;
     for (uint32_t i = threadIdx.x; i < g.stride; i += 128)
         dst[i] = i < g.gs ? row[i] : (int16_t)0;
     const float sq = chunk_tree_sum128((int)g.gs, [&](int i) { return (float)row[i]; }, part);
     if (threadIdx.x == 0)
-        out_sq[blockIdx.x] = sq;
+        out_sq[item] = sq;
 }
 
 // Query rows: densify the CSR row over the fine-tune genes, gather G_sub
@@ -173,77 +209,273 @@ struct FtCorrTile
     uint32_t pad_;
 };
 
-template <int TC, int TR>
-__global__ __launch_bounds__(TC * TR) void k_ft_corr(const FtCorrTile* __restrict__ tiles,
-                                                     const FtGroup* __restrict__ groups,
-                                                     const int16_t* __restrict__ cell_rank,
-                                                     const float* __restrict__ cell_sq,
-                                                     const int16_t* __restrict__ ref_rank,
-                                                     const float* __restrict__ ref_sq,
-                                                     float* __restrict__ scores)
+template <int TC, int TR, int RC, int RR>
+__global__ __launch_bounds__((TC / RC) * (TR / RR)) void k_ft_corr(
+    const FtCorrTile* __restrict__ tiles, const FtGroup* __restrict__ groups,
+    const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
+    const int16_t* __restrict__ ref_rank, const float* __restrict__ ref_sq,
+    float* __restrict__ scores)
 {
-    extern __shared__ int16_t srow[];
+    extern __shared__ int srow32[];
+    constexpr int    NT    = (TC / RC) * (TR / RR);
+    const FtCorrTile tl    = tiles[blockIdx.x];
+    const FtGroup    g     = groups[tl.group];
+    const int        gs    = (int)g.gs;
+    const int        st    = (int)g.stride;  // multiple of 8
+    // LDS row pitch st + 2 int16 = an odd number of 32-bit words, so lanes
+    // reading one gene of different rows hit different banks
+    const int        pw    = st / 2 + 1;     // pitch in words
+    const int        ncell = min(TC, (int)g.ncell - (int)tl.cell0);
+    const int        nref  = min(TR, (int)g.nref - (int)tl.ref0);
+    int*             cs32  = srow32;             // TC rows
+    int*             rs32  = srow32 + TC * pw;   // TR rows
+    const int        w4    = st / 8;             // int4 per row
+    const int16_t*   cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
+    const int16_t*   rbase = ref_rank + g.ref_dst0 + (uint64_t)tl.ref0 * st;
+    for (int v = threadIdx.x; v < ncell * w4; v += NT)
+    {
+        const int  row = v / w4, c = v % w4;
+        const int4 val = *(const int4*)(cbase + (uint64_t)row * st + c * 8);
+        int*       d   = cs32 + row * pw + c * 4;
+        d[0] = val.x;
+        d[1] = val.y;
+        d[2] = val.z;
+        d[3] = val.w;
+    }
+    for (int v = threadIdx.x; v < nref * w4; v += NT)
+    {
+        const int  row = v / w4, c = v % w4;
+        const int4 val = *(const int4*)(rbase + (uint64_t)row * st + c * 8);
+        int*       d   = rs32 + row * pw + c * 4;
+        d[0] = val.x;
+        d[1] = val.y;
+        d[2] = val.z;
+        d[3] = val.w;
+    }
+    __syncthreads();
+    const int16_t* cs = (const int16_t*)cs32;
+    const int16_t* rs = (const int16_t*)rs32;
+    const int      p16 = 2 * pw;  // pitch in int16
+    // thread -> RC cells x RR refs; the RR refs of a thread are TR/RR apart
+    const int tc = threadIdx.x / (TR / RR);
+    const int tr = threadIdx.x % (TR / RR);
+    int       ci[RC], ri[RR];
+#pragma unroll
+    for (int a = 0; a < RC; ++a)
+        ci[a] = min(tc + a * (TC / RC), TC - 1);
+#pragma unroll
+    for (int b = 0; b < RR; ++b)
+        ri[b] = min(tr + b * (TR / RR), TR - 1);
+    const int step = (gs + 127) / 128;
+    const int nz   = (gs + step - 1) / step;  // chunks holding genes; later ones are 0
+    float     stk[RC][RR][7];
+    float     full[RC][RR];
+#pragma unroll
+    for (int a = 0; a < RC; ++a)
+#pragma unroll
+        for (int b = 0; b < RR; ++b)
+        {
+            full[a][b] = 0.f;
+#pragma unroll
+            for (int l = 0; l < 7; ++l)
+                stk[a][b][l] = 0.f;
+        }
+    for (int ch = 0; ch < nz; ++ch)
+    {
+        const int lo = ch * step;
+        const int hi = min(lo + step, gs);
+        float     acc[RC][RR];
+#pragma unroll
+        for (int a = 0; a < RC; ++a)
+#pragma unroll
+            for (int b = 0; b < RR; ++b)
+                acc[a][b] = 0.f;
+        for (int i = lo; i < hi; ++i)
+        {
+            float x[RC], y[RR];
+#pragma unroll
+            for (int a = 0; a < RC; ++a)
+                x[a] = (float)cs[ci[a] * p16 + i];
+#pragma unroll
+            for (int b = 0; b < RR; ++b)
+                y[b] = (float)rs[ri[b] * p16 + i];
+#pragma unroll
+            for (int a = 0; a < RC; ++a)
+#pragma unroll
+                for (int b = 0; b < RR; ++b)
+                    acc[a][b] = __builtin_fmaf(x[a], y[b], acc[a][b]);
+        }
+        // merge the chunk into the left-to-right tree: while the low bits of
+        // ch are ones the node closes a pair with its left sibling
+#pragma unroll
+        for (int a = 0; a < RC; ++a)
+#pragma unroll
+            for (int b = 0; b < RR; ++b)
+            {
+                float v    = acc[a][b];
+                int   cc   = ch;
+                bool  open = true;
+#pragma unroll
+                for (int l = 0; l < 7; ++l)
+                {
+                    if (open)
+                    {
+                        if (cc & 1)
+                        {
+                            v = stk[a][b][l] + v;
+                            cc >>= 1;
+                        }
+                        else
+                        {
+                            stk[a][b][l] = v;
+                            open         = false;
+                        }
+                    }
+                }
+                if (open)
+                    full[a][b] = v;  // ch == 127 closes the whole tree
+            }
+    }
+    // The remaining chunks are zero; adding a zero subtree leaves a sum
+    // unchanged, so the root is the pending left subtrees (set bits of nz)
+    // folded from the lowest level up.
+    float root[RC][RR];
+#pragma unroll
+    for (int a = 0; a < RC; ++a)
+#pragma unroll
+        for (int b = 0; b < RR; ++b)
+        {
+            float v    = 0.f;
+            bool  have = false;
+#pragma unroll
+            for (int l = 0; l < 7; ++l)
+                if ((nz >> l) & 1)
+                {
+                    v    = have ? stk[a][b][l] + v : stk[a][b][l];
+                    have = true;
+                }
+            root[a][b] = nz == 128 ? full[a][b] : v;
+        }
+#pragma unroll
+    for (int a = 0; a < RC; ++a)
+    {
+        const int c = tc + a * (TC / RC);
+        if (c >= ncell)
+            continue;
+        const float sq  = cell_sq[g.cell_item0 + tl.cell0 + c];
+        float*      out = scores + g.score0 + (uint64_t)(tl.cell0 + c) * g.nref + tl.ref0;
+#pragma unroll
+        for (int b = 0; b < RR; ++b)
+        {
+            const int r = tr + b * (TR / RR);
+            if (r < nref)
+                out[r] = corr_from_sums(root[a][b], sq, ref_sq[g.ref_item0 + tl.ref0 + r]);
+        }
+    }
+}
+
+// Thin tile for groups with few cells: up to TC cells (rows in LDS) x 256
+// reference rows, one reference row per thread read straight from memory
+// (each row is used by so few cells that staging it gains nothing).  Same
+// 128-chunk + tree order as k_ft_corr.
+template <int TC>
+__global__ __launch_bounds__(256) void k_ft_corr_thin(const FtCorrTile* __restrict__ tiles,
+                                                      const FtGroup* __restrict__ groups,
+                                                      const int16_t* __restrict__ cell_rank,
+                                                      const float* __restrict__ cell_sq,
+                                                      const int16_t* __restrict__ ref_rank,
+                                                      const float* __restrict__ ref_sq,
+                                                      float* __restrict__ scores)
+{
+    extern __shared__ int16_t cthin[];
     const FtCorrTile tl    = tiles[blockIdx.x];
     const FtGroup    g     = groups[tl.group];
     const int        gs    = (int)g.gs;
     const int        st    = (int)g.stride;
     const int        ncell = min(TC, (int)g.ncell - (int)tl.cell0);
-    const int        nref  = min(TR, (int)g.nref - (int)tl.ref0);
-    int16_t*         cs    = srow;           // TC x stride
-    int16_t*         rs    = srow + TC * st; // TR x stride
-    constexpr int    NT    = TC * TR;
-    const int        v8    = st / 8;
+    const int        nref  = min(256, (int)g.nref - (int)tl.ref0);
     const int16_t*   cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
-    const int16_t*   rbase = ref_rank + g.ref_dst0 + (uint64_t)tl.ref0 * st;
-    for (int v = threadIdx.x; v < ncell * v8; v += NT)
-        *(int4*)(cs + v * 8) = *(const int4*)(cbase + (uint64_t)v * 8);
-    for (int v = threadIdx.x; v < nref * v8; v += NT)
-        *(int4*)(rs + v * 8) = *(const int4*)(rbase + (uint64_t)v * 8);
+    for (int v = threadIdx.x; v < ncell * st / 8; v += 256)
+        *(int4*)(cthin + v * 8) = *(const int4*)(cbase + (uint64_t)v * 8);
     __syncthreads();
-    const int c = threadIdx.x / TR;
-    const int r = threadIdx.x % TR;
-    if (c >= ncell || r >= nref)
+    const int r = threadIdx.x;
+    if (r >= nref)
         return;
-    const int16_t* x    = cs + c * st;
-    const int16_t* y    = rs + r * st;
+    const int16_t* y    = ref_rank + g.ref_dst0 + (uint64_t)(tl.ref0 + r) * st;
     const int      step = (gs + 127) / 128;
-    float          stk[7];
+    const int      nz   = (gs + step - 1) / step;
+    float          stk[TC][7];
+    float          full[TC];
 #pragma unroll
-    for (int l = 0; l < 7; ++l)
-        stk[l] = 0.f;
-    float root = 0.f;
-    for (int ch = 0; ch < 128; ++ch)
+    for (int a = 0; a < TC; ++a)
     {
-        const int lo  = ch * step;
-        const int hi  = min(lo + step, gs);
-        float     acc = 0.f;
-        for (int i = lo; i < hi; ++i)
-            acc = __builtin_fmaf((float)x[i], (float)y[i], acc);
-        // merge: while the low bits of ch are ones the node closes a pair
-        float v  = acc;
-        int   cc = ch;
+        full[a] = 0.f;
 #pragma unroll
         for (int l = 0; l < 7; ++l)
-        {
-            if (cc & 1)
-            {
-                v = stk[l] + v;
-                cc >>= 1;
-            }
-            else
-            {
-                stk[l] = v;
-                cc     = -1;
-                break;
-            }
-        }
-        if (cc != -1)
-            root = v;  // ch == 127: the whole tree
+            stk[a][l] = 0.f;
     }
-    const uint32_t ci = g.cell_item0 + tl.cell0 + c;
-    const uint32_t ri = g.ref_item0 + tl.ref0 + r;
-    scores[g.score0 + (uint64_t)(tl.cell0 + c) * g.nref + tl.ref0 + r] =
-        corr_from_sums(root, cell_sq[ci], ref_sq[ri]);
+    for (int ch = 0; ch < nz; ++ch)
+    {
+        const int lo = ch * step;
+        const int hi = min(lo + step, gs);
+        float     acc[TC];
+#pragma unroll
+        for (int a = 0; a < TC; ++a)
+            acc[a] = 0.f;
+        for (int i = lo; i < hi; ++i)
+        {
+            const float yv = (float)y[i];
+#pragma unroll
+            for (int a = 0; a < TC; ++a)
+                if (a < ncell)
+                    acc[a] = __builtin_fmaf((float)cthin[a * st + i], yv, acc[a]);
+        }
+#pragma unroll
+        for (int a = 0; a < TC; ++a)
+        {
+            float v    = acc[a];
+            int   cc   = ch;
+            bool  open = true;
+#pragma unroll
+            for (int l = 0; l < 7; ++l)
+            {
+                if (open)
+                {
+                    if (cc & 1)
+                    {
+                        v = stk[a][l] + v;
+                        cc >>= 1;
+                    }
+                    else
+                    {
+                        stk[a][l] = v;
+                        open      = false;
+                    }
+                }
+            }
+            if (open)
+                full[a] = v;
+        }
+    }
+    const float syy = ref_sq[g.ref_item0 + tl.ref0 + r];
+#pragma unroll
+    for (int a = 0; a < TC; ++a)
+    {
+        if (a >= ncell)
+            continue;
+        float v    = 0.f;
+        bool  have = false;
+#pragma unroll
+        for (int l = 0; l < 7; ++l)
+            if ((nz >> l) & 1)
+            {
+                v    = have ? stk[a][l] + v : stk[a][l];
+                have = true;
+            }
+        const float root = nz == 128 ? full[a] : v;
+        scores[g.score0 + (uint64_t)(tl.cell0 + a) * g.nref + tl.ref0 + r] =
+            corr_from_sums(root, cell_sq[g.cell_item0 + tl.cell0 + a], syy);
+    }
 }
 
 // New candidate list of one cell from its per-label scores, following
@@ -256,6 +488,37 @@ struct FtCell
     uint32_t cell;     // query row
     uint32_t group;
     uint64_t seg0;     // first per-label score (ascending label order)
+};
+
+// Segments of the fine-tune select: item s belongs to the cell whose seg0 is
+// the last one <= s; its label is the group's (s - seg0)-th, its scores the
+// cell's row of the group's score block at that label's offset.
+struct FtSegments
+{
+    const FtCell*   cells;
+    uint32_t        ncells;
+    const FtGroup*  groups;
+    const uint32_t* labels;
+    const uint32_t* label_pos;  // offset of each group label within the group's rows
+    const uint32_t* llen;
+    __device__ Segment get(uint64_t s) const
+    {
+        uint32_t lo = 0, hi = ncells;
+        while (hi - lo > 1)
+        {
+            const uint32_t mid = (lo + hi) / 2;
+            if (cells[mid].seg0 <= s)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const FtCell   fc = cells[lo];
+        const FtGroup  g  = groups[fc.group];
+        const uint32_t j  = (uint32_t)(s - fc.seg0);
+        const uint32_t l  = labels[g.label_off + j];
+        const uint32_t c  = lo - g.cell_item0;  // cell within the group
+        return Segment{ g.score0 + (uint64_t)c * g.nref + label_pos[g.label_off + j], llen[l], l };
+    }
 };
 
 __global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,

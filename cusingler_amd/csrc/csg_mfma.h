@@ -208,6 +208,141 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
         }
 }
 
+// Wide-tile variant for |X| <= 1919 (gene sets of at most 1920 genes).  Then
+// the high limb h lies in [-8, 7], 16*h is still an int8 and
+//     X*Y = 256 * (h*l' + l*h' + (16h)*(16h')) + l*l'
+// so two int32 partials suffice: acc_x takes three products, acc_ll one.  16*h
+// is formed in registers from the packed high limbs ((w << 4) & 0xF0F0F0F0 per
+// dword: the low nibble of each byte is h in two's complement).  With half the
+// accumulators a wave holds a 64 x 64 tile and the block 128 x 256, which
+// halves the operand traffic per product of the 64 x 128 kernel.
+constexpr int W8_BM = 128, W8_BN = 256, W8_BK = 64, W8_LD = 80;
+
+__device__ inline v4i times16(v4i h)
+{
+    v4i r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        r[i] = (int)(((unsigned)h[i] << 4) & 0xF0F0F0F0u);
+    return r;
+}
+
+__global__ __launch_bounds__(512) void k_gemm_int8w(
+    const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
+    const int8_t* __restrict__ rl, int Kp, const float* __restrict__ sq_q,
+    const float* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+{
+    // LDS rows: [A hi 128][A lo 128][B hi 256][B lo 256], 80 bytes each
+    __shared__ __attribute__((aligned(16))) int8_t sm[(2 * W8_BM + 2 * W8_BN) * W8_LD];
+    const int tid  = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid  = tid >> 6;
+    const int wm   = wid >> 2;  // 0..1: 64-row half of the block
+    const int wn   = wid & 3;   // 0..3: 64-column quarter
+    const int m0   = blockIdx.y * W8_BM;
+    const int n0   = blockIdx.x * W8_BN;
+
+    // staging map: chunk c (16 bytes) -> (plane, row, 16-byte column)
+    // A: 2 planes x 128 rows x 4 = 1024 chunks (2 per thread);
+    // B: 2 planes x 256 rows x 4 = 2048 chunks (4 per thread)
+    const int ca0 = tid, ca1 = tid + 512;
+    const int cb0 = tid, cb1 = tid + 512, cb2 = tid + 1024, cb3 = tid + 1536;
+#define W8_ASRC(c, k0) ((((c) >> 9) ? ql : qh) + (size_t)(m0 + (((c) >> 2) & 127)) * Kp + (k0) + ((c) & 3) * 16)
+#define W8_BSRC(c, k0) ((((c) >> 10) ? rl : rh) + (size_t)(n0 + (((c) >> 2) & 255)) * Kp + (k0) + ((c) & 3) * 16)
+#define W8_ADST(c) (sm + ((((c) >> 9) * W8_BM) + (((c) >> 2) & 127)) * W8_LD + ((c) & 3) * 16)
+#define W8_BDST(c) (sm + (2 * W8_BM + (((c) >> 10) * W8_BN) + (((c) >> 2) & 255)) * W8_LD + ((c) & 3) * 16)
+
+    v4i acc_x[4][4], acc_ll[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            acc_x[i][j]  = v4i{ 0, 0, 0, 0 };
+            acc_ll[i][j] = v4i{ 0, 0, 0, 0 };
+        }
+    int4 pa0 = *(const int4*)W8_ASRC(ca0, 0);
+    int4 pa1 = *(const int4*)W8_ASRC(ca1, 0);
+    int4 pb0 = *(const int4*)W8_BSRC(cb0, 0);
+    int4 pb1 = *(const int4*)W8_BSRC(cb1, 0);
+    int4 pb2 = *(const int4*)W8_BSRC(cb2, 0);
+    int4 pb3 = *(const int4*)W8_BSRC(cb3, 0);
+    const int r16  = lane & 15;
+    const int kofs = (lane >> 4) * 16;
+    for (int k0 = 0; k0 < Kp; k0 += W8_BK)
+    {
+        __syncthreads();  // the previous stage is no longer read
+        *(int4*)W8_ADST(ca0) = pa0;
+        *(int4*)W8_ADST(ca1) = pa1;
+        *(int4*)W8_BDST(cb0) = pb0;
+        *(int4*)W8_BDST(cb1) = pb1;
+        *(int4*)W8_BDST(cb2) = pb2;
+        *(int4*)W8_BDST(cb3) = pb3;
+        __syncthreads();
+        if (k0 + W8_BK < Kp)
+        {
+            const int kn = k0 + W8_BK;
+            pa0 = *(const int4*)W8_ASRC(ca0, kn);
+            pa1 = *(const int4*)W8_ASRC(ca1, kn);
+            pb0 = *(const int4*)W8_BSRC(cb0, kn);
+            pb1 = *(const int4*)W8_BSRC(cb1, kn);
+            pb2 = *(const int4*)W8_BSRC(cb2, kn);
+            pb3 = *(const int4*)W8_BSRC(cb3, kn);
+        }
+        v4i ah[4], al[4], ah16[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+            const int row = wm * 64 + i * 16 + r16;
+            ah[i]   = *(const v4i*)(sm + row * W8_LD + kofs);
+            al[i]   = *(const v4i*)(sm + (W8_BM + row) * W8_LD + kofs);
+            ah16[i] = times16(ah[i]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int row = wn * 64 + j * 16 + r16;
+            const v4i bh  = *(const v4i*)(sm + (2 * W8_BM + row) * W8_LD + kofs);
+            const v4i bl  = *(const v4i*)(sm + (2 * W8_BM + W8_BN + row) * W8_LD + kofs);
+            const v4i bh16 = times16(bh);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bl, acc_x[i][j], 0, 0, 0);
+                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bh, acc_x[i][j], 0, 0, 0);
+                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah16[i], bh16, acc_x[i][j], 0, 0, 0);
+                acc_ll[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bl, acc_ll[i][j], 0, 0, 0);
+            }
+        }
+    }
+#undef W8_ASRC
+#undef W8_BSRC
+#undef W8_ADST
+#undef W8_BDST
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + e;
+            if (m >= nq)
+                continue;
+            const float sq  = sq_q[m];
+            float*      out = S + (size_t)m * ldS;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int n = n0 + wn * 64 + j * 16 + r16;
+                if (n >= nr)
+                    continue;
+                const long long s4 = ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
+                const float prod = sq * sq_r[n];
+                const float d    = sqrtf(prod);
+                out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
+            }
+        }
+}
+
 // ------------------------------------- approximate MFMA contractions ---
 
 // FP16 operands, FP32 accumulation (v_mfma_f32_16x16x32_f16).  Same tiling as

@@ -72,6 +72,7 @@ def lib():
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
     L.csg_selftest.argtypes = [_V, C.c_int, P(C.c_double)]
+    L.csg_selftest_ft.argtypes = [_V, C.c_int, P(_I64)]
     L.csg_host_percentile.argtypes = [_V, _I64, _F, P(_F)]
     L.csg_host_minmax.argtypes = [_V, C.c_int, P(C.c_int), P(C.c_int)]
     L.csg_prep_train.argtypes = [C.c_int, _U32, _U32, _V, _V, _V, _U32, _U32, _V, _V, _V, _U32, _V,
@@ -250,6 +251,11 @@ class Engine:
     def selftest(self, which: int) -> float:
         out = C.c_double()
         self._check(lib().csg_selftest(self._h, which, C.byref(out)), "selftest")
+        return out.value
+
+    def selftest_ft(self, gs: int) -> int:
+        out = _I64()
+        self._check(lib().csg_selftest_ft(self._h, gs, C.byref(out)), "selftest_ft")
         return out.value
 
     def counters(self) -> Dict[str, int]:

@@ -301,3 +301,24 @@ def tiny(n_ref_per_label=6, n_labels=3, n_genes=300, n_qry=10, nonzero=0.3, mark
     cfg = SynthConfig("tiny", n_ref_per_label * n_labels, n_qry, n_common, n_ref_only, n_qry_only,
                       n_labels, markers, 3.0, 6.0, nonzero, props=props, seed=seed)
     return generate(cfg)
+
+
+def with_empty_rows(ds: Dataset, ref_rows, qry_rows) -> Dataset:
+    """Copy of `ds` whose listed reference / query rows store no values (an
+    all-zero cell: its ranks are constant, so its correlations are NaN)."""
+    def strip(data, indices, indptr, rows):
+        rows = set(int(r) for r in rows)
+        keep = np.ones(data.shape[0], dtype=bool)
+        for r in rows:
+            keep[indptr[r]:indptr[r + 1]] = False
+        counts = np.diff(indptr)
+        for r in rows:
+            counts[r] = 0
+        new_ptr = np.zeros_like(indptr)
+        new_ptr[1:] = np.cumsum(counts)
+        return data[keep], indices[keep], new_ptr
+
+    rd, ri, rp = strip(ds.ref_data, ds.ref_indices, ds.ref_indptr, ref_rows)
+    qd, qi, qp = strip(ds.qry_data, ds.qry_indices, ds.qry_indptr, qry_rows)
+    return dataclasses.replace(ds, ref_data=rd, ref_indices=ri, ref_indptr=rp, qry_data=qd,
+                               qry_indices=qi, qry_indptr=qp)

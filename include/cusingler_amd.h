@@ -126,12 +126,20 @@ int csg_counters(csg_engine* eng, int64_t* out);
 
 /* Diagnostic of the contraction kernels on seeded random ranks (|X| <= 1500,
  * 256 x 256 rows, K = 192): which = 0 INT8 limbs, 1 FP16, 2 FP32 MFMA,
- * 3 plain FP32.  *max_err receives 0 when the device results equal the host's
- * exact (0) / sequential-fma (3) values, else the largest difference (+1); for
+ * 3 plain FP32, 4 INT8 limbs wide-tile kernel (gene sets up to 1920).  *max_err receives 0 when the device results equal the host's
+ * exact (0, 4) / sequential-fma (3) values, else the largest difference (+1); for
  * 1 and 2 the largest error of the dot product relative to sqrt(sxx * syy),
  * i.e. in correlation units.  No
  * counterpart in the reference: it checks the MFMA operand layouts. */
 int csg_selftest(csg_engine* eng, int which, double* max_err);
+
+/* Diagnostic of the fine-tune correlation kernels (wide and thin tiles): 40 x
+ * 70 pairs of seeded random ranks over `gs` genes, compared bit for bit with
+ * the 128-chunk + tree FP32 order of spearman_reduce
+ * (src/cusingler.cu:804-856) computed on the host; *mismatches receives the
+ * number of differing sums over both kernels.  No counterpart in
+ * the reference. */
+int csg_selftest_ft(csg_engine* eng, int gs, int64_t* mismatches);
 
 /* Host-side evaluation of the reference's `percentile` (src/cusingler.cu:
  * 947-978) with the same libstdc++ sort emulation the device uses; returns

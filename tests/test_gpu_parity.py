@@ -35,10 +35,17 @@ def _datasets():
                                              props="dirichlet", seed=5),
         "small": lambda: synth.generate("small"),
         "pancreas_2k": lambda: synth.generate("pancreas", n_qry=2000),
+        # empty reference and query cells: constant ranks, NaN correlations
+        "tiny_nan": lambda: synth.with_empty_rows(
+            synth.tiny(n_ref_per_label=10, n_labels=3, n_genes=320, n_qry=30, seed=21),
+            [3, 17], [5, 11]),
     }
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_filtered", "tiny_dirichlet", "small", "pancreas_2k"])
+ALL = ["tiny", "tiny_filtered", "tiny_dirichlet", "tiny_nan", "small", "pancreas_2k"]
+
+
+@pytest.mark.parametrize("name", ALL)
 def test_score_step_plain_bit_exact(eng, name):
     from cusingler_amd import engine
     from oracle import oracle as O
@@ -56,7 +63,7 @@ def test_score_step_plain_bit_exact(eng, name):
     assert np.array_equal(mask, om), f"{name} candidate masks differ"
 
 
-@pytest.mark.parametrize("name", ["tiny", "tiny_filtered", "tiny_dirichlet", "small", "pancreas_2k"])
+@pytest.mark.parametrize("name", ALL)
 def test_finetune_plain_matches_oracle(eng, name):
     from cusingler_amd import engine
     from oracle import oracle as O
