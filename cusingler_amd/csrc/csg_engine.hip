@@ -117,6 +117,9 @@ struct csg_engine
     size_t    d_ft_out_cap = 0;
     uint8_t * h_ft_stage = nullptr, *h_ft_out = nullptr;
     size_t    h_ft_stage_cap = 0, h_ft_out_cap = 0;
+    uint8_t * d_ft_redo = nullptr, *h_ft_redo = nullptr;
+    size_t    d_ft_redo_cap = 0, h_ft_redo_cap = 0;
+    bool      ft_exact = false;  // exact (certified) fine-tune contraction
 
     // ---- timing / counters ----
     double  phase_ms[CSG_N_PHASES]  = {};
@@ -467,6 +470,12 @@ int csg_destroy(csg_engine* eng)
     dfree(eng->d_ft_P);
     dfree(eng->d_ft_nan);
     dfree(eng->d_ft_out);
+    dfree(eng->d_ft_redo);
+    eng->d_ft_redo_cap = 0;
+    if (eng->h_ft_redo)
+        (void)hipHostFree(eng->h_ft_redo);
+    eng->h_ft_redo     = nullptr;
+    eng->h_ft_redo_cap = 0;
     eng->d_ft_tables_cap = eng->d_ft_rr_cap = eng->d_ft_cr_cap = eng->d_ft_rsq_cap = 0;
     eng->d_ft_csq_cap = eng->d_ft_scores_cap = eng->d_ft_P_cap = eng->d_ft_nan_cap = 0;
     eng->d_ft_out_cap = 0;
@@ -749,6 +758,7 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     if (rc)
         return rc;
     std::memset(eng->counters, 0, sizeof(eng->counters));
+    eng->ft_exact    = path == CSG_PATH_EXACT_INT8;
     eng->counters[0] = eng->Nq;
     eng->counters[1] = eng->G;
     eng->counters[2] = eng->Nr;
@@ -1019,6 +1029,107 @@ int ensure_host(csg_engine* eng, uint8_t** p, size_t* cap, size_t n)
 
 size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
+// Cells of an exact fine-tune batch whose decisions could not be certified:
+// their scores are recomputed in the reference's FP32 chunk-tree order (thin
+// tiles limited to the one cell), their quantiles re-selected (libstdc++ sort
+// emulation for NaN segments) and the reference's filter applied.  The
+// group tables of the batch are still on the device.
+int redo_ft_cells(csg_engine* eng, const FtBatch& b, const std::vector<uint32_t>& redo,
+                  const FtGroup* d_groups, const uint32_t* d_labels, const uint32_t* d_lpos,
+                  float quantile, float thre, std::vector<uint8_t>& lists,
+                  std::vector<uint32_t>& counts)
+{
+    const uint32_t          L = eng->fL;
+    const uint32_t          n = (uint32_t)redo.size();
+    std::vector<FtCell>     cells(n);
+    std::vector<FtCorrTile> tiles;
+    std::vector<uint8_t>    in_lists((size_t)n * L);
+    std::vector<uint32_t>   in_counts(n);
+    uint64_t                nseg = 0;
+    for (uint32_t t = 0; t < n; ++t)
+    {
+        const FtCell&  fc = b.cells[redo[t]];
+        const FtGroup& g  = b.groups[fc.group];
+        cells[t]          = FtCell{ fc.cell, fc.group, fc.pos, 0, nseg };
+        nseg += g.nlabel;
+        for (uint32_t r0 = 0; r0 < g.nref; r0 += 256)
+            tiles.push_back(FtCorrTile{ fc.group, fc.pos, r0, 1 });
+        std::memcpy(&in_lists[(size_t)t * L], &b.in_lists[(size_t)redo[t] * L], L);
+        in_counts[t] = b.in_counts[redo[t]];
+    }
+    size_t off   = 0;
+    auto   place = [&](size_t bytes) {
+        const size_t o = off;
+        off            = align16(off + bytes);
+        return o;
+    };
+    const size_t o_cells = place(n * sizeof(FtCell));
+    const size_t o_tiles = place(tiles.size() * sizeof(FtCorrTile));
+    const size_t o_inl   = place(in_lists.size());
+    const size_t o_inc   = place(n * 4);
+    const size_t o_out   = place((size_t)n * (L + 4));
+    const size_t o_P     = place(nseg * 4);
+    int          rc;
+    if ((rc = ensure(eng, &eng->d_ft_redo, &eng->d_ft_redo_cap, off)) ||
+        (rc = ensure_host(eng, &eng->h_ft_redo, &eng->h_ft_redo_cap, off)))
+        return rc;
+    uint8_t* h = eng->h_ft_redo;
+    std::memcpy(h + o_cells, cells.data(), n * sizeof(FtCell));
+    std::memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(FtCorrTile));
+    std::memcpy(h + o_inl, in_lists.data(), in_lists.size());
+    std::memcpy(h + o_inc, in_counts.data(), n * 4);
+    HIPCHK(hipMemcpyAsync(eng->d_ft_redo, h, o_out, hipMemcpyHostToDevice, eng->stream));
+    uint8_t*      d       = eng->d_ft_redo;
+    const FtCell* d_cells = (const FtCell*)(d + o_cells);
+    uint32_t      max_st  = 8;
+    for (auto& g : b.groups)
+        max_st = std::max(max_st, g.stride);
+    {
+        PhaseScope ps(eng, 6, 1);
+        k_ft_corr_thin<FT_THIN><<<(unsigned)tiles.size(), 256, (size_t)FT_THIN * max_st * 2,
+                                  eng->stream>>>((const FtCorrTile*)(d + o_tiles), d_groups,
+                                                 eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
+                                                 eng->d_ft_rsq, eng->d_ft_scores);
+        LAUNCHCHK();
+    }
+    FtSegments src{ d_cells, n, d_groups, d_labels, d_lpos, eng->d_fllen };
+    float*     d_P = (float*)(d + o_P);
+    {
+        PhaseScope ps(eng, 8, 1);
+        HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
+        k_select<FtSegments><<<(unsigned)((nseg + 3) / 4), 256, 0, eng->stream>>>(
+            src, nseg, eng->d_ft_scores, eng->d_fpct, quantile, d_P, eng->d_ft_nan,
+            eng->d_nan_count);
+        LAUNCHCHK();
+    }
+    std::swap(eng->d_nan_list, eng->d_ft_nan);
+    rc = resolve_nan_segments(eng, src, eng->d_ft_scores, eng->d_fpct, quantile, d_P);
+    std::swap(eng->d_nan_list, eng->d_ft_nan);
+    if (rc)
+        return rc;
+    uint8_t*  d_ol = d + o_out;
+    uint32_t* d_oc = (uint32_t*)(d + o_out + (size_t)n * L);
+    {
+        PhaseScope ps(eng, 8, 1);
+        k_ft_filter<<<(n + 127) / 128, 128, 0, eng->stream>>>(
+            d_cells, n, d_groups, d_labels, d_P, (const uint8_t*)(d + o_inl),
+            (const uint32_t*)(d + o_inc), L, thre, d_ol, d_oc);
+        LAUNCHCHK();
+    }
+    HIPCHK(hipMemcpyAsync(h + o_out, d + o_out, (size_t)n * (L + 4), hipMemcpyDeviceToHost,
+                          eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    const uint8_t*  ol = h + o_out;
+    const uint32_t* oc = (const uint32_t*)(h + o_out + (size_t)n * L);
+    for (uint32_t t = 0; t < n; ++t)
+    {
+        const uint32_t c = b.cell_ids[redo[t]];
+        counts[c]        = oc[t];
+        std::memcpy(&lists[(size_t)c * L], &ol[(size_t)t * L], L);
+    }
+    return CSG_OK;
+}
+
 // Uploads a batch (one packed copy), runs rank / correlate / select / filter,
 // and writes the new candidate lists of the batch's cells into `lists` /
 // `counts`.
@@ -1027,6 +1138,34 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
 {
     const uint32_t L     = eng->fL;
     const uint32_t ncell = (uint32_t)b.cells.size();
+    uint32_t       max_gs = 0;
+    for (auto& g : b.groups)
+        max_gs = std::max(max_gs, g.gs);
+    // the exact form needs |X| <= 1919 for the folded limb products
+    const bool exact = eng->ft_exact && max_gs <= 1919;
+    b.tiles.clear();
+    b.thin_tiles.clear();
+    for (uint32_t gi = 0; gi < (uint32_t)b.groups.size(); ++gi)
+    {
+        const FtGroup& g = b.groups[gi];
+        if (exact)
+        {
+            for (uint32_t c0 = 0; c0 < g.ncell; c0 += 16)
+                for (uint32_t r0 = 0; r0 < g.nref; r0 += 256)
+                    b.tiles.push_back(FtCorrTile{ gi, c0, r0, 0 });
+        }
+        else if (g.ncell <= (uint32_t)FT_THIN)
+        {
+            for (uint32_t r0 = 0; r0 < g.nref; r0 += 256)
+                b.thin_tiles.push_back(FtCorrTile{ gi, 0, r0, 0 });
+        }
+        else
+        {
+            for (uint32_t c0 = 0; c0 < g.ncell; c0 += FT_TC)
+                for (uint32_t r0 = 0; r0 < g.nref; r0 += FT_TR)
+                    b.tiles.push_back(FtCorrTile{ gi, c0, r0, 0 });
+        }
+    }
     // table layout in the packed upload
     size_t off = 0;
     auto   place = [&](size_t bytes) {
@@ -1085,15 +1224,13 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
         (rc = ensure(eng, &eng->d_ft_scores, &eng->d_ft_scores_cap, b.n_scores)) ||
         (rc = ensure(eng, &eng->d_ft_P, &eng->d_ft_P_cap, nseg)) ||
         (rc = ensure(eng, &eng->d_ft_nan, &eng->d_ft_nan_cap, nseg)) ||
-        (rc = ensure(eng, &eng->d_ft_out, &eng->d_ft_out_cap, (size_t)ncell * (L + 4))) ||
-        (rc = ensure_host(eng, &eng->h_ft_out, &eng->h_ft_out_cap, (size_t)ncell * (L + 4))))
+        (rc = ensure(eng, &eng->d_ft_out, &eng->d_ft_out_cap, (size_t)ncell * (L + 5))) ||
+        (rc = ensure_host(eng, &eng->h_ft_out, &eng->h_ft_out_cap, (size_t)ncell * (L + 5))))
         return rc;
     uint8_t*  d_out_lists  = eng->d_ft_out;
     uint32_t* d_out_counts = (uint32_t*)(eng->d_ft_out + (size_t)ncell * L);
+    uint8_t*  d_status     = eng->d_ft_out + (size_t)ncell * (L + 4);
 
-    uint32_t max_gs = 0;
-    for (auto& g : b.groups)
-        max_gs = std::max(max_gs, g.gs);
     const size_t stride_max = round_up(max_gs, 8);
     const size_t lds_rank_q = (size_t)((eng->fnbins + 3) & ~3) * 4 +
                               round_up(std::max<uint32_t>(eng->Gf, 1), 8) * 2 + stride_max * 4;
@@ -1116,6 +1253,15 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
             std::max<uint32_t>(eng->Gf, 1), eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
         LAUNCHCHK();
     }
+    if (exact)
+    {
+        PhaseScope ps(eng, 6, 1);
+        k_ft_mfma<<<(unsigned)b.tiles.size(), 256, 0, eng->stream>>>(
+            d_tiles, d_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr, eng->d_ft_rsq,
+            eng->d_ft_scores);
+        LAUNCHCHK();
+    }
+    else
     {
         PhaseScope ps(eng, 6, (int)!b.tiles.empty() + (int)!b.thin_tiles.empty());
         if (!b.tiles.empty())
@@ -1142,6 +1288,41 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
             src, nseg, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P, eng->d_ft_nan,
             eng->d_nan_count);
         LAUNCHCHK();
+    }
+    if (exact)
+    {
+        // certified filter; NaN quantiles are left NaN and send their cells
+        // to the FP32 redo below
+        {
+            PhaseScope ps(eng, 8, 1);
+            k_ft_filter_exact<<<(ncell + 127) / 128, 128, 0, eng->stream>>>(
+                d_cells, ncell, d_groups, d_labels, eng->d_ft_P, d_in_lists, d_in_cnt, L, thre,
+                d_out_lists, d_out_counts, d_status);
+            LAUNCHCHK();
+        }
+        HIPCHK(hipMemcpyAsync(eng->h_ft_out, eng->d_ft_out, (size_t)ncell * (L + 5),
+                              hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        const uint8_t*        ol = eng->h_ft_out;
+        const uint32_t*       oc = (const uint32_t*)(eng->h_ft_out + (size_t)ncell * L);
+        const uint8_t*        st = eng->h_ft_out + (size_t)ncell * (L + 4);
+        std::vector<uint32_t> redo;
+        for (uint32_t i = 0; i < ncell; ++i)
+        {
+            if (st[i])
+            {
+                redo.push_back(i);
+                continue;
+            }
+            const uint32_t c = b.cell_ids[i];
+            counts[c]        = oc[i];
+            std::memcpy(&lists[(size_t)c * L], &ol[(size_t)i * L], L);
+        }
+        if (redo.empty())
+            return CSG_OK;
+        eng->counters[11] += (int64_t)redo.size();
+        return redo_ft_cells(eng, b, redo, d_groups, d_labels, d_lpos, quantile, thre, lists,
+                             counts);
     }
     std::swap(eng->d_nan_list, eng->d_ft_nan);
     rc = resolve_nan_segments(eng, FtSegments{ d_cells, ncell, d_groups, d_labels, d_lpos, eng->d_fllen },
@@ -1416,20 +1597,13 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
                     const uint32_t c = cells[cpos + t];
                     b.cell_items.push_back(FtRankItem{ c, gidx, b.cell_elems });
                     b.cell_elems += stride;
-                    b.cells.push_back(FtCell{ c, gidx, b.n_segs });
+                    b.cells.push_back(FtCell{ c, gidx, (uint32_t)t, 0, b.n_segs });
                     b.cell_ids.push_back(c);
                     b.n_segs += labs.size();
                     b.in_lists.insert(b.in_lists.end(), &lists[(size_t)c * L],
                                       &lists[(size_t)c * L] + L);
                     b.in_counts.push_back(counts[c]);
                 }
-                if (take <= FT_THIN)
-                    for (uint32_t r0 = 0; r0 < nref; r0 += 256)
-                        b.thin_tiles.push_back(FtCorrTile{ gidx, 0, r0, 0 });
-                else
-                    for (uint32_t c0 = 0; c0 < take; c0 += FT_TC)
-                        for (uint32_t r0 = 0; r0 < nref; r0 += FT_TR)
-                            b.tiles.push_back(FtCorrTile{ gidx, c0, r0, 0 });
                 eng->counters[9] += (int64_t)take * nref;
                 eng->counters[10] += (int64_t)take * nref * gsub.size();
                 b.n_scores += (uint64_t)take * nref;
@@ -1454,6 +1628,14 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         if (rounds_out)
             rounds_out[c] = nrounds[c];
     }
+    return CSG_OK;
+}
+
+int csg_set_finetune_path(csg_engine* eng, int path)
+{
+    if (!eng || path < CSG_PATH_PLAIN || path > CSG_PATH_FP16_MFMA)
+        return CSG_ERR_ARGUMENT;
+    eng->ft_exact = path == CSG_PATH_EXACT_INT8;
     return CSG_OK;
 }
 

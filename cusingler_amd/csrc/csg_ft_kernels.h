@@ -206,7 +206,7 @@ struct FtCorrTile
     uint32_t group;
     uint32_t cell0;  // first cell of the tile within the group
     uint32_t ref0;   // first reference row of the tile within the group
-    uint32_t pad_;
+    uint32_t lim;    // at most this many cells from cell0 (0: no limit)
 };
 
 template <int TC, int TR, int RC, int RR>
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_ft_corr_thin(const FtCorrTile* __restri
     const FtGroup    g     = groups[tl.group];
     const int        gs    = (int)g.gs;
     const int        st    = (int)g.stride;
-    const int        ncell = min(TC, (int)g.ncell - (int)tl.cell0);
+    const int        ncell = min(tl.lim ? (int)tl.lim : TC, min(TC, (int)g.ncell - (int)tl.cell0));
     const int        nref  = min(256, (int)g.nref - (int)tl.ref0);
     const int16_t*   cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
     for (int v = threadIdx.x; v < ncell * st / 8; v += 256)
@@ -485,9 +485,11 @@ __global__ __launch_bounds__(256) void k_ft_corr_thin(const FtCorrTile* __restri
 // nothing kept -> {front}.
 struct FtCell
 {
-    uint32_t cell;     // query row
+    uint32_t cell;   // query row
     uint32_t group;
-    uint64_t seg0;     // first per-label score (ascending label order)
+    uint32_t pos;    // cell position within the group's rows
+    uint32_t pad_;
+    uint64_t seg0;   // first per-label quantile (ascending label order)
 };
 
 // Segments of the fine-tune select: item s belongs to the cell whose seg0 is
@@ -516,7 +518,7 @@ struct FtSegments
         const FtGroup  g  = groups[fc.group];
         const uint32_t j  = (uint32_t)(s - fc.seg0);
         const uint32_t l  = labels[g.label_off + j];
-        const uint32_t c  = lo - g.cell_item0;  // cell within the group
+        const uint32_t c  = fc.pos;
         return Segment{ g.score0 + (uint64_t)c * g.nref + label_pos[g.label_off + j], llen[l], l };
     }
 };
@@ -638,6 +640,236 @@ __global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
     if (n == 0)
         out[n++] = in[0];
     out_counts[i] = n;
+}
+
+// Exact fine-tune contraction (INT8 limbs on MFMA) for gene subsets of at most
+// 1919 genes: S4 = sum_i X_cell[i] * X_ref[i] exactly, divided by the
+// reference's own chunk-tree FP32 divisor sqrt(sxx * syy).  Tile: 16 cells x
+// 256 reference rows per 256-thread block (4 waves of 16 x 64), K staged 64
+// at a time; the ranks are split into limbs while staging (X = 256 h + l) and
+// the hi*hi product is folded into the cross partial as (16h)(16h).
+typedef int v4i_ft __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_ft_mfma(const FtCorrTile* __restrict__ tiles,
+                                                 const FtGroup* __restrict__ groups,
+                                                 const int16_t* __restrict__ cell_rank,
+                                                 const float* __restrict__ cell_sq,
+                                                 const int16_t* __restrict__ ref_rank,
+                                                 const float* __restrict__ ref_sq,
+                                                 float* __restrict__ scores)
+{
+    constexpr int LD = 80;  // bytes per LDS row
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][16 * LD];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][256 * LD];
+    const FtCorrTile tl    = tiles[blockIdx.x];
+    const FtGroup    g     = groups[tl.group];
+    const int        st    = (int)g.stride;
+    const int        ncell = min(16, (int)g.ncell - (int)tl.cell0);
+    const int        nref  = min(256, (int)g.nref - (int)tl.ref0);
+    const int16_t*   cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
+    const int16_t*   rbase = ref_rank + g.ref_dst0 + (uint64_t)tl.ref0 * st;
+    const int        tid   = threadIdx.x;
+    const int        lane  = tid & 63;
+    const int        wid   = tid >> 6;
+    const int        r16   = lane & 15;
+    const int        kofs  = (lane >> 4) * 16;
+    v4i_ft           acc_x[4], acc_ll[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+        acc_x[j]  = v4i_ft{ 0, 0, 0, 0 };
+        acc_ll[j] = v4i_ft{ 0, 0, 0, 0 };
+    }
+    auto split8 = [](int4 v, int2& hi, int2& lo) {
+        const int16_t* x = (const int16_t*)&v;
+        int8_t         h[8], l[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const int lv = (int)(int8_t)(x[u] & 0xff);
+            l[u]         = (int8_t)lv;
+            h[u]         = (int8_t)((x[u] - lv) >> 8);
+        }
+        hi = *(const int2*)h;
+        lo = *(const int2*)l;
+    };
+    for (int k0 = 0; k0 < st; k0 += 64)
+    {
+        __syncthreads();
+        // A: 16 rows x 8 chunks of 8 values = 128 chunks
+        if (tid < 128)
+        {
+            const int row = tid >> 3, ch = tid & 7, k = k0 + ch * 8;
+            int4      v   = int4{ 0, 0, 0, 0 };
+            if (row < ncell && k < st)
+                v = *(const int4*)(cbase + (uint64_t)row * st + k);
+            int2 hi, lo;
+            split8(v, hi, lo);
+            *(int2*)(&sa[0][row * LD + ch * 8]) = hi;
+            *(int2*)(&sa[1][row * LD + ch * 8]) = lo;
+        }
+        // B: 256 rows x 8 chunks = 2048 chunks, 8 per thread
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const int c = tid + u * 256, row = c >> 3, ch = c & 7, k = k0 + ch * 8;
+            int4      v = int4{ 0, 0, 0, 0 };
+            if (row < nref && k < st)
+                v = *(const int4*)(rbase + (uint64_t)row * st + k);
+            int2 hi, lo;
+            split8(v, hi, lo);
+            *(int2*)(&sb[0][row * LD + ch * 8]) = hi;
+            *(int2*)(&sb[1][row * LD + ch * 8]) = lo;
+        }
+        __syncthreads();
+        const v4i_ft ah = *(const v4i_ft*)(&sa[0][r16 * LD + kofs]);
+        const v4i_ft al = *(const v4i_ft*)(&sa[1][r16 * LD + kofs]);
+        v4i_ft       ah16;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            ah16[i] = (int)(((unsigned)ah[i] << 4) & 0xF0F0F0F0u);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int    row = wid * 64 + j * 16 + r16;
+            const v4i_ft bh  = *(const v4i_ft*)(&sb[0][row * LD + kofs]);
+            const v4i_ft bl  = *(const v4i_ft*)(&sb[1][row * LD + kofs]);
+            v4i_ft       bh16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                bh16[i] = (int)(((unsigned)bh[i] << 4) & 0xF0F0F0F0u);
+            acc_x[j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah, bl, acc_x[j], 0, 0, 0);
+            acc_x[j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bh, acc_x[j], 0, 0, 0);
+            acc_x[j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah16, bh16, acc_x[j], 0, 0, 0);
+            acc_ll[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bl, acc_ll[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+    {
+        const int m = (lane >> 4) * 4 + e;
+        if (m >= ncell)
+            continue;
+        const float sq  = cell_sq[g.cell_item0 + tl.cell0 + m];
+        float*      out = scores + g.score0 + (uint64_t)(tl.cell0 + m) * g.nref + tl.ref0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int n = wid * 64 + j * 16 + r16;
+            if (n >= nref)
+                continue;
+            const long long s4   = ((long long)acc_x[j][e] << 8) + (long long)acc_ll[j][e];
+            const float     prod = sq * ref_sq[g.ref_item0 + tl.ref0 + n];
+            const float     d    = sqrtf(prod);
+            out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
+        }
+    }
+}
+
+// Certified filter for the exact fine-tune: the reference's chunk-tree FP32
+// sum differs from the exact one by at most gamma_{step+6} * sum|p| (chunk
+// sums of `step` exact products, then a 7-level tree), so every per-label
+// quantile of the reference lies within `wp` of the exact one.  When the
+// maximum and the minimum are unique and every threshold decision is clear,
+// the new list is [max label] + [kept labels in list order]; otherwise (or
+// with a NaN) status = 1 and the cell is redone in the reference's FP32 order.
+__device__ inline float ft_next_up(float x)
+{
+    if (x == 0.f)
+        return __uint_as_float(1u);
+    const unsigned int b = __float_as_uint(x);
+    return __uint_as_float(x > 0.f ? b + 1u : b - 1u);
+}
+
+__global__ void k_ft_filter_exact(const FtCell* __restrict__ cells, uint32_t ncells,
+                                  const FtGroup* __restrict__ groups,
+                                  const uint32_t* __restrict__ labels, const float* __restrict__ P,
+                                  const uint8_t* __restrict__ in_lists,
+                                  const uint32_t* __restrict__ in_counts, uint32_t L, float thre,
+                                  uint8_t* __restrict__ out_lists, uint32_t* __restrict__ out_counts,
+                                  uint8_t* __restrict__ status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncells)
+        return;
+    const FtCell    fc = cells[i];
+    const FtGroup   g  = groups[fc.group];
+    const uint8_t*  in = in_lists + (size_t)i * L;
+    const uint32_t  k  = in_counts[i];
+    const uint32_t* gl = labels + g.label_off;
+    auto score_of = [&](uint32_t pos) {
+        const uint32_t lab = in[pos];
+        uint32_t       j   = 0;
+        while (j < g.nlabel && gl[j] != lab)
+            ++j;
+        return j < g.nlabel ? P[fc.seg0 + j] : __builtin_nanf("");
+    };
+    const double u     = 5.9604644775390625e-08;
+    const int    step  = ((int)g.gs + 127) / 128;
+    const double n     = (double)(step + 6);
+    const double gamma = n * u / (1.0 - n * u);
+    const double wp    = (gamma / (1.0 - gamma - 2 * u) + 3 * u) * (1.0 + 1e-6) + 16 * u;
+    double       mlo = -1e300, mhi = -1e300, nlo = 1e300, nhi = 1e300;
+    for (uint32_t t = 0; t < k; ++t)
+    {
+        const float v = score_of(t);
+        if (__builtin_isnan(v))
+        {
+            status[i] = 1;
+            return;
+        }
+        mlo = fmax(mlo, (double)v - wp);
+        mhi = fmax(mhi, (double)v + wp);
+        nlo = fmin(nlo, (double)v - wp);
+        nhi = fmin(nhi, (double)v + wp);
+    }
+    int nmax = 0, nmin = 0, imax = -1, imin = -1;
+    for (uint32_t t = 0; t < k; ++t)
+    {
+        const double v = (double)score_of(t);
+        if (v + wp >= mlo)
+        {
+            ++nmax;
+            imax = (int)t;
+        }
+        if (v - wp <= nhi)
+        {
+            ++nmin;
+            imin = (int)t;
+        }
+    }
+    if (nmax != 1 || nmin != 1 || imax == imin)
+    {
+        status[i] = 1;
+        return;
+    }
+    // float bounds of the reference's maximum, then of max - thre
+    float flo = (float)mlo;
+    if ((double)flo > mlo)
+        flo = -ft_next_up(-flo);
+    float fhi = (float)mhi;
+    if ((double)fhi < mhi)
+        fhi = ft_next_up(fhi);
+    const double tlo = (double)(flo - thre);
+    const double thi = (double)(fhi - thre);
+    uint8_t*     out = out_lists + (size_t)i * L;
+    uint32_t     cnt = 0;
+    out[cnt++]       = in[imax];
+    for (uint32_t t = 0; t < k; ++t)
+    {
+        if ((int)t == imax || (int)t == imin)
+            continue;
+        const double v = (double)score_of(t);
+        if (v - wp >= thi)
+            out[cnt++] = in[t];
+        else if (!(v + wp < tlo))
+        {
+            status[i] = 1;
+            return;
+        }
+    }
+    out_counts[i] = cnt;
+    status[i]     = 0;
 }
 
 }  // namespace csg

@@ -23,9 +23,9 @@ PATHS = {"plain": 0, "exact_int8": 1, "fp32_mfma": 2, "fp16": 3}
 PHASES = ["rank", "corr", "select", "label", "recheck", "finetune", "ft_corr", "ft_rank",
           "ft_select"]
 COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells", "nan_segments",
-            "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes"]
+            "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes", "ft_redone_cells"]
 N_PHASES = 9
-N_COUNTERS = 11
+N_COUNTERS = 12
 
 _V = C.c_void_p
 _U32 = C.c_uint32
@@ -69,6 +69,7 @@ def lib():
                              _U64, _V]
     L.csg_finetune.argtypes = [_V, _F, _F, C.c_int, _V, _V]
     L.csg_destroy.argtypes = [_V]
+    L.csg_set_finetune_path.argtypes = [_V, C.c_int]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
     L.csg_selftest.argtypes = [_V, C.c_int, P(C.c_double)]
@@ -231,6 +232,9 @@ class Engine:
             _p(c.ctidx), _p(c.ctdidx), _p(c.ctdiff), len(c.ctdiff), c.max_uniq_gene,
             _p(m) if m is not None else None), "copyin")
         self.fNq = c.qry_height
+
+    def set_finetune_path(self, path: str):
+        self._check(lib().csg_set_finetune_path(self._h, PATHS[path]), "set_finetune_path")
 
     def finetune(self, quantile: float = 0.8, thre: float = 0.05, times: int = 0):
         final = np.zeros(self.fNq, np.uint32)

@@ -102,6 +102,14 @@ int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint1
 int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
                  uint32_t* final_labels, int32_t* rounds);
 
+/* Correlation form of the fine-tune step: CSG_PATH_EXACT_INT8 contracts the
+ * ranks exactly on INT8-limb MFMA and certifies every cell's filter decisions
+ * against the reference's FP32 chunk-tree order, redoing the cells it cannot
+ * certify in that order (same results as the plain form); any other value
+ * uses the plain FP32 chunk-tree kernels.  csg_get_label sets it to the path
+ * it ran; this call overrides that.  No counterpart in the reference. */
+int csg_set_finetune_path(csg_engine* eng, int path);
+
 /* destroy (src/cusingler.cuh:25): frees the fine-tune buffers. */
 int csg_destroy(csg_engine* eng);
 
@@ -120,8 +128,9 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
  * [4] cells the exact path could not certify, [5] NaN-carrying segments,
  * [6] fine-tune rounds, [7] fine-tune (cell, round) items,
  * [8] fine-tune candidate-set groups, [9] fine-tune correlation pairs,
- * [10] fine-tune sum over items of gene-subset size * reference rows. */
-#define CSG_N_COUNTERS 11
+ * [10] fine-tune sum over items of gene-subset size * reference rows,
+ * [11] fine-tune cells the exact form redid in FP32 order. */
+#define CSG_N_COUNTERS 12
 int csg_counters(csg_engine* eng, int64_t* out);
 
 /* Diagnostic of the contraction kernels on seeded random ranks (|X| <= 1500,
