@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -36,6 +37,13 @@ struct PhaseEvents
 };
 
 size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+int64_t now_us()
+{
+    return std::chrono::duration_cast<std::chrono::microseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
 
 // fine-tune correlation tile: 32 cells x 32 reference rows, 2 x 2 per thread
 constexpr int FT_TC = 32, FT_TR = 32, FT_RC = 2, FT_RR = 2;
@@ -120,6 +128,7 @@ struct csg_engine
     uint8_t * d_ft_redo = nullptr, *h_ft_redo = nullptr;
     size_t    d_ft_redo_cap = 0, h_ft_redo_cap = 0;
     bool      ft_exact = false;  // exact (certified) fine-tune contraction
+    std::unordered_map<uint64_t, std::vector<uint64_t>> window_cache;  // pair gene windows
 
     // ---- timing / counters ----
     double  phase_ms[CSG_N_PHASES]  = {};
@@ -487,6 +496,7 @@ int csg_destroy(csg_engine* eng)
     eng->h_ft_stage_cap = eng->h_ft_out_cap = 0;
     eng->fpct_quantile = -1.f;
     eng->ft_loaded     = false;
+    eng->window_cache.clear();
     eng->host_mask.clear();
     return CSG_OK;
 }
@@ -1351,6 +1361,30 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
     return CSG_OK;
 }
 
+// Bitset over the fine-tune genes of the first min(len, cut) genes of the gene
+// window of ordered label pair (i, j); built on first use and kept until the
+// fine-tune inputs change.
+const uint64_t* window_bits(csg_engine* eng, uint32_t i, uint32_t j, int cut)
+{
+    const uint64_t key = ((uint64_t)i << 40) | ((uint64_t)j << 24) | (uint64_t)(uint32_t)cut;
+    auto           it  = eng->window_cache.find(key);
+    if (it != eng->window_cache.end())
+        return it->second.data();
+    const size_t          words = (std::max<uint32_t>(eng->Gf, 1) + 63) / 64;
+    std::vector<uint64_t> bits(words, 0ull);
+    const uint32_t        L   = eng->fL;
+    const uint32_t        pos = eng->ctdidx[((size_t)i * L + j) * 2];
+    int                   len = (int)eng->ctdidx[((size_t)i * L + j) * 2 + 1];
+    if (len > cut)
+        len = cut;
+    for (int t = 0; t < len; ++t)
+    {
+        const uint32_t g = eng->ctdiff[pos + t];
+        bits[g / 64] |= 1ull << (g % 64);
+    }
+    return eng->window_cache.emplace(key, std::move(bits)).first->second.data();
+}
+
 // Mask rows packed into 64-bit words (bit l of word l/64), so the host reads
 // Nq * ceil(L/64) words instead of Nq * L floats.
 __global__ void k_pack_mask(const float* __restrict__ mask, uint32_t nq, uint32_t L,
@@ -1409,7 +1443,8 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         return rc;
     for (int i = 6; i < CSG_N_COUNTERS; ++i)
         eng->counters[i] = 0;
-    PhaseScope total(eng, 5, 0);
+    PhaseScope    total(eng, 5, 0);
+    const int64_t t_start = now_us();
     // candidate lists from the mask, ascending label order (src/cusingler.cu:1307-1312)
     std::vector<uint8_t>  lists((size_t)Nq * L);
     std::vector<uint32_t> counts(Nq, 0);
@@ -1453,8 +1488,9 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
             }
     }
 
-    std::vector<uint32_t> mark(std::max<uint32_t>(eng->Gf, 1), 0);
-    uint32_t              stamp        = 0;
+    const size_t          gwords = (std::max<uint32_t>(eng->Gf, 1) + 63) / 64;
+    std::vector<uint64_t> gbits(gwords);
+    std::vector<uint32_t> gsub;
     const uint64_t        budget_elems = (uint64_t)1 << 28;  // per batch (scores + ranks)
     FtBatch               b;
     std::vector<uint32_t> active;
@@ -1463,6 +1499,8 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         if (counts[c] > 1 && (finetune_times == 0 || nrounds[c] < finetune_times))
             active.push_back(c);
 
+    int64_t t_mark = now_us();
+    eng->counters[12] += t_mark - t_start;  // mask -> candidate lists
     std::unordered_map<SetKey, uint32_t, SetKeyHash> gid;
     std::vector<std::vector<uint32_t>>              gcells;
     std::vector<SetKey>                             gkey;
@@ -1491,12 +1529,21 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
             gcells[it->second].push_back(c);
         }
         eng->counters[6]++;
+        {
+            const int64_t t = now_us();
+            eng->counters[13] += t - t_mark;  // grouping
+            t_mark = t;
+        }
         b.clear();
         auto flush = [&]() -> int {
             if (b.groups.empty())
                 return CSG_OK;
+            const int64_t t0 = now_us();
+            eng->counters[14] += t0 - t_mark;  // table building
             int r = run_ft_batch(eng, b, quantile, finetune_thre, lists, counts);
             b.clear();
+            t_mark = now_us();
+            eng->counters[15] += t_mark - t0;  // batch upload + kernels + download
             return r;
         };
         for (size_t gi = 0; gi < gcells.size(); ++gi)
@@ -1507,29 +1554,29 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
                     labs.push_back(l);
             const int k = (int)labs.size();
             // G_sub: union of each ordered pair's first min(len, gene_thre)
-            // genes (src/cusingler.cu:1140-1156)
+            // genes (src/cusingler.cu:1140-1156), as a union of cached window
+            // bitsets; scanning the bits yields the genes in ascending order
             const int gene_thre = (int)std::round(500 * std::pow((2 / 3.0), std::log2((double)k)));
-            ++stamp;
-            std::vector<uint32_t> gsub;
+            std::fill(gbits.begin(), gbits.end(), 0ull);
             for (uint32_t i : labs)
                 for (uint32_t j : labs)
                 {
                     if (i == j)
                         continue;
-                    const uint32_t pos = eng->ctdidx[((size_t)i * L + j) * 2];
-                    int            len = (int)eng->ctdidx[((size_t)i * L + j) * 2 + 1];
-                    if (len > gene_thre)
-                        len = gene_thre;
-                    for (int t = 0; t < len; ++t)
-                    {
-                        const uint32_t gg = eng->ctdiff[pos + t];
-                        if (mark[gg] != stamp)
-                        {
-                            mark[gg] = stamp;
-                            gsub.push_back(gg);
-                        }
-                    }
+                    const uint64_t* w = window_bits(eng, i, j, gene_thre);
+                    for (size_t t = 0; t < gwords; ++t)
+                        gbits[t] |= w[t];
                 }
+            gsub.clear();
+            for (size_t t = 0; t < gwords; ++t)
+            {
+                uint64_t v = gbits[t];
+                while (v)
+                {
+                    gsub.push_back((uint32_t)(t * 64 + __builtin_ctzll(v)));
+                    v &= v - 1;
+                }
+            }
             auto& cells = gcells[gi];
             eng->counters[7] += cells.size();
             eng->counters[8] += 1;
@@ -1542,7 +1589,6 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
                     counts[c] = 1;
                 continue;
             }
-            std::sort(gsub.begin(), gsub.end());
             uint32_t nref = 0;
             for (uint32_t l : labs)
                 nref += eng->fctidx[2 * l + 1];
@@ -1616,6 +1662,9 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         }
         if ((rc = flush()))
             return rc;
+        const int64_t t_keep = now_us();
+        eng->counters[14] += t_keep - t_mark;
+        t_mark = t_keep;
         size_t keep = 0;
         for (uint32_t c : active)
             if (counts[c] > 1 && (finetune_times == 0 || nrounds[c] < finetune_times))

@@ -121,7 +121,8 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
 // FP32 scores of one query row against every reference row of one label in
 // the reference's sequential order (the plain contraction restricted to a
 // (cell, label) pair).  One wave per item; 64 reference rows at a time, one
-// per lane, K staged through LDS in 64-wide tiles.
+// per lane; each 64 x 64 tile of ranks is staged with 16-byte loads into LDS
+// rows of 33 words (an odd pitch, so the lanes' column reads do not collide).
 __global__ __launch_bounds__(64) void k_rescore(const RecheckItem* __restrict__ items,
                                                 const uint64_t* __restrict__ out_off,
                                                 const int16_t* __restrict__ Xq,
@@ -132,7 +133,8 @@ __global__ __launch_bounds__(64) void k_rescore(const RecheckItem* __restrict__ 
                                                 const uint32_t* __restrict__ llen,
                                                 float* __restrict__ out)
 {
-    __shared__ int16_t tile[64][66];
+    __shared__ int     tile[64 * 33];
+    __shared__ int16_t xs[64];
     const RecheckItem it   = items[blockIdx.x];
     const int         lane = threadIdx.x;
     const uint32_t    r0   = lstart[it.l];
@@ -140,24 +142,35 @@ __global__ __launch_bounds__(64) void k_rescore(const RecheckItem* __restrict__ 
     const int16_t*    xq   = Xq + (size_t)it.q * Kp;
     const float       sq   = sq_q[it.q];
     float*            o    = out + out_off[blockIdx.x];
+    const int16_t*    t16  = (const int16_t*)tile;
     for (uint32_t base = 0; base < n; base += 64)
     {
         const uint32_t nrow = min(64u, n - base);
         float          acc  = 0.f;
         for (int k0 = 0; k0 < Kp; k0 += 64)
         {
-            // stage tile[row][k] for 64 rows x 64 k, each lane one row chunk
-            for (uint32_t rr = 0; rr < nrow; ++rr)
-                tile[rr][lane] = Xr[(size_t)(r0 + base + rr) * Kp + k0 + lane];
-            __builtin_amdgcn_wave_barrier();
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+            {
+                const int c = lane + 64 * t, row = c >> 3, ch = c & 7;
+                int4      v = int4{ 0, 0, 0, 0 };
+                if ((uint32_t)row < nrow)
+                    v = *(const int4*)(Xr + (size_t)(r0 + base + row) * Kp + k0 + ch * 8);
+                int* d = tile + row * 33 + ch * 4;
+                d[0]   = v.x;
+                d[1]   = v.y;
+                d[2]   = v.z;
+                d[3]   = v.w;
+            }
+            xs[lane] = xq[k0 + lane];
             __syncthreads();
             if ((uint32_t)lane < nrow)
             {
-#pragma unroll 8
+#pragma unroll 16
                 for (int k = 0; k < 64; ++k)
-                    acc = __builtin_fmaf((float)xq[k0 + k], (float)tile[lane][k], acc);
+                    acc = __builtin_fmaf((float)xs[k], (float)t16[lane * 66 + k], acc);
             }
-            __syncthreads();
         }
         if ((uint32_t)lane < nrow)
             o[base + lane] = corr_from_sums(acc, sq, sq_r[r0 + base + lane]);

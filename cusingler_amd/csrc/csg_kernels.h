@@ -329,8 +329,9 @@ struct ScoreSegments
     uint64_t        ldS;
     __device__ Segment get(uint64_t item) const
     {
-        const uint32_t q = (uint32_t)(item / L);
-        const uint32_t l = (uint32_t)(item % L);
+        const uint32_t it = (uint32_t)item;  // a chunk holds fewer than 2^32 items
+        const uint32_t q  = it / L;
+        const uint32_t l  = it - q * L;
         return Segment{ (uint64_t)q * ldS + lstart[l], llen[l], l };
     }
 };
@@ -538,6 +539,142 @@ __device__ inline void wave_select2(const float* __restrict__ v, uint32_t n, uin
     *out1 = r1;
 }
 
+// Register-resident variant for segments of at most 64 * R values: every
+// lane holds its values in registers (slot u is element lane + 64 u), the
+// range [lo, hi] is cut into 64 buckets (one per lane, monotone in value),
+// the two target ranks are located by a prefix over the bucket counts, and the
+// values of the buckets that hold them are ranked exactly in LDS.  While those
+// buckets hold more than 64 values the range narrows to them and repeats; a
+// last level ranks the candidates against the whole segment.
+template <int R>
+__device__ inline void wave_select2_reg(const float (&x)[R], const float* __restrict__ v,
+                                        uint32_t n, float lo, float hi, uint32_t k0, uint32_t k1,
+                                        WaveSelectSmem& sm, float* out0, float* out1)
+{
+    const int lane  = threadIdx.x & (kWave - 1);
+    uint32_t  below = 0;
+    float    r0 = lo, r1 = lo;
+    for (int level = 0;; ++level)
+    {
+        if (lo == hi)
+        {
+            r0 = r1 = lo;
+            break;
+        }
+        const double dscale = 64.0 / ((double)hi - (double)lo);
+        sm.hist[lane]       = 0;
+        __builtin_amdgcn_wave_barrier();
+        int b[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+        {
+            const uint32_t i = lane + 64 * u;
+            b[u]             = -1;
+            if (i < n && x[u] >= lo && x[u] <= hi)
+            {
+                int t = (int)(((double)x[u] - (double)lo) * dscale);
+                b[u]  = t < 0 ? 0 : (t > 63 ? 63 : t);
+                atomicAdd(&sm.hist[b[u]], 1);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int c    = sm.hist[lane];
+        const int incl = wave_incl_scan(c);
+        const int ex   = incl - c + (int)below;
+        const unsigned long long m0 = __ballot((uint32_t)ex <= k0 && k0 < (uint32_t)(ex + c));
+        const unsigned long long m1 = __ballot((uint32_t)ex <= k1 && k1 < (uint32_t)(ex + c));
+        const int blo = m0 ? __builtin_ctzll(m0) : 0;
+        const int bhi = m1 ? __builtin_ctzll(m1) : 63;
+        const int cb  = __shfl(ex, blo, kWave) - (int)below;  // values below bucket blo
+        const int cc  = __shfl(incl, bhi, kWave) - __shfl(incl - c, blo, kWave);
+        const bool last = cc <= kWave || level >= kMaxLevel;
+        if (last)
+        {
+            bool  hit0 = false, hit1 = false;
+            float v0 = 0.f, v1 = 0.f;
+            if (cc <= kWave)
+            {
+                if (lane == 0)
+                    sm.ncand = 0;
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int u = 0; u < R; ++u)
+                    if (b[u] >= blo && b[u] <= bhi)
+                        sm.cand[atomicAdd(&sm.ncand, 1)] = x[u];
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t base = below + (uint32_t)cb;
+                if (lane < cc)
+                {
+                    const float xv   = sm.cand[lane];
+                    uint32_t    less = 0, eq = 0;
+                    for (int j = 0; j < cc; ++j)
+                    {
+                        const float y = sm.cand[j];
+                        less += y < xv;
+                        eq += y == xv;
+                    }
+                    const uint32_t p0 = base + less, p1 = p0 + eq;
+                    if (p0 <= k0 && k0 < p1)
+                    {
+                        hit0 = true;
+                        v0   = xv;
+                    }
+                    if (p0 <= k1 && k1 < p1)
+                    {
+                        hit1 = true;
+                        v1   = xv;
+                    }
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int u = 0; u < R; ++u)
+                {
+                    if (!(b[u] >= blo && b[u] <= bhi))
+                        continue;
+                    uint32_t less = 0, eq = 0;
+                    for (uint32_t j = 0; j < n; ++j)
+                    {
+                        const float y = v[j];
+                        less += y < x[u];
+                        eq += y == x[u];
+                    }
+                    if (less <= k0 && k0 < less + eq)
+                    {
+                        hit0 = true;
+                        v0   = x[u];
+                    }
+                    if (less <= k1 && k1 < less + eq)
+                    {
+                        hit1 = true;
+                        v1   = x[u];
+                    }
+                }
+            }
+            const unsigned long long f0 = __ballot(hit0);
+            const unsigned long long f1 = __ballot(hit1);
+            r0 = __shfl(v0, f0 ? __builtin_ctzll(f0) : 0, kWave);
+            r1 = __shfl(v1, f1 ? __builtin_ctzll(f1) : 0, kWave);
+            break;
+        }
+        float nlo = __builtin_inff(), nhi = -__builtin_inff();
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+            if (b[u] >= blo && b[u] <= bhi)
+            {
+                nlo = fminf(nlo, x[u]);
+                nhi = fmaxf(nhi, x[u]);
+            }
+        below += (uint32_t)cb;
+        lo = wave_min(nlo);
+        hi = wave_max(nhi);
+        __builtin_amdgcn_wave_barrier();
+    }
+    *out0 = r0;
+    *out1 = r1;
+}
+
 // Reference percentile of a NaN-free segment from its order statistics.
 __device__ inline float pct_from_stats(const PctParam& pp, float p, float a_km1, float a_k,
                                        float vmin, float vmax)
@@ -574,6 +711,44 @@ __global__ __launch_bounds__(256) void k_select(Src src, uint64_t n_items,
         // len <= 1: the reference returns arr.front() unsorted
         if (lane == 0)
             out[item] = sg.n > 0 ? v[0] : __builtin_nanf("");
+        return;
+    }
+    if (sg.n <= 1024)
+    {
+        // the segment stays in registers for every pass
+        float x[16];
+        bool  nan = false;
+        float lo = __builtin_inff(), hi = -__builtin_inff();
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+        {
+            const uint32_t i = lane + 64 * u;
+            x[u]             = i < sg.n ? v[i] : 0.f;
+            if (i < sg.n)
+            {
+                nan |= __builtin_isnan(x[u]);
+                lo = fminf(lo, x[u]);
+                hi = fmaxf(hi, x[u]);
+            }
+        }
+        if (__ballot(nan))
+        {
+            if (lane == 0)
+            {
+                const unsigned int s = atomicAdd(nan_count, 1u);
+                nan_list[s]          = item;
+                out[item]            = __builtin_nanf("");
+            }
+            return;
+        }
+        lo = wave_min(lo);
+        hi = wave_max(hi);
+        float a0 = 0.f, a1 = 0.f;
+        if (pp.kind == 3)
+            wave_select2_reg<16>(x, v, sg.n, lo, hi, (uint32_t)(pp.k - 1), (uint32_t)pp.k, sm[wid],
+                                 &a0, &a1);
+        if (lane == 0)
+            out[item] = pct_from_stats(pp, p, a0, a1, lo, hi);
         return;
     }
     bool nan = false;

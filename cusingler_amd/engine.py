@@ -23,9 +23,10 @@ PATHS = {"plain": 0, "exact_int8": 1, "fp32_mfma": 2, "fp16": 3}
 PHASES = ["rank", "corr", "select", "label", "recheck", "finetune", "ft_corr", "ft_rank",
           "ft_select"]
 COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells", "nan_segments",
-            "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes", "ft_redone_cells"]
+            "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes", "ft_redone_cells",
+            "ft_host_lists_us", "ft_host_group_us", "ft_host_build_us", "ft_batch_us"]
 N_PHASES = 9
-N_COUNTERS = 12
+N_COUNTERS = 16
 
 _V = C.c_void_p
 _U32 = C.c_uint32

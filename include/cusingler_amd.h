@@ -129,8 +129,10 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
  * [6] fine-tune rounds, [7] fine-tune (cell, round) items,
  * [8] fine-tune candidate-set groups, [9] fine-tune correlation pairs,
  * [10] fine-tune sum over items of gene-subset size * reference rows,
- * [11] fine-tune cells the exact form redid in FP32 order. */
-#define CSG_N_COUNTERS 12
+ * [11] fine-tune cells the exact form redid in FP32 order,
+ * [12..15] fine-tune host microseconds: candidate lists from the mask,
+ * grouping by candidate set, work-table building, batch round trips. */
+#define CSG_N_COUNTERS 16
 int csg_counters(csg_engine* eng, int64_t* out);
 
 /* Diagnostic of the contraction kernels on seeded random ranks (|X| <= 1500,
