@@ -152,6 +152,19 @@ def main():
                 "ops_per_launch": "2 * query_rows_in_chunk * reference_cells * G_s"}
     if chosen == "exact_int8":
         roofline["limb_mfma_utilisation"] = round(4 * achieved / peak, 5)
+    # HBM bytes per launch from the committed counter runs (FETCH_SIZE and
+    # WRITE_SIZE, scaled by a copy kernel of known size), when they cover this
+    # workload and kernel
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            tr = json.load(f)
+        workload = (f"{args.config}: {Nq} query x {Nr} reference cells, {len(ds.codes)} labels, "
+                    f"G_s={Gs}, G_f={c1.width}")
+        if tr.get("kernel") == kname and tr.get("workload") == workload:
+            roofline["traffic"] = round(tr["traffic_bytes_per_launch"])
+            roofline["traffic_source"] = tr.get("source")
+    except (OSError, ValueError, KeyError):
+        pass
     plain = res.get("plain", r)
     match = {"first": float((r["first"] == plain["first"]).mean()),
              "final": float((r["final"] == plain["final"]).mean())}
