@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -24,6 +25,8 @@
 #include "csg_ft_kernels.h"
 #include "csg_kernels.h"
 #include "csg_mfma.h"
+#include "csg_mlp.h"
+#include "csg_proposal.h"
 
 using namespace csg;
 
@@ -130,6 +133,36 @@ struct csg_engine
     bool      ft_exact = false;  // exact (certified) fine-tune contraction
     std::unordered_map<uint64_t, std::vector<uint64_t>> window_cache;  // pair gene windows
 
+    // ---- learned proposal ----
+    // Network G_s -> H -> H -> L (GELU), parameters in one FP32 array
+    // [W1 (Kp x H) | b1 (H) | W2 (H x H) | b2 (H) | W3 (H x Lp) | b3 (Lp)],
+    // row-major, input rows >= G_s and output columns >= L inert (zero).
+    bool     mlp_ready = false;
+    uint32_t mG = 0, mKp = 0, mH = 0, mL = 0, mLp = 0;
+    size_t   m_off[6] = {}, m_n = 0;
+    float *  d_mw = nullptr, *d_mg = nullptr, *d_mm = nullptr, *d_mv = nullptr;
+    __bf16*  d_mwb = nullptr;  // BF16 copy of every parameter
+    int64_t  m_step = 0;       // Adam steps taken
+    // activations (grow-only, in rows)
+    size_t   m_rows_cap = 0, m_train_cap = 0;
+    __bf16 * d_mx = nullptr, *d_mh1 = nullptr, *d_mh2 = nullptr, *d_mdlb = nullptr, *d_mdzb = nullptr;
+    float *  d_macc = nullptr, *d_mz1 = nullptr, *d_mz2 = nullptr, *d_mdl = nullptr, *d_mdh = nullptr,
+          *d_mdz = nullptr, *d_mloss = nullptr;
+    int32_t* d_my = nullptr;
+    // proposal work lists (allocated with the exact path's buffers)
+    int        prop_k = 0, prop_kp = 0;  // proposal size, labels ranked per cell
+    float      prop_margin = 0.2f;
+    int32_t*   d_ptop = nullptr;
+    float*     d_pmargin = nullptr;
+    uint32_t * d_pcnt = nullptr, *d_pslot = nullptr, *d_pfb = nullptr, *d_pcells = nullptr;
+    uint32_t*  d_plist = nullptr;  // cells the edge tests list (two lists of Qc)
+    uint32_t * d_pabase = nullptr, *d_pacnt = nullptr;
+    uint64_t*  d_pcoff = nullptr;
+    RecheckItem* d_pitems = nullptr;
+    Segment*   d_pseg = nullptr;
+    GroupTile* d_ptiles = nullptr;
+    size_t     d_ptiles_cap = 0;
+
     // ---- timing / counters ----
     double  phase_ms[CSG_N_PHASES]  = {};
     int64_t phase_cnt[CSG_N_PHASES] = {};
@@ -193,6 +226,20 @@ template <class T> int upload(csg_engine* eng, T** p, const T* src, size_t count
         HIPCHK(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, eng->stream));
     return CSG_OK;
 }
+
+// Grow-only device allocation.
+template <class T> int ensure(csg_engine* eng, T** p, size_t* cap, size_t n)
+{
+    if (n <= *cap && *p)
+        return CSG_OK;
+    const size_t want = std::max(n, *cap + *cap / 2);
+    int          rc   = dalloc(eng, p, want);
+    *cap              = rc ? 0 : want;
+    return rc;
+}
+
+void free_proposal(csg_engine* eng);
+void free_mlp(csg_engine* eng);
 
 hipEvent_t take_event(csg_engine* eng)
 {
@@ -450,6 +497,7 @@ int csg_destroy_score(csg_engine* eng)
     dfree(eng->d_Pref);
     dfree(eng->d_item_off);
     dfree(eng->d_item_seg);
+    free_proposal(eng);
     eng->ref_prepared_path = -1;
     eng->pct_quantile = -1.f;
     eng->score_loaded = false;
@@ -507,6 +555,7 @@ void csg_release(csg_engine* eng)
         return;
     csg_destroy_score(eng);
     csg_destroy(eng);
+    free_mlp(eng);
     dfree(eng->d_P);
     dfree(eng->d_first);
     dfree(eng->d_mask);
@@ -752,6 +801,412 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
     return CSG_OK;
 }
 
+
+// ------------------------------------------------------- learned proposal
+
+unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+extern "C++" {
+template <bool TA, bool TB>
+int bgemm(csg_engine* eng, const __bf16* A, int lda, const __bf16* B, int ldb, float* C, int ldc,
+          int M, int N, int K)
+{
+    const dim3 grid((N + 63) / 64, (M + 63) / 64);
+    k_bgemm<TA, TB><<<grid, 256, 0, eng->stream>>>(A, lda, B, ldb, C, ldc, M, N, K, 0.f);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+}
+
+// Activation buffers for `rows` rows; with `train` also those of the
+// backward pass.
+int mlp_work(csg_engine* eng, size_t rows, bool train)
+{
+    int          rc;
+    const size_t H = eng->mH, Kp = eng->mKp, Lp = eng->mLp, W = std::max(H, Lp);
+    if (rows > eng->m_rows_cap || !eng->d_mx)
+    {
+        eng->m_rows_cap = 0;
+        if ((rc = dalloc(eng, &eng->d_mx, rows * Kp)) || (rc = dalloc(eng, &eng->d_macc, rows * W)) ||
+            (rc = dalloc(eng, &eng->d_mh1, rows * H)) || (rc = dalloc(eng, &eng->d_mh2, rows * H)))
+            return rc;
+        eng->m_rows_cap = rows;
+    }
+    if (train && (rows > eng->m_train_cap || !eng->d_mz1))
+    {
+        eng->m_train_cap = 0;
+        if ((rc = dalloc(eng, &eng->d_mz1, rows * H)) || (rc = dalloc(eng, &eng->d_mz2, rows * H)) ||
+            (rc = dalloc(eng, &eng->d_mdl, rows * Lp)) || (rc = dalloc(eng, &eng->d_mdlb, rows * Lp)) ||
+            (rc = dalloc(eng, &eng->d_mdh, rows * H)) || (rc = dalloc(eng, &eng->d_mdz, rows * H)) ||
+            (rc = dalloc(eng, &eng->d_mdzb, rows * H)) || (rc = dalloc(eng, &eng->d_my, rows)) ||
+            (rc = dalloc(eng, &eng->d_mloss, 1)))
+            return rc;
+        eng->m_train_cap = rows;
+    }
+    return CSG_OK;
+}
+
+// Forward pass over the BF16 rows in d_mx: logits without the output bias
+// land in d_macc [rows x Lp]; the pre-activations z1, z2 are kept for the
+// backward pass when `keep`.
+int mlp_forward(csg_engine* eng, int rows, bool keep)
+{
+    const int     H = (int)eng->mH, Kp = (int)eng->mKp, Lp = (int)eng->mLp;
+    const __bf16* wb = eng->d_mwb;
+    const float*  w  = eng->d_mw;
+    int           rc;
+    if ((rc = bgemm<false, false>(eng, eng->d_mx, Kp, wb + eng->m_off[0], H, eng->d_macc, H, rows, H,
+                                  Kp)))
+        return rc;
+    k_bias_gelu<<<nblk((size_t)rows * H), 256, 0, eng->stream>>>(
+        eng->d_macc, w + eng->m_off[1], rows, H, keep ? eng->d_mz1 : nullptr, eng->d_mh1);
+    LAUNCHCHK();
+    if ((rc = bgemm<false, false>(eng, eng->d_mh1, H, wb + eng->m_off[2], H, eng->d_macc, H, rows, H,
+                                  H)))
+        return rc;
+    k_bias_gelu<<<nblk((size_t)rows * H), 256, 0, eng->stream>>>(
+        eng->d_macc, w + eng->m_off[3], rows, H, keep ? eng->d_mz2 : nullptr, eng->d_mh2);
+    LAUNCHCHK();
+    return bgemm<false, false>(eng, eng->d_mh2, H, wb + eng->m_off[4], Lp, eng->d_macc, Lp, rows, Lp,
+                               H);
+}
+
+// Gradients of the mean cross-entropy into d_mg, from the output gradient
+// the softmax kernel left in d_mdl / d_mdlb and the forward pass's state.
+int mlp_backward(csg_engine* eng, int rows)
+{
+    const int     H = (int)eng->mH, Kp = (int)eng->mKp, Lp = (int)eng->mLp;
+    const __bf16* wb = eng->d_mwb;
+    float*        g  = eng->d_mg;
+    const size_t  n  = (size_t)rows * H;
+    int           rc;
+    if ((rc = bgemm<true, false>(eng, eng->d_mh2, H, eng->d_mdlb, Lp, g + eng->m_off[4], Lp, H, Lp,
+                                 rows)))
+        return rc;
+    k_colsum<<<nblk(Lp), 256, 0, eng->stream>>>(eng->d_mdl, rows, Lp, g + eng->m_off[5]);
+    LAUNCHCHK();
+    if ((rc = bgemm<false, true>(eng, eng->d_mdlb, Lp, wb + eng->m_off[4], Lp, eng->d_mdh, H, rows,
+                                 H, Lp)))
+        return rc;
+    k_gelu_back<<<nblk(n), 256, 0, eng->stream>>>(eng->d_mdh, eng->d_mz2, n, eng->d_mdz, eng->d_mdzb);
+    LAUNCHCHK();
+    if ((rc = bgemm<true, false>(eng, eng->d_mh1, H, eng->d_mdzb, H, g + eng->m_off[2], H, H, H,
+                                 rows)))
+        return rc;
+    k_colsum<<<nblk(H), 256, 0, eng->stream>>>(eng->d_mdz, rows, H, g + eng->m_off[3]);
+    LAUNCHCHK();
+    if ((rc = bgemm<false, true>(eng, eng->d_mdzb, H, wb + eng->m_off[2], H, eng->d_mdh, H, rows, H,
+                                 H)))
+        return rc;
+    k_gelu_back<<<nblk(n), 256, 0, eng->stream>>>(eng->d_mdh, eng->d_mz1, n, eng->d_mdz, eng->d_mdzb);
+    LAUNCHCHK();
+    if ((rc = bgemm<true, false>(eng, eng->d_mx, Kp, eng->d_mdzb, H, g + eng->m_off[0], H, Kp, H,
+                                 rows)))
+        return rc;
+    k_colsum<<<nblk(H), 256, 0, eng->stream>>>(eng->d_mdz, rows, H, g + eng->m_off[1]);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// Loss and output gradient of the rows in d_mx with labels in d_my (the
+// batch mean, added to d_mloss).
+int mlp_loss(csg_engine* eng, int rows)
+{
+    k_softmax_xent<<<nblk(rows), 256, 0, eng->stream>>>(eng->d_macc, eng->d_mw + eng->m_off[5],
+                                                       eng->d_my, rows, (int)eng->mL,
+                                                       (int)eng->mLp, eng->d_mdl, eng->d_mdlb,
+                                                       eng->d_mloss);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+int mlp_adam(csg_engine* eng, float lr)
+{
+    eng->m_step++;
+    const float c1 = (float)(1.0 - std::pow(0.9, (double)eng->m_step));
+    const float c2 = (float)(1.0 - std::pow(0.999, (double)eng->m_step));
+    k_adam<<<nblk(eng->m_n), 256, 0, eng->stream>>>(eng->d_mw, eng->d_mg, eng->d_mm, eng->d_mv,
+                                                    eng->m_n, lr, 0.9f, 0.999f, 1e-8f, c1, c2,
+                                                    eng->d_mwb);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+int mlp_check(csg_engine* eng)
+{
+    if (!eng->mlp_ready)
+        return eng->fail(CSG_ERR_STATE, "no proposal network: call csg_mlp_init first");
+    if (!eng->score_loaded || eng->mG != eng->G || eng->mL != eng->L)
+        return eng->fail(CSG_ERR_STATE,
+                         "the proposal network's gene set / label count differ from the loaded data");
+    return CSG_OK;
+}
+
+// Top-K labels and softmax margin of the `nq` ranked query rows in d_Xq.
+int mlp_propose(csg_engine* eng, uint32_t nq, int K, int32_t* top, float* margin)
+{
+    int rc;
+    if ((rc = mlp_work(eng, nq, false)))
+        return rc;
+    const size_t n = (size_t)nq * eng->Kp;
+    k_rank_features<<<nblk(n), 256, 0, eng->stream>>>(eng->d_Xq, n, 1.f / (float)eng->G, eng->d_mx);
+    LAUNCHCHK();
+    if ((rc = mlp_forward(eng, (int)nq, false)))
+        return rc;
+    const float* b3 = eng->d_mw + eng->m_off[5];
+    const int    L = (int)eng->mL, Lp = (int)eng->mLp;
+    switch (K)
+    {
+#define CSG_TOPK(k)                                                                                \
+    case k:                                                                                        \
+        k_topk<k><<<nblk(nq), 256, 0, eng->stream>>>(eng->d_macc, b3, (int)nq, L, Lp, top, margin); \
+        break;
+        CSG_TOPK(1)
+        CSG_TOPK(2)
+        CSG_TOPK(3)
+        CSG_TOPK(4)
+        CSG_TOPK(5)
+        CSG_TOPK(6)
+        CSG_TOPK(7)
+        CSG_TOPK(8)
+#undef CSG_TOPK
+    default:
+        return eng->fail(CSG_ERR_ARGUMENT, "proposal size must be in [1, 8]");
+    }
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// One grouped scoring round of a chunk.  Label l's list holds cw[l] cells
+// scored on the window [t0, t1) of their ranked labels (`win` names them, or
+// NULL for every confident cell of the chunk with n_win = nq), followed by
+// the n_all cells of `all`, scored on every label (`base`: device copy of
+// cw, their slot offset).  Every (cell, label) item is contracted, its
+// quantile selected and stored in the chunk's quantile rows P.
+struct PropRound
+{
+    std::vector<uint32_t> cw;
+    const uint32_t*       win   = nullptr;
+    uint32_t              n_win = 0;
+    int                   t0 = 0, t1 = 0;
+    const uint32_t*       all   = nullptr;
+    uint32_t              n_all = 0;
+    const uint32_t*       base  = nullptr;
+};
+
+int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
+{
+    const uint32_t L = eng->L;
+    std::vector<uint32_t>  abase(L), acnt(L);
+    std::vector<uint64_t>  coff(L);
+    std::vector<GroupTile> tiles;
+    uint64_t               n_items = 0, n_scores = 0;
+    for (uint32_t l = 0; l < L; ++l)
+    {
+        const uint32_t na = r.cw[l] + r.n_all;
+        const uint32_t nl = eng->ctidx[2 * l + 1];
+        abase[l]          = (uint32_t)n_items;
+        acnt[l]           = na;
+        coff[l]           = n_scores;
+        n_items += na;
+        n_scores += (uint64_t)na * nl;
+        for (uint32_t m0 = 0; m0 < na; m0 += W8_BM)
+            for (uint32_t n0 = 0; n0 < nl; n0 += W8_BN)
+                tiles.push_back(GroupTile{ l, m0, n0, 0u });
+    }
+    if (n_items == 0)
+        return CSG_OK;
+    if (n_scores > (uint64_t)eng->Qc * eng->Nr || n_items > (uint64_t)eng->Qc * L)
+        return eng->fail(CSG_ERR_MEMORY, "proposal round exceeds the score buffer");
+    int rc;
+    if ((rc = ensure(eng, &eng->d_ptiles, &eng->d_ptiles_cap, std::max<size_t>(tiles.size(), 1))))
+        return rc;
+    HIPCHK(hipMemcpyAsync(eng->d_pabase, abase.data(), L * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          eng->stream));
+    HIPCHK(hipMemcpyAsync(eng->d_pacnt, acnt.data(), L * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          eng->stream));
+    HIPCHK(hipMemcpyAsync(eng->d_pcoff, coff.data(), L * sizeof(uint64_t), hipMemcpyHostToDevice,
+                          eng->stream));
+    if (!tiles.empty())
+        HIPCHK(hipMemcpyAsync(eng->d_ptiles, tiles.data(), tiles.size() * sizeof(GroupTile),
+                              hipMemcpyHostToDevice, eng->stream));
+    {
+        PhaseScope ps(eng, 9, 1);
+        if (r.t1 > r.t0 && r.n_win)
+        {
+            k_prop_scatter_win<<<nblk(r.n_win), 256, 0, eng->stream>>>(
+                eng->d_ptop, eng->d_pmargin, eng->prop_margin, r.win, r.n_win, eng->prop_kp, r.t0,
+                r.t1, L, eng->d_pslot, eng->d_pabase, eng->d_pcoff, eng->d_llen, eng->d_pcells,
+                eng->d_pitems, eng->d_pseg);
+            LAUNCHCHK();
+        }
+        if (r.n_all)
+        {
+            k_prop_scatter_all<<<nblk(r.n_all), 256, 0, eng->stream>>>(
+                r.all, r.n_all, L, r.base, eng->d_pabase, eng->d_pcoff, eng->d_llen, eng->d_pcells,
+                eng->d_pitems, eng->d_pseg);
+            LAUNCHCHK();
+        }
+    }
+    if (!tiles.empty())
+    {
+        PhaseScope ps(eng, 1, 1);
+        const uint32_t b_max = (uint32_t)round_up(eng->Nr, 256) - 1;
+        k_gemm_int8w_group<<<(unsigned)tiles.size(), 512, 0, eng->stream>>>(
+            eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart, eng->d_llen,
+            eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl, (int)eng->Kp, b_max,
+            eng->d_sqq_f, eng->d_sqr_f, eng->d_S);
+        LAUNCHCHK();
+    }
+    {
+        PhaseScope ps(eng, 2, 2);
+        HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
+        // NaN quantiles stay NaN: the certification re-scores those cells
+        k_select<TableSegments><<<(unsigned)((n_items + 3) / 4), 256, 0, eng->stream>>>(
+            TableSegments{ eng->d_pseg }, n_items, eng->d_S, eng->d_pct, quantile, eng->d_Pref,
+            eng->d_nan_list, eng->d_nan_count);
+        LAUNCHCHK();
+        k_prop_scatter_p<<<nblk(n_items), 256, 0, eng->stream>>>(eng->d_pitems, (uint32_t)n_items,
+                                                                eng->d_Pref, P, L);
+        LAUNCHCHK();
+    }
+    eng->counters[19] += (int64_t)n_items;
+    return CSG_OK;
+}
+
+int read_counts(csg_engine* eng, const uint32_t* d, std::vector<uint32_t>& h)
+{
+    HIPCHK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return CSG_OK;
+}
+
+// Proposal-restricted exact quantiles of one chunk (ranked, limbs split).
+// The network ranks KP = min(2K, 8, L) labels per cell.  Stage 1 scores the
+// confident cells on their first K labels and the rest on every label;
+// cells whose K-th label is at the mask's edge (k_prop_check) are scored on
+// their labels K .. KP-1 in stage 2, and those whose KP-th label is still at
+// the edge on every label in stage 3.  Unscored labels keep -inf, which the
+// certification treats as outside every mask.
+int propose_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, float thre)
+{
+    const uint32_t L  = eng->L;
+    const int      K  = eng->prop_k;
+    const int      KP = std::min<int>(std::min(2 * K, 8), (int)L);
+    eng->prop_kp      = KP;
+    const double   wp = certify_width(eng->G);
+    uint32_t*      cA = eng->d_pcnt;
+    uint32_t*      cB = eng->d_pcnt + (L + 1);
+    uint32_t*      cC = eng->d_pcnt + 2 * (L + 1);
+    int            rc;
+    {
+        PhaseScope ps(eng, 9, 5);
+        if ((rc = mlp_propose(eng, nq, KP, eng->d_ptop, eng->d_pmargin)))
+            return rc;
+        HIPCHK(hipMemsetAsync(eng->d_pcnt, 0, 3 * (L + 1) * sizeof(uint32_t), eng->stream));
+        k_prop_count<<<nblk(nq), 256, 0, eng->stream>>>(eng->d_ptop, eng->d_pmargin, nq, KP, K, L,
+                                                       eng->prop_margin, cA, eng->d_pslot,
+                                                       eng->d_pfb);
+        LAUNCHCHK();
+    }
+    std::vector<uint32_t> cnt(L + 1);
+    if ((rc = read_counts(eng, cA, cnt)))
+        return rc;
+    const uint32_t nfb = cnt[L];
+    float*         P   = eng->d_P + (size_t)q0 * L;
+    k_fill_f32<<<nblk((size_t)nq * L), 256, 0, eng->stream>>>(P, (size_t)nq * L, -__builtin_inff());
+    LAUNCHCHK();
+    PropRound r1;
+    r1.cw.assign(cnt.begin(), cnt.begin() + L);
+    r1.n_win = nq;
+    r1.t1    = K;
+    r1.all   = eng->d_pfb;
+    r1.n_all = nfb;
+    r1.base  = cA;
+    if ((rc = prop_round(eng, r1, quantile, P)))
+        return rc;
+    // stage 2: widen the edge cells to their next ranked labels
+    const bool widen = KP > K;
+    k_prop_check<<<nblk(nq), 256, 0, eng->stream>>>(P, nq, L, eng->d_ptop, eng->d_pmargin, KP,
+                                                   K - 1, eng->prop_margin, thre, wp, nullptr,
+                                                   widen ? 1 : 0, cB, eng->d_pslot, eng->d_plist,
+                                                   cB + L);
+    LAUNCHCHK();
+    if ((rc = read_counts(eng, cB, cnt)))
+        return rc;
+    const uint32_t n1 = cnt[L];
+    uint32_t       n_full = n1;
+    const uint32_t* full  = eng->d_plist;
+    if (n1 && widen)
+    {
+        PropRound r2;
+        r2.cw.assign(cnt.begin(), cnt.begin() + L);
+        r2.win   = eng->d_plist;
+        r2.n_win = n1;
+        r2.t0    = K;
+        r2.t1    = KP;
+        if ((rc = prop_round(eng, r2, quantile, P)))
+            return rc;
+        k_prop_check<<<nblk(n1), 256, 0, eng->stream>>>(P, n1, L, eng->d_ptop, eng->d_pmargin, KP,
+                                                       KP - 1, eng->prop_margin, thre, wp,
+                                                       eng->d_plist, 0, nullptr, nullptr,
+                                                       eng->d_plist + eng->Qc, cC + L);
+        LAUNCHCHK();
+        std::vector<uint32_t> n2(1);
+        if ((rc = read_counts(eng, cC + L, n2)))
+            return rc;
+        n_full = n2[0];
+        full   = eng->d_plist + eng->Qc;
+        eng->counters[20] += n1;
+    }
+    if (n_full)
+    {
+        PropRound r3;
+        r3.cw.assign(L, 0u);
+        r3.all   = full;
+        r3.n_all = n_full;
+        if ((rc = prop_round(eng, r3, quantile, P)))
+            return rc;
+    }
+    eng->counters[16] += (int64_t)(nq - nfb - n_full);
+    eng->counters[17] += nfb;
+    eng->counters[18] += n_full;
+    return CSG_OK;
+}
+
+int alloc_proposal(csg_engine* eng)
+{
+    const size_t qc = eng->Qc, L = eng->L;
+    int          rc;
+    if ((rc = dalloc(eng, &eng->d_ptop, qc * 8)) || (rc = dalloc(eng, &eng->d_pmargin, qc)) ||
+        (rc = dalloc(eng, &eng->d_pcnt, 3 * (L + 1))) || (rc = dalloc(eng, &eng->d_pslot, qc * 8)) ||
+        (rc = dalloc(eng, &eng->d_pfb, qc)) || (rc = dalloc(eng, &eng->d_plist, 2 * qc)) || (rc = dalloc(eng, &eng->d_pcells, qc * L)) ||
+        (rc = dalloc(eng, &eng->d_pitems, qc * L)) || (rc = dalloc(eng, &eng->d_pseg, qc * L)) ||
+        (rc = dalloc(eng, &eng->d_pabase, L)) || (rc = dalloc(eng, &eng->d_pacnt, L)) ||
+        (rc = dalloc(eng, &eng->d_pcoff, L)))
+        return rc;
+    return CSG_OK;
+}
+
+void free_proposal(csg_engine* eng)
+{
+    dfree(eng->d_ptop);
+    dfree(eng->d_pmargin);
+    dfree(eng->d_pcnt);
+    dfree(eng->d_pslot);
+    dfree(eng->d_pfb);
+    dfree(eng->d_plist);
+    dfree(eng->d_pcells);
+    dfree(eng->d_pitems);
+    dfree(eng->d_pseg);
+    dfree(eng->d_pabase);
+    dfree(eng->d_pacnt);
+    dfree(eng->d_pcoff);
+    dfree(eng->d_ptiles);
+    eng->d_ptiles_cap = 0;
+}
+
 }  // namespace
 
 int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path,
@@ -763,9 +1218,18 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
         return eng->fail(CSG_ERR_STATE, "csg_get_label before csg_copyin_score");
     if (path < CSG_PATH_PLAIN || path > CSG_PATH_FP16_MFMA)
         return eng->fail(CSG_ERR_ARGUMENT, "unknown correlation path");
+    const bool prop = eng->prop_k > 0;
+    if (prop && path != CSG_PATH_EXACT_INT8)
+        return eng->fail(CSG_ERR_ARGUMENT, "the learned proposal runs on the exact INT8 path");
     HIPCHK(hipSetDevice(eng->device));
-    int rc = set_pct(eng, &eng->d_pct, &eng->pct_quantile, eng->ctidx, eng->L, quantile);
-    if (rc)
+    int rc;
+    if (prop && (rc = mlp_check(eng)))
+        return rc;
+    if (prop && (uint32_t)eng->prop_k > eng->L)
+        return eng->fail(CSG_ERR_ARGUMENT, "proposal size exceeds the label count");
+    if ((rc = set_pct(eng, &eng->d_pct, &eng->pct_quantile, eng->ctidx, eng->L, quantile)))
+        return rc;
+    if (prop && !eng->d_pcells && (rc = alloc_proposal(eng)))
         return rc;
     std::memset(eng->counters, 0, sizeof(eng->counters));
     eng->ft_exact    = path == CSG_PATH_EXACT_INT8;
@@ -819,6 +1283,13 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
             if ((rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh,
                                        eng->d_ql, eng->d_qf16)))
                 return rc;
+        }
+        if (prop)
+        {
+            if ((rc = propose_chunk(eng, q0, nq, quantile, finetune_thre)) ||
+                (rc = certify_chunk(eng, q0, nq, quantile, finetune_thre)))
+                return rc;
+            continue;
         }
         {
             PhaseScope ps(eng, 1, 1);
@@ -1010,18 +1481,6 @@ struct FtBatch
 };
 
 
-// Grow-only device allocation.
-extern "C++" {
-template <class T> int ensure(csg_engine* eng, T** p, size_t* cap, size_t n)
-{
-    if (n <= *cap && *p)
-        return CSG_OK;
-    const size_t want = std::max(n, *cap + *cap / 2);
-    int          rc   = dalloc(eng, p, want);
-    *cap              = rc ? 0 : want;
-    return rc;
-}
-}
 
 // Grow-only pinned host allocation.
 int ensure_host(csg_engine* eng, uint8_t** p, size_t* cap, size_t n)
@@ -1441,7 +1900,7 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
     int            rc = set_pct(eng, &eng->d_fpct, &eng->fpct_quantile, eng->fctidx, L, quantile);
     if (rc)
         return rc;
-    for (int i = 6; i < CSG_N_COUNTERS; ++i)
+    for (int i = 6; i < 16; ++i)  // the score step's own counters stay
         eng->counters[i] = 0;
     PhaseScope    total(eng, 5, 0);
     const int64_t t_start = now_us();
@@ -1949,5 +2408,347 @@ int csg_host_percentile(const float* arr, int64_t len, float quantile, float* ou
 }
 
 void csg_host_minmax(const float* arr, int n, int* imin, int* imax) { std_minmax(arr, n, imin, imax); }
+
+
+}  // extern "C"
+
+// ================================================================ learned proposal
+
+namespace {
+
+void free_mlp(csg_engine* eng)
+{
+    dfree(eng->d_mw);
+    dfree(eng->d_mg);
+    dfree(eng->d_mm);
+    dfree(eng->d_mv);
+    dfree(eng->d_mwb);
+    dfree(eng->d_mx);
+    dfree(eng->d_macc);
+    dfree(eng->d_mh1);
+    dfree(eng->d_mh2);
+    dfree(eng->d_mz1);
+    dfree(eng->d_mz2);
+    dfree(eng->d_mdl);
+    dfree(eng->d_mdlb);
+    dfree(eng->d_mdh);
+    dfree(eng->d_mdz);
+    dfree(eng->d_mdzb);
+    dfree(eng->d_my);
+    dfree(eng->d_mloss);
+    eng->m_rows_cap = eng->m_train_cap = 0;
+    eng->mlp_ready = false;
+}
+
+// Sets the FP32 parameters (host array of m_n values), refreshes the BF16
+// copy and clears the optimiser state.
+int mlp_load(csg_engine* eng, const float* w)
+{
+    HIPCHK(hipMemcpyAsync(eng->d_mw, w, eng->m_n * sizeof(float), hipMemcpyHostToDevice,
+                          eng->stream));
+    HIPCHK(hipMemsetAsync(eng->d_mm, 0, eng->m_n * sizeof(float), eng->stream));
+    HIPCHK(hipMemsetAsync(eng->d_mv, 0, eng->m_n * sizeof(float), eng->stream));
+    HIPCHK(hipMemsetAsync(eng->d_mg, 0, eng->m_n * sizeof(float), eng->stream));
+    k_to_bf16<<<nblk(eng->m_n), 256, 0, eng->stream>>>(eng->d_mw, eng->m_n, eng->d_mwb);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    eng->m_step = 0;
+    return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_mlp_init(csg_engine* eng, int hidden, uint64_t seed)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->score_loaded)
+        return eng->fail(CSG_ERR_STATE, "csg_mlp_init before csg_copyin_score");
+    if (hidden < 64 || hidden > 4096 || hidden % 64)
+        return eng->fail(CSG_ERR_ARGUMENT, "hidden width must be a multiple of 64 in [64, 4096]");
+    HIPCHK(hipSetDevice(eng->device));
+    free_mlp(eng);
+    eng->mG  = eng->G;
+    eng->mKp = eng->Kp;
+    eng->mH  = (uint32_t)hidden;
+    eng->mL  = eng->L;
+    eng->mLp = (uint32_t)round_up(eng->L, 64);
+    const size_t Kp = eng->mKp, H = eng->mH, Lp = eng->mLp;
+    const size_t sizes[6] = { Kp * H, H, H * H, H, H * Lp, Lp };
+    size_t       off      = 0;
+    for (int i = 0; i < 6; ++i)
+    {
+        eng->m_off[i] = off;
+        off += sizes[i];
+    }
+    eng->m_n = off;
+    int rc;
+    if ((rc = dalloc(eng, &eng->d_mw, off)) || (rc = dalloc(eng, &eng->d_mg, off)) ||
+        (rc = dalloc(eng, &eng->d_mm, off)) || (rc = dalloc(eng, &eng->d_mv, off)) ||
+        (rc = dalloc(eng, &eng->d_mwb, off)))
+        return rc;
+    // He-normal hidden layers, variance 1 / fan_in at the output layer, zero
+    // biases; padded input rows and output columns stay zero
+    std::vector<float>              w(off, 0.f);
+    std::mt19937_64                 rng(seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    const float s1 = std::sqrt(2.f / (float)eng->mG), s2 = std::sqrt(2.f / (float)H),
+                s3 = std::sqrt(1.f / (float)H);
+    for (size_t i = 0; i < eng->mG; ++i)
+        for (size_t o = 0; o < H; ++o)
+            w[eng->m_off[0] + i * H + o] = s1 * nd(rng);
+    for (size_t i = 0; i < H * H; ++i)
+        w[eng->m_off[2] + i] = s2 * nd(rng);
+    for (size_t i = 0; i < H; ++i)
+        for (size_t j = 0; j < eng->mL; ++j)
+            w[eng->m_off[4] + i * Lp + j] = s3 * nd(rng);
+    if ((rc = mlp_load(eng, w.data())))
+        return rc;
+    eng->mlp_ready = true;
+    return CSG_OK;
+}
+
+int csg_mlp_dims(const csg_engine* eng, int64_t* dims)
+{
+    if (!eng || !dims)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->mlp_ready)
+        return CSG_ERR_STATE;
+    dims[0] = eng->mG;
+    dims[1] = eng->mKp;
+    dims[2] = eng->mH;
+    dims[3] = eng->mL;
+    dims[4] = eng->mLp;
+    dims[5] = (int64_t)eng->m_n;
+    return CSG_OK;
+}
+
+int csg_mlp_get_params(csg_engine* eng, float* out)
+{
+    if (!eng || !out)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->mlp_ready)
+        return eng->fail(CSG_ERR_STATE, "no proposal network");
+    HIPCHK(hipSetDevice(eng->device));
+    HIPCHK(hipMemcpyAsync(out, eng->d_mw, eng->m_n * sizeof(float), hipMemcpyDeviceToHost,
+                          eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return CSG_OK;
+}
+
+int csg_mlp_set_params(csg_engine* eng, const float* in)
+{
+    if (!eng || !in)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->mlp_ready)
+        return eng->fail(CSG_ERR_STATE, "no proposal network: call csg_mlp_init first");
+    for (size_t i = 0; i < eng->m_n; ++i)
+        if (!std::isfinite(in[i]))
+            return eng->fail(CSG_ERR_ARGUMENT, "non-finite network parameter");
+    HIPCHK(hipSetDevice(eng->device));
+    return mlp_load(eng, in);
+}
+
+int csg_mlp_grad(csg_engine* eng, uint32_t n, const float* x, const int32_t* labels, float* grad,
+                 float* loss)
+{
+    if (!eng || !x || !labels || !grad)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->mlp_ready)
+        return eng->fail(CSG_ERR_STATE, "no proposal network: call csg_mlp_init first");
+    if (n == 0 || n > 65536)
+        return eng->fail(CSG_ERR_ARGUMENT, "row count must be in [1, 65536]");
+    for (uint32_t i = 0; i < n; ++i)
+        if (labels[i] < 0 || (uint32_t)labels[i] >= eng->mL)
+            return eng->fail(CSG_ERR_ARGUMENT, "label out of range");
+    HIPCHK(hipSetDevice(eng->device));
+    int rc;
+    if ((rc = mlp_work(eng, n, true)))
+        return rc;
+    float* dx = nullptr;
+    if ((rc = upload(eng, &dx, x, (size_t)n * eng->mG)))
+        return rc;
+    k_pad_bf16<<<nblk((size_t)n * eng->mKp), 256, 0, eng->stream>>>(dx, (int)n, (int)eng->mG,
+                                                                  (int)eng->mKp, eng->d_mx);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(eng->d_my, labels, n * sizeof(int32_t), hipMemcpyHostToDevice,
+                           eng->stream);
+    if (e == hipSuccess)
+        e = hipMemsetAsync(eng->d_mloss, 0, sizeof(float), eng->stream);
+    if (e != hipSuccess)
+    {
+        (void)hipStreamSynchronize(eng->stream);
+        dfree(dx);
+        return eng->fail(CSG_ERR_DEVICE, std::string("csg_mlp_grad: ") + hipGetErrorString(e));
+    }
+    rc = mlp_forward(eng, (int)n, true);
+    if (!rc)
+        rc = mlp_loss(eng, (int)n);
+    if (!rc)
+        rc = mlp_backward(eng, (int)n);
+    (void)hipStreamSynchronize(eng->stream);
+    dfree(dx);
+    if (rc)
+        return rc;
+    HIPCHK(hipMemcpy(grad, eng->d_mg, eng->m_n * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss)
+        HIPCHK(hipMemcpy(loss, eng->d_mloss, sizeof(float), hipMemcpyDeviceToHost));
+    return CSG_OK;
+}
+
+int csg_mlp_train(csg_engine* eng, const int32_t* labels, int epochs, int batch, float lr,
+                  uint64_t seed, float* epoch_loss)
+{
+    if (!eng || !labels)
+        return CSG_ERR_ARGUMENT;
+    int rc;
+    if ((rc = mlp_check(eng)))
+        return rc;
+    if (epochs < 1 || batch < 1 || batch > 65536 || !(lr > 0.f) || !std::isfinite(lr))
+        return eng->fail(CSG_ERR_ARGUMENT, "epochs >= 1, batch in [1, 65536] and lr > 0 required");
+    const uint32_t N = eng->Nq;
+    for (uint32_t i = 0; i < N; ++i)
+        if (labels[i] < 0 || (uint32_t)labels[i] >= eng->mL)
+            return eng->fail(CSG_ERR_ARGUMENT, "label out of range");
+    HIPCHK(hipSetDevice(eng->device));
+    const size_t Kp = eng->Kp;
+    __bf16*      feat = nullptr;
+    int32_t*     lab  = nullptr;
+    uint32_t*    perm = nullptr;
+    auto         cleanup = [&]() {
+        (void)hipStreamSynchronize(eng->stream);
+        dfree(feat);
+        dfree(lab);
+        dfree(perm);
+    };
+    if ((rc = dalloc(eng, &feat, (size_t)N * Kp)) || (rc = upload(eng, &lab, labels, N)) ||
+        (rc = dalloc(eng, &perm, N)) || (rc = mlp_work(eng, (size_t)batch, true)))
+    {
+        cleanup();
+        return rc;
+    }
+    // features of every loaded query cell: ranks over the score gene set / G
+    for (uint32_t q0 = 0; q0 < N && !rc; q0 += eng->Qc)
+    {
+        const uint32_t nq = std::min(eng->Qc, N - q0);
+        rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq, eng->d_sqq_f,
+                       eng->d_sqq_i);
+        if (!rc)
+        {
+            const size_t n = (size_t)nq * Kp;
+            k_rank_features<<<nblk(n), 256, 0, eng->stream>>>(eng->d_Xq, n, 1.f / (float)eng->G,
+                                                             feat + (size_t)q0 * Kp);
+            if (hipGetLastError() != hipSuccess)
+                rc = eng->fail(CSG_ERR_DEVICE, "feature launch failed");
+        }
+    }
+    std::vector<uint32_t> order(N);
+    std::mt19937_64       rng(seed ^ 0x9E3779B97F4A7C15ull);
+    const int64_t         per_epoch = (N + batch - 1) / batch;
+    const int64_t         total     = per_epoch * epochs;
+    int64_t               it        = 0;
+    for (int e = 0; e < epochs && !rc; ++e)
+    {
+        for (uint32_t i = 0; i < N; ++i)
+            order[i] = i;
+        std::shuffle(order.begin(), order.end(), rng);
+        hipError_t he = hipMemcpyAsync(perm, order.data(), N * sizeof(uint32_t),
+                                       hipMemcpyHostToDevice, eng->stream);
+        if (he == hipSuccess)
+            he = hipMemsetAsync(eng->d_mloss, 0, sizeof(float), eng->stream);
+        if (he != hipSuccess)
+        {
+            rc = eng->fail(CSG_ERR_DEVICE, std::string("csg_mlp_train: ") + hipGetErrorString(he));
+            break;
+        }
+        for (uint32_t b0 = 0; b0 < N && !rc; b0 += (uint32_t)batch, ++it)
+        {
+            const int rows = (int)std::min<uint32_t>((uint32_t)batch, N - b0);
+            k_gather_batch<<<nblk((size_t)rows * (Kp / 8)), 256, 0, eng->stream>>>(
+                feat, lab, perm + b0, rows, (int)Kp, eng->d_mx, eng->d_my);
+            if (hipGetLastError() != hipSuccess)
+            {
+                rc = eng->fail(CSG_ERR_DEVICE, "batch gather launch failed");
+                break;
+            }
+            // cosine decay from lr to lr / 20 over all steps
+            const double f    = total > 1 ? (double)it / (double)(total - 1) : 0.0;
+            const float  lr_t = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.14159265358979323846 * f))));
+            if ((rc = mlp_forward(eng, rows, true)) || (rc = mlp_loss(eng, rows)) ||
+                (rc = mlp_backward(eng, rows)) || (rc = mlp_adam(eng, lr_t)))
+                break;
+        }
+        if (!rc)
+        {
+            float sum = 0.f;
+            he        = hipMemcpyAsync(&sum, eng->d_mloss, sizeof(float), hipMemcpyDeviceToHost,
+                                       eng->stream);
+            if (he == hipSuccess)
+                he = hipStreamSynchronize(eng->stream);
+            if (he != hipSuccess)
+                rc = eng->fail(CSG_ERR_DEVICE, std::string("csg_mlp_train: ") + hipGetErrorString(he));
+            else if (epoch_loss)
+                epoch_loss[e] = sum / (float)per_epoch;
+        }
+    }
+    cleanup();
+    return rc;
+}
+
+int csg_mlp_predict(csg_engine* eng, int k, int32_t* top, float* margin)
+{
+    if (!eng || !top || !margin)
+        return CSG_ERR_ARGUMENT;
+    int rc;
+    if ((rc = mlp_check(eng)))
+        return rc;
+    if (k < 1 || k > 8 || (uint32_t)k > eng->L)
+        return eng->fail(CSG_ERR_ARGUMENT, "k must be in [1, min(8, labels)]");
+    HIPCHK(hipSetDevice(eng->device));
+    int32_t* dtop = nullptr;
+    float*   dmar = nullptr;
+    if ((rc = dalloc(eng, &dtop, (size_t)eng->Qc * k)) || (rc = dalloc(eng, &dmar, eng->Qc)))
+    {
+        dfree(dtop);
+        return rc;
+    }
+    for (uint32_t q0 = 0; q0 < eng->Nq && !rc; q0 += eng->Qc)
+    {
+        const uint32_t nq = std::min(eng->Qc, eng->Nq - q0);
+        if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
+                            eng->d_sqq_f, eng->d_sqq_i)) ||
+            (rc = mlp_propose(eng, nq, k, dtop, dmar)))
+            break;
+        hipError_t he = hipMemcpyAsync(top + (size_t)q0 * k, dtop, (size_t)nq * k * sizeof(int32_t),
+                                       hipMemcpyDeviceToHost, eng->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(margin + q0, dmar, nq * sizeof(float), hipMemcpyDeviceToHost,
+                                eng->stream);
+        if (he == hipSuccess)
+            he = hipStreamSynchronize(eng->stream);
+        if (he != hipSuccess)
+            rc = eng->fail(CSG_ERR_DEVICE, std::string("csg_mlp_predict: ") + hipGetErrorString(he));
+    }
+    (void)hipStreamSynchronize(eng->stream);
+    dfree(dtop);
+    dfree(dmar);
+    return rc;
+}
+
+int csg_set_proposal(csg_engine* eng, int k, float min_margin)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (k < 0 || k > 8)
+        return eng->fail(CSG_ERR_ARGUMENT, "proposal size must be in [0, 8]");
+    if (!(min_margin >= 0.f && min_margin <= 1.f))
+        return eng->fail(CSG_ERR_ARGUMENT, "minimum margin must be in [0, 1]");
+    eng->prop_k      = k;
+    eng->prop_margin = min_margin;
+    return CSG_OK;
+}
 
 }  // extern "C"

@@ -227,10 +227,52 @@ __device__ inline v4i times16(v4i h)
     return r;
 }
 
-__global__ __launch_bounds__(512) void k_gemm_int8w(
-    const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
-    const int8_t* __restrict__ rl, int Kp, const float* __restrict__ sq_q,
-    const float* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+// Row / column maps of one 128 x 256 block of the wide kernel.  a_row(r) and
+// b_row(r) give the operand rows staged for block rows / columns r (always
+// inside the operand buffers); row_ok / col_ok say which results are stored.
+struct DenseW8Map
+{
+    int          m0, n0, nq, nr;
+    const float* sq_q;
+    const float* sq_r;
+    float*       S;
+    size_t       ldS;
+    __device__ size_t a_row(int r) const { return (size_t)(m0 + r); }
+    __device__ size_t b_row(int r) const { return (size_t)(n0 + r); }
+    __device__ bool   row_ok(int m) const { return m0 + m < nq; }
+    __device__ float  row_sq(int m) const { return sq_q[m0 + m]; }
+    __device__ float* row_out(int m) const { return S + (size_t)(m0 + m) * ldS + n0; }
+    __device__ bool   col_ok(int n) const { return n0 + n < nr; }
+    __device__ float  col_sq(int n) const { return sq_r[n0 + n]; }
+};
+
+// One label of a grouped contraction: the query rows listed in cells[0, mcount)
+// against the reference rows b0 .. b0 + ncount - 1; row m of the result is
+// stored contiguously at out + m * ld.  Rows past the list repeat its last
+// row, reference rows past b_max read row b_max.
+struct GroupW8Map
+{
+    const uint32_t* cells;
+    int             mcount, ncount;
+    uint32_t        b0, b_max;
+    const float*    sq_q;
+    const float*    sq_r;
+    float*          out;
+    size_t          ld;
+    __device__ size_t a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
+    __device__ size_t b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
+    __device__ bool   row_ok(int m) const { return m < mcount; }
+    __device__ float  row_sq(int m) const { return sq_q[cells[m]]; }
+    __device__ float* row_out(int m) const { return out + (size_t)m * ld; }
+    __device__ bool   col_ok(int n) const { return n < ncount; }
+    __device__ float  col_sq(int n) const { return sq_r[b0 + n]; }
+};
+
+template <class Map>
+__device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
+                                        const int8_t* __restrict__ ql,
+                                        const int8_t* __restrict__ rh,
+                                        const int8_t* __restrict__ rl, int Kp, const Map& map)
 {
     // LDS rows: [A hi 128][A lo 128][B hi 256][B lo 256], 80 bytes each
     __shared__ __attribute__((aligned(16))) int8_t sm[(2 * W8_BM + 2 * W8_BN) * W8_LD];
@@ -239,18 +281,21 @@ __global__ __launch_bounds__(512) void k_gemm_int8w(
     const int wid  = tid >> 6;
     const int wm   = wid >> 2;  // 0..1: 64-row half of the block
     const int wn   = wid & 3;   // 0..3: 64-column quarter
-    const int m0   = blockIdx.y * W8_BM;
-    const int n0   = blockIdx.x * W8_BN;
 
-    // staging map: chunk c (16 bytes) -> (plane, row, 16-byte column)
-    // A: 2 planes x 128 rows x 4 = 1024 chunks (2 per thread);
-    // B: 2 planes x 256 rows x 4 = 2048 chunks (4 per thread)
-    const int ca0 = tid, ca1 = tid + 512;
-    const int cb0 = tid, cb1 = tid + 512, cb2 = tid + 1024, cb3 = tid + 1536;
-#define W8_ASRC(c, k0) ((((c) >> 9) ? ql : qh) + (size_t)(m0 + (((c) >> 2) & 127)) * Kp + (k0) + ((c) & 3) * 16)
-#define W8_BSRC(c, k0) ((((c) >> 10) ? rl : rh) + (size_t)(n0 + (((c) >> 2) & 255)) * Kp + (k0) + ((c) & 3) * 16)
-#define W8_ADST(c) (sm + ((((c) >> 9) * W8_BM) + (((c) >> 2) & 127)) * W8_LD + ((c) & 3) * 16)
-#define W8_BDST(c) (sm + (2 * W8_BM + (((c) >> 10) * W8_BN) + (((c) >> 2) & 255)) * W8_LD + ((c) & 3) * 16)
+    // staging map: 16-byte chunk c -> (plane, row, 16-byte column).  A: 2
+    // planes x 128 rows x 4 chunks (thread tid stages row tid/4 of both
+    // planes); B: 2 planes x 256 rows x 4 (rows tid/4 and tid/4 + 128).
+    const int    col  = (tid & 3) * 16;
+    const int    rA   = tid >> 2;
+    const size_t ga   = map.a_row(rA) * (size_t)Kp + col;
+    const size_t gb0  = map.b_row(rA) * (size_t)Kp + col;
+    const size_t gb1  = map.b_row(rA + 128) * (size_t)Kp + col;
+    int8_t*      dA0  = sm + rA * W8_LD + col;
+    int8_t*      dA1  = sm + (W8_BM + rA) * W8_LD + col;
+    int8_t*      dB0  = sm + (2 * W8_BM + rA) * W8_LD + col;
+    int8_t*      dB1  = sm + (2 * W8_BM + rA + 128) * W8_LD + col;
+    int8_t*      dB2  = sm + (2 * W8_BM + W8_BN + rA) * W8_LD + col;
+    int8_t*      dB3  = sm + (2 * W8_BM + W8_BN + rA + 128) * W8_LD + col;
 
     v4i acc_x[4][4], acc_ll[4][4];
 #pragma unroll
@@ -261,33 +306,33 @@ __global__ __launch_bounds__(512) void k_gemm_int8w(
             acc_x[i][j]  = v4i{ 0, 0, 0, 0 };
             acc_ll[i][j] = v4i{ 0, 0, 0, 0 };
         }
-    int4 pa0 = *(const int4*)W8_ASRC(ca0, 0);
-    int4 pa1 = *(const int4*)W8_ASRC(ca1, 0);
-    int4 pb0 = *(const int4*)W8_BSRC(cb0, 0);
-    int4 pb1 = *(const int4*)W8_BSRC(cb1, 0);
-    int4 pb2 = *(const int4*)W8_BSRC(cb2, 0);
-    int4 pb3 = *(const int4*)W8_BSRC(cb3, 0);
+    int4 pa0 = *(const int4*)(qh + ga);
+    int4 pa1 = *(const int4*)(ql + ga);
+    int4 pb0 = *(const int4*)(rh + gb0);
+    int4 pb1 = *(const int4*)(rh + gb1);
+    int4 pb2 = *(const int4*)(rl + gb0);
+    int4 pb3 = *(const int4*)(rl + gb1);
     const int r16  = lane & 15;
     const int kofs = (lane >> 4) * 16;
     for (int k0 = 0; k0 < Kp; k0 += W8_BK)
     {
         __syncthreads();  // the previous stage is no longer read
-        *(int4*)W8_ADST(ca0) = pa0;
-        *(int4*)W8_ADST(ca1) = pa1;
-        *(int4*)W8_BDST(cb0) = pb0;
-        *(int4*)W8_BDST(cb1) = pb1;
-        *(int4*)W8_BDST(cb2) = pb2;
-        *(int4*)W8_BDST(cb3) = pb3;
+        *(int4*)dA0 = pa0;
+        *(int4*)dA1 = pa1;
+        *(int4*)dB0 = pb0;
+        *(int4*)dB1 = pb1;
+        *(int4*)dB2 = pb2;
+        *(int4*)dB3 = pb3;
         __syncthreads();
         if (k0 + W8_BK < Kp)
         {
             const int kn = k0 + W8_BK;
-            pa0 = *(const int4*)W8_ASRC(ca0, kn);
-            pa1 = *(const int4*)W8_ASRC(ca1, kn);
-            pb0 = *(const int4*)W8_BSRC(cb0, kn);
-            pb1 = *(const int4*)W8_BSRC(cb1, kn);
-            pb2 = *(const int4*)W8_BSRC(cb2, kn);
-            pb3 = *(const int4*)W8_BSRC(cb3, kn);
+            pa0 = *(const int4*)(qh + ga + kn);
+            pa1 = *(const int4*)(ql + ga + kn);
+            pb0 = *(const int4*)(rh + gb0 + kn);
+            pb1 = *(const int4*)(rh + gb1 + kn);
+            pb2 = *(const int4*)(rl + gb0 + kn);
+            pb3 = *(const int4*)(rl + gb1 + kn);
         }
         v4i ah[4], al[4], ah16[4];
 #pragma unroll
@@ -315,32 +360,71 @@ __global__ __launch_bounds__(512) void k_gemm_int8w(
             }
         }
     }
-#undef W8_ASRC
-#undef W8_BSRC
-#undef W8_ADST
-#undef W8_BDST
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
         {
-            const int m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + e;
-            if (m >= nq)
+            const int m = wm * 64 + i * 16 + (lane >> 4) * 4 + e;
+            if (!map.row_ok(m))
                 continue;
-            const float sq  = sq_q[m];
-            float*      out = S + (size_t)m * ldS;
+            const float sq  = map.row_sq(m);
+            float*      out = map.row_out(m);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
             {
-                const int n = n0 + wn * 64 + j * 16 + r16;
-                if (n >= nr)
+                const int n = wn * 64 + j * 16 + r16;
+                if (!map.col_ok(n))
                     continue;
                 const long long s4 = ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
-                const float prod = sq * sq_r[n];
+                const float prod = sq * map.col_sq(n);
                 const float d    = sqrtf(prod);
                 out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
             }
         }
+}
+
+__global__ __launch_bounds__(512) void k_gemm_int8w(
+    const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
+    const int8_t* __restrict__ rl, int Kp, const float* __restrict__ sq_q,
+    const float* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+{
+    const DenseW8Map map{ (int)blockIdx.y * W8_BM, (int)blockIdx.x * W8_BN, nq, nr, sq_q, sq_r, S,
+                          ldS };
+    gemm_int8w_block(qh, ql, rh, rl, Kp, map);
+}
+
+// Grouped contraction: block b takes tile tiles[b] = (label l, m0, n0), the
+// cells cells[a_base[l] + m0 ..] (a_cnt[l] in all) against reference rows
+// lstart[l] + n0 .. of label l (llen[l] in all); each cell's scores against
+// the label's rows are stored contiguously, label l's block starting at
+// S + c_off[l].
+struct GroupTile
+{
+    uint32_t label, m0, n0, pad_;
+};
+
+__global__ __launch_bounds__(512) void k_gemm_int8w_group(
+    const GroupTile* __restrict__ tiles, const uint32_t* __restrict__ cells,
+    const uint32_t* __restrict__ a_base, const uint32_t* __restrict__ a_cnt,
+    const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
+    const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
+    const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
+    int Kp, uint32_t b_max, const float* __restrict__ sq_q, const float* __restrict__ sq_r,
+    float* __restrict__ S)
+{
+    const GroupTile t  = tiles[blockIdx.x];
+    const uint32_t  nl = llen[t.label];
+    const GroupW8Map map{ cells + a_base[t.label] + t.m0,
+                          (int)(a_cnt[t.label] - t.m0),
+                          (int)(nl - t.n0),
+                          lstart[t.label] + t.n0,
+                          b_max,
+                          sq_q,
+                          sq_r,
+                          S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
+                          nl };
+    gemm_int8w_block(qh, ql, rh, rl, Kp, map);
 }
 
 // ------------------------------------- approximate MFMA contractions ---

@@ -1,0 +1,319 @@
+// Kernels of the learned label proposal: a 3-layer perceptron over the score
+// step's rank features (doubled centred ranks / G, in BF16),
+//     h1 = gelu(x W1 + b1), h2 = gelu(h1 W2 + b2), logits = h2 W3 + b3,
+// trained with softmax cross-entropy and Adam (FP32 master weights, BF16
+// copies for the contractions).  All contractions run on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace csg {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float  v4f_m __attribute__((ext_vector_type(4)));
+
+// C[M x N] (FP32, row-major, ldc) = sum_k A(m, k) * B(k, n) with
+//   A(m, k) = TA ? A[k * lda + m] : A[m * lda + k]
+//   B(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n]
+// (BF16 operands).  Block tile 64 x 64 (4 waves of 32 x 32), K staged 32 at a
+// time through LDS in k-contiguous rows; out-of-range rows / k read as zero.
+// A k-contiguous operand needs K % 32 == 0 and a leading dimension that is a
+// multiple of 8.  beta = 0 overwrites C, beta = 1 accumulates into it.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_bgemm(const __bf16* __restrict__ A, int lda,
+                                               const __bf16* __restrict__ B, int ldb,
+                                               float* __restrict__ C, int ldc, int M, int N,
+                                               int K, float beta)
+{
+    constexpr int BM = 64, BN = 64, BK = 32, LD = 40;  // LDS row: 32 k + 8 pad
+    __shared__ __attribute__((aligned(16))) __bf16 sa[BM * LD];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[BN * LD];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    v4f_m     acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            acc[i][j] = v4f_m{ 0.f, 0.f, 0.f, 0.f };
+    const __bf16 zero = (__bf16)0.f;
+    for (int k0 = 0; k0 < K; k0 += BK)
+    {
+        // 64 x 32 values per operand.  A k-contiguous operand (A untransposed,
+        // B transposed) is staged with one 16-byte load per thread (K is a
+        // multiple of 32 and the leading dimension of 8); the other layout
+        // with 8 coalesced 2-byte loads per thread.
+        if constexpr (!TA)
+        {
+            const int r = tid >> 2, k = (tid & 3) * 8, gm = m0 + r;
+            int4      v = int4{ 0, 0, 0, 0 };
+            if (gm < M)
+                v = *(const int4*)(A + (size_t)gm * lda + k0 + k);
+            *(int4*)(&sa[r * LD + k]) = v;
+        }
+        else
+        {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                const int e = tid + u * 256;
+                const int k = e >> 6, r = e & 63;  // consecutive threads read consecutive m
+                const int gm = m0 + r, gk = k0 + k;
+                __bf16    v  = zero;
+                if (gm < M && gk < K)
+                    v = A[(size_t)gk * lda + gm];
+                sa[r * LD + k] = v;
+            }
+        }
+        if constexpr (TB)
+        {
+            const int c = tid >> 2, k = (tid & 3) * 8, gn = n0 + c;
+            int4      v = int4{ 0, 0, 0, 0 };
+            if (gn < N)
+                v = *(const int4*)(B + (size_t)gn * ldb + k0 + k);
+            *(int4*)(&sb[c * LD + k]) = v;
+        }
+        else
+        {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                const int e = tid + u * 256;
+                const int k = e >> 6, c = e & 63;
+                const int gn = n0 + c, gk = k0 + k;
+                __bf16    v  = zero;
+                if (gn < N && gk < K)
+                    v = B[(size_t)gk * ldb + gn];
+                sb[c * LD + k] = v;
+            }
+        }
+        __syncthreads();
+        const int r16 = lane & 15, kofs = (lane >> 4) * 8;
+        v8bf      a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            a[i] = *(const v8bf*)(&sa[(wm * 32 + i * 16 + r16) * LD + kofs]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            b[j] = *(const v8bf*)(&sb[(wn * 32 + j * 16 + r16) * LD + kofs]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + e;
+            if (m >= M)
+                continue;
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+            {
+                const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+                if (n < N)
+                {
+                    float* c = C + (size_t)m * ldc + n;
+                    *c       = beta != 0.f ? *c + acc[i][j][e] : acc[i][j][e];
+                }
+            }
+        }
+}
+
+__device__ inline float gelu_f(float z)
+{
+    return 0.5f * z * (1.f + erff(z * 0.70710678118654752f));
+}
+
+__device__ inline float gelu_grad(float z)
+{
+    const float cdf = 0.5f * (1.f + erff(z * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+    return cdf + z * pdf;
+}
+
+// z = acc + bias (kept for the backward pass when z is given), h = gelu(z)
+// in BF16.
+__global__ void k_bias_gelu(const float* __restrict__ acc, const float* __restrict__ bias,
+                            int rows, int cols, float* __restrict__ z, __bf16* __restrict__ h)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * cols)
+        return;
+    const float v = acc[i] + bias[i % cols];
+    if (z)
+        z[i] = v;
+    h[i]          = (__bf16)gelu_f(v);
+}
+
+// dz = dh * gelu'(z), FP32 for the bias gradient and BF16 for the next product.
+__global__ void k_gelu_back(const float* __restrict__ dh, const float* __restrict__ z, size_t n,
+                            float* __restrict__ dz, __bf16* __restrict__ dzb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float v = dh[i] * gelu_grad(z[i]);
+    dz[i]         = v;
+    dzb[i]        = (__bf16)v;
+}
+
+// Softmax cross-entropy over the first `L` of `ldl` logit columns (+ bias):
+// dlogits = (softmax - onehot) / rows (FP32 and BF16); loss summed atomically.
+__global__ void k_softmax_xent(const float* __restrict__ acc, const float* __restrict__ bias,
+                               const int32_t* __restrict__ labels, int rows, int L, int ldl,
+                               float* __restrict__ dlog, __bf16* __restrict__ dlogb,
+                               float* __restrict__ loss)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows)
+        return;
+    const float* a  = acc + (size_t)r * ldl;
+    float        mx = -3.0e38f;
+    for (int j = 0; j < L; ++j)
+        mx = fmaxf(mx, a[j] + bias[j]);
+    float s = 0.f;
+    for (int j = 0; j < L; ++j)
+        s += expf(a[j] + bias[j] - mx);
+    const int   y   = labels[r];
+    const float inv = 1.f / (float)rows;
+    for (int j = 0; j < ldl; ++j)
+    {
+        float g = 0.f;
+        if (j < L)
+            g = (expf(a[j] + bias[j] - mx) / s - (j == y ? 1.f : 0.f)) * inv;
+        dlog[(size_t)r * ldl + j]  = g;
+        dlogb[(size_t)r * ldl + j] = (__bf16)g;
+    }
+    atomicAdd(loss, (logf(s) + mx - (a[y] + bias[y])) * inv);
+}
+
+// Column sums of a [rows x cols] FP32 matrix (bias gradients).
+__global__ void k_colsum(const float* __restrict__ x, int rows, int cols, float* __restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols)
+        return;
+    float s = 0.f;
+    for (int r = 0; r < rows; ++r)
+        s += x[(size_t)r * cols + c];
+    out[c] = s;
+}
+
+// Adam on FP32 master weights; refreshes the BF16 copy (when given).
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, __bf16* __restrict__ wb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i]           = mi;
+    v[i]           = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i]           = wi;
+    if (wb)
+        wb[i] = (__bf16)wi;
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, size_t n, __bf16* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = (__bf16)x[i];
+}
+
+// Rank features: X / G in BF16 for `rows` rows of a [rows x Kp] int16 matrix.
+__global__ void k_rank_features(const int16_t* __restrict__ X, size_t n, float inv_g,
+                                __bf16* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = (__bf16)((float)X[i] * inv_g);
+}
+
+// Per row: the K largest logits (+ bias) in descending order (ties keep the
+// lower label first) and the softmax margin p(top1) - p(top2).
+template <int K>
+__global__ void k_topk(const float* __restrict__ acc, const float* __restrict__ bias, int rows,
+                       int L, int ldl, int32_t* __restrict__ top, float* __restrict__ margin)
+{
+    constexpr int KK = K < 2 ? 2 : K;  // the margin needs the runner-up
+    const int     r  = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows)
+        return;
+    const float* a = acc + (size_t)r * ldl;
+    float        best[KK];
+    int          idx[KK];
+#pragma unroll
+    for (int t = 0; t < KK; ++t)
+    {
+        best[t] = -3.0e38f;
+        idx[t]  = -1;
+    }
+    float mx = -3.0e38f;
+    for (int j = 0; j < L; ++j)
+        mx = fmaxf(mx, a[j] + bias[j]);
+    float s = 0.f;
+    for (int j = 0; j < L; ++j)
+    {
+        const float v = a[j] + bias[j];
+        s += expf(v - mx);
+        int p = KK;
+        while (p > 0 && v > best[p - 1])
+            --p;
+        if (p < KK)
+        {
+            for (int t = KK - 1; t > p; --t)
+            {
+                best[t] = best[t - 1];
+                idx[t]  = idx[t - 1];
+            }
+            best[p] = v;
+            idx[p]  = j;
+        }
+    }
+    for (int t = 0; t < K; ++t)
+        top[(size_t)r * K + t] = idx[t];
+    margin[r] = L > 1 ? (expf(best[0] - mx) - expf(best[1] - mx)) / s : 1.f;
+}
+
+// Rows perm[0, rows) of a [* x cols] BF16 matrix (cols a multiple of 8) and
+// their labels, gathered into a minibatch.
+__global__ void k_gather_batch(const __bf16* __restrict__ feat, const int32_t* __restrict__ labels,
+                               const uint32_t* __restrict__ perm, int rows, int cols,
+                               __bf16* __restrict__ xb, int32_t* __restrict__ yb)
+{
+    const int per = cols / 8;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * per)
+        return;
+    const int r = (int)(i / per), c = (int)(i % per) * 8;
+    const uint32_t src = perm[r];
+    *(int4*)(xb + (size_t)r * cols + c) = *(const int4*)(feat + (size_t)src * cols + c);
+    if (c == 0)
+        yb[r] = labels[src];
+}
+
+// [rows x g] FP32 features into a zero-padded [rows x cols] BF16 matrix.
+__global__ void k_pad_bf16(const float* __restrict__ x, int rows, int g, int cols,
+                           __bf16* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * cols)
+        return;
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    out[i]      = (__bf16)(c < g ? x[(size_t)r * g + c] : 0.f);
+}
+
+}  // namespace csg

@@ -21,12 +21,14 @@ from . import _lib
 
 PATHS = {"plain": 0, "exact_int8": 1, "fp32_mfma": 2, "fp16": 3}
 PHASES = ["rank", "corr", "select", "label", "recheck", "finetune", "ft_corr", "ft_rank",
-          "ft_select"]
+          "ft_select", "propose"]
 COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells", "nan_segments",
             "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes", "ft_redone_cells",
-            "ft_host_lists_us", "ft_host_group_us", "ft_host_build_us", "ft_batch_us"]
-N_PHASES = 9
-N_COUNTERS = 16
+            "ft_host_lists_us", "ft_host_group_us", "ft_host_build_us", "ft_batch_us",
+            "prop_cells", "prop_fallback_margin", "prop_fallback_check", "prop_scored_pairs",
+            "prop_widened"]
+N_PHASES = 10
+N_COUNTERS = 21
 
 _V = C.c_void_p
 _U32 = C.c_uint32
@@ -77,6 +79,14 @@ def lib():
     L.csg_selftest_ft.argtypes = [_V, C.c_int, P(_I64)]
     L.csg_host_percentile.argtypes = [_V, _I64, _F, P(_F)]
     L.csg_host_minmax.argtypes = [_V, C.c_int, P(C.c_int), P(C.c_int)]
+    L.csg_mlp_init.argtypes = [_V, C.c_int, _U64]
+    L.csg_mlp_dims.argtypes = [_V, _V]
+    L.csg_mlp_get_params.argtypes = [_V, _V]
+    L.csg_mlp_set_params.argtypes = [_V, _V]
+    L.csg_mlp_grad.argtypes = [_V, _U32, _V, _V, _V, _V]
+    L.csg_mlp_train.argtypes = [_V, _V, C.c_int, C.c_int, _F, _U64, _V]
+    L.csg_mlp_predict.argtypes = [_V, C.c_int, _V, _V]
+    L.csg_set_proposal.argtypes = [_V, C.c_int, _F]
     L.csg_prep_train.argtypes = [C.c_int, _U32, _U32, _V, _V, _V, _U32, _U32, _V, _V, _V, _U32, _V,
                                  _V, _U32, _V, _U32, P(_V), C.c_char_p, C.c_size_t]
     L.csg_prep_generate.argtypes = [_V, C.c_int, P(CodedStruct)]
@@ -246,6 +256,58 @@ class Engine:
 
     def destroy(self):
         self._check(lib().csg_destroy(self._h), "destroy")
+
+    # ---- learned proposal network ----
+
+    def mlp_init(self, hidden: int = 512, seed: int = 0):
+        self._check(lib().csg_mlp_init(self._h, hidden, seed), "mlp_init")
+
+    def mlp_dims(self) -> Dict[str, int]:
+        d = np.zeros(6, np.int64)
+        self._check(lib().csg_mlp_dims(self._h, _p(d)), "mlp_dims")
+        return dict(zip(["G", "Kp", "H", "L", "Lp", "n_params"], (int(v) for v in d)))
+
+    def mlp_params(self) -> np.ndarray:
+        w = np.zeros(self.mlp_dims()["n_params"], np.float32)
+        self._check(lib().csg_mlp_get_params(self._h, _p(w)), "mlp_get_params")
+        return w
+
+    def set_mlp_params(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.shape != (self.mlp_dims()["n_params"],):
+            raise ValueError("parameter array has the wrong size")
+        self._check(lib().csg_mlp_set_params(self._h, _p(w)), "mlp_set_params")
+
+    def mlp_grad(self, x: np.ndarray, y: np.ndarray):
+        """Loss and gradient (flat, parameter layout) of the mean cross-entropy
+        over the rows of x ([n x G_s] features) with labels y."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        g = np.zeros(self.mlp_dims()["n_params"], np.float32)
+        loss = C.c_float()
+        self._check(lib().csg_mlp_grad(self._h, x.shape[0], _p(x), _p(y), _p(g), C.byref(loss)),
+                    "mlp_grad")
+        return g, loss.value
+
+    def mlp_train(self, labels: np.ndarray, epochs: int = 4, batch: int = 1024, lr: float = 1e-3,
+                  seed: int = 0) -> np.ndarray:
+        y = np.ascontiguousarray(labels, dtype=np.int32)
+        if y.shape != (self.Nq,):
+            raise ValueError("one label per loaded query cell required")
+        losses = np.zeros(epochs, np.float32)
+        self._check(lib().csg_mlp_train(self._h, _p(y), epochs, batch, lr, seed, _p(losses)),
+                    "mlp_train")
+        return losses
+
+    def mlp_predict(self, k: int = 4):
+        top = np.zeros((self.Nq, k), np.int32)
+        margin = np.zeros(self.Nq, np.float32)
+        self._check(lib().csg_mlp_predict(self._h, k, _p(top), _p(margin)), "mlp_predict")
+        return top, margin
+
+    def set_proposal(self, k: int, min_margin: float = 0.2):
+        """Proposal-restricted exact scoring (k labels per cell; 0 turns it off)."""
+        self._check(lib().csg_set_proposal(self._h, k, min_margin), "set_proposal")
 
     def phase_times(self, reset: bool = False) -> Dict[str, tuple]:
         ms = np.zeros(N_PHASES, np.float64)

@@ -294,6 +294,19 @@ def generate(cfg: SynthConfig | str, n_qry: Optional[int] = None, n_ref: Optiona
     return ds
 
 
+def training_set(ds: Dataset) -> Dataset:
+    """The reference of `ds` with its training cells (cfg.n_train) as the query
+    set: the input on which the plain method labels the proposal network's
+    training data."""
+    t = ds.train
+    if t is None:
+        raise ValueError("dataset has no training cells (n_train = 0)")
+    return dataclasses.replace(ds, qry_height=t.qry_height, qry_data=t.qry_data,
+                               qry_indices=t.qry_indices, qry_indptr=t.qry_indptr,
+                               cellnames=[f"train_{i:07d}" for i in range(t.qry_height)],
+                               qry_truth=t.qry_truth, train=None)
+
+
 def tiny(n_ref_per_label=6, n_labels=3, n_genes=300, n_qry=10, nonzero=0.3, markers=6,
          seed=7, n_ref_only=0, n_qry_only=0, props="uniform") -> Dataset:
     """Small dataset for brute-force checks."""

@@ -117,9 +117,10 @@ int csg_destroy(csg_engine* eng);
  * the engine's stream, in milliseconds: [0] rank, [1] correlation,
  * [2] quantile select, [3] label/mask, [4] certification recheck,
  * [5] fine-tune total, [6] fine-tune correlation, [7] fine-tune rank,
- * [8] fine-tune select+filter.  counts[i] is the number of launches timed in
- * phase i.  Either pointer may be NULL. */
-#define CSG_N_PHASES 9
+ * [8] fine-tune select+filter, [9] learned proposal (network inference and
+ * work lists).  counts[i] is the number of launches timed in phase i.  Either
+ * pointer may be NULL. */
+#define CSG_N_PHASES 10
 int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
 
 /* Counters of the last csg_get_label / csg_finetune call:
@@ -131,9 +132,66 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
  * [10] fine-tune sum over items of gene-subset size * reference rows,
  * [11] fine-tune cells the exact form redid in FP32 order,
  * [12..15] fine-tune host microseconds: candidate lists from the mask,
- * grouping by candidate set, work-table building, batch round trips. */
-#define CSG_N_COUNTERS 16
+ * grouping by candidate set, work-table building, batch round trips,
+ * [16] cells scored on proposed labels only, [17] cells below the proposal
+ * margin (scored on every label), [18] confident cells the edge tests sent
+ * to scoring on every label, [19] (cell, label) pairs the proposal path
+ * scored, [20] cells widened to their second group of ranked labels. */
+#define CSG_N_COUNTERS 21
 int csg_counters(csg_engine* eng, int64_t* out);
+
+/* ---- learned label proposal (no counterpart in the reference) ----
+ *
+ * A perceptron over the score step's rank features (doubled centred ranks
+ * over the score gene set, divided by G_s, in BF16): G_s -> H -> H -> L with
+ * GELU, trained with softmax cross-entropy and Adam (FP32 master weights,
+ * BF16 operands on MFMA, FP32 accumulation).  Parameters form one FP32 array
+ * [W1 (Kp x H) | b1 (H) | W2 (H x H) | b2 (H) | W3 (H x Lp) | b3 (Lp)],
+ * row-major, Kp = G_s rounded up to 64, Lp = L rounded up to 64; input rows
+ * >= G_s and output columns >= L are inert.  The network belongs to the
+ * engine and survives csg_copyin_score calls with the same G_s and L. */
+
+/* Random initialisation (seeded) for the loaded score data's G_s and L;
+ * hidden = H, a multiple of 64 in [64, 4096]. */
+int csg_mlp_init(csg_engine* eng, int hidden, uint64_t seed);
+
+/* dims[0..5] = G_s, Kp, H, L, Lp, number of parameters. */
+int csg_mlp_dims(const csg_engine* eng, int64_t* dims);
+
+/* Copy the FP32 parameters out / in (setting them clears the Adam state). */
+int csg_mlp_get_params(csg_engine* eng, float* out);
+int csg_mlp_set_params(csg_engine* eng, const float* in);
+
+/* Loss and gradient of the mean cross-entropy over n rows of features x
+ * ([n x G_s] FP32, rounded to BF16 on the device) with the given labels, at
+ * the current parameters; grad receives one value per parameter.  For
+ * checking the training kernels; the parameters are not changed. */
+int csg_mlp_grad(csg_engine* eng, uint32_t n, const float* x, const int32_t* labels, float* grad,
+                 float* loss);
+
+/* Train on the loaded query cells (csg_copyin_score) with one label per cell
+ * (e.g. the final labels of the plain method): `epochs` shuffled passes of
+ * minibatches of `batch` rows, Adam at learning rate lr with cosine decay to
+ * lr / 20.  epoch_loss (optional) receives the mean loss of each epoch. */
+int csg_mlp_train(csg_engine* eng, const int32_t* labels, int epochs, int batch, float lr,
+                  uint64_t seed, float* epoch_loss);
+
+/* Top-k labels ([Nq x k], descending logits) and softmax margin p1 - p2 of
+ * every loaded query cell, 1 <= k <= min(8, L). */
+int csg_mlp_predict(csg_engine* eng, int k, int32_t* top, float* margin);
+
+/* Proposal-restricted scoring for csg_get_label on the exact INT8 path
+ * (k = 0 turns it off).  The network ranks kp = min(2k, 8, L) labels per
+ * cell.  Cells with margin >= min_margin are scored on their first k labels
+ * only and the rest on every label.  A proposal-scored cell whose k-th label
+ * may lie within finetune_thre of its exact winner (so that the candidate
+ * set, or the winner, could extend past the proposal) is scored on its labels
+ * k .. kp-1 too, and on every label if its kp-th label is still that close.
+ * Unscored labels get quantile -inf, so they stay out of the mask; the labels
+ * then differ from the full path's only where a label the network ranked
+ * below the last scored one would have entered the mask although that one
+ * did not. */
+int csg_set_proposal(csg_engine* eng, int k, float min_margin);
 
 /* Diagnostic of the contraction kernels on seeded random ranks (|X| <= 1500,
  * 256 x 256 rows, K = 192): which = 0 INT8 limbs, 1 FP16, 2 FP32 MFMA,
