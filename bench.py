@@ -2,13 +2,18 @@
 """Throughput benchmark of the SingleR annotation path on one MI355X.
 
 A step is one pass of the hot path over the whole query batch: the score step
-(rank, correlate every query cell with every reference cell, per-label 0.8
-quantile, first label and candidate mask) followed by the fine-tune loop, with
-the coded inputs already resident in HBM.  `value` is annotated query cells
-per second over the timed steps.
+(rank, correlate query cells with reference cells, per-label 0.8 quantile,
+first label and candidate mask) followed by the fine-tune loop, with the coded
+inputs already resident in HBM.  `value` is annotated query cells per second
+over the timed steps.
 
 Default workload: the 200,000-query x 20,000-reference, 50-label atlas
-(synthetic, seeded).  Prints one JSON line.
+(synthetic, seeded) with 500,000 extra training cells for the learned
+proposal.  The chosen form is the exact INT8-limb score path restricted by the
+learned top-4 proposal (the network is trained on the GPU beforehand, on the
+plain method's final labels of the training cells; training is not timed but
+reported).  The plain FP32 path and the exact path without the proposal are
+measured beside it.  Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -69,11 +74,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="atlas")
+    ap.add_argument("--config", default="learned")
     ap.add_argument("--nq", type=int, default=None, help="override the query cell count")
-    ap.add_argument("--path", default="exact_int8",
-                    choices=["plain", "exact_int8", "fp32_mfma", "fp16"],
-                    help="the chosen form measured as `value`; plain is always measured beside it")
+    ap.add_argument("--form", "--path", dest="form", default="proposal",
+                    choices=["plain", "exact_int8", "fp32_mfma", "fp16", "proposal"],
+                    help="the chosen form measured as `value` (proposal = exact_int8 restricted "
+                         "by the learned proposal); plain is always measured beside it")
+    ap.add_argument("--train-cells", type=int, default=None,
+                    help="training cells for the proposal network (default: the config's, "
+                         "else 500,000)")
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--proposal-k", type=int, default=4)
+    ap.add_argument("--min-margin", type=float, default=0.2)
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cores", type=int, default=min(16, os.cpu_count() or 1))
@@ -88,27 +100,67 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    import dataclasses
+    import numpy as np
     from cusingler_amd import engine, synth
     if args.calibrate_copy:
         calibrate_copy()
 
+    chosen = args.form
+    cfg = synth.CONFIGS[args.config]
+    if chosen == "proposal" and (args.train_cells is not None or cfg.n_train == 0):
+        cfg = dataclasses.replace(cfg, n_train=args.train_cells or 500000)
+    elif chosen != "proposal":
+        cfg = dataclasses.replace(cfg, n_train=0)
     t0 = time.perf_counter()
-    ds = synth.generate(args.config, n_qry=args.nq)
+    ds = synth.generate(cfg, n_qry=args.nq)
     t_gen = time.perf_counter() - t0
     print(f"[bench] generated {args.config}: {ds.qry_height} query x {ds.ref_height} reference "
-          f"cells in {t_gen:.1f} s", file=sys.stderr, flush=True)
+          f"cells (+{cfg.n_train} training cells) in {t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    eng = engine.Engine(0)
+    training = None
+    if chosen == "proposal":
+        # label the training cells with the plain method, then train the
+        # network on the GPU (not timed)
+        t0 = time.perf_counter()
+        tprep = engine.Prep(synth.training_set(ds), args.cores)
+        eng.copyin_score(tprep.generate(0))
+        eng.copyin(tprep.generate(1))
+        del tprep
+        t1 = time.perf_counter()
+        eng.get_label(0.8, 0.05, "plain", outputs=False)
+        ylab, _ = eng.finetune(0.8, 0.05, 0)
+        _hip_sync()
+        t2 = time.perf_counter()
+        eng.mlp_init(512, seed=1)
+        losses = eng.mlp_train(ylab.astype(np.int32), epochs=args.epochs, batch=1024, lr=1e-3,
+                               seed=2)
+        t3 = time.perf_counter()
+        training = {"cells": int(ylab.shape[0]), "labels_from": "plain finalLabel",
+                    "network": f"G_s -> 512 -> 512 -> {len(ds.codes)} (GELU), BF16 MFMA, FP32 Adam",
+                    "epochs": args.epochs, "batch": 1024, "epoch_loss": [round(float(v), 5) for v in losses],
+                    "prep_s": round(t1 - t0, 2), "plain_labelling_s": round(t2 - t1, 3),
+                    "train_s": round(t3 - t2, 3)}
+        print(f"[bench] proposal network trained on {ylab.shape[0]} cells in {t3 - t2:.2f} s, "
+              f"losses {losses}", file=sys.stderr, flush=True)
+        eng.destroy_score()
+        eng.destroy()
+        ds.train = None
+
     prep = engine.Prep(ds, args.cores)
     c0 = prep.generate(0)
     c1 = prep.generate(1)
     print(f"[bench] preprocessing {prep.train_seconds:.1f} s, G_s={c0.width} G_f={c1.width}",
           file=sys.stderr, flush=True)
-
-    eng = engine.Engine(0)
     eng.copyin_score(c0)
     eng.copyin(c1)  # fine-tune inputs; the candidate mask comes from the device
     Nq, Nr, Gs = ds.qry_height, ds.ref_height, c0.width
 
-    def measure(path):
+    def measure(form):
+        path = "exact_int8" if form == "proposal" else form
+        eng.set_proposal(args.proposal_k if form == "proposal" else 0, args.min_margin)
+
         def step():
             eng.get_label(0.8, 0.05, path, outputs=False)
             return eng.finetune(0.8, 0.05, 0)
@@ -120,26 +172,37 @@ def main():
         t0 = time.perf_counter()
         for i in range(args.steps):
             step()
-            print(f"[bench] {path} step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
+            print(f"[bench] {form} step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
         _hip_sync()
         dt = time.perf_counter() - t0
         ph = eng.phase_times()
-        cnt = eng.counters()
-        # labels of one untimed pass for the agreement check
+        # labels (and counters) of one untimed pass for the agreement check
         first, _, _ = eng.get_label(0.8, 0.05, path, outputs=True)
+        cnt = eng.counters()
         final, _ = eng.finetune(0.8, 0.05, 0)
+        cnt.update({k: v for k, v in eng.counters().items() if k.startswith("ft_")})
         return dict(value=Nq * args.steps / dt, ms=dt * 1e3 / args.steps, phases=ph, counters=cnt,
                     first=first, final=final)
 
-    chosen = args.path
-    res = {p: measure(p) for p in dict.fromkeys([chosen] if args.only_chosen else [chosen, "plain"])}
+    forms = [chosen] if args.only_chosen else list(dict.fromkeys(
+        [chosen, "plain"] + (["exact_int8"] if chosen == "proposal" else [])))
+    res = {f: measure(f) for f in forms}
     r = res[chosen]
     steps = args.steps
     corr_ms, corr_n = r["phases"]["corr"]
-    logical_ops = 2.0 * Nq * Nr * Gs * steps          # one logical MAC = 2 ops
+    if chosen == "proposal":
+        # correlations the proposal-restricted score step computed per pass
+        corr_per_step = r["counters"]["prop_correlations"]
+        ops_desc = "2 * G_s * (correlations of the launch's (cell, label) pairs)"
+    else:
+        corr_per_step = Nq * Nr
+        ops_desc = "2 * G_s * query_rows_in_chunk * reference_cells"
+    logical_ops = 2.0 * corr_per_step * Gs * steps          # one logical MAC = 2 ops
     achieved = logical_ops / (corr_ms * 1e-3) / 1e12 if corr_ms > 0 else 0.0
-    if chosen == "exact_int8":
-        peak, unit, kname = MI355X_PEAK["int8_tops"], "TOP/s", "k_gemm_int8"
+    if chosen in ("exact_int8", "proposal"):
+        kname = "k_gemm_int8w_group" if chosen == "proposal" else (
+            "k_gemm_int8w" if Gs <= 1920 else "k_gemm_int8")
+        peak, unit = MI355X_PEAK["int8_tops"], "TOP/s"
     elif chosen == "fp16":
         peak, unit, kname = MI355X_PEAK["fp16_tflops"], "TFLOP/s", "k_gemm_f16"
     elif chosen == "fp32_mfma":
@@ -149,20 +212,21 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                 "frac": round(achieved / peak, 5), "traffic": None, "kernel": kname,
                 "launches": corr_n, "avg_launch_ms": round(corr_ms / max(corr_n, 1), 4),
-                "ops_per_launch": "2 * query_rows_in_chunk * reference_cells * G_s"}
-    if chosen == "exact_int8":
+                "ops_per_launch": ops_desc}
+    if chosen in ("exact_int8", "proposal"):
         roofline["limb_mfma_utilisation"] = round(4 * achieved / peak, 5)
+    workload = (f"{args.config}: {Nq} query x {Nr} reference cells, {len(ds.codes)} labels, "
+                f"G_s={Gs}, G_f={c1.width}")
     # HBM bytes per launch from the committed counter runs (FETCH_SIZE and
     # WRITE_SIZE, scaled by a copy kernel of known size), when they cover this
     # workload and kernel
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             tr = json.load(f)
-        workload = (f"{args.config}: {Nq} query x {Nr} reference cells, {len(ds.codes)} labels, "
-                    f"G_s={Gs}, G_f={c1.width}")
-        if tr.get("kernel") == kname and tr.get("workload") == workload:
-            roofline["traffic"] = round(tr["traffic_bytes_per_launch"])
-            roofline["traffic_source"] = tr.get("source")
+        for t in (tr if isinstance(tr, list) else [tr]):
+            if t.get("kernel") == kname and t.get("workload") == workload:
+                roofline["traffic"] = round(t["traffic_bytes_per_launch"])
+                roofline["traffic_source"] = t.get("source")
     except (OSError, ValueError, KeyError):
         pass
     plain = res.get("plain", r)
@@ -174,11 +238,10 @@ def main():
         "value": round(r["value"], 1), "unit": "cells/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(r["ms"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"plain": "f32", "fp32_mfma": "f32", "exact_int8": "int8", "fp16": "f16"}[chosen],
+        "dtype": {"plain": "f32", "fp32_mfma": "f32", "exact_int8": "int8", "fp16": "f16",
+                  "proposal": "int8"}[chosen],
         "data": "synthetic (seeded NB counts, log2(1+x), float32 CSR)",
-        "config": {"workload": f"{args.config}: {Nq} query x {Nr} reference cells, "
-                               f"{len(ds.codes)} labels, G_s={Gs}, G_f={c1.width}",
-                   "path": chosen, "query_cells": Nq, "reference_cells": Nr,
+        "config": {"workload": workload, "form": chosen, "query_cells": Nq, "reference_cells": Nr,
                    "labels": len(ds.codes), "score_genes": Gs, "finetune_genes": c1.width},
         "plain": {"value": round(plain["value"], 1), "ms_per_step": round(plain["ms"], 3),
                   "phases_ms": {k: round(v[0] / steps, 3) for k, v in plain["phases"].items()}},
@@ -188,6 +251,20 @@ def main():
         "phases_ms_per_step": {k: round(v[0] / steps, 3) for k, v in r["phases"].items()},
         "counters": r["counters"],
     }
+    if "exact_int8" in res and chosen != "exact_int8":
+        e = res["exact_int8"]
+        out["exact_int8"] = {"value": round(e["value"], 1), "ms_per_step": round(e["ms"], 3),
+                             "label_match_vs_plain": {
+                                 "first": float((e["first"] == plain["first"]).mean()),
+                                 "final": float((e["final"] == plain["final"]).mean())}}
+    if chosen == "proposal":
+        c = r["counters"]
+        out["proposal"] = dict(training or {}, k=args.proposal_k, min_margin=args.min_margin,
+                               scored_pair_fraction=round(c["prop_scored_pairs"] / (Nq * len(ds.codes)), 5),
+                               proposal_only_cells=c["prop_cells"],
+                               fallback_margin_cells=c["prop_fallback_margin"],
+                               fallback_edge_cells=c["prop_fallback_check"],
+                               widened_cells=c["prop_widened"])
     eng.close()
     if not args.no_cpu:
         n = min(args.cpu_sample, Nq)

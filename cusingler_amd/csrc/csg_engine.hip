@@ -1071,6 +1071,7 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
         LAUNCHCHK();
     }
     eng->counters[19] += (int64_t)n_items;
+    eng->counters[21] += (int64_t)n_scores;
     return CSG_OK;
 }
 

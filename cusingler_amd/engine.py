@@ -26,9 +26,9 @@ COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells",
             "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes", "ft_redone_cells",
             "ft_host_lists_us", "ft_host_group_us", "ft_host_build_us", "ft_batch_us",
             "prop_cells", "prop_fallback_margin", "prop_fallback_check", "prop_scored_pairs",
-            "prop_widened"]
+            "prop_widened", "prop_correlations"]
 N_PHASES = 10
-N_COUNTERS = 21
+N_COUNTERS = 22
 
 _V = C.c_void_p
 _U32 = C.c_uint32

@@ -136,8 +136,10 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
  * [16] cells scored on proposed labels only, [17] cells below the proposal
  * margin (scored on every label), [18] confident cells the edge tests sent
  * to scoring on every label, [19] (cell, label) pairs the proposal path
- * scored, [20] cells widened to their second group of ranked labels. */
-#define CSG_N_COUNTERS 21
+ * scored, [20] cells widened to their second group of ranked labels,
+ * [21] correlations the proposal path computed (sum over its pairs of the
+ * label's reference cells). */
+#define CSG_N_COUNTERS 22
 int csg_counters(csg_engine* eng, int64_t* out);
 
 /* ---- learned label proposal (no counterpart in the reference) ----
