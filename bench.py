@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--proposal-k", type=int, default=4)
     ap.add_argument("--min-margin", type=float, default=0.2)
+    ap.add_argument("--net-cache", default=None,
+                    help="load the proposal network's parameters from this .npy file if it "
+                         "exists (skipping training), else save them there after training")
     ap.add_argument("--cpu-sample", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cores", type=int, default=min(16, os.cpu_count() or 1))
@@ -108,7 +111,10 @@ def main():
 
     chosen = args.form
     cfg = synth.CONFIGS[args.config]
-    if chosen == "proposal" and (args.train_cells is not None or cfg.n_train == 0):
+    cached = chosen == "proposal" and args.net_cache and os.path.exists(args.net_cache)
+    if cached:
+        cfg = dataclasses.replace(cfg, n_train=0)  # training cells are drawn after the atlas
+    elif chosen == "proposal" and (args.train_cells is not None or cfg.n_train == 0):
         cfg = dataclasses.replace(cfg, n_train=args.train_cells or 500000)
     elif chosen != "proposal":
         cfg = dataclasses.replace(cfg, n_train=0)
@@ -120,7 +126,10 @@ def main():
 
     eng = engine.Engine(0)
     training = None
-    if chosen == "proposal":
+    if cached:
+        net = np.load(args.net_cache, allow_pickle=False)
+        training = {"loaded_from": args.net_cache}
+    elif chosen == "proposal":
         # label the training cells with the plain method, then train the
         # network on the GPU (not timed)
         t0 = time.perf_counter()
@@ -144,6 +153,9 @@ def main():
                     "train_s": round(t3 - t2, 3)}
         print(f"[bench] proposal network trained on {ylab.shape[0]} cells in {t3 - t2:.2f} s, "
               f"losses {losses}", file=sys.stderr, flush=True)
+        net = eng.mlp_params()
+        if args.net_cache:
+            np.save(args.net_cache, net, allow_pickle=False)
         eng.destroy_score()
         eng.destroy()
         ds.train = None
@@ -156,6 +168,9 @@ def main():
     eng.copyin_score(c0)
     eng.copyin(c1)  # fine-tune inputs; the candidate mask comes from the device
     Nq, Nr, Gs = ds.qry_height, ds.ref_height, c0.width
+    if cached:
+        eng.mlp_init(512, seed=1)
+        eng.set_mlp_params(net)
 
     def measure(form):
         path = "exact_int8" if form == "proposal" else form

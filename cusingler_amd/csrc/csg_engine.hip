@@ -141,7 +141,8 @@ struct csg_engine
     uint32_t mG = 0, mKp = 0, mH = 0, mL = 0, mLp = 0;
     size_t   m_off[6] = {}, m_n = 0;
     float *  d_mw = nullptr, *d_mg = nullptr, *d_mm = nullptr, *d_mv = nullptr;
-    __bf16*  d_mwb = nullptr;  // BF16 copy of every parameter
+    __bf16*  d_mwb = nullptr;   // BF16 copy of every parameter
+    __bf16*  d_mwbT = nullptr;  // BF16 transposes of W1, W2, W3 (same offsets)
     int64_t  m_step = 0;       // Adam steps taken
     // activations (grow-only, in rows)
     size_t   m_rows_cap = 0, m_train_cap = 0;
@@ -852,23 +853,37 @@ int mlp_work(csg_engine* eng, size_t rows, bool train)
 int mlp_forward(csg_engine* eng, int rows, bool keep)
 {
     const int     H = (int)eng->mH, Kp = (int)eng->mKp, Lp = (int)eng->mLp;
-    const __bf16* wb = eng->d_mwb;
+    const __bf16* wt = eng->d_mwbT;
     const float*  w  = eng->d_mw;
-    int           rc;
-    if ((rc = bgemm<false, false>(eng, eng->d_mx, Kp, wb + eng->m_off[0], H, eng->d_macc, H, rows, H,
-                                  Kp)))
-        return rc;
-    k_bias_gelu<<<nblk((size_t)rows * H), 256, 0, eng->stream>>>(
-        eng->d_macc, w + eng->m_off[1], rows, H, keep ? eng->d_mz1 : nullptr, eng->d_mh1);
+    const dim3    gh((H + 127) / 128, (rows + 127) / 128), gl((Lp + 127) / 128, (rows + 127) / 128);
+    k_bgemm_nt<1><<<gh, 256, 0, eng->stream>>>(eng->d_mx, Kp, wt + eng->m_off[0], Kp, rows, H, Kp,
+                                                H, nullptr, w + eng->m_off[1],
+                                                keep ? eng->d_mz1 : nullptr, eng->d_mh1);
     LAUNCHCHK();
-    if ((rc = bgemm<false, false>(eng, eng->d_mh1, H, wb + eng->m_off[2], H, eng->d_macc, H, rows, H,
-                                  H)))
-        return rc;
-    k_bias_gelu<<<nblk((size_t)rows * H), 256, 0, eng->stream>>>(
-        eng->d_macc, w + eng->m_off[3], rows, H, keep ? eng->d_mz2 : nullptr, eng->d_mh2);
+    k_bgemm_nt<1><<<gh, 256, 0, eng->stream>>>(eng->d_mh1, H, wt + eng->m_off[2], H, rows, H, H, H,
+                                                nullptr, w + eng->m_off[3],
+                                                keep ? eng->d_mz2 : nullptr, eng->d_mh2);
     LAUNCHCHK();
-    return bgemm<false, false>(eng, eng->d_mh2, H, wb + eng->m_off[4], Lp, eng->d_macc, Lp, rows, Lp,
-                               H);
+    k_bgemm_nt<0><<<gl, 256, 0, eng->stream>>>(eng->d_mh2, H, wt + eng->m_off[4], H, rows, Lp, H,
+                                                Lp, eng->d_macc, nullptr, nullptr, nullptr);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// BF16 transposes of the three weight matrices (the forward operands).
+int mlp_transpose(csg_engine* eng)
+{
+    const int    Kp = (int)eng->mKp, H = (int)eng->mH, Lp = (int)eng->mLp;
+    const int    R[3] = { Kp, H, H }, Cn[3] = { H, H, Lp };
+    const size_t off[3] = { eng->m_off[0], eng->m_off[2], eng->m_off[4] };
+    for (int i = 0; i < 3; ++i)
+    {
+        const dim3 g((Cn[i] + 31) / 32, (R[i] + 31) / 32);
+        k_transpose_bf16<<<g, 256, 0, eng->stream>>>(eng->d_mw + off[i], R[i], Cn[i],
+                                                     eng->d_mwbT + off[i]);
+        LAUNCHCHK();
+    }
+    return CSG_OK;
 }
 
 // Gradients of the mean cross-entropy into d_mg, from the output gradient
@@ -929,7 +944,7 @@ int mlp_adam(csg_engine* eng, float lr)
                                                     eng->m_n, lr, 0.9f, 0.999f, 1e-8f, c1, c2,
                                                     eng->d_mwb);
     LAUNCHCHK();
-    return CSG_OK;
+    return mlp_transpose(eng);
 }
 
 int mlp_check(csg_engine* eng)
@@ -959,7 +974,7 @@ int mlp_propose(csg_engine* eng, uint32_t nq, int K, int32_t* top, float* margin
     {
 #define CSG_TOPK(k)                                                                                \
     case k:                                                                                        \
-        k_topk<k><<<nblk(nq), 256, 0, eng->stream>>>(eng->d_macc, b3, (int)nq, L, Lp, top, margin); \
+        k_topk<k><<<(nq + 3) / 4, 256, 0, eng->stream>>>(eng->d_macc, b3, (int)nq, L, Lp, top, margin); \
         break;
         CSG_TOPK(1)
         CSG_TOPK(2)
@@ -1226,8 +1241,9 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     int rc;
     if (prop && (rc = mlp_check(eng)))
         return rc;
-    if (prop && (uint32_t)eng->prop_k > eng->L)
-        return eng->fail(CSG_ERR_ARGUMENT, "proposal size exceeds the label count");
+    if (prop && ((uint32_t)eng->prop_k > eng->L || eng->L > 256))
+        return eng->fail(CSG_ERR_ARGUMENT, "proposal size exceeds the label count, or more than "
+                                           "256 labels");
     if ((rc = set_pct(eng, &eng->d_pct, &eng->pct_quantile, eng->ctidx, eng->L, quantile)))
         return rc;
     if (prop && !eng->d_pcells && (rc = alloc_proposal(eng)))
@@ -2424,6 +2440,7 @@ void free_mlp(csg_engine* eng)
     dfree(eng->d_mm);
     dfree(eng->d_mv);
     dfree(eng->d_mwb);
+    dfree(eng->d_mwbT);
     dfree(eng->d_mx);
     dfree(eng->d_macc);
     dfree(eng->d_mh1);
@@ -2452,6 +2469,9 @@ int mlp_load(csg_engine* eng, const float* w)
     HIPCHK(hipMemsetAsync(eng->d_mg, 0, eng->m_n * sizeof(float), eng->stream));
     k_to_bf16<<<nblk(eng->m_n), 256, 0, eng->stream>>>(eng->d_mw, eng->m_n, eng->d_mwb);
     LAUNCHCHK();
+    int rc = mlp_transpose(eng);
+    if (rc)
+        return rc;
     HIPCHK(hipStreamSynchronize(eng->stream));
     eng->m_step = 0;
     return CSG_OK;
@@ -2488,7 +2508,7 @@ int csg_mlp_init(csg_engine* eng, int hidden, uint64_t seed)
     int rc;
     if ((rc = dalloc(eng, &eng->d_mw, off)) || (rc = dalloc(eng, &eng->d_mg, off)) ||
         (rc = dalloc(eng, &eng->d_mm, off)) || (rc = dalloc(eng, &eng->d_mv, off)) ||
-        (rc = dalloc(eng, &eng->d_mwb, off)))
+        (rc = dalloc(eng, &eng->d_mwb, off)) || (rc = dalloc(eng, &eng->d_mwbT, off)))
         return rc;
     // He-normal hidden layers, variance 1 / fan_in at the output layer, zero
     // biases; padded input rows and output columns stay zero

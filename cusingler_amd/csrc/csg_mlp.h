@@ -14,6 +14,18 @@ namespace csg {
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float  v4f_m __attribute__((ext_vector_type(4)));
 
+__device__ inline float gelu_f(float z)
+{
+    return 0.5f * z * (1.f + erff(z * 0.70710678118654752f));
+}
+
+__device__ inline float gelu_grad(float z)
+{
+    const float cdf = 0.5f * (1.f + erff(z * 0.70710678118654752f));
+    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
+    return cdf + z * pdf;
+}
+
 // C[M x N] (FP32, row-major, ldc) = sum_k A(m, k) * B(k, n) with
 //   A(m, k) = TA ? A[k * lda + m] : A[m * lda + k]
 //   B(k, n) = TB ? B[n * ldb + k] : B[k * ldb + n]
@@ -127,30 +139,128 @@ __global__ __launch_bounds__(256) void k_bgemm(const __bf16* __restrict__ A, int
         }
 }
 
-__device__ inline float gelu_f(float z)
+// Forward layer product C = A B^T with both operands k-contiguous:
+// A [M x K] (lda), B [N x K] (ldb, the layer's transposed BF16 weights),
+// K % 32 == 0, lda and ldb multiples of 8.  Block tile 128 x 128, four waves
+// of 64 x 64 (4 x 4 MFMA tiles each), K staged 32 at a time through two LDS
+// buffers.  Epilogue (row-major, leading dimension ldc):
+//   MODE 0: C = acc (FP32);
+//   MODE 1: v = acc + bias[n]; z = v (FP32, when given); H = gelu(v) (BF16).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_bgemm_nt(const __bf16* __restrict__ A, int lda,
+                                                  const __bf16* __restrict__ B, int ldb, int M,
+                                                  int N, int K, int ldc, float* __restrict__ C,
+                                                  const float* __restrict__ bias,
+                                                  float* __restrict__ z, __bf16* __restrict__ H)
 {
-    return 0.5f * z * (1.f + erff(z * 0.70710678118654752f));
+    constexpr int T = 128, BK = 32, LD = 40;
+    __shared__ __attribute__((aligned(16))) __bf16 sa[2][T * LD];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[2][T * LD];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = wid >> 1, wn = wid & 1;
+    const int m0 = blockIdx.y * T, n0 = blockIdx.x * T;
+    const int r0 = tid >> 2, kc = (tid & 3) * 8;  // rows r0 and r0 + 64, 8 values at kc
+    int4      va[2], vb[2];
+    auto      load = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            const int gm = m0 + r0 + 64 * u, gn = n0 + r0 + 64 * u;
+            va[u] = gm < M ? *(const int4*)(A + (size_t)gm * lda + k0 + kc) : int4{ 0, 0, 0, 0 };
+            vb[u] = gn < N ? *(const int4*)(B + (size_t)gn * ldb + k0 + kc) : int4{ 0, 0, 0, 0 };
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            *(int4*)(&sa[buf][(r0 + 64 * u) * LD + kc]) = va[u];
+            *(int4*)(&sb[buf][(r0 + 64 * u) * LD + kc]) = vb[u];
+        }
+    };
+    v4f_m acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            acc[i][j] = v4f_m{ 0.f, 0.f, 0.f, 0.f };
+    load(0);
+    store(0);
+    __syncthreads();
+    const int r16 = lane & 15, kofs = (lane >> 4) * 8;
+    int       buf = 0;
+    for (int k0 = 0; k0 < K; k0 += BK)
+    {
+        const bool more = k0 + BK < K;
+        if (more)
+            load(k0 + BK);
+        v8bf a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            a[i] = *(const v8bf*)(&sa[buf][(wm * 64 + i * 16 + r16) * LD + kofs]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            b[j] = *(const v8bf*)(&sb[buf][(wn * 64 + j * 16 + r16) * LD + kofs]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (more)
+        {
+            // buffer buf ^ 1 was last read before the previous barrier
+            store(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + e;
+            if (m >= M)
+                continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int n = n0 + wn * 64 + j * 16 + r16;
+                if (n >= N)
+                    continue;
+                const size_t o = (size_t)m * ldc + n;
+                if constexpr (MODE == 0)
+                    C[o] = acc[i][j][e];
+                else
+                {
+                    const float v = acc[i][j][e] + bias[n];
+                    if (z)
+                        z[o] = v;
+                    H[o] = (__bf16)gelu_f(v);
+                }
+            }
+        }
 }
 
-__device__ inline float gelu_grad(float z)
+// dst [C x R] (BF16) = transpose of src [R x C] (FP32).
+__global__ void k_transpose_bf16(const float* __restrict__ src, int R, int Cn,
+                                 __bf16* __restrict__ dst)
 {
-    const float cdf = 0.5f * (1.f + erff(z * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
-    return cdf + z * pdf;
-}
-
-// z = acc + bias (kept for the backward pass when z is given), h = gelu(z)
-// in BF16.
-__global__ void k_bias_gelu(const float* __restrict__ acc, const float* __restrict__ bias,
-                            int rows, int cols, float* __restrict__ z, __bf16* __restrict__ h)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)rows * cols)
-        return;
-    const float v = acc[i] + bias[i % cols];
-    if (z)
-        z[i] = v;
-    h[i]          = (__bf16)gelu_f(v);
+    __shared__ float tile[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+    for (int y = ty; y < 32; y += 8)
+    {
+        const int r = r0 + y, c = c0 + tx;
+        tile[y][tx] = r < R && c < Cn ? src[(size_t)r * Cn + c] : 0.f;
+    }
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8)
+    {
+        const int c = c0 + y, r = r0 + tx;
+        if (c < Cn && r < R)
+            dst[(size_t)c * R + r] = (__bf16)tile[tx][y];
+    }
 }
 
 // dz = dh * gelu'(z), FP32 for the bias gradient and BF16 for the next product.
@@ -243,49 +353,88 @@ __global__ void k_rank_features(const int16_t* __restrict__ X, size_t n, float i
 }
 
 // Per row: the K largest logits (+ bias) in descending order (ties keep the
-// lower label first) and the softmax margin p(top1) - p(top2).
+// lower label first) and the softmax margin p(top1) - p(top2); one wave per
+// row, L <= 256 logits (four per lane).  A row with a NaN logit gets margin
+// NaN and labels -1.
 template <int K>
-__global__ void k_topk(const float* __restrict__ acc, const float* __restrict__ bias, int rows,
-                       int L, int ldl, int32_t* __restrict__ top, float* __restrict__ margin)
+__global__ __launch_bounds__(256) void k_topk(const float* __restrict__ acc,
+                                              const float* __restrict__ bias, int rows, int L,
+                                              int ldl, int32_t* __restrict__ top,
+                                              float* __restrict__ margin)
 {
-    constexpr int KK = K < 2 ? 2 : K;  // the margin needs the runner-up
-    const int     r  = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int KK   = K < 2 ? 2 : K;  // the margin needs the runner-up
+    const int     lane = threadIdx.x & 63;
+    const int     r    = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= rows)
         return;
     const float* a = acc + (size_t)r * ldl;
-    float        best[KK];
-    int          idx[KK];
+    float        v[4];
+    bool         nan = false;
 #pragma unroll
+    for (int u = 0; u < 4; ++u)
+    {
+        const int j = lane + 64 * u;
+        v[u]        = j < L ? a[j] + bias[j] : -__builtin_inff();
+        nan |= __builtin_isnan(v[u]);
+    }
+    if (__ballot(nan))
+    {
+        if (lane < K)
+            top[(size_t)r * K + lane] = -1;
+        if (lane == 0)
+            margin[r] = __builtin_nanf("");
+        return;
+    }
+    float mx = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
+    for (int o = 32; o > 0; o >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, o));
+    float s = 0.f;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        s += lane + 64 * u < L ? expf(v[u] - mx) : 0.f;
+    for (int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o);
+    float b0 = 0.f, b1 = 0.f;
     for (int t = 0; t < KK; ++t)
     {
-        best[t] = -3.0e38f;
-        idx[t]  = -1;
-    }
-    float mx = -3.0e38f;
-    for (int j = 0; j < L; ++j)
-        mx = fmaxf(mx, a[j] + bias[j]);
-    float s = 0.f;
-    for (int j = 0; j < L; ++j)
-    {
-        const float v = a[j] + bias[j];
-        s += expf(v - mx);
-        int p = KK;
-        while (p > 0 && v > best[p - 1])
-            --p;
-        if (p < KK)
+        // the largest remaining value, lowest label among equals
+        float bv = -__builtin_inff();
+        int   bj = 1 << 30;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
         {
-            for (int t = KK - 1; t > p; --t)
+            const int j = lane + 64 * u;
+            if (j < L && (v[u] > bv || (v[u] == bv && j < bj)))
             {
-                best[t] = best[t - 1];
-                idx[t]  = idx[t - 1];
+                bv = v[u];
+                bj = j;
             }
-            best[p] = v;
-            idx[p]  = j;
         }
+        for (int o = 32; o > 0; o >>= 1)
+        {
+            const float ov = __shfl_xor(bv, o);
+            const int   oj = __shfl_xor(bj, o);
+            if (ov > bv || (ov == bv && oj < bj))
+            {
+                bv = ov;
+                bj = oj;
+            }
+        }
+        if (bj >= L)
+            bj = -1;  // fewer than KK labels
+        if (t < K && lane == 0)
+            top[(size_t)r * K + t] = bj;
+        if (t == 0)
+            b0 = bv;
+        if (t == 1)
+            b1 = bv;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (lane + 64 * u == bj)
+                v[u] = -__builtin_inff() * 0.f;  // taken: NaN never wins a comparison
     }
-    for (int t = 0; t < K; ++t)
-        top[(size_t)r * K + t] = idx[t];
-    margin[r] = L > 1 ? (expf(best[0] - mx) - expf(best[1] - mx)) / s : 1.f;
+    if (lane == 0)
+        margin[r] = L > 1 ? (expf(b0 - mx) - expf(b1 - mx)) / s : 1.f;
 }
 
 // Rows perm[0, rows) of a [* x cols] BF16 matrix (cols a multiple of 8) and

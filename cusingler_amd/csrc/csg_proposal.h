@@ -42,22 +42,43 @@ __device__ inline bool prop_confident(const int32_t* top, const float* margin, u
 
 // First stage: every confident cell takes a slot in the lists of its first
 // K ranked labels; the other cells are appended to the fallback list.
-// cnt[0, L) counts per label, cnt[L] the fallback cells.
-__global__ void k_prop_count(const int32_t* __restrict__ top, const float* __restrict__ margin,
-                             uint32_t nq, int KP, int K, uint32_t L, float thr,
-                             uint32_t* __restrict__ cnt, uint32_t* __restrict__ slot,
-                             uint32_t* __restrict__ fb)
+// cnt[0, L) counts per label, cnt[L] the fallback cells (L <= 256).  Slots
+// are taken in LDS first and each block adds its per-label totals to the
+// global counters once.
+__global__ __launch_bounds__(256) void k_prop_count(const int32_t* __restrict__ top,
+                                                    const float* __restrict__ margin, uint32_t nq,
+                                                    int KP, int K, uint32_t L, float thr,
+                                                    uint32_t* __restrict__ cnt,
+                                                    uint32_t* __restrict__ slot,
+                                                    uint32_t* __restrict__ fb)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nq)
-        return;
-    if (prop_confident(top, margin, c, KP, L, thr))
-    {
+    __shared__ uint32_t hist[257], base[257];
+    for (uint32_t i = threadIdx.x; i <= L; i += blockDim.x)
+        hist[i] = 0;
+    __syncthreads();
+    const uint32_t c    = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool     live = c < nq;
+    const bool     conf = live && prop_confident(top, margin, c, KP, L, thr);
+    uint32_t       ls[8];
+    int            lab[8];
+    if (conf)
         for (int t = 0; t < K; ++t)
-            slot[(size_t)c * KP + t] = atomicAdd(&cnt[top[(size_t)c * KP + t]], 1u);
-    }
-    else
-        fb[atomicAdd(&cnt[L], 1u)] = c;
+        {
+            lab[t] = top[(size_t)c * KP + t];
+            ls[t]  = atomicAdd(&hist[lab[t]], 1u);
+        }
+    else if (live)
+        ls[0] = atomicAdd(&hist[L], 1u);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i <= L; i += blockDim.x)
+        if (hist[i])
+            base[i] = atomicAdd(&cnt[i], hist[i]);
+    __syncthreads();
+    if (conf)
+        for (int t = 0; t < K; ++t)
+            slot[(size_t)c * KP + t] = base[lab[t]] + ls[t];
+    else if (live)
+        fb[base[L] + ls[0]] = c;
 }
 
 // Work tables of the cells scored on a window [t0, t1) of their ranked
