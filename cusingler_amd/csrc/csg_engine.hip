@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <random>
 #include <string>
 #include <unordered_map>
@@ -23,6 +24,7 @@
 #include "../../include/cusingler_amd.h"
 #include "csg_exact.h"
 #include "csg_ft_kernels.h"
+#include "csg_ftd.h"
 #include "csg_kernels.h"
 #include "csg_mfma.h"
 #include "csg_mlp.h"
@@ -49,10 +51,6 @@ int64_t now_us()
 }
 
 // fine-tune correlation tile: 32 cells x 32 reference rows, 2 x 2 per thread
-constexpr int FT_TC = 32, FT_TR = 32, FT_RC = 2, FT_RR = 2;
-constexpr int FT_NT = (FT_TC / FT_RC) * (FT_TR / FT_RR);
-// groups of at most FT_THIN cells use the thin tile (FT_THIN cells x 256 rows)
-constexpr int FT_THIN = 8;
 
 }  // namespace
 
@@ -132,6 +130,47 @@ struct csg_engine
     size_t    d_ft_redo_cap = 0, h_ft_redo_cap = 0;
     bool      ft_exact = false;  // exact (certified) fine-tune contraction
     std::unordered_map<uint64_t, std::vector<uint64_t>> window_cache;  // pair gene windows
+
+    // device-resident fine-tune rounds (schedule 0)
+    int       ft_schedule = 0;  // 0: device-resident rounds, 1: host-grouped batches
+    bool      ftd_static  = false;
+    uint8_t*  d_fl_lists = nullptr;
+    uint32_t *d_fl_counts = nullptr, *d_fl_act = nullptr, *d_fl_next = nullptr;
+    int32_t*  d_fl_rounds = nullptr;
+    uint32_t *d_fl_cslot = nullptr, *d_fl_cgid = nullptr, *d_fl_cpos = nullptr;
+    uint64_t* d_fl_htab = nullptr;
+    uint32_t *d_fl_howner = nullptr, *d_fl_hgid = nullptr, *d_fl_gowner = nullptr,
+             *d_fl_gcount = nullptr;
+    FtGroup*  d_fl_groups = nullptr;
+    uint8_t*  d_fl_gflag  = nullptr;
+    uint32_t *d_fl_glabels = nullptr, *d_fl_glpos = nullptr, *d_fl_ggenes = nullptr,
+             *d_fl_gtile0 = nullptr, *d_fl_gthin0 = nullptr;
+    uint64_t*  d_fl_gseg0 = nullptr;
+    FtdCounts* d_fl_cnt   = nullptr;
+    FtdCounts* h_fl_cnt   = nullptr;  // pinned
+    uint32_t * d_fl_cut = nullptr, *d_fl_ctdidx = nullptr, *d_fl_ctdiff = nullptr,
+             *d_fl_final = nullptr;
+    float*      d_fl_mask  = nullptr;
+    FtRankItem* d_fl_items = nullptr;
+    FtCell*     d_fl_cells = nullptr;
+    FtCorrTile *d_fl_tiles = nullptr, *d_fl_thin = nullptr, *d_fl_rthin = nullptr;
+    uint8_t*    d_fl_stage = nullptr;
+    uint32_t*   d_fl_redo  = nullptr;
+    uint64_t *  d_fl_rseg = nullptr, *d_fl_nan1 = nullptr, *d_fl_nan2 = nullptr;
+    float*      d_fl_pred = nullptr;
+    struct
+    {
+        size_t items, cells, stage, redo, tiles, thin, rthin, rseg, pred, nan1, nan2;
+    } fl_cap{};
+    uint32_t               ftd_gcap = 0, ftd_T = 0;
+    size_t                 ftd_gstride = 0;
+    hipGraphExec_t         ftd_gA = nullptr, ftd_gB = nullptr;
+    std::vector<uintptr_t> ftd_sigA, ftd_sigB;
+    // launch settings of the work-queue kernels (blocks), tunable
+    int   ftd_grid_threads = 1024, ftd_grid_setup = 512, ftd_grid_rank = 2048,
+        ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_thin_max = 8;
+    bool  ftd_autotune = true, ftd_tuned = false;
+    float ftd_tune_ms[3] = { 0.f, 0.f, 0.f };  // best times of the tuned kernels
 
     // ---- learned proposal ----
     // Network G_s -> H -> H -> L (GELU), parameters in one FP32 array
@@ -241,6 +280,7 @@ template <class T> int ensure(csg_engine* eng, T** p, size_t* cap, size_t n)
 
 void free_proposal(csg_engine* eng);
 void free_mlp(csg_engine* eng);
+int ftd_free(csg_engine* eng);
 
 hipEvent_t take_event(csg_engine* eng)
 {
@@ -447,9 +487,22 @@ int csg_init_gpu(int gpuid, csg_engine** out)
         const bool ok2  = hipFuncSetAttribute((const void*)k_ft_rank_qry,
                                               hipFuncAttributeMaxDynamicSharedMemorySize,
                                               want) == hipSuccess;
-        if (ok1 && ok2)
+        bool       ok3  = true;
+        for (const void* f : { (const void*)k_ftd_rank_ref, (const void*)k_ftd_rank_qry,
+                               (const void*)k_ftd_corr })
+            ok3 = ok3 && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             want) == hipSuccess;
+        if (ok1 && ok2 && ok3)
             eng->lds_limit = (size_t)want;
         (void)hipGetLastError();  // a refused attribute must not surface at the next launch
+    }
+    {
+        const int cu          = std::max(1, eng->prop.multiProcessorCount);
+        eng->ftd_grid_threads = cu * 4;
+        eng->ftd_grid_setup   = cu * 4;
+        eng->ftd_grid_rank    = cu * 8;
+        eng->ftd_grid_corr    = cu * 2;
+        eng->ftd_grid_select  = cu * 8;
     }
     if (dalloc(eng, &eng->d_nan_count, 1) || dalloc(eng, &eng->d_status, 1))
     {
@@ -543,6 +596,7 @@ int csg_destroy(csg_engine* eng)
         (void)hipHostFree(eng->h_ft_out);
     eng->h_ft_stage = eng->h_ft_out = nullptr;
     eng->h_ft_stage_cap = eng->h_ft_out_cap = 0;
+    ftd_free(eng);
     eng->fpct_quantile = -1.f;
     eng->ft_loaded     = false;
     eng->window_cache.clear();
@@ -1904,66 +1958,570 @@ struct SetKeyHash
 
 }  // namespace
 
-int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
-                 uint32_t* final_labels, int32_t* rounds_out)
+}  // extern "C"
+
+namespace {
+
+// ------------------------------------------------- device-resident rounds
+
+constexpr int FTD_TPB = 256;  // threads of the thread-level work-queue kernels
+
+int ftd_free(csg_engine* eng)
 {
-    if (!eng)
-        return CSG_ERR_ARGUMENT;
-    if (!eng->ft_loaded)
-        return eng->fail(CSG_ERR_STATE, "csg_finetune before csg_copyin");
-    HIPCHK(hipSetDevice(eng->device));
-    const uint32_t L  = eng->fL;
-    const uint32_t Nq = eng->fNq;
-    int            rc = set_pct(eng, &eng->d_fpct, &eng->fpct_quantile, eng->fctidx, L, quantile);
-    if (rc)
+    if (eng->ftd_gA)
+        (void)hipGraphExecDestroy(eng->ftd_gA);
+    if (eng->ftd_gB)
+        (void)hipGraphExecDestroy(eng->ftd_gB);
+    eng->ftd_gA = eng->ftd_gB = nullptr;
+    eng->ftd_sigA.clear();
+    eng->ftd_sigB.clear();
+    dfree(eng->d_fl_lists);
+    dfree(eng->d_fl_counts);
+    dfree(eng->d_fl_rounds);
+    dfree(eng->d_fl_act);
+    dfree(eng->d_fl_next);
+    dfree(eng->d_fl_cslot);
+    dfree(eng->d_fl_cgid);
+    dfree(eng->d_fl_cpos);
+    dfree(eng->d_fl_htab);
+    dfree(eng->d_fl_howner);
+    dfree(eng->d_fl_hgid);
+    dfree(eng->d_fl_gowner);
+    dfree(eng->d_fl_gcount);
+    dfree(eng->d_fl_groups);
+    dfree(eng->d_fl_gflag);
+    dfree(eng->d_fl_glabels);
+    dfree(eng->d_fl_glpos);
+    dfree(eng->d_fl_ggenes);
+    dfree(eng->d_fl_gtile0);
+    dfree(eng->d_fl_gthin0);
+    dfree(eng->d_fl_gseg0);
+    dfree(eng->d_fl_cnt);
+    dfree(eng->d_fl_cut);
+    dfree(eng->d_fl_ctdidx);
+    dfree(eng->d_fl_ctdiff);
+    dfree(eng->d_fl_final);
+    dfree(eng->d_fl_mask);
+    dfree(eng->d_fl_items);
+    dfree(eng->d_fl_cells);
+    dfree(eng->d_fl_tiles);
+    dfree(eng->d_fl_thin);
+    dfree(eng->d_fl_rthin);
+    dfree(eng->d_fl_stage);
+    dfree(eng->d_fl_redo);
+    dfree(eng->d_fl_rseg);
+    dfree(eng->d_fl_pred);
+    dfree(eng->d_fl_nan1);
+    dfree(eng->d_fl_nan2);
+    eng->fl_cap = {};
+    eng->ftd_gcap = 0;
+    eng->ftd_T    = 0;
+    if (eng->h_fl_cnt)
+        (void)hipHostFree(eng->h_fl_cnt);
+    eng->h_fl_cnt = nullptr;
+    eng->ftd_static = false;
+    eng->ftd_tuned  = false;
+    return CSG_OK;
+}
+
+// Per-cell state, the pair windows and the k -> k_t table (sized by the
+// fine-tune inputs; kept until they change).
+int ftd_static(csg_engine* eng)
+{
+    if (eng->ftd_static)
+        return CSG_OK;
+    const uint32_t Nq = eng->fNq, L = eng->fL;
+    int            rc;
+    std::vector<uint32_t> cut(L + 1, 0u);
+    for (uint32_t k = 1; k <= L; ++k)
+        cut[k] = (uint32_t)std::round(500 * std::pow((2 / 3.0), std::log2((double)k)));
+    if ((rc = dalloc(eng, &eng->d_fl_lists, (size_t)Nq * L)) ||
+        (rc = dalloc(eng, &eng->d_fl_counts, Nq)) || (rc = dalloc(eng, &eng->d_fl_rounds, Nq)) ||
+        (rc = dalloc(eng, &eng->d_fl_act, Nq)) || (rc = dalloc(eng, &eng->d_fl_next, Nq)) ||
+        (rc = dalloc(eng, &eng->d_fl_cslot, Nq)) || (rc = dalloc(eng, &eng->d_fl_cgid, Nq)) ||
+        (rc = dalloc(eng, &eng->d_fl_cpos, Nq)) || (rc = dalloc(eng, &eng->d_fl_cnt, 1)) ||
+        (rc = dalloc(eng, &eng->d_fl_final, Nq)) ||
+        (rc = upload(eng, &eng->d_fl_cut, cut.data(), cut.size())) ||
+        (rc = upload(eng, &eng->d_fl_ctdidx, eng->ctdidx.data(), eng->ctdidx.size())) ||
+        (rc = upload(eng, &eng->d_fl_ctdiff, eng->ctdiff.data(), std::max<size_t>(eng->ctdiff.size(), 1))))
         return rc;
-    for (int i = 6; i < 16; ++i)  // the score step's own counters stay
-        eng->counters[i] = 0;
-    PhaseScope    total(eng, 5, 0);
-    const int64_t t_start = now_us();
-    // candidate lists from the mask, ascending label order (src/cusingler.cu:1307-1312)
-    std::vector<uint8_t>  lists((size_t)Nq * L);
-    std::vector<uint32_t> counts(Nq, 0);
-    std::vector<int32_t>  nrounds(Nq, 0);
+    if (!eng->h_fl_cnt)
+        HIPCHK(hipHostMalloc((void**)&eng->h_fl_cnt, sizeof(FtdCounts), hipHostMallocDefault));
+    eng->ftd_gstride = round_up(std::max<uint32_t>(eng->Gf, 1), 4);
+    eng->ftd_static  = true;
+    return CSG_OK;
+}
+
+// Group tables and the hash table for up to `n` active cells.
+int ftd_groups(csg_engine* eng, uint32_t n)
+{
+    if (n <= eng->ftd_gcap && eng->d_fl_groups)
+        return CSG_OK;
+    const uint32_t cap = std::max(n, eng->ftd_gcap + eng->ftd_gcap / 2);
+    uint32_t       T   = 1024;
+    while (T < 2 * cap)
+        T <<= 1;
+    const size_t L = eng->fL, gst = eng->ftd_gstride;
+    int          rc;
+    eng->ftd_gcap = 0;
+    if ((rc = dalloc(eng, &eng->d_fl_htab, T)) || (rc = dalloc(eng, &eng->d_fl_howner, T)) ||
+        (rc = dalloc(eng, &eng->d_fl_hgid, T)) || (rc = dalloc(eng, &eng->d_fl_gowner, cap)) ||
+        (rc = dalloc(eng, &eng->d_fl_gcount, cap)) || (rc = dalloc(eng, &eng->d_fl_groups, cap)) ||
+        (rc = dalloc(eng, &eng->d_fl_gflag, cap)) ||
+        (rc = dalloc(eng, &eng->d_fl_glabels, (size_t)cap * L)) ||
+        (rc = dalloc(eng, &eng->d_fl_glpos, (size_t)cap * L)) ||
+        (rc = dalloc(eng, &eng->d_fl_ggenes, (size_t)cap * gst)) ||
+        (rc = dalloc(eng, &eng->d_fl_gtile0, cap)) || (rc = dalloc(eng, &eng->d_fl_gthin0, cap)) ||
+        (rc = dalloc(eng, &eng->d_fl_gseg0, cap)))
+        return rc;
+    eng->ftd_gcap = cap;
+    eng->ftd_T    = T;
+    return CSG_OK;
+}
+
+// Work tables for one round's counts (grow-only).
+int ftd_work(csg_engine* eng, const FtdCounts& h)
+{
+    auto& c  = eng->fl_cap;
+    const size_t L = eng->fL;
+    int   rc;
+    if ((rc = ensure(eng, &eng->d_ft_rr, &eng->d_ft_rr_cap, std::max<uint64_t>(h.ref_elems, 8))) ||
+        (rc = ensure(eng, &eng->d_ft_rsq, &eng->d_ft_rsq_cap, std::max<uint64_t>(h.n_ref_items, 1))) ||
+        (rc = ensure(eng, &eng->d_ft_cr, &eng->d_ft_cr_cap, std::max<uint64_t>(h.cell_elems, 8))) ||
+        (rc = ensure(eng, &eng->d_ft_csq, &eng->d_ft_csq_cap, std::max<uint64_t>(h.n_cells, 1))) ||
+        (rc = ensure(eng, &eng->d_ft_scores, &eng->d_ft_scores_cap, std::max<uint64_t>(h.n_scores, 1))) ||
+        (rc = ensure(eng, &eng->d_ft_P, &eng->d_ft_P_cap, std::max<uint64_t>(h.n_segs, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_items, &c.items, std::max<uint64_t>(h.n_cells, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_cells, &c.cells, std::max<uint64_t>(h.n_cells, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_stage, &c.stage, std::max<uint64_t>((uint64_t)h.n_cells * L, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_redo, &c.redo, std::max<uint64_t>(h.n_cells, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_tiles, &c.tiles, std::max<uint64_t>(h.n_tiles, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_thin, &c.thin, std::max<uint64_t>(h.n_thin, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_rthin, &c.rthin, std::max<uint64_t>(h.rthin_bound, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_rseg, &c.rseg, std::max<uint64_t>(h.n_segs, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_pred, &c.pred, std::max<uint64_t>(h.n_segs, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_nan1, &c.nan1, std::max<uint64_t>(h.n_segs, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_nan2, &c.nan2, std::max<uint64_t>(h.n_segs, 1))))
+        return rc;
+    return CSG_OK;
+}
+
+// Dynamic LDS of the work-queue kernels for gene subsets of up to Gf genes.
+struct FtdLds
+{
+    size_t setup, rank_ref, rank_qry, corr, thin;
+    int    wave_ints;  // per-wave LDS of the wave-per-row reference rank kernel
+    size_t rank_ref_w;
+};
+
+FtdLds ftd_lds(const csg_engine* eng)
+{
+    const size_t st = round_up(std::max<uint32_t>(eng->Gf, 1), 8);
+    const size_t nb = (size_t)((eng->fnbins + 3) & ~3) * 4;
+    FtdLds       d;
+    d.setup    = ((size_t)std::max<uint32_t>(eng->Gf, 1) + 31) / 32 * 4;
+    d.rank_ref = nb + st * 4;
+    d.rank_qry = nb + st * 2 + st * 2 + st * 2;
+    d.corr     = (size_t)(FT_TC + FT_TR) * (st + 2) * 2;
+    d.thin     = (size_t)FT_THIN * st * 2;
+    d.wave_ints  = (int)(((eng->fnbins + 3) & ~3) + st / 2);
+    d.rank_ref_w = (size_t)4 * d.wave_ints * 4;
+    return d;
+}
+
+// Round part A: grouping, group set-up and the prefix sums.
+int ftd_launch_a(csg_engine* eng)
+{
+    const uint32_t L = eng->fL, T = eng->ftd_T;
+    const unsigned g = (unsigned)eng->ftd_grid_threads;
+    const FtdLds   d = ftd_lds(eng);
+    FtdCounts*     cnt = eng->d_fl_cnt;
+    k_ftd_begin<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_next,
+                                                (unsigned long long*)eng->d_fl_htab, T,
+                                                eng->d_fl_gcount);
+    k_ftd_counters<<<1, 64, 0, eng->stream>>>(cnt);
+    k_ftd_hash<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_lists,
+                                               eng->d_fl_counts, L,
+                                               (unsigned long long*)eng->d_fl_htab,
+                                               eng->d_fl_howner, T, eng->d_fl_cslot);
+    k_ftd_gid<<<g, FTD_TPB, 0, eng->stream>>>(cnt, (const unsigned long long*)eng->d_fl_htab,
+                                              eng->d_fl_howner, T, eng->d_fl_hgid,
+                                              eng->d_fl_gowner);
+    k_ftd_member<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_lists,
+                                                 eng->d_fl_counts, L, eng->d_fl_cslot,
+                                                 eng->d_fl_hgid, eng->d_fl_gowner,
+                                                 eng->d_fl_gcount, eng->d_fl_cgid, eng->d_fl_cpos);
+    k_ftd_setup<<<(unsigned)eng->ftd_grid_setup, FTD_SETUP_NT, d.setup, eng->stream>>>(
+        cnt, eng->d_fl_gowner, eng->d_fl_gcount, eng->d_fl_lists, eng->d_fl_counts, L,
+        eng->d_fl_ctdidx, eng->d_fl_ctdiff, eng->d_fl_cut, eng->d_fllen,
+        std::max<uint32_t>(eng->Gf, 1), (uint32_t)eng->ftd_gstride, eng->ft_exact ? 1 : 0,
+        eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_fl_glpos, eng->d_fl_ggenes);
+    k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
+                                                   (uint32_t)eng->ftd_thin_max, eng->d_fl_gtile0,
+                                                   eng->d_fl_gthin0, eng->d_fl_gseg0);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// Reference-row ranks: one wave per row when the per-wave LDS fits, else
+// one block per row.
+int ftd_launch_rank_ref(csg_engine* eng, int grid)
+{
+    const FtdLds   d   = ftd_lds(eng);
+    const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1);
+    FtdCounts*     cnt = eng->d_fl_cnt;
+    if (d.rank_ref_w <= 65536)
+        k_ftd_rank_ref_w<<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
+            cnt, eng->d_fl_groups, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
+            eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
+            eng->d_ft_rsq);
+    else
+        k_ftd_rank_ref<<<(unsigned)grid, 128, d.rank_ref, eng->stream>>>(
+            cnt, eng->d_fl_groups, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
+            eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, eng->d_ft_rr, eng->d_ft_rsq);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+int ftd_launch_corr(csg_engine* eng, int grid)
+{
+    const FtdLds d   = ftd_lds(eng);
+    FtdCounts*   cnt = eng->d_fl_cnt;
+    k_ftd_mfma<<<(unsigned)grid, 256, 0, eng->stream>>>(
+        cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
+        eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
+    k_ftd_corr<<<(unsigned)grid, FT_NT, d.corr, eng->stream>>>(
+        cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
+        eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
+    k_ftd_thin<<<(unsigned)grid, 256, d.thin, eng->stream>>>(
+        cnt, 0, eng->d_fl_thin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
+        eng->d_ft_rsq, eng->d_ft_scores);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+int ftd_launch_select(csg_engine* eng, int grid, float quantile)
+{
+    FtdCounts*        cnt = eng->d_fl_cnt;
+    const FtdSegments seg{ eng->d_fl_cells, &cnt->n_cells, eng->d_fl_groups, eng->d_fl_glabels,
+                           eng->d_fl_glpos, eng->d_fllen };
+    k_ftd_select<FtdSegments><<<(unsigned)grid, 256, 0, eng->stream>>>(
+        cnt, 0, seg, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P, eng->d_fl_nan1);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// Round part B: ranks, correlations, quantiles, filters, the redo of the
+// undecided exact-form cells and the next round's active list.
+int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
+{
+    const uint32_t   L   = eng->fL;
+    const uint32_t   Gf  = std::max<uint32_t>(eng->Gf, 1);
+    const unsigned   g   = (unsigned)eng->ftd_grid_threads;
+    const FtdLds     d   = ftd_lds(eng);
+    FtdCounts*       cnt = eng->d_fl_cnt;
+    const FtdSegments seg{ eng->d_fl_cells, &cnt->n_cells, eng->d_fl_groups, eng->d_fl_glabels,
+                           eng->d_fl_glpos, eng->d_fllen };
+    const FtdListSegments rseg{ seg, eng->d_fl_rseg };
+    k_ftd_cells<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_cgid, eng->d_fl_cpos,
+                                                eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_gseg0,
+                                                eng->d_fl_counts, eng->d_fl_rounds,
+                                                eng->d_fl_items, eng->d_fl_cells);
+    if (ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
+        return CSG_ERR_DEVICE;
+    k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
+        cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
+        eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
+    k_ftd_tiles<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
+                                                (uint32_t)eng->ftd_thin_max, eng->d_fl_gtile0,
+                                                eng->d_fl_gthin0, eng->d_fl_tiles, eng->d_fl_thin);
+    if (ftd_launch_corr(eng, eng->ftd_grid_corr) ||
+        ftd_launch_select(eng, eng->ftd_grid_select, quantile))
+        return CSG_ERR_DEVICE;
+    k_ftd_select_nan<FtdSegments><<<g, FTD_TPB, 0, eng->stream>>>(
+        cnt, 0, seg, eng->d_fl_nan1, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P,
+        eng->d_fl_cells, eng->d_fl_gflag);
+    k_ftd_filter<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_cells, eng->d_fl_groups,
+                                                 eng->d_fl_gflag, eng->d_fl_glabels, eng->d_ft_P,
+                                                 eng->d_fl_lists, eng->d_fl_counts, L, thre,
+                                                 eng->d_fl_stage, eng->d_fl_redo,
+                                                 eng->d_fl_rthin, eng->d_fl_rseg);
+    // FP32 redo of the undecided exact-form cells
+    k_ftd_thin<<<(unsigned)eng->ftd_grid_corr, 256, d.thin, eng->stream>>>(
+        cnt, 1, eng->d_fl_rthin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
+        eng->d_ft_rsq, eng->d_ft_scores);
+    k_ftd_select<FtdListSegments><<<(unsigned)eng->ftd_grid_select, 256, 0, eng->stream>>>(
+        cnt, 1, rseg, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_fl_pred, eng->d_fl_nan2);
+    k_ftd_select_nan<FtdListSegments><<<g, FTD_TPB, 0, eng->stream>>>(
+        cnt, 1, rseg, eng->d_fl_nan2, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_fl_pred,
+        eng->d_fl_cells, eng->d_fl_gflag);
+    k_ftd_redo_scatter<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_rseg, eng->d_fl_pred,
+                                                       eng->d_ft_P);
+    k_ftd_filter_redo<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_redo, eng->d_fl_cells,
+                                                      eng->d_fl_groups, eng->d_fl_glabels,
+                                                      eng->d_ft_P, eng->d_fl_lists,
+                                                      eng->d_fl_counts, L, thre,
+                                                      eng->d_fl_stage);
+    k_ftd_compact<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_counts,
+                                                  eng->d_fl_rounds, times, eng->d_fl_next);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// Autotuner of the work-queue launch settings.  On the first device round
+// after the fine-tune inputs change (with enough work to time), each
+// candidate block count of the reference-rank, correlation and select
+// kernels is timed on that round's own work -- the kernels are pure
+// functions of their inputs, so repeating them changes nothing -- and the
+// fastest count is kept for the graphs.  The round's cell, query-rank and
+// tile kernels run first (they are idempotent too); the graph then runs the
+// whole round.
+float ftd_time(csg_engine* eng, const std::function<int()>& launch, int reps)
+{
+    hipEvent_t a = take_event(eng), b = take_event(eng);
+    float      best = 1e30f;
+    for (int r = 0; r < reps; ++r)
+    {
+        (void)hipEventRecord(a, eng->stream);
+        if (launch())
+        {
+            best = 1e30f;
+            break;
+        }
+        (void)hipEventRecord(b, eng->stream);
+        float ms = 0.f;
+        if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess)
+            best = std::min(best, ms);
+    }
+    eng->event_pool.push_back(a);
+    eng->event_pool.push_back(b);
+    return best;
+}
+
+int ftd_tune(csg_engine* eng, float quantile)
+{
+    const int      cu  = std::max(1, eng->prop.multiProcessorCount);
+    const unsigned g   = (unsigned)eng->ftd_grid_threads;
+    const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1);
+    const FtdLds   d   = ftd_lds(eng);
+    FtdCounts*     cnt = eng->d_fl_cnt;
+    k_ftd_cells<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_cgid, eng->d_fl_cpos,
+                                                eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_gseg0,
+                                                eng->d_fl_counts, eng->d_fl_rounds,
+                                                eng->d_fl_items, eng->d_fl_cells);
+    k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
+        cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
+        eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
+    k_ftd_tiles<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
+                                                (uint32_t)eng->ftd_thin_max, eng->d_fl_gtile0,
+                                                eng->d_fl_gthin0, eng->d_fl_tiles, eng->d_fl_thin);
+    LAUNCHCHK();
+    auto pick = [&](const std::vector<int>& mult, const std::function<int(int)>& run, int* grid,
+                    float* best_ms) {
+        float best = 1e30f;
+        int   bg   = *grid;
+        for (int m : mult)
+        {
+            const float t = ftd_time(eng, [&]() { return run(m * cu); }, 2);
+            if (t < best)
+            {
+                best = t;
+                bg   = m * cu;
+            }
+        }
+        *grid    = bg;
+        *best_ms = best;
+    };
+    pick({ 2, 4, 8, 16, 32 }, [&](int gr) { return ftd_launch_rank_ref(eng, gr); },
+         &eng->ftd_grid_rank, &eng->ftd_tune_ms[0]);
+    (void)ftd_launch_rank_ref(eng, eng->ftd_grid_rank);
+    pick({ 1, 2, 3, 4, 6 }, [&](int gr) { return ftd_launch_corr(eng, gr); }, &eng->ftd_grid_corr,
+         &eng->ftd_tune_ms[1]);
+    pick({ 2, 4, 8, 16 },
+         [&](int gr) {
+             k_ftd_reset_nan<<<1, 64, 0, eng->stream>>>(cnt);
+             return ftd_launch_select(eng, gr, quantile);
+         },
+         &eng->ftd_grid_select, &eng->ftd_tune_ms[2]);
+    k_ftd_reset_nan<<<1, 64, 0, eng->stream>>>(cnt);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    eng->ftd_tuned = true;
+    return CSG_OK;
+}
+
+// Captures `launch` into a graph (re-captured when any pointer or launch
+// setting in `sig` changed).
+template <class F>
+int ftd_graph(csg_engine* eng, hipGraphExec_t* exec, std::vector<uintptr_t>* have,
+              const std::vector<uintptr_t>& sig, F launch)
+{
+    if (*exec && *have == sig)
+        return CSG_OK;
+    if (*exec)
+        (void)hipGraphExecDestroy(*exec);
+    *exec = nullptr;
+    have->clear();
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(eng->stream, hipStreamCaptureModeThreadLocal));
+    const int  rc = launch();
+    hipError_t e  = hipStreamEndCapture(eng->stream, &graph);
+    if (rc)
+    {
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess)
+        return eng->fail(CSG_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+    {
+        *exec = nullptr;
+        return eng->fail(CSG_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    *have = sig;
+    return CSG_OK;
+}
+
+std::vector<uintptr_t> ftd_sig_a(const csg_engine* eng)
+{
+    return { (uintptr_t)eng->d_fl_htab, (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes,
+             (uintptr_t)eng->d_fl_glabels, (uintptr_t)eng->d_fl_gseg0, (uintptr_t)eng->ftd_T,
+             (uintptr_t)eng->ftd_grid_setup, (uintptr_t)eng->ftd_thin_max,
+             (uintptr_t)eng->ft_exact, (uintptr_t)eng->ftd_grid_threads };
+}
+
+std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float thre, int times)
+{
+    uint32_t q, t;
+    std::memcpy(&q, &quantile, 4);
+    std::memcpy(&t, &thre, 4);
+    return { (uintptr_t)eng->d_ft_rr, (uintptr_t)eng->d_ft_rsq, (uintptr_t)eng->d_ft_cr,
+             (uintptr_t)eng->d_ft_csq, (uintptr_t)eng->d_ft_scores, (uintptr_t)eng->d_ft_P,
+             (uintptr_t)eng->d_fl_items, (uintptr_t)eng->d_fl_cells, (uintptr_t)eng->d_fl_stage,
+             (uintptr_t)eng->d_fl_redo, (uintptr_t)eng->d_fl_tiles, (uintptr_t)eng->d_fl_thin,
+             (uintptr_t)eng->d_fl_rthin, (uintptr_t)eng->d_fl_rseg, (uintptr_t)eng->d_fl_pred,
+             (uintptr_t)eng->d_fl_nan1, (uintptr_t)eng->d_fl_nan2, (uintptr_t)eng->d_fpct,
+             (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)q,
+             (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,
+             (uintptr_t)eng->ftd_grid_corr, (uintptr_t)eng->ftd_grid_select,
+             (uintptr_t)eng->ftd_thin_max, (uintptr_t)eng->ftd_grid_threads };
+}
+
+// The whole fine-tune on the device.  Returns CSG_OK with *fallback set when
+// a hash collision was met: lists / counts / rounds then hold the state at
+// the start of that round for the host-grouped schedule to continue from.
+int finetune_device(csg_engine* eng, float quantile, float thre, int times,
+                    uint32_t* final_labels, int32_t* rounds_out, bool* fallback,
+                    std::vector<uint8_t>& lists, std::vector<uint32_t>& counts,
+                    std::vector<int32_t>& nrounds)
+{
+    *fallback         = false;
+    const uint32_t Nq = eng->fNq, L = eng->fL;
+    int            rc;
+    const FtdLds   d = ftd_lds(eng);
+    if (d.rank_qry > eng->lds_limit || d.corr > eng->lds_limit || d.thin > 65536 ||
+        d.setup > 65536 || d.rank_ref > eng->lds_limit)
+        return eng->fail(CSG_ERR_ARGUMENT, "fine-tune gene set exceeds the kernels' LDS plan");
+    if ((rc = ftd_static(eng)))
+        return rc;
+    const float* mask = nullptr;
     if (!eng->host_mask.empty())
     {
-        const float* mask = eng->host_mask.data();
-        for (uint32_t c = 0; c < Nq; ++c)
-            for (uint32_t l = 0; l < L; ++l)
-                if (mask[(size_t)c * L + l] != 0)
-                    lists[(size_t)c * L + counts[c]++] = (uint8_t)l;
+        if ((rc = upload(eng, &eng->d_fl_mask, eng->host_mask.data(), eng->host_mask.size())))
+            return rc;
+        mask = eng->d_fl_mask;
     }
     else
     {
         if (!eng->mask_valid || eng->Nq != Nq || eng->L != L)
             return eng->fail(CSG_ERR_STATE, "no candidate mask: pass one to csg_copyin or run "
                                             "csg_get_label on the same cells first");
-        const uint32_t W = (L + 63) / 64;
-        uint64_t*      d_bits = nullptr;
-        if ((rc = dalloc(eng, &d_bits, (size_t)Nq * W)))
-            return rc;
-        std::vector<uint64_t> bits((size_t)Nq * W);
-        k_pack_mask<<<(Nq + 255) / 256, 256, 0, eng->stream>>>(eng->d_mask, Nq, L, d_bits);
-        hipError_t e1 = hipGetLastError();
-        hipError_t e2 = hipMemcpyAsync(bits.data(), d_bits, bits.size() * 8, hipMemcpyDeviceToHost,
-                                       eng->stream);
-        hipError_t e3 = hipStreamSynchronize(eng->stream);
-        dfree(d_bits);
-        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
-            return eng->fail(CSG_ERR_DEVICE, "mask copy failed");
-        for (uint32_t c = 0; c < Nq; ++c)
-            for (uint32_t w = 0; w < W; ++w)
-            {
-                uint64_t v = bits[(size_t)c * W + w];
-                while (v)
-                {
-                    const int t = __builtin_ctzll(v);
-                    v &= v - 1;
-                    lists[(size_t)c * L + counts[c]++] = (uint8_t)(w * 64 + t);
-                }
-            }
+        mask = eng->d_mask;
     }
+    HIPCHK(hipMemsetAsync(eng->d_fl_cnt, 0, sizeof(FtdCounts), eng->stream));
+    k_ftd_init<<<nblk(Nq), 256, 0, eng->stream>>>(mask, Nq, L, times, eng->d_fl_lists,
+                                                 eng->d_fl_counts, eng->d_fl_rounds,
+                                                 eng->d_fl_next, eng->d_fl_cnt);
+    LAUNCHCHK();
+    FtdCounts& h = *eng->h_fl_cnt;
+    HIPCHK(hipMemcpyAsync(&h, eng->d_fl_cnt, sizeof(FtdCounts), hipMemcpyDeviceToHost,
+                          eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    if ((rc = ftd_groups(eng, std::max<uint32_t>(h.n_next, 1))))
+        return rc;
+    for (int round = 0; h.n_next > 0; ++round)
+    {
+        const int64_t t0 = now_us();
+        if ((rc = ftd_graph(eng, &eng->ftd_gA, &eng->ftd_sigA, ftd_sig_a(eng),
+                            [&]() { return ftd_launch_a(eng); })))
+            return rc;
+        HIPCHK(hipGraphLaunch(eng->ftd_gA, eng->stream));
+        HIPCHK(hipMemcpyAsync(&h, eng->d_fl_cnt, sizeof(FtdCounts), hipMemcpyDeviceToHost,
+                              eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        if (h.undefined)
+            return eng->fail(CSG_ERR_UNDEFINED,
+                             "a NaN-carrying fine-tune segment makes the reference's sort leave "
+                             "its array (undefined behaviour)");
+        if (h.n_active == 0)
+            break;
+        if (h.collision)
+        {
+            // state at the start of this round back to the host
+            *fallback = true;
+            HIPCHK(hipMemcpy(lists.data(), eng->d_fl_lists, (size_t)Nq * L, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(counts.data(), eng->d_fl_counts, (size_t)Nq * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(nrounds.data(), eng->d_fl_rounds, (size_t)Nq * 4, hipMemcpyDeviceToHost));
+            return CSG_OK;
+        }
+        eng->counters[6]++;
+        eng->counters[7] += h.n_active;
+        eng->counters[8] += h.n_groups;
+        eng->counters[9] += (int64_t)h.n_scores;
+        if ((rc = ftd_work(eng, h)))
+            return rc;
+        if (!eng->ftd_tuned && eng->ftd_autotune && h.n_cells >= 4096 && (rc = ftd_tune(eng, quantile)))
+            return rc;
+        if ((rc = ftd_graph(eng, &eng->ftd_gB, &eng->ftd_sigB, ftd_sig_b(eng, quantile, thre, times),
+                            [&]() { return ftd_launch_b(eng, quantile, thre, times); })))
+            return rc;
+        HIPCHK(hipGraphLaunch(eng->ftd_gB, eng->stream));
+        // part B leaves the next round's count in n_next (read with part A)
+        h.n_next = 1;
+        eng->counters[15] += now_us() - t0;
+    }
+    k_ftd_final<<<nblk(Nq), 256, 0, eng->stream>>>(eng->d_fl_lists, eng->d_fl_counts, Nq, L,
+                                                  eng->d_fl_final);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(final_labels, eng->d_fl_final, (size_t)Nq * 4, hipMemcpyDeviceToHost,
+                          eng->stream));
+    if (rounds_out)
+        HIPCHK(hipMemcpyAsync(rounds_out, eng->d_fl_rounds, (size_t)Nq * 4,
+                              hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipMemcpyAsync(&h, eng->d_fl_cnt, sizeof(FtdCounts), hipMemcpyDeviceToHost,
+                          eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    eng->counters[11] += h.tot_redo;
+    if (h.undefined)
+        return eng->fail(CSG_ERR_UNDEFINED,
+                         "a NaN-carrying fine-tune segment makes the reference's sort leave its "
+                         "array (undefined behaviour)");
+    return CSG_OK;
+}
 
+// Host-grouped schedule: rounds grouped by candidate set on the host, the
+// device ranking, correlating and filtering every group in batches.  Starts
+// from the given lists / counts / round counts.
+int finetune_host(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
+                  std::vector<uint8_t>& lists, std::vector<uint32_t>& counts,
+                  std::vector<int32_t>& nrounds, int64_t t_start)
+{
+    const uint32_t L  = eng->fL;
+    const uint32_t Nq = eng->fNq;
+    int            rc = CSG_OK;
     const size_t          gwords = (std::max<uint32_t>(eng->Gf, 1) + 63) / 64;
     std::vector<uint64_t> gbits(gwords);
     std::vector<uint32_t> gsub;
@@ -2147,12 +2705,130 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
                 active[keep++] = c;
         active.resize(keep);
     }
+    return CSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
+                 uint32_t* final_labels, int32_t* rounds_out)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->ft_loaded)
+        return eng->fail(CSG_ERR_STATE, "csg_finetune before csg_copyin");
+    HIPCHK(hipSetDevice(eng->device));
+    const uint32_t L  = eng->fL;
+    const uint32_t Nq = eng->fNq;
+    int            rc = set_pct(eng, &eng->d_fpct, &eng->fpct_quantile, eng->fctidx, L, quantile);
+    if (rc)
+        return rc;
+    for (int i = 6; i < 16; ++i)  // the score step's own counters stay
+        eng->counters[i] = 0;
+    PhaseScope    total(eng, 5, 0);
+    const int64_t t_start = now_us();
+    std::vector<uint8_t>  lists((size_t)Nq * L);
+    std::vector<uint32_t> counts(Nq, 0);
+    std::vector<int32_t>  nrounds(Nq, 0);
+    if (eng->ft_schedule == 0 && L <= 256)
+    {
+        bool fallback = false;
+        rc = finetune_device(eng, quantile, finetune_thre, finetune_times, final_labels, rounds_out,
+                             &fallback, lists, counts, nrounds);
+        if (rc || !fallback)
+            return rc;
+        if ((rc = finetune_host(eng, quantile, finetune_thre, finetune_times, lists, counts,
+                                nrounds, t_start)))
+            return rc;
+        for (uint32_t c = 0; c < Nq; ++c)
+        {
+            final_labels[c] = counts[c] == 0 ? 0u : (uint32_t)lists[(size_t)c * L];
+            if (rounds_out)
+                rounds_out[c] = nrounds[c];
+        }
+        return CSG_OK;
+    }
+    // candidate lists from the mask, ascending label order (src/cusingler.cu:1307-1312)
+    if (!eng->host_mask.empty())
+    {
+        const float* mask = eng->host_mask.data();
+        for (uint32_t c = 0; c < Nq; ++c)
+            for (uint32_t l = 0; l < L; ++l)
+                if (mask[(size_t)c * L + l] != 0)
+                    lists[(size_t)c * L + counts[c]++] = (uint8_t)l;
+    }
+    else
+    {
+        if (!eng->mask_valid || eng->Nq != Nq || eng->L != L)
+            return eng->fail(CSG_ERR_STATE, "no candidate mask: pass one to csg_copyin or run "
+                                            "csg_get_label on the same cells first");
+        const uint32_t W = (L + 63) / 64;
+        uint64_t*      d_bits = nullptr;
+        if ((rc = dalloc(eng, &d_bits, (size_t)Nq * W)))
+            return rc;
+        std::vector<uint64_t> bits((size_t)Nq * W);
+        k_pack_mask<<<(Nq + 255) / 256, 256, 0, eng->stream>>>(eng->d_mask, Nq, L, d_bits);
+        hipError_t e1 = hipGetLastError();
+        hipError_t e2 = hipMemcpyAsync(bits.data(), d_bits, bits.size() * 8, hipMemcpyDeviceToHost,
+                                       eng->stream);
+        hipError_t e3 = hipStreamSynchronize(eng->stream);
+        dfree(d_bits);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+            return eng->fail(CSG_ERR_DEVICE, "mask copy failed");
+        for (uint32_t c = 0; c < Nq; ++c)
+            for (uint32_t w = 0; w < W; ++w)
+            {
+                uint64_t v = bits[(size_t)c * W + w];
+                while (v)
+                {
+                    const int t = __builtin_ctzll(v);
+                    v &= v - 1;
+                    lists[(size_t)c * L + counts[c]++] = (uint8_t)(w * 64 + t);
+                }
+            }
+    }
+
+    if ((rc = finetune_host(eng, quantile, finetune_thre, finetune_times, lists, counts, nrounds,
+                            t_start)))
+        return rc;
     for (uint32_t c = 0; c < Nq; ++c)
     {
         final_labels[c] = counts[c] == 0 ? 0u : (uint32_t)lists[(size_t)c * L];
         if (rounds_out)
             rounds_out[c] = nrounds[c];
     }
+    return CSG_OK;
+}
+
+int csg_set_finetune_schedule(csg_engine* eng, int schedule)
+{
+    if (!eng || schedule < 0 || schedule > 1)
+        return CSG_ERR_ARGUMENT;
+    eng->ft_schedule = schedule;
+    return CSG_OK;
+}
+
+int csg_finetune_tuning(csg_engine* eng, int autotune, int32_t* grids, float* best_ms)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (autotune >= 0)
+    {
+        eng->ftd_autotune = autotune != 0;
+        eng->ftd_tuned    = false;
+    }
+    if (grids)
+    {
+        grids[0] = eng->ftd_tuned ? 1 : 0;
+        grids[1] = eng->ftd_grid_rank;
+        grids[2] = eng->ftd_grid_corr;
+        grids[3] = eng->ftd_grid_select;
+    }
+    if (best_ms)
+        for (int i = 0; i < 3; ++i)
+            best_ms[i] = eng->ftd_tune_ms[i];
     return CSG_OK;
 }
 

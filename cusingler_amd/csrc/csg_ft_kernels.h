@@ -42,6 +42,13 @@ template <class F> __device__ inline float chunk_tree_sum128(int n, F f, float* 
     return r;
 }
 
+// FP32 correlation tiles: TC cells x TR reference rows per block, RC x RR
+// pairs per thread; groups of at most FT_THIN cells use the thin tile
+// (FT_THIN cells x 256 rows).
+constexpr int FT_TC = 32, FT_TR = 32, FT_RC = 2, FT_RR = 2;
+constexpr int FT_NT   = (FT_TC / FT_RC) * (FT_TR / FT_RR);
+constexpr int FT_THIN = 8;
+
 // A row to rank over a group's gene subset.
 struct FtRankItem
 {
@@ -107,25 +114,19 @@ __device__ inline uint32_t find_group(const FtGroup* __restrict__ groups, uint32
 // dense fine-tune codes at G_sub (get_device_ref_lines, src/cusingler.cu:
 // 566-577), rank them (rankdata_bin3, :745-801) and form the chunk-tree sum of
 // squares.
-__global__ __launch_bounds__(128) void k_ft_rank_ref(const FtGroup* __restrict__ groups,
-                                                     uint32_t ngroups,
-                                                     const uint32_t* __restrict__ labels,
-                                                     const uint32_t* __restrict__ lstart,
-                                                     const uint32_t* __restrict__ llen,
-                                                     const uint32_t* __restrict__ genes,
-                                                     const uint16_t* __restrict__ ref_codes,
-                                                     uint32_t Gf, int nbins,
-                                                     int16_t* __restrict__ out,
-                                                     float* __restrict__ out_sq)
+__device__ inline void ft_rank_ref_item(uint32_t item, const FtGroup* __restrict__ groups,
+                                        uint32_t ngroups, const uint32_t* __restrict__ labels,
+                                        const uint32_t* __restrict__ lstart,
+                                        const uint32_t* __restrict__ llen,
+                                        const uint32_t* __restrict__ genes,
+                                        const uint16_t* __restrict__ ref_codes, uint32_t Gf,
+                                        int nbins, int16_t* __restrict__ out,
+                                        float* __restrict__ out_sq, int* smem, float* part, int* tmp)
 {
-    extern __shared__ int smem[];
-    __shared__ float part[128];
-    __shared__ int   tmp[2];
-    const uint32_t   item = blockIdx.x;
-    const uint32_t   gi   = find_group(groups, ngroups, item, true);
-    const FtGroup    g    = groups[gi];
-    uint32_t         j    = item - g.ref_item0;
-    uint32_t         src_row = 0;
+    const uint32_t gi      = find_group(groups, ngroups, item, true);
+    const FtGroup  g       = groups[gi];
+    uint32_t       j       = item - g.ref_item0;
+    uint32_t       src_row = 0;
     for (uint32_t t = 0; t < g.nlabel; ++t)
     {
         const uint32_t l = labels[g.label_off + t];
@@ -154,27 +155,40 @@ __global__ __launch_bounds__(128) void k_ft_rank_ref(const FtGroup* __restrict__
         out_sq[item] = sq;
 }
 
-// Query rows: densify the CSR row over the fine-tune genes, gather G_sub
-// (csr2dense single-row form, src/cusingler.cu:241-264), rank (rankdata,
-// :159-179) and form the chunk-tree sum of squares.
-__global__ __launch_bounds__(128) void k_ft_rank_qry(const FtRankItem* __restrict__ items,
-                                                     const FtGroup* __restrict__ groups,
+__global__ __launch_bounds__(128) void k_ft_rank_ref(const FtGroup* __restrict__ groups,
+                                                     uint32_t ngroups,
+                                                     const uint32_t* __restrict__ labels,
+                                                     const uint32_t* __restrict__ lstart,
+                                                     const uint32_t* __restrict__ llen,
                                                      const uint32_t* __restrict__ genes,
-                                                     const uint16_t* __restrict__ qdata,
-                                                     const int* __restrict__ qind,
-                                                     const int* __restrict__ qptr, uint32_t Gf,
-                                                     int nbins, int16_t* __restrict__ out,
+                                                     const uint16_t* __restrict__ ref_codes,
+                                                     uint32_t Gf, int nbins,
+                                                     int16_t* __restrict__ out,
                                                      float* __restrict__ out_sq)
 {
     extern __shared__ int smem[];
     __shared__ float part[128];
     __shared__ int   tmp[2];
-    const FtRankItem it = items[blockIdx.x];
-    const FtGroup    g  = groups[it.group];
-    int*             bins  = smem;
-    uint16_t*        dense = (uint16_t*)(smem + ((nbins + 3) & ~3));
-    uint16_t*        codes = dense + ((Gf + 7) & ~7u);
-    int16_t*         row   = (int16_t*)(codes + ((g.gs + 7) & ~7u));
+    ft_rank_ref_item(blockIdx.x, groups, ngroups, labels, lstart, llen, genes, ref_codes, Gf,
+                     nbins, out, out_sq, smem, part, tmp);
+}
+
+// Query rows: densify the CSR row over the fine-tune genes, gather G_sub
+// (csr2dense single-row form, src/cusingler.cu:241-264), rank (rankdata,
+// :159-179) and form the chunk-tree sum of squares.
+__device__ inline void ft_rank_qry_item(const FtRankItem& it, uint32_t idx,
+                                        const FtGroup* __restrict__ groups,
+                                        const uint32_t* __restrict__ genes,
+                                        const uint16_t* __restrict__ qdata,
+                                        const int* __restrict__ qind, const int* __restrict__ qptr,
+                                        uint32_t Gf, int nbins, int16_t* __restrict__ out,
+                                        float* __restrict__ out_sq, int* smem, float* part, int* tmp)
+{
+    const FtGroup g     = groups[it.group];
+    int*          bins  = smem;
+    uint16_t*     dense = (uint16_t*)(smem + ((nbins + 3) & ~3));
+    uint16_t*     codes = dense + ((Gf + 7) & ~7u);
+    int16_t*      row   = (int16_t*)(codes + ((g.gs + 7) & ~7u));
     for (uint32_t i = threadIdx.x; i < Gf; i += 128)
         dense[i] = 0;
     __syncthreads();
@@ -192,7 +206,23 @@ __global__ __launch_bounds__(128) void k_ft_rank_qry(const FtRankItem* __restric
         dst[i] = i < g.gs ? row[i] : (int16_t)0;
     const float sq = chunk_tree_sum128((int)g.gs, [&](int i) { return (float)row[i]; }, part);
     if (threadIdx.x == 0)
-        out_sq[blockIdx.x] = sq;
+        out_sq[idx] = sq;
+}
+
+__global__ __launch_bounds__(128) void k_ft_rank_qry(const FtRankItem* __restrict__ items,
+                                                     const FtGroup* __restrict__ groups,
+                                                     const uint32_t* __restrict__ genes,
+                                                     const uint16_t* __restrict__ qdata,
+                                                     const int* __restrict__ qind,
+                                                     const int* __restrict__ qptr, uint32_t Gf,
+                                                     int nbins, int16_t* __restrict__ out,
+                                                     float* __restrict__ out_sq)
+{
+    extern __shared__ int smem[];
+    __shared__ float part[128];
+    __shared__ int   tmp[2];
+    ft_rank_qry_item(items[blockIdx.x], blockIdx.x, groups, genes, qdata, qind, qptr, Gf, nbins,
+                     out, out_sq, smem, part, tmp);
 }
 
 // Correlation tile of one group: TC cells x TR reference rows, one pair per
@@ -210,15 +240,14 @@ struct FtCorrTile
 };
 
 template <int TC, int TR, int RC, int RR>
-__global__ __launch_bounds__((TC / RC) * (TR / RR)) void k_ft_corr(
-    const FtCorrTile* __restrict__ tiles, const FtGroup* __restrict__ groups,
-    const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
-    const int16_t* __restrict__ ref_rank, const float* __restrict__ ref_sq,
-    float* __restrict__ scores)
+__device__ inline void ft_corr_tile(const FtCorrTile& tl, const FtGroup* __restrict__ groups,
+                                    const int16_t* __restrict__ cell_rank,
+                                    const float* __restrict__ cell_sq,
+                                    const int16_t* __restrict__ ref_rank,
+                                    const float* __restrict__ ref_sq, float* __restrict__ scores,
+                                    int* srow32)
 {
-    extern __shared__ int srow32[];
     constexpr int    NT    = (TC / RC) * (TR / RR);
-    const FtCorrTile tl    = tiles[blockIdx.x];
     const FtGroup    g     = groups[tl.group];
     const int        gs    = (int)g.gs;
     const int        st    = (int)g.stride;  // multiple of 8
@@ -374,21 +403,30 @@ __global__ __launch_bounds__((TC / RC) * (TR / RR)) void k_ft_corr(
     }
 }
 
+template <int TC, int TR, int RC, int RR>
+__global__ __launch_bounds__((TC / RC) * (TR / RR)) void k_ft_corr(
+    const FtCorrTile* __restrict__ tiles, const FtGroup* __restrict__ groups,
+    const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
+    const int16_t* __restrict__ ref_rank, const float* __restrict__ ref_sq,
+    float* __restrict__ scores)
+{
+    extern __shared__ int srow32[];
+    ft_corr_tile<TC, TR, RC, RR>(tiles[blockIdx.x], groups, cell_rank, cell_sq, ref_rank, ref_sq,
+                                 scores, srow32);
+}
+
 // Thin tile for groups with few cells: up to TC cells (rows in LDS) x 256
 // reference rows, one reference row per thread read straight from memory
 // (each row is used by so few cells that staging it gains nothing).  Same
 // 128-chunk + tree order as k_ft_corr.
 template <int TC>
-__global__ __launch_bounds__(256) void k_ft_corr_thin(const FtCorrTile* __restrict__ tiles,
-                                                      const FtGroup* __restrict__ groups,
-                                                      const int16_t* __restrict__ cell_rank,
-                                                      const float* __restrict__ cell_sq,
-                                                      const int16_t* __restrict__ ref_rank,
-                                                      const float* __restrict__ ref_sq,
-                                                      float* __restrict__ scores)
+__device__ inline void ft_corr_thin_tile(const FtCorrTile& tl, const FtGroup* __restrict__ groups,
+                                         const int16_t* __restrict__ cell_rank,
+                                         const float* __restrict__ cell_sq,
+                                         const int16_t* __restrict__ ref_rank,
+                                         const float* __restrict__ ref_sq,
+                                         float* __restrict__ scores, int16_t* cthin)
 {
-    extern __shared__ int16_t cthin[];
-    const FtCorrTile tl    = tiles[blockIdx.x];
     const FtGroup    g     = groups[tl.group];
     const int        gs    = (int)g.gs;
     const int        st    = (int)g.stride;
@@ -400,7 +438,7 @@ __global__ __launch_bounds__(256) void k_ft_corr_thin(const FtCorrTile* __restri
     __syncthreads();
     const int r = threadIdx.x;
     if (r >= nref)
-        return;
+        return;  // no barrier follows in this tile
     const int16_t* y    = ref_rank + g.ref_dst0 + (uint64_t)(tl.ref0 + r) * st;
     const int      step = (gs + 127) / 128;
     const int      nz   = (gs + step - 1) / step;
@@ -478,6 +516,20 @@ __global__ __launch_bounds__(256) void k_ft_corr_thin(const FtCorrTile* __restri
     }
 }
 
+template <int TC>
+__global__ __launch_bounds__(256) void k_ft_corr_thin(const FtCorrTile* __restrict__ tiles,
+                                                      const FtGroup* __restrict__ groups,
+                                                      const int16_t* __restrict__ cell_rank,
+                                                      const float* __restrict__ cell_sq,
+                                                      const int16_t* __restrict__ ref_rank,
+                                                      const float* __restrict__ ref_sq,
+                                                      float* __restrict__ scores)
+{
+    extern __shared__ int16_t cthin[];
+    ft_corr_thin_tile<TC>(tiles[blockIdx.x], groups, cell_rank, cell_sq, ref_rank, ref_sq, scores,
+                          cthin);
+}
+
 // New candidate list of one cell from its per-label scores, following
 // finetune_round's tail (src/cusingler.cu:1270-1295): NaN maximum -> {front};
 // keep labels with score > min and score >= max - thre, labels equal to the
@@ -523,21 +575,13 @@ struct FtSegments
     }
 };
 
-__global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
-                            const FtGroup* __restrict__ groups,
-                            const uint32_t* __restrict__ labels, const float* __restrict__ P,
-                            const uint8_t* __restrict__ in_lists,
-                            const uint32_t* __restrict__ in_counts, uint32_t L, float thre,
-                            uint8_t* __restrict__ out_lists, uint32_t* __restrict__ out_counts)
+// One cell of the FP32 filter: `in` holds its k candidates, the new list goes
+// to `out`; returns its length.
+__device__ inline uint32_t ft_filter_cell(const FtCell& fc, const FtGroup& g,
+                                          const uint32_t* __restrict__ gl,
+                                          const float* __restrict__ P, const uint8_t* in,
+                                          uint32_t k, float thre, uint8_t* out)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncells)
-        return;
-    const FtCell   fc = cells[i];
-    const FtGroup  g  = groups[fc.group];
-    const uint8_t* in = in_lists + (size_t)i * L;
-    const uint32_t k  = in_counts[i];
-    const uint32_t* gl = labels + g.label_off;
     auto score_of = [&](uint32_t pos) {
         const uint32_t lab = in[pos];
         uint32_t       j   = 0;
@@ -611,13 +655,11 @@ __global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
     }
     const float vmin = score_of(mn);
     const float vmax = score_of(mx);
-    uint8_t*    out  = out_lists + (size_t)i * L;
     uint32_t    n    = 0;
     if (__builtin_isnan(vmax))
     {
-        out[0]        = in[0];
-        out_counts[i] = 1;
-        return;
+        out[0] = in[0];
+        return 1;
     }
     const float th = vmax - thre;
     // labels equal to the maximum, latest first
@@ -639,7 +681,23 @@ __global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
     }
     if (n == 0)
         out[n++] = in[0];
-    out_counts[i] = n;
+    return n;
+}
+
+__global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
+                            const FtGroup* __restrict__ groups,
+                            const uint32_t* __restrict__ labels, const float* __restrict__ P,
+                            const uint8_t* __restrict__ in_lists,
+                            const uint32_t* __restrict__ in_counts, uint32_t L, float thre,
+                            uint8_t* __restrict__ out_lists, uint32_t* __restrict__ out_counts)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncells)
+        return;
+    const FtCell  fc = cells[i];
+    const FtGroup g  = groups[fc.group];
+    out_counts[i]    = ft_filter_cell(fc, g, labels + g.label_off, P, in_lists + (size_t)i * L,
+                                      in_counts[i], thre, out_lists + (size_t)i * L);
 }
 
 // Exact fine-tune contraction (INT8 limbs on MFMA) for gene subsets of at most
@@ -650,18 +708,16 @@ __global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
 // the hi*hi product is folded into the cross partial as (16h)(16h).
 typedef int v4i_ft __attribute__((ext_vector_type(4)));
 
-__global__ __launch_bounds__(256) void k_ft_mfma(const FtCorrTile* __restrict__ tiles,
-                                                 const FtGroup* __restrict__ groups,
-                                                 const int16_t* __restrict__ cell_rank,
-                                                 const float* __restrict__ cell_sq,
-                                                 const int16_t* __restrict__ ref_rank,
-                                                 const float* __restrict__ ref_sq,
-                                                 float* __restrict__ scores)
+constexpr int FT_MFMA_LD = 80;  // bytes per LDS row of the exact fine-tune tile
+
+__device__ inline void ft_mfma_tile(const FtCorrTile& tl, const FtGroup* __restrict__ groups,
+                                    const int16_t* __restrict__ cell_rank,
+                                    const float* __restrict__ cell_sq,
+                                    const int16_t* __restrict__ ref_rank,
+                                    const float* __restrict__ ref_sq, float* __restrict__ scores,
+                                    int8_t (*sa)[16 * FT_MFMA_LD], int8_t (*sb)[256 * FT_MFMA_LD])
 {
-    constexpr int LD = 80;  // bytes per LDS row
-    __shared__ __attribute__((aligned(16))) int8_t sa[2][16 * LD];
-    __shared__ __attribute__((aligned(16))) int8_t sb[2][256 * LD];
-    const FtCorrTile tl    = tiles[blockIdx.x];
+    constexpr int LD = FT_MFMA_LD;
     const FtGroup    g     = groups[tl.group];
     const int        st    = (int)g.stride;
     const int        ncell = min(16, (int)g.ncell - (int)tl.cell0);
@@ -766,6 +822,19 @@ __global__ __launch_bounds__(256) void k_ft_mfma(const FtCorrTile* __restrict__ 
     }
 }
 
+__global__ __launch_bounds__(256) void k_ft_mfma(const FtCorrTile* __restrict__ tiles,
+                                                 const FtGroup* __restrict__ groups,
+                                                 const int16_t* __restrict__ cell_rank,
+                                                 const float* __restrict__ cell_sq,
+                                                 const int16_t* __restrict__ ref_rank,
+                                                 const float* __restrict__ ref_sq,
+                                                 float* __restrict__ scores)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][16 * FT_MFMA_LD];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][256 * FT_MFMA_LD];
+    ft_mfma_tile(tiles[blockIdx.x], groups, cell_rank, cell_sq, ref_rank, ref_sq, scores, sa, sb);
+}
+
 // Certified filter for the exact fine-tune: the reference's chunk-tree FP32
 // sum differs from the exact one by at most gamma_{step+6} * sum|p| (chunk
 // sums of `step` exact products, then a 7-level tree), so every per-label
@@ -781,22 +850,14 @@ __device__ inline float ft_next_up(float x)
     return __uint_as_float(x > 0.f ? b + 1u : b - 1u);
 }
 
-__global__ void k_ft_filter_exact(const FtCell* __restrict__ cells, uint32_t ncells,
-                                  const FtGroup* __restrict__ groups,
-                                  const uint32_t* __restrict__ labels, const float* __restrict__ P,
-                                  const uint8_t* __restrict__ in_lists,
-                                  const uint32_t* __restrict__ in_counts, uint32_t L, float thre,
-                                  uint8_t* __restrict__ out_lists, uint32_t* __restrict__ out_counts,
-                                  uint8_t* __restrict__ status)
+// One cell of the certified filter: returns 0 and writes the new list
+// (length in *cnt_out) when certified, else 1 (list untouched).
+__device__ inline int ft_filter_exact_cell(const FtCell& fc, const FtGroup& g,
+                                           const uint32_t* __restrict__ gl,
+                                           const float* __restrict__ P, const uint8_t* in,
+                                           uint32_t k, float thre, uint8_t* out,
+                                           uint32_t* cnt_out)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncells)
-        return;
-    const FtCell    fc = cells[i];
-    const FtGroup   g  = groups[fc.group];
-    const uint8_t*  in = in_lists + (size_t)i * L;
-    const uint32_t  k  = in_counts[i];
-    const uint32_t* gl = labels + g.label_off;
     auto score_of = [&](uint32_t pos) {
         const uint32_t lab = in[pos];
         uint32_t       j   = 0;
@@ -814,10 +875,7 @@ __global__ void k_ft_filter_exact(const FtCell* __restrict__ cells, uint32_t nce
     {
         const float v = score_of(t);
         if (__builtin_isnan(v))
-        {
-            status[i] = 1;
-            return;
-        }
+            return 1;
         mlo = fmax(mlo, (double)v - wp);
         mhi = fmax(mhi, (double)v + wp);
         nlo = fmin(nlo, (double)v - wp);
@@ -839,10 +897,7 @@ __global__ void k_ft_filter_exact(const FtCell* __restrict__ cells, uint32_t nce
         }
     }
     if (nmax != 1 || nmin != 1 || imax == imin)
-    {
-        status[i] = 1;
-        return;
-    }
+        return 1;
     // float bounds of the reference's maximum, then of max - thre
     float flo = (float)mlo;
     if ((double)flo > mlo)
@@ -852,24 +907,45 @@ __global__ void k_ft_filter_exact(const FtCell* __restrict__ cells, uint32_t nce
         fhi = ft_next_up(fhi);
     const double tlo = (double)(flo - thre);
     const double thi = (double)(fhi - thre);
-    uint8_t*     out = out_lists + (size_t)i * L;
-    uint32_t     cnt = 0;
-    out[cnt++]       = in[imax];
+    // decide every label before writing (out may be read by nobody else, but
+    // an uncertified cell must leave it untouched)
     for (uint32_t t = 0; t < k; ++t)
     {
         if ((int)t == imax || (int)t == imin)
             continue;
         const double v = (double)score_of(t);
-        if (v - wp >= thi)
-            out[cnt++] = in[t];
-        else if (!(v + wp < tlo))
-        {
-            status[i] = 1;
-            return;
-        }
+        if (!(v - wp >= thi) && !(v + wp < tlo))
+            return 1;
     }
-    out_counts[i] = cnt;
-    status[i]     = 0;
+    uint32_t cnt = 0;
+    out[cnt++]   = in[imax];
+    for (uint32_t t = 0; t < k; ++t)
+    {
+        if ((int)t == imax || (int)t == imin)
+            continue;
+        if ((double)score_of(t) - wp >= thi)
+            out[cnt++] = in[t];
+    }
+    *cnt_out = cnt;
+    return 0;
+}
+
+__global__ void k_ft_filter_exact(const FtCell* __restrict__ cells, uint32_t ncells,
+                                  const FtGroup* __restrict__ groups,
+                                  const uint32_t* __restrict__ labels, const float* __restrict__ P,
+                                  const uint8_t* __restrict__ in_lists,
+                                  const uint32_t* __restrict__ in_counts, uint32_t L, float thre,
+                                  uint8_t* __restrict__ out_lists, uint32_t* __restrict__ out_counts,
+                                  uint8_t* __restrict__ status)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncells)
+        return;
+    const FtCell  fc = cells[i];
+    const FtGroup g  = groups[fc.group];
+    status[i] = (uint8_t)ft_filter_exact_cell(fc, g, labels + g.label_off, P,
+                                              in_lists + (size_t)i * L, in_counts[i], thre,
+                                              out_lists + (size_t)i * L, out_counts + i);
 }
 
 }  // namespace csg

@@ -1,0 +1,940 @@
+// Device-resident fine-tune rounds (cufinetune / finetune_round,
+// src/cusingler.cu:1138-1333).
+//
+// Every round runs on the device with its work counts kept in device memory,
+// so that a round is a fixed sequence of launches (captured in two HIP
+// graphs by the host) and the host reads back one small counter block per
+// round:
+//   1. group the active cells by candidate set: a 64-bit hash of the set's
+//      label bitset is inserted into an open-addressing table (the first
+//      inserter owns the slot), slots receive group ids, and every cell
+//      checks its full set against its group owner's (a mismatch -- two sets
+//      under one hash -- is reported and the host falls back);
+//   2. per group: the ascending label list, G_sub = the union of every
+//      ordered pair's first min(len, k_t) window genes (k_t =
+//      round(500 (2/3)^log2 k)), built as a bitset in LDS and scanned into an
+//      ascending gene list;
+//   3. one block prefix-sums the groups' work sizes into offsets;
+//   4. work-queue ("persistent") kernels -- grids sized to the machine, each
+//      block pulling items until the device-side count is exhausted -- rank
+//      the reference and query rows over G_sub, correlate (exact INT8 MFMA
+//      tiles or FP32 chunk-tree tiles), select the per-label quantiles and
+//      filter every cell's candidate list in place; cells the certified
+//      filter cannot decide are redone in FP32 order;
+//   5. the cells that still have several candidates form the next round.
+// Work placement depends on atomics, values do not: every (cell, reference
+// row) sum and every per-segment quantile is computed exactly as in the
+// host-grouped path, so labels and round counts are identical.
+#pragma once
+
+#include "csg_ft_kernels.h"
+#include "csg_stdalgo.h"
+
+namespace csg {
+
+struct FtdCounts
+{
+    uint32_t n_active;   // cells active in this round
+    uint32_t n_next;     // cells active in the next round
+    uint32_t n_groups;
+    uint32_t n_cells;    // cells with work this round (groups of >= 20 genes)
+    uint32_t n_tiles;    // wide correlation tiles (exact 16 x 256 or FP32 32 x 32)
+    uint32_t n_thin;     // FP32 thin tiles
+    uint32_t n_redo;     // cells the certified filter left undecided
+    uint32_t n_rthin;    // thin tiles of their FP32 redo
+    uint32_t n_nan;      // NaN segments of the select
+    uint32_t n_nan2;     // NaN segments of the redo select
+    uint32_t collision;  // two candidate sets met under one hash
+    uint32_t undefined;  // the libstdc++ sort emulation would leave its array
+    uint32_t max_stride;
+    uint32_t round;
+    uint32_t tot_redo;  // redone cells over all rounds
+    uint32_t pad_;
+    uint64_t n_ref_items;
+    uint64_t ref_elems;
+    uint64_t cell_elems;
+    uint64_t n_scores;
+    uint64_t n_segs;
+    uint64_t n_redo_segs;
+    uint64_t rthin_bound;  // tiles a redo of every exact-form cell would take
+};
+
+// Per-group flags (bit 0: fewer than 20 genes, bit 1: exact form).
+constexpr uint8_t FTD_SHORT = 1, FTD_EXACT = 2;
+
+__device__ inline void ftd_key(const uint8_t* __restrict__ list, uint32_t k, uint64_t key[4])
+{
+    key[0] = key[1] = key[2] = key[3] = 0ull;
+    for (uint32_t t = 0; t < k; ++t)
+    {
+        const uint32_t l = list[t];
+        key[l >> 6] |= 1ull << (l & 63);
+    }
+}
+
+__device__ inline uint64_t ftd_mix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ inline uint64_t ftd_hash(const uint64_t key[4])
+{
+    uint64_t h = ftd_mix(key[0]);
+    h          = ftd_mix(h ^ key[1]);
+    h          = ftd_mix(h ^ key[2]);
+    h          = ftd_mix(h ^ key[3]);
+    return h ? h : 1ull;  // 0 marks an empty slot
+}
+
+// Initial candidate lists: ascending labels of each cell's mask row
+// (src/cusingler.cu:1307-1312); cells with several candidates (and rounds
+// left) are listed as the first round's active cells.
+__global__ void k_ftd_init(const float* __restrict__ mask, uint32_t Nq, uint32_t L, int times,
+                           uint8_t* __restrict__ lists, uint32_t* __restrict__ counts,
+                           int32_t* __restrict__ rounds, uint32_t* __restrict__ act_next,
+                           FtdCounts* __restrict__ cnt)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Nq)
+        return;
+    const float* m = mask + (size_t)c * L;
+    uint8_t*     o = lists + (size_t)c * L;
+    uint32_t     k = 0;
+    for (uint32_t l = 0; l < L; ++l)
+        if (m[l] != 0.f)
+            o[k++] = (uint8_t)l;
+    counts[c] = k;
+    rounds[c] = 0;
+    if (k > 1 && (times == 0 || 0 < times))
+        act_next[atomicAdd(&cnt->n_next, 1u)] = c;
+}
+
+// Round start, part 1: the next round's active list becomes this round's;
+// the hash table and the group member counters are cleared.
+__global__ void k_ftd_begin(const FtdCounts* __restrict__ cnt, uint32_t* __restrict__ act,
+                            const uint32_t* __restrict__ act_next,
+                            unsigned long long* __restrict__ htab, uint32_t T,
+                            uint32_t* __restrict__ gcount)
+{
+    const uint32_t n      = cnt->n_next;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        act[i]    = act_next[i];
+        gcount[i] = 0;
+    }
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < T; s += stride)
+        htab[s] = 0ull;
+}
+
+// Round start, part 2 (one thread): counters.
+__global__ void k_ftd_counters(FtdCounts* __restrict__ cnt)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    cnt->n_active    = cnt->n_next;
+    cnt->n_next      = 0;
+    cnt->n_groups    = 0;
+    cnt->n_cells     = 0;
+    cnt->n_tiles     = 0;
+    cnt->n_thin      = 0;
+    cnt->n_redo      = 0;
+    cnt->n_rthin     = 0;
+    cnt->n_nan       = 0;
+    cnt->n_nan2      = 0;
+    cnt->collision   = 0;
+    cnt->max_stride  = 0;
+    cnt->round      += 1;
+    cnt->n_ref_items = 0;
+    cnt->ref_elems   = 0;
+    cnt->cell_elems  = 0;
+    cnt->n_scores    = 0;
+    cnt->n_segs      = 0;
+    cnt->n_redo_segs = 0;
+    cnt->rthin_bound = 0;
+}
+
+__global__ void k_ftd_hash(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+                           const uint8_t* __restrict__ lists, const uint32_t* __restrict__ counts,
+                           uint32_t L, unsigned long long* __restrict__ htab,
+                           uint32_t* __restrict__ howner, uint32_t T, uint32_t* __restrict__ cslot)
+{
+    const uint32_t n = cnt->n_active;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t c = act[i];
+        uint64_t       key[4];
+        ftd_key(lists + (size_t)c * L, counts[c], key);
+        const unsigned long long h = ftd_hash(key);
+        uint32_t                 s = (uint32_t)h & (T - 1);
+        // T >= 2 n: a free slot is always reached
+        for (;;)
+        {
+            const unsigned long long prev = atomicCAS(&htab[s], 0ull, h);
+            if (prev == 0ull)
+            {
+                howner[s] = c;
+                break;
+            }
+            if (prev == h)
+                break;
+            s = (s + 1) & (T - 1);
+        }
+        cslot[i] = s;
+    }
+}
+
+__global__ void k_ftd_gid(FtdCounts* __restrict__ cnt, const unsigned long long* __restrict__ htab,
+                          const uint32_t* __restrict__ howner, uint32_t T,
+                          uint32_t* __restrict__ hgid, uint32_t* __restrict__ gowner)
+{
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < T; s += gridDim.x * blockDim.x)
+        if (htab[s])
+        {
+            const uint32_t g = atomicAdd(&cnt->n_groups, 1u);
+            hgid[s]          = g;
+            gowner[g]        = howner[s];
+        }
+}
+
+__global__ void k_ftd_member(FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+                             const uint8_t* __restrict__ lists,
+                             const uint32_t* __restrict__ counts, uint32_t L,
+                             const uint32_t* __restrict__ cslot, const uint32_t* __restrict__ hgid,
+                             const uint32_t* __restrict__ gowner, uint32_t* __restrict__ gcount,
+                             uint32_t* __restrict__ cgid, uint32_t* __restrict__ cpos)
+{
+    const uint32_t n = cnt->n_active;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t c = act[i];
+        const uint32_t g = hgid[cslot[i]];
+        const uint32_t o = gowner[g];
+        if (o != c)
+        {
+            uint64_t a[4], b[4];
+            ftd_key(lists + (size_t)c * L, counts[c], a);
+            ftd_key(lists + (size_t)o * L, counts[o], b);
+            if (a[0] != b[0] || a[1] != b[1] || a[2] != b[2] || a[3] != b[3])
+                atomicOr(&cnt->collision, 1u);
+        }
+        cgid[i] = g;
+        cpos[i] = atomicAdd(&gcount[g], 1u);
+    }
+}
+
+// One block per group: ascending labels, their row offsets, G_sub.
+// `exact_ok`: the exact form is wanted (groups of at most 1919 genes use it).
+constexpr int FTD_SETUP_NT = 256;
+
+__global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
+    const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ gowner,
+    const uint32_t* __restrict__ gcount, const uint8_t* __restrict__ lists,
+    const uint32_t* __restrict__ counts, uint32_t L, const uint32_t* __restrict__ ctdidx,
+    const uint32_t* __restrict__ ctdiff, const uint32_t* __restrict__ cut_tab,
+    const uint32_t* __restrict__ llen, uint32_t Gf, uint32_t gstride, int exact_ok,
+    FtGroup* __restrict__ groups, uint8_t* __restrict__ gflag, uint32_t* __restrict__ glabels,
+    uint32_t* __restrict__ glpos, uint32_t* __restrict__ ggenes)
+{
+    extern __shared__ uint32_t sbits[];  // ceil(Gf / 32) words
+    __shared__ uint32_t slab[256];
+    __shared__ uint32_t spart[FTD_SETUP_NT];
+    __shared__ uint32_t sk, snref;
+    const uint32_t ng  = cnt->n_groups;
+    const uint32_t W   = (Gf + 31) / 32;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x)
+    {
+        __syncthreads();  // the previous group's LDS is no longer read
+        if (tid == 0)
+        {
+            const uint32_t o = gowner[g];
+            uint64_t       key[4];
+            ftd_key(lists + (size_t)o * L, counts[o], key);
+            uint32_t k = 0, nref = 0;
+            for (uint32_t w = 0; w < 4; ++w)
+            {
+                uint64_t v = key[w];
+                while (v)
+                {
+                    const uint32_t l = w * 64 + (uint32_t)__builtin_ctzll(v);
+                    v &= v - 1;
+                    slab[k]                       = l;
+                    glabels[(size_t)g * L + k]    = l;
+                    glpos[(size_t)g * L + k]      = nref;
+                    nref += llen[l];
+                    ++k;
+                }
+            }
+            sk    = k;
+            snref = nref;
+        }
+        for (uint32_t w = tid; w < W; w += FTD_SETUP_NT)
+            sbits[w] = 0u;
+        __syncthreads();
+        const uint32_t k   = sk;
+        const uint32_t cut = k ? cut_tab[k] : 0u;
+        // ordered pairs spread over the waves, genes over the lanes
+        for (uint32_t pi = tid >> 6; pi < k * k; pi += FTD_SETUP_NT / 64)
+        {
+            const uint32_t i = slab[pi / k], j = slab[pi % k];
+            if (i == j)
+                continue;
+            const uint32_t pos = ctdidx[((size_t)i * L + j) * 2];
+            const uint32_t len = min(ctdidx[((size_t)i * L + j) * 2 + 1], cut);
+            for (uint32_t t = tid & 63; t < len; t += 64)
+            {
+                const uint32_t gene = ctdiff[pos + t];
+                atomicOr(&sbits[gene >> 5], 1u << (gene & 31));
+            }
+        }
+        __syncthreads();
+        // ascending gene list: contiguous word ranges per thread, block scan
+        const uint32_t per = (W + FTD_SETUP_NT - 1) / FTD_SETUP_NT;
+        const uint32_t w0 = min(tid * per, W), w1 = min(w0 + per, W);
+        uint32_t       own = 0;
+        for (uint32_t w = w0; w < w1; ++w)
+            own += __builtin_popcount(sbits[w]);
+        spart[tid] = own;
+        __syncthreads();
+        for (uint32_t off = 1; off < FTD_SETUP_NT; off <<= 1)
+        {
+            const uint32_t v = tid >= off ? spart[tid - off] : 0u;
+            __syncthreads();
+            spart[tid] += v;
+            __syncthreads();
+        }
+        uint32_t       at  = spart[tid] - own;
+        const uint32_t gs  = spart[FTD_SETUP_NT - 1];
+        uint32_t*      dst = ggenes + (size_t)g * gstride;
+        for (uint32_t w = w0; w < w1; ++w)
+        {
+            uint32_t v = sbits[w];
+            while (v)
+            {
+                dst[at++] = w * 32 + (uint32_t)__builtin_ctz(v);
+                v &= v - 1;
+            }
+        }
+        if (tid == 0)
+        {
+            FtGroup G{};
+            G.gene_off  = (uint32_t)((size_t)g * gstride);
+            G.gs        = gs;
+            G.stride    = (gs + 7) & ~7u;
+            G.nref      = snref;
+            G.ncell     = gcount[g];
+            G.label_off = (uint32_t)((size_t)g * L);
+            G.nlabel    = k;
+            groups[g]   = G;
+            // fewer than 20 genes: the round returns {front}
+            // (src/cusingler.cu:1158-1159)
+            gflag[g] = (uint8_t)((gs < 20 ? FTD_SHORT : 0) |
+                                 (exact_ok && gs <= 1919 ? FTD_EXACT : 0));
+        }
+    }
+}
+
+// Exclusive prefix sums of the groups' work sizes (one block of FTD_SCAN_NT
+// threads): row, element, score and segment offsets, tile offsets, totals.
+// FP32-form groups of at most `thin_max` cells use thin tiles.
+constexpr int FTD_SCAN_NT = 512;
+
+struct FtdSizes
+{
+    uint64_t ref_items, cell_items, ref_elems, cell_elems, scores, tiles, thin, segs, rthin;
+};
+
+__device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, uint32_t thin_max)
+{
+    FtdSizes z{};
+    if (flag & FTD_SHORT)
+        return z;
+    const uint64_t nc = G.ncell, nr = G.nref;
+    z.ref_items       = nr;
+    z.cell_items      = nc;
+    z.ref_elems       = nr * G.stride;
+    z.cell_elems      = nc * G.stride;
+    z.scores          = nc * nr;
+    z.segs            = nc * G.nlabel;
+    if (flag & FTD_EXACT)
+    {
+        z.tiles = ((nc + 15) / 16) * ((nr + 255) / 256);
+        z.rthin = nc * ((nr + 255) / 256);
+    }
+    else if (nc <= thin_max)
+        z.thin = (nr + 255) / 256;
+    else
+        z.tiles = ((nc + FT_TC - 1) / FT_TC) * ((nr + FT_TR - 1) / FT_TR);
+    return z;
+}
+
+__global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict__ cnt,
+                                                          FtGroup* __restrict__ groups,
+                                                          const uint8_t* __restrict__ gflag,
+                                                          uint32_t thin_max,
+                                                          uint32_t* __restrict__ gtile0,
+                                                          uint32_t* __restrict__ gthin0,
+                                                          uint64_t* __restrict__ gseg0)
+{
+    constexpr int NQ = 9;
+    __shared__ uint64_t part[NQ][FTD_SCAN_NT];
+    const uint32_t      ng  = cnt->n_groups;
+    const uint32_t      tid = threadIdx.x;
+    const uint32_t      per = (ng + FTD_SCAN_NT - 1) / FTD_SCAN_NT;
+    const uint32_t      g0 = min(tid * per, ng), g1 = min(g0 + per, ng);
+    uint64_t            acc[NQ] = {};
+    uint32_t            smax    = 0;
+    for (uint32_t g = g0; g < g1; ++g)
+    {
+        const FtdSizes z = ftd_sizes(groups[g], gflag[g], thin_max);
+        acc[0] += z.ref_items;
+        acc[1] += z.cell_items;
+        acc[2] += z.ref_elems;
+        acc[3] += z.cell_elems;
+        acc[4] += z.scores;
+        acc[5] += z.tiles;
+        acc[6] += z.thin;
+        acc[7] += z.segs;
+        acc[8] += z.rthin;
+        if (!(gflag[g] & FTD_SHORT))
+            smax = max(smax, groups[g].stride);
+    }
+    for (int q = 0; q < NQ; ++q)
+        part[q][tid] = acc[q];
+    __syncthreads();
+    for (uint32_t off = 1; off < FTD_SCAN_NT; off <<= 1)
+    {
+        uint64_t v[NQ];
+        for (int q = 0; q < NQ; ++q)
+            v[q] = tid >= off ? part[q][tid - off] : 0ull;
+        __syncthreads();
+        for (int q = 0; q < NQ; ++q)
+            part[q][tid] += v[q];
+        __syncthreads();
+    }
+    uint64_t run[NQ];
+    for (int q = 0; q < NQ; ++q)
+        run[q] = part[q][tid] - acc[q];
+    for (uint32_t g = g0; g < g1; ++g)
+    {
+        FtGroup&       G = groups[g];
+        const FtdSizes z = ftd_sizes(G, gflag[g], thin_max);
+        G.ref_item0      = (uint32_t)run[0];
+        G.cell_item0     = (uint32_t)run[1];
+        G.ref_dst0       = run[2];
+        G.cell_dst0      = run[3];
+        G.score0         = run[4];
+        gtile0[g]        = (uint32_t)run[5];
+        gthin0[g]        = (uint32_t)run[6];
+        gseg0[g]         = run[7];
+        run[0] += z.ref_items;
+        run[1] += z.cell_items;
+        run[2] += z.ref_elems;
+        run[3] += z.cell_elems;
+        run[4] += z.scores;
+        run[5] += z.tiles;
+        run[6] += z.thin;
+        run[7] += z.segs;
+        run[8] += z.rthin;
+    }
+    if (smax)
+        atomicMax(&cnt->max_stride, smax);
+    if (tid == FTD_SCAN_NT - 1)
+    {
+        cnt->n_ref_items = part[0][tid];
+        cnt->n_cells     = (uint32_t)part[1][tid];
+        cnt->ref_elems   = part[2][tid];
+        cnt->cell_elems  = part[3][tid];
+        cnt->n_scores    = part[4][tid];
+        cnt->n_tiles     = (uint32_t)part[5][tid];
+        cnt->n_thin      = (uint32_t)part[6][tid];
+        cnt->n_segs      = part[7][tid];
+        cnt->rthin_bound = part[8][tid];
+    }
+}
+
+// Cells with work get their rank item and FtCell (ordered by group, then
+// position, so that their segment offsets increase); cells of groups with
+// fewer than 20 genes end with {front}.  Idempotent (the round is counted by
+// k_ftd_compact), so the autotuner may run it ahead of the round's graph.
+__global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+                            const uint32_t* __restrict__ cgid, const uint32_t* __restrict__ cpos,
+                            const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
+                            const uint64_t* __restrict__ gseg0, uint32_t* __restrict__ counts,
+                            int32_t* __restrict__ rounds, FtRankItem* __restrict__ items,
+                            FtCell* __restrict__ cells)
+{
+    const uint32_t n = cnt->n_active;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t c = act[i];
+        const uint32_t g = cgid[i];
+        if (gflag[g] & FTD_SHORT)
+        {
+            counts[c] = 1;
+            continue;
+        }
+        const FtGroup  G   = groups[g];
+        const uint32_t pos = cpos[i];
+        const uint32_t idx = G.cell_item0 + pos;
+        items[idx]         = FtRankItem{ c, g, G.cell_dst0 + (uint64_t)pos * G.stride };
+        cells[idx]         = FtCell{ c, g, pos, 0u, gseg0[g] + (uint64_t)pos * G.nlabel };
+    }
+}
+
+// Correlation tile lists of every group with work.
+__global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+                            const uint8_t* __restrict__ gflag, uint32_t thin_max,
+                            const uint32_t* __restrict__ gtile0, const uint32_t* __restrict__ gthin0,
+                            FtCorrTile* __restrict__ tiles, FtCorrTile* __restrict__ thin)
+{
+    const uint32_t ng = cnt->n_groups;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x)
+    {
+        const uint8_t f = gflag[g];
+        if (f & FTD_SHORT)
+            continue;
+        const FtGroup G = groups[g];
+        if (G.ncell == 0)
+            continue;
+        if (f & FTD_EXACT)
+        {
+            uint32_t t = gtile0[g];
+            for (uint32_t c0 = 0; c0 < G.ncell; c0 += 16)
+                for (uint32_t r0 = 0; r0 < G.nref; r0 += 256)
+                    tiles[t++] = FtCorrTile{ g, c0, r0, 0u };
+        }
+        else if (G.ncell <= thin_max)
+        {
+            uint32_t t = gthin0[g];
+            for (uint32_t r0 = 0; r0 < G.nref; r0 += 256)
+                thin[t++] = FtCorrTile{ g, 0u, r0, 0u };
+        }
+        else
+        {
+            uint32_t t = gtile0[g];
+            for (uint32_t c0 = 0; c0 < G.ncell; c0 += FT_TC)
+                for (uint32_t r0 = 0; r0 < G.nref; r0 += FT_TR)
+                    tiles[t++] = FtCorrTile{ g, c0, r0, 0u };
+        }
+    }
+}
+
+// ---- work-queue kernels: the grid covers the machine, items come from the
+// ---- device-side counts
+
+// Reference rows, one wave per row (4 rows per block, no block barrier): the
+// block kernel's steps with a per-wave LDS histogram of `nbins` code bins and
+// the row's codes, the code -> doubled centred rank table from a wave scan,
+// and the 128-chunk + stride-tree sum of squares with lane l holding chunks
+// 2l and 2l+1 (their sum is the tree's first level) and shuffles for the
+// remaining six levels.  Per-wave LDS: `wave_ints` ints.
+__device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict__ groups,
+                                        uint32_t ngroups, const uint32_t* __restrict__ labels,
+                                        const uint32_t* __restrict__ lstart,
+                                        const uint32_t* __restrict__ llen,
+                                        const uint32_t* __restrict__ genes,
+                                        const uint16_t* __restrict__ ref_codes, uint32_t Gf,
+                                        int nbins, int16_t* __restrict__ out,
+                                        float* __restrict__ out_sq, int* bins, uint16_t* codes,
+                                        int lane)
+{
+    const uint32_t gi      = find_group(groups, ngroups, item, true);
+    const FtGroup  g       = groups[gi];
+    uint32_t       j       = item - g.ref_item0;
+    uint32_t       src_row = 0;
+    for (uint32_t t = 0; t < g.nlabel; ++t)
+    {
+        const uint32_t l = labels[g.label_off + t];
+        if (j < llen[l])
+        {
+            src_row = lstart[l] + j;
+            break;
+        }
+        j -= llen[l];
+    }
+    const uint32_t  local = item - g.ref_item0;
+    const int       gs    = (int)g.gs;
+    const uint16_t* src   = ref_codes + (size_t)src_row * Gf;
+    const uint32_t* gl    = genes + g.gene_off;
+    for (int i = lane; i < nbins; i += 64)
+        bins[i] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < gs; i += 64)
+    {
+        const uint16_t c = src[gl[i]];
+        codes[i]         = c;
+        atomicAdd(&bins[c], 1);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // code -> 2 * (#less) + (#equal) - gs
+    const int per = (nbins + 63) / 64;
+    const int b0 = min(lane * per, nbins), b1 = min(b0 + per, nbins);
+    int       s  = 0;
+    for (int i = b0; i < b1; ++i)
+        s += bins[i];
+    int start = wave_incl_scan(s) - s;
+    for (int i = b0; i < b1; ++i)
+    {
+        const int c = bins[i];
+        bins[i]     = 2 * start + c - gs;
+        start += c;
+    }
+    __builtin_amdgcn_wave_barrier();
+    int16_t* dst = out + g.ref_dst0 + (uint64_t)local * g.stride;
+    for (int i = lane; i < (int)g.stride; i += 64)
+        dst[i] = i < gs ? (int16_t)bins[codes[i]] : (int16_t)0;
+    const int step = (gs + 127) / 128;
+    float     a0 = 0.f, a1 = 0.f;
+    {
+        const int lo = 2 * lane * step, hi = min(lo + step, gs);
+        for (int i = lo; i < hi; ++i)
+        {
+            const float v = (float)bins[codes[i]];
+            a0            = __builtin_fmaf(v, v, a0);
+        }
+        const int lo1 = lo + step, hi1 = min(lo1 + step, gs);
+        for (int i = lo1; i < hi1; ++i)
+        {
+            const float v = (float)bins[codes[i]];
+            a1            = __builtin_fmaf(v, v, a1);
+        }
+    }
+    float p = a0 + a1;
+    for (int off = 1; off < 64; off <<= 1)
+        p = p + __shfl_down(p, off, 64);
+    if (lane == 0)
+        out_sq[item] = p;
+    __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
+}
+
+__global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
+    const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+    const uint32_t* __restrict__ labels, const uint32_t* __restrict__ lstart,
+    const uint32_t* __restrict__ llen, const uint32_t* __restrict__ genes,
+    const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
+    int16_t* __restrict__ out, float* __restrict__ out_sq)
+{
+    extern __shared__ int smem[];
+    const int      wid   = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int*           bins  = smem + wid * wave_ints;
+    uint16_t*      codes = (uint16_t*)(bins + ((nbins + 3) & ~3));
+    const uint32_t n     = (uint32_t)cnt->n_ref_items;
+    const uint32_t ng    = cnt->n_groups;
+    for (uint32_t item = blockIdx.x * 4 + wid; item < n; item += gridDim.x * 4)
+        ft_rank_ref_wave(item, groups, ng, labels, lstart, llen, genes, ref_codes, Gf, nbins, out,
+                         out_sq, bins, codes, lane);
+}
+
+__global__ __launch_bounds__(128) void k_ftd_rank_ref(
+    const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+    const uint32_t* __restrict__ labels, const uint32_t* __restrict__ lstart,
+    const uint32_t* __restrict__ llen, const uint32_t* __restrict__ genes,
+    const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int16_t* __restrict__ out,
+    float* __restrict__ out_sq)
+{
+    extern __shared__ int smem[];
+    __shared__ float part[128];
+    __shared__ int   tmp[2];
+    const uint32_t   n  = (uint32_t)cnt->n_ref_items;
+    const uint32_t   ng = cnt->n_groups;
+    for (uint32_t item = blockIdx.x; item < n; item += gridDim.x)
+    {
+        __syncthreads();
+        ft_rank_ref_item(item, groups, ng, labels, lstart, llen, genes, ref_codes, Gf, nbins, out,
+                         out_sq, smem, part, tmp);
+    }
+}
+
+__global__ __launch_bounds__(128) void k_ftd_rank_qry(
+    const FtdCounts* __restrict__ cnt, const FtRankItem* __restrict__ items,
+    const FtGroup* __restrict__ groups, const uint32_t* __restrict__ genes,
+    const uint16_t* __restrict__ qdata, const int* __restrict__ qind,
+    const int* __restrict__ qptr, uint32_t Gf, int nbins, int16_t* __restrict__ out,
+    float* __restrict__ out_sq)
+{
+    extern __shared__ int smem[];
+    __shared__ float part[128];
+    __shared__ int   tmp[2];
+    const uint32_t   n = cnt->n_cells;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
+    {
+        __syncthreads();
+        ft_rank_qry_item(items[i], i, groups, genes, qdata, qind, qptr, Gf, nbins, out, out_sq,
+                         smem, part, tmp);
+    }
+}
+
+// Exact tiles (16 cells x 256 rows, INT8 limbs on MFMA).
+__global__ __launch_bounds__(256) void k_ftd_mfma(
+    const FtdCounts* __restrict__ cnt, const FtCorrTile* __restrict__ tiles,
+    const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
+    const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
+    const int16_t* __restrict__ ref_rank, const float* __restrict__ ref_sq,
+    float* __restrict__ scores)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][16 * FT_MFMA_LD];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][256 * FT_MFMA_LD];
+    const uint32_t n = cnt->n_tiles;
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x)
+    {
+        const FtCorrTile tl = tiles[t];
+        if (!(gflag[tl.group] & FTD_EXACT))
+            continue;  // uniform per block: the whole tile is skipped
+        __syncthreads();
+        ft_mfma_tile(tl, groups, cell_rank, cell_sq, ref_rank, ref_sq, scores, sa, sb);
+    }
+}
+
+// FP32 wide tiles (FT_TC x FT_TR) of the FP32-form groups.
+__global__ __launch_bounds__(FT_NT) void k_ftd_corr(
+    const FtdCounts* __restrict__ cnt, const FtCorrTile* __restrict__ tiles,
+    const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
+    const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
+    const int16_t* __restrict__ ref_rank, const float* __restrict__ ref_sq,
+    float* __restrict__ scores)
+{
+    extern __shared__ int srow32[];
+    const uint32_t n = cnt->n_tiles;
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x)
+    {
+        const FtCorrTile tl = tiles[t];
+        if (gflag[tl.group] & FTD_EXACT)
+            continue;
+        __syncthreads();
+        ft_corr_tile<FT_TC, FT_TR, FT_RC, FT_RR>(tl, groups, cell_rank, cell_sq, ref_rank, ref_sq,
+                                                 scores, srow32);
+    }
+}
+
+// FP32 thin tiles; `redo` selects the redo tile list and its count.
+__global__ __launch_bounds__(256) void k_ftd_thin(
+    const FtdCounts* __restrict__ cnt, int redo, const FtCorrTile* __restrict__ tiles,
+    const FtGroup* __restrict__ groups, const int16_t* __restrict__ cell_rank,
+    const float* __restrict__ cell_sq, const int16_t* __restrict__ ref_rank,
+    const float* __restrict__ ref_sq, float* __restrict__ scores)
+{
+    extern __shared__ int16_t cthin[];
+    const uint32_t n = redo ? cnt->n_rthin : cnt->n_thin;
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x)
+    {
+        __syncthreads();
+        ft_corr_thin_tile<FT_THIN>(tiles[t], groups, cell_rank, cell_sq, ref_rank, ref_sq, scores,
+                                   cthin);
+    }
+}
+
+// Segments of the device-resident select: the FtSegments rule with the cell
+// count read from the device.
+struct FtdSegments
+{
+    const FtCell*   cells;
+    const uint32_t* ncells;
+    const FtGroup*  groups;
+    const uint32_t* labels;
+    const uint32_t* label_pos;
+    const uint32_t* llen;
+    __device__ Segment get(uint64_t s) const
+    {
+        FtSegments f{ cells, *ncells, groups, labels, label_pos, llen };
+        return f.get(s);
+    }
+};
+
+// The redo select's items are the global segments listed in `list`.
+struct FtdListSegments
+{
+    FtdSegments     inner;
+    const uint64_t* list;
+    __device__ Segment get(uint64_t i) const { return inner.get(list[i]); }
+};
+
+// One wave per segment, waves pulling segments until the count is reached.
+// redo = 0: all segments (count n_segs, NaN list counted in n_nan);
+// redo = 1: the listed segments (n_redo_segs, n_nan2).
+template <class Src>
+__global__ __launch_bounds__(256) void k_ftd_select(FtdCounts* __restrict__ cnt, int redo, Src src,
+                                                    const float* __restrict__ scores,
+                                                    const PctParam* __restrict__ params, float p,
+                                                    float* __restrict__ out,
+                                                    uint64_t* __restrict__ nan_list)
+{
+    __shared__ WaveSelectSmem sm[256 / kWave];
+    const int      wid  = threadIdx.x / kWave;
+    const int      lane = threadIdx.x & (kWave - 1);
+    const uint64_t n    = redo ? cnt->n_redo_segs : cnt->n_segs;
+    unsigned int*  nanc = redo ? &cnt->n_nan2 : &cnt->n_nan;
+    for (uint64_t item = (uint64_t)blockIdx.x * (256 / kWave) + wid; item < n;
+         item += (uint64_t)gridDim.x * (256 / kWave))
+        select_item(src, item, scores, params, p, out, nan_list, nanc, sm[wid], lane);
+}
+
+// NaN-carrying segments: the libstdc++ sort emulation, in place in the
+// score buffer (the segment is not read again this round).
+// Exact-form segments of the first select keep their NaN (gflag given): the
+// certified filter sends their cells to the FP32 redo.
+template <class Src>
+__global__ void k_ftd_select_nan(FtdCounts* __restrict__ cnt, int redo, Src src,
+                                 const uint64_t* __restrict__ nan_list, float* __restrict__ scores,
+                                 const PctParam* __restrict__ params, float p,
+                                 float* __restrict__ out, const FtCell* __restrict__ cells,
+                                 const uint8_t* __restrict__ gflag)
+{
+    const uint32_t n = redo ? cnt->n_nan2 : cnt->n_nan;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint64_t item = nan_list[i];
+        if (!redo)
+        {
+            // the cell holding segment `item` (last seg0 <= item)
+            uint32_t lo = 0, hi = cnt->n_cells;
+            while (hi - lo > 1)
+            {
+                const uint32_t mid = (lo + hi) / 2;
+                if (cells[mid].seg0 <= item)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            if (gflag[cells[lo].group] & FTD_EXACT)
+                continue;
+        }
+        const Segment  sg   = src.get(item);
+        const PctParam pp   = params[sg.param];
+        float*         a    = scores + sg.off;
+        if (!std_sort(a, (long)sg.n))
+        {
+            atomicOr(&cnt->undefined, 1u);
+            out[item] = __builtin_nanf("");
+            continue;
+        }
+        float r;
+        if (pp.kind == 1)
+            r = a[0];
+        else if (pp.kind == 2)
+            r = a[sg.n - 1];
+        else
+            r = pct_interp(p, pp.prev_index, pp.step, a[pp.k - 1], a[pp.k]);
+        out[item] = r;
+    }
+}
+
+// Filter of every cell with work, candidate lists updated in place (the new
+// list is formed in `stage` first).  Exact-form cells use the certified
+// filter; an undecided one is listed for the FP32 redo, with its tiles and
+// segments.
+__global__ void k_ftd_filter(FtdCounts* __restrict__ cnt, const FtCell* __restrict__ cells,
+                             const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
+                             const uint32_t* __restrict__ labels, const float* __restrict__ P,
+                             uint8_t* __restrict__ lists, uint32_t* __restrict__ counts,
+                             uint32_t L, float thre, uint8_t* __restrict__ stage,
+                             uint32_t* __restrict__ redo, FtCorrTile* __restrict__ rthin,
+                             uint64_t* __restrict__ rseg)
+{
+    const uint32_t n = cnt->n_cells;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const FtCell  fc  = cells[i];
+        const FtGroup G   = groups[fc.group];
+        uint8_t*      in  = lists + (size_t)fc.cell * L;
+        uint8_t*      out = stage + (size_t)i * L;
+        const uint32_t k  = counts[fc.cell];
+        uint32_t      m   = 0;
+        if (gflag[fc.group] & FTD_EXACT)
+        {
+            if (ft_filter_exact_cell(fc, G, labels + G.label_off, P, in, k, thre, out, &m))
+            {
+                uint32_t t = atomicAdd(&cnt->n_rthin, (G.nref + 255) / 256);
+                for (uint32_t r0 = 0; r0 < G.nref; r0 += 256)
+                    rthin[t++] = FtCorrTile{ fc.group, fc.pos, r0, 1u };
+                const uint64_t s0 = atomicAdd((unsigned long long*)&cnt->n_redo_segs,
+                                              (unsigned long long)G.nlabel);
+                for (uint32_t j = 0; j < G.nlabel; ++j)
+                    rseg[s0 + j] = fc.seg0 + j;
+                redo[atomicAdd(&cnt->n_redo, 1u)] = i;
+                atomicAdd(&cnt->tot_redo, 1u);
+                continue;
+            }
+        }
+        else
+            m = ft_filter_cell(fc, G, labels + G.label_off, P, in, k, thre, out);
+        for (uint32_t t = 0; t < m; ++t)
+            in[t] = out[t];
+        counts[fc.cell] = m;
+    }
+}
+
+// Quantiles of the redo segments back into the per-segment array.
+__global__ void k_ftd_redo_scatter(const FtdCounts* __restrict__ cnt,
+                                   const uint64_t* __restrict__ rseg,
+                                   const float* __restrict__ pred, float* __restrict__ P)
+{
+    const uint64_t n = cnt->n_redo_segs;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        P[rseg[i]] = pred[i];
+}
+
+// FP32 filter of the redo cells (their certified filter left the lists
+// untouched).
+__global__ void k_ftd_filter_redo(const FtdCounts* __restrict__ cnt,
+                                  const uint32_t* __restrict__ redo,
+                                  const FtCell* __restrict__ cells,
+                                  const FtGroup* __restrict__ groups,
+                                  const uint32_t* __restrict__ labels, const float* __restrict__ P,
+                                  uint8_t* __restrict__ lists, uint32_t* __restrict__ counts,
+                                  uint32_t L, float thre, uint8_t* __restrict__ stage)
+{
+    const uint32_t n = cnt->n_redo;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x)
+    {
+        const uint32_t i  = redo[r];
+        const FtCell   fc = cells[i];
+        const FtGroup  G  = groups[fc.group];
+        uint8_t*       in = lists + (size_t)fc.cell * L;
+        uint8_t*       out = stage + (size_t)i * L;
+        const uint32_t m  = ft_filter_cell(fc, G, labels + G.label_off, P, in, counts[fc.cell],
+                                           thre, out);
+        for (uint32_t t = 0; t < m; ++t)
+            in[t] = out[t];
+        counts[fc.cell] = m;
+    }
+}
+
+// Every active cell counts the round; the next round takes the cells with
+// several candidates and rounds left.
+__global__ void k_ftd_compact(FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+                              const uint32_t* __restrict__ counts, int32_t* __restrict__ rounds,
+                              int times, uint32_t* __restrict__ act_next)
+{
+    const uint32_t n = cnt->n_active;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t c = act[i];
+        const int32_t  r = rounds[c] + 1;
+        rounds[c]        = r;
+        if (counts[c] > 1 && (times == 0 || r < times))
+            act_next[atomicAdd(&cnt->n_next, 1u)] = c;
+    }
+}
+
+// Counter reset for the autotuner's repeated select launches.
+__global__ void k_ftd_reset_nan(FtdCounts* __restrict__ cnt)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        cnt->n_nan = 0;
+}
+
+__global__ void k_ftd_final(const uint8_t* __restrict__ lists, const uint32_t* __restrict__ counts,
+                            uint32_t Nq, uint32_t L, uint32_t* __restrict__ final_labels)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < Nq)
+        final_labels[c] = counts[c] == 0 ? 0u : (uint32_t)lists[(size_t)c * L];
+}
+
+}  // namespace csg

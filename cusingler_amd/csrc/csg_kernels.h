@@ -690,19 +690,13 @@ __device__ inline float pct_from_stats(const PctParam& pp, float p, float a_km1,
 // of the segment's scores.  Segments holding a NaN are appended to `nan_list`
 // (the libstdc++ sort emulation runs on them in k_select_nan).
 template <class Src>
-__global__ __launch_bounds__(256) void k_select(Src src, uint64_t n_items,
-                                                const float* __restrict__ scores,
-                                                const PctParam* __restrict__ params, float p,
-                                                float* __restrict__ out,
-                                                uint64_t* __restrict__ nan_list,
-                                                unsigned int* __restrict__ nan_count)
+__device__ inline void select_item(const Src& src, uint64_t item,
+                                   const float* __restrict__ scores,
+                                   const PctParam* __restrict__ params, float p,
+                                   float* __restrict__ out, uint64_t* __restrict__ nan_list,
+                                   unsigned int* __restrict__ nan_count, WaveSelectSmem& wsm,
+                                   int lane)
 {
-    __shared__ WaveSelectSmem sm[256 / kWave];
-    const int      wid  = threadIdx.x / kWave;
-    const int      lane = threadIdx.x & (kWave - 1);
-    const uint64_t item = (uint64_t)blockIdx.x * (256 / kWave) + wid;
-    if (item >= n_items)
-        return;
     const Segment  sg = src.get(item);
     const float*   v  = scores + sg.off;
     const PctParam pp = params[sg.param];
@@ -745,7 +739,7 @@ __global__ __launch_bounds__(256) void k_select(Src src, uint64_t n_items,
         hi = wave_max(hi);
         float a0 = 0.f, a1 = 0.f;
         if (pp.kind == 3)
-            wave_select2_reg<16>(x, v, sg.n, lo, hi, (uint32_t)(pp.k - 1), (uint32_t)pp.k, sm[wid],
+            wave_select2_reg<16>(x, v, sg.n, lo, hi, (uint32_t)(pp.k - 1), (uint32_t)pp.k, wsm,
                                  &a0, &a1);
         if (lane == 0)
             out[item] = pct_from_stats(pp, p, a0, a1, lo, hi);
@@ -766,7 +760,7 @@ __global__ __launch_bounds__(256) void k_select(Src src, uint64_t n_items,
     }
     float a0 = 0.f, a1 = 0.f, vmin = 0.f, vmax = 0.f;
     if (pp.kind == 3)
-        wave_select2(v, sg.n, (uint32_t)(pp.k - 1), (uint32_t)pp.k, sm[wid], &a0, &a1);
+        wave_select2(v, sg.n, (uint32_t)(pp.k - 1), (uint32_t)pp.k, wsm, &a0, &a1);
     else
     {
         float lo = __builtin_inff(), hi = -__builtin_inff();
@@ -780,6 +774,23 @@ __global__ __launch_bounds__(256) void k_select(Src src, uint64_t n_items,
     }
     if (lane == 0)
         out[item] = pct_from_stats(pp, p, a0, a1, vmin, vmax);
+}
+
+template <class Src>
+__global__ __launch_bounds__(256) void k_select(Src src, uint64_t n_items,
+                                                const float* __restrict__ scores,
+                                                const PctParam* __restrict__ params, float p,
+                                                float* __restrict__ out,
+                                                uint64_t* __restrict__ nan_list,
+                                                unsigned int* __restrict__ nan_count)
+{
+    __shared__ WaveSelectSmem sm[256 / kWave];
+    const int      wid  = threadIdx.x / kWave;
+    const int      lane = threadIdx.x & (kWave - 1);
+    const uint64_t item = (uint64_t)blockIdx.x * (256 / kWave) + wid;
+    if (item >= n_items)
+        return;
+    select_item(src, item, scores, params, p, out, nan_list, nan_count, sm[wid], lane);
 }
 
 // NaN-carrying segments: one lane copies the segment to `work` and runs the

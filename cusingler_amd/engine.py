@@ -73,6 +73,8 @@ def lib():
     L.csg_finetune.argtypes = [_V, _F, _F, C.c_int, _V, _V]
     L.csg_destroy.argtypes = [_V]
     L.csg_set_finetune_path.argtypes = [_V, C.c_int]
+    L.csg_set_finetune_schedule.argtypes = [_V, C.c_int]
+    L.csg_finetune_tuning.argtypes = [_V, C.c_int, _V, _V]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
     L.csg_selftest.argtypes = [_V, C.c_int, P(C.c_double)]
@@ -246,6 +248,21 @@ class Engine:
 
     def set_finetune_path(self, path: str):
         self._check(lib().csg_set_finetune_path(self._h, PATHS[path]), "set_finetune_path")
+
+    def set_finetune_schedule(self, schedule: str):
+        """"device" (rounds on the device, HIP graphs) or "host" (host grouping)."""
+        self._check(lib().csg_set_finetune_schedule(self._h, {"device": 0, "host": 1}[schedule]),
+                    "set_finetune_schedule")
+
+    def finetune_tuning(self, autotune: Optional[bool] = None) -> Dict[str, object]:
+        """Autotuner state of the device-resident fine-tune (optionally switching
+        it on / off, which asks for a new tuning)."""
+        g = np.zeros(4, np.int32)
+        ms = np.zeros(3, np.float32)
+        a = -1 if autotune is None else int(bool(autotune))
+        self._check(lib().csg_finetune_tuning(self._h, a, _p(g), _p(ms)), "finetune_tuning")
+        return {"tuned": bool(g[0]), "rank_blocks": int(g[1]), "corr_blocks": int(g[2]),
+                "select_blocks": int(g[3]), "best_ms": [float(v) for v in ms]}
 
     def finetune(self, quantile: float = 0.8, thre: float = 0.05, times: int = 0):
         final = np.zeros(self.fNq, np.uint32)

@@ -110,6 +110,23 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
  * it ran; this call overrides that.  No counterpart in the reference. */
 int csg_set_finetune_path(csg_engine* eng, int path);
 
+/* Fine-tune schedule: 0 (default) runs every round on the device -- cells
+ * grouped by candidate set through a device hash table, per-group gene
+ * subsets built from the pair windows, work-queue kernels over device-side
+ * counts, each round replayed from two captured HIP graphs with one counter
+ * read-back; 1 groups on the host and runs the device work in batches.
+ * Labels and round counts are the same.  No counterpart in the reference. */
+int csg_set_finetune_schedule(csg_engine* eng, int schedule);
+
+/* Autotuner of the device-resident fine-tune: on the first round with enough
+ * work after the fine-tune inputs change, the block counts of the reference
+ * rank, correlation and select work-queue kernels are chosen by timing each
+ * candidate on that round's work.  autotune = 1 / 0 turns it on / off (and
+ * asks for a new tuning), -1 leaves it; grids[0..3] receive (tuned, rank,
+ * correlation, select block counts) and best_ms[0..2] the tuned kernels'
+ * times (either may be NULL).  No counterpart in the reference. */
+int csg_finetune_tuning(csg_engine* eng, int autotune, int32_t* grids, float* best_ms);
+
 /* destroy (src/cusingler.cuh:25): frees the fine-tune buffers. */
 int csg_destroy(csg_engine* eng);
 

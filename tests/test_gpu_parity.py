@@ -63,8 +63,9 @@ def test_score_step_plain_bit_exact(eng, name):
     assert np.array_equal(mask, om), f"{name} candidate masks differ"
 
 
+@pytest.mark.parametrize("schedule", ["device", "host"])
 @pytest.mark.parametrize("name", ALL)
-def test_finetune_plain_matches_oracle(eng, name):
+def test_finetune_plain_matches_oracle(eng, name, schedule):
     from cusingler_amd import engine
     from oracle import oracle as O
     ds = _datasets()[name]()
@@ -73,15 +74,20 @@ def test_finetune_plain_matches_oracle(eng, name):
     _, om, _ = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
     of, orounds = orc.finetune(0, ds.qry_height, om, 0.8, 0.05, 0, O.VAR_SEQ)
     c1 = prep.generate(1)
-    _say("finetune plain", name, ds.qry_height, c1.width)
+    _say("finetune plain", name, schedule, ds.qry_height, c1.width)
     eng.copyin(c1, om)
-    final, rounds = eng.finetune(0.8, 0.05, 0)
+    eng.set_finetune_schedule(schedule)
+    try:
+        final, rounds = eng.finetune(0.8, 0.05, 0)
+    finally:
+        eng.set_finetune_schedule("device")
     eng.destroy()
     assert np.array_equal(final, of), f"{name}: {(final != of).sum()} final labels differ"
     assert np.array_equal(rounds, orounds), f"{name}: round counts differ"
 
 
-def test_finetune_times_limit(eng):
+@pytest.mark.parametrize("schedule", ["device", "host"])
+def test_finetune_times_limit(eng, schedule):
     from cusingler_amd import engine, synth
     from oracle import oracle as O
     ds = synth.generate("small")
@@ -92,9 +98,13 @@ def test_finetune_times_limit(eng):
     _, om2, _ = orc.score(0, ds.qry_height, 0.8, 0.3, O.VAR_SEQ)
     for times in (1, 2):
         of, _ = orc.finetune(0, ds.qry_height, om2, 0.8, 0.3, times, O.VAR_SEQ)
-        _say("finetune times", times)
+        _say("finetune times", schedule, times)
         eng.copyin(prep.generate(1), om2)
-        final, rounds = eng.finetune(0.8, 0.3, times)
+        eng.set_finetune_schedule(schedule)
+        try:
+            final, rounds = eng.finetune(0.8, 0.3, times)
+        finally:
+            eng.set_finetune_schedule("device")
         eng.destroy()
         assert np.array_equal(final, of)
         assert rounds.max() <= times

@@ -105,9 +105,10 @@ def test_approximate_paths_error(eng, path):
     assert agree > 0.99
 
 
+@pytest.mark.parametrize("schedule", ["device", "host"])
 @pytest.mark.parametrize("name", ["tiny", "tiny_dirichlet", "tiny_nan", "small", "pancreas_2k",
                                   "atlas_1k"])
-def test_exact_finetune_matches_reference_order(eng, name):
+def test_exact_finetune_matches_reference_order(eng, name, schedule):
     """The certified exact fine-tune gives the reference's final labels and
     round counts (cells it cannot certify are redone in FP32 order)."""
     from cusingler_amd import engine
@@ -117,13 +118,17 @@ def test_exact_finetune_matches_reference_order(eng, name):
     orc = O.Oracle(ds, 8)
     _, om, _ = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
     of, orounds = orc.finetune(0, ds.qry_height, om, 0.8, 0.05, 0, O.VAR_SEQ)
-    _say("exact finetune", name, ds.qry_height)
+    _say("exact finetune", name, schedule, ds.qry_height)
     eng.copyin(prep.generate(1), om)
     eng.set_finetune_path("exact_int8")
-    final, rounds = eng.finetune(0.8, 0.05, 0)
+    eng.set_finetune_schedule(schedule)
+    try:
+        final, rounds = eng.finetune(0.8, 0.05, 0)
+    finally:
+        eng.set_finetune_schedule("device")
+        eng.set_finetune_path("plain")
     cnt = eng.counters()
     eng.destroy()
-    eng.set_finetune_path("plain")
     assert np.array_equal(final, of), f"{(final != of).sum()} final labels differ"
     assert np.array_equal(rounds, orounds)
     print(f"[exact-ft] {name}: redone cells {cnt['ft_redone_cells']} of {cnt['ft_items']} "
