@@ -830,7 +830,11 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
         return rc;
     {
         PhaseScope ps(eng, 4, 3);
-        k_rescore<<<n, 64, 0, eng->stream>>>(eng->d_items, eng->d_item_off, eng->d_Xq, eng->d_Xr,
+        uint32_t max_len = 1;
+        for (uint32_t l = 0; l < eng->L; ++l)
+            max_len = std::max(max_len, eng->ctidx[2 * l + 1]);
+        k_rescore<<<dim3(n, (max_len + 63) / 64), 64, 0, eng->stream>>>(
+            eng->d_items, eng->d_item_off, eng->d_Xq, eng->d_Xr,
                                              (int)eng->Kp, eng->d_sqq_f, eng->d_sqr_f,
                                              eng->d_lstart, eng->d_llen, eng->d_S);
         LAUNCHCHK();

@@ -120,8 +120,8 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
 
 // FP32 scores of one query row against every reference row of one label in
 // the reference's sequential order (the plain contraction restricted to a
-// (cell, label) pair).  One wave per item; 64 reference rows at a time, one
-// per lane; each 64 x 64 tile of ranks is staged with 16-byte loads into LDS
+// (cell, label) pair).  One wave per (item, 64 reference rows), one row per
+// lane; each 64 x 64 tile of ranks is staged with 16-byte loads into LDS
 // rows of 33 words (an odd pitch, so the lanes' column reads do not collide).
 __global__ __launch_bounds__(64) void k_rescore(const RecheckItem* __restrict__ items,
                                                 const uint64_t* __restrict__ out_off,
@@ -143,7 +143,10 @@ __global__ __launch_bounds__(64) void k_rescore(const RecheckItem* __restrict__ 
     const float       sq   = sq_q[it.q];
     float*            o    = out + out_off[blockIdx.x];
     const int16_t*    t16  = (const int16_t*)tile;
-    for (uint32_t base = 0; base < n; base += 64)
+    // block (item, y) takes reference rows [64 y, 64 y + 64) of the label
+    const uint32_t    base = blockIdx.y * 64;
+    if (base >= n)
+        return;  // uniform over the block
     {
         const uint32_t nrow = min(64u, n - base);
         float          acc  = 0.f;

@@ -133,3 +133,40 @@ def test_exact_finetune_matches_reference_order(eng, name, schedule):
     assert np.array_equal(rounds, orounds)
     print(f"[exact-ft] {name}: redone cells {cnt['ft_redone_cells']} of {cnt['ft_items']} "
           f"(cell, round) items", flush=True)
+
+
+def test_device_finetune_graph_replays_and_autotuner(eng):
+    """The device-resident fine-tune replays its captured round graphs across
+    rounds and calls: three calls in a row on one engine (with eager work and
+    synchronisations between them), autotuned and untuned, all give the
+    host-grouped schedule's labels and round counts."""
+    from cusingler_amd import engine, synth
+    ds = synth.generate("atlas", n_qry=8000, n_ref=5000)
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    _say("device fine-tune replays", ds.qry_height)
+    eng.copyin_score(c0)
+    eng.get_label(0.8, 0.05, "exact_int8", outputs=False)
+    eng.copyin(c1)
+    eng.set_finetune_path("exact_int8")
+    try:
+        eng.set_finetune_schedule("host")
+        ref_final, ref_rounds = eng.finetune(0.8, 0.05, 0)
+        eng.set_finetune_schedule("device")
+        for autotune in (True, False):
+            eng.finetune_tuning(autotune)
+            for call in range(3):
+                final, rounds = eng.finetune(0.8, 0.05, 0)
+                assert np.array_equal(final, ref_final), (autotune, call)
+                assert np.array_equal(rounds, ref_rounds), (autotune, call)
+                # eager work between the replays
+                eng.get_label(0.8, 0.05, "plain", outputs=True)
+            t = eng.finetune_tuning()
+            _say("tuning", t)
+            assert t["tuned"] == autotune
+        eng.finetune_tuning(True)
+    finally:
+        eng.set_finetune_schedule("device")
+        eng.set_finetune_path("plain")
+    eng.destroy()
+    eng.destroy_score()
