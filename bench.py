@@ -269,6 +269,7 @@ def main():
     if "exact_int8" in res and chosen != "exact_int8":
         e = res["exact_int8"]
         out["exact_int8"] = {"value": round(e["value"], 1), "ms_per_step": round(e["ms"], 3),
+                             "phases_ms": {k: round(v[0] / steps, 3) for k, v in e["phases"].items()},
                              "label_match_vs_plain": {
                                  "first": float((e["first"] == plain["first"]).mean()),
                                  "final": float((e["final"] == plain["final"]).mean())}}
@@ -280,6 +281,7 @@ def main():
                                fallback_margin_cells=c["prop_fallback_margin"],
                                fallback_edge_cells=c["prop_fallback_check"],
                                widened_cells=c["prop_widened"])
+    out["finetune_autotuner"] = eng.finetune_tuning()
     eng.close()
     if not args.no_cpu:
         n = min(args.cpu_sample, Nq)
