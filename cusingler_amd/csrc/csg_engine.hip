@@ -157,10 +157,12 @@ struct csg_engine
     uint8_t*    d_fl_stage = nullptr;
     uint32_t*   d_fl_redo  = nullptr;
     uint64_t *  d_fl_rseg = nullptr, *d_fl_nan1 = nullptr, *d_fl_nan2 = nullptr;
+    uint32_t *  d_fl_segcell = nullptr, *d_fl_refgroup = nullptr;
     float*      d_fl_pred = nullptr;
     struct
     {
-        size_t items, cells, stage, redo, tiles, thin, rthin, rseg, pred, nan1, nan2;
+        size_t items, cells, stage, redo, tiles, thin, rthin, rseg, pred, nan1, nan2, segcell,
+            refgroup;
     } fl_cap{};
     uint32_t               ftd_gcap = 0, ftd_T = 0;
     size_t                 ftd_gstride = 0;
@@ -2017,6 +2019,8 @@ int ftd_free(csg_engine* eng)
     dfree(eng->d_fl_pred);
     dfree(eng->d_fl_nan1);
     dfree(eng->d_fl_nan2);
+    dfree(eng->d_fl_segcell);
+    dfree(eng->d_fl_refgroup);
     eng->fl_cap = {};
     eng->ftd_gcap = 0;
     eng->ftd_T    = 0;
@@ -2105,7 +2109,9 @@ int ftd_work(csg_engine* eng, const FtdCounts& h)
         (rc = ensure(eng, &eng->d_fl_rseg, &c.rseg, std::max<uint64_t>(h.n_segs, 1))) ||
         (rc = ensure(eng, &eng->d_fl_pred, &c.pred, std::max<uint64_t>(h.n_segs, 1))) ||
         (rc = ensure(eng, &eng->d_fl_nan1, &c.nan1, std::max<uint64_t>(h.n_segs, 1))) ||
-        (rc = ensure(eng, &eng->d_fl_nan2, &c.nan2, std::max<uint64_t>(h.n_segs, 1))))
+        (rc = ensure(eng, &eng->d_fl_nan2, &c.nan2, std::max<uint64_t>(h.n_segs, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_segcell, &c.segcell, std::max<uint64_t>(h.n_segs, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_refgroup, &c.refgroup, std::max<uint64_t>(h.n_ref_items, 1))))
         return rc;
     return CSG_OK;
 }
@@ -2176,7 +2182,8 @@ int ftd_launch_rank_ref(csg_engine* eng, int grid)
     FtdCounts*     cnt = eng->d_fl_cnt;
     if (d.rank_ref_w <= 65536)
         k_ftd_rank_ref_w<<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
-            cnt, eng->d_fl_groups, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
+            cnt, eng->d_fl_groups, eng->d_fl_refgroup, eng->d_fl_glabels, eng->d_flstart,
+            eng->d_fllen,
             eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
             eng->d_ft_rsq);
     else
@@ -2207,8 +2214,8 @@ int ftd_launch_corr(csg_engine* eng, int grid)
 int ftd_launch_select(csg_engine* eng, int grid, float quantile)
 {
     FtdCounts*        cnt = eng->d_fl_cnt;
-    const FtdSegments seg{ eng->d_fl_cells, &cnt->n_cells, eng->d_fl_groups, eng->d_fl_glabels,
-                           eng->d_fl_glpos, eng->d_fllen };
+    const FtdSegments seg{ eng->d_fl_cells, eng->d_fl_segcell, eng->d_fl_groups,
+                           eng->d_fl_glabels, eng->d_fl_glpos, eng->d_fllen };
     k_ftd_select<FtdSegments><<<(unsigned)grid, 256, 0, eng->stream>>>(
         cnt, 0, seg, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P, eng->d_fl_nan1);
     LAUNCHCHK();
@@ -2224,13 +2231,16 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
     const unsigned   g   = (unsigned)eng->ftd_grid_threads;
     const FtdLds     d   = ftd_lds(eng);
     FtdCounts*       cnt = eng->d_fl_cnt;
-    const FtdSegments seg{ eng->d_fl_cells, &cnt->n_cells, eng->d_fl_groups, eng->d_fl_glabels,
-                           eng->d_fl_glpos, eng->d_fllen };
+    const FtdSegments seg{ eng->d_fl_cells, eng->d_fl_segcell, eng->d_fl_groups,
+                           eng->d_fl_glabels, eng->d_fl_glpos, eng->d_fllen };
     const FtdListSegments rseg{ seg, eng->d_fl_rseg };
     k_ftd_cells<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_cgid, eng->d_fl_cpos,
                                                 eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_gseg0,
                                                 eng->d_fl_counts, eng->d_fl_rounds,
-                                                eng->d_fl_items, eng->d_fl_cells);
+                                                eng->d_fl_items, eng->d_fl_cells,
+                                                eng->d_fl_segcell);
+    k_ftd_refmap<<<(unsigned)eng->ftd_grid_setup, 256, 0, eng->stream>>>(
+        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup);
     if (ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
         return CSG_ERR_DEVICE;
     k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
@@ -2244,7 +2254,7 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
         return CSG_ERR_DEVICE;
     k_ftd_select_nan<FtdSegments><<<g, FTD_TPB, 0, eng->stream>>>(
         cnt, 0, seg, eng->d_fl_nan1, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P,
-        eng->d_fl_cells, eng->d_fl_gflag);
+        eng->d_fl_cells, eng->d_fl_segcell, eng->d_fl_gflag);
     k_ftd_filter<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_cells, eng->d_fl_groups,
                                                  eng->d_fl_gflag, eng->d_fl_glabels, eng->d_ft_P,
                                                  eng->d_fl_lists, eng->d_fl_counts, L, thre,
@@ -2258,7 +2268,7 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
         cnt, 1, rseg, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_fl_pred, eng->d_fl_nan2);
     k_ftd_select_nan<FtdListSegments><<<g, FTD_TPB, 0, eng->stream>>>(
         cnt, 1, rseg, eng->d_fl_nan2, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_fl_pred,
-        eng->d_fl_cells, eng->d_fl_gflag);
+        eng->d_fl_cells, eng->d_fl_segcell, eng->d_fl_gflag);
     k_ftd_redo_scatter<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_rseg, eng->d_fl_pred,
                                                        eng->d_ft_P);
     k_ftd_filter_redo<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_redo, eng->d_fl_cells,
@@ -2312,7 +2322,10 @@ int ftd_tune(csg_engine* eng, float quantile)
     k_ftd_cells<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_cgid, eng->d_fl_cpos,
                                                 eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_gseg0,
                                                 eng->d_fl_counts, eng->d_fl_rounds,
-                                                eng->d_fl_items, eng->d_fl_cells);
+                                                eng->d_fl_items, eng->d_fl_cells,
+                                                eng->d_fl_segcell);
+    k_ftd_refmap<<<(unsigned)eng->ftd_grid_setup, 256, 0, eng->stream>>>(
+        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup);
     k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
         cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
         eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
@@ -2408,6 +2421,7 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->d_fl_redo, (uintptr_t)eng->d_fl_tiles, (uintptr_t)eng->d_fl_thin,
              (uintptr_t)eng->d_fl_rthin, (uintptr_t)eng->d_fl_rseg, (uintptr_t)eng->d_fl_pred,
              (uintptr_t)eng->d_fl_nan1, (uintptr_t)eng->d_fl_nan2, (uintptr_t)eng->d_fpct,
+             (uintptr_t)eng->d_fl_segcell, (uintptr_t)eng->d_fl_refgroup,
              (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)q,
              (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,
              (uintptr_t)eng->ftd_grid_corr, (uintptr_t)eng->ftd_grid_select,

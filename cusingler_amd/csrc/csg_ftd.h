@@ -459,14 +459,17 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
 
 // Cells with work get their rank item and FtCell (ordered by group, then
 // position, so that their segment offsets increase); cells of groups with
-// fewer than 20 genes end with {front}.  Idempotent (the round is counted by
-// k_ftd_compact), so the autotuner may run it ahead of the round's graph.
+// fewer than 20 genes end with {front}.  Each segment records its cell
+// (segcell), each reference rank item its group (k_ftd_refmap), so that the
+// work-queue kernels find them without a search.  Idempotent (the round is
+// counted by k_ftd_compact), so the autotuner may run it ahead of the
+// round's graph.
 __global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
                             const uint32_t* __restrict__ cgid, const uint32_t* __restrict__ cpos,
                             const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
                             const uint64_t* __restrict__ gseg0, uint32_t* __restrict__ counts,
                             int32_t* __restrict__ rounds, FtRankItem* __restrict__ items,
-                            FtCell* __restrict__ cells)
+                            FtCell* __restrict__ cells, uint32_t* __restrict__ segcell)
 {
     const uint32_t n = cnt->n_active;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -481,8 +484,26 @@ __global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* _
         const FtGroup  G   = groups[g];
         const uint32_t pos = cpos[i];
         const uint32_t idx = G.cell_item0 + pos;
+        const uint64_t s0  = gseg0[g] + (uint64_t)pos * G.nlabel;
         items[idx]         = FtRankItem{ c, g, G.cell_dst0 + (uint64_t)pos * G.stride };
-        cells[idx]         = FtCell{ c, g, pos, 0u, gseg0[g] + (uint64_t)pos * G.nlabel };
+        cells[idx]         = FtCell{ c, g, pos, 0u, s0 };
+        for (uint32_t j = 0; j < G.nlabel; ++j)
+            segcell[s0 + j] = idx;
+    }
+}
+
+// Group of every reference rank item (one block per group).
+__global__ void k_ftd_refmap(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+                             const uint8_t* __restrict__ gflag, uint32_t* __restrict__ refgroup)
+{
+    const uint32_t ng = cnt->n_groups;
+    for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x)
+    {
+        if (gflag[g] & FTD_SHORT)
+            continue;
+        const FtGroup G = groups[g];
+        for (uint32_t r = threadIdx.x; r < G.nref; r += blockDim.x)
+            refgroup[G.ref_item0 + r] = g;
     }
 }
 
@@ -534,7 +555,8 @@ __global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __
 // 2l and 2l+1 (their sum is the tree's first level) and shuffles for the
 // remaining six levels.  Per-wave LDS: `wave_ints` ints.
 __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict__ groups,
-                                        uint32_t ngroups, const uint32_t* __restrict__ labels,
+                                        const uint32_t* __restrict__ refgroup,
+                                        const uint32_t* __restrict__ labels,
                                         const uint32_t* __restrict__ lstart,
                                         const uint32_t* __restrict__ llen,
                                         const uint32_t* __restrict__ genes,
@@ -543,8 +565,7 @@ __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict
                                         float* __restrict__ out_sq, int* bins, uint16_t* codes,
                                         int lane)
 {
-    const uint32_t gi      = find_group(groups, ngroups, item, true);
-    const FtGroup  g       = groups[gi];
+    const FtGroup  g       = groups[refgroup[item]];
     uint32_t       j       = item - g.ref_item0;
     uint32_t       src_row = 0;
     for (uint32_t t = 0; t < g.nlabel; ++t)
@@ -614,6 +635,7 @@ __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict
 
 __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+    const uint32_t* __restrict__ refgroup,
     const uint32_t* __restrict__ labels, const uint32_t* __restrict__ lstart,
     const uint32_t* __restrict__ llen, const uint32_t* __restrict__ genes,
     const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
@@ -624,10 +646,9 @@ __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     int*           bins  = smem + wid * wave_ints;
     uint16_t*      codes = (uint16_t*)(bins + ((nbins + 3) & ~3));
     const uint32_t n     = (uint32_t)cnt->n_ref_items;
-    const uint32_t ng    = cnt->n_groups;
     for (uint32_t item = blockIdx.x * 4 + wid; item < n; item += gridDim.x * 4)
-        ft_rank_ref_wave(item, groups, ng, labels, lstart, llen, genes, ref_codes, Gf, nbins, out,
-                         out_sq, bins, codes, lane);
+        ft_rank_ref_wave(item, groups, refgroup, labels, lstart, llen, genes, ref_codes, Gf, nbins,
+                         out, out_sq, bins, codes, lane);
 }
 
 __global__ __launch_bounds__(128) void k_ftd_rank_ref(
@@ -729,19 +750,23 @@ __global__ __launch_bounds__(256) void k_ftd_thin(
 }
 
 // Segments of the device-resident select: the FtSegments rule with the cell
-// count read from the device.
+// of each segment looked up instead of searched.
 struct FtdSegments
 {
     const FtCell*   cells;
-    const uint32_t* ncells;
+    const uint32_t* segcell;  // cell item of every segment
     const FtGroup*  groups;
     const uint32_t* labels;
     const uint32_t* label_pos;
     const uint32_t* llen;
     __device__ Segment get(uint64_t s) const
     {
-        FtSegments f{ cells, *ncells, groups, labels, label_pos, llen };
-        return f.get(s);
+        const FtCell   fc = cells[segcell[s]];
+        const FtGroup  g  = groups[fc.group];
+        const uint32_t j  = (uint32_t)(s - fc.seg0);
+        const uint32_t l  = labels[g.label_off + j];
+        return Segment{ g.score0 + (uint64_t)fc.pos * g.nref + label_pos[g.label_off + j], llen[l],
+                        l };
     }
 };
 
@@ -782,27 +807,15 @@ __global__ void k_ftd_select_nan(FtdCounts* __restrict__ cnt, int redo, Src src,
                                  const uint64_t* __restrict__ nan_list, float* __restrict__ scores,
                                  const PctParam* __restrict__ params, float p,
                                  float* __restrict__ out, const FtCell* __restrict__ cells,
+                                 const uint32_t* __restrict__ segcell,
                                  const uint8_t* __restrict__ gflag)
 {
     const uint32_t n = redo ? cnt->n_nan2 : cnt->n_nan;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
         const uint64_t item = nan_list[i];
-        if (!redo)
-        {
-            // the cell holding segment `item` (last seg0 <= item)
-            uint32_t lo = 0, hi = cnt->n_cells;
-            while (hi - lo > 1)
-            {
-                const uint32_t mid = (lo + hi) / 2;
-                if (cells[mid].seg0 <= item)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            if (gflag[cells[lo].group] & FTD_EXACT)
-                continue;
-        }
+        if (!redo && (gflag[cells[segcell[item]].group] & FTD_EXACT))
+            continue;
         const Segment  sg   = src.get(item);
         const PctParam pp   = params[sg.param];
         float*         a    = scores + sg.off;
