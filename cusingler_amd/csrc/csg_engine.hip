@@ -2255,7 +2255,8 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
     k_ftd_select_nan<FtdSegments><<<g, FTD_TPB, 0, eng->stream>>>(
         cnt, 0, seg, eng->d_fl_nan1, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P,
         eng->d_fl_cells, eng->d_fl_segcell, eng->d_fl_gflag);
-    k_ftd_filter<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_cells, eng->d_fl_groups,
+    k_ftd_filter<<<g * (FTD_TPB / FTD_FILTER_NT), FTD_FILTER_NT, 0, eng->stream>>>(
+        cnt, eng->d_fl_cells, eng->d_fl_groups,
                                                  eng->d_fl_gflag, eng->d_fl_glabels, eng->d_ft_P,
                                                  eng->d_fl_lists, eng->d_fl_counts, L, thre,
                                                  eng->d_fl_stage, eng->d_fl_redo,

@@ -577,18 +577,10 @@ struct FtSegments
 
 // One cell of the FP32 filter: `in` holds its k candidates, the new list goes
 // to `out`; returns its length.
-__device__ inline uint32_t ft_filter_cell(const FtCell& fc, const FtGroup& g,
-                                          const uint32_t* __restrict__ gl,
-                                          const float* __restrict__ P, const uint8_t* in,
-                                          uint32_t k, float thre, uint8_t* out)
+template <class SF>
+__device__ inline uint32_t ft_filter_list(const SF& score_of, const uint8_t* in, uint32_t k,
+                                          float thre, uint8_t* out)
 {
-    auto score_of = [&](uint32_t pos) {
-        const uint32_t lab = in[pos];
-        uint32_t       j   = 0;
-        while (j < g.nlabel && gl[j] != lab)
-            ++j;
-        return j < g.nlabel ? P[fc.seg0 + j] : __builtin_nanf("");
-    };
     // minmax_element over the scores in list order
     int mn = 0, mx = 0;
     if (k >= 2)
@@ -682,6 +674,21 @@ __device__ inline uint32_t ft_filter_cell(const FtCell& fc, const FtGroup& g,
     if (n == 0)
         out[n++] = in[0];
     return n;
+}
+
+__device__ inline uint32_t ft_filter_cell(const FtCell& fc, const FtGroup& g,
+                                          const uint32_t* __restrict__ gl,
+                                          const float* __restrict__ P, const uint8_t* in,
+                                          uint32_t k, float thre, uint8_t* out)
+{
+    auto score_of = [&](uint32_t pos) {
+        const uint32_t lab = in[pos];
+        uint32_t       j   = 0;
+        while (j < g.nlabel && gl[j] != lab)
+            ++j;
+        return j < g.nlabel ? P[fc.seg0 + j] : __builtin_nanf("");
+    };
+    return ft_filter_list(score_of, in, k, thre, out);
 }
 
 __global__ void k_ft_filter(const FtCell* __restrict__ cells, uint32_t ncells,
@@ -852,21 +859,13 @@ __device__ inline float ft_next_up(float x)
 
 // One cell of the certified filter: returns 0 and writes the new list
 // (length in *cnt_out) when certified, else 1 (list untouched).
-__device__ inline int ft_filter_exact_cell(const FtCell& fc, const FtGroup& g,
-                                           const uint32_t* __restrict__ gl,
-                                           const float* __restrict__ P, const uint8_t* in,
+template <class SF>
+__device__ inline int ft_filter_exact_list(const SF& score_of, uint32_t gs, const uint8_t* in,
                                            uint32_t k, float thre, uint8_t* out,
                                            uint32_t* cnt_out)
 {
-    auto score_of = [&](uint32_t pos) {
-        const uint32_t lab = in[pos];
-        uint32_t       j   = 0;
-        while (j < g.nlabel && gl[j] != lab)
-            ++j;
-        return j < g.nlabel ? P[fc.seg0 + j] : __builtin_nanf("");
-    };
     const double u     = 5.9604644775390625e-08;
-    const int    step  = ((int)g.gs + 127) / 128;
+    const int    step  = ((int)gs + 127) / 128;
     const double n     = (double)(step + 6);
     const double gamma = n * u / (1.0 - n * u);
     const double wp    = (gamma / (1.0 - gamma - 2 * u) + 3 * u) * (1.0 + 1e-6) + 16 * u;
@@ -928,6 +927,22 @@ __device__ inline int ft_filter_exact_cell(const FtCell& fc, const FtGroup& g,
     }
     *cnt_out = cnt;
     return 0;
+}
+
+__device__ inline int ft_filter_exact_cell(const FtCell& fc, const FtGroup& g,
+                                           const uint32_t* __restrict__ gl,
+                                           const float* __restrict__ P, const uint8_t* in,
+                                           uint32_t k, float thre, uint8_t* out,
+                                           uint32_t* cnt_out)
+{
+    auto score_of = [&](uint32_t pos) {
+        const uint32_t lab = in[pos];
+        uint32_t       j   = 0;
+        while (j < g.nlabel && gl[j] != lab)
+            ++j;
+        return j < g.nlabel ? P[fc.seg0 + j] : __builtin_nanf("");
+    };
+    return ft_filter_exact_list(score_of, g.gs, in, k, thre, out, cnt_out);
 }
 
 __global__ void k_ft_filter_exact(const FtCell* __restrict__ cells, uint32_t ncells,

@@ -228,7 +228,7 @@ __global__ void k_ftd_member(FtdCounts* __restrict__ cnt, const uint32_t* __rest
 
 // One block per group: ascending labels, their row offsets, G_sub.
 // `exact_ok`: the exact form is wanted (groups of at most 1919 genes use it).
-constexpr int FTD_SETUP_NT = 256;
+constexpr int FTD_SETUP_NT = 1024;
 
 __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
     const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ gowner,
@@ -836,47 +836,87 @@ __global__ void k_ftd_select_nan(FtdCounts* __restrict__ cnt, int redo, Src src,
     }
 }
 
+// A cell's candidate list and its quantiles in list order, staged in LDS
+// (lists of up to FTD_FAST labels).  The list holds exactly the group's label
+// set, so label l's position in the group's ascending label list is the
+// number of listed labels below l (a popcount of the list's bitset).
+constexpr int FTD_FAST = 64, FTD_FILTER_NT = 128;
+
+__device__ inline void ftd_stage_cell(const uint8_t* __restrict__ gin, uint32_t k,
+                                      const float* __restrict__ P, uint64_t seg0, uint8_t* sin,
+                                      float* ssc)
+{
+    uint64_t m[4] = { 0ull, 0ull, 0ull, 0ull };
+    for (uint32_t t = 0; t < k; ++t)
+    {
+        const uint32_t l = gin[t];
+        sin[t]           = (uint8_t)l;
+        m[l >> 6] |= 1ull << (l & 63);
+    }
+    for (uint32_t t = 0; t < k; ++t)
+    {
+        const uint32_t l = sin[t], w = l >> 6;
+        uint32_t       j = __builtin_popcountll(m[w] & ((1ull << (l & 63)) - 1ull));
+        for (uint32_t v = 0; v < w; ++v)
+            j += __builtin_popcountll(m[v]);
+        ssc[t] = P[seg0 + j];
+    }
+}
+
 // Filter of every cell with work, candidate lists updated in place (the new
 // list is formed in `stage` first).  Exact-form cells use the certified
 // filter; an undecided one is listed for the FP32 redo, with its tiles and
 // segments.
-__global__ void k_ftd_filter(FtdCounts* __restrict__ cnt, const FtCell* __restrict__ cells,
-                             const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
-                             const uint32_t* __restrict__ labels, const float* __restrict__ P,
-                             uint8_t* __restrict__ lists, uint32_t* __restrict__ counts,
-                             uint32_t L, float thre, uint8_t* __restrict__ stage,
-                             uint32_t* __restrict__ redo, FtCorrTile* __restrict__ rthin,
-                             uint64_t* __restrict__ rseg)
+__global__ __launch_bounds__(FTD_FILTER_NT) void k_ftd_filter(
+    FtdCounts* __restrict__ cnt, const FtCell* __restrict__ cells,
+    const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
+    const uint32_t* __restrict__ labels, const float* __restrict__ P, uint8_t* __restrict__ lists,
+    uint32_t* __restrict__ counts, uint32_t L, float thre, uint8_t* __restrict__ stage,
+    uint32_t* __restrict__ redo, FtCorrTile* __restrict__ rthin, uint64_t* __restrict__ rseg)
 {
-    const uint32_t n = cnt->n_cells;
+    __shared__ uint8_t sin_[FTD_FILTER_NT][FTD_FAST];
+    __shared__ float   ssc_[FTD_FILTER_NT][FTD_FAST];
+    const uint32_t     n = cnt->n_cells;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        const FtCell  fc  = cells[i];
-        const FtGroup G   = groups[fc.group];
-        uint8_t*      in  = lists + (size_t)fc.cell * L;
-        uint8_t*      out = stage + (size_t)i * L;
-        const uint32_t k  = counts[fc.cell];
-        uint32_t      m   = 0;
-        if (gflag[fc.group] & FTD_EXACT)
+        const FtCell   fc  = cells[i];
+        const FtGroup  G   = groups[fc.group];
+        uint8_t*       gin = lists + (size_t)fc.cell * L;
+        uint8_t*       out = stage + (size_t)i * L;
+        const uint32_t k   = counts[fc.cell];
+        const bool     ex  = gflag[fc.group] & FTD_EXACT;
+        uint32_t       m   = 0;
+        int            st  = 0;
+        if (k <= (uint32_t)FTD_FAST)
         {
-            if (ft_filter_exact_cell(fc, G, labels + G.label_off, P, in, k, thre, out, &m))
-            {
-                uint32_t t = atomicAdd(&cnt->n_rthin, (G.nref + 255) / 256);
-                for (uint32_t r0 = 0; r0 < G.nref; r0 += 256)
-                    rthin[t++] = FtCorrTile{ fc.group, fc.pos, r0, 1u };
-                const uint64_t s0 = atomicAdd((unsigned long long*)&cnt->n_redo_segs,
-                                              (unsigned long long)G.nlabel);
-                for (uint32_t j = 0; j < G.nlabel; ++j)
-                    rseg[s0 + j] = fc.seg0 + j;
-                redo[atomicAdd(&cnt->n_redo, 1u)] = i;
-                atomicAdd(&cnt->tot_redo, 1u);
-                continue;
-            }
+            uint8_t* sin = sin_[threadIdx.x];
+            float*   ssc = ssc_[threadIdx.x];
+            ftd_stage_cell(gin, k, P, fc.seg0, sin, ssc);
+            auto score_of = [&](uint32_t t) { return ssc[t]; };
+            if (ex)
+                st = ft_filter_exact_list(score_of, G.gs, sin, k, thre, out, &m);
+            else
+                m = ft_filter_list(score_of, sin, k, thre, out);
         }
+        else if (ex)
+            st = ft_filter_exact_cell(fc, G, labels + G.label_off, P, gin, k, thre, out, &m);
         else
-            m = ft_filter_cell(fc, G, labels + G.label_off, P, in, k, thre, out);
+            m = ft_filter_cell(fc, G, labels + G.label_off, P, gin, k, thre, out);
+        if (st)
+        {
+            uint32_t t = atomicAdd(&cnt->n_rthin, (G.nref + 255) / 256);
+            for (uint32_t r0 = 0; r0 < G.nref; r0 += 256)
+                rthin[t++] = FtCorrTile{ fc.group, fc.pos, r0, 1u };
+            const uint64_t s0 = atomicAdd((unsigned long long*)&cnt->n_redo_segs,
+                                          (unsigned long long)G.nlabel);
+            for (uint32_t j = 0; j < G.nlabel; ++j)
+                rseg[s0 + j] = fc.seg0 + j;
+            redo[atomicAdd(&cnt->n_redo, 1u)] = i;
+            atomicAdd(&cnt->tot_redo, 1u);
+            continue;
+        }
         for (uint32_t t = 0; t < m; ++t)
-            in[t] = out[t];
+            gin[t] = out[t];
         counts[fc.cell] = m;
     }
 }
