@@ -727,7 +727,7 @@ int score_chunk(csg_engine* eng, int path, uint32_t nq)
     {
         dim3 grid((eng->Nr + W8_BN - 1) / W8_BN, (nq + W8_BM - 1) / W8_BM);
         k_gemm_int8w<<<grid, 512, 0, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                                                    (int)eng->Kp, eng->d_sqq_f, eng->d_sqr_f,
+                                                    (int)eng->Kp, eng->d_sqq_i, eng->d_sqr_i,
                                                     (int)nq, (int)eng->Nr, eng->d_S,
                                                     (size_t)eng->Nr);
     }
@@ -735,7 +735,7 @@ int score_chunk(csg_engine* eng, int path, uint32_t nq)
     {
         dim3 grid((eng->Nr + I8_BN - 1) / I8_BN, (nq + I8_BM - 1) / I8_BM);
         k_gemm_int8<<<grid, 256, 0, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                                                   (int)eng->Kp, eng->d_sqq_f, eng->d_sqr_f,
+                                                   (int)eng->Kp, eng->d_sqq_i, eng->d_sqr_i,
                                                    (int)nq, (int)eng->Nr, eng->d_S,
                                                    (size_t)eng->Nr);
     }
@@ -774,14 +774,18 @@ int prepare_operands(csg_engine* eng, int path, const int16_t* X, size_t rows, i
 }
 
 // Half-width of the reference's per-label quantile around the exact path's
-// (see csg_exact.h): gamma_{G-1} / (1 - gamma - 2u) + 3u per score, plus 16u
-// for the two float evaluations of the quantile formula.
+// (see csg_exact.h): per score, gamma_{G-1} / (1 - e_D) for the reference's
+// FP32 sum of products, e_D / (1 - e_D) for its FP32 divisor (relative error
+// e_D = gamma_{G-1} + 2u (1 + gamma_{G-1})) and 2u for the two roundings;
+// plus 16u for the two
+// float evaluations of the quantile formula.
 double certify_width(uint32_t G)
 {
     const double u     = 5.9604644775390625e-08;
     const double n     = G > 1 ? (double)(G - 1) : 0.0;
     const double gamma = n * u / (1.0 - n * u);
-    const double wpair = gamma / (1.0 - gamma - 2 * u) + 3 * u;
+    const double ed    = gamma + 2 * u * (1.0 + gamma);
+    const double wpair = (gamma + ed) / (1.0 - ed) + 2 * u;
     return wpair * (1.0 + 1e-6) + 16 * u;
 }
 
@@ -1130,7 +1134,7 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
         k_gemm_int8w_group<<<(unsigned)tiles.size(), 512, 0, eng->stream>>>(
             eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart, eng->d_llen,
             eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl, (int)eng->Kp, b_max,
-            eng->d_sqq_f, eng->d_sqr_f, eng->d_S);
+            eng->d_sqq_i, eng->d_sqr_i, eng->d_S);
         LAUNCHCHK();
     }
     {
@@ -2821,6 +2825,64 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
     return CSG_OK;
 }
 
+int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float* out)
+{
+    if (!eng || !out)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->score_loaded)
+        return eng->fail(CSG_ERR_STATE, "csg_score_matrix before csg_copyin_score");
+    if (path < CSG_PATH_PLAIN || path > CSG_PATH_FP16_MFMA)
+        return eng->fail(CSG_ERR_ARGUMENT, "unknown correlation path");
+    if (nq == 0 || nq > eng->Qc || q0 > eng->Nq || nq > eng->Nq - q0)
+        return eng->fail(CSG_ERR_ARGUMENT, "query rows outside the loaded cells or above one chunk");
+    HIPCHK(hipSetDevice(eng->device));
+    const size_t nr_pad = round_up(eng->Nr, 256);
+    const size_t qc     = eng->Qc;
+    int          rc;
+    if (path == CSG_PATH_EXACT_INT8 && !eng->d_rh)
+    {
+        if ((rc = dalloc(eng, &eng->d_rh, nr_pad * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_rl, nr_pad * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_qh, qc * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_ql, qc * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_cstatus, qc)) ||
+            (rc = dalloc(eng, &eng->d_exact, qc * eng->L)) ||
+            (rc = dalloc(eng, &eng->d_items, qc * eng->L)) ||
+            (rc = dalloc(eng, &eng->d_nitems, 1)) || (rc = dalloc(eng, &eng->d_Pref, qc * eng->L)))
+            return rc;
+    }
+    if (path == CSG_PATH_FP16_MFMA && !eng->d_rf16)
+    {
+        if ((rc = dalloc(eng, &eng->d_rf16, nr_pad * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_qf16, qc * eng->Kp)))
+            return rc;
+    }
+    if (!eng->ref_ranked)
+    {
+        if ((rc = rank_rows(eng, eng->d_rdata, eng->d_rind, eng->d_rptr, 0, eng->Nr, eng->d_Xr,
+                            eng->d_sqr_f, eng->d_sqr_i)))
+            return rc;
+        eng->ref_ranked        = true;
+        eng->ref_prepared_path = -1;
+    }
+    if ((path == CSG_PATH_EXACT_INT8 || path == CSG_PATH_FP16_MFMA) && eng->ref_prepared_path != path)
+    {
+        if ((rc = prepare_operands(eng, path, eng->d_Xr, nr_pad, eng->d_rh, eng->d_rl, eng->d_rf16)))
+            return rc;
+        eng->ref_prepared_path = path;
+    }
+    if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
+                        eng->d_sqq_f, eng->d_sqq_i)) ||
+        (rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh, eng->d_ql,
+                               eng->d_qf16)) ||
+        (rc = score_chunk(eng, path, nq)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(out, eng->d_S, (size_t)nq * eng->Nr * sizeof(float),
+                          hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    return CSG_OK;
+}
+
 int csg_set_finetune_schedule(csg_engine* eng, int schedule)
 {
     if (!eng || schedule < 0 || schedule > 1)
@@ -2906,18 +2968,21 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
         v = (int16_t)rnd();
     for (auto& v : xr)
         v = (int16_t)rnd();
-    std::vector<float> ones(std::max(nq, nr), 1.f);
+    std::vector<float>     ones(std::max(nq, nr), 1.f);
+    std::vector<long long> ones64(std::max(nq, nr), 1ll);
     int16_t *dq = nullptr, *dr = nullptr;
     int8_t *qh = nullptr, *ql = nullptr, *rh = nullptr, *rl = nullptr;
     _Float16 *qf = nullptr, *rf = nullptr;
     float *d1 = nullptr, *dS = nullptr;
+    long long* d1i = nullptr;
     int    rc = CSG_OK;
     auto   cleanup = [&]() {
         dfree(dq); dfree(dr); dfree(qh); dfree(ql); dfree(rh); dfree(rl); dfree(qf); dfree(rf);
-        dfree(d1); dfree(dS);
+        dfree(d1); dfree(dS); dfree(d1i);
     };
     if ((rc = upload(eng, &dq, xq.data(), xq.size())) || (rc = upload(eng, &dr, xr.data(), xr.size())) ||
-        (rc = upload(eng, &d1, ones.data(), ones.size())) || (rc = dalloc(eng, &dS, (size_t)nq * nr)) ||
+        (rc = upload(eng, &d1, ones.data(), ones.size())) ||
+        (rc = upload(eng, &d1i, ones64.data(), ones64.size())) || (rc = dalloc(eng, &dS, (size_t)nq * nr)) ||
         (rc = dalloc(eng, &qh, xq.size())) || (rc = dalloc(eng, &ql, xq.size())) ||
         (rc = dalloc(eng, &rh, xr.size())) || (rc = dalloc(eng, &rl, xr.size())) ||
         (rc = dalloc(eng, &qf, xq.size())) || (rc = dalloc(eng, &rf, xr.size())))
@@ -2930,8 +2995,8 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
     {
         k_split_limbs<<<bq, 256, 0, eng->stream>>>(dq, xq.size(), qh, ql);
         k_split_limbs<<<br, 256, 0, eng->stream>>>(dr, xr.size(), rh, rl);
-        k_gemm_int8<<<dim3(nr / I8_BN, nq / I8_BM), 256, 0, eng->stream>>>(qh, ql, rh, rl, Kp, d1, d1,
-                                                                          nq, nr, dS, nr);
+        k_gemm_int8<<<dim3(nr / I8_BN, nq / I8_BM), 256, 0, eng->stream>>>(qh, ql, rh, rl, Kp, d1i,
+                                                                          d1i, nq, nr, dS, nr);
     }
     else if (which == 1)
     {
@@ -2944,8 +3009,8 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
     {
         k_split_limbs<<<bq, 256, 0, eng->stream>>>(dq, xq.size(), qh, ql);
         k_split_limbs<<<br, 256, 0, eng->stream>>>(dr, xr.size(), rh, rl);
-        k_gemm_int8w<<<dim3(nr / W8_BN, nq / W8_BM), 512, 0, eng->stream>>>(qh, ql, rh, rl, Kp, d1,
-                                                                            d1, nq, nr, dS, nr);
+        k_gemm_int8w<<<dim3(nr / W8_BN, nq / W8_BM), 512, 0, eng->stream>>>(qh, ql, rh, rl, Kp, d1i,
+                                                                            d1i, nq, nr, dS, nr);
     }
     else if (which == 2)
         k_gemm_f32mfma<<<dim3(nr / F_BN, nq / F_BM), 256, 0, eng->stream>>>(dq, dr, Kp, d1, d1, nq,

@@ -1,14 +1,17 @@
 // Certified labels for the exact score path.
 //
-// The exact path knows every rank dot product exactly (S4) and divides it by
-// the reference's own FP32 divisor D, so its score differs from the
-// reference's FP32 score only through the rounding of the reference's
-// sequential FP32 sum of products.  That sum is bounded by the classical
-// recursive-summation bound  |sxy_f - S4| <= gamma_{G-1} * sum |p_k|
-// <= gamma_{G-1} * sqrt(Sxx * Syy), gamma_n = n*u / (1 - n*u), u = 2^-24,
-// giving a per-score half-width W (host-computed).  Order statistics move by
-// at most W and the quantile formula is a convex combination of two of them,
-// so every per-label quantile of the reference lies within W_P of the exact
+// The exact path knows every rank dot product exactly (S4) and the exact sums
+// of squares Sxx, Syy of both rows, and returns S4 / sqrt(Sxx Syy) in double,
+// rounded once to float: the correlation to within half a float ulp.  The
+// reference computes the same quantity in FP32: its sequential sum of
+// products differs from S4 by at most gamma_{G-1} * sum |p_k| <= gamma_{G-1} *
+// sqrt(Sxx Syy) (gamma_n = n u / (1 - n u), u = 2^-24), and its divisor
+// sqrtf(sxx_f * syy_f) differs from sqrt(Sxx Syy) by a relative e_D <=
+// gamma_{G-1} + 2u (1 + gamma_{G-1}) (two sequential sums of exact squares,
+// each within a factor 1 +- gamma, one product, one root), so each of its scores lies within W = (gamma + e_D) / (1 - e_D) + 2u
+// of the exact path's (host-computed).  Order statistics move by at most W
+// and the quantile formula is a convex combination of two of them, so every
+// per-label quantile of the reference lies within W_P = W + 16u of the exact
 // path's.  A cell's first label and mask are decided from those intervals
 // when they are unambiguous; the (cell, label) pairs that are not are scored
 // again in the reference's exact FP32 order and the decision is redone with

@@ -17,6 +17,15 @@ typedef int      v4i __attribute__((ext_vector_type(4)));
 typedef float    v4f __attribute__((ext_vector_type(4)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
+// Correlation of the exact path: the exact rank dot product over the exact
+// root of the two rows' exact sums of squares, in double, rounded once to
+// float (NaN when a row has no variance, as the reference's 0 divisor).
+__device__ inline float exact_corr(long long s4, double sq_row, long long sq_col)
+{
+    const double d = sqrt(sq_row * (double)sq_col);
+    return d != 0.0 ? (float)((double)s4 / d) : __builtin_nanf("");
+}
+
 // ------------------------------------------------------------ limb split ---
 
 // X = 256*h + l with l = signed low byte, h = (X - l) / 256; both in int8 for
@@ -71,8 +80,8 @@ constexpr int I8_BM = 64, I8_BN = 128, I8_BK = 64, I8_LD = 80;
 
 __global__ __launch_bounds__(256) void k_gemm_int8(
     const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
-    const int8_t* __restrict__ rl, int Kp, const float* __restrict__ sq_q,
-    const float* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+    const int8_t* __restrict__ rl, int Kp, const long long* __restrict__ sq_q,
+    const long long* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
 {
     __shared__ __attribute__((aligned(16))) int8_t sm[2][(2 * I8_BM + 2 * I8_BN) * I8_LD];
     const int tid  = threadIdx.x;
@@ -191,8 +200,8 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
             const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + e;
             if (m >= nq)
                 continue;
-            const float sq  = sq_q[m];
-            float*      out = S + (size_t)m * ldS;
+            const double sq  = (double)sq_q[m];
+            float*       out = S + (size_t)m * ldS;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
             {
@@ -201,9 +210,7 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
                     continue;
                 const long long s4 = ((long long)acc_hh[i][j][e] << 16) +
                                      ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
-                const float prod = sq * sq_r[n];
-                const float d    = sqrtf(prod);
-                out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
+                out[n] = exact_corr(s4, sq, sq_r[n]);
             }
         }
 }
@@ -233,17 +240,17 @@ __device__ inline v4i times16(v4i h)
 struct DenseW8Map
 {
     int          m0, n0, nq, nr;
-    const float* sq_q;
-    const float* sq_r;
-    float*       S;
-    size_t       ldS;
-    __device__ size_t a_row(int r) const { return (size_t)(m0 + r); }
-    __device__ size_t b_row(int r) const { return (size_t)(n0 + r); }
-    __device__ bool   row_ok(int m) const { return m0 + m < nq; }
-    __device__ float  row_sq(int m) const { return sq_q[m0 + m]; }
-    __device__ float* row_out(int m) const { return S + (size_t)(m0 + m) * ldS + n0; }
-    __device__ bool   col_ok(int n) const { return n0 + n < nr; }
-    __device__ float  col_sq(int n) const { return sq_r[n0 + n]; }
+    const long long* sq_q;
+    const long long* sq_r;
+    float*           S;
+    size_t           ldS;
+    __device__ size_t    a_row(int r) const { return (size_t)(m0 + r); }
+    __device__ size_t    b_row(int r) const { return (size_t)(n0 + r); }
+    __device__ bool      row_ok(int m) const { return m0 + m < nq; }
+    __device__ long long row_sq(int m) const { return sq_q[m0 + m]; }
+    __device__ float*    row_out(int m) const { return S + (size_t)(m0 + m) * ldS + n0; }
+    __device__ bool      col_ok(int n) const { return n0 + n < nr; }
+    __device__ long long col_sq(int n) const { return sq_r[n0 + n]; }
 };
 
 // One label of a grouped contraction: the query rows listed in cells[0, mcount)
@@ -255,17 +262,17 @@ struct GroupW8Map
     const uint32_t* cells;
     int             mcount, ncount;
     uint32_t        b0, b_max;
-    const float*    sq_q;
-    const float*    sq_r;
-    float*          out;
-    size_t          ld;
-    __device__ size_t a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
-    __device__ size_t b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
-    __device__ bool   row_ok(int m) const { return m < mcount; }
-    __device__ float  row_sq(int m) const { return sq_q[cells[m]]; }
-    __device__ float* row_out(int m) const { return out + (size_t)m * ld; }
-    __device__ bool   col_ok(int n) const { return n < ncount; }
-    __device__ float  col_sq(int n) const { return sq_r[b0 + n]; }
+    const long long* sq_q;
+    const long long* sq_r;
+    float*           out;
+    size_t           ld;
+    __device__ size_t    a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
+    __device__ size_t    b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
+    __device__ bool      row_ok(int m) const { return m < mcount; }
+    __device__ long long row_sq(int m) const { return sq_q[cells[m]]; }
+    __device__ float*    row_out(int m) const { return out + (size_t)m * ld; }
+    __device__ bool      col_ok(int n) const { return n < ncount; }
+    __device__ long long col_sq(int n) const { return sq_r[b0 + n]; }
 };
 
 template <class Map>
@@ -368,8 +375,8 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             const int m = wm * 64 + i * 16 + (lane >> 4) * 4 + e;
             if (!map.row_ok(m))
                 continue;
-            const float sq  = map.row_sq(m);
-            float*      out = map.row_out(m);
+            const double sq  = (double)map.row_sq(m);
+            float*       out = map.row_out(m);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
             {
@@ -377,17 +384,15 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
                 if (!map.col_ok(n))
                     continue;
                 const long long s4 = ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
-                const float prod = sq * map.col_sq(n);
-                const float d    = sqrtf(prod);
-                out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
+                out[n] = exact_corr(s4, sq, map.col_sq(n));
             }
         }
 }
 
 __global__ __launch_bounds__(512) void k_gemm_int8w(
     const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
-    const int8_t* __restrict__ rl, int Kp, const float* __restrict__ sq_q,
-    const float* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+    const int8_t* __restrict__ rl, int Kp, const long long* __restrict__ sq_q,
+    const long long* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
 {
     const DenseW8Map map{ (int)blockIdx.y * W8_BM, (int)blockIdx.x * W8_BN, nq, nr, sq_q, sq_r, S,
                           ldS };
@@ -410,7 +415,8 @@ __global__ __launch_bounds__(512) void k_gemm_int8w_group(
     const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
     const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
     const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
-    int Kp, uint32_t b_max, const float* __restrict__ sq_q, const float* __restrict__ sq_r,
+    int Kp, uint32_t b_max, const long long* __restrict__ sq_q,
+    const long long* __restrict__ sq_r,
     float* __restrict__ S)
 {
     const GroupTile t  = tiles[blockIdx.x];

@@ -74,6 +74,7 @@ def lib():
     L.csg_destroy.argtypes = [_V]
     L.csg_set_finetune_path.argtypes = [_V, C.c_int]
     L.csg_set_finetune_schedule.argtypes = [_V, C.c_int]
+    L.csg_score_matrix.argtypes = [_V, C.c_int, _U32, _U32, _V]
     L.csg_finetune_tuning.argtypes = [_V, C.c_int, _V, _V]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
@@ -221,7 +222,7 @@ class Engine:
             self._h, c.ref_height, c.width, _p(c.ref_data), _p(c.ref_indices), _p(c.ref_indptr),
             c.qry_height, _p(c.qry_data), _p(c.qry_indices), _p(c.qry_indptr), c.ct_num,
             _p(c.ctidx), c.max_uniq_gene), "copyin_score")
-        self.Nq, self.L = c.qry_height, c.ct_num
+        self.Nq, self.L, self.Nr = c.qry_height, c.ct_num, c.ref_height
 
     def get_label(self, quantile: float = 0.8, thre: float = 0.05, path: str = "plain",
                   outputs: bool = True):
@@ -248,6 +249,13 @@ class Engine:
 
     def set_finetune_path(self, path: str):
         self._check(lib().csg_set_finetune_path(self._h, PATHS[path]), "set_finetune_path")
+
+    def score_matrix(self, path: str, q0: int, nq: int) -> np.ndarray:
+        """Per-pair correlations of query rows [q0, q0 + nq) against every
+        reference cell by `path`'s contraction (diagnostic)."""
+        out = np.zeros((nq, self.Nr), np.float32)
+        self._check(lib().csg_score_matrix(self._h, PATHS[path], q0, nq, _p(out)), "score_matrix")
+        return out
 
     def set_finetune_schedule(self, schedule: str):
         """"device" (rounds on the device, HIP graphs) or "host" (host grouping)."""
@@ -354,7 +362,8 @@ def certify_width(G: int) -> float:
     u = 2.0 ** -24
     n = max(G - 1, 0)
     gamma = n * u / (1 - n * u)
-    wpair = gamma / (1 - gamma - 2 * u) + 3 * u
+    ed = gamma + 2 * u * (1 + gamma)
+    wpair = (gamma + ed) / (1 - ed) + 2 * u
     return wpair * (1 + 1e-6) + 16 * u
 
 

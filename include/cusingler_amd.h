@@ -212,6 +212,14 @@ int csg_mlp_predict(csg_engine* eng, int k, int32_t* top, float* margin);
  * did not. */
 int csg_set_proposal(csg_engine* eng, int k, float min_margin);
 
+/* Diagnostic: the per-pair correlation matrix [nq x ref_height] of query rows
+ * [q0, q0 + nq) of the loaded score data (nq at most one query chunk) by the
+ * given path's contraction (the exact path's values before certification),
+ * for stating each path's error against FP64.  No counterpart in the
+ * reference (the reference keeps this matrix internal, src/cusingler.cu:
+ * 1071-1090). */
+int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float* out);
+
 /* Diagnostic of the contraction kernels on seeded random ranks (|X| <= 1500,
  * 256 x 256 rows, K = 192): which = 0 INT8 limbs, 1 FP16, 2 FP32 MFMA,
  * 3 plain FP32, 4 INT8 limbs wide-tile kernel (gene sets up to 1920).  *max_err receives 0 when the device results equal the host's

@@ -170,3 +170,35 @@ def test_device_finetune_graph_replays_and_autotuner(eng):
         eng.set_finetune_path("plain")
     eng.destroy()
     eng.destroy_score()
+
+
+@pytest.mark.parametrize("name", ["pancreas_2k", "atlas_1k"])
+def test_correlation_error_vs_fp64(eng, name):
+    """Per-pair correlations of every path against the FP64 oracle (exact
+    integer rank sums, FP64 normalisation): the plain path equals the
+    reference's FP32 order bit for bit, the exact path stays within 1e-6, the
+    approximate paths' errors are stated."""
+    from cusingler_amd import engine
+    from oracle import oracle as O
+    ds = _data(name)
+    prep = engine.Prep(ds, 8)
+    c0 = prep.generate(0)
+    eng.copyin_score(c0)
+    nq = min(256, ds.qry_height)
+    orc = O.Oracle(ds, 8)
+    seq, f64 = orc.score_matrix(0, nq, O.VAR_SEQ, want64=True)
+    errs = {}
+    for path in ["plain", "exact_int8", "fp32_mfma", "fp16"]:
+        s = eng.score_matrix(path, 0, nq).astype(np.float64)
+        ok = ~np.isnan(f64)
+        assert np.array_equal(np.isnan(s), ~ok), path
+        errs[path] = float(np.max(np.abs(s[ok] - f64[ok])))
+        if path == "plain":
+            assert_bits_equal(s.astype(np.float32), seq, f"{name} plain scores")
+    eng.destroy_score()
+    _say(f"{name} (G_s={c0.width}, {nq} x {ds.ref_height} pairs) max |corr - FP64|: " +
+         ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert errs["exact_int8"] <= 1e-6
+    assert errs["plain"] <= 1e-5
+    assert errs["fp32_mfma"] <= 1e-5
+    assert errs["fp16"] <= 1e-4
