@@ -171,7 +171,9 @@ struct csg_engine
     // launch settings of the work-queue kernels (blocks), tunable
     int   ftd_grid_threads = 1024, ftd_grid_setup = 512, ftd_grid_rank = 2048,
         ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_thin_max = 8;
-    bool  ftd_autotune = true, ftd_tuned = false;
+    bool     ftd_autotune = true, ftd_tuned = false;
+    uint64_t ftd_budget   = (uint64_t)8 << 30;  // device bytes of one work batch
+    int64_t  ftd_batches  = 0;                   // work batches of oversized rounds
     float ftd_tune_ms[3] = { 0.f, 0.f, 0.f };  // best times of the tuned kernels
 
     // ---- learned proposal ----
@@ -1308,6 +1310,9 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     if (prop && ((uint32_t)eng->prop_k > eng->L || eng->L > 256))
         return eng->fail(CSG_ERR_ARGUMENT, "proposal size exceeds the label count, or more than "
                                            "256 labels");
+    if (prop && eng->G > 1920)
+        return eng->fail(CSG_ERR_ARGUMENT, "the learned proposal's grouped contraction needs a "
+                                           "score gene set of at most 1920 genes");
     if ((rc = set_pct(eng, &eng->d_pct, &eng->pct_quantile, eng->ctidx, eng->L, quantile)))
         return rc;
     if (prop && !eng->d_pcells && (rc = alloc_proposal(eng)))
@@ -2281,10 +2286,25 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
                                                       eng->d_ft_P, eng->d_fl_lists,
                                                       eng->d_fl_counts, L, thre,
                                                       eng->d_fl_stage);
-    k_ftd_compact<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_counts,
-                                                  eng->d_fl_rounds, times, eng->d_fl_next);
     LAUNCHCHK();
     return CSG_OK;
+}
+
+// Round end: the round is counted and the next round's active list formed.
+int ftd_launch_compact(csg_engine* eng, int times)
+{
+    k_ftd_compact<<<(unsigned)eng->ftd_grid_threads, FTD_TPB, 0, eng->stream>>>(
+        eng->d_fl_cnt, eng->d_fl_act, eng->d_fl_counts, eng->d_fl_rounds, times, eng->d_fl_next);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// Device bytes of one round's work tables for the given totals.
+uint64_t ftd_bytes(const FtdCounts& h, uint32_t L)
+{
+    return 2 * h.ref_elems + 2 * h.cell_elems + 4 * h.n_scores + 4 * h.n_ref_items +
+           (uint64_t)h.n_cells * (sizeof(FtRankItem) + sizeof(FtCell) + L + 12) +
+           (uint64_t)h.n_segs * 36 + ((uint64_t)h.n_tiles + h.n_thin + h.rthin_bound) * 16;
 }
 
 // Autotuner of the work-queue launch settings.  On the first device round
@@ -2433,6 +2453,85 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->ftd_thin_max, (uintptr_t)eng->ftd_grid_threads };
 }
 
+// A round whose work tables exceed the budget: its groups are processed in
+// ranges whose tables fit (each range re-scanned on the device), every range
+// through the same round graph.
+int ftd_batches(csg_engine* eng, const FtdCounts& h, float quantile, float thre, int times)
+{
+    const uint32_t        ng = h.n_groups, L = eng->fL;
+    std::vector<FtGroup>  groups(ng);
+    std::vector<uint8_t>  flags(ng);
+    HIPCHK(hipMemcpy(groups.data(), eng->d_fl_groups, ng * sizeof(FtGroup), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(flags.data(), eng->d_fl_gflag, ng, hipMemcpyDeviceToHost));
+    // greedy ranges under the budget; the totals of the largest range size the tables
+    std::vector<std::pair<uint32_t, uint32_t>> ranges;
+    FtdCounts                                  cap{}, cur{};
+    uint32_t                                   g0 = 0;
+    auto add = [](FtdCounts& t, const FtdSizes& z) {
+        t.n_ref_items += z.ref_items;
+        t.n_cells += (uint32_t)z.cell_items;
+        t.ref_elems += z.ref_elems;
+        t.cell_elems += z.cell_elems;
+        t.n_scores += z.scores;
+        t.n_tiles += (uint32_t)z.tiles;
+        t.n_thin += (uint32_t)z.thin;
+        t.n_segs += z.segs;
+        t.rthin_bound += z.rthin;
+    };
+    auto keep_max = [&](const FtdCounts& t) {
+        cap.n_ref_items = std::max(cap.n_ref_items, t.n_ref_items);
+        cap.n_cells     = std::max(cap.n_cells, t.n_cells);
+        cap.ref_elems   = std::max(cap.ref_elems, t.ref_elems);
+        cap.cell_elems  = std::max(cap.cell_elems, t.cell_elems);
+        cap.n_scores    = std::max(cap.n_scores, t.n_scores);
+        cap.n_tiles     = std::max(cap.n_tiles, t.n_tiles);
+        cap.n_thin      = std::max(cap.n_thin, t.n_thin);
+        cap.n_segs      = std::max(cap.n_segs, t.n_segs);
+        cap.rthin_bound = std::max(cap.rthin_bound, t.rthin_bound);
+    };
+    for (uint32_t g = 0; g < ng; ++g)
+    {
+        const FtdSizes z = ftd_sizes(groups[g], flags[g], (uint32_t)eng->ftd_thin_max);
+        FtdCounts      t = cur;
+        add(t, z);
+        if (g > g0 && ftd_bytes(t, L) > eng->ftd_budget)
+        {
+            ranges.emplace_back(g0, g);
+            keep_max(cur);
+            cur = FtdCounts{};
+            g0  = g;
+            add(cur, z);
+        }
+        else
+            cur = t;
+    }
+    ranges.emplace_back(g0, ng);
+    keep_max(cur);
+    int rc;
+    if ((rc = ftd_work(eng, cap)))
+        return rc;
+    if ((rc = ftd_graph(eng, &eng->ftd_gB, &eng->ftd_sigB, ftd_sig_b(eng, quantile, thre, times),
+                        [&]() { return ftd_launch_b(eng, quantile, thre, times); })))
+        return rc;
+    for (auto& r : ranges)
+    {
+        const uint32_t lohi[2] = { r.first, r.second };
+        HIPCHK(hipMemcpyAsync(&eng->d_fl_cnt->g_lo, lohi, sizeof(lohi), hipMemcpyHostToDevice,
+                              eng->stream));
+        k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(eng->d_fl_cnt, eng->d_fl_groups,
+                                                       eng->d_fl_gflag,
+                                                       (uint32_t)eng->ftd_thin_max,
+                                                       eng->d_fl_gtile0, eng->d_fl_gthin0,
+                                                       eng->d_fl_gseg0);
+        LAUNCHCHK();
+        HIPCHK(hipGraphLaunch(eng->ftd_gB, eng->stream));
+        // the range values above are read before the next copy replaces them
+        HIPCHK(hipStreamSynchronize(eng->stream));
+    }
+    eng->ftd_batches += (int64_t)ranges.size();
+    return CSG_OK;
+}
+
 // The whole fine-tune on the device.  Returns CSG_OK with *fallback set when
 // a hash collision was met: lists / counts / rounds then hold the state at
 // the start of that round for the host-grouped schedule to continue from.
@@ -2504,14 +2603,23 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
         eng->counters[7] += h.n_active;
         eng->counters[8] += h.n_groups;
         eng->counters[9] += (int64_t)h.n_scores;
-        if ((rc = ftd_work(eng, h)))
+        if (ftd_bytes(h, L) <= eng->ftd_budget)
+        {
+            if ((rc = ftd_work(eng, h)))
+                return rc;
+            if (!eng->ftd_tuned && eng->ftd_autotune && h.n_cells >= 4096 &&
+                (rc = ftd_tune(eng, quantile)))
+                return rc;
+            if ((rc = ftd_graph(eng, &eng->ftd_gB, &eng->ftd_sigB,
+                                ftd_sig_b(eng, quantile, thre, times),
+                                [&]() { return ftd_launch_b(eng, quantile, thre, times); })))
+                return rc;
+            HIPCHK(hipGraphLaunch(eng->ftd_gB, eng->stream));
+        }
+        else if ((rc = ftd_batches(eng, h, quantile, thre, times)))
             return rc;
-        if (!eng->ftd_tuned && eng->ftd_autotune && h.n_cells >= 4096 && (rc = ftd_tune(eng, quantile)))
+        if ((rc = ftd_launch_compact(eng, times)))
             return rc;
-        if ((rc = ftd_graph(eng, &eng->ftd_gB, &eng->ftd_sigB, ftd_sig_b(eng, quantile, thre, times),
-                            [&]() { return ftd_launch_b(eng, quantile, thre, times); })))
-            return rc;
-        HIPCHK(hipGraphLaunch(eng->ftd_gB, eng->stream));
         // part B leaves the next round's count in n_next (read with part A)
         h.n_next = 1;
         eng->counters[15] += now_us() - t0;
@@ -2888,6 +2996,17 @@ int csg_set_finetune_schedule(csg_engine* eng, int schedule)
     if (!eng || schedule < 0 || schedule > 1)
         return CSG_ERR_ARGUMENT;
     eng->ft_schedule = schedule;
+    return CSG_OK;
+}
+
+int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (bytes)
+        eng->ftd_budget = bytes;
+    if (batches)
+        *batches = eng->ftd_batches;
     return CSG_OK;
 }
 

@@ -49,6 +49,7 @@ struct FtdCounts
     uint32_t max_stride;
     uint32_t round;
     uint32_t tot_redo;  // redone cells over all rounds
+    uint32_t g_lo, g_hi;  // group range of the current work batch
     uint32_t pad_;
     uint64_t n_ref_items;
     uint64_t ref_elems;
@@ -148,6 +149,8 @@ __global__ void k_ftd_counters(FtdCounts* __restrict__ cnt)
     cnt->collision   = 0;
     cnt->max_stride  = 0;
     cnt->round      += 1;
+    cnt->g_lo        = 0u;
+    cnt->g_hi        = 0xFFFFFFFFu;
     cnt->n_ref_items = 0;
     cnt->ref_elems   = 0;
     cnt->cell_elems  = 0;
@@ -348,7 +351,7 @@ struct FtdSizes
     uint64_t ref_items, cell_items, ref_elems, cell_elems, scores, tiles, thin, segs, rthin;
 };
 
-__device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, uint32_t thin_max)
+__host__ __device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, uint32_t thin_max)
 {
     FtdSizes z{};
     if (flag & FTD_SHORT)
@@ -382,10 +385,14 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
 {
     constexpr int NQ = 9;
     __shared__ uint64_t part[NQ][FTD_SCAN_NT];
-    const uint32_t      ng  = cnt->n_groups;
+    // groups [lo, hi) of the current batch (all groups unless the host set a
+    // range); the batch's work counters start at zero
+    const uint32_t      lo  = cnt->g_lo;
+    const uint32_t      hi  = min(cnt->g_hi, cnt->n_groups);
+    const uint32_t      ng  = hi > lo ? hi - lo : 0u;
     const uint32_t      tid = threadIdx.x;
     const uint32_t      per = (ng + FTD_SCAN_NT - 1) / FTD_SCAN_NT;
-    const uint32_t      g0 = min(tid * per, ng), g1 = min(g0 + per, ng);
+    const uint32_t      g0 = lo + min(tid * per, ng), g1 = lo + min(tid * per + per, ng);
     uint64_t            acc[NQ] = {};
     uint32_t            smax    = 0;
     for (uint32_t g = g0; g < g1; ++g)
@@ -443,6 +450,14 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
     }
     if (smax)
         atomicMax(&cnt->max_stride, smax);
+    if (tid == 0)
+    {
+        cnt->n_redo      = 0;
+        cnt->n_rthin     = 0;
+        cnt->n_nan       = 0;
+        cnt->n_nan2      = 0;
+        cnt->n_redo_segs = 0;
+    }
     if (tid == FTD_SCAN_NT - 1)
     {
         cnt->n_ref_items = part[0][tid];
@@ -476,6 +491,8 @@ __global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* _
     {
         const uint32_t c = act[i];
         const uint32_t g = cgid[i];
+        if (g < cnt->g_lo || g >= cnt->g_hi)
+            continue;  // another batch
         if (gflag[g] & FTD_SHORT)
         {
             counts[c] = 1;
@@ -496,8 +513,8 @@ __global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* _
 __global__ void k_ftd_refmap(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
                              const uint8_t* __restrict__ gflag, uint32_t* __restrict__ refgroup)
 {
-    const uint32_t ng = cnt->n_groups;
-    for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x)
+    const uint32_t hi = min(cnt->g_hi, cnt->n_groups);
+    for (uint32_t g = cnt->g_lo + blockIdx.x; g < hi; g += gridDim.x)
     {
         if (gflag[g] & FTD_SHORT)
             continue;
@@ -513,8 +530,9 @@ __global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __
                             const uint32_t* __restrict__ gtile0, const uint32_t* __restrict__ gthin0,
                             FtCorrTile* __restrict__ tiles, FtCorrTile* __restrict__ thin)
 {
-    const uint32_t ng = cnt->n_groups;
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x)
+    const uint32_t hi = min(cnt->g_hi, cnt->n_groups);
+    for (uint32_t g = cnt->g_lo + blockIdx.x * blockDim.x + threadIdx.x; g < hi;
+         g += gridDim.x * blockDim.x)
     {
         const uint8_t f = gflag[g];
         if (f & FTD_SHORT)

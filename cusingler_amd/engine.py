@@ -76,6 +76,7 @@ def lib():
     L.csg_set_finetune_schedule.argtypes = [_V, C.c_int]
     L.csg_score_matrix.argtypes = [_V, C.c_int, _U32, _U32, _V]
     L.csg_finetune_tuning.argtypes = [_V, C.c_int, _V, _V]
+    L.csg_set_finetune_budget.argtypes = [_V, _U64, _V]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
     L.csg_selftest.argtypes = [_V, C.c_int, P(C.c_double)]
@@ -261,6 +262,13 @@ class Engine:
         """"device" (rounds on the device, HIP graphs) or "host" (host grouping)."""
         self._check(lib().csg_set_finetune_schedule(self._h, {"device": 0, "host": 1}[schedule]),
                     "set_finetune_schedule")
+
+    def set_finetune_budget(self, nbytes: int = 0) -> int:
+        """Device bytes of one fine-tune work batch (0 keeps it); returns the
+        number of range batches run so far."""
+        b = C.c_int64()
+        self._check(lib().csg_set_finetune_budget(self._h, nbytes, C.byref(b)), "set_finetune_budget")
+        return b.value
 
     def finetune_tuning(self, autotune: Optional[bool] = None) -> Dict[str, object]:
         """Autotuner state of the device-resident fine-tune (optionally switching

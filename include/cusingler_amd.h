@@ -127,6 +127,13 @@ int csg_set_finetune_schedule(csg_engine* eng, int schedule);
  * times (either may be NULL).  No counterpart in the reference. */
 int csg_finetune_tuning(csg_engine* eng, int autotune, int32_t* grids, float* best_ms);
 
+/* Device memory budget (bytes) of one device-resident fine-tune work batch:
+ * a round whose work tables would exceed it is processed in group ranges
+ * that fit (default 8 GiB; 0 leaves it).  *batches (optional) receives the
+ * number of such range batches run so far.  No counterpart in the
+ * reference. */
+int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches);
+
 /* destroy (src/cusingler.cuh:25): frees the fine-tune buffers. */
 int csg_destroy(csg_engine* eng);
 
@@ -209,7 +216,8 @@ int csg_mlp_predict(csg_engine* eng, int k, int32_t* top, float* margin);
  * Unscored labels get quantile -inf, so they stay out of the mask; the labels
  * then differ from the full path's only where a label the network ranked
  * below the last scored one would have entered the mask although that one
- * did not. */
+ * did not.  Needs a score gene set of at most 1920 genes (the grouped
+ * contraction folds the high limbs) and at most 256 labels. */
 int csg_set_proposal(csg_engine* eng, int k, float min_margin);
 
 /* Diagnostic: the per-pair correlation matrix [nq x ref_height] of query rows

@@ -202,3 +202,35 @@ def test_correlation_error_vs_fp64(eng, name):
     assert errs["plain"] <= 1e-5
     assert errs["fp32_mfma"] <= 1e-5
     assert errs["fp16"] <= 1e-4
+
+
+def test_device_finetune_batches_oversized_rounds(eng):
+    """A round whose work tables exceed the budget runs in group ranges: with
+    a tiny budget every round is split, and labels and round counts still
+    equal the host-grouped schedule's, in FP32 and exact form."""
+    from cusingler_amd import engine, synth
+    ds = synth.generate("small")
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    eng.copyin_score(c0)
+    eng.get_label(0.8, 0.3, "plain", outputs=False)  # wide threshold: many groups
+    eng.copyin(c1)
+    try:
+        for path in ("plain", "exact_int8"):
+            eng.set_finetune_path(path)
+            eng.set_finetune_schedule("host")
+            ref_final, ref_rounds = eng.finetune(0.8, 0.3, 0)
+            eng.set_finetune_schedule("device")
+            before = eng.set_finetune_budget(1 << 16)
+            final, rounds = eng.finetune(0.8, 0.3, 0)
+            batches = eng.set_finetune_budget(8 << 30) - before
+            _say("batched fine-tune", path, "range batches", batches)
+            assert batches > 1
+            assert np.array_equal(final, ref_final), path
+            assert np.array_equal(rounds, ref_rounds), path
+    finally:
+        eng.set_finetune_budget(8 << 30)
+        eng.set_finetune_schedule("device")
+        eng.set_finetune_path("plain")
+    eng.destroy()
+    eng.destroy_score()
