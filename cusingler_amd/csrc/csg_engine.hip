@@ -170,7 +170,7 @@ struct csg_engine
     std::vector<uintptr_t> ftd_sigA, ftd_sigB;
     // launch settings of the work-queue kernels (blocks), tunable
     int   ftd_grid_threads = 1024, ftd_grid_setup = 512, ftd_grid_rank = 2048,
-        ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_thin_max = 8;
+        ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_thin_max = 8, ftd_t64 = 48;
     bool     ftd_autotune = true, ftd_tuned = false;
     uint64_t ftd_budget   = (uint64_t)8 << 30;  // device bytes of one work batch
     int64_t  ftd_batches  = 0;                   // work batches of oversized rounds
@@ -2176,7 +2176,7 @@ int ftd_launch_a(csg_engine* eng)
         std::max<uint32_t>(eng->Gf, 1), (uint32_t)eng->ftd_gstride, eng->ft_exact ? 1 : 0,
         eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_fl_glpos, eng->d_fl_ggenes);
     k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
-                                                   (uint32_t)eng->ftd_thin_max, eng->d_fl_gtile0,
+                                                   (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0,
                                                    eng->d_fl_gthin0, eng->d_fl_gseg0);
     LAUNCHCHK();
     return CSG_OK;
@@ -2208,6 +2208,9 @@ int ftd_launch_corr(csg_engine* eng, int grid)
     const FtdLds d   = ftd_lds(eng);
     FtdCounts*   cnt = eng->d_fl_cnt;
     k_ftd_mfma<<<(unsigned)grid, 256, 0, eng->stream>>>(
+        cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
+        eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
+    k_ftd_mfma64<<<(unsigned)grid, 256, 0, eng->stream>>>(
         cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
         eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
     k_ftd_corr<<<(unsigned)grid, FT_NT, d.corr, eng->stream>>>(
@@ -2256,7 +2259,7 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
         cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
         eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
     k_ftd_tiles<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
-                                                (uint32_t)eng->ftd_thin_max, eng->d_fl_gtile0,
+                                                (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0,
                                                 eng->d_fl_gthin0, eng->d_fl_tiles, eng->d_fl_thin);
     if (ftd_launch_corr(eng, eng->ftd_grid_corr) ||
         ftd_launch_select(eng, eng->ftd_grid_select, quantile))
@@ -2339,6 +2342,7 @@ float ftd_time(csg_engine* eng, const std::function<int()>& launch, int reps)
 
 int ftd_tune(csg_engine* eng, float quantile)
 {
+    int rc;
     const int      cu  = std::max(1, eng->prop.multiProcessorCount);
     const unsigned g   = (unsigned)eng->ftd_grid_threads;
     const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1);
@@ -2355,7 +2359,7 @@ int ftd_tune(csg_engine* eng, float quantile)
         cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
         eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
     k_ftd_tiles<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
-                                                (uint32_t)eng->ftd_thin_max, eng->d_fl_gtile0,
+                                                (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0,
                                                 eng->d_fl_gthin0, eng->d_fl_tiles, eng->d_fl_thin);
     LAUNCHCHK();
     auto pick = [&](const std::vector<int>& mult, const std::function<int(int)>& run, int* grid,
@@ -2377,8 +2381,50 @@ int ftd_tune(csg_engine* eng, float quantile)
     pick({ 2, 4, 8, 16, 32 }, [&](int gr) { return ftd_launch_rank_ref(eng, gr); },
          &eng->ftd_grid_rank, &eng->ftd_tune_ms[0]);
     (void)ftd_launch_rank_ref(eng, eng->ftd_grid_rank);
-    pick({ 1, 2, 3, 4, 6 }, [&](int gr) { return ftd_launch_corr(eng, gr); }, &eng->ftd_grid_corr,
-         &eng->ftd_tune_ms[1]);
+    // big-tile threshold x correlation grid: the scan and tile kernels are
+    // re-run for each threshold (only the tile offsets depend on it)
+    auto retile = [&](uint32_t t64) -> int {
+        eng->ftd_t64 = (int)t64;
+        k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
+                                                       (uint32_t)eng->ftd_thin_max, t64,
+                                                       eng->d_fl_gtile0, eng->d_fl_gthin0,
+                                                       eng->d_fl_gseg0);
+        LAUNCHCHK();
+        FtdCounts hc;
+        HIPCHK(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        int rc2;
+        if ((rc2 = ensure(eng, &eng->d_fl_tiles, &eng->fl_cap.tiles, std::max<uint64_t>(hc.n_tiles, 1))))
+            return rc2;
+        k_ftd_tiles<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
+                                                    (uint32_t)eng->ftd_thin_max, t64,
+                                                    eng->d_fl_gtile0, eng->d_fl_gthin0,
+                                                    eng->d_fl_tiles, eng->d_fl_thin);
+        LAUNCHCHK();
+        return CSG_OK;
+    };
+    float    best_corr = 1e30f;
+    uint32_t best_t64  = (uint32_t)eng->ftd_t64;
+    int      best_grid = eng->ftd_grid_corr;
+    for (uint32_t t64 : { 0xFFFFFFFFu, 32u, 64u, 128u })
+    {
+        int rc2;
+        if ((rc2 = retile(t64)))
+            return rc2;
+        int   grid = eng->ftd_grid_corr;
+        float ms   = 0.f;
+        pick({ 1, 2, 3, 4, 6 }, [&](int gr) { return ftd_launch_corr(eng, gr); }, &grid, &ms);
+        if (ms < best_corr)
+        {
+            best_corr = ms;
+            best_t64  = t64;
+            best_grid = grid;
+        }
+    }
+    eng->ftd_grid_corr  = best_grid;
+    eng->ftd_tune_ms[1] = best_corr;
+    if ((rc = retile(best_t64)))
+        return rc;
     pick({ 2, 4, 8, 16 },
          [&](int gr) {
              k_ftd_reset_nan<<<1, 64, 0, eng->stream>>>(cnt);
@@ -2432,6 +2478,7 @@ std::vector<uintptr_t> ftd_sig_a(const csg_engine* eng)
     return { (uintptr_t)eng->d_fl_htab, (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes,
              (uintptr_t)eng->d_fl_glabels, (uintptr_t)eng->d_fl_gseg0, (uintptr_t)eng->ftd_T,
              (uintptr_t)eng->ftd_grid_setup, (uintptr_t)eng->ftd_thin_max,
+             (uintptr_t)eng->ftd_t64,
              (uintptr_t)eng->ft_exact, (uintptr_t)eng->ftd_grid_threads };
 }
 
@@ -2450,7 +2497,8 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)q,
              (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,
              (uintptr_t)eng->ftd_grid_corr, (uintptr_t)eng->ftd_grid_select,
-             (uintptr_t)eng->ftd_thin_max, (uintptr_t)eng->ftd_grid_threads };
+             (uintptr_t)eng->ftd_thin_max, (uintptr_t)eng->ftd_t64,
+             (uintptr_t)eng->ftd_grid_threads };
 }
 
 // A round whose work tables exceed the budget: its groups are processed in
@@ -2491,7 +2539,8 @@ int ftd_batches(csg_engine* eng, const FtdCounts& h, float quantile, float thre,
     };
     for (uint32_t g = 0; g < ng; ++g)
     {
-        const FtdSizes z = ftd_sizes(groups[g], flags[g], (uint32_t)eng->ftd_thin_max);
+        const FtdSizes z = ftd_sizes(groups[g], flags[g], (uint32_t)eng->ftd_thin_max,
+                                     (uint32_t)eng->ftd_t64);
         FtdCounts      t = cur;
         add(t, z);
         if (g > g0 && ftd_bytes(t, L) > eng->ftd_budget)
@@ -2521,6 +2570,7 @@ int ftd_batches(csg_engine* eng, const FtdCounts& h, float quantile, float thre,
         k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(eng->d_fl_cnt, eng->d_fl_groups,
                                                        eng->d_fl_gflag,
                                                        (uint32_t)eng->ftd_thin_max,
+                                                       (uint32_t)eng->ftd_t64,
                                                        eng->d_fl_gtile0, eng->d_fl_gthin0,
                                                        eng->d_fl_gseg0);
         LAUNCHCHK();
@@ -3025,6 +3075,7 @@ int csg_finetune_tuning(csg_engine* eng, int autotune, int32_t* grids, float* be
         grids[1] = eng->ftd_grid_rank;
         grids[2] = eng->ftd_grid_corr;
         grids[3] = eng->ftd_grid_select;
+        grids[4] = eng->ftd_t64 == (int)0xFFFFFFFF ? -1 : eng->ftd_t64;
     }
     if (best_ms)
         for (int i = 0; i < 3; ++i)

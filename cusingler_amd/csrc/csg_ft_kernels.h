@@ -717,32 +717,39 @@ typedef int v4i_ft __attribute__((ext_vector_type(4)));
 
 constexpr int FT_MFMA_LD = 80;  // bytes per LDS row of the exact fine-tune tile
 
-__device__ inline void ft_mfma_tile(const FtCorrTile& tl, const FtGroup* __restrict__ groups,
-                                    const int16_t* __restrict__ cell_rank,
-                                    const float* __restrict__ cell_sq,
-                                    const int16_t* __restrict__ ref_rank,
-                                    const float* __restrict__ ref_sq, float* __restrict__ scores,
-                                    int8_t (*sa)[16 * FT_MFMA_LD], int8_t (*sb)[256 * FT_MFMA_LD])
+// TC cells (16 or 64) x 256 reference rows; wave w takes reference rows
+// [64 w, 64 w + 64) of all TC cells (TC / 16 x 4 MFMA tiles).
+template <int TC>
+__device__ inline void ft_mfma_tile_t(const FtCorrTile& tl, const FtGroup* __restrict__ groups,
+                                      const int16_t* __restrict__ cell_rank,
+                                      const float* __restrict__ cell_sq,
+                                      const int16_t* __restrict__ ref_rank,
+                                      const float* __restrict__ ref_sq,
+                                      float* __restrict__ scores, int8_t (*sa)[TC * FT_MFMA_LD],
+                                      int8_t (*sb)[256 * FT_MFMA_LD])
 {
     constexpr int LD = FT_MFMA_LD;
-    const FtGroup    g     = groups[tl.group];
-    const int        st    = (int)g.stride;
-    const int        ncell = min(16, (int)g.ncell - (int)tl.cell0);
-    const int        nref  = min(256, (int)g.nref - (int)tl.ref0);
-    const int16_t*   cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
-    const int16_t*   rbase = ref_rank + g.ref_dst0 + (uint64_t)tl.ref0 * st;
-    const int        tid   = threadIdx.x;
-    const int        lane  = tid & 63;
-    const int        wid   = tid >> 6;
-    const int        r16   = lane & 15;
-    const int        kofs  = (lane >> 4) * 16;
-    v4i_ft           acc_x[4], acc_ll[4];
+    constexpr int NI = TC / 16;
+    const FtGroup  g     = groups[tl.group];
+    const int      st    = (int)g.stride;
+    const int      ncell = min(TC, (int)g.ncell - (int)tl.cell0);
+    const int      nref  = min(256, (int)g.nref - (int)tl.ref0);
+    const int16_t* cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
+    const int16_t* rbase = ref_rank + g.ref_dst0 + (uint64_t)tl.ref0 * st;
+    const int      tid   = threadIdx.x;
+    const int      lane  = tid & 63;
+    const int      wid   = tid >> 6;
+    const int      r16   = lane & 15;
+    const int      kofs  = (lane >> 4) * 16;
+    v4i_ft         acc_x[NI][4], acc_ll[NI][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-    {
-        acc_x[j]  = v4i_ft{ 0, 0, 0, 0 };
-        acc_ll[j] = v4i_ft{ 0, 0, 0, 0 };
-    }
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            acc_x[i][j]  = v4i_ft{ 0, 0, 0, 0 };
+            acc_ll[i][j] = v4i_ft{ 0, 0, 0, 0 };
+        }
     auto split8 = [](int4 v, int2& hi, int2& lo) {
         const int16_t* x = (const int16_t*)&v;
         int8_t         h[8], l[8];
@@ -759,10 +766,14 @@ __device__ inline void ft_mfma_tile(const FtCorrTile& tl, const FtGroup* __restr
     for (int k0 = 0; k0 < st; k0 += 64)
     {
         __syncthreads();
-        // A: 16 rows x 8 chunks of 8 values = 128 chunks
-        if (tid < 128)
+        // A: TC rows x 8 chunks of 8 values
+#pragma unroll
+        for (int u = 0; u < (TC * 8 + 255) / 256; ++u)
         {
-            const int row = tid >> 3, ch = tid & 7, k = k0 + ch * 8;
+            const int c = tid + u * 256;
+            if (c >= TC * 8)
+                break;
+            const int row = c >> 3, ch = c & 7, k = k0 + ch * 8;
             int4      v   = int4{ 0, 0, 0, 0 };
             if (row < ncell && k < st)
                 v = *(const int4*)(cbase + (uint64_t)row * st + k);
@@ -785,12 +796,16 @@ __device__ inline void ft_mfma_tile(const FtCorrTile& tl, const FtGroup* __restr
             *(int2*)(&sb[1][row * LD + ch * 8]) = lo;
         }
         __syncthreads();
-        const v4i_ft ah = *(const v4i_ft*)(&sa[0][r16 * LD + kofs]);
-        const v4i_ft al = *(const v4i_ft*)(&sa[1][r16 * LD + kofs]);
-        v4i_ft       ah16;
+        v4i_ft ah[NI], al[NI], ah16[NI];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            ah16[i] = (int)(((unsigned)ah[i] << 4) & 0xF0F0F0F0u);
+        for (int i = 0; i < NI; ++i)
+        {
+            ah[i] = *(const v4i_ft*)(&sa[0][(i * 16 + r16) * LD + kofs]);
+            al[i] = *(const v4i_ft*)(&sa[1][(i * 16 + r16) * LD + kofs]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                ah16[i][q] = (int)(((unsigned)ah[i][q] << 4) & 0xF0F0F0F0u);
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
@@ -799,34 +814,50 @@ __device__ inline void ft_mfma_tile(const FtCorrTile& tl, const FtGroup* __restr
             const v4i_ft bl  = *(const v4i_ft*)(&sb[1][row * LD + kofs]);
             v4i_ft       bh16;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                bh16[i] = (int)(((unsigned)bh[i] << 4) & 0xF0F0F0F0u);
-            acc_x[j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah, bl, acc_x[j], 0, 0, 0);
-            acc_x[j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bh, acc_x[j], 0, 0, 0);
-            acc_x[j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah16, bh16, acc_x[j], 0, 0, 0);
-            acc_ll[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bl, acc_ll[j], 0, 0, 0);
+            for (int q = 0; q < 4; ++q)
+                bh16[q] = (int)(((unsigned)bh[q] << 4) & 0xF0F0F0F0u);
+#pragma unroll
+            for (int i = 0; i < NI; ++i)
+            {
+                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bl, acc_x[i][j], 0, 0, 0);
+                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bh, acc_x[i][j], 0, 0, 0);
+                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah16[i], bh16, acc_x[i][j], 0, 0, 0);
+                acc_ll[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bl, acc_ll[i][j], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-    {
-        const int m = (lane >> 4) * 4 + e;
-        if (m >= ncell)
-            continue;
-        const float sq  = cell_sq[g.cell_item0 + tl.cell0 + m];
-        float*      out = scores + g.score0 + (uint64_t)(tl.cell0 + m) * g.nref + tl.ref0;
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int e = 0; e < 4; ++e)
         {
-            const int n = wid * 64 + j * 16 + r16;
-            if (n >= nref)
+            const int m = i * 16 + (lane >> 4) * 4 + e;
+            if (m >= ncell)
                 continue;
-            const long long s4   = ((long long)acc_x[j][e] << 8) + (long long)acc_ll[j][e];
-            const float     prod = sq * ref_sq[g.ref_item0 + tl.ref0 + n];
-            const float     d    = sqrtf(prod);
-            out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
+            const float sq  = cell_sq[g.cell_item0 + tl.cell0 + m];
+            float*      out = scores + g.score0 + (uint64_t)(tl.cell0 + m) * g.nref + tl.ref0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int n = wid * 64 + j * 16 + r16;
+                if (n >= nref)
+                    continue;
+                const long long s4   = ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
+                const float     prod = sq * ref_sq[g.ref_item0 + tl.ref0 + n];
+                const float     d    = sqrtf(prod);
+                out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
+            }
         }
-    }
+}
+
+__device__ inline void ft_mfma_tile(const FtCorrTile& tl, const FtGroup* __restrict__ groups,
+                                    const int16_t* __restrict__ cell_rank,
+                                    const float* __restrict__ cell_sq,
+                                    const int16_t* __restrict__ ref_rank,
+                                    const float* __restrict__ ref_sq, float* __restrict__ scores,
+                                    int8_t (*sa)[16 * FT_MFMA_LD], int8_t (*sb)[256 * FT_MFMA_LD])
+{
+    ft_mfma_tile_t<16>(tl, groups, cell_rank, cell_sq, ref_rank, ref_sq, scores, sa, sb);
 }
 
 __global__ __launch_bounds__(256) void k_ft_mfma(const FtCorrTile* __restrict__ tiles,

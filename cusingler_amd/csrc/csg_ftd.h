@@ -60,6 +60,9 @@ struct FtdCounts
     uint64_t rthin_bound;  // tiles a redo of every exact-form cell would take
 };
 
+// Height of the big exact fine-tune tile (cells).
+constexpr int FTD_BIG = 32;
+
 // Per-group flags (bit 0: fewer than 20 genes, bit 1: exact form).
 constexpr uint8_t FTD_SHORT = 1, FTD_EXACT = 2;
 
@@ -351,7 +354,10 @@ struct FtdSizes
     uint64_t ref_items, cell_items, ref_elems, cell_elems, scores, tiles, thin, segs, rthin;
 };
 
-__host__ __device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, uint32_t thin_max)
+// Exact-form groups of at least t64 cells use FTD_BIG-cell tiles, the others
+// 16-cell tiles.
+__host__ __device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, uint32_t thin_max,
+                                              uint32_t t64)
 {
     FtdSizes z{};
     if (flag & FTD_SHORT)
@@ -365,8 +371,9 @@ __host__ __device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, ui
     z.segs            = nc * G.nlabel;
     if (flag & FTD_EXACT)
     {
-        z.tiles = ((nc + 15) / 16) * ((nr + 255) / 256);
-        z.rthin = nc * ((nr + 255) / 256);
+        const uint64_t tc = nc >= t64 ? FTD_BIG : 16;
+        z.tiles           = ((nc + tc - 1) / tc) * ((nr + 255) / 256);
+        z.rthin           = nc * ((nr + 255) / 256);
     }
     else if (nc <= thin_max)
         z.thin = (nr + 255) / 256;
@@ -378,7 +385,7 @@ __host__ __device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, ui
 __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict__ cnt,
                                                           FtGroup* __restrict__ groups,
                                                           const uint8_t* __restrict__ gflag,
-                                                          uint32_t thin_max,
+                                                          uint32_t thin_max, uint32_t t64,
                                                           uint32_t* __restrict__ gtile0,
                                                           uint32_t* __restrict__ gthin0,
                                                           uint64_t* __restrict__ gseg0)
@@ -397,7 +404,7 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
     uint32_t            smax    = 0;
     for (uint32_t g = g0; g < g1; ++g)
     {
-        const FtdSizes z = ftd_sizes(groups[g], gflag[g], thin_max);
+        const FtdSizes z = ftd_sizes(groups[g], gflag[g], thin_max, t64);
         acc[0] += z.ref_items;
         acc[1] += z.cell_items;
         acc[2] += z.ref_elems;
@@ -429,7 +436,7 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
     for (uint32_t g = g0; g < g1; ++g)
     {
         FtGroup&       G = groups[g];
-        const FtdSizes z = ftd_sizes(G, gflag[g], thin_max);
+        const FtdSizes z = ftd_sizes(G, gflag[g], thin_max, t64);
         G.ref_item0      = (uint32_t)run[0];
         G.cell_item0     = (uint32_t)run[1];
         G.ref_dst0       = run[2];
@@ -526,7 +533,7 @@ __global__ void k_ftd_refmap(const FtdCounts* __restrict__ cnt, const FtGroup* _
 
 // Correlation tile lists of every group with work.
 __global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
-                            const uint8_t* __restrict__ gflag, uint32_t thin_max,
+                            const uint8_t* __restrict__ gflag, uint32_t thin_max, uint32_t t64,
                             const uint32_t* __restrict__ gtile0, const uint32_t* __restrict__ gthin0,
                             FtCorrTile* __restrict__ tiles, FtCorrTile* __restrict__ thin)
 {
@@ -542,10 +549,12 @@ __global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __
             continue;
         if (f & FTD_EXACT)
         {
-            uint32_t t = gtile0[g];
-            for (uint32_t c0 = 0; c0 < G.ncell; c0 += 16)
+            // lim marks the tile height: 0 for 16 cells, FTD_BIG for the big tile
+            const uint32_t tc = G.ncell >= t64 ? (uint32_t)FTD_BIG : 16u;
+            uint32_t       t  = gtile0[g];
+            for (uint32_t c0 = 0; c0 < G.ncell; c0 += tc)
                 for (uint32_t r0 = 0; r0 < G.nref; r0 += 256)
-                    tiles[t++] = FtCorrTile{ g, c0, r0, 0u };
+                    tiles[t++] = FtCorrTile{ g, c0, r0, tc == 16 ? 0u : tc };
         }
         else if (G.ncell <= thin_max)
         {
@@ -722,10 +731,31 @@ __global__ __launch_bounds__(256) void k_ftd_mfma(
     for (uint32_t t = blockIdx.x; t < n; t += gridDim.x)
     {
         const FtCorrTile tl = tiles[t];
-        if (!(gflag[tl.group] & FTD_EXACT))
+        if (!(gflag[tl.group] & FTD_EXACT) || tl.lim != 0)
             continue;  // uniform per block: the whole tile is skipped
         __syncthreads();
         ft_mfma_tile(tl, groups, cell_rank, cell_sq, ref_rank, ref_sq, scores, sa, sb);
+    }
+}
+
+// Exact tiles of FTD_BIG cells x 256 rows (groups of at least t64 cells).
+__global__ __launch_bounds__(256) void k_ftd_mfma64(
+    const FtdCounts* __restrict__ cnt, const FtCorrTile* __restrict__ tiles,
+    const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
+    const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
+    const int16_t* __restrict__ ref_rank, const float* __restrict__ ref_sq,
+    float* __restrict__ scores)
+{
+    __shared__ __attribute__((aligned(16))) int8_t sa[2][FTD_BIG * FT_MFMA_LD];
+    __shared__ __attribute__((aligned(16))) int8_t sb[2][256 * FT_MFMA_LD];
+    const uint32_t n = cnt->n_tiles;
+    for (uint32_t t = blockIdx.x; t < n; t += gridDim.x)
+    {
+        const FtCorrTile tl = tiles[t];
+        if (!(gflag[tl.group] & FTD_EXACT) || tl.lim != (uint32_t)FTD_BIG)
+            continue;
+        __syncthreads();
+        ft_mfma_tile_t<FTD_BIG>(tl, groups, cell_rank, cell_sq, ref_rank, ref_sq, scores, sa, sb);
     }
 }
 
