@@ -3,10 +3,13 @@
 //
 // Score step (get_label, src/cusingler.cu:986-1136): the reference rows are
 // ranked once; query rows are processed in chunks of Qc rows: rank, contract
-// against every reference row, per-label quantile select, then one label pass
-// over all cells.  Fine-tune (cufinetune, :1297-1333): level-synchronous over
-// all cells; each round groups the active cells by candidate set on the host,
-// and the device ranks, correlates and filters every group in batches.
+// against every reference row (or, with the learned proposal, against the
+// proposed labels' rows), per-label quantile select, then one label pass over
+// all cells.  Fine-tune (cufinetune, :1297-1333): level-synchronous over all
+// cells; by default each round groups the active cells by candidate set on the
+// device and ranks, correlates and filters every group in work-queue kernels
+// replayed from captured graphs; the host-grouped schedule does the grouping
+// on the host.
 
 #include <hip/hip_runtime.h>
 
@@ -684,9 +687,16 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
         (rc = dalloc(eng, &eng->d_sqr_f, nr_pad)) || (rc = dalloc(eng, &eng->d_sqr_i, nr_pad)))
         return rc;
     HIPCHK(hipMemsetAsync(eng->d_Xr, 0, nr_pad * eng->Kp * sizeof(int16_t), eng->stream));
-    // query chunk: the score chunk holds at most ~2 GiB
-    const size_t budget = (size_t)2 << 30;
-    size_t       qc     = budget / ((size_t)ref_height * sizeof(float));
+    // query chunk: the score chunk holds at most 2^31 - 2^20 scores (8 GiB, so
+    // that every chunk-local score index stays below 2^31) and at most a
+    // quarter of the free device memory
+    size_t budget = ((size_t)1 << 31) - ((size_t)1 << 20);
+    {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b / 16 < budget)
+            budget = std::max<size_t>(free_b / 16, (size_t)1 << 26);
+    }
+    size_t qc = budget / (size_t)ref_height;
     qc                  = std::max<size_t>(128, qc / 128 * 128);
     qc                  = std::min<size_t>(qc, round_up(qry_height, 128));
     eng->Qc             = (uint32_t)qc;
