@@ -68,6 +68,7 @@ struct csg_engine
     // ---- score step ----
     bool      score_loaded = false;
     uint32_t  Nr = 0, Nq = 0, G = 0, Kp = 0, L = 0;
+    bool      w8_ok = false;  // the wide INT8 kernels' LDS could be granted
     int       nbins = 0;
     std::vector<uint32_t> ctidx;
     uint16_t *d_rdata = nullptr, *d_qdata = nullptr;
@@ -78,12 +79,12 @@ struct csg_engine
     float     pct_quantile = -1.f;
     int16_t*  d_Xr = nullptr;
     float*    d_sqr_f = nullptr;
-    long long* d_sqr_i = nullptr;
+    double*    d_irr   = nullptr;  // inverse roots of the exact sums of X^2 (reference rows)
     bool      ref_ranked = false;
     uint32_t  Qc = 0;
     int16_t*  d_Xq = nullptr;
     float*    d_sqq_f = nullptr;
-    long long* d_sqq_i = nullptr;
+    double*    d_irq   = nullptr;  // the same for the query chunk
     float*    d_S = nullptr;
     float*    d_P = nullptr;
     int32_t*  d_first = nullptr;
@@ -503,6 +504,15 @@ int csg_init_gpu(int gpuid, csg_engine** out)
             eng->lds_limit = (size_t)want;
         (void)hipGetLastError();  // a refused attribute must not surface at the next launch
     }
+    // the wide INT8 kernels stage three K steps of operands in dynamic LDS
+    eng->w8_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)W8_LDS &&
+                 hipFuncSetAttribute((const void*)k_gemm_int8w,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     W8_LDS) == hipSuccess &&
+                 hipFuncSetAttribute((const void*)k_gemm_int8w_group,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     W8_LDS) == hipSuccess;
+    (void)hipGetLastError();
     {
         const int cu          = std::max(1, eng->prop.multiProcessorCount);
         eng->ftd_grid_threads = cu * 4;
@@ -539,10 +549,10 @@ int csg_destroy_score(csg_engine* eng)
     dfree(eng->d_pct);
     dfree(eng->d_Xr);
     dfree(eng->d_sqr_f);
-    dfree(eng->d_sqr_i);
+    dfree(eng->d_irr);
     dfree(eng->d_Xq);
     dfree(eng->d_sqq_f);
-    dfree(eng->d_sqq_i);
+    dfree(eng->d_irq);
     dfree(eng->d_S);
     dfree(eng->d_nan_list);
     dfree(eng->d_rh);
@@ -684,7 +694,7 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
     // reference ranks (rows padded to the 128-row tile)
     const size_t nr_pad = round_up(ref_height, 256);
     if ((rc = dalloc(eng, &eng->d_Xr, nr_pad * eng->Kp)) ||
-        (rc = dalloc(eng, &eng->d_sqr_f, nr_pad)) || (rc = dalloc(eng, &eng->d_sqr_i, nr_pad)))
+        (rc = dalloc(eng, &eng->d_sqr_f, nr_pad)) || (rc = dalloc(eng, &eng->d_irr, nr_pad)))
         return rc;
     HIPCHK(hipMemsetAsync(eng->d_Xr, 0, nr_pad * eng->Kp * sizeof(int16_t), eng->stream));
     // query chunk: the score chunk holds at most 2^31 - 2^20 scores (8 GiB, so
@@ -701,7 +711,7 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
     qc                  = std::min<size_t>(qc, round_up(qry_height, 128));
     eng->Qc             = (uint32_t)qc;
     if ((rc = dalloc(eng, &eng->d_Xq, qc * eng->Kp)) || (rc = dalloc(eng, &eng->d_sqq_f, qc)) ||
-        (rc = dalloc(eng, &eng->d_sqq_i, qc)) || (rc = dalloc(eng, &eng->d_S, qc * ref_height)) ||
+        (rc = dalloc(eng, &eng->d_irq, qc)) || (rc = dalloc(eng, &eng->d_S, qc * ref_height)) ||
         (rc = dalloc(eng, &eng->d_nan_list, qc * ct_num)) ||
         (rc = dalloc(eng, &eng->d_P, (size_t)qry_height * ct_num)) ||
         (rc = dalloc(eng, &eng->d_first, qry_height)) ||
@@ -717,11 +727,11 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
 namespace {
 
 int rank_rows(csg_engine* eng, const uint16_t* data, const int* ind, const int* ptr, uint32_t row0,
-              uint32_t nrows, int16_t* X, float* sq_f, long long* sq_i)
+              uint32_t nrows, int16_t* X, float* sq_f, double* ir)
 {
     const size_t smem = rank_smem(eng->nbins, eng->Kp);
     k_rank_csr<256><<<nrows, 256, smem, eng->stream>>>(data, ind, ptr, row0, nrows, eng->G, eng->Kp,
-                                                       eng->nbins, X, sq_f, sq_i);
+                                                       eng->nbins, X, sq_f, ir);
     LAUNCHCHK();
     return CSG_OK;
 }
@@ -735,11 +745,11 @@ int score_chunk(csg_engine* eng, int path, uint32_t nq)
                                                     eng->d_sqq_f, eng->d_sqr_f, (int)nq,
                                                     (int)eng->Nr, eng->d_S, (size_t)eng->Nr);
     }
-    else if (path == CSG_PATH_EXACT_INT8 && eng->G <= 1920)
+    else if (path == CSG_PATH_EXACT_INT8 && eng->G <= 1920 && eng->w8_ok)
     {
         dim3 grid((eng->Nr + W8_BN - 1) / W8_BN, (nq + W8_BM - 1) / W8_BM);
-        k_gemm_int8w<<<grid, 512, 0, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                                                    (int)eng->Kp, eng->d_sqq_i, eng->d_sqr_i,
+        k_gemm_int8w<<<grid, 512, W8_LDS, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                                                    (int)eng->Kp, eng->d_irq, eng->d_irr,
                                                     (int)nq, (int)eng->Nr, eng->d_S,
                                                     (size_t)eng->Nr);
     }
@@ -747,7 +757,7 @@ int score_chunk(csg_engine* eng, int path, uint32_t nq)
     {
         dim3 grid((eng->Nr + I8_BN - 1) / I8_BN, (nq + I8_BM - 1) / I8_BM);
         k_gemm_int8<<<grid, 256, 0, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                                                   (int)eng->Kp, eng->d_sqq_i, eng->d_sqr_i,
+                                                   (int)eng->Kp, eng->d_irq, eng->d_irr,
                                                    (int)nq, (int)eng->Nr, eng->d_S,
                                                    (size_t)eng->Nr);
     }
@@ -1143,10 +1153,10 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
     {
         PhaseScope ps(eng, 1, 1);
         const uint32_t b_max = (uint32_t)round_up(eng->Nr, 256) - 1;
-        k_gemm_int8w_group<<<(unsigned)tiles.size(), 512, 0, eng->stream>>>(
+        k_gemm_int8w_group<<<(unsigned)tiles.size(), 512, W8_LDS, eng->stream>>>(
             eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart, eng->d_llen,
             eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl, (int)eng->Kp, b_max,
-            eng->d_sqq_i, eng->d_sqr_i, eng->d_S);
+            eng->d_irq, eng->d_irr, eng->d_S);
         LAUNCHCHK();
     }
     {
@@ -1323,6 +1333,9 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     if (prop && eng->G > 1920)
         return eng->fail(CSG_ERR_ARGUMENT, "the learned proposal's grouped contraction needs a "
                                            "score gene set of at most 1920 genes");
+    if (prop && !eng->w8_ok)
+        return eng->fail(CSG_ERR_DEVICE, "the device does not grant the grouped contraction's "
+                                         "LDS");
     if ((rc = set_pct(eng, &eng->d_pct, &eng->pct_quantile, eng->ctidx, eng->L, quantile)))
         return rc;
     if (prop && !eng->d_pcells && (rc = alloc_proposal(eng)))
@@ -1356,7 +1369,7 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     {
         PhaseScope ps(eng, 0, 1);
         if ((rc = rank_rows(eng, eng->d_rdata, eng->d_rind, eng->d_rptr, 0, eng->Nr, eng->d_Xr,
-                            eng->d_sqr_f, eng->d_sqr_i)))
+                            eng->d_sqr_f, eng->d_irr)))
             return rc;
         eng->ref_ranked        = true;
         eng->ref_prepared_path = -1;
@@ -1374,7 +1387,7 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
         {
             PhaseScope ps(eng, 0, path == CSG_PATH_PLAIN || path == CSG_PATH_FP32_MFMA ? 1 : 2);
             if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
-                                eng->d_sqq_f, eng->d_sqq_i)))
+                                eng->d_sqq_f, eng->d_irq)))
                 return rc;
             if ((rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh,
                                        eng->d_ql, eng->d_qf16)))
@@ -3028,7 +3041,7 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
     if (!eng->ref_ranked)
     {
         if ((rc = rank_rows(eng, eng->d_rdata, eng->d_rind, eng->d_rptr, 0, eng->Nr, eng->d_Xr,
-                            eng->d_sqr_f, eng->d_sqr_i)))
+                            eng->d_sqr_f, eng->d_irr)))
             return rc;
         eng->ref_ranked        = true;
         eng->ref_prepared_path = -1;
@@ -3040,7 +3053,7 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
         eng->ref_prepared_path = path;
     }
     if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
-                        eng->d_sqq_f, eng->d_sqq_i)) ||
+                        eng->d_sqq_f, eng->d_irq)) ||
         (rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh, eng->d_ql,
                                eng->d_qf16)) ||
         (rc = score_chunk(eng, path, nq)))
@@ -3134,6 +3147,8 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
 {
     if (!eng || !max_err || which < 0 || which > 4)
         return CSG_ERR_ARGUMENT;
+    if (which == 4 && !eng->w8_ok)
+        return eng->fail(CSG_ERR_DEVICE, "the device does not grant the wide kernel's LDS");
     HIPCHK(hipSetDevice(eng->device));
     const int nq = 256, nr = 256, Kp = 192, G = 1500;
     std::vector<int16_t> xq((size_t)nq * Kp), xr((size_t)nr * Kp);
@@ -3149,12 +3164,12 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
     for (auto& v : xr)
         v = (int16_t)rnd();
     std::vector<float>     ones(std::max(nq, nr), 1.f);
-    std::vector<long long> ones64(std::max(nq, nr), 1ll);
+    std::vector<double>    ones64(std::max(nq, nr), 1.0);
     int16_t *dq = nullptr, *dr = nullptr;
     int8_t *qh = nullptr, *ql = nullptr, *rh = nullptr, *rl = nullptr;
     _Float16 *qf = nullptr, *rf = nullptr;
     float *d1 = nullptr, *dS = nullptr;
-    long long* d1i = nullptr;
+    double* d1i = nullptr;
     int    rc = CSG_OK;
     auto   cleanup = [&]() {
         dfree(dq); dfree(dr); dfree(qh); dfree(ql); dfree(rh); dfree(rl); dfree(qf); dfree(rf);
@@ -3189,7 +3204,7 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
     {
         k_split_limbs<<<bq, 256, 0, eng->stream>>>(dq, xq.size(), qh, ql);
         k_split_limbs<<<br, 256, 0, eng->stream>>>(dr, xr.size(), rh, rl);
-        k_gemm_int8w<<<dim3(nr / W8_BN, nq / W8_BM), 512, 0, eng->stream>>>(qh, ql, rh, rl, Kp, d1i,
+        k_gemm_int8w<<<dim3(nr / W8_BN, nq / W8_BM), 512, W8_LDS, eng->stream>>>(qh, ql, rh, rl, Kp, d1i,
                                                                             d1i, nq, nr, dS, nr);
     }
     else if (which == 2)
@@ -3596,7 +3611,7 @@ int csg_mlp_train(csg_engine* eng, const int32_t* labels, int epochs, int batch,
     {
         const uint32_t nq = std::min(eng->Qc, N - q0);
         rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq, eng->d_sqq_f,
-                       eng->d_sqq_i);
+                       eng->d_irq);
         if (!rc)
         {
             const size_t n = (size_t)nq * Kp;
@@ -3680,7 +3695,7 @@ int csg_mlp_predict(csg_engine* eng, int k, int32_t* top, float* margin)
     {
         const uint32_t nq = std::min(eng->Qc, eng->Nq - q0);
         if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
-                            eng->d_sqq_f, eng->d_sqq_i)) ||
+                            eng->d_sqq_f, eng->d_irq)) ||
             (rc = mlp_propose(eng, nq, k, dtop, dmar)))
             break;
         hipError_t he = hipMemcpyAsync(top + (size_t)q0 * k, dtop, (size_t)nq * k * sizeof(int32_t),

@@ -1,8 +1,10 @@
 // Certified labels for the exact score path.
 //
 // The exact path knows every rank dot product exactly (S4) and the exact sums
-// of squares Sxx, Syy of both rows, and returns S4 / sqrt(Sxx Syy) in double,
-// rounded once to float: the correlation to within half a float ulp.  The
+// of squares Sxx, Syy of both rows, and returns (S4 / sqrt(Sxx)) / sqrt(Syy)
+// as S4 times the two correctly rounded inverse roots in double (relative
+// error below 2^-50), rounded once to float: the correlation to within half a
+// float ulp plus 1e-15.  The
 // reference computes the same quantity in FP32: its sequential sum of
 // products differs from S4 by at most gamma_{G-1} * sum |p_k| <= gamma_{G-1} *
 // sqrt(Sxx Syy) (gamma_n = n u / (1 - n u), u = 2^-24), and its divisor

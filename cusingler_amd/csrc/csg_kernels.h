@@ -127,10 +127,18 @@ template <int NT> __device__ inline void bins_to_ranks(int* bins, int nbins, int
 
 // ------------------------------------------------- score-step rank rows ---
 
+// 1/sqrt(s) of an exact sum of squares s < 2^53: the conversion is exact and
+// the root and the division are correctly rounded IEEE double operations, so
+// host and device give the same bits (+inf for s = 0).
+__host__ __device__ inline double inv_root(long long s)
+{
+    return 1.0 / sqrt((double)s);
+}
+
 // One block per row of a coded CSR matrix over G genes.  Writes the dense
 // doubled centred ranks X[row, 0..Kp) (zero past G), the FP32 sum of X^2 in
 // gene order (the reference's `sumxx += x*x` loop of src/cusingler.cu:931-938,
-// contracted to fma) and the exact int64 sum of X^2.
+// contracted to fma) and the inverse root of the exact int64 sum of X^2.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ data,
                                                  const int* __restrict__ indices,
@@ -138,7 +146,7 @@ __global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ da
                                                  uint32_t nrows, uint32_t G, uint32_t Kp,
                                                  int nbins, int16_t* __restrict__ X,
                                                  float* __restrict__ sq_f32,
-                                                 long long* __restrict__ sq_i64)
+                                                 double* __restrict__ ir_exact)
 {
     extern __shared__ int smem[];
     int*     bins = smem;                          // nbins
@@ -188,7 +196,7 @@ __global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ da
         long long t = 0;
         for (int w = 0; w < NT / kWave; ++w)
             t += red[w];
-        sq_i64[r] = t;
+        ir_exact[r] = inv_root(t);
         float acc = 0.f;
         uint32_t g = 0;
         for (; g + 8 <= G; g += 8)

@@ -17,13 +17,14 @@ typedef int      v4i __attribute__((ext_vector_type(4)));
 typedef float    v4f __attribute__((ext_vector_type(4)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
-// Correlation of the exact path: the exact rank dot product over the exact
-// root of the two rows' exact sums of squares, in double, rounded once to
-// float (NaN when a row has no variance, as the reference's 0 divisor).
-__device__ inline float exact_corr(long long s4, double sq_row, long long sq_col)
+// Correlation of the exact path: the exact rank dot product s4 (an integer
+// below 2^53, held exactly in a double) times the two rows' inverse roots
+// 1/sqrt(sum X^2) (inv_root, csg_kernels.h), in double, rounded
+// once to float.  A row without variance has s4 = 0 and an infinite inverse
+// root, so the score is NaN, as the reference's 0 divisor makes it.
+__device__ inline float exact_corr(double s4, double ir_row, double ir_col)
 {
-    const double d = sqrt(sq_row * (double)sq_col);
-    return d != 0.0 ? (float)((double)s4 / d) : __builtin_nanf("");
+    return (float)((s4 * ir_row) * ir_col);
 }
 
 // ------------------------------------------------------------ limb split ---
@@ -70,18 +71,15 @@ __global__ void k_to_half(const int16_t* __restrict__ X, size_t n, _Float16* __r
 // Block tile 64 query rows x 128 reference rows, 4 waves in a 2 x 2 grid of
 // 32 x 64 wave tiles, K staged 64 at a time through double-buffered LDS
 // (rows padded to 80 bytes).  Every output keeps three int32 partials
-// (hi*hi, hi*lo + lo*hi, lo*lo) recombined in int64, so
+// (hi*hi, hi*lo + lo*hi, lo*lo) recombined exactly in double, so
 //     S4 = sum_k Xq[m,k] * Xr[n,k]
-// is exact.  The epilogue forms the score with the reference's own FP32
-// divisor  D = sqrtf(sxx_f * syy_f)  (per-row FP32 sequential sums, computed
-// as the reference does) and one correctly rounded division:
-//     score = (float)((double)S4 / D),  NaN if D == 0.
+// is exact, and the score is exact_corr(S4, ir_q[m], ir_r[n]).
 constexpr int I8_BM = 64, I8_BN = 128, I8_BK = 64, I8_LD = 80;
 
 __global__ __launch_bounds__(256) void k_gemm_int8(
     const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
-    const int8_t* __restrict__ rl, int Kp, const long long* __restrict__ sq_q,
-    const long long* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+    const int8_t* __restrict__ rl, int Kp, const double* __restrict__ ir_q,
+    const double* __restrict__ ir_r, int nq, int nr, float* __restrict__ S, size_t ldS)
 {
     __shared__ __attribute__((aligned(16))) int8_t sm[2][(2 * I8_BM + 2 * I8_BN) * I8_LD];
     const int tid  = threadIdx.x;
@@ -200,7 +198,7 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
             const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + e;
             if (m >= nq)
                 continue;
-            const double sq  = (double)sq_q[m];
+            const double ir  = ir_q[m];
             float*       out = S + (size_t)m * ldS;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -208,9 +206,9 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
                 const int n = n0 + wn * 64 + j * 16 + r16;
                 if (n >= nr)
                     continue;
-                const long long s4 = ((long long)acc_hh[i][j][e] << 16) +
-                                     ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
-                out[n] = exact_corr(s4, sq, sq_r[n]);
+                const double s4 = (double)acc_hh[i][j][e] * 65536.0 +
+                                  (double)acc_x[i][j][e] * 256.0 + (double)acc_ll[i][j][e];
+                out[n] = exact_corr(s4, ir, ir_r[n]);
             }
         }
 }
@@ -221,9 +219,19 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
 // so two int32 partials suffice: acc_x takes three products, acc_ll one.  16*h
 // is formed in registers from the packed high limbs ((w << 4) & 0xF0F0F0F0 per
 // dword: the low nibble of each byte is h in two's complement).  With half the
-// accumulators a wave holds a 64 x 64 tile and the block 128 x 256, which
-// halves the operand traffic per product of the 64 x 128 kernel.
-constexpr int W8_BM = 128, W8_BN = 256, W8_BK = 64, W8_LD = 80;
+// accumulators a wave holds a 64 x 64 tile and the block 128 x 256.
+//
+// Operand staging.  K advances 64 at a time through W8_NS stages of LDS, each
+// [A hi 128 rows][A lo 128][B hi 256][B lo 256] of 64 bytes, filled by
+// direct-to-LDS loads (global_load_lds_dwordx4: lane i of a wave writes 16
+// bytes at the wave's LDS address + 16 i) so that two stages are in flight
+// while a third is read, without staging registers.  A wave instruction fills
+// one group of 16 rows; the 16-byte slot of row r holding K chunk c is
+// c ^ w8_swz(r), which keeps the 16-row ds_read_b128 of the MFMA operands free
+// of bank conflicts.
+constexpr int W8_BM = 128, W8_BN = 256, W8_BK = 64, W8_NS = 3;
+constexpr int W8_STAGE = (2 * W8_BM + 2 * W8_BN) * W8_BK;  // 48 KB
+constexpr int W8_LDS   = W8_NS * W8_STAGE;                  // dynamic LDS per block
 
 __device__ inline v4i times16(v4i h)
 {
@@ -234,23 +242,36 @@ __device__ inline v4i times16(v4i h)
     return r;
 }
 
+__device__ __forceinline__ int w8_swz(int r) { return ((r >> 1) + (r >> 3)) & 3; }
+
+// Dispatch order -> tile order: workgroup b runs on XCD b % 8 (round-robin
+// dispatch), and XCD x takes the contiguous range [start(x), start(x + 1)) of
+// the n tiles, so tiles that share operands (neighbours in the tile order)
+// share one XCD's L2.
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n)
+{
+    const uint32_t x = b % 8, local = b / 8, q = n / 8, r = n % 8;
+    return x * q + min(x, r) + local;
+}
+
 // Row / column maps of one 128 x 256 block of the wide kernel.  a_row(r) and
 // b_row(r) give the operand rows staged for block rows / columns r (always
-// inside the operand buffers); row_ok / col_ok say which results are stored.
+// inside the operand buffers); row_ok / col_ok say which results are stored;
+// row_ir / col_ir are the rows' inverse roots 1/sqrt(sum X^2).
 struct DenseW8Map
 {
-    int          m0, n0, nq, nr;
-    const long long* sq_q;
-    const long long* sq_r;
-    float*           S;
-    size_t           ldS;
-    __device__ size_t    a_row(int r) const { return (size_t)(m0 + r); }
-    __device__ size_t    b_row(int r) const { return (size_t)(n0 + r); }
-    __device__ bool      row_ok(int m) const { return m0 + m < nq; }
-    __device__ long long row_sq(int m) const { return sq_q[m0 + m]; }
-    __device__ float*    row_out(int m) const { return S + (size_t)(m0 + m) * ldS + n0; }
-    __device__ bool      col_ok(int n) const { return n0 + n < nr; }
-    __device__ long long col_sq(int n) const { return sq_r[n0 + n]; }
+    int           m0, n0, nq, nr;
+    const double* ir_q;
+    const double* ir_r;
+    float*        S;
+    size_t        ldS;
+    __device__ size_t a_row(int r) const { return (size_t)(m0 + r); }
+    __device__ size_t b_row(int r) const { return (size_t)(n0 + r); }
+    __device__ bool   row_ok(int m) const { return m0 + m < nq; }
+    __device__ double row_ir(int m) const { return ir_q[m0 + m]; }
+    __device__ float* row_out(int m) const { return S + (size_t)(m0 + m) * ldS + n0; }
+    __device__ bool   col_ok(int n) const { return n0 + n < nr; }
+    __device__ double col_ir(int n) const { return ir_r[n0 + n]; }
 };
 
 // One label of a grouped contraction: the query rows listed in cells[0, mcount)
@@ -262,48 +283,62 @@ struct GroupW8Map
     const uint32_t* cells;
     int             mcount, ncount;
     uint32_t        b0, b_max;
-    const long long* sq_q;
-    const long long* sq_r;
-    float*           out;
-    size_t           ld;
-    __device__ size_t    a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
-    __device__ size_t    b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
-    __device__ bool      row_ok(int m) const { return m < mcount; }
-    __device__ long long row_sq(int m) const { return sq_q[cells[m]]; }
-    __device__ float*    row_out(int m) const { return out + (size_t)m * ld; }
-    __device__ bool      col_ok(int n) const { return n < ncount; }
-    __device__ long long col_sq(int n) const { return sq_r[b0 + n]; }
+    const double*   ir_q;
+    const double*   ir_r;
+    float*          out;
+    size_t          ld;
+    __device__ size_t a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
+    __device__ size_t b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
+    __device__ bool   row_ok(int m) const { return m < mcount; }
+    __device__ double row_ir(int m) const { return ir_q[cells[m]]; }
+    __device__ float* row_out(int m) const { return out + (size_t)m * ld; }
+    __device__ bool   col_ok(int n) const { return n < ncount; }
+    __device__ double col_ir(int n) const { return ir_r[b0 + n]; }
 };
 
+// Wave `wid`'s six direct-to-LDS loads of one stage: row group g = wid + 8 u
+// (groups 0-7 A hi, 8-15 A lo, 16-31 B hi, 32-47 B lo); src[u] is this lane's
+// source row offset plus its swizzled 16-byte chunk.
+__device__ __forceinline__ void w8_issue(const int8_t* __restrict__ qh,
+                                         const int8_t* __restrict__ ql,
+                                         const int8_t* __restrict__ rh,
+                                         const int8_t* __restrict__ rl, const size_t (&src)[6],
+                                         int8_t* stage, int wid, int k0)
+{
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+    {
+        const int     g    = wid + 8 * u;
+        const int8_t* base = g < 8 ? qh : g < 16 ? ql : g < 32 ? rh : rl;
+        __builtin_amdgcn_global_load_lds(
+            base + src[u] + k0, (__attribute__((address_space(3))) void*)(stage + g * 1024), 16,
+            0, 0);
+    }
+}
+
+// `sm` is the block's W8_LDS bytes of dynamic LDS; Kp is a multiple of 64.
 template <class Map>
 __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
                                         const int8_t* __restrict__ ql,
                                         const int8_t* __restrict__ rh,
-                                        const int8_t* __restrict__ rl, int Kp, const Map& map)
+                                        const int8_t* __restrict__ rl, int Kp, const Map& map,
+                                        int8_t* sm)
 {
-    // LDS rows: [A hi 128][A lo 128][B hi 256][B lo 256], 80 bytes each
-    __shared__ __attribute__((aligned(16))) int8_t sm[(2 * W8_BM + 2 * W8_BN) * W8_LD];
     const int tid  = threadIdx.x;
     const int lane = tid & 63;
     const int wid  = tid >> 6;
     const int wm   = wid >> 2;  // 0..1: 64-row half of the block
     const int wn   = wid & 3;   // 0..3: 64-column quarter
 
-    // staging map: 16-byte chunk c -> (plane, row, 16-byte column).  A: 2
-    // planes x 128 rows x 4 chunks (thread tid stages row tid/4 of both
-    // planes); B: 2 planes x 256 rows x 4 (rows tid/4 and tid/4 + 128).
-    const int    col  = (tid & 3) * 16;
-    const int    rA   = tid >> 2;
-    const size_t ga   = map.a_row(rA) * (size_t)Kp + col;
-    const size_t gb0  = map.b_row(rA) * (size_t)Kp + col;
-    const size_t gb1  = map.b_row(rA + 128) * (size_t)Kp + col;
-    int8_t*      dA0  = sm + rA * W8_LD + col;
-    int8_t*      dA1  = sm + (W8_BM + rA) * W8_LD + col;
-    int8_t*      dB0  = sm + (2 * W8_BM + rA) * W8_LD + col;
-    int8_t*      dB1  = sm + (2 * W8_BM + rA + 128) * W8_LD + col;
-    int8_t*      dB2  = sm + (2 * W8_BM + W8_BN + rA) * W8_LD + col;
-    int8_t*      dB3  = sm + (2 * W8_BM + W8_BN + rA + 128) * W8_LD + col;
-
+    size_t src[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+    {
+        const int g = wid + 8 * u;
+        const int r = (g < 16 ? (g & 7) : ((g - 16) & 15)) * 16 + (lane >> 2);  // row in its plane
+        src[u]      = (g < 16 ? map.a_row(r) : map.b_row(r)) * (size_t)Kp +
+                 (size_t)(((lane & 3) ^ w8_swz(r)) * 16);
+    }
     v4i acc_x[4][4], acc_ll[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -313,49 +348,44 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             acc_x[i][j]  = v4i{ 0, 0, 0, 0 };
             acc_ll[i][j] = v4i{ 0, 0, 0, 0 };
         }
-    int4 pa0 = *(const int4*)(qh + ga);
-    int4 pa1 = *(const int4*)(ql + ga);
-    int4 pb0 = *(const int4*)(rh + gb0);
-    int4 pb1 = *(const int4*)(rh + gb1);
-    int4 pb2 = *(const int4*)(rl + gb0);
-    int4 pb3 = *(const int4*)(rl + gb1);
-    const int r16  = lane & 15;
-    const int kofs = (lane >> 4) * 16;
-    for (int k0 = 0; k0 < Kp; k0 += W8_BK)
+    const int T = Kp / W8_BK;
+    w8_issue(qh, ql, rh, rl, src, sm, wid, 0);
+    if (T > 1)
+        w8_issue(qh, ql, rh, rl, src, sm + W8_STAGE, wid, W8_BK);
+    const int r16 = lane & 15;
+    const int kc  = lane >> 4;
+    for (int t = 0; t < T; ++t)
     {
-        __syncthreads();  // the previous stage is no longer read
-        *(int4*)dA0 = pa0;
-        *(int4*)dA1 = pa1;
-        *(int4*)dB0 = pb0;
-        *(int4*)dB1 = pb1;
-        *(int4*)dB2 = pb2;
-        *(int4*)dB3 = pb3;
-        __syncthreads();
-        if (k0 + W8_BK < Kp)
-        {
-            const int kn = k0 + W8_BK;
-            pa0 = *(const int4*)(qh + ga + kn);
-            pa1 = *(const int4*)(ql + ga + kn);
-            pb0 = *(const int4*)(rh + gb0 + kn);
-            pb1 = *(const int4*)(rh + gb1 + kn);
-            pb2 = *(const int4*)(rl + gb0 + kn);
-            pb3 = *(const int4*)(rl + gb1 + kn);
-        }
+        // this wave's loads of stage t are done when at most the six of stage
+        // t + 1 are outstanding; the barrier then makes every wave's loads of
+        // stage t visible and ends every wave's reads of stage t - 1, whose
+        // buffer the loads of stage t + 2 overwrite
+        if (t + 1 < T)
+            __builtin_amdgcn_s_waitcnt(0x0F76);  // vmcnt <= 6
+        else
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt 0
+        asm volatile("s_barrier" ::: "memory");
+        if (t + 2 < T)
+            w8_issue(qh, ql, rh, rl, src, sm + ((t + 2) % W8_NS) * W8_STAGE, wid,
+                     (t + 2) * W8_BK);
+        const int8_t* s = sm + (t % W8_NS) * W8_STAGE;
         v4i ah[4], al[4], ah16[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
         {
             const int row = wm * 64 + i * 16 + r16;
-            ah[i]   = *(const v4i*)(sm + row * W8_LD + kofs);
-            al[i]   = *(const v4i*)(sm + (W8_BM + row) * W8_LD + kofs);
-            ah16[i] = times16(ah[i]);
+            const int o   = row * W8_BK + ((kc ^ w8_swz(row)) << 4);
+            ah[i]         = *(const v4i*)(s + o);
+            al[i]         = *(const v4i*)(s + W8_BM * W8_BK + o);
+            ah16[i]       = times16(ah[i]);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
-            const int row = wn * 64 + j * 16 + r16;
-            const v4i bh  = *(const v4i*)(sm + (2 * W8_BM + row) * W8_LD + kofs);
-            const v4i bl  = *(const v4i*)(sm + (2 * W8_BM + W8_BN + row) * W8_LD + kofs);
+            const int row  = wn * 64 + j * 16 + r16;
+            const int o    = row * W8_BK + ((kc ^ w8_swz(row)) << 4);
+            const v4i bh   = *(const v4i*)(s + 2 * W8_BM * W8_BK + o);
+            const v4i bl   = *(const v4i*)(s + (2 * W8_BM + W8_BN) * W8_BK + o);
             const v4i bh16 = times16(bh);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
@@ -375,7 +405,7 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             const int m = wm * 64 + i * 16 + (lane >> 4) * 4 + e;
             if (!map.row_ok(m))
                 continue;
-            const double sq  = (double)map.row_sq(m);
+            const double ir  = map.row_ir(m);
             float*       out = map.row_out(m);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -383,27 +413,35 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
                 const int n = wn * 64 + j * 16 + r16;
                 if (!map.col_ok(n))
                     continue;
-                const long long s4 = ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
-                out[n] = exact_corr(s4, sq, map.col_sq(n));
+                const double s4 = (double)acc_x[i][j][e] * 256.0 + (double)acc_ll[i][j][e];
+                __builtin_nontemporal_store(exact_corr(s4, ir, map.col_ir(n)), out + n);
             }
         }
 }
 
 __global__ __launch_bounds__(512) void k_gemm_int8w(
     const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
-    const int8_t* __restrict__ rl, int Kp, const long long* __restrict__ sq_q,
-    const long long* __restrict__ sq_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+    const int8_t* __restrict__ rl, int Kp, const double* __restrict__ ir_q,
+    const double* __restrict__ ir_r, int nq, int nr, float* __restrict__ S, size_t ldS)
 {
-    const DenseW8Map map{ (int)blockIdx.y * W8_BM, (int)blockIdx.x * W8_BN, nq, nr, sq_q, sq_r, S,
-                          ldS };
-    gemm_int8w_block(qh, ql, rh, rl, Kp, map);
+    extern __shared__ __attribute__((aligned(16))) int8_t w8_sm[];
+    const uint32_t gx = gridDim.x;
+    const uint32_t t  = xcd_order(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+    // XCD ranges are walked in groups of 8 row tiles, column-major inside a
+    // group, so a group's query rows stay in L2 while its columns stream
+    const uint32_t gsz = 8 * gx, grp = t / gsz, in = t % gsz;
+    const uint32_t rows = min(8u, gridDim.y - grp * 8);
+    const DenseW8Map map{ (int)(grp * 8 + in % rows) * W8_BM, (int)(in / rows) * W8_BN, nq, nr,
+                          ir_q, ir_r, S, ldS };
+    gemm_int8w_block(qh, ql, rh, rl, Kp, map, w8_sm);
 }
 
-// Grouped contraction: block b takes tile tiles[b] = (label l, m0, n0), the
-// cells cells[a_base[l] + m0 ..] (a_cnt[l] in all) against reference rows
-// lstart[l] + n0 .. of label l (llen[l] in all); each cell's scores against
-// the label's rows are stored contiguously, label l's block starting at
-// S + c_off[l].
+// Grouped contraction: block b takes tile tiles[xcd_order(b)] = (label l, m0,
+// n0), the cells cells[a_base[l] + m0 ..] (a_cnt[l] in all) against reference
+// rows lstart[l] + n0 .. of label l (llen[l] in all); each cell's scores
+// against the label's rows are stored contiguously, label l's block starting
+// at S + c_off[l].  Tiles are listed label by label, so a label's tiles run
+// on one XCD.
 struct GroupTile
 {
     uint32_t label, m0, n0, pad_;
@@ -415,22 +453,22 @@ __global__ __launch_bounds__(512) void k_gemm_int8w_group(
     const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
     const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
     const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
-    int Kp, uint32_t b_max, const long long* __restrict__ sq_q,
-    const long long* __restrict__ sq_r,
+    int Kp, uint32_t b_max, const double* __restrict__ ir_q, const double* __restrict__ ir_r,
     float* __restrict__ S)
 {
-    const GroupTile t  = tiles[blockIdx.x];
+    extern __shared__ __attribute__((aligned(16))) int8_t w8_sm[];
+    const GroupTile t  = tiles[xcd_order(blockIdx.x, gridDim.x)];
     const uint32_t  nl = llen[t.label];
     const GroupW8Map map{ cells + a_base[t.label] + t.m0,
                           (int)(a_cnt[t.label] - t.m0),
                           (int)(nl - t.n0),
                           lstart[t.label] + t.n0,
                           b_max,
-                          sq_q,
-                          sq_r,
+                          ir_q,
+                          ir_r,
                           S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
                           nl };
-    gemm_int8w_block(qh, ql, rh, rl, Kp, map);
+    gemm_int8w_block(qh, ql, rh, rl, Kp, map, w8_sm);
 }
 
 // ------------------------------------- approximate MFMA contractions ---

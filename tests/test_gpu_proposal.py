@@ -167,17 +167,18 @@ def _ranks_X(data, indices, indptr, rows, G):
 
 def _exact_quantile(c0, q, l):
     """The exact path's quantile of (cell q, label l) before certification:
-    exact integer dot products over the exact root of the exact sums of
-    squares, in double, rounded to float, then the reference's percentile of
-    the label's scores."""
+    exact integer dot products times the two rows' inverse roots 1/sqrt(sum
+    X^2) (correctly rounded doubles), in double, rounded to float, then the
+    reference's percentile of the label's scores."""
     from cusingler_amd import engine
     G = c0.width
     s0, n = int(c0.ctidx[2 * l]), int(c0.ctidx[2 * l + 1])
     xq = _ranks_X(c0.qry_data, c0.qry_indices, c0.qry_indptr, [q], G)
     xr = _ranks_X(c0.ref_data, c0.ref_indices, c0.ref_indptr, range(s0, s0 + n), G)
-    dots = xr @ xq[0]
-    d = np.sqrt(float((xq[0] * xq[0]).sum()) * (xr * xr).sum(axis=1).astype(np.float64))
-    scores = (dots.astype(np.float64) / d).astype(np.float32)
+    dots = (xr @ xq[0]).astype(np.float64)
+    ir_q = 1.0 / np.sqrt(float((xq[0] * xq[0]).sum()))
+    ir_r = 1.0 / np.sqrt((xr * xr).sum(axis=1).astype(np.float64))
+    scores = ((dots * ir_q) * ir_r).astype(np.float32)
     return engine.host_percentile(scores, 0.8)
 
 
