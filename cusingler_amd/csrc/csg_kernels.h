@@ -694,6 +694,50 @@ __device__ inline float pct_from_stats(const PctParam& pp, float p, float a_km1,
     return pct_interp(p, pp.prev_index, pp.step, a_km1, a_k);
 }
 
+// A segment of at most 64 R scores: load into R register slots, send a
+// segment with a NaN to the NaN list, otherwise select its order statistics.
+template <int R>
+__device__ inline void select_segment_reg(const Segment& sg, const float* __restrict__ v,
+                                          const PctParam& pp, float p, uint64_t item,
+                                          float* __restrict__ out, uint64_t* __restrict__ nan_list,
+                                          unsigned int* __restrict__ nan_count,
+                                          WaveSelectSmem& wsm, int lane)
+{
+    float x[R];
+    bool  nan = false;
+    float lo = __builtin_inff(), hi = -__builtin_inff();
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+    {
+        const uint32_t i = lane + 64 * u;
+        x[u]             = i < sg.n ? v[i] : 0.f;
+        if (i < sg.n)
+        {
+            nan |= __builtin_isnan(x[u]);
+            lo = fminf(lo, x[u]);
+            hi = fmaxf(hi, x[u]);
+        }
+    }
+    if (__ballot(nan))
+    {
+        if (lane == 0)
+        {
+            const unsigned int s = atomicAdd(nan_count, 1u);
+            nan_list[s]          = item;
+            out[item]            = __builtin_nanf("");
+        }
+        return;
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    float a0 = 0.f, a1 = 0.f;
+    if (pp.kind == 3)
+        wave_select2_reg<R>(x, v, sg.n, lo, hi, (uint32_t)(pp.k - 1), (uint32_t)pp.k, wsm, &a0,
+                            &a1);
+    if (lane == 0)
+        out[item] = pct_from_stats(pp, p, a0, a1, lo, hi);
+}
+
 // One wave per segment: the reference's percentile (src/cusingler.cu:947-978)
 // of the segment's scores.  Segments holding a NaN are appended to `nan_list`
 // (the libstdc++ sort emulation runs on them in k_select_nan).
@@ -717,40 +761,18 @@ __device__ inline void select_item(const Src& src, uint64_t item,
     }
     if (sg.n <= 1024)
     {
-        // the segment stays in registers for every pass
-        float x[16];
-        bool  nan = false;
-        float lo = __builtin_inff(), hi = -__builtin_inff();
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-        {
-            const uint32_t i = lane + 64 * u;
-            x[u]             = i < sg.n ? v[i] : 0.f;
-            if (i < sg.n)
-            {
-                nan |= __builtin_isnan(x[u]);
-                lo = fminf(lo, x[u]);
-                hi = fmaxf(hi, x[u]);
-            }
-        }
-        if (__ballot(nan))
-        {
-            if (lane == 0)
-            {
-                const unsigned int s = atomicAdd(nan_count, 1u);
-                nan_list[s]          = item;
-                out[item]            = __builtin_nanf("");
-            }
-            return;
-        }
-        lo = wave_min(lo);
-        hi = wave_max(hi);
-        float a0 = 0.f, a1 = 0.f;
-        if (pp.kind == 3)
-            wave_select2_reg<16>(x, v, sg.n, lo, hi, (uint32_t)(pp.k - 1), (uint32_t)pp.k, wsm,
-                                 &a0, &a1);
-        if (lane == 0)
-            out[item] = pct_from_stats(pp, p, a0, a1, lo, hi);
+        // the segment stays in registers for every pass, in the fewest slots
+        // (a power of two) that hold it
+        if (sg.n <= 64)
+            select_segment_reg<1>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
+        else if (sg.n <= 128)
+            select_segment_reg<2>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
+        else if (sg.n <= 256)
+            select_segment_reg<4>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
+        else if (sg.n <= 512)
+            select_segment_reg<8>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
+        else
+            select_segment_reg<16>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
         return;
     }
     bool nan = false;
