@@ -175,6 +175,7 @@ struct csg_engine
     // launch settings of the work-queue kernels (blocks), tunable
     int   ftd_grid_threads = 1024, ftd_grid_setup = 512, ftd_grid_rank = 2048,
         ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_thin_max = 8, ftd_t64 = 48;
+    int ftd_fuse_max = FTD_FUSE_MAX;  // groups of at most this many cells take the fused form
     bool     ftd_autotune = true, ftd_tuned = false;
     uint64_t ftd_budget   = (uint64_t)8 << 30;  // device bytes of one work batch
     int64_t  ftd_batches  = 0;                   // work batches of oversized rounds
@@ -2171,6 +2172,13 @@ FtdLds ftd_lds(const csg_engine* eng)
     return d;
 }
 
+// Largest group (cells) of the fused form: the engine's setting when the
+// per-wave LDS of the wave rank kernel fits, else 0 (no fused groups).
+uint32_t ftd_fuse(const csg_engine* eng)
+{
+    return ftd_lds(eng).rank_ref_w <= 65536 ? (uint32_t)eng->ftd_fuse_max : 0u;
+}
+
 // Round part A: grouping, group set-up and the prefix sums.
 int ftd_launch_a(csg_engine* eng)
 {
@@ -2197,7 +2205,8 @@ int ftd_launch_a(csg_engine* eng)
         cnt, eng->d_fl_gowner, eng->d_fl_gcount, eng->d_fl_lists, eng->d_fl_counts, L,
         eng->d_fl_ctdidx, eng->d_fl_ctdiff, eng->d_fl_cut, eng->d_fllen,
         std::max<uint32_t>(eng->Gf, 1), (uint32_t)eng->ftd_gstride, eng->ft_exact ? 1 : 0,
-        eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_fl_glpos, eng->d_fl_ggenes);
+        ftd_fuse(eng), eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_fl_glpos,
+        eng->d_fl_ggenes);
     k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
                                                    (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0,
                                                    eng->d_fl_gthin0, eng->d_fl_gseg0);
@@ -2214,7 +2223,8 @@ int ftd_launch_rank_ref(csg_engine* eng, int grid)
     FtdCounts*     cnt = eng->d_fl_cnt;
     if (d.rank_ref_w <= 65536)
         k_ftd_rank_ref_w<<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
-            cnt, eng->d_fl_groups, eng->d_fl_refgroup, eng->d_fl_glabels, eng->d_flstart,
+            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_glabels,
+            eng->d_flstart,
             eng->d_fllen,
             eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
             eng->d_ft_rsq);
@@ -2242,6 +2252,12 @@ int ftd_launch_corr(csg_engine* eng, int grid)
     k_ftd_thin<<<(unsigned)grid, 256, d.thin, eng->stream>>>(
         cnt, 0, eng->d_fl_thin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
         eng->d_ft_rsq, eng->d_ft_scores);
+    if (ftd_fuse(eng))
+        k_ftd_fused<<<(unsigned)eng->ftd_grid_rank, 256, d.rank_ref_w, eng->stream>>>(
+            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_glabels,
+            eng->d_flstart, eng->d_fllen, eng->d_fl_ggenes, eng->d_fref_codes,
+            std::max<uint32_t>(eng->Gf, 1), eng->fnbins, d.wave_ints, eng->d_ft_cr, eng->d_ft_csq,
+            eng->d_ft_scores);
     LAUNCHCHK();
     return CSG_OK;
 }
@@ -2501,7 +2517,7 @@ std::vector<uintptr_t> ftd_sig_a(const csg_engine* eng)
     return { (uintptr_t)eng->d_fl_htab, (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes,
              (uintptr_t)eng->d_fl_glabels, (uintptr_t)eng->d_fl_gseg0, (uintptr_t)eng->ftd_T,
              (uintptr_t)eng->ftd_grid_setup, (uintptr_t)eng->ftd_thin_max,
-             (uintptr_t)eng->ftd_t64,
+             (uintptr_t)eng->ftd_t64, (uintptr_t)ftd_fuse(eng),
              (uintptr_t)eng->ft_exact, (uintptr_t)eng->ftd_grid_threads };
 }
 
@@ -2517,7 +2533,8 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->d_fl_rthin, (uintptr_t)eng->d_fl_rseg, (uintptr_t)eng->d_fl_pred,
              (uintptr_t)eng->d_fl_nan1, (uintptr_t)eng->d_fl_nan2, (uintptr_t)eng->d_fpct,
              (uintptr_t)eng->d_fl_segcell, (uintptr_t)eng->d_fl_refgroup,
-             (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)q,
+             (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)ftd_fuse(eng),
+             (uintptr_t)q,
              (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,
              (uintptr_t)eng->ftd_grid_corr, (uintptr_t)eng->ftd_grid_select,
              (uintptr_t)eng->ftd_thin_max, (uintptr_t)eng->ftd_t64,
@@ -2676,6 +2693,7 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
         eng->counters[7] += h.n_active;
         eng->counters[8] += h.n_groups;
         eng->counters[9] += (int64_t)h.n_scores;
+        eng->counters[22] += h.n_fused;
         if (ftd_bytes(h, L) <= eng->ftd_budget)
         {
             if ((rc = ftd_work(eng, h)))
@@ -3080,6 +3098,16 @@ int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches)
         eng->ftd_budget = bytes;
     if (batches)
         *batches = eng->ftd_batches;
+    return CSG_OK;
+}
+
+int csg_set_finetune_fuse(csg_engine* eng, int max_cells)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (max_cells < 0 || max_cells > FTD_FUSE_MAX)
+        return eng->fail(CSG_ERR_ARGUMENT, "fused fine-tune groups hold 0 to 4 cells");
+    eng->ftd_fuse_max = max_cells;
     return CSG_OK;
 }
 

@@ -50,7 +50,7 @@ struct FtdCounts
     uint32_t round;
     uint32_t tot_redo;  // redone cells over all rounds
     uint32_t g_lo, g_hi;  // group range of the current work batch
-    uint32_t pad_;
+    uint32_t n_fused;     // groups of the fused FP32 form
     uint64_t n_ref_items;
     uint64_t ref_elems;
     uint64_t cell_elems;
@@ -63,8 +63,11 @@ struct FtdCounts
 // Height of the big exact fine-tune tile (cells).
 constexpr int FTD_BIG = 32;
 
-// Per-group flags (bit 0: fewer than 20 genes, bit 1: exact form).
-constexpr uint8_t FTD_SHORT = 1, FTD_EXACT = 2;
+// Per-group flags (bit 0: fewer than 20 genes, bit 1: exact form, bit 2:
+// fused FP32 form -- a group of at most FTD_FUSE_MAX cells whose reference
+// rows are ranked and correlated in one pass without storing their ranks).
+constexpr uint8_t FTD_SHORT = 1, FTD_EXACT = 2, FTD_FUSED = 4;
+constexpr int     FTD_FUSE_MAX = 4;
 
 __device__ inline void ftd_key(const uint8_t* __restrict__ list, uint32_t k, uint64_t key[4])
 {
@@ -154,6 +157,7 @@ __global__ void k_ftd_counters(FtdCounts* __restrict__ cnt)
     cnt->round      += 1;
     cnt->g_lo        = 0u;
     cnt->g_hi        = 0xFFFFFFFFu;
+    cnt->n_fused     = 0;
     cnt->n_ref_items = 0;
     cnt->ref_elems   = 0;
     cnt->cell_elems  = 0;
@@ -233,16 +237,18 @@ __global__ void k_ftd_member(FtdCounts* __restrict__ cnt, const uint32_t* __rest
 }
 
 // One block per group: ascending labels, their row offsets, G_sub.
-// `exact_ok`: the exact form is wanted (groups of at most 1919 genes use it).
+// `exact_ok`: the exact form is wanted (groups of at most 1919 genes use it);
+// groups of at most `fuse_max` cells take the fused FP32 form instead.
 constexpr int FTD_SETUP_NT = 1024;
 
 __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
-    const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ gowner,
+    FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ gowner,
     const uint32_t* __restrict__ gcount, const uint8_t* __restrict__ lists,
     const uint32_t* __restrict__ counts, uint32_t L, const uint32_t* __restrict__ ctdidx,
     const uint32_t* __restrict__ ctdiff, const uint32_t* __restrict__ cut_tab,
     const uint32_t* __restrict__ llen, uint32_t Gf, uint32_t gstride, int exact_ok,
-    FtGroup* __restrict__ groups, uint8_t* __restrict__ gflag, uint32_t* __restrict__ glabels,
+    uint32_t fuse_max, FtGroup* __restrict__ groups, uint8_t* __restrict__ gflag,
+    uint32_t* __restrict__ glabels,
     uint32_t* __restrict__ glpos, uint32_t* __restrict__ ggenes)
 {
     extern __shared__ uint32_t sbits[];  // ceil(Gf / 32) words
@@ -338,8 +344,11 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
             groups[g]   = G;
             // fewer than 20 genes: the round returns {front}
             // (src/cusingler.cu:1158-1159)
-            gflag[g] = (uint8_t)((gs < 20 ? FTD_SHORT : 0) |
-                                 (exact_ok && gs <= 1919 ? FTD_EXACT : 0));
+            const bool fused = G.ncell <= fuse_max;
+            gflag[g]         = (uint8_t)((gs < 20 ? FTD_SHORT : 0) | (fused ? FTD_FUSED : 0) |
+                                 (!fused && exact_ok && gs <= 1919 ? FTD_EXACT : 0));
+            if (fused && gs >= 20)
+                atomicAdd(&cnt->n_fused, 1u);
         }
     }
 }
@@ -369,7 +378,9 @@ __host__ __device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, ui
     z.cell_elems      = nc * G.stride;
     z.scores          = nc * nr;
     z.segs            = nc * G.nlabel;
-    if (flag & FTD_EXACT)
+    if (flag & FTD_FUSED)
+        z.ref_elems = 0;  // the fused pass stores no reference ranks and no tiles
+    else if (flag & FTD_EXACT)
     {
         const uint64_t tc = nc >= t64 ? FTD_BIG : 16;
         z.tiles           = ((nc + tc - 1) / tc) * ((nr + 255) / 256);
@@ -545,7 +556,7 @@ __global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __
         if (f & FTD_SHORT)
             continue;
         const FtGroup G = groups[g];
-        if (G.ncell == 0)
+        if (G.ncell == 0 || (f & FTD_FUSED))
             continue;
         if (f & FTD_EXACT)
         {
@@ -660,9 +671,10 @@ __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict
     __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
 }
 
+// Reference rows of every group except the fused ones.
 __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
-    const uint32_t* __restrict__ refgroup,
+    const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
     const uint32_t* __restrict__ labels, const uint32_t* __restrict__ lstart,
     const uint32_t* __restrict__ llen, const uint32_t* __restrict__ genes,
     const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
@@ -674,8 +686,124 @@ __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     uint16_t*      codes = (uint16_t*)(bins + ((nbins + 3) & ~3));
     const uint32_t n     = (uint32_t)cnt->n_ref_items;
     for (uint32_t item = blockIdx.x * 4 + wid; item < n; item += gridDim.x * 4)
+    {
+        if (gflag[refgroup[item]] & FTD_FUSED)
+            continue;  // uniform per wave
         ft_rank_ref_wave(item, groups, refgroup, labels, lstart, llen, genes, ref_codes, Gf, nbins,
                          out, out_sq, bins, codes, lane);
+    }
+}
+
+// Fused FP32 form, one wave per reference row of a group of at most
+// FTD_FUSE_MAX cells: the row is ranked over G_sub as in ft_rank_ref_wave
+// (LDS code histogram, wave scan), and instead of storing its ranks the wave
+// forms, in the same pass and the same 128-chunk + stride-tree FP32 order as
+// the thin tile (lane l sums chunks 2l and 2l + 1 sequentially by fma, six
+// shuffle levels complete the tree), the row's sum of squares and its dot
+// product with each of the group's cell rank rows, and stores the group's
+// scores corr_from_sums(sxy, sxx, syy) -- the same values the thin tile
+// computes from stored ranks.
+__global__ __launch_bounds__(256) void k_ftd_fused(
+    const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+    const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
+    const uint32_t* __restrict__ labels, const uint32_t* __restrict__ lstart,
+    const uint32_t* __restrict__ llen, const uint32_t* __restrict__ genes,
+    const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
+    const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
+    float* __restrict__ scores)
+{
+    extern __shared__ int smem[];
+    const int      wid   = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int*           bins  = smem + wid * wave_ints;
+    uint16_t*      codes = (uint16_t*)(bins + ((nbins + 3) & ~3));
+    const uint32_t n     = (uint32_t)cnt->n_ref_items;
+    for (uint32_t item = blockIdx.x * 4 + wid; item < n; item += gridDim.x * 4)
+    {
+        const uint32_t gi = refgroup[item];
+        if (!(gflag[gi] & FTD_FUSED))
+            continue;  // uniform per wave
+        const FtGroup g       = groups[gi];
+        uint32_t      j       = item - g.ref_item0;
+        uint32_t      src_row = 0;
+        for (uint32_t t = 0; t < g.nlabel; ++t)
+        {
+            const uint32_t l = labels[g.label_off + t];
+            if (j < llen[l])
+            {
+                src_row = lstart[l] + j;
+                break;
+            }
+            j -= llen[l];
+        }
+        const uint32_t  local = item - g.ref_item0;
+        const int       gs    = (int)g.gs;
+        const uint16_t* src   = ref_codes + (size_t)src_row * Gf;
+        const uint32_t* gl    = genes + g.gene_off;
+        for (int i = lane; i < nbins; i += 64)
+            bins[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < gs; i += 64)
+        {
+            const uint16_t c = src[gl[i]];
+            codes[i]         = c;
+            atomicAdd(&bins[c], 1);
+        }
+        __builtin_amdgcn_wave_barrier();
+        // code -> 2 * (#less) + (#equal) - gs
+        const int per = (nbins + 63) / 64;
+        const int b0 = min(lane * per, nbins), b1 = min(b0 + per, nbins);
+        int       sc = 0;
+        for (int i = b0; i < b1; ++i)
+            sc += bins[i];
+        int start = wave_incl_scan(sc) - sc;
+        for (int i = b0; i < b1; ++i)
+        {
+            const int c = bins[i];
+            bins[i]     = 2 * start + c - gs;
+            start += c;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int      nc   = (int)min(g.ncell, (uint32_t)FTD_FUSE_MAX);
+        const int      st   = (int)g.stride;
+        const int16_t* xc   = cell_rank + g.cell_dst0;
+        const int      step = (gs + 127) / 128;
+        float          q[2] = { 0.f, 0.f };
+        float          d[2][FTD_FUSE_MAX];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+        {
+#pragma unroll
+            for (int c = 0; c < FTD_FUSE_MAX; ++c)
+                d[h][c] = 0.f;
+            const int lo = (2 * lane + h) * step, hi = min(lo + step, gs);
+            for (int i = lo; i < hi; ++i)
+            {
+                const float y = (float)bins[codes[i]];
+                q[h]          = __builtin_fmaf(y, y, q[h]);
+#pragma unroll
+                for (int c = 0; c < FTD_FUSE_MAX; ++c)
+                    if (c < nc)
+                        d[h][c] = __builtin_fmaf((float)xc[c * st + i], y, d[h][c]);
+            }
+        }
+        float syy = q[0] + q[1];
+        for (int off = 1; off < 64; off <<= 1)
+            syy = syy + __shfl_down(syy, off, 64);
+        syy = __shfl(syy, 0, 64);
+#pragma unroll
+        for (int c = 0; c < FTD_FUSE_MAX; ++c)
+        {
+            if (c >= nc)
+                break;
+            float sxy = d[0][c] + d[1][c];
+            for (int off = 1; off < 64; off <<= 1)
+                sxy = sxy + __shfl_down(sxy, off, 64);
+            if (lane == 0)
+                scores[g.score0 + (uint64_t)c * g.nref + local] =
+                    corr_from_sums(sxy, cell_sq[g.cell_item0 + c], syy);
+        }
+        __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
+    }
 }
 
 __global__ __launch_bounds__(128) void k_ftd_rank_ref(

@@ -26,9 +26,9 @@ COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells",
             "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes", "ft_redone_cells",
             "ft_host_lists_us", "ft_host_group_us", "ft_host_build_us", "ft_batch_us",
             "prop_cells", "prop_fallback_margin", "prop_fallback_check", "prop_scored_pairs",
-            "prop_widened", "prop_correlations"]
+            "prop_widened", "prop_correlations", "ft_fused_groups"]
 N_PHASES = 10
-N_COUNTERS = 22
+N_COUNTERS = 23
 
 _V = C.c_void_p
 _U32 = C.c_uint32
@@ -77,6 +77,7 @@ def lib():
     L.csg_score_matrix.argtypes = [_V, C.c_int, _U32, _U32, _V]
     L.csg_finetune_tuning.argtypes = [_V, C.c_int, _V, _V]
     L.csg_set_finetune_budget.argtypes = [_V, _U64, _V]
+    L.csg_set_finetune_fuse.argtypes = [_V, C.c_int]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
     L.csg_selftest.argtypes = [_V, C.c_int, P(C.c_double)]
@@ -269,6 +270,10 @@ class Engine:
         b = C.c_int64()
         self._check(lib().csg_set_finetune_budget(self._h, nbytes, C.byref(b)), "set_finetune_budget")
         return b.value
+
+    def set_finetune_fuse(self, max_cells: int):
+        """Largest fine-tune group (cells) of the fused FP32 form (0 = off)."""
+        self._check(lib().csg_set_finetune_fuse(self._h, int(max_cells)), "set_finetune_fuse")
 
     def finetune_tuning(self, autotune: Optional[bool] = None) -> Dict[str, object]:
         """Autotuner state of the device-resident fine-tune (optionally switching

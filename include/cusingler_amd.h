@@ -118,6 +118,14 @@ int csg_set_finetune_path(csg_engine* eng, int path);
  * Labels and round counts are the same.  No counterpart in the reference. */
 int csg_set_finetune_schedule(csg_engine* eng, int schedule);
 
+/* Largest fine-tune group (in cells) that the device-resident schedule
+ * correlates in the fused FP32 form: one pass per reference row ranks the row
+ * over the round's gene subset and forms its scores against the group's cells
+ * in the reference's 128-chunk + stride-tree order, without storing the
+ * row's ranks (default 4; 0 turns the fused form off; at most 4).  Labels
+ * and round counts do not depend on it.  No counterpart in the reference. */
+int csg_set_finetune_fuse(csg_engine* eng, int max_cells);
+
 /* Autotuner of the device-resident fine-tune: on the first round with enough
  * work after the fine-tune inputs change, the block counts of the reference
  * rank, correlation and select work-queue kernels and the exact correlation
@@ -164,8 +172,9 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
  * to scoring on every label, [19] (cell, label) pairs the proposal path
  * scored, [20] cells widened to their second group of ranked labels,
  * [21] correlations the proposal path computed (sum over its pairs of the
- * label's reference cells). */
-#define CSG_N_COUNTERS 22
+ * label's reference cells), [22] fine-tune groups of the device-resident
+ * schedule that took the fused FP32 form (csg_set_finetune_fuse). */
+#define CSG_N_COUNTERS 23
 int csg_counters(csg_engine* eng, int64_t* out);
 
 /* ---- learned label proposal (no counterpart in the reference) ----

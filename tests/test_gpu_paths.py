@@ -234,3 +234,39 @@ def test_device_finetune_batches_oversized_rounds(eng):
         eng.set_finetune_path("plain")
     eng.destroy()
     eng.destroy_score()
+
+
+def test_device_finetune_fused_form(eng):
+    """Groups of at most four cells take the fused FP32 form (ranks and
+    correlations in one pass, no stored reference ranks): labels and round
+    counts equal those with the fused form off and the host-grouped
+    schedule's, in FP32 and exact form."""
+    from cusingler_amd import engine, synth
+    ds = synth.generate("small")
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    eng.copyin_score(c0)
+    eng.get_label(0.8, 0.3, "plain", outputs=False)  # wide threshold: many small groups
+    eng.copyin(c1)
+    try:
+        for path in ("plain", "exact_int8"):
+            eng.set_finetune_path(path)
+            eng.set_finetune_schedule("host")
+            ref_final, ref_rounds = eng.finetune(0.8, 0.3, 0)
+            eng.set_finetune_schedule("device")
+            out = {}
+            for fuse in (0, 1, 4):
+                eng.set_finetune_fuse(fuse)
+                before = eng.counters()["ft_fused_groups"]  # counters add up over calls
+                final, rounds = eng.finetune(0.8, 0.3, 0)
+                out[fuse] = (final, rounds, eng.counters()["ft_fused_groups"] - before)
+                _say("fused fine-tune", path, "max cells", fuse, "fused groups", out[fuse][2])
+                assert np.array_equal(final, ref_final), (path, fuse)
+                assert np.array_equal(rounds, ref_rounds), (path, fuse)
+            assert out[0][2] == 0 and out[4][2] >= out[1][2] > 0
+    finally:
+        eng.set_finetune_fuse(4)
+        eng.set_finetune_schedule("device")
+        eng.set_finetune_path("plain")
+    eng.destroy()
+    eng.destroy_score()
