@@ -170,8 +170,18 @@ struct csg_engine
     } fl_cap{};
     uint32_t               ftd_gcap = 0, ftd_T = 0;
     size_t                 ftd_gstride = 0;
-    hipGraphExec_t         ftd_gA = nullptr, ftd_gB = nullptr;
-    std::vector<uintptr_t> ftd_sigA, ftd_sigB;
+    hipGraphExec_t         ftd_gA = nullptr;
+    std::vector<uintptr_t> ftd_sigA;
+    // part B graphs, one per launch mask (which work-queue kernels a round
+    // needs, from its counts)
+    struct FtdGraphB
+    {
+        uint32_t               mask = 0;
+        hipGraphExec_t         exec = nullptr;
+        std::vector<uintptr_t> sig;
+    };
+    std::vector<FtdGraphB> ftd_gB;
+    uint32_t               ftd_mask = 0xFFFFFFFFu;  // launch mask of the current round
     // launch settings of the work-queue kernels (blocks), tunable
     int   ftd_grid_threads = 1024, ftd_grid_setup = 512, ftd_grid_rank = 2048,
         ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_thin_max = 8, ftd_t64 = 48;
@@ -2009,11 +2019,12 @@ int ftd_free(csg_engine* eng)
 {
     if (eng->ftd_gA)
         (void)hipGraphExecDestroy(eng->ftd_gA);
-    if (eng->ftd_gB)
-        (void)hipGraphExecDestroy(eng->ftd_gB);
-    eng->ftd_gA = eng->ftd_gB = nullptr;
+    for (auto& b : eng->ftd_gB)
+        if (b.exec)
+            (void)hipGraphExecDestroy(b.exec);
+    eng->ftd_gB.clear();
+    eng->ftd_gA = nullptr;
     eng->ftd_sigA.clear();
-    eng->ftd_sigB.clear();
     dfree(eng->d_fl_lists);
     dfree(eng->d_fl_counts);
     dfree(eng->d_fl_rounds);
@@ -2179,6 +2190,22 @@ uint32_t ftd_fuse(const csg_engine* eng)
     return ftd_lds(eng).rank_ref_w <= 65536 ? (uint32_t)eng->ftd_fuse_max : 0u;
 }
 
+// Launch mask of a round: the work-queue kernels whose work lists can be
+// non-empty, from the round's totals (a superset for every group batch of
+// the round).  Kernels outside the mask are not launched; each mask has its
+// own part B graph.
+constexpr uint32_t FTDM_RANK = 1, FTDM_MFMA = 2, FTDM_MFMA64 = 4, FTDM_CORR = 8, FTDM_THIN = 16,
+                   FTDM_FUSED = 32, FTDM_REDO = 64, FTDM_ALL = 127;
+
+uint32_t ftd_mask_of(const csg_engine* eng, const FtdCounts& h)
+{
+    const uint32_t ftiles = h.n_tiles - h.n_etiles - h.n_btiles;
+    return (h.ref_elems ? FTDM_RANK : 0u) | (h.n_etiles ? FTDM_MFMA : 0u) |
+           (h.n_btiles ? FTDM_MFMA64 : 0u) | (ftiles ? FTDM_CORR : 0u) |
+           (h.n_thin ? FTDM_THIN : 0u) | (h.n_fused && ftd_fuse(eng) ? FTDM_FUSED : 0u) |
+           (h.n_etiles || h.n_btiles ? FTDM_REDO : 0u);
+}
+
 // Round part A: grouping, group set-up and the prefix sums.
 int ftd_launch_a(csg_engine* eng)
 {
@@ -2238,21 +2265,26 @@ int ftd_launch_rank_ref(csg_engine* eng, int grid)
 
 int ftd_launch_corr(csg_engine* eng, int grid)
 {
-    const FtdLds d   = ftd_lds(eng);
-    FtdCounts*   cnt = eng->d_fl_cnt;
-    k_ftd_mfma<<<(unsigned)grid, 256, 0, eng->stream>>>(
-        cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
-        eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
-    k_ftd_mfma64<<<(unsigned)grid, 256, 0, eng->stream>>>(
-        cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
-        eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
-    k_ftd_corr<<<(unsigned)grid, FT_NT, d.corr, eng->stream>>>(
-        cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
-        eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
-    k_ftd_thin<<<(unsigned)grid, 256, d.thin, eng->stream>>>(
-        cnt, 0, eng->d_fl_thin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
-        eng->d_ft_rsq, eng->d_ft_scores);
-    if (ftd_fuse(eng))
+    const FtdLds   d   = ftd_lds(eng);
+    FtdCounts*     cnt = eng->d_fl_cnt;
+    const uint32_t m   = eng->ftd_mask;
+    if (m & FTDM_MFMA)
+        k_ftd_mfma<<<(unsigned)grid, 256, 0, eng->stream>>>(
+            cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
+            eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
+    if (m & FTDM_MFMA64)
+        k_ftd_mfma64<<<(unsigned)grid, 256, 0, eng->stream>>>(
+            cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
+            eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
+    if (m & FTDM_CORR)
+        k_ftd_corr<<<(unsigned)grid, FT_NT, d.corr, eng->stream>>>(
+            cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
+            eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
+    if (m & FTDM_THIN)
+        k_ftd_thin<<<(unsigned)grid, 256, d.thin, eng->stream>>>(
+            cnt, 0, eng->d_fl_thin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
+            eng->d_ft_rsq, eng->d_ft_scores);
+    if ((m & FTDM_FUSED) && ftd_fuse(eng))
         k_ftd_fused<<<(unsigned)eng->ftd_grid_rank, 256, d.rank_ref_w, eng->stream>>>(
             cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_glabels,
             eng->d_flstart, eng->d_fllen, eng->d_fl_ggenes, eng->d_fref_codes,
@@ -2292,7 +2324,7 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
                                                 eng->d_fl_segcell);
     k_ftd_refmap<<<(unsigned)eng->ftd_grid_setup, 256, 0, eng->stream>>>(
         cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup);
-    if (ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
+    if ((eng->ftd_mask & FTDM_RANK) && ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
         return CSG_ERR_DEVICE;
     k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
         cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
@@ -2312,6 +2344,9 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
                                                  eng->d_fl_lists, eng->d_fl_counts, L, thre,
                                                  eng->d_fl_stage, eng->d_fl_redo,
                                                  eng->d_fl_rthin, eng->d_fl_rseg);
+    LAUNCHCHK();
+    if (!(eng->ftd_mask & FTDM_REDO))
+        return CSG_OK;
     // FP32 redo of the undecided exact-form cells
     k_ftd_thin<<<(unsigned)eng->ftd_grid_corr, 256, d.thin, eng->stream>>>(
         cnt, 1, eng->d_fl_rthin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
@@ -2534,11 +2569,31 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->d_fl_nan1, (uintptr_t)eng->d_fl_nan2, (uintptr_t)eng->d_fpct,
              (uintptr_t)eng->d_fl_segcell, (uintptr_t)eng->d_fl_refgroup,
              (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)ftd_fuse(eng),
-             (uintptr_t)q,
+             (uintptr_t)eng->ftd_mask, (uintptr_t)q,
              (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,
              (uintptr_t)eng->ftd_grid_corr, (uintptr_t)eng->ftd_grid_select,
              (uintptr_t)eng->ftd_thin_max, (uintptr_t)eng->ftd_t64,
              (uintptr_t)eng->ftd_grid_threads };
+}
+
+// The part B graph of the current launch mask (captured on first use and
+// re-captured when a pointer or setting in its signature changed).
+int ftd_graph_b(csg_engine* eng, float quantile, float thre, int times, hipGraphExec_t* out)
+{
+    csg_engine::FtdGraphB* b = nullptr;
+    for (auto& e : eng->ftd_gB)
+        if (e.mask == eng->ftd_mask)
+            b = &e;
+    if (!b)
+    {
+        eng->ftd_gB.emplace_back();
+        b       = &eng->ftd_gB.back();
+        b->mask = eng->ftd_mask;
+    }
+    const int rc = ftd_graph(eng, &b->exec, &b->sig, ftd_sig_b(eng, quantile, thre, times),
+                             [&]() { return ftd_launch_b(eng, quantile, thre, times); });
+    *out = b->exec;
+    return rc;
 }
 
 // A round whose work tables exceed the budget: its groups are processed in
@@ -2599,8 +2654,8 @@ int ftd_batches(csg_engine* eng, const FtdCounts& h, float quantile, float thre,
     int rc;
     if ((rc = ftd_work(eng, cap)))
         return rc;
-    if ((rc = ftd_graph(eng, &eng->ftd_gB, &eng->ftd_sigB, ftd_sig_b(eng, quantile, thre, times),
-                        [&]() { return ftd_launch_b(eng, quantile, thre, times); })))
+    hipGraphExec_t gB = nullptr;
+    if ((rc = ftd_graph_b(eng, quantile, thre, times, &gB)))
         return rc;
     for (auto& r : ranges)
     {
@@ -2614,7 +2669,7 @@ int ftd_batches(csg_engine* eng, const FtdCounts& h, float quantile, float thre,
                                                        eng->d_fl_gtile0, eng->d_fl_gthin0,
                                                        eng->d_fl_gseg0);
         LAUNCHCHK();
-        HIPCHK(hipGraphLaunch(eng->ftd_gB, eng->stream));
+        HIPCHK(hipGraphLaunch(gB, eng->stream));
         // the range values above are read before the next copy replaces them
         HIPCHK(hipStreamSynchronize(eng->stream));
     }
@@ -2694,6 +2749,7 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
         eng->counters[8] += h.n_groups;
         eng->counters[9] += (int64_t)h.n_scores;
         eng->counters[22] += h.n_fused;
+        eng->ftd_mask = ftd_mask_of(eng, h);
         if (ftd_bytes(h, L) <= eng->ftd_budget)
         {
             if ((rc = ftd_work(eng, h)))
@@ -2701,11 +2757,10 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
             if (!eng->ftd_tuned && eng->ftd_autotune && h.n_cells >= 4096 &&
                 (rc = ftd_tune(eng, quantile)))
                 return rc;
-            if ((rc = ftd_graph(eng, &eng->ftd_gB, &eng->ftd_sigB,
-                                ftd_sig_b(eng, quantile, thre, times),
-                                [&]() { return ftd_launch_b(eng, quantile, thre, times); })))
+            hipGraphExec_t gB = nullptr;
+            if ((rc = ftd_graph_b(eng, quantile, thre, times, &gB)))
                 return rc;
-            HIPCHK(hipGraphLaunch(eng->ftd_gB, eng->stream));
+            HIPCHK(hipGraphLaunch(gB, eng->stream));
         }
         else if ((rc = ftd_batches(eng, h, quantile, thre, times)))
             return rc;

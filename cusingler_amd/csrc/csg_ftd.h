@@ -51,6 +51,9 @@ struct FtdCounts
     uint32_t tot_redo;  // redone cells over all rounds
     uint32_t g_lo, g_hi;  // group range of the current work batch
     uint32_t n_fused;     // groups of the fused FP32 form
+    uint32_t n_etiles;    // exact 16-cell tiles (of n_tiles)
+    uint32_t n_btiles;    // exact FTD_BIG-cell tiles (of n_tiles)
+    uint32_t pad_;
     uint64_t n_ref_items;
     uint64_t ref_elems;
     uint64_t cell_elems;
@@ -361,6 +364,7 @@ constexpr int FTD_SCAN_NT = 512;
 struct FtdSizes
 {
     uint64_t ref_items, cell_items, ref_elems, cell_elems, scores, tiles, thin, segs, rthin;
+    uint64_t etiles, btiles;  // exact 16-cell and FTD_BIG-cell tiles (of tiles)
 };
 
 // Exact-form groups of at least t64 cells use FTD_BIG-cell tiles, the others
@@ -385,6 +389,7 @@ __host__ __device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, ui
         const uint64_t tc = nc >= t64 ? FTD_BIG : 16;
         z.tiles           = ((nc + tc - 1) / tc) * ((nr + 255) / 256);
         z.rthin           = nc * ((nr + 255) / 256);
+        (tc == 16 ? z.etiles : z.btiles) = z.tiles;
     }
     else if (nc <= thin_max)
         z.thin = (nr + 255) / 256;
@@ -401,7 +406,7 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
                                                           uint32_t* __restrict__ gthin0,
                                                           uint64_t* __restrict__ gseg0)
 {
-    constexpr int NQ = 9;
+    constexpr int NQ = 11;
     __shared__ uint64_t part[NQ][FTD_SCAN_NT];
     // groups [lo, hi) of the current batch (all groups unless the host set a
     // range); the batch's work counters start at zero
@@ -425,6 +430,8 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
         acc[6] += z.thin;
         acc[7] += z.segs;
         acc[8] += z.rthin;
+        acc[9] += z.etiles;
+        acc[10] += z.btiles;
         if (!(gflag[g] & FTD_SHORT))
             smax = max(smax, groups[g].stride);
     }
@@ -465,6 +472,8 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
         run[6] += z.thin;
         run[7] += z.segs;
         run[8] += z.rthin;
+        run[9] += z.etiles;
+        run[10] += z.btiles;
     }
     if (smax)
         atomicMax(&cnt->max_stride, smax);
@@ -487,6 +496,8 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
         cnt->n_thin      = (uint32_t)part[6][tid];
         cnt->n_segs      = part[7][tid];
         cnt->rthin_bound = part[8][tid];
+        cnt->n_etiles    = (uint32_t)part[9][tid];
+        cnt->n_btiles    = (uint32_t)part[10][tid];
     }
 }
 
