@@ -161,12 +161,12 @@ struct csg_engine
     uint8_t*    d_fl_stage = nullptr;
     uint32_t*   d_fl_redo  = nullptr;
     uint64_t *  d_fl_rseg = nullptr, *d_fl_nan1 = nullptr, *d_fl_nan2 = nullptr;
-    uint32_t *  d_fl_segcell = nullptr, *d_fl_refgroup = nullptr;
+    uint32_t *  d_fl_segcell = nullptr, *d_fl_refgroup = nullptr, *d_fl_refrow = nullptr;
     float*      d_fl_pred = nullptr;
     struct
     {
         size_t items, cells, stage, redo, tiles, thin, rthin, rseg, pred, nan1, nan2, segcell,
-            refgroup;
+            refgroup, refrow;
     } fl_cap{};
     uint32_t               ftd_gcap = 0, ftd_T = 0;
     size_t                 ftd_gstride = 0;
@@ -2065,6 +2065,7 @@ int ftd_free(csg_engine* eng)
     dfree(eng->d_fl_nan2);
     dfree(eng->d_fl_segcell);
     dfree(eng->d_fl_refgroup);
+    dfree(eng->d_fl_refrow);
     eng->fl_cap = {};
     eng->ftd_gcap = 0;
     eng->ftd_T    = 0;
@@ -2155,7 +2156,8 @@ int ftd_work(csg_engine* eng, const FtdCounts& h)
         (rc = ensure(eng, &eng->d_fl_nan1, &c.nan1, std::max<uint64_t>(h.n_segs, 1))) ||
         (rc = ensure(eng, &eng->d_fl_nan2, &c.nan2, std::max<uint64_t>(h.n_segs, 1))) ||
         (rc = ensure(eng, &eng->d_fl_segcell, &c.segcell, std::max<uint64_t>(h.n_segs, 1))) ||
-        (rc = ensure(eng, &eng->d_fl_refgroup, &c.refgroup, std::max<uint64_t>(h.n_ref_items, 1))))
+        (rc = ensure(eng, &eng->d_fl_refgroup, &c.refgroup, std::max<uint64_t>(h.n_ref_items, 1))) ||
+        (rc = ensure(eng, &eng->d_fl_refrow, &c.refrow, std::max<uint64_t>(h.n_ref_items, 1))))
         return rc;
     return CSG_OK;
 }
@@ -2250,9 +2252,7 @@ int ftd_launch_rank_ref(csg_engine* eng, int grid)
     FtdCounts*     cnt = eng->d_fl_cnt;
     if (d.rank_ref_w <= 65536)
         k_ftd_rank_ref_w<<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
-            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_glabels,
-            eng->d_flstart,
-            eng->d_fllen,
+            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
             eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
             eng->d_ft_rsq);
     else
@@ -2286,8 +2286,8 @@ int ftd_launch_corr(csg_engine* eng, int grid)
             eng->d_ft_rsq, eng->d_ft_scores);
     if ((m & FTDM_FUSED) && ftd_fuse(eng))
         k_ftd_fused<<<(unsigned)eng->ftd_grid_rank, 256, d.rank_ref_w, eng->stream>>>(
-            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_glabels,
-            eng->d_flstart, eng->d_fllen, eng->d_fl_ggenes, eng->d_fref_codes,
+            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+            eng->d_fl_ggenes, eng->d_fref_codes,
             std::max<uint32_t>(eng->Gf, 1), eng->fnbins, d.wave_ints, eng->d_ft_cr, eng->d_ft_csq,
             eng->d_ft_scores);
     LAUNCHCHK();
@@ -2323,7 +2323,8 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
                                                 eng->d_fl_items, eng->d_fl_cells,
                                                 eng->d_fl_segcell);
     k_ftd_refmap<<<(unsigned)eng->ftd_grid_setup, 256, 0, eng->stream>>>(
-        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup);
+        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
+        eng->d_fl_refgroup, eng->d_fl_refrow);
     if ((eng->ftd_mask & FTDM_RANK) && ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
         return CSG_ERR_DEVICE;
     k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
@@ -2428,7 +2429,8 @@ int ftd_tune(csg_engine* eng, float quantile)
                                                 eng->d_fl_items, eng->d_fl_cells,
                                                 eng->d_fl_segcell);
     k_ftd_refmap<<<(unsigned)eng->ftd_grid_setup, 256, 0, eng->stream>>>(
-        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup);
+        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
+        eng->d_fl_refgroup, eng->d_fl_refrow);
     k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
         cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
         eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
@@ -2568,6 +2570,7 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->d_fl_rthin, (uintptr_t)eng->d_fl_rseg, (uintptr_t)eng->d_fl_pred,
              (uintptr_t)eng->d_fl_nan1, (uintptr_t)eng->d_fl_nan2, (uintptr_t)eng->d_fpct,
              (uintptr_t)eng->d_fl_segcell, (uintptr_t)eng->d_fl_refgroup,
+             (uintptr_t)eng->d_fl_refrow,
              (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)ftd_fuse(eng),
              (uintptr_t)eng->ftd_mask, (uintptr_t)q,
              (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,

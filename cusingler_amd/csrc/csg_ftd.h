@@ -538,18 +538,30 @@ __global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* _
     }
 }
 
-// Group of every reference rank item (one block per group).
+// Group and source row of every reference rank item (one block per group;
+// the group's rows are its labels' rows in ascending label order).
 __global__ void k_ftd_refmap(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
-                             const uint8_t* __restrict__ gflag, uint32_t* __restrict__ refgroup)
+                             const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ labels,
+                             const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
+                             uint32_t* __restrict__ refgroup, uint32_t* __restrict__ refrow)
 {
     const uint32_t hi = min(cnt->g_hi, cnt->n_groups);
     for (uint32_t g = cnt->g_lo + blockIdx.x; g < hi; g += gridDim.x)
     {
         if (gflag[g] & FTD_SHORT)
             continue;
-        const FtGroup G = groups[g];
-        for (uint32_t r = threadIdx.x; r < G.nref; r += blockDim.x)
-            refgroup[G.ref_item0 + r] = g;
+        const FtGroup G   = groups[g];
+        uint32_t      off = G.ref_item0;
+        for (uint32_t t = 0; t < G.nlabel; ++t)
+        {
+            const uint32_t l = labels[G.label_off + t], n = llen[l], s0 = lstart[l];
+            for (uint32_t r = threadIdx.x; r < n; r += blockDim.x)
+            {
+                refgroup[off + r] = g;
+                refrow[off + r]   = s0 + r;
+            }
+            off += n;
+        }
     }
 }
 
@@ -597,51 +609,33 @@ __global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __
 // ---- work-queue kernels: the grid covers the machine, items come from the
 // ---- device-side counts
 
-// Reference rows, one wave per row (4 rows per block, no block barrier): the
-// block kernel's steps with a per-wave LDS histogram of `nbins` code bins and
-// the row's codes, the code -> doubled centred rank table from a wave scan,
-// and the 128-chunk + stride-tree sum of squares with lane l holding chunks
-// 2l and 2l+1 (their sum is the tree's first level) and shuffles for the
-// remaining six levels.  Per-wave LDS: `wave_ints` ints.
-__device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict__ groups,
-                                        const uint32_t* __restrict__ refgroup,
-                                        const uint32_t* __restrict__ labels,
-                                        const uint32_t* __restrict__ lstart,
-                                        const uint32_t* __restrict__ llen,
-                                        const uint32_t* __restrict__ genes,
-                                        const uint16_t* __restrict__ ref_codes, uint32_t Gf,
-                                        int nbins, int16_t* __restrict__ out,
-                                        float* __restrict__ out_sq, int* bins, uint16_t* codes,
-                                        int lane)
+// Doubled centred rank table of one reference row over G_sub, built by one
+// wave in LDS: codes[i] = the row's code at gene gl[i] (all gathered first,
+// so that the loads overlap), the code histogram (the zero code -- most of a
+// row -- counted by ballots rather than LDS atomics), and
+// bins[c] = 2 (#less) + (#equal) - gs from a wave scan over the bins.
+__device__ inline void ftd_row_table(const uint16_t* __restrict__ src,
+                                     const uint32_t* __restrict__ gl, int gs, int nbins,
+                                     int* bins, uint16_t* codes, int lane)
 {
-    const FtGroup  g       = groups[refgroup[item]];
-    uint32_t       j       = item - g.ref_item0;
-    uint32_t       src_row = 0;
-    for (uint32_t t = 0; t < g.nlabel; ++t)
-    {
-        const uint32_t l = labels[g.label_off + t];
-        if (j < llen[l])
-        {
-            src_row = lstart[l] + j;
-            break;
-        }
-        j -= llen[l];
-    }
-    const uint32_t  local = item - g.ref_item0;
-    const int       gs    = (int)g.gs;
-    const uint16_t* src   = ref_codes + (size_t)src_row * Gf;
-    const uint32_t* gl    = genes + g.gene_off;
     for (int i = lane; i < nbins; i += 64)
         bins[i] = 0;
-    __builtin_amdgcn_wave_barrier();
+#pragma unroll 4
     for (int i = lane; i < gs; i += 64)
-    {
-        const uint16_t c = src[gl[i]];
-        codes[i]         = c;
-        atomicAdd(&bins[c], 1);
-    }
+        codes[i] = src[gl[i]];
     __builtin_amdgcn_wave_barrier();
-    // code -> 2 * (#less) + (#equal) - gs
+    int zeros = 0;
+    for (int i0 = 0; i0 < gs; i0 += 64)
+    {
+        const int      i = i0 + lane;
+        const uint16_t c = i < gs ? codes[i] : (uint16_t)0;
+        zeros += __builtin_popcountll(__ballot(i < gs && c == 0));
+        if (i < gs && c != 0)
+            atomicAdd(&bins[c], 1);
+    }
+    if (lane == 0)
+        bins[0] = zeros;  // no atomic touched bin 0
+    __builtin_amdgcn_wave_barrier();
     const int per = (nbins + 63) / 64;
     const int b0 = min(lane * per, nbins), b1 = min(b0 + per, nbins);
     int       s  = 0;
@@ -655,6 +649,27 @@ __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict
         start += c;
     }
     __builtin_amdgcn_wave_barrier();
+}
+
+// Reference rows, one wave per row (4 rows per block, no block barrier): the
+// rank table of ftd_row_table (per-wave LDS of `nbins` code bins and the
+// row's codes, `wave_ints` ints), the ranks stored, and the 128-chunk +
+// stride-tree sum of squares with lane l holding chunks 2l and 2l+1 (their
+// sum is the tree's first level) and shuffles for the remaining six levels.
+__device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict__ groups,
+                                        const uint32_t* __restrict__ refgroup,
+                                        const uint32_t* __restrict__ refrow,
+                                        const uint32_t* __restrict__ genes,
+                                        const uint16_t* __restrict__ ref_codes, uint32_t Gf,
+                                        int nbins, int16_t* __restrict__ out,
+                                        float* __restrict__ out_sq, int* bins, uint16_t* codes,
+                                        int lane)
+{
+    const FtGroup  g     = groups[refgroup[item]];
+    const uint32_t local = item - g.ref_item0;
+    const int      gs    = (int)g.gs;
+    ftd_row_table(ref_codes + (size_t)refrow[item] * Gf, genes + g.gene_off, gs, nbins, bins,
+                  codes, lane);
     int16_t* dst = out + g.ref_dst0 + (uint64_t)local * g.stride;
     for (int i = lane; i < (int)g.stride; i += 64)
         dst[i] = i < gs ? (int16_t)bins[codes[i]] : (int16_t)0;
@@ -686,8 +701,7 @@ __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict
 __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
     const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
-    const uint32_t* __restrict__ labels, const uint32_t* __restrict__ lstart,
-    const uint32_t* __restrict__ llen, const uint32_t* __restrict__ genes,
+    const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
     const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
     int16_t* __restrict__ out, float* __restrict__ out_sq)
 {
@@ -700,8 +714,8 @@ __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     {
         if (gflag[refgroup[item]] & FTD_FUSED)
             continue;  // uniform per wave
-        ft_rank_ref_wave(item, groups, refgroup, labels, lstart, llen, genes, ref_codes, Gf, nbins,
-                         out, out_sq, bins, codes, lane);
+        ft_rank_ref_wave(item, groups, refgroup, refrow, genes, ref_codes, Gf, nbins, out,
+                         out_sq, bins, codes, lane);
     }
 }
 
@@ -717,8 +731,7 @@ __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
 __global__ __launch_bounds__(256) void k_ftd_fused(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
     const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
-    const uint32_t* __restrict__ labels, const uint32_t* __restrict__ lstart,
-    const uint32_t* __restrict__ llen, const uint32_t* __restrict__ genes,
+    const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
     const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
     const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
     float* __restrict__ scores)
@@ -733,47 +746,11 @@ __global__ __launch_bounds__(256) void k_ftd_fused(
         const uint32_t gi = refgroup[item];
         if (!(gflag[gi] & FTD_FUSED))
             continue;  // uniform per wave
-        const FtGroup g       = groups[gi];
-        uint32_t      j       = item - g.ref_item0;
-        uint32_t      src_row = 0;
-        for (uint32_t t = 0; t < g.nlabel; ++t)
-        {
-            const uint32_t l = labels[g.label_off + t];
-            if (j < llen[l])
-            {
-                src_row = lstart[l] + j;
-                break;
-            }
-            j -= llen[l];
-        }
-        const uint32_t  local = item - g.ref_item0;
-        const int       gs    = (int)g.gs;
-        const uint16_t* src   = ref_codes + (size_t)src_row * Gf;
-        const uint32_t* gl    = genes + g.gene_off;
-        for (int i = lane; i < nbins; i += 64)
-            bins[i] = 0;
-        __builtin_amdgcn_wave_barrier();
-        for (int i = lane; i < gs; i += 64)
-        {
-            const uint16_t c = src[gl[i]];
-            codes[i]         = c;
-            atomicAdd(&bins[c], 1);
-        }
-        __builtin_amdgcn_wave_barrier();
-        // code -> 2 * (#less) + (#equal) - gs
-        const int per = (nbins + 63) / 64;
-        const int b0 = min(lane * per, nbins), b1 = min(b0 + per, nbins);
-        int       sc = 0;
-        for (int i = b0; i < b1; ++i)
-            sc += bins[i];
-        int start = wave_incl_scan(sc) - sc;
-        for (int i = b0; i < b1; ++i)
-        {
-            const int c = bins[i];
-            bins[i]     = 2 * start + c - gs;
-            start += c;
-        }
-        __builtin_amdgcn_wave_barrier();
+        const FtGroup  g     = groups[gi];
+        const uint32_t local = item - g.ref_item0;
+        const int      gs    = (int)g.gs;
+        ftd_row_table(ref_codes + (size_t)refrow[item] * Gf, genes + g.gene_off, gs, nbins, bins,
+                      codes, lane);
         const int      nc   = (int)min(g.ncell, (uint32_t)FTD_FUSE_MAX);
         const int      st   = (int)g.stride;
         const int16_t* xc   = cell_rank + g.cell_dst0;
