@@ -102,8 +102,13 @@ struct csg_engine
     RecheckItem* d_items = nullptr;
     unsigned int* d_nitems = nullptr;
     float*    d_Pref = nullptr;
-    uint64_t* d_item_off = nullptr;
     Segment*  d_item_seg = nullptr;
+    // re-scored pairs grouped by label for the grouped FP32 contraction
+    uint32_t * d_rc_cells = nullptr, *d_rc_abase = nullptr, *d_rc_acnt = nullptr;
+    uint64_t*  d_rc_coff  = nullptr;
+    GroupTile* d_rc_tiles = nullptr;
+    size_t     rc_cap_seg = 0, rc_cap_cells = 0, rc_cap_l = 0, rc_cap_l2 = 0,
+           rc_cap_coff = 0, rc_cap_tiles = 0;
 
     // ---- fine-tune ----
     bool      ft_loaded = false;
@@ -295,6 +300,17 @@ template <class T> int ensure(csg_engine* eng, T** p, size_t* cap, size_t n)
     int          rc   = dalloc(eng, p, want);
     *cap              = rc ? 0 : want;
     return rc;
+}
+
+// Grow-only upload: `count` elements into *p (capacity *cap).
+template <class T> int upload_keep(csg_engine* eng, T** p, size_t* cap, const T* src, size_t count)
+{
+    int rc = ensure(eng, p, cap, std::max<size_t>(count, 1));
+    if (rc)
+        return rc;
+    if (count)
+        HIPCHK(hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, eng->stream));
+    return CSG_OK;
 }
 
 void free_proposal(csg_engine* eng);
@@ -577,8 +593,14 @@ int csg_destroy_score(csg_engine* eng)
     dfree(eng->d_items);
     dfree(eng->d_nitems);
     dfree(eng->d_Pref);
-    dfree(eng->d_item_off);
     dfree(eng->d_item_seg);
+    dfree(eng->d_rc_cells);
+    dfree(eng->d_rc_abase);
+    dfree(eng->d_rc_acnt);
+    dfree(eng->d_rc_coff);
+    dfree(eng->d_rc_tiles);
+    eng->rc_cap_seg = eng->rc_cap_cells = eng->rc_cap_l = eng->rc_cap_l2 =
+        eng->rc_cap_coff = eng->rc_cap_tiles = 0;
     free_proposal(eng);
     eng->ref_prepared_path = -1;
     eng->pct_quantile = -1.f;
@@ -844,38 +866,65 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
         return CSG_OK;
     std::vector<RecheckItem> items(n);
     HIPCHK(hipMemcpy(items.data(), eng->d_items, n * sizeof(RecheckItem), hipMemcpyDeviceToHost));
-    std::vector<uint64_t> off(n);
-    std::vector<Segment>  segs(n);
-    std::vector<uint8_t>  cell_seen(nq, 0);
-    uint64_t              tot = 0;
+    // group the pairs by label: label l's cells are listed from a_base[l], and
+    // a cell's scores against the label's rows follow each other from
+    // c_off[l] on, so that every pair's scores are one contiguous segment
+    std::vector<uint32_t> acnt(L, 0), abase(L, 0);
+    std::vector<uint64_t> coff(L, 0);
+    for (unsigned int i = 0; i < n; ++i)
+        acnt[items[i].l]++;
+    uint64_t tot = 0;
+    for (uint32_t l = 0, a = 0; l < L; ++l)
+    {
+        abase[l] = a;
+        coff[l]  = tot;
+        a += acnt[l];
+        tot += (uint64_t)acnt[l] * eng->ctidx[2 * l + 1];
+    }
+    if (tot > (uint64_t)eng->Qc * eng->Nr)
+        return eng->fail(CSG_ERR_MEMORY, "re-scored pairs exceed the score buffer");
+    std::vector<uint32_t>    fill(L, 0), cells(n);
+    std::vector<RecheckItem> sorted(n);
+    std::vector<Segment>     segs(n);
+    std::vector<uint8_t>     cell_seen(nq, 0);
     for (unsigned int i = 0; i < n; ++i)
     {
-        const uint32_t len = eng->ctidx[2 * items[i].l + 1];
-        off[i]             = tot;
-        segs[i]            = Segment{ tot, len, items[i].l };
-        tot += len;
-        if (!cell_seen[items[i].q])
+        const RecheckItem it  = items[i];
+        const uint32_t    len = eng->ctidx[2 * it.l + 1];
+        const uint32_t    p   = fill[it.l]++;
+        cells[abase[it.l] + p]  = it.q;
+        sorted[abase[it.l] + p] = it;
+        segs[abase[it.l] + p]   = Segment{ coff[it.l] + (uint64_t)p * len, len, it.l };
+        if (!cell_seen[it.q])
         {
-            cell_seen[items[i].q] = 1;
+            cell_seen[it.q] = 1;
             eng->counters[4]++;
         }
     }
     eng->counters[3] += n;
-    if (tot > (uint64_t)eng->Qc * eng->Nr)
-        return eng->fail(CSG_ERR_MEMORY, "re-scored pairs exceed the score buffer");
+    std::vector<GroupTile> tiles;
+    for (uint32_t l = 0; l < L; ++l)
+        for (uint32_t m0 = 0; m0 < acnt[l]; m0 += PG_BM)
+            for (uint32_t n0 = 0; n0 < eng->ctidx[2 * l + 1]; n0 += PG_BN)
+                tiles.push_back(GroupTile{ l, m0, n0, 0u });
     int rc;
-    if ((rc = upload(eng, &eng->d_item_off, off.data(), n)) ||
-        (rc = upload(eng, &eng->d_item_seg, segs.data(), n)))
+    // the label-sorted pairs replace the certification's list in place (n is
+    // at most the list's capacity, nq * L)
+    HIPCHK(hipMemcpyAsync(eng->d_items, sorted.data(), n * sizeof(RecheckItem),
+                          hipMemcpyHostToDevice, eng->stream));
+    if ((rc = upload_keep(eng, &eng->d_item_seg, &eng->rc_cap_seg, segs.data(), n)) ||
+        (rc = upload_keep(eng, &eng->d_rc_cells, &eng->rc_cap_cells, cells.data(), n)) ||
+        (rc = upload_keep(eng, &eng->d_rc_abase, &eng->rc_cap_l, abase.data(), L)) ||
+        (rc = upload_keep(eng, &eng->d_rc_acnt, &eng->rc_cap_l2, acnt.data(), L)) ||
+        (rc = upload_keep(eng, &eng->d_rc_coff, &eng->rc_cap_coff, coff.data(), L)) ||
+        (rc = upload_keep(eng, &eng->d_rc_tiles, &eng->rc_cap_tiles, tiles.data(), tiles.size())))
         return rc;
     {
         PhaseScope ps(eng, 4, 3);
-        uint32_t max_len = 1;
-        for (uint32_t l = 0; l < eng->L; ++l)
-            max_len = std::max(max_len, eng->ctidx[2 * l + 1]);
-        k_rescore<<<dim3(n, (max_len + 63) / 64), 64, 0, eng->stream>>>(
-            eng->d_items, eng->d_item_off, eng->d_Xq, eng->d_Xr,
-                                             (int)eng->Kp, eng->d_sqq_f, eng->d_sqr_f,
-                                             eng->d_lstart, eng->d_llen, eng->d_S);
+        k_gemm_plain_group<<<(unsigned)tiles.size(), 256, 0, eng->stream>>>(
+            eng->d_rc_tiles, eng->d_rc_cells, eng->d_rc_abase, eng->d_rc_acnt, eng->d_lstart,
+            eng->d_llen, eng->d_rc_coff, eng->d_Xq, eng->d_Xr, (int)eng->Kp,
+            (uint32_t)round_up(eng->Nr, 256) - 1, eng->d_sqq_f, eng->d_sqr_f, eng->d_S);
         LAUNCHCHK();
         HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
         TableSegments src{ eng->d_item_seg };

@@ -123,68 +123,6 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
     }
 }
 
-// FP32 scores of one query row against every reference row of one label in
-// the reference's sequential order (the plain contraction restricted to a
-// (cell, label) pair).  One wave per (item, 64 reference rows), one row per
-// lane; each 64 x 64 tile of ranks is staged with 16-byte loads into LDS
-// rows of 33 words (an odd pitch, so the lanes' column reads do not collide).
-__global__ __launch_bounds__(64) void k_rescore(const RecheckItem* __restrict__ items,
-                                                const uint64_t* __restrict__ out_off,
-                                                const int16_t* __restrict__ Xq,
-                                                const int16_t* __restrict__ Xr, int Kp,
-                                                const float* __restrict__ sq_q,
-                                                const float* __restrict__ sq_r,
-                                                const uint32_t* __restrict__ lstart,
-                                                const uint32_t* __restrict__ llen,
-                                                float* __restrict__ out)
-{
-    __shared__ int     tile[64 * 33];
-    __shared__ int16_t xs[64];
-    const RecheckItem it   = items[blockIdx.x];
-    const int         lane = threadIdx.x;
-    const uint32_t    r0   = lstart[it.l];
-    const uint32_t    n    = llen[it.l];
-    const int16_t*    xq   = Xq + (size_t)it.q * Kp;
-    const float       sq   = sq_q[it.q];
-    float*            o    = out + out_off[blockIdx.x];
-    const int16_t*    t16  = (const int16_t*)tile;
-    // block (item, y) takes reference rows [64 y, 64 y + 64) of the label
-    const uint32_t    base = blockIdx.y * 64;
-    if (base >= n)
-        return;  // uniform over the block
-    {
-        const uint32_t nrow = min(64u, n - base);
-        float          acc  = 0.f;
-        for (int k0 = 0; k0 < Kp; k0 += 64)
-        {
-            __syncthreads();
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-            {
-                const int c = lane + 64 * t, row = c >> 3, ch = c & 7;
-                int4      v = int4{ 0, 0, 0, 0 };
-                if ((uint32_t)row < nrow)
-                    v = *(const int4*)(Xr + (size_t)(r0 + base + row) * Kp + k0 + ch * 8);
-                int* d = tile + row * 33 + ch * 4;
-                d[0]   = v.x;
-                d[1]   = v.y;
-                d[2]   = v.z;
-                d[3]   = v.w;
-            }
-            xs[lane] = xq[k0 + lane];
-            __syncthreads();
-            if ((uint32_t)lane < nrow)
-            {
-#pragma unroll 16
-                for (int k = 0; k < 64; ++k)
-                    acc = __builtin_fmaf((float)xs[k], (float)t16[lane * 66 + k], acc);
-            }
-        }
-        if ((uint32_t)lane < nrow)
-            o[base + lane] = corr_from_sums(acc, sq, sq_r[r0 + base + lane]);
-    }
-}
-
 // Writes the re-scored quantiles into P and redoes the decision of every cell
 // that needed them (status 1 or 2).
 __global__ void k_apply_recheck(const RecheckItem* __restrict__ items, unsigned int n,

@@ -223,22 +223,59 @@ __global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ da
 // S[m, n] = corr(Xq[m], Xr[n]) with sum_k Xq[m,k]*Xr[n,k] accumulated by fma in
 // k order 0..Kp-1 (zero padding past G adds exact zeros), i.e. the reference's
 // sequential loop (src/cusingler.cu:931-938).  128x128 tile per 256-thread
-// block, 8x8 outputs per thread, K staged through LDS 16 at a time.
+// block, 8x8 outputs per thread, K staged through LDS 16 at a time.  The row
+// map gives the operand rows of the tile (a_row / b_row, always inside the
+// operand buffers), which results are stored (row_ok / col_ok), the FP32 sums
+// of squares and the output rows.
 constexpr int PG_BM = 128, PG_BN = 128, PG_BK = 16;
 
-__global__ __launch_bounds__(256) void k_gemm_plain(const int16_t* __restrict__ Xq,
-                                                    const int16_t* __restrict__ Xr, int Kp,
-                                                    const float* __restrict__ sq_q,
-                                                    const float* __restrict__ sq_r, int nq, int nr,
-                                                    float* __restrict__ S, size_t ldS)
+struct DensePMap
+{
+    int          m0, n0, nq, nr;
+    const float* sq_q;
+    const float* sq_r;
+    float*       S;
+    size_t       ldS;
+    __device__ size_t a_row(int r) const { return (size_t)(m0 + r); }
+    __device__ size_t b_row(int r) const { return (size_t)(n0 + r); }
+    __device__ bool   row_ok(int m) const { return m0 + m < nq; }
+    __device__ float  row_sq(int m) const { return sq_q[m0 + m]; }
+    __device__ float* row_out(int m) const { return S + (size_t)(m0 + m) * ldS + n0; }
+    __device__ bool   col_ok(int n) const { return n0 + n < nr; }
+    __device__ float  col_sq(int n) const { return sq_r[n0 + n]; }
+};
+
+// One label of a grouped contraction: the query rows listed in cells[0, mcount)
+// against the reference rows b0 .. b0 + ncount - 1; row m of the result is
+// stored contiguously at out + m * ld.  Rows past the list repeat its last
+// row, reference rows past b_max read row b_max.
+struct GroupPMap
+{
+    const uint32_t* cells;
+    int             mcount, ncount;
+    uint32_t        b0, b_max;
+    const float*    sq_q;
+    const float*    sq_r;
+    float*          out;
+    size_t          ld;
+    __device__ size_t a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
+    __device__ size_t b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
+    __device__ bool   row_ok(int m) const { return m < mcount; }
+    __device__ float  row_sq(int m) const { return sq_q[cells[m]]; }
+    __device__ float* row_out(int m) const { return out + (size_t)m * ld; }
+    __device__ bool   col_ok(int n) const { return n < ncount; }
+    __device__ float  col_sq(int n) const { return sq_r[b0 + n]; }
+};
+
+template <class Map>
+__device__ inline void plain_block(const int16_t* __restrict__ Xq, const int16_t* __restrict__ Xr,
+                                   int Kp, const Map& map)
 {
     __shared__ float As[2][PG_BK][PG_BM + 4];
     __shared__ float Bs[2][PG_BK][PG_BN + 4];
     const int tid = threadIdx.x;
     const int tx  = tid & 15;
     const int ty  = tid >> 4;
-    const int m0  = blockIdx.y * PG_BM;
-    const int n0  = blockIdx.x * PG_BN;
 
     float acc[8][8];
 #pragma unroll
@@ -249,8 +286,8 @@ __global__ __launch_bounds__(256) void k_gemm_plain(const int16_t* __restrict__ 
 
     const int lr = tid >> 1;        // tile row loaded by this thread
     const int lk = (tid & 1) * 8;   // first of 8 k values
-    const int16_t* qa = Xq + (size_t)(m0 + lr) * Kp + lk;
-    const int16_t* rb = Xr + (size_t)(n0 + lr) * Kp + lk;
+    const int16_t* qa = Xq + map.a_row(lr) * (size_t)Kp + lk;
+    const int16_t* rb = Xr + map.b_row(lr) * (size_t)Kp + lk;
 
     auto stage = [&](int buf, int4 av, int4 bv) {
         const int16_t* a = (const int16_t*)&av;
@@ -302,19 +339,63 @@ __global__ __launch_bounds__(256) void k_gemm_plain(const int16_t* __restrict__ 
 #pragma unroll
     for (int i = 0; i < 8; ++i)
     {
-        const int m = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4));
-        if (m >= nq)
+        const int m = i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4);
+        if (!map.row_ok(m))
             continue;
-        const float sq = sq_q[m];
-        float*      out = S + (size_t)m * ldS;
+        const float sq  = map.row_sq(m);
+        float*      out = map.row_out(m);
 #pragma unroll
         for (int j = 0; j < 8; ++j)
         {
-            const int n = n0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4));
-            if (n < nr)
-                out[n] = corr_from_sums(acc[i][j], sq, sq_r[n]);
+            const int n = j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4);
+            if (map.col_ok(n))
+                out[n] = corr_from_sums(acc[i][j], sq, map.col_sq(n));
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_gemm_plain(const int16_t* __restrict__ Xq,
+                                                    const int16_t* __restrict__ Xr, int Kp,
+                                                    const float* __restrict__ sq_q,
+                                                    const float* __restrict__ sq_r, int nq, int nr,
+                                                    float* __restrict__ S, size_t ldS)
+{
+    const DensePMap map{ (int)blockIdx.y * PG_BM, (int)blockIdx.x * PG_BN, nq, nr, sq_q, sq_r, S,
+                         ldS };
+    plain_block(Xq, Xr, Kp, map);
+}
+
+// Tile (label, first cell, first reference row) of a grouped contraction.
+struct GroupTile
+{
+    uint32_t label, m0, n0, pad_;
+};
+
+// Grouped FP32 contraction in the reference's order: block b takes tile
+// tiles[b] = (label l, m0, n0), the cells cells[a_base[l] + m0 ..] (a_cnt[l]
+// in all) against reference rows lstart[l] + n0 .. of label l (llen[l] in
+// all); each cell's scores against the label's rows are stored contiguously,
+// label l's block starting at S + c_off[l].
+__global__ __launch_bounds__(256) void k_gemm_plain_group(
+    const GroupTile* __restrict__ tiles, const uint32_t* __restrict__ cells,
+    const uint32_t* __restrict__ a_base, const uint32_t* __restrict__ a_cnt,
+    const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
+    const uint64_t* __restrict__ c_off, const int16_t* __restrict__ Xq,
+    const int16_t* __restrict__ Xr, int Kp, uint32_t b_max, const float* __restrict__ sq_q,
+    const float* __restrict__ sq_r, float* __restrict__ S)
+{
+    const GroupTile t  = tiles[blockIdx.x];
+    const uint32_t  nl = llen[t.label];
+    const GroupPMap map{ cells + a_base[t.label] + t.m0,
+                         (int)(a_cnt[t.label] - t.m0),
+                         (int)(nl - t.n0),
+                         lstart[t.label] + t.n0,
+                         b_max,
+                         sq_q,
+                         sq_r,
+                         S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
+                         nl };
+    plain_block(Xq, Xr, Kp, map);
 }
 
 // ------------------------------------------------ per-label quantile select ---

@@ -442,11 +442,6 @@ __global__ __launch_bounds__(512) void k_gemm_int8w(
 // against the label's rows are stored contiguously, label l's block starting
 // at S + c_off[l].  Tiles are listed label by label, so a label's tiles run
 // on one XCD.
-struct GroupTile
-{
-    uint32_t label, m0, n0, pad_;
-};
-
 __global__ __launch_bounds__(512) void k_gemm_int8w_group(
     const GroupTile* __restrict__ tiles, const uint32_t* __restrict__ cells,
     const uint32_t* __restrict__ a_base, const uint32_t* __restrict__ a_cnt,
