@@ -104,10 +104,11 @@ struct csg_engine
     float*    d_Pref = nullptr;
     Segment*  d_item_seg = nullptr;
     // re-scored pairs grouped by label for the grouped FP32 contraction
-    uint32_t * d_rc_cells = nullptr, *d_rc_abase = nullptr, *d_rc_acnt = nullptr;
+    uint32_t * d_rc_cells = nullptr, *d_rc_abase = nullptr, *d_rc_acnt = nullptr,
+             *d_rc_ucells = nullptr;
     uint64_t*  d_rc_coff  = nullptr;
     GroupTile* d_rc_tiles = nullptr;
-    size_t     rc_cap_seg = 0, rc_cap_cells = 0, rc_cap_l = 0, rc_cap_l2 = 0,
+    size_t     rc_cap_seg = 0, rc_cap_ucells = 0, rc_cap_cells = 0, rc_cap_l = 0, rc_cap_l2 = 0,
            rc_cap_coff = 0, rc_cap_tiles = 0;
 
     // ---- fine-tune ----
@@ -599,6 +600,8 @@ int csg_destroy_score(csg_engine* eng)
     dfree(eng->d_rc_acnt);
     dfree(eng->d_rc_coff);
     dfree(eng->d_rc_tiles);
+    dfree(eng->d_rc_ucells);
+    eng->rc_cap_ucells = 0;
     eng->rc_cap_seg = eng->rc_cap_cells = eng->rc_cap_l = eng->rc_cap_l2 =
         eng->rc_cap_coff = eng->rc_cap_tiles = 0;
     free_proposal(eng);
@@ -759,12 +762,30 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
 
 namespace {
 
+// Score-step ranks of rows [row0, row0 + nrows) (k_rank_csr_w, one wave per
+// row, when its per-wave LDS fits 16 KiB; else one block per row).  sq_f
+// (the FP32 sums of squares) and feat (the network features) may be null.
 int rank_rows(csg_engine* eng, const uint16_t* data, const int* ind, const int* ptr, uint32_t row0,
-              uint32_t nrows, int16_t* X, float* sq_f, double* ir)
+              uint32_t nrows, int16_t* X, float* sq_f, double* ir, __bf16* feat = nullptr)
 {
-    const size_t smem = rank_smem(eng->nbins, eng->Kp);
-    k_rank_csr<256><<<nrows, 256, smem, eng->stream>>>(data, ind, ptr, row0, nrows, eng->G, eng->Kp,
-                                                       eng->nbins, X, sq_f, ir);
+    if (nrows == 0)
+        return CSG_OK;
+    const float inv_g     = 1.f / (float)std::max<uint32_t>(eng->G, 1);
+    const int   wave_ints = ((eng->nbins + 3) & ~3) + (int)eng->Kp / 2;
+    if ((size_t)wave_ints * 4 * 4 <= 65536)
+    {
+        const unsigned grid = (unsigned)std::min<size_t>((nrows + 3) / 4, 16384);
+        k_rank_csr_w<<<grid, 256, (size_t)wave_ints * 4 * 4, eng->stream>>>(
+            data, ind, ptr, row0, nrows, eng->G, eng->Kp, eng->nbins, wave_ints, X, sq_f, ir,
+            feat, inv_g);
+    }
+    else
+    {
+        const size_t smem = rank_smem(eng->nbins, eng->Kp);
+        k_rank_csr<256><<<nrows, 256, smem, eng->stream>>>(data, ind, ptr, row0, nrows, eng->G,
+                                                           eng->Kp, eng->nbins, X, sq_f, ir, feat,
+                                                           inv_g);
+    }
     LAUNCHCHK();
     return CSG_OK;
 }
@@ -902,6 +923,10 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
         }
     }
     eng->counters[3] += n;
+    std::vector<uint32_t> ucells;  // the re-scored cells, once each
+    for (uint32_t q = 0; q < nq; ++q)
+        if (cell_seen[q])
+            ucells.push_back(q);
     std::vector<GroupTile> tiles;
     for (uint32_t l = 0; l < L; ++l)
         for (uint32_t m0 = 0; m0 < acnt[l]; m0 += PG_BM)
@@ -917,10 +942,15 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
         (rc = upload_keep(eng, &eng->d_rc_abase, &eng->rc_cap_l, abase.data(), L)) ||
         (rc = upload_keep(eng, &eng->d_rc_acnt, &eng->rc_cap_l2, acnt.data(), L)) ||
         (rc = upload_keep(eng, &eng->d_rc_coff, &eng->rc_cap_coff, coff.data(), L)) ||
-        (rc = upload_keep(eng, &eng->d_rc_tiles, &eng->rc_cap_tiles, tiles.data(), tiles.size())))
+        (rc = upload_keep(eng, &eng->d_rc_tiles, &eng->rc_cap_tiles, tiles.data(), tiles.size())) ||
+        (rc = upload_keep(eng, &eng->d_rc_ucells, &eng->rc_cap_ucells, ucells.data(), ucells.size())))
         return rc;
     {
         PhaseScope ps(eng, 4, 3);
+        // the reference's FP32 sums of squares of the re-scored cells (the
+        // exact path's rank pass leaves them out)
+        k_sq_seq<<<(unsigned)((ucells.size() + 255) / 256), 256, 0, eng->stream>>>(
+            eng->d_Xq, eng->Kp, eng->G, eng->d_rc_ucells, (uint32_t)ucells.size(), eng->d_sqq_f);
         k_gemm_plain_group<<<(unsigned)tiles.size(), 256, 0, eng->stream>>>(
             eng->d_rc_tiles, eng->d_rc_cells, eng->d_rc_abase, eng->d_rc_acnt, eng->d_lstart,
             eng->d_llen, eng->d_rc_coff, eng->d_Xq, eng->d_Xr, (int)eng->Kp,
@@ -1104,14 +1134,13 @@ int mlp_check(csg_engine* eng)
 }
 
 // Top-K labels and softmax margin of the `nq` ranked query rows in d_Xq.
+// Proposals of nq query rows whose network features are already in d_mx
+// (rank_rows writes them).
 int mlp_propose(csg_engine* eng, uint32_t nq, int K, int32_t* top, float* margin)
 {
     int rc;
     if ((rc = mlp_work(eng, nq, false)))
         return rc;
-    const size_t n = (size_t)nq * eng->Kp;
-    k_rank_features<<<nblk(n), 256, 0, eng->stream>>>(eng->d_Xq, n, 1.f / (float)eng->G, eng->d_mx);
-    LAUNCHCHK();
     if ((rc = mlp_forward(eng, (int)nq, false)))
         return rc;
     const float* b3 = eng->d_mw + eng->m_off[5];
@@ -1446,8 +1475,13 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
         const uint32_t nq = std::min(eng->Qc, eng->Nq - q0);
         {
             PhaseScope ps(eng, 0, path == CSG_PATH_PLAIN || path == CSG_PATH_FP32_MFMA ? 1 : 2);
+            // the exact path needs the FP32 sums only for re-scored cells
+            // (certify_chunk), the proposal the network features
+            if (prop && (rc = mlp_work(eng, nq, false)))
+                return rc;
             if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
-                                eng->d_sqq_f, eng->d_irq)))
+                                path == CSG_PATH_EXACT_INT8 ? nullptr : eng->d_sqq_f, eng->d_irq,
+                                prop ? eng->d_mx : nullptr)))
                 return rc;
             if ((rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh,
                                        eng->d_ql, eng->d_qf16)))
@@ -3178,7 +3212,7 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
         eng->ref_prepared_path = path;
     }
     if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
-                        eng->d_sqq_f, eng->d_irq)) ||
+                        path == CSG_PATH_EXACT_INT8 ? nullptr : eng->d_sqq_f, eng->d_irq)) ||
         (rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh, eng->d_ql,
                                eng->d_qf16)) ||
         (rc = score_chunk(eng, path, nq)))
@@ -3745,16 +3779,8 @@ int csg_mlp_train(csg_engine* eng, const int32_t* labels, int epochs, int batch,
     for (uint32_t q0 = 0; q0 < N && !rc; q0 += eng->Qc)
     {
         const uint32_t nq = std::min(eng->Qc, N - q0);
-        rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq, eng->d_sqq_f,
-                       eng->d_irq);
-        if (!rc)
-        {
-            const size_t n = (size_t)nq * Kp;
-            k_rank_features<<<nblk(n), 256, 0, eng->stream>>>(eng->d_Xq, n, 1.f / (float)eng->G,
-                                                             feat + (size_t)q0 * Kp);
-            if (hipGetLastError() != hipSuccess)
-                rc = eng->fail(CSG_ERR_DEVICE, "feature launch failed");
-        }
+        rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq, nullptr,
+                       eng->d_irq, feat + (size_t)q0 * Kp);
     }
     std::vector<uint32_t> order(N);
     std::mt19937_64       rng(seed ^ 0x9E3779B97F4A7C15ull);
@@ -3829,8 +3855,9 @@ int csg_mlp_predict(csg_engine* eng, int k, int32_t* top, float* margin)
     for (uint32_t q0 = 0; q0 < eng->Nq && !rc; q0 += eng->Qc)
     {
         const uint32_t nq = std::min(eng->Qc, eng->Nq - q0);
-        if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
-                            eng->d_sqq_f, eng->d_irq)) ||
+        if ((rc = mlp_work(eng, nq, false)) ||
+            (rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
+                            nullptr, eng->d_irq, eng->d_mx)) ||
             (rc = mlp_propose(eng, nq, k, dtop, dmar)))
             break;
         hipError_t he = hipMemcpyAsync(top + (size_t)q0 * k, dtop, (size_t)nq * k * sizeof(int32_t),

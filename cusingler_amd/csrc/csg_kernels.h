@@ -135,10 +135,34 @@ __host__ __device__ inline double inv_root(long long s)
     return 1.0 / sqrt((double)s);
 }
 
+// The reference's FP32 sum of squares of one rank row: `sumxx += x*x` in gene
+// order (src/cusingler.cu:931-938), contracted to fma, by one thread.
+__device__ inline float seq_sum_sq(const int16_t* row, uint32_t G)
+{
+    float    acc = 0.f;
+    uint32_t g   = 0;
+    for (; g + 8 <= G; g += 8)
+    {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            v[u] = (float)row[g + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            acc = __builtin_fmaf(v[u], v[u], acc);
+    }
+    for (; g < G; ++g)
+    {
+        const float v = (float)row[g];
+        acc           = __builtin_fmaf(v, v, acc);
+    }
+    return acc;
+}
+
 // One block per row of a coded CSR matrix over G genes.  Writes the dense
-// doubled centred ranks X[row, 0..Kp) (zero past G), the FP32 sum of X^2 in
-// gene order (the reference's `sumxx += x*x` loop of src/cusingler.cu:931-938,
-// contracted to fma) and the inverse root of the exact int64 sum of X^2.
+// doubled centred ranks X[row, 0..Kp) (zero past G), the inverse root of the
+// exact int64 sum of X^2, and where asked (non-null) the FP32 sum of X^2 of
+// seq_sum_sq and the network features BF16(X * inv_g).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ data,
                                                  const int* __restrict__ indices,
@@ -146,7 +170,8 @@ __global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ da
                                                  uint32_t nrows, uint32_t G, uint32_t Kp,
                                                  int nbins, int16_t* __restrict__ X,
                                                  float* __restrict__ sq_f32,
-                                                 double* __restrict__ ir_exact)
+                                                 double* __restrict__ ir_exact,
+                                                 __bf16* __restrict__ feat, float inv_g)
 {
     extern __shared__ int smem[];
     int*     bins = smem;                          // nbins
@@ -181,6 +206,9 @@ __global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ da
     int16_t* out = X + (size_t)r * Kp;
     for (uint32_t g = threadIdx.x * 8; g < Kp; g += NT * 8)
         *(int4*)(out + g) = *(const int4*)(row + g);
+    if (feat)
+        for (uint32_t g = threadIdx.x; g < Kp; g += NT)
+            feat[(size_t)r * Kp + g] = (__bf16)((float)row[g] * inv_g);
     long long s = 0;
     for (uint32_t g = threadIdx.x; g < G; g += NT)
     {
@@ -197,25 +225,98 @@ __global__ __launch_bounds__(NT) void k_rank_csr(const uint16_t* __restrict__ da
         for (int w = 0; w < NT / kWave; ++w)
             t += red[w];
         ir_exact[r] = inv_root(t);
-        float acc = 0.f;
-        uint32_t g = 0;
-        for (; g + 8 <= G; g += 8)
-        {
-            float v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                v[u] = (float)row[g + u];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                acc = __builtin_fmaf(v[u], v[u], acc);
-        }
-        for (; g < G; ++g)
-        {
-            const float v = (float)row[g];
-            acc           = __builtin_fmaf(v, v, acc);
-        }
-        sq_f32[r] = acc;
+        if (sq_f32)
+            sq_f32[r] = seq_sum_sq(row, G);
     }
+}
+
+// The same outputs with one wave per row (four rows per 256-thread block,
+// rows strided over the grid, no block barrier): per-wave LDS of `wave_ints`
+// ints holds the code bins and the row.
+__global__ __launch_bounds__(256) void k_rank_csr_w(
+    const uint16_t* __restrict__ data, const int* __restrict__ indices,
+    const int* __restrict__ indptr, uint32_t row0, uint32_t nrows, uint32_t G, uint32_t Kp,
+    int nbins, int wave_ints, int16_t* __restrict__ X, float* __restrict__ sq_f32,
+    double* __restrict__ ir_exact, __bf16* __restrict__ feat, float inv_g)
+{
+    extern __shared__ int smem[];
+    const int wid  = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    int*      bins = smem + wid * wave_ints;
+    int16_t*  row  = (int16_t*)(bins + ((nbins + 3) & ~3));
+    for (uint32_t r = blockIdx.x * 4 + wid; r < nrows; r += gridDim.x * 4)
+    {
+        const int beg = indptr[row0 + r];
+        const int end = indptr[row0 + r + 1];
+        const int nnz = end - beg;
+        for (int i = lane; i < nbins; i += 64)
+            bins[i] = 0;
+        __builtin_amdgcn_wave_barrier();
+        for (int i = beg + lane; i < end; i += 64)
+            atomicAdd(&bins[data[i]], 1);
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0)
+            bins[0] = 0;  // code 0 (implicit zeros) is not stored; its count is G - nnz
+        __builtin_amdgcn_wave_barrier();
+        {
+            const int per = (nbins + 63) / 64;
+            const int b0 = min(lane * per, nbins), b1 = min(b0 + per, nbins);
+            int       sc = 0;
+            for (int i = b0; i < b1; ++i)
+                sc += bins[i];
+            int start = wave_incl_scan(sc) - sc + ((int)G - nnz);
+            for (int i = b0; i < b1; ++i)
+            {
+                const int c = bins[i];
+                bins[i]     = 2 * start + c - (int)G;
+                start += c;
+            }
+        }
+        const int16_t x0 = (int16_t)(-nnz);  // 2*0 + (G-nnz) - G
+        for (uint32_t g = lane; g < Kp; g += 64)
+            row[g] = g < G ? x0 : (int16_t)0;
+        __builtin_amdgcn_wave_barrier();
+        for (int i = beg + lane; i < end; i += 64)
+            row[indices[i]] = (int16_t)bins[data[i]];
+        __builtin_amdgcn_wave_barrier();
+        int16_t* out = X + (size_t)r * Kp;
+        for (uint32_t g = lane * 8; g < Kp; g += 64 * 8)
+            *(int4*)(out + g) = *(const int4*)(row + g);
+        if (feat)
+            for (uint32_t g = lane * 8; g < Kp; g += 64 * 8)
+            {
+                const int4     v = *(const int4*)(row + g);
+                const int16_t* x = (const int16_t*)&v;
+                __bf16         h[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    h[u] = (__bf16)((float)x[u] * inv_g);
+                *(int4*)(feat + (size_t)r * Kp + g) = *(const int4*)h;
+            }
+        long long s = 0;
+        for (uint32_t g = lane; g < G; g += 64)
+        {
+            const long long v = row[g];
+            s += v * v;
+        }
+        s = wave_sum64(s);
+        if (lane == 0)
+        {
+            ir_exact[r] = inv_root(s);
+            if (sq_f32)
+                sq_f32[r] = seq_sum_sq(row, G);
+        }
+        __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
+    }
+}
+
+// The reference's FP32 sums of squares of the listed rows of a rank matrix.
+__global__ void k_sq_seq(const int16_t* __restrict__ X, uint32_t Kp, uint32_t G,
+                         const uint32_t* __restrict__ rows, uint32_t n, float* __restrict__ sq_f32)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        sq_f32[rows[i]] = seq_sum_sq(X + (size_t)rows[i] * Kp, G);
 }
 
 // ------------------------------------------- plain FP32 score contraction ---
