@@ -69,6 +69,7 @@ struct csg_engine
     bool      score_loaded = false;
     uint32_t  Nr = 0, Nq = 0, G = 0, Kp = 0, L = 0;
     bool      w8_ok = false;  // the wide INT8 kernels' LDS could be granted
+    bool      bw_ok = false;  // the wide BF16 network kernel's LDS could be granted
     int       nbins = 0;
     std::vector<uint32_t> ctidx;
     uint16_t *d_rdata = nullptr, *d_qdata = nullptr;
@@ -540,6 +541,10 @@ int csg_init_gpu(int gpuid, csg_engine** out)
                  hipFuncSetAttribute((const void*)k_gemm_int8w_group,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      W8_LDS) == hipSuccess;
+    eng->bw_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)BW_LDS &&
+                 hipFuncSetAttribute((const void*)k_bgemm_nt_w<1>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     BW_LDS) == hipSuccess;
     (void)hipGetLastError();
     {
         const int cu          = std::max(1, eng->prop.multiProcessorCount);
@@ -1032,13 +1037,26 @@ int mlp_forward(csg_engine* eng, int rows, bool keep)
     const __bf16* wt = eng->d_mwbT;
     const float*  w  = eng->d_mw;
     const dim3    gh((H + 127) / 128, (rows + 127) / 128), gl((Lp + 127) / 128, (rows + 127) / 128);
-    k_bgemm_nt<1><<<gh, 256, 0, eng->stream>>>(eng->d_mx, Kp, wt + eng->m_off[0], Kp, rows, H, Kp,
-                                                H, nullptr, w + eng->m_off[1],
-                                                keep ? eng->d_mz1 : nullptr, eng->d_mh1);
+    // the hidden layers in the pipelined wide kernel where its shapes and LDS allow
+    const bool     wide = eng->bw_ok && H % BW_BN == 0 && Kp % BW_BK == 0 && H % BW_BK == 0;
+    const unsigned gw   = (unsigned)(((rows + BW_BM - 1) / BW_BM) * (H / BW_BN));
+    if (wide)
+        k_bgemm_nt_w<1><<<gw, 512, BW_LDS, eng->stream>>>(
+            eng->d_mx, Kp, wt + eng->m_off[0], Kp, rows, H, Kp, H, nullptr, w + eng->m_off[1],
+            keep ? eng->d_mz1 : nullptr, eng->d_mh1);
+    else
+        k_bgemm_nt<1><<<gh, 256, 0, eng->stream>>>(eng->d_mx, Kp, wt + eng->m_off[0], Kp, rows, H,
+                                                    Kp, H, nullptr, w + eng->m_off[1],
+                                                    keep ? eng->d_mz1 : nullptr, eng->d_mh1);
     LAUNCHCHK();
-    k_bgemm_nt<1><<<gh, 256, 0, eng->stream>>>(eng->d_mh1, H, wt + eng->m_off[2], H, rows, H, H, H,
-                                                nullptr, w + eng->m_off[3],
-                                                keep ? eng->d_mz2 : nullptr, eng->d_mh2);
+    if (wide)
+        k_bgemm_nt_w<1><<<gw, 512, BW_LDS, eng->stream>>>(
+            eng->d_mh1, H, wt + eng->m_off[2], H, rows, H, H, H, nullptr, w + eng->m_off[3],
+            keep ? eng->d_mz2 : nullptr, eng->d_mh2);
+    else
+        k_bgemm_nt<1><<<gh, 256, 0, eng->stream>>>(eng->d_mh1, H, wt + eng->m_off[2], H, rows, H,
+                                                    H, H, nullptr, w + eng->m_off[3],
+                                                    keep ? eng->d_mz2 : nullptr, eng->d_mh2);
     LAUNCHCHK();
     k_bgemm_nt<0><<<gl, 256, 0, eng->stream>>>(eng->d_mh2, H, wt + eng->m_off[4], H, rows, Lp, H,
                                                 Lp, eng->d_macc, nullptr, nullptr, nullptr);

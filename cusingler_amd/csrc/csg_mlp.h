@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "csg_mfma.h"
+
 namespace csg {
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -228,6 +230,129 @@ __global__ __launch_bounds__(256) void k_bgemm_nt(const __bf16* __restrict__ A, 
                 const int n = n0 + wn * 64 + j * 16 + r16;
                 if (n >= N)
                     continue;
+                const size_t o = (size_t)m * ldc + n;
+                if constexpr (MODE == 0)
+                    C[o] = acc[i][j][e];
+                else
+                {
+                    const float v = acc[i][j][e] + bias[n];
+                    if (z)
+                        z[o] = v;
+                    H[o] = (__bf16)gelu_f(v);
+                }
+            }
+        }
+}
+
+// The same product for wide layers (N a multiple of 256, K of 64): block tile
+// 128 x 256 (eight waves of 64 x 64), K staged 64 at a time through three LDS
+// stages filled by direct-to-LDS loads, two stages in flight (the INT8
+// contraction's pipeline, csg_mfma.h).  A stage holds [A 128 rows][B 256
+// rows] of 64 BF16 (128 bytes); the 16-byte slot of row r holding K chunk c is
+// c ^ bw_swz(r).  Within a stage the two 32-wide K halves are accumulated in
+// order, so every sum equals k_bgemm_nt's bit for bit.  Tiles run in XCD
+// order, the two column tiles of a row tile next to each other.
+constexpr int BW_BM = 128, BW_BN = 256, BW_BK = 64, BW_NS = 3;
+constexpr int BW_STAGE = (BW_BM + BW_BN) * BW_BK * 2;  // 48 KB
+constexpr int BW_LDS   = BW_NS * BW_STAGE;
+
+__device__ __forceinline__ int bw_swz(int r) { return (r ^ (r >> 3)) & 7; }
+
+template <int MODE>
+__global__ __launch_bounds__(512) void k_bgemm_nt_w(const __bf16* __restrict__ A, int lda,
+                                                    const __bf16* __restrict__ B, int ldb, int M,
+                                                    int N, int K, int ldc, float* __restrict__ C,
+                                                    const float* __restrict__ bias,
+                                                    float* __restrict__ z, __bf16* __restrict__ H)
+{
+    extern __shared__ __attribute__((aligned(16))) int8_t bw_sm[];
+    const int      tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int      wm = wid >> 2, wn = wid & 3;
+    const uint32_t nct = (uint32_t)(N / BW_BN);
+    const uint32_t t   = xcd_order(blockIdx.x, gridDim.x);
+    const int      m0 = (int)(t / nct) * BW_BM, n0 = (int)(t % nct) * BW_BN;
+    // this lane's source (element offset, swizzled chunk) for its six row
+    // groups g = wid + 8 u of 8 rows: groups 0-15 A rows, 16-47 B rows
+    size_t src[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+    {
+        const int g = wid + 8 * u;
+        const int r = (g < 16 ? g : g - 16) * 8 + (lane >> 3);
+        const int c = (lane & 7) ^ bw_swz(r);
+        src[u]      = g < 16 ? (size_t)min(m0 + r, M - 1) * lda + c * 8
+                             : (size_t)min(n0 + r, N - 1) * ldb + c * 8;
+    }
+    auto issue = [&](int8_t* stage, int k0) {
+#pragma unroll
+        for (int u = 0; u < 6; ++u)
+        {
+            const int     g    = wid + 8 * u;
+            const __bf16* base = g < 16 ? A : B;
+            __builtin_amdgcn_global_load_lds(
+                base + src[u] + k0, (__attribute__((address_space(3))) void*)(stage + g * 1024), 16,
+                0, 0);
+        }
+    };
+    v4f_m acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            acc[i][j] = v4f_m{ 0.f, 0.f, 0.f, 0.f };
+    const int T = K / BW_BK;
+    issue(bw_sm, 0);
+    if (T > 1)
+        issue(bw_sm + BW_STAGE, BW_BK);
+    const int r16 = lane & 15, kq = lane >> 4;
+    for (int s = 0; s < T; ++s)
+    {
+        // stage s is complete when at most the six loads of stage s + 1 are
+        // outstanding; the barrier publishes every wave's loads and ends every
+        // wave's reads of stage s - 1, whose buffer stage s + 2 reuses
+        if (s + 1 < T)
+            __builtin_amdgcn_s_waitcnt(0x0F76);  // vmcnt <= 6
+        else
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt 0
+        asm volatile("s_barrier" ::: "memory");
+        if (s + 2 < T)
+            issue(bw_sm + ((s + 2) % BW_NS) * BW_STAGE, (s + 2) * BW_BK);
+        const int8_t* st = bw_sm + (s % BW_NS) * BW_STAGE;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+        {
+            v8bf a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+            {
+                const int row = wm * 64 + i * 16 + r16;
+                a[i] = *(const v8bf*)(st + row * 128 + (((kq + 4 * h) ^ bw_swz(row)) << 4));
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int row = wn * 64 + j * 16 + r16;
+                b[j] = *(const v8bf*)(st + (BW_BM + row) * 128 + (((kq + 4 * h) ^ bw_swz(row)) << 4));
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + e;
+            if (m >= M)
+                continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int    n = n0 + wn * 64 + j * 16 + r16;
                 const size_t o = (size_t)m * ldc + n;
                 if constexpr (MODE == 0)
                     C[o] = acc[i][j][e];
