@@ -934,7 +934,7 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
             ucells.push_back(q);
     std::vector<GroupTile> tiles;
     for (uint32_t l = 0; l < L; ++l)
-        for (uint32_t m0 = 0; m0 < acnt[l]; m0 += PG_BM)
+        for (uint32_t m0 = 0; m0 < acnt[l]; m0 += PG_GBM)
             for (uint32_t n0 = 0; n0 < eng->ctidx[2 * l + 1]; n0 += PG_BN)
                 tiles.push_back(GroupTile{ l, m0, n0, 0u });
     int rc;

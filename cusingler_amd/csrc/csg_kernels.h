@@ -368,26 +368,29 @@ struct GroupPMap
     __device__ float  col_sq(int n) const { return sq_r[b0 + n]; }
 };
 
-template <class Map>
+// BM = 128 or 64 query rows per tile (MI = BM / 16 rows per thread).
+template <class Map, int BM = PG_BM>
 __device__ inline void plain_block(const int16_t* __restrict__ Xq, const int16_t* __restrict__ Xr,
                                    int Kp, const Map& map)
 {
-    __shared__ float As[2][PG_BK][PG_BM + 4];
+    constexpr int MI = BM / 16;
+    __shared__ float As[2][PG_BK][BM + 4];
     __shared__ float Bs[2][PG_BK][PG_BN + 4];
     const int tid = threadIdx.x;
     const int tx  = tid & 15;
     const int ty  = tid >> 4;
 
-    float acc[8][8];
+    float acc[MI][8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j)
             acc[i][j] = 0.f;
 
-    const int lr = tid >> 1;        // tile row loaded by this thread
-    const int lk = (tid & 1) * 8;   // first of 8 k values
-    const int16_t* qa = Xq + map.a_row(lr) * (size_t)Kp + lk;
+    const int  lr   = tid >> 1;        // tile row loaded by this thread
+    const int  lk   = (tid & 1) * 8;   // first of 8 k values
+    const bool a_ld = lr < BM;         // the thread also stages a query row
+    const int16_t* qa = Xq + map.a_row(a_ld ? lr : 0) * (size_t)Kp + lk;
     const int16_t* rb = Xr + map.b_row(lr) * (size_t)Kp + lk;
 
     auto stage = [&](int buf, int4 av, int4 bv) {
@@ -396,12 +399,13 @@ __device__ inline void plain_block(const int16_t* __restrict__ Xq, const int16_t
 #pragma unroll
         for (int u = 0; u < 8; ++u)
         {
-            As[buf][lk + u][lr] = (float)a[u];
+            if (a_ld)
+                As[buf][lk + u][lr] = (float)a[u];
             Bs[buf][lk + u][lr] = (float)b[u];
         }
     };
 
-    int4 av = *(const int4*)qa;
+    int4 av = a_ld ? *(const int4*)qa : int4{ 0, 0, 0, 0 };
     int4 bv = *(const int4*)rb;
     stage(0, av, bv);
     __syncthreads();
@@ -411,20 +415,28 @@ __device__ inline void plain_block(const int16_t* __restrict__ Xq, const int16_t
         const bool more = k0 + PG_BK < Kp;
         if (more)
         {
-            av = *(const int4*)(qa + k0 + PG_BK);
+            if (a_ld)
+                av = *(const int4*)(qa + k0 + PG_BK);
             bv = *(const int4*)(rb + k0 + PG_BK);
         }
-#pragma unroll
+#pragma unroll 4
         for (int kk = 0; kk < PG_BK; ++kk)
         {
-            const float4 a0 = *(const float4*)&As[buf][kk][ty * 4];
-            const float4 a1 = *(const float4*)&As[buf][kk][64 + ty * 4];
+            float a[MI];
+            {
+                const float4 a0 = *(const float4*)&As[buf][kk][ty * 4];
+                a[0] = a0.x, a[1] = a0.y, a[2] = a0.z, a[3] = a0.w;
+                if constexpr (MI == 8)
+                {
+                    const float4 a1 = *(const float4*)&As[buf][kk][64 + ty * 4];
+                    a[4] = a1.x, a[5] = a1.y, a[6] = a1.z, a[7] = a1.w;
+                }
+            }
             const float4 b0 = *(const float4*)&Bs[buf][kk][tx * 4];
             const float4 b1 = *(const float4*)&Bs[buf][kk][64 + tx * 4];
-            const float  a[8] = { a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w };
             const float  b[8] = { b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w };
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int i = 0; i < MI; ++i)
 #pragma unroll
                 for (int j = 0; j < 8; ++j)
                     acc[i][j] = __builtin_fmaf(a[i], b[j], acc[i][j]);
@@ -438,7 +450,7 @@ __device__ inline void plain_block(const int16_t* __restrict__ Xq, const int16_t
     }
 
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < MI; ++i)
     {
         const int m = i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4);
         if (!map.row_ok(m))
@@ -466,14 +478,17 @@ __global__ __launch_bounds__(256) void k_gemm_plain(const int16_t* __restrict__ 
     plain_block(Xq, Xr, Kp, map);
 }
 
+// Query rows per tile of the grouped FP32 contraction (its lists are short).
+constexpr int PG_GBM = 64;
+
 // Tile (label, first cell, first reference row) of a grouped contraction.
 struct GroupTile
 {
     uint32_t label, m0, n0, pad_;
 };
 
-// Grouped FP32 contraction in the reference's order: block b takes tile
-// tiles[b] = (label l, m0, n0), the cells cells[a_base[l] + m0 ..] (a_cnt[l]
+// Grouped FP32 contraction in the reference's order (PG_GBM x PG_BN tiles):
+// block b takes tile tiles[b] = (label l, m0, n0), the cells cells[a_base[l] + m0 ..] (a_cnt[l]
 // in all) against reference rows lstart[l] + n0 .. of label l (llen[l] in
 // all); each cell's scores against the label's rows are stored contiguously,
 // label l's block starting at S + c_off[l].
@@ -496,7 +511,7 @@ __global__ __launch_bounds__(256) void k_gemm_plain_group(
                          sq_r,
                          S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
                          nl };
-    plain_block(Xq, Xr, Kp, map);
+    plain_block<GroupPMap, PG_GBM>(Xq, Xr, Kp, map);
 }
 
 // ------------------------------------------------ per-label quantile select ---
