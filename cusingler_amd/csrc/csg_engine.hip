@@ -119,6 +119,7 @@ struct csg_engine
     std::vector<uint32_t> fctidx, ctdidx, ctdiff;
     std::vector<float>    host_mask;  // empty: use the device mask
     uint16_t* d_fref_codes = nullptr;
+    uint8_t*  d_fref_codes8 = nullptr;  // the same codes in bytes, when all are below 256
     uint32_t *d_flstart = nullptr, *d_fllen = nullptr;
     uint16_t *d_fqdata = nullptr;
     int *d_fqind = nullptr, *d_fqptr = nullptr;
@@ -624,6 +625,7 @@ int csg_destroy(csg_engine* eng)
     (void)hipSetDevice(eng->device);
     (void)hipStreamSynchronize(eng->stream);
     dfree(eng->d_fref_codes);
+    dfree(eng->d_fref_codes8);
     dfree(eng->d_flstart);
     dfree(eng->d_fllen);
     dfree(eng->d_fqdata);
@@ -1566,19 +1568,22 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
 
 namespace {
 
+extern "C++" {
+template <class CT>
 __global__ void k_densify(const uint16_t* __restrict__ data, const int* __restrict__ ind,
                           const int* __restrict__ ptr, uint32_t nrows, uint32_t width,
-                          uint16_t* __restrict__ out)
+                          CT* __restrict__ out)
 {
     const uint32_t r = blockIdx.x;
     if (r >= nrows)
         return;
-    uint16_t* o = out + (size_t)r * width;
+    CT* o = out + (size_t)r * width;
     for (uint32_t g = threadIdx.x; g < width; g += blockDim.x)
         o[g] = 0;
     __syncthreads();
     for (int i = ptr[r] + (int)threadIdx.x; i < ptr[r + 1]; i += blockDim.x)
-        o[ind[i]] = data[i];
+        o[ind[i]] = (CT)data[i];
+}
 }
 
 }  // namespace
@@ -1651,8 +1656,21 @@ int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint1
         dfree(d_rp);
         return rc;
     }
-    k_densify<<<ref_height, 256, 0, eng->stream>>>(d_rd, d_ri, d_rp, ref_height, w,
-                                                   eng->d_fref_codes);
+    k_densify<uint16_t><<<ref_height, 256, 0, eng->stream>>>(d_rd, d_ri, d_rp, ref_height, w,
+                                                             eng->d_fref_codes);
+    dfree(eng->d_fref_codes8);
+    if (eng->fnbins <= 256)
+    {
+        if ((rc = dalloc(eng, &eng->d_fref_codes8, (size_t)ref_height * w)))
+        {
+            dfree(d_rd);
+            dfree(d_ri);
+            dfree(d_rp);
+            return rc;
+        }
+        k_densify<uint8_t><<<ref_height, 256, 0, eng->stream>>>(d_rd, d_ri, d_rp, ref_height, w,
+                                                                eng->d_fref_codes8);
+    }
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(eng->stream);
     dfree(d_rd);
@@ -2352,10 +2370,18 @@ int ftd_launch_rank_ref(csg_engine* eng, int grid)
     const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1);
     FtdCounts*     cnt = eng->d_fl_cnt;
     if (d.rank_ref_w <= 65536)
-        k_ftd_rank_ref_w<<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
-            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-            eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
-            eng->d_ft_rsq);
+    {
+        if (eng->d_fref_codes8)
+            k_ftd_rank_ref_w<uint8_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
+                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+                eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
+                eng->d_ft_rsq);
+        else
+            k_ftd_rank_ref_w<uint16_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
+                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+                eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
+                eng->d_ft_rsq);
+    }
     else
         k_ftd_rank_ref<<<(unsigned)grid, 128, d.rank_ref, eng->stream>>>(
             cnt, eng->d_fl_groups, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
@@ -2386,11 +2412,18 @@ int ftd_launch_corr(csg_engine* eng, int grid)
             cnt, 0, eng->d_fl_thin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
             eng->d_ft_rsq, eng->d_ft_scores);
     if ((m & FTDM_FUSED) && ftd_fuse(eng))
-        k_ftd_fused<<<(unsigned)eng->ftd_grid_rank, 256, d.rank_ref_w, eng->stream>>>(
-            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-            eng->d_fl_ggenes, eng->d_fref_codes,
-            std::max<uint32_t>(eng->Gf, 1), eng->fnbins, d.wave_ints, eng->d_ft_cr, eng->d_ft_csq,
-            eng->d_ft_scores);
+    {
+        if (eng->d_fref_codes8)
+            k_ftd_fused<uint8_t><<<(unsigned)eng->ftd_grid_rank, 256, d.rank_ref_w, eng->stream>>>(
+                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+                eng->d_fl_ggenes, eng->d_fref_codes8, std::max<uint32_t>(eng->Gf, 1), eng->fnbins,
+                d.wave_ints, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_scores);
+        else
+            k_ftd_fused<uint16_t><<<(unsigned)eng->ftd_grid_rank, 256, d.rank_ref_w, eng->stream>>>(
+                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+                eng->d_fl_ggenes, eng->d_fref_codes, std::max<uint32_t>(eng->Gf, 1), eng->fnbins,
+                d.wave_ints, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_scores);
+    }
     LAUNCHCHK();
     return CSG_OK;
 }

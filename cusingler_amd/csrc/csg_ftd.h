@@ -610,13 +610,14 @@ __global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __
 // ---- device-side counts
 
 // Doubled centred rank table of one reference row over G_sub, built by one
-// wave in LDS: codes[i] = the row's code at gene gl[i] (all gathered first,
+// wave in LDS (codes stored as CT: 8 bits when every code is below 256, else
+// 16): codes[i] = the row's code at gene gl[i] (all gathered first,
 // so that the loads overlap), the code histogram (the zero code -- most of a
 // row -- counted by ballots rather than LDS atomics), and
 // bins[c] = 2 (#less) + (#equal) - gs from a wave scan over the bins.
-__device__ inline void ftd_row_table(const uint16_t* __restrict__ src,
-                                     const uint32_t* __restrict__ gl, int gs, int nbins,
-                                     int* bins, uint16_t* codes, int lane)
+template <class CT>
+__device__ inline void ftd_row_table(const CT* __restrict__ src, const uint32_t* __restrict__ gl,
+                                     int gs, int nbins, int* bins, uint16_t* codes, int lane)
 {
     for (int i = lane; i < nbins; i += 64)
         bins[i] = 0;
@@ -656,11 +657,12 @@ __device__ inline void ftd_row_table(const uint16_t* __restrict__ src,
 // row's codes, `wave_ints` ints), the ranks stored, and the 128-chunk +
 // stride-tree sum of squares with lane l holding chunks 2l and 2l+1 (their
 // sum is the tree's first level) and shuffles for the remaining six levels.
+template <class CT>
 __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict__ groups,
                                         const uint32_t* __restrict__ refgroup,
                                         const uint32_t* __restrict__ refrow,
                                         const uint32_t* __restrict__ genes,
-                                        const uint16_t* __restrict__ ref_codes, uint32_t Gf,
+                                        const CT* __restrict__ ref_codes, uint32_t Gf,
                                         int nbins, int16_t* __restrict__ out,
                                         float* __restrict__ out_sq, int* bins, uint16_t* codes,
                                         int lane)
@@ -698,11 +700,12 @@ __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict
 }
 
 // Reference rows of every group except the fused ones.
+template <class CT>
 __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
     const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
     const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
-    const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
+    const CT* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
     int16_t* __restrict__ out, float* __restrict__ out_sq)
 {
     extern __shared__ int smem[];
@@ -728,11 +731,12 @@ __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
 // product with each of the group's cell rank rows, and stores the group's
 // scores corr_from_sums(sxy, sxx, syy) -- the same values the thin tile
 // computes from stored ranks.
+template <class CT>
 __global__ __launch_bounds__(256) void k_ftd_fused(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
     const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
     const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
-    const uint16_t* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
+    const CT* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
     const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
     float* __restrict__ scores)
 {
