@@ -192,12 +192,13 @@ struct csg_engine
     uint32_t               ftd_mask = 0xFFFFFFFFu;  // launch mask of the current round
     // launch settings of the work-queue kernels (blocks), tunable
     int   ftd_grid_threads = 1024, ftd_grid_setup = 512, ftd_grid_rank = 2048,
-        ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_thin_max = 8, ftd_t64 = 48;
+        ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_grid_fused = 4096, ftd_thin_max = 8,
+        ftd_t64 = 48;
     int ftd_fuse_max = FTD_FUSE_MAX;  // groups of at most this many cells take the fused form
     bool     ftd_autotune = true, ftd_tuned = false;
     uint64_t ftd_budget   = (uint64_t)8 << 30;  // device bytes of one work batch
     int64_t  ftd_batches  = 0;                   // work batches of oversized rounds
-    float ftd_tune_ms[3] = { 0.f, 0.f, 0.f };  // best times of the tuned kernels
+    float ftd_tune_ms[4] = { 0.f, 0.f, 0.f, 0.f };  // best times of the tuned kernels
 
     // ---- learned proposal ----
     // Network G_s -> H -> H -> L (GELU), parameters in one FP32 array
@@ -552,6 +553,7 @@ int csg_init_gpu(int gpuid, csg_engine** out)
         eng->ftd_grid_threads = cu * 4;
         eng->ftd_grid_setup   = cu * 4;
         eng->ftd_grid_rank    = cu * 8;
+        eng->ftd_grid_fused   = cu * 16;
         eng->ftd_grid_corr    = cu * 2;
         eng->ftd_grid_select  = cu * 8;
     }
@@ -2411,19 +2413,27 @@ int ftd_launch_corr(csg_engine* eng, int grid)
         k_ftd_thin<<<(unsigned)grid, 256, d.thin, eng->stream>>>(
             cnt, 0, eng->d_fl_thin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
             eng->d_ft_rsq, eng->d_ft_scores);
-    if ((m & FTDM_FUSED) && ftd_fuse(eng))
-    {
-        if (eng->d_fref_codes8)
-            k_ftd_fused<uint8_t><<<(unsigned)eng->ftd_grid_rank, 256, d.rank_ref_w, eng->stream>>>(
-                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-                eng->d_fl_ggenes, eng->d_fref_codes8, std::max<uint32_t>(eng->Gf, 1), eng->fnbins,
-                d.wave_ints, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_scores);
-        else
-            k_ftd_fused<uint16_t><<<(unsigned)eng->ftd_grid_rank, 256, d.rank_ref_w, eng->stream>>>(
-                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-                eng->d_fl_ggenes, eng->d_fref_codes, std::max<uint32_t>(eng->Gf, 1), eng->fnbins,
-                d.wave_ints, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_scores);
-    }
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+// Fused-form groups (k_ftd_fused, one wave per reference row).
+int ftd_launch_fused(csg_engine* eng, int grid)
+{
+    if (!(eng->ftd_mask & FTDM_FUSED) || !ftd_fuse(eng))
+        return CSG_OK;
+    const FtdLds d   = ftd_lds(eng);
+    FtdCounts*   cnt = eng->d_fl_cnt;
+    if (eng->d_fref_codes8)
+        k_ftd_fused<uint8_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
+            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+            eng->d_fl_ggenes, eng->d_fref_codes8, std::max<uint32_t>(eng->Gf, 1), eng->fnbins,
+            d.wave_ints, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_scores);
+    else
+        k_ftd_fused<uint16_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
+            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+            eng->d_fl_ggenes, eng->d_fref_codes, std::max<uint32_t>(eng->Gf, 1), eng->fnbins,
+            d.wave_ints, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_scores);
     LAUNCHCHK();
     return CSG_OK;
 }
@@ -2467,7 +2477,7 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
     k_ftd_tiles<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
                                                 (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0,
                                                 eng->d_fl_gthin0, eng->d_fl_tiles, eng->d_fl_thin);
-    if (ftd_launch_corr(eng, eng->ftd_grid_corr) ||
+    if (ftd_launch_corr(eng, eng->ftd_grid_corr) || ftd_launch_fused(eng, eng->ftd_grid_fused) ||
         ftd_launch_select(eng, eng->ftd_grid_select, quantile))
         return CSG_ERR_DEVICE;
     k_ftd_select_nan<FtdSegments><<<g, FTD_TPB, 0, eng->stream>>>(
@@ -2635,6 +2645,9 @@ int ftd_tune(csg_engine* eng, float quantile)
     eng->ftd_tune_ms[1] = best_corr;
     if ((rc = retile(best_t64)))
         return rc;
+    if ((eng->ftd_mask & FTDM_FUSED) && ftd_fuse(eng))
+        pick({ 4, 8, 16, 32 }, [&](int gr) { return ftd_launch_fused(eng, gr); },
+             &eng->ftd_grid_fused, &eng->ftd_tune_ms[3]);
     pick({ 2, 4, 8, 16 },
          [&](int gr) {
              k_ftd_reset_nan<<<1, 64, 0, eng->stream>>>(cnt);
@@ -2709,6 +2722,7 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->ftd_mask, (uintptr_t)q,
              (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,
              (uintptr_t)eng->ftd_grid_corr, (uintptr_t)eng->ftd_grid_select,
+             (uintptr_t)eng->ftd_grid_fused,
              (uintptr_t)eng->ftd_thin_max, (uintptr_t)eng->ftd_t64,
              (uintptr_t)eng->ftd_grid_threads };
 }
@@ -3319,9 +3333,10 @@ int csg_finetune_tuning(csg_engine* eng, int autotune, int32_t* grids, float* be
         grids[2] = eng->ftd_grid_corr;
         grids[3] = eng->ftd_grid_select;
         grids[4] = eng->ftd_t64 == (int)0xFFFFFFFF ? -1 : eng->ftd_t64;
+        grids[5] = eng->ftd_grid_fused;
     }
     if (best_ms)
-        for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 4; ++i)
             best_ms[i] = eng->ftd_tune_ms[i];
     return CSG_OK;
 }

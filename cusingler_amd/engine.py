@@ -278,13 +278,13 @@ class Engine:
     def finetune_tuning(self, autotune: Optional[bool] = None) -> Dict[str, object]:
         """Autotuner state of the device-resident fine-tune (optionally switching
         it on / off, which asks for a new tuning)."""
-        g = np.zeros(5, np.int32)
-        ms = np.zeros(3, np.float32)
+        g = np.zeros(6, np.int32)
+        ms = np.zeros(4, np.float32)
         a = -1 if autotune is None else int(bool(autotune))
         self._check(lib().csg_finetune_tuning(self._h, a, _p(g), _p(ms)), "finetune_tuning")
         return {"tuned": bool(g[0]), "rank_blocks": int(g[1]), "corr_blocks": int(g[2]),
                 "select_blocks": int(g[3]), "big_tile_min_cells": int(g[4]),
-                "best_ms": [float(v) for v in ms]}
+                "fused_blocks": int(g[5]), "best_ms": [float(v) for v in ms]}
 
     def finetune(self, quantile: float = 0.8, thre: float = 0.05, times: int = 0):
         final = np.zeros(self.fNq, np.uint32)

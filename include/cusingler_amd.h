@@ -131,10 +131,11 @@ int csg_set_finetune_fuse(csg_engine* eng, int max_cells);
  * rank, correlation and select work-queue kernels and the exact correlation
  * tile shape (16-cell tiles, or 32-cell tiles for groups of at least t cells)
  * are chosen by timing each candidate on that round's work.  autotune = 1 / 0
- * turns it on / off (and asks for a new tuning), -1 leaves it; grids[0..4]
+ * turns it on / off (and asks for a new tuning), -1 leaves it; grids[0..5]
  * receive (tuned, rank, correlation, select block counts, t or -1 for 16-cell
- * tiles only) and best_ms[0..2] the tuned kernels' times (either may be
- * NULL).  No counterpart in the reference. */
+ * tiles only, fused-form block count) and best_ms[0..3] the tuned kernels'
+ * times (rank, correlation, select, fused; either array may be NULL).  No
+ * counterpart in the reference. */
 int csg_finetune_tuning(csg_engine* eng, int autotune, int32_t* grids, float* best_ms);
 
 /* Device memory budget (bytes) of one device-resident fine-tune work batch:
