@@ -2461,22 +2461,17 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
     const FtdSegments seg{ eng->d_fl_cells, eng->d_fl_segcell, eng->d_fl_groups,
                            eng->d_fl_glabels, eng->d_fl_glpos, eng->d_fllen };
     const FtdListSegments rseg{ seg, eng->d_fl_rseg };
-    k_ftd_cells<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_cgid, eng->d_fl_cpos,
-                                                eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_gseg0,
-                                                eng->d_fl_counts, eng->d_fl_rounds,
-                                                eng->d_fl_items, eng->d_fl_cells,
-                                                eng->d_fl_segcell);
-    k_ftd_refmap<<<(unsigned)eng->ftd_grid_setup, 256, 0, eng->stream>>>(
-        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
-        eng->d_fl_refgroup, eng->d_fl_refrow);
+    k_ftd_prep<<<std::max<unsigned>(g, (unsigned)eng->ftd_grid_setup), FTD_TPB, 0, eng->stream>>>(
+        cnt, eng->d_fl_act, eng->d_fl_cgid, eng->d_fl_cpos, eng->d_fl_groups, eng->d_fl_gflag,
+        eng->d_fl_gseg0, eng->d_fl_counts, eng->d_fl_items, eng->d_fl_cells, eng->d_fl_segcell,
+        eng->d_fl_glabels, eng->d_flstart, eng->d_fllen, eng->d_fl_refgroup, eng->d_fl_refrow,
+        (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0, eng->d_fl_gthin0,
+        eng->d_fl_tiles, eng->d_fl_thin);
     if ((eng->ftd_mask & FTDM_RANK) && ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
         return CSG_ERR_DEVICE;
     k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
         cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
         eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
-    k_ftd_tiles<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
-                                                (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0,
-                                                eng->d_fl_gthin0, eng->d_fl_tiles, eng->d_fl_thin);
     if (ftd_launch_corr(eng, eng->ftd_grid_corr) || ftd_launch_fused(eng, eng->ftd_grid_fused) ||
         ftd_launch_select(eng, eng->ftd_grid_select, quantile))
         return CSG_ERR_DEVICE;
@@ -2569,9 +2564,8 @@ int ftd_tune(csg_engine* eng, float quantile)
     FtdCounts*     cnt = eng->d_fl_cnt;
     k_ftd_cells<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_cgid, eng->d_fl_cpos,
                                                 eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_gseg0,
-                                                eng->d_fl_counts, eng->d_fl_rounds,
-                                                eng->d_fl_items, eng->d_fl_cells,
-                                                eng->d_fl_segcell);
+                                                eng->d_fl_counts, eng->d_fl_items,
+                                                eng->d_fl_cells, eng->d_fl_segcell);
     k_ftd_refmap<<<(unsigned)eng->ftd_grid_setup, 256, 0, eng->stream>>>(
         cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
         eng->d_fl_refgroup, eng->d_fl_refrow);

@@ -508,11 +508,11 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
 // work-queue kernels find them without a search.  Idempotent (the round is
 // counted by k_ftd_compact), so the autotuner may run it ahead of the
 // round's graph.
-__global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+__device__ inline void ftd_cells_body(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
                             const uint32_t* __restrict__ cgid, const uint32_t* __restrict__ cpos,
                             const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
                             const uint64_t* __restrict__ gseg0, uint32_t* __restrict__ counts,
-                            int32_t* __restrict__ rounds, FtRankItem* __restrict__ items,
+                            FtRankItem* __restrict__ items,
                             FtCell* __restrict__ cells, uint32_t* __restrict__ segcell)
 {
     const uint32_t n = cnt->n_active;
@@ -540,7 +540,7 @@ __global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* _
 
 // Group and source row of every reference rank item (one block per group;
 // the group's rows are its labels' rows in ascending label order).
-__global__ void k_ftd_refmap(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+__device__ inline void ftd_refmap_body(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
                              const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ labels,
                              const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
                              uint32_t* __restrict__ refgroup, uint32_t* __restrict__ refrow)
@@ -566,7 +566,7 @@ __global__ void k_ftd_refmap(const FtdCounts* __restrict__ cnt, const FtGroup* _
 }
 
 // Correlation tile lists of every group with work.
-__global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+__device__ inline void ftd_tiles_body(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
                             const uint8_t* __restrict__ gflag, uint32_t thin_max, uint32_t t64,
                             const uint32_t* __restrict__ gtile0, const uint32_t* __restrict__ gthin0,
                             FtCorrTile* __restrict__ tiles, FtCorrTile* __restrict__ thin)
@@ -604,6 +604,51 @@ __global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __
                     tiles[t++] = FtCorrTile{ g, c0, r0, 0u };
         }
     }
+}
+
+__global__ void k_ftd_cells(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+                            const uint32_t* __restrict__ cgid, const uint32_t* __restrict__ cpos,
+                            const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
+                            const uint64_t* __restrict__ gseg0, uint32_t* __restrict__ counts,
+                            FtRankItem* __restrict__ items, FtCell* __restrict__ cells,
+                            uint32_t* __restrict__ segcell)
+{
+    ftd_cells_body(cnt, act, cgid, cpos, groups, gflag, gseg0, counts, items, cells, segcell);
+}
+
+__global__ void k_ftd_refmap(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+                             const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ labels,
+                             const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
+                             uint32_t* __restrict__ refgroup, uint32_t* __restrict__ refrow)
+{
+    ftd_refmap_body(cnt, groups, gflag, labels, lstart, llen, refgroup, refrow);
+}
+
+__global__ void k_ftd_tiles(const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+                            const uint8_t* __restrict__ gflag, uint32_t thin_max, uint32_t t64,
+                            const uint32_t* __restrict__ gtile0, const uint32_t* __restrict__ gthin0,
+                            FtCorrTile* __restrict__ tiles, FtCorrTile* __restrict__ thin)
+{
+    ftd_tiles_body(cnt, groups, gflag, thin_max, t64, gtile0, gthin0, tiles, thin);
+}
+
+// The three set-up passes of part B in one launch (they only read the scan's
+// results and write disjoint tables): cells, reference maps, tile lists.
+__global__ void k_ftd_prep(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+                           const uint32_t* __restrict__ cgid, const uint32_t* __restrict__ cpos,
+                           const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
+                           const uint64_t* __restrict__ gseg0, uint32_t* __restrict__ counts,
+                           FtRankItem* __restrict__ items, FtCell* __restrict__ cells,
+                           uint32_t* __restrict__ segcell, const uint32_t* __restrict__ labels,
+                           const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
+                           uint32_t* __restrict__ refgroup, uint32_t* __restrict__ refrow,
+                           uint32_t thin_max, uint32_t t64, const uint32_t* __restrict__ gtile0,
+                           const uint32_t* __restrict__ gthin0, FtCorrTile* __restrict__ tiles,
+                           FtCorrTile* __restrict__ thin)
+{
+    ftd_cells_body(cnt, act, cgid, cpos, groups, gflag, gseg0, counts, items, cells, segcell);
+    ftd_refmap_body(cnt, groups, gflag, labels, lstart, llen, refgroup, refrow);
+    ftd_tiles_body(cnt, groups, gflag, thin_max, t64, gtile0, gthin0, tiles, thin);
 }
 
 // ---- work-queue kernels: the grid covers the machine, items come from the
