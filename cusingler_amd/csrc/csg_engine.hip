@@ -105,11 +105,11 @@ struct csg_engine
     float*    d_Pref = nullptr;
     Segment*  d_item_seg = nullptr;
     // re-scored pairs grouped by label for the grouped FP32 contraction
-    uint32_t * d_rc_cells = nullptr, *d_rc_abase = nullptr, *d_rc_acnt = nullptr,
-             *d_rc_ucells = nullptr;
+    uint32_t * d_rc_cells = nullptr, *d_rc_abase = nullptr, *d_rc_ucells = nullptr;
+    RecheckItem* d_rc_sorted = nullptr;
     uint64_t*  d_rc_coff  = nullptr;
     GroupTile* d_rc_tiles = nullptr;
-    size_t     rc_cap_seg = 0, rc_cap_ucells = 0, rc_cap_cells = 0, rc_cap_l = 0, rc_cap_l2 = 0,
+    size_t     rc_cap_seg = 0, rc_cap_ucells = 0, rc_cap_sorted = 0, rc_cap_cells = 0, rc_cap_l = 0,
            rc_cap_coff = 0, rc_cap_tiles = 0;
 
     // ---- fine-tune ----
@@ -605,12 +605,12 @@ int csg_destroy_score(csg_engine* eng)
     dfree(eng->d_item_seg);
     dfree(eng->d_rc_cells);
     dfree(eng->d_rc_abase);
-    dfree(eng->d_rc_acnt);
+    dfree(eng->d_rc_sorted);
     dfree(eng->d_rc_coff);
     dfree(eng->d_rc_tiles);
     dfree(eng->d_rc_ucells);
-    eng->rc_cap_ucells = 0;
-    eng->rc_cap_seg = eng->rc_cap_cells = eng->rc_cap_l = eng->rc_cap_l2 =
+    eng->rc_cap_ucells = eng->rc_cap_sorted = 0;
+    eng->rc_cap_seg = eng->rc_cap_cells = eng->rc_cap_l =
         eng->rc_cap_coff = eng->rc_cap_tiles = 0;
     free_proposal(eng);
     eng->ref_prepared_path = -1;
@@ -881,7 +881,7 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
     int32_t*       F  = eng->d_first + q0;
     float*         M  = eng->d_mask + (size_t)q0 * L;
     const double   wp = certify_width(eng->G);
-    HIPCHK(hipMemsetAsync(eng->d_nitems, 0, sizeof(unsigned int), eng->stream));
+    HIPCHK(hipMemsetAsync(eng->d_nitems, 0, 2 * sizeof(unsigned int), eng->stream));
     HIPCHK(hipMemsetAsync(eng->d_exact, 0, (size_t)nq * L, eng->stream));
     {
         PhaseScope ps(eng, 3, 1);
@@ -889,81 +889,62 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
             P, nq, L, wp, thre, F, M, eng->d_cstatus, eng->d_items, eng->d_nitems);
         LAUNCHCHK();
     }
-    unsigned int n = 0;
-    HIPCHK(hipMemcpyAsync(&n, eng->d_nitems, sizeof(n), hipMemcpyDeviceToHost, eng->stream));
+    unsigned int hn[2] = { 0, 0 };  // pairs to re-score, uncertified cells
+    HIPCHK(hipMemcpyAsync(hn, eng->d_nitems, sizeof(hn), hipMemcpyDeviceToHost, eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
+    const unsigned int n = hn[0];
     if (n == 0)
         return CSG_OK;
-    std::vector<RecheckItem> items(n);
-    HIPCHK(hipMemcpy(items.data(), eng->d_items, n * sizeof(RecheckItem), hipMemcpyDeviceToHost));
-    // group the pairs by label: label l's cells are listed from a_base[l], and
-    // a cell's scores against the label's rows follow each other from
-    // c_off[l] on, so that every pair's scores are one contiguous segment
-    std::vector<uint32_t> acnt(L, 0), abase(L, 0);
-    std::vector<uint64_t> coff(L, 0);
-    for (unsigned int i = 0; i < n; ++i)
-        acnt[items[i].l]++;
-    uint64_t tot = 0;
-    for (uint32_t l = 0, a = 0; l < L; ++l)
-    {
-        abase[l] = a;
-        coff[l]  = tot;
-        a += acnt[l];
-        tot += (uint64_t)acnt[l] * eng->ctidx[2 * l + 1];
-    }
-    if (tot > (uint64_t)eng->Qc * eng->Nr)
-        return eng->fail(CSG_ERR_MEMORY, "re-scored pairs exceed the score buffer");
-    std::vector<uint32_t>    fill(L, 0), cells(n);
-    std::vector<RecheckItem> sorted(n);
-    std::vector<Segment>     segs(n);
-    std::vector<uint8_t>     cell_seen(nq, 0);
-    for (unsigned int i = 0; i < n; ++i)
-    {
-        const RecheckItem it  = items[i];
-        const uint32_t    len = eng->ctidx[2 * it.l + 1];
-        const uint32_t    p   = fill[it.l]++;
-        cells[abase[it.l] + p]  = it.q;
-        sorted[abase[it.l] + p] = it;
-        segs[abase[it.l] + p]   = Segment{ coff[it.l] + (uint64_t)p * len, len, it.l };
-        if (!cell_seen[it.q])
-        {
-            cell_seen[it.q] = 1;
-            eng->counters[4]++;
-        }
-    }
     eng->counters[3] += n;
-    std::vector<uint32_t> ucells;  // the re-scored cells, once each
-    for (uint32_t q = 0; q < nq; ++q)
-        if (cell_seen[q])
-            ucells.push_back(q);
-    std::vector<GroupTile> tiles;
+    eng->counters[4] += hn[1];
+    // the pairs grouped by label on the device: label l's cells listed from
+    // abase[l], a cell's scores against the label's rows contiguous from
+    // coff[l] on, so that every pair's scores are one contiguous segment
+    uint32_t max_len = 1;
     for (uint32_t l = 0; l < L; ++l)
-        for (uint32_t m0 = 0; m0 < acnt[l]; m0 += PG_GBM)
-            for (uint32_t n0 = 0; n0 < eng->ctidx[2 * l + 1]; n0 += PG_BN)
-                tiles.push_back(GroupTile{ l, m0, n0, 0u });
+        max_len = std::max(max_len, eng->ctidx[2 * l + 1]);
+    const size_t tile_bound = ((size_t)n / PG_GBM + L) * ((max_len + PG_BN - 1) / PG_BN);
+    // a cell's pairs take at most Nr score slots (one per reference row of its
+    // labels), so the grouped scores fit the score chunk
+    if ((uint64_t)hn[1] * eng->Nr > (uint64_t)eng->Qc * eng->Nr)
+        return eng->fail(CSG_ERR_MEMORY, "re-scored pairs exceed the score buffer");
     int rc;
-    // the label-sorted pairs replace the certification's list in place (n is
-    // at most the list's capacity, nq * L)
-    HIPCHK(hipMemcpyAsync(eng->d_items, sorted.data(), n * sizeof(RecheckItem),
-                          hipMemcpyHostToDevice, eng->stream));
-    if ((rc = upload_keep(eng, &eng->d_item_seg, &eng->rc_cap_seg, segs.data(), n)) ||
-        (rc = upload_keep(eng, &eng->d_rc_cells, &eng->rc_cap_cells, cells.data(), n)) ||
-        (rc = upload_keep(eng, &eng->d_rc_abase, &eng->rc_cap_l, abase.data(), L)) ||
-        (rc = upload_keep(eng, &eng->d_rc_acnt, &eng->rc_cap_l2, acnt.data(), L)) ||
-        (rc = upload_keep(eng, &eng->d_rc_coff, &eng->rc_cap_coff, coff.data(), L)) ||
-        (rc = upload_keep(eng, &eng->d_rc_tiles, &eng->rc_cap_tiles, tiles.data(), tiles.size())) ||
-        (rc = upload_keep(eng, &eng->d_rc_ucells, &eng->rc_cap_ucells, ucells.data(), ucells.size())))
+    if ((rc = ensure(eng, &eng->d_item_seg, &eng->rc_cap_seg, n)) ||
+        (rc = ensure(eng, &eng->d_rc_cells, &eng->rc_cap_cells, n)) ||
+        (rc = ensure(eng, &eng->d_rc_sorted, &eng->rc_cap_sorted, n)) ||
+        (rc = ensure(eng, &eng->d_rc_abase, &eng->rc_cap_l, 4 * (size_t)L + 2)) ||
+        (rc = ensure(eng, &eng->d_rc_coff, &eng->rc_cap_coff, L)) ||
+        (rc = ensure(eng, &eng->d_rc_tiles, &eng->rc_cap_tiles, tile_bound)) ||
+        (rc = ensure(eng, &eng->d_rc_ucells, &eng->rc_cap_ucells, std::max<uint32_t>(hn[1], 1))))
         return rc;
+    // one uint32 table: [cnt L | abase L | tbase L | fill L | ntiles | nu]
+    uint32_t* cnt    = eng->d_rc_abase;
+    uint32_t* abase  = cnt + L;
+    uint32_t* tbase  = abase + L;
+    uint32_t* fill   = tbase + L;
+    uint32_t* ntiles = fill + L;
+    uint32_t* nu     = ntiles + 1;
     {
         PhaseScope ps(eng, 4, 3);
+        k_fill_u32<<<1, 256, 0, eng->stream>>>(cnt, (size_t)4 * L + 2, 0u);
+        k_rc_count<<<(n + 255) / 256, 256, 0, eng->stream>>>(eng->d_items, n, cnt);
+        k_rc_scan<<<1, 64, 0, eng->stream>>>(cnt, eng->d_llen, L, PG_GBM, PG_BN, abase,
+                                             eng->d_rc_coff, tbase, fill, ntiles);
+        k_rc_scatter<<<(n + 255) / 256, 256, 0, eng->stream>>>(
+            eng->d_items, n, abase, eng->d_rc_coff, eng->d_llen, fill, eng->d_rc_cells,
+            eng->d_rc_sorted, eng->d_item_seg);
+        k_rc_tiles<<<(L + 63) / 64, 64, 0, eng->stream>>>(cnt, eng->d_llen, tbase, L, PG_GBM,
+                                                          PG_BN, eng->d_rc_tiles);
+        k_rc_cells<<<(nq + 255) / 256, 256, 0, eng->stream>>>(eng->d_cstatus, nq,
+                                                              eng->d_rc_ucells, nu);
         // the reference's FP32 sums of squares of the re-scored cells (the
         // exact path's rank pass leaves them out)
-        k_sq_seq<<<(unsigned)((ucells.size() + 255) / 256), 256, 0, eng->stream>>>(
-            eng->d_Xq, eng->Kp, eng->G, eng->d_rc_ucells, (uint32_t)ucells.size(), eng->d_sqq_f);
-        k_gemm_plain_group<<<(unsigned)tiles.size(), 256, 0, eng->stream>>>(
-            eng->d_rc_tiles, eng->d_rc_cells, eng->d_rc_abase, eng->d_rc_acnt, eng->d_lstart,
-            eng->d_llen, eng->d_rc_coff, eng->d_Xq, eng->d_Xr, (int)eng->Kp,
-            (uint32_t)round_up(eng->Nr, 256) - 1, eng->d_sqq_f, eng->d_sqr_f, eng->d_S);
+        k_sq_seq<<<std::max(1u, (hn[1] + 255) / 256), 256, 0, eng->stream>>>(eng->d_Xq, eng->Kp, eng->G,
+                                                               eng->d_rc_ucells, nu, eng->d_sqq_f);
+        k_gemm_plain_group<<<(unsigned)tile_bound, 256, 0, eng->stream>>>(
+            eng->d_rc_tiles, eng->d_rc_cells, abase, cnt, eng->d_lstart, eng->d_llen,
+            eng->d_rc_coff, eng->d_Xq, eng->d_Xr, (int)eng->Kp,
+            (uint32_t)round_up(eng->Nr, 256) - 1, eng->d_sqq_f, eng->d_sqr_f, eng->d_S, ntiles);
         LAUNCHCHK();
         HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
         TableSegments src{ eng->d_item_seg };
@@ -977,7 +958,7 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
         return rc;
     {
         PhaseScope ps(eng, 4, 2);
-        k_apply_recheck<<<(n + 255) / 256, 256, 0, eng->stream>>>(eng->d_items, n, eng->d_Pref, P,
+        k_apply_recheck<<<(n + 255) / 256, 256, 0, eng->stream>>>(eng->d_rc_sorted, n, eng->d_Pref, P,
                                                                   L, eng->d_exact);
         LAUNCHCHK();
         k_label_final<<<(nq + 255) / 256, 256, 0, eng->stream>>>(P, eng->d_cstatus, eng->d_exact,
@@ -1467,7 +1448,7 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
             (rc = dalloc(eng, &eng->d_cstatus, qc)) ||
             (rc = dalloc(eng, &eng->d_exact, qc * eng->L)) ||
             (rc = dalloc(eng, &eng->d_items, qc * eng->L)) ||
-            (rc = dalloc(eng, &eng->d_nitems, 1)) || (rc = dalloc(eng, &eng->d_Pref, qc * eng->L)))
+            (rc = dalloc(eng, &eng->d_nitems, 2)) || (rc = dalloc(eng, &eng->d_Pref, qc * eng->L)))
             return rc;
     }
     if (path == CSG_PATH_FP16_MFMA && !eng->d_rf16)
@@ -3247,7 +3228,7 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
             (rc = dalloc(eng, &eng->d_cstatus, qc)) ||
             (rc = dalloc(eng, &eng->d_exact, qc * eng->L)) ||
             (rc = dalloc(eng, &eng->d_items, qc * eng->L)) ||
-            (rc = dalloc(eng, &eng->d_nitems, 1)) || (rc = dalloc(eng, &eng->d_Pref, qc * eng->L)))
+            (rc = dalloc(eng, &eng->d_nitems, 2)) || (rc = dalloc(eng, &eng->d_Pref, qc * eng->L)))
             return rc;
     }
     if (path == CSG_PATH_FP16_MFMA && !eng->d_rf16)

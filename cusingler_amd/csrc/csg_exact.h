@@ -49,6 +49,7 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
                                 uint8_t* __restrict__ status, RecheckItem* __restrict__ items,
                                 unsigned int* __restrict__ n_items)
 {
+    // n_items[0]: (cell, label) pairs listed; n_items[1]: cells left uncertified
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq)
         return;
@@ -70,6 +71,7 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
     if (nan)
     {
         status[q] = 2;
+        atomicAdd(n_items + 1, 1u);
         for (uint32_t j = 0; j < L; ++j)
         {
             const unsigned int t = atomicAdd(n_items, 1u);
@@ -110,6 +112,7 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
         return;
     }
     status[q] = 1;
+    atomicAdd(n_items + 1, 1u);
     for (uint32_t j = 0; j < L; ++j)
     {
         const double v   = (double)s[j];
@@ -121,6 +124,94 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
             items[t]             = RecheckItem{ q, j };
         }
     }
+}
+
+// ---- re-scoring lists grouped by label, built on the device
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Pairs per label.
+__global__ void k_rc_count(const RecheckItem* __restrict__ items, uint32_t n,
+                           uint32_t* __restrict__ cnt)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        atomicAdd(&cnt[items[i].l], 1u);
+}
+
+// One thread: label l's cells start at abase[l] of the grouped list, its
+// scores at coff[l] (acnt[l] cells x llen[l] rows), its tiles at tbase[l]
+// (TM-cell x TN-row tiles); the fill counters are cleared; ntiles[0] = tiles.
+__global__ void k_rc_scan(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ llen,
+                          uint32_t L, uint32_t TM, uint32_t TN, uint32_t* __restrict__ abase,
+                          uint64_t* __restrict__ coff, uint32_t* __restrict__ tbase,
+                          uint32_t* __restrict__ fill, uint32_t* __restrict__ ntiles)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    uint32_t a = 0, t = 0;
+    uint64_t c = 0;
+    for (uint32_t l = 0; l < L; ++l)
+    {
+        abase[l] = a;
+        coff[l]  = c;
+        tbase[l] = t;
+        fill[l]  = 0;
+        a += cnt[l];
+        c += (uint64_t)cnt[l] * llen[l];
+        t += ((cnt[l] + TM - 1) / TM) * ((llen[l] + TN - 1) / TN);
+    }
+    ntiles[0] = t;
+}
+
+// Each pair takes the next place of its label: the label's cell list, the
+// pair in label order and its score segment (contiguous within the label's
+// block).  Places within a label follow atomics; every pair's values do not
+// depend on its place.
+__global__ void k_rc_scatter(const RecheckItem* __restrict__ items, uint32_t n,
+                             const uint32_t* __restrict__ abase, const uint64_t* __restrict__ coff,
+                             const uint32_t* __restrict__ llen, uint32_t* __restrict__ fill,
+                             uint32_t* __restrict__ cells, RecheckItem* __restrict__ sorted,
+                             Segment* __restrict__ segs)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const RecheckItem it  = items[i];
+    const uint32_t    len = llen[it.l];
+    const uint32_t    p   = atomicAdd(&fill[it.l], 1u);
+    const uint32_t    pos = abase[it.l] + p;
+    cells[pos]            = it.q;
+    sorted[pos]           = it;
+    segs[pos]             = Segment{ coff[it.l] + (uint64_t)p * len, len, it.l };
+}
+
+// Tile list of the grouped contraction, one thread per label.
+__global__ void k_rc_tiles(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ llen,
+                           const uint32_t* __restrict__ tbase, uint32_t L, uint32_t TM,
+                           uint32_t TN, GroupTile* __restrict__ tiles)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L)
+        return;
+    uint32_t t = tbase[l];
+    for (uint32_t m0 = 0; m0 < cnt[l]; m0 += TM)
+        for (uint32_t n0 = 0; n0 < llen[l]; n0 += TN)
+            tiles[t++] = GroupTile{ l, m0, n0, 0u };
+}
+
+// The uncertified cells (status 1 or 2), listed once each.
+__global__ void k_rc_cells(const uint8_t* __restrict__ status, uint32_t nq,
+                           uint32_t* __restrict__ list, uint32_t* __restrict__ n_list)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq && status[q] != 0)
+        list[atomicAdd(n_list, 1u)] = q;
 }
 
 // Writes the re-scored quantiles into P and redoes the decision of every cell

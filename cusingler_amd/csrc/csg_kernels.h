@@ -312,10 +312,11 @@ __global__ __launch_bounds__(256) void k_rank_csr_w(
 
 // The reference's FP32 sums of squares of the listed rows of a rank matrix.
 __global__ void k_sq_seq(const int16_t* __restrict__ X, uint32_t Kp, uint32_t G,
-                         const uint32_t* __restrict__ rows, uint32_t n, float* __restrict__ sq_f32)
+                         const uint32_t* __restrict__ rows, const uint32_t* __restrict__ n_rows,
+                         float* __restrict__ sq_f32)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
+    if (i < n_rows[0])
         sq_f32[rows[i]] = seq_sum_sq(X + (size_t)rows[i] * Kp, G);
 }
 
@@ -498,8 +499,10 @@ __global__ __launch_bounds__(256) void k_gemm_plain_group(
     const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
     const uint64_t* __restrict__ c_off, const int16_t* __restrict__ Xq,
     const int16_t* __restrict__ Xr, int Kp, uint32_t b_max, const float* __restrict__ sq_q,
-    const float* __restrict__ sq_r, float* __restrict__ S)
+    const float* __restrict__ sq_r, float* __restrict__ S, const uint32_t* __restrict__ ntiles)
 {
+    if (blockIdx.x >= ntiles[0])
+        return;  // the grid is a bound on the device-built tile count
     const GroupTile t  = tiles[blockIdx.x];
     const uint32_t  nl = llen[t.label];
     const GroupPMap map{ cells + a_base[t.label] + t.m0,
