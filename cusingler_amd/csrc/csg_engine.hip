@@ -224,12 +224,17 @@ struct csg_engine
     float*     d_pmargin = nullptr;
     uint32_t * d_pcnt = nullptr, *d_pslot = nullptr, *d_pfb = nullptr, *d_pcells = nullptr;
     uint32_t*  d_plist = nullptr;  // cells the edge tests list (two lists of Qc)
+    // per-round label tables of the grouped contraction, views into d_pmeta:
+    // [c_off L (u64) | a_base L | a_cnt L | tiles], uploaded in one copy from
+    // the pinned h_pmeta
     uint32_t * d_pabase = nullptr, *d_pacnt = nullptr;
     uint64_t*  d_pcoff = nullptr;
+    uint8_t*   d_pmeta = nullptr;
+    uint8_t*   h_pmeta = nullptr;
+    size_t     pmeta_cap = 0, h_pmeta_cap = 0;
     RecheckItem* d_pitems = nullptr;
     Segment*   d_pseg = nullptr;
     GroupTile* d_ptiles = nullptr;
-    size_t     d_ptiles_cap = 0;
 
     // ---- timing / counters ----
     double  phase_ms[CSG_N_PHASES]  = {};
@@ -1212,17 +1217,32 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
     if (n_scores > (uint64_t)eng->Qc * eng->Nr || n_items > (uint64_t)eng->Qc * L)
         return eng->fail(CSG_ERR_MEMORY, "proposal round exceeds the score buffer");
     int rc;
-    if ((rc = ensure(eng, &eng->d_ptiles, &eng->d_ptiles_cap, std::max<size_t>(tiles.size(), 1))))
+    // one pinned upload of the round's tables (the previous round's copy has
+    // completed: a host read of counts separates the rounds)
+    const size_t o_ab = (size_t)L * 8, o_ac = o_ab + (size_t)L * 4;
+    const size_t o_t  = (o_ac + (size_t)L * 4 + 15) & ~(size_t)15;
+    const size_t bytes = o_t + tiles.size() * sizeof(GroupTile);
+    if ((rc = ensure(eng, &eng->d_pmeta, &eng->pmeta_cap, bytes)))
         return rc;
-    HIPCHK(hipMemcpyAsync(eng->d_pabase, abase.data(), L * sizeof(uint32_t), hipMemcpyHostToDevice,
-                          eng->stream));
-    HIPCHK(hipMemcpyAsync(eng->d_pacnt, acnt.data(), L * sizeof(uint32_t), hipMemcpyHostToDevice,
-                          eng->stream));
-    HIPCHK(hipMemcpyAsync(eng->d_pcoff, coff.data(), L * sizeof(uint64_t), hipMemcpyHostToDevice,
-                          eng->stream));
-    if (!tiles.empty())
-        HIPCHK(hipMemcpyAsync(eng->d_ptiles, tiles.data(), tiles.size() * sizeof(GroupTile),
-                              hipMemcpyHostToDevice, eng->stream));
+    if (bytes > eng->h_pmeta_cap)
+    {
+        if (eng->h_pmeta)
+            (void)hipHostFree(eng->h_pmeta);
+        eng->h_pmeta     = nullptr;
+        eng->h_pmeta_cap = 0;
+        const size_t want = std::max(bytes, eng->h_pmeta_cap * 3 / 2);
+        HIPCHK(hipHostMalloc((void**)&eng->h_pmeta, want, hipHostMallocDefault));
+        eng->h_pmeta_cap = want;
+    }
+    std::memcpy(eng->h_pmeta, coff.data(), (size_t)L * 8);
+    std::memcpy(eng->h_pmeta + o_ab, abase.data(), (size_t)L * 4);
+    std::memcpy(eng->h_pmeta + o_ac, acnt.data(), (size_t)L * 4);
+    std::memcpy(eng->h_pmeta + o_t, tiles.data(), tiles.size() * sizeof(GroupTile));
+    HIPCHK(hipMemcpyAsync(eng->d_pmeta, eng->h_pmeta, bytes, hipMemcpyHostToDevice, eng->stream));
+    eng->d_pcoff  = (uint64_t*)eng->d_pmeta;
+    eng->d_pabase = (uint32_t*)(eng->d_pmeta + o_ab);
+    eng->d_pacnt  = (uint32_t*)(eng->d_pmeta + o_ac);
+    eng->d_ptiles = (GroupTile*)(eng->d_pmeta + o_t);
     {
         PhaseScope ps(eng, 9, 1);
         if (r.t1 > r.t0 && r.n_win)
@@ -1377,8 +1397,7 @@ int alloc_proposal(csg_engine* eng)
         (rc = dalloc(eng, &eng->d_pcnt, 3 * (L + 1))) || (rc = dalloc(eng, &eng->d_pslot, qc * 8)) ||
         (rc = dalloc(eng, &eng->d_pfb, qc)) || (rc = dalloc(eng, &eng->d_plist, 2 * qc)) || (rc = dalloc(eng, &eng->d_pcells, qc * L)) ||
         (rc = dalloc(eng, &eng->d_pitems, qc * L)) || (rc = dalloc(eng, &eng->d_pseg, qc * L)) ||
-        (rc = dalloc(eng, &eng->d_pabase, L)) || (rc = dalloc(eng, &eng->d_pacnt, L)) ||
-        (rc = dalloc(eng, &eng->d_pcoff, L)))
+        (rc = ensure(eng, &eng->d_pmeta, &eng->pmeta_cap, (size_t)16 * L + 16)))
         return rc;
     return CSG_OK;
 }
@@ -1394,11 +1413,15 @@ void free_proposal(csg_engine* eng)
     dfree(eng->d_pcells);
     dfree(eng->d_pitems);
     dfree(eng->d_pseg);
-    dfree(eng->d_pabase);
-    dfree(eng->d_pacnt);
-    dfree(eng->d_pcoff);
-    dfree(eng->d_ptiles);
-    eng->d_ptiles_cap = 0;
+    dfree(eng->d_pmeta);
+    eng->pmeta_cap = 0;
+    eng->d_pabase = eng->d_pacnt = nullptr;
+    eng->d_pcoff  = nullptr;
+    eng->d_ptiles = nullptr;
+    if (eng->h_pmeta)
+        (void)hipHostFree(eng->h_pmeta);
+    eng->h_pmeta     = nullptr;
+    eng->h_pmeta_cap = 0;
 }
 
 }  // namespace
