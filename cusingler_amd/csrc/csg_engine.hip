@@ -2340,16 +2340,16 @@ int ftd_launch_a(csg_engine* eng)
     const unsigned g = (unsigned)eng->ftd_grid_threads;
     const FtdLds   d = ftd_lds(eng);
     FtdCounts*     cnt = eng->d_fl_cnt;
+    k_ftd_counters<<<1, 64, 0, eng->stream>>>(cnt, T);
     k_ftd_begin<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_next,
-                                                (unsigned long long*)eng->d_fl_htab, T,
+                                                (unsigned long long*)eng->d_fl_htab,
                                                 eng->d_fl_gcount);
-    k_ftd_counters<<<1, 64, 0, eng->stream>>>(cnt);
     k_ftd_hash<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_lists,
                                                eng->d_fl_counts, L,
                                                (unsigned long long*)eng->d_fl_htab,
-                                               eng->d_fl_howner, T, eng->d_fl_cslot);
+                                               eng->d_fl_howner, eng->d_fl_cslot);
     k_ftd_gid<<<g, FTD_TPB, 0, eng->stream>>>(cnt, (const unsigned long long*)eng->d_fl_htab,
-                                              eng->d_fl_howner, T, eng->d_fl_hgid,
+                                              eng->d_fl_howner, eng->d_fl_hgid,
                                               eng->d_fl_gowner);
     k_ftd_member<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_lists,
                                                  eng->d_fl_counts, L, eng->d_fl_cslot,

@@ -53,7 +53,7 @@ struct FtdCounts
     uint32_t n_fused;     // groups of the fused FP32 form
     uint32_t n_etiles;    // exact 16-cell tiles (of n_tiles)
     uint32_t n_btiles;    // exact FTD_BIG-cell tiles (of n_tiles)
-    uint32_t pad_;
+    uint32_t hT;          // hash slots in use this round (a power of two, >= 2 n_active)
     uint64_t n_ref_items;
     uint64_t ref_elems;
     uint64_t cell_elems;
@@ -122,14 +122,16 @@ __global__ void k_ftd_init(const float* __restrict__ mask, uint32_t Nq, uint32_t
         act_next[atomicAdd(&cnt->n_next, 1u)] = c;
 }
 
-// Round start, part 1: the next round's active list becomes this round's;
-// the hash table and the group member counters are cleared.
+// Round start, part 2 (after k_ftd_counters): the previous round's next list
+// becomes this round's active list; the group member counters and the hash
+// slots this round uses are cleared.
 __global__ void k_ftd_begin(const FtdCounts* __restrict__ cnt, uint32_t* __restrict__ act,
                             const uint32_t* __restrict__ act_next,
-                            unsigned long long* __restrict__ htab, uint32_t T,
+                            unsigned long long* __restrict__ htab,
                             uint32_t* __restrict__ gcount)
 {
-    const uint32_t n      = cnt->n_next;
+    const uint32_t n      = cnt->n_active;
+    const uint32_t T      = cnt->hT;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     {
@@ -140,12 +142,19 @@ __global__ void k_ftd_begin(const FtdCounts* __restrict__ cnt, uint32_t* __restr
         htab[s] = 0ull;
 }
 
-// Round start, part 2 (one thread): counters.
-__global__ void k_ftd_counters(FtdCounts* __restrict__ cnt)
+// Round start, part 1 (one thread): counters, and the hash slots of the
+// round: the smallest power of two of at least 2 n_active (and 1024), at most
+// T, so that an insert always finds a free slot.
+__global__ void k_ftd_counters(FtdCounts* __restrict__ cnt, uint32_t T)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0)
         return;
-    cnt->n_active    = cnt->n_next;
+    const uint32_t na = cnt->n_next;
+    uint32_t       hT = 1024;
+    while (hT < 2 * na && hT < T)
+        hT <<= 1;
+    cnt->hT          = hT;
+    cnt->n_active    = na;
     cnt->n_next      = 0;
     cnt->n_groups    = 0;
     cnt->n_cells     = 0;
@@ -173,9 +182,10 @@ __global__ void k_ftd_counters(FtdCounts* __restrict__ cnt)
 __global__ void k_ftd_hash(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
                            const uint8_t* __restrict__ lists, const uint32_t* __restrict__ counts,
                            uint32_t L, unsigned long long* __restrict__ htab,
-                           uint32_t* __restrict__ howner, uint32_t T, uint32_t* __restrict__ cslot)
+                           uint32_t* __restrict__ howner, uint32_t* __restrict__ cslot)
 {
     const uint32_t n = cnt->n_active;
+    const uint32_t T = cnt->hT;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
         const uint32_t c = act[i];
@@ -201,9 +211,10 @@ __global__ void k_ftd_hash(const FtdCounts* __restrict__ cnt, const uint32_t* __
 }
 
 __global__ void k_ftd_gid(FtdCounts* __restrict__ cnt, const unsigned long long* __restrict__ htab,
-                          const uint32_t* __restrict__ howner, uint32_t T,
-                          uint32_t* __restrict__ hgid, uint32_t* __restrict__ gowner)
+                          const uint32_t* __restrict__ howner, uint32_t* __restrict__ hgid,
+                          uint32_t* __restrict__ gowner)
 {
+    const uint32_t T = cnt->hT;
     for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < T; s += gridDim.x * blockDim.x)
         if (htab[s])
         {
