@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--net-cache", default=None,
                     help="load the proposal network's parameters from this .npy file if it "
                          "exists (skipping training), else save them there after training")
-    ap.add_argument("--cpu-sample", type=int, default=1000)
+    ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--only-chosen", action="store_true",
