@@ -2235,6 +2235,15 @@ int ftd_groups(csg_engine* eng, uint32_t n)
 {
     if (n <= eng->ftd_gcap && eng->d_fl_groups)
         return CSG_OK;
+    // the captured rounds name the tables about to move
+    if (eng->ftd_gA)
+        (void)hipGraphExecDestroy(eng->ftd_gA);
+    eng->ftd_gA = nullptr;
+    eng->ftd_sigA.clear();
+    for (auto& b : eng->ftd_gB)
+        if (b.exec)
+            (void)hipGraphExecDestroy(b.exec);
+    eng->ftd_gB.clear();
     const uint32_t cap = std::max(n, eng->ftd_gcap + eng->ftd_gcap / 2);
     uint32_t       T   = 1024;
     while (T < 2 * cap)
@@ -2715,7 +2724,8 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->d_fl_rthin, (uintptr_t)eng->d_fl_rseg, (uintptr_t)eng->d_fl_pred,
              (uintptr_t)eng->d_fl_nan1, (uintptr_t)eng->d_fl_nan2, (uintptr_t)eng->d_fpct,
              (uintptr_t)eng->d_fl_segcell, (uintptr_t)eng->d_fl_refgroup,
-             (uintptr_t)eng->d_fl_refrow,
+             (uintptr_t)eng->d_fl_refrow, (uintptr_t)eng->d_fref_codes8,
+             (uintptr_t)eng->d_fref_codes,
              (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)ftd_fuse(eng),
              (uintptr_t)eng->ftd_mask, (uintptr_t)q,
              (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,
