@@ -2435,18 +2435,34 @@ int ftd_launch_fused(csg_engine* eng, int grid)
 {
     if (!(eng->ftd_mask & FTDM_FUSED) || !ftd_fuse(eng))
         return CSG_OK;
-    const FtdLds d   = ftd_lds(eng);
-    FtdCounts*   cnt = eng->d_fl_cnt;
+    const FtdLds   d   = ftd_lds(eng);
+    FtdCounts*     cnt = eng->d_fl_cnt;
+    const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1);
     if (eng->d_fref_codes8)
-        k_ftd_fused<uint8_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
+    {
+        // byte codes: chunk lengths (0, 4] and (4, 8] with the cells in
+        // registers, longer ones (G_sub above 1024 genes) through the
+        // LDS-table kernel
+        k_ftd_fused_r<4, 0><<<(unsigned)grid, 256, 0, eng->stream>>>(
             cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-            eng->d_fl_ggenes, eng->d_fref_codes8, std::max<uint32_t>(eng->Gf, 1), eng->fnbins,
-            d.wave_ints, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_scores);
+            eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->d_ft_cr, eng->d_ft_csq,
+            eng->d_ft_scores);
+        if (Gf > 512)
+            k_ftd_fused_r<8, 4><<<(unsigned)grid, 256, 0, eng->stream>>>(
+                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+                eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->d_ft_cr, eng->d_ft_csq,
+                eng->d_ft_scores);
+        if (Gf > 1024)
+            k_ftd_fused<uint8_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
+                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+                eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->fnbins, d.wave_ints, eng->d_ft_cr,
+                eng->d_ft_csq, eng->d_ft_scores, 8);
+    }
     else
         k_ftd_fused<uint16_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
             cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-            eng->d_fl_ggenes, eng->d_fref_codes, std::max<uint32_t>(eng->Gf, 1), eng->fnbins,
-            d.wave_ints, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_scores);
+            eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, d.wave_ints, eng->d_ft_cr,
+            eng->d_ft_csq, eng->d_ft_scores, 0);
     LAUNCHCHK();
     return CSG_OK;
 }

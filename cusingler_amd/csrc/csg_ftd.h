@@ -665,6 +665,30 @@ __global__ void k_ftd_prep(const FtdCounts* __restrict__ cnt, const uint32_t* __
 // ---- work-queue kernels: the grid covers the machine, items come from the
 // ---- device-side counts
 
+// The last step of a wave's rank table: bins[0] = the zero count (the zero
+// code is counted by ballots, never by the LDS atomics), then the code
+// counts become bins[c] = 2 (#less) + (#equal) - gs by a wave scan (lane l
+// sums a contiguous run of ceil(nbins / 64) bins).
+__device__ inline void ftd_bins_to_ranks(int* bins, int nbins, int gs, int zeros, int lane)
+{
+    if (lane == 0)
+        bins[0] = zeros;
+    __builtin_amdgcn_wave_barrier();
+    const int per = (nbins + 63) / 64;
+    const int b0 = min(lane * per, nbins), b1 = min(b0 + per, nbins);
+    int       s  = 0;
+    for (int i = b0; i < b1; ++i)
+        s += bins[i];
+    int start = wave_incl_scan(s) - s;
+    for (int i = b0; i < b1; ++i)
+    {
+        const int c = bins[i];
+        bins[i]     = 2 * start + c - gs;
+        start += c;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Doubled centred rank table of one reference row over G_sub, built by one
 // wave in LDS (codes stored as CT: 8 bits when every code is below 256, else
 // 16): codes[i] = the row's code at gene gl[i] (all gathered first,
@@ -690,22 +714,7 @@ __device__ inline void ftd_row_table(const CT* __restrict__ src, const uint32_t*
         if (i < gs && c != 0)
             atomicAdd(&bins[c], 1);
     }
-    if (lane == 0)
-        bins[0] = zeros;  // no atomic touched bin 0
-    __builtin_amdgcn_wave_barrier();
-    const int per = (nbins + 63) / 64;
-    const int b0 = min(lane * per, nbins), b1 = min(b0 + per, nbins);
-    int       s  = 0;
-    for (int i = b0; i < b1; ++i)
-        s += bins[i];
-    int start = wave_incl_scan(s) - s;
-    for (int i = b0; i < b1; ++i)
-    {
-        const int c = bins[i];
-        bins[i]     = 2 * start + c - gs;
-        start += c;
-    }
-    __builtin_amdgcn_wave_barrier();
+    ftd_bins_to_ranks(bins, nbins, gs, zeros, lane);
 }
 
 // Reference rows, one wave per row (4 rows per block, no block barrier): the
@@ -786,7 +795,8 @@ __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
 // shuffle levels complete the tree), the row's sum of squares and its dot
 // product with each of the group's cell rank rows, and stores the group's
 // scores corr_from_sums(sxy, sxx, syy) -- the same values the thin tile
-// computes from stored ranks.
+// computes from stored ranks.  Takes the fused groups whose chunk length
+// ceil(gs / 128) exceeds step_lo (the others are k_ftd_fused_r's).
 template <class CT>
 __global__ __launch_bounds__(256) void k_ftd_fused(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
@@ -794,7 +804,7 @@ __global__ __launch_bounds__(256) void k_ftd_fused(
     const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
     const CT* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
     const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
-    float* __restrict__ scores)
+    float* __restrict__ scores, int step_lo)
 {
     extern __shared__ int smem[];
     const int      wid   = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -809,6 +819,8 @@ __global__ __launch_bounds__(256) void k_ftd_fused(
         const FtGroup  g     = groups[gi];
         const uint32_t local = item - g.ref_item0;
         const int      gs    = (int)g.gs;
+        if ((gs + 127) / 128 <= step_lo)
+            continue;  // uniform per wave
         ftd_row_table(ref_codes + (size_t)refrow[item] * Gf, genes + g.gene_off, gs, nbins, bins,
                       codes, lane);
         const int      nc   = (int)min(g.ncell, (uint32_t)FTD_FUSE_MAX);
@@ -851,6 +863,196 @@ __global__ __launch_bounds__(256) void k_ftd_fused(
                     corr_from_sums(sxy, cell_sq[g.cell_item0 + c], syy);
         }
         __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
+    }
+}
+
+// Most consecutive reference rows one wave of k_ftd_fused_r takes at a time.
+constexpr int FTD_FUSED_RUN = 16;
+
+// Two cells' rank values of one element, in half precision: a doubled
+// centred rank over at most 1024 genes has |x| <= 1025, which FP16 holds
+// exactly.
+typedef _Float16 ftd_h2 __attribute__((ext_vector_type(2)));
+
+// Chunk sums of one row for a group of NC cells (lane l: chunks 2l + h of
+// `step` elements, element (2l + h) step + k in slot [h][k], bit h STEP + k
+// of `ok` set for the slots inside G_sub).  A slot outside adds y = 0: its
+// fma leaves the sums' bits as they are (no sum here is ever -0), so the
+// sums are those of the sequential loops over the valid elements.
+template <int STEP, int NC>
+__device__ inline void ftd_fused_row(const int* bins, const uint32_t (&code)[2][STEP],
+                                     const ftd_h2 (&x)[2][STEP][FTD_FUSE_MAX / 2], uint32_t ok,
+                                     float& syy, float (&sxy)[FTD_FUSE_MAX])
+{
+    float q[2] = { 0.f, 0.f };
+    float d[2][NC];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            d[h][c] = 0.f;
+#pragma unroll
+        for (int k = 0; k < STEP; ++k)
+        {
+            const int   v = bins[code[h][k]];
+            const float y = (ok >> (h * STEP + k)) & 1u ? (float)v : 0.f;
+            q[h]          = __builtin_fmaf(y, y, q[h]);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                d[h][c] = __builtin_fmaf((float)x[h][k][c >> 1][c & 1], y, d[h][c]);
+        }
+    }
+    syy = q[0] + q[1];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        sxy[c] = d[0][c] + d[1][c];
+}
+
+// Fused FP32 form with the cells held in registers, for byte codes (at most
+// 256 code bins): the scores of k_ftd_fused (the same rank tables, chunk
+// sums and tree, so the same bits) for the fused groups whose chunk length
+// ceil(gs / 128) lies in (STEP_LO, STEP].  A wave takes runs of consecutive
+// reference items (one vector load of their groups and flags, a ballot of
+// the fused ones): up to FTD_FUSED_RUN items, fewer when the items would
+// leave waves idle (a round of a few cells has a few thousand rows).
+// Consecutive rows of a group share its gene list and cells, so a lane
+// loads its chunk elements' genes and the cells' rank values once per group
+// and keeps them in registers (the ranks as FP16 pairs, exact for the at
+// most 1024 genes these groups have).  Per row: the row's codes gathered at the
+// lane's chunk elements, counted into the wave's 256 LDS bins (ballots for
+// code 0, LDS atomics for the rest), the counts turned into rank values by
+// a DPP scan (lane l holds bins 4l .. 4l + 3: one 16-byte read and write),
+// the chunk sums accumulated from the codes the lane still holds, and the
+// trees completed by DPP (wave_tree_sum) -- no LDS traffic beyond the bins.
+template <int STEP, int STEP_LO>
+__global__ __launch_bounds__(256) void k_ftd_fused_r(
+    const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
+    const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
+    const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
+    const uint8_t* __restrict__ ref_codes, uint32_t Gf, const int16_t* __restrict__ cell_rank,
+    const float* __restrict__ cell_sq, float* __restrict__ scores)
+{
+    __shared__ __attribute__((aligned(16))) int sbins[4][256];
+    const int      wid   = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int*           bins  = sbins[wid];
+    int4*          bins4 = (int4*)bins;
+    const uint64_t n     = cnt->n_ref_items;
+    uint32_t       cur   = 0xFFFFFFFFu;
+    bool           mine  = false;
+    FtGroup        g{};
+    int            gs = 0, nc = 0;
+    uint32_t       ok = 0;
+    uint32_t       gl[2][STEP];
+    ftd_h2         x[2][STEP][FTD_FUSE_MAX / 2];
+    const uint64_t waves = (uint64_t)gridDim.x * 4;
+    const int      rl    = (int)min((uint64_t)FTD_FUSED_RUN, max((uint64_t)1, n / (2 * waves)));
+    for (uint64_t run = (uint64_t)blockIdx.x * 4 + wid; run * rl < n; run += waves)
+    {
+        const uint64_t base = run * rl;
+        uint32_t       rg   = 0;
+        bool           fz   = false;
+        if (lane < rl && base + lane < n)
+        {
+            rg = refgroup[base + lane];
+            fz = (gflag[rg] & FTD_FUSED) != 0;
+        }
+        uint64_t m = __ballot(fz);
+        while (m)
+        {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1;
+            const uint32_t gi   = __builtin_amdgcn_readlane(rg, k);
+            const uint64_t item = base + (uint64_t)k;
+            if (gi != cur)
+            {
+                cur            = gi;
+                g              = groups[gi];
+                gs             = (int)g.gs;
+                const int step = (gs + 127) / 128;
+                nc             = (int)min(g.ncell, (uint32_t)FTD_FUSE_MAX);
+                mine           = step > STEP_LO && step <= STEP;
+                if (mine)
+                {
+                    const int16_t* xc = cell_rank + g.cell_dst0;
+                    const int      st = (int)g.stride;
+                    ok                = 0;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int j = 0; j < STEP; ++j)
+                        {
+                            const int  i = (2 * lane + h) * step + j;
+                            const bool v = j < step && i < gs;
+                            ok |= (uint32_t)v << (h * STEP + j);
+                            gl[h][j] = v ? genes[g.gene_off + i] : 0u;
+#pragma unroll
+                            for (int c = 0; c < FTD_FUSE_MAX; ++c)
+                                x[h][j][c >> 1][c & 1] =
+                                    (_Float16)(v && c < nc ? xc[c * st + i] : (int16_t)0);
+                        }
+                }
+            }
+            if (!mine)
+                continue;
+            const uint8_t* src = ref_codes + (size_t)refrow[item] * Gf;
+            bins4[lane]        = make_int4(0, 0, 0, 0);
+            uint32_t code[2][STEP];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int j = 0; j < STEP; ++j)
+                    code[h][j] = src[gl[h][j]];  // gene 0 (in the row) for slots outside
+            __builtin_amdgcn_wave_barrier();
+            int zeros = 0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int j = 0; j < STEP; ++j)
+                {
+                    const bool v = (ok >> (h * STEP + j)) & 1u;
+                    zeros += __builtin_popcountll(__ballot(v && code[h][j] == 0));
+                    if (v && code[h][j] != 0)
+                        atomicAdd(&bins[code[h][j]], 1);
+                }
+            __builtin_amdgcn_wave_barrier();
+            int4 b = bins4[lane];
+            if (lane == 0)
+                b.x = zeros;  // no atomic touched bin 0
+            const int s  = b.x + b.y + b.z + b.w;
+            int       ex = wave_incl_scan_dpp(s) - s;
+            int4      r;
+            r.x = 2 * ex + b.x - gs;
+            ex += b.x;
+            r.y = 2 * ex + b.y - gs;
+            ex += b.y;
+            r.z = 2 * ex + b.z - gs;
+            ex += b.z;
+            r.w         = 2 * ex + b.w - gs;
+            bins4[lane] = r;
+            __builtin_amdgcn_wave_barrier();
+            float syy, sxy[FTD_FUSE_MAX];
+            switch (nc)
+            {
+            case 1: ftd_fused_row<STEP, 1>(bins, code, x, ok, syy, sxy); break;
+            case 2: ftd_fused_row<STEP, 2>(bins, code, x, ok, syy, sxy); break;
+            case 3: ftd_fused_row<STEP, 3>(bins, code, x, ok, syy, sxy); break;
+            default: ftd_fused_row<STEP, 4>(bins, code, x, ok, syy, sxy); break;
+            }
+            syy                  = wave_tree_sum(syy);
+            const uint64_t local = item - g.ref_item0;
+#pragma unroll
+            for (int c = 0; c < FTD_FUSE_MAX; ++c)
+            {
+                if (c >= nc)
+                    break;
+                const float v = wave_tree_sum(sxy[c]);
+                if (lane == 0)
+                    scores[g.score0 + (uint64_t)c * g.nref + local] =
+                        corr_from_sums(v, cell_sq[g.cell_item0 + c], syy);
+            }
+            __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
+        }
     }
 }
 

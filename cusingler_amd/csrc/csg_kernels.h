@@ -35,6 +35,53 @@ __device__ inline int wave_incl_scan(int v)
     return v;
 }
 
+// DPP lane moves within rows of 16 lanes (no LDS traffic): row_shr:n gives
+// lane l the value of lane l - n, row_shl:n that of lane l + n; a source
+// outside the row reads 0.
+#define CSG_DPP_ROW_SHR(v, n) __builtin_amdgcn_update_dpp(0, (v), 0x110 + (n), 0xF, 0xF, true)
+#define CSG_DPP_ROW_SHL(v, n) __builtin_amdgcn_update_dpp(0, (v), 0x100 + (n), 0xF, 0xF, true)
+
+// Inclusive prefix sum over the wave by DPP: Hillis-Steele steps 1, 2, 4, 8
+// inside each row of 16, then the totals of the lower rows (lanes 15, 31,
+// 47) added.  Every lane of the wave must be active.
+__device__ inline int wave_incl_scan_dpp(int v)
+{
+    v += CSG_DPP_ROW_SHR(v, 1);
+    v += CSG_DPP_ROW_SHR(v, 2);
+    v += CSG_DPP_ROW_SHR(v, 4);
+    v += CSG_DPP_ROW_SHR(v, 8);
+    const int r0  = __builtin_amdgcn_readlane(v, 15);
+    const int r1  = __builtin_amdgcn_readlane(v, 31);
+    const int r2  = __builtin_amdgcn_readlane(v, 47);
+    const int row = (threadIdx.x & (kWave - 1)) >> 4;
+    return v + (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
+}
+
+// Lane 0's result of the shuffle-down tree `for (o = 1; o < 64; o *= 2)
+// v += shfl_down(v, o)`, returned to every lane: lane l + o's partial is
+// added to lane l's at each level, so lane 0 ends with (T0 + T1) + (T2 + T3)
+// where Tk is the same pairwise tree over row k (lanes 16k .. 16k + 15).
+// The rows' trees are built by DPP row shifts toward lane 0, the four row
+// totals read from lanes 0, 16, 32 and 48: the same additions of the same
+// operands, so the same bits.  Every lane of the wave must be active.
+__device__ inline float wave_tree_sum(float v)
+{
+    int b = __builtin_bit_cast(int, v);
+    v += __builtin_bit_cast(float, CSG_DPP_ROW_SHL(b, 1));
+    b = __builtin_bit_cast(int, v);
+    v += __builtin_bit_cast(float, CSG_DPP_ROW_SHL(b, 2));
+    b = __builtin_bit_cast(int, v);
+    v += __builtin_bit_cast(float, CSG_DPP_ROW_SHL(b, 4));
+    b = __builtin_bit_cast(int, v);
+    v += __builtin_bit_cast(float, CSG_DPP_ROW_SHL(b, 8));
+    b              = __builtin_bit_cast(int, v);
+    const float t0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 0));
+    const float t1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 16));
+    const float t2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 32));
+    const float t3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 48));
+    return (t0 + t1) + (t2 + t3);
+}
+
 __device__ inline int wave_sum(int v)
 {
 #pragma unroll
