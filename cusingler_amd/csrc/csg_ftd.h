@@ -1020,7 +1020,7 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
             if (lane == 0)
                 b.x = zeros;  // no atomic touched bin 0
             const int s  = b.x + b.y + b.z + b.w;
-            int       ex = wave_incl_scan_dpp(s) - s;
+            int       ex = wave_incl_scan(s) - s;
             int4      r;
             r.x = 2 * ex + b.x - gs;
             ex += b.x;

@@ -22,29 +22,19 @@ constexpr int kWave = 64;
 
 // ------------------------------------------------------------- utilities ---
 
-__device__ inline int wave_incl_scan(int v)
-{
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1)
-    {
-        int t = __shfl_up(v, d, kWave);
-        if (lane >= d)
-            v += t;
-    }
-    return v;
-}
-
 // DPP lane moves within rows of 16 lanes (no LDS traffic): row_shr:n gives
 // lane l the value of lane l - n, row_shl:n that of lane l + n; a source
 // outside the row reads 0.
 #define CSG_DPP_ROW_SHR(v, n) __builtin_amdgcn_update_dpp(0, (v), 0x110 + (n), 0xF, 0xF, true)
 #define CSG_DPP_ROW_SHL(v, n) __builtin_amdgcn_update_dpp(0, (v), 0x100 + (n), 0xF, 0xF, true)
+// row_shr:n where a lane whose source is outside the row keeps its own value
+#define CSG_DPP_ROW_SHR_KEEP(v, n) \
+    __builtin_amdgcn_update_dpp((v), (v), 0x110 + (n), 0xF, 0xF, false)
 
 // Inclusive prefix sum over the wave by DPP: Hillis-Steele steps 1, 2, 4, 8
 // inside each row of 16, then the totals of the lower rows (lanes 15, 31,
 // 47) added.  Every lane of the wave must be active.
-__device__ inline int wave_incl_scan_dpp(int v)
+__device__ inline int wave_incl_scan(int v)
 {
     v += CSG_DPP_ROW_SHR(v, 1);
     v += CSG_DPP_ROW_SHR(v, 2);
@@ -82,12 +72,16 @@ __device__ inline float wave_tree_sum(float v)
     return (t0 + t1) + (t2 + t3);
 }
 
+// Wave sum, in every lane: row totals by DPP, then the four rows' totals
+// (lanes 15, 31, 47, 63).  Every lane of the wave must be active.
 __device__ inline int wave_sum(int v)
 {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1)
-        v += __shfl_xor(v, d, kWave);
-    return v;
+    v += CSG_DPP_ROW_SHR(v, 1);
+    v += CSG_DPP_ROW_SHR(v, 2);
+    v += CSG_DPP_ROW_SHR(v, 4);
+    v += CSG_DPP_ROW_SHR(v, 8);
+    return __builtin_amdgcn_readlane(v, 15) + __builtin_amdgcn_readlane(v, 31) +
+           __builtin_amdgcn_readlane(v, 47) + __builtin_amdgcn_readlane(v, 63);
 }
 
 __device__ inline long long wave_sum64(long long v)
@@ -98,20 +92,53 @@ __device__ inline long long wave_sum64(long long v)
     return v;
 }
 
+// Wave minimum / maximum, in every lane: row results by DPP (a lane whose
+// source lies outside its row compares with itself), then the four rows'
+// results.  Every lane of the wave must be active.
 __device__ inline float wave_min(float v)
 {
 #pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1)
-        v = fminf(v, __shfl_xor(v, d, kWave));
-    return v;
+    for (int s = 0; s < 4; ++s)
+    {
+        const int b = __builtin_bit_cast(int, v);
+        int       t;
+        switch (s)
+        {
+        case 0: t = CSG_DPP_ROW_SHR_KEEP(b, 1); break;
+        case 1: t = CSG_DPP_ROW_SHR_KEEP(b, 2); break;
+        case 2: t = CSG_DPP_ROW_SHR_KEEP(b, 4); break;
+        default: t = CSG_DPP_ROW_SHR_KEEP(b, 8); break;
+        }
+        v = fminf(v, __builtin_bit_cast(float, t));
+    }
+    const int b = __builtin_bit_cast(int, v);
+    return fminf(fminf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 15)),
+                       __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 31))),
+                 fminf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 47)),
+                       __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 63))));
 }
 
 __device__ inline float wave_max(float v)
 {
 #pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1)
-        v = fmaxf(v, __shfl_xor(v, d, kWave));
-    return v;
+    for (int s = 0; s < 4; ++s)
+    {
+        const int b = __builtin_bit_cast(int, v);
+        int       t;
+        switch (s)
+        {
+        case 0: t = CSG_DPP_ROW_SHR_KEEP(b, 1); break;
+        case 1: t = CSG_DPP_ROW_SHR_KEEP(b, 2); break;
+        case 2: t = CSG_DPP_ROW_SHR_KEEP(b, 4); break;
+        default: t = CSG_DPP_ROW_SHR_KEEP(b, 8); break;
+        }
+        v = fmaxf(v, __builtin_bit_cast(float, t));
+    }
+    const int b = __builtin_bit_cast(int, v);
+    return fmaxf(fmaxf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 15)),
+                       __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 31))),
+                 fmaxf(__builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 47)),
+                       __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 63))));
 }
 
 // Correlation from the three FP32 sums, in the reference's operation order:
@@ -798,7 +825,7 @@ __device__ inline void wave_select2(const float* __restrict__ v, uint32_t n, uin
 // lane holds its values in registers (slot u is element lane + 64 u), the
 // range [lo, hi] is cut into 64 buckets (one per lane, monotone in value),
 // the two target ranks are located by a prefix over the bucket counts, and the
-// values of the buckets that hold them are ranked exactly in LDS.  While those
+// values of the buckets that hold them are ranked exactly across the lanes.  While those
 // buckets hold more than 64 values the range narrows to them and repeats; a
 // last level ranks the candidates against the whole segment.
 template <int R>
@@ -840,8 +867,9 @@ __device__ inline void wave_select2_reg(const float (&x)[R], const float* __rest
         const unsigned long long m1 = __ballot((uint32_t)ex <= k1 && k1 < (uint32_t)(ex + c));
         const int blo = m0 ? __builtin_ctzll(m0) : 0;
         const int bhi = m1 ? __builtin_ctzll(m1) : 63;
-        const int cb  = __shfl(ex, blo, kWave) - (int)below;  // values below bucket blo
-        const int cc  = __shfl(incl, bhi, kWave) - __shfl(incl - c, blo, kWave);
+        // values below bucket blo, and in buckets blo .. bhi
+        const int cb = __builtin_amdgcn_readlane(ex, blo) - (int)below;
+        const int cc = __builtin_amdgcn_readlane(incl, bhi) - __builtin_amdgcn_readlane(incl - c, blo);
         const bool last = cc <= kWave || level >= kMaxLevel;
         if (last)
         {
@@ -849,36 +877,42 @@ __device__ inline void wave_select2_reg(const float (&x)[R], const float* __rest
             float v0 = 0.f, v1 = 0.f;
             if (cc <= kWave)
             {
-                if (lane == 0)
-                    sm.ncand = 0;
-                __builtin_amdgcn_wave_barrier();
+                // the candidates compacted into lanes 0 .. cc - 1 (ballot
+                // prefix counts give the slots), then each ranked against
+                // all of them by reading the other lanes' values
+                uint32_t pos = 0;
 #pragma unroll
                 for (int u = 0; u < R; ++u)
-                    if (b[u] >= blo && b[u] <= bhi)
-                        sm.cand[atomicAdd(&sm.ncand, 1)] = x[u];
+                {
+                    const bool               f  = b[u] >= blo && b[u] <= bhi;
+                    const unsigned long long mk = __ballot(f);
+                    if (f)
+                        sm.cand[pos + __builtin_amdgcn_mbcnt_hi(
+                                          (uint32_t)(mk >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u))] = x[u];
+                    pos += (uint32_t)__builtin_popcountll(mk);
+                }
                 __builtin_amdgcn_wave_barrier();
                 const uint32_t base = below + (uint32_t)cb;
-                if (lane < cc)
+                const float    xv   = lane < cc ? sm.cand[lane] : 0.f;
+                const int      xb   = __builtin_bit_cast(int, xv);
+                uint32_t       less = 0, eq = 0;
+                for (int j = 0; j < cc; ++j)
                 {
-                    const float xv   = sm.cand[lane];
-                    uint32_t    less = 0, eq = 0;
-                    for (int j = 0; j < cc; ++j)
-                    {
-                        const float y = sm.cand[j];
-                        less += y < xv;
-                        eq += y == xv;
-                    }
-                    const uint32_t p0 = base + less, p1 = p0 + eq;
-                    if (p0 <= k0 && k0 < p1)
-                    {
-                        hit0 = true;
-                        v0   = xv;
-                    }
-                    if (p0 <= k1 && k1 < p1)
-                    {
-                        hit1 = true;
-                        v1   = xv;
-                    }
+                    const float y = __builtin_bit_cast(float, __builtin_amdgcn_readlane(xb, j));
+                    less += y < xv;
+                    eq += y == xv;
+                }
+                const uint32_t p0 = base + less, p1 = p0 + eq;
+                if (lane < cc && p0 <= k0 && k0 < p1)
+                {
+                    hit0 = true;
+                    v0   = xv;
+                }
+                if (lane < cc && p0 <= k1 && k1 < p1)
+                {
+                    hit1 = true;
+                    v1   = xv;
                 }
             }
             else
@@ -909,8 +943,10 @@ __device__ inline void wave_select2_reg(const float (&x)[R], const float* __rest
             }
             const unsigned long long f0 = __ballot(hit0);
             const unsigned long long f1 = __ballot(hit1);
-            r0 = __shfl(v0, f0 ? __builtin_ctzll(f0) : 0, kWave);
-            r1 = __shfl(v1, f1 ? __builtin_ctzll(f1) : 0, kWave);
+            r0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v0),
+                                                                     f0 ? __builtin_ctzll(f0) : 0));
+            r1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v1),
+                                                                     f1 ? __builtin_ctzll(f1) : 0));
             break;
         }
         float nlo = __builtin_inff(), nhi = -__builtin_inff();
