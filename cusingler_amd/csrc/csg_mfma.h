@@ -231,7 +231,9 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
 // of bank conflicts.
 constexpr int W8_BM = 128, W8_BN = 256, W8_BK = 64, W8_NS = 3;
 constexpr int W8_STAGE = (2 * W8_BM + 2 * W8_BN) * W8_BK;  // 48 KB
-constexpr int W8_LDS   = W8_NS * W8_STAGE;                  // dynamic LDS per block
+// the tile's row and column inverse roots (doubles) follow the stages
+constexpr int W8_IR_OFF = W8_NS * W8_STAGE;
+constexpr int W8_LDS    = W8_IR_OFF + (W8_BM + W8_BN) * 8;  // dynamic LDS per block
 
 __device__ inline v4i times16(v4i h)
 {
@@ -257,7 +259,8 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n)
 // Row / column maps of one 128 x 256 block of the wide kernel.  a_row(r) and
 // b_row(r) give the operand rows staged for block rows / columns r (always
 // inside the operand buffers); row_ok / col_ok say which results are stored;
-// row_ir / col_ir are the rows' inverse roots 1/sqrt(sum X^2).
+// row_ir_at / col_ir_at give the inverse root 1/sqrt(sum X^2) of an A / B row
+// (indexed like a_row / b_row).
 struct DenseW8Map
 {
     int           m0, n0, nq, nr;
@@ -268,10 +271,10 @@ struct DenseW8Map
     __device__ size_t a_row(int r) const { return (size_t)(m0 + r); }
     __device__ size_t b_row(int r) const { return (size_t)(n0 + r); }
     __device__ bool   row_ok(int m) const { return m0 + m < nq; }
-    __device__ double row_ir(int m) const { return ir_q[m0 + m]; }
+    __device__ double row_ir_at(size_t a) const { return ir_q[a]; }
     __device__ float* row_out(int m) const { return S + (size_t)(m0 + m) * ldS + n0; }
     __device__ bool   col_ok(int n) const { return n0 + n < nr; }
-    __device__ double col_ir(int n) const { return ir_r[n0 + n]; }
+    __device__ double col_ir_at(size_t b) const { return ir_r[b]; }
 };
 
 // One label of a grouped contraction: the query rows listed in cells[0, mcount)
@@ -290,10 +293,10 @@ struct GroupW8Map
     __device__ size_t a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
     __device__ size_t b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
     __device__ bool   row_ok(int m) const { return m < mcount; }
-    __device__ double row_ir(int m) const { return ir_q[cells[m]]; }
+    __device__ double row_ir_at(size_t a) const { return ir_q[a]; }
     __device__ float* row_out(int m) const { return out + (size_t)m * ld; }
     __device__ bool   col_ok(int n) const { return n < ncount; }
-    __device__ double col_ir(int n) const { return ir_r[b0 + n]; }
+    __device__ double col_ir_at(size_t b) const { return ir_r[b]; }
 };
 
 // Wave `wid`'s six direct-to-LDS loads of one stage: row group g = wid + 8 u
@@ -330,13 +333,16 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
     const int wm   = wid >> 2;  // 0..1: 64-row half of the block
     const int wn   = wid & 3;   // 0..3: 64-column quarter
 
-    size_t src[6];
+    // A rows: both planes take row rq; B rows: rq and rq + 128
+    const int    rq   = wid * 16 + (lane >> 2);
+    const size_t arow = map.a_row(rq);
+    size_t       src[6];
 #pragma unroll
     for (int u = 0; u < 6; ++u)
     {
         const int g = wid + 8 * u;
         const int r = (g < 16 ? (g & 7) : ((g - 16) & 15)) * 16 + (lane >> 2);  // row in its plane
-        src[u]      = (g < 16 ? map.a_row(r) : map.b_row(r)) * (size_t)Kp +
+        src[u]      = (g < 16 ? arow : map.b_row(r)) * (size_t)Kp +
                  (size_t)(((lane & 3) ^ w8_swz(r)) * 16);
     }
     v4i acc_x[4][4], acc_ll[4][4];
@@ -348,8 +354,18 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             acc_x[i][j]  = v4i{ 0, 0, 0, 0 };
             acc_ll[i][j] = v4i{ 0, 0, 0, 0 };
         }
+    // inverse roots of the rows this lane's loads fetch (A row rq, B rows rq
+    // and rq + 128, the same clamped row indices), loaded with them -- the
+    // pipeline's counted waits only ever cover more loads than their own --
+    // and stored to LDS after the K loop for the epilogue
     const int T = Kp / W8_BK;
     w8_issue(qh, ql, rh, rl, src, sm, wid, 0);
+    // (between the stages: complete by the first stage's wait)
+    asm volatile("" ::: "memory");
+    const double ir_a  = map.row_ir_at(arow);
+    const double ir_b0 = map.col_ir_at(map.b_row(rq));
+    const double ir_b1 = map.col_ir_at(map.b_row(rq + 128));
+    asm volatile("" ::: "memory");
     if (T > 1)
         w8_issue(qh, ql, rh, rl, src, sm + W8_STAGE, wid, W8_BK);
     const int r16 = lane & 15;
@@ -397,6 +413,23 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             }
         }
     }
+    double* s_ir = (double*)(sm + W8_IR_OFF);
+    if ((lane & 3) == 0)
+    {
+        s_ir[rq]               = ir_a;
+        s_ir[W8_BM + rq]       = ir_b0;
+        s_ir[W8_BM + 128 + rq] = ir_b1;
+    }
+    __syncthreads();
+    double rir[4][4], cir[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        cir[j] = s_ir[W8_BM + wn * 64 + j * 16 + r16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            rir[i][e] = s_ir[wm * 64 + i * 16 + (lane >> 4) * 4 + e];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -405,8 +438,7 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             const int m = wm * 64 + i * 16 + (lane >> 4) * 4 + e;
             if (!map.row_ok(m))
                 continue;
-            const double ir  = map.row_ir(m);
-            float*       out = map.row_out(m);
+            float* out = map.row_out(m);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
             {
@@ -414,7 +446,7 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
                 if (!map.col_ok(n))
                     continue;
                 const double s4 = (double)acc_x[i][j][e] * 256.0 + (double)acc_ll[i][j][e];
-                __builtin_nontemporal_store(exact_corr(s4, ir, map.col_ir(n)), out + n);
+                __builtin_nontemporal_store(exact_corr(s4, rir[i][e], cir[j]), out + n);
             }
         }
 }
