@@ -763,22 +763,42 @@ __device__ inline void ft_mfma_tile_t(const FtCorrTile& tl, const FtGroup* __res
         hi = *(const int2*)h;
         lo = *(const int2*)l;
     };
+    // the K loop keeps the next step's rank chunks in registers while the
+    // current step's MFMAs run (loads of step k0 + 64 overlap them)
+    constexpr int NA = (TC * 8 + 255) / 256;
+    int4          pa[NA], pb[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u)
+        {
+            const int c = tid + u * 256, row = c >> 3, ch = c & 7, k = k0 + ch * 8;
+            pa[u]       = int4{ 0, 0, 0, 0 };
+            if (c < TC * 8 && row < ncell && k < st)
+                pa[u] = *(const int4*)(cbase + (uint64_t)row * st + k);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const int c = tid + u * 256, row = c >> 3, ch = c & 7, k = k0 + ch * 8;
+            pb[u]       = int4{ 0, 0, 0, 0 };
+            if (row < nref && k < st)
+                pb[u] = *(const int4*)(rbase + (uint64_t)row * st + k);
+        }
+    };
+    fetch(0);
     for (int k0 = 0; k0 < st; k0 += 64)
     {
         __syncthreads();
         // A: TC rows x 8 chunks of 8 values
 #pragma unroll
-        for (int u = 0; u < (TC * 8 + 255) / 256; ++u)
+        for (int u = 0; u < NA; ++u)
         {
             const int c = tid + u * 256;
             if (c >= TC * 8)
                 break;
-            const int row = c >> 3, ch = c & 7, k = k0 + ch * 8;
-            int4      v   = int4{ 0, 0, 0, 0 };
-            if (row < ncell && k < st)
-                v = *(const int4*)(cbase + (uint64_t)row * st + k);
-            int2 hi, lo;
-            split8(v, hi, lo);
+            const int row = c >> 3, ch = c & 7;
+            int2      hi, lo;
+            split8(pa[u], hi, lo);
             *(int2*)(&sa[0][row * LD + ch * 8]) = hi;
             *(int2*)(&sa[1][row * LD + ch * 8]) = lo;
         }
@@ -786,16 +806,15 @@ __device__ inline void ft_mfma_tile_t(const FtCorrTile& tl, const FtGroup* __res
 #pragma unroll
         for (int u = 0; u < 8; ++u)
         {
-            const int c = tid + u * 256, row = c >> 3, ch = c & 7, k = k0 + ch * 8;
-            int4      v = int4{ 0, 0, 0, 0 };
-            if (row < nref && k < st)
-                v = *(const int4*)(rbase + (uint64_t)row * st + k);
-            int2 hi, lo;
-            split8(v, hi, lo);
+            const int c = tid + u * 256, row = c >> 3, ch = c & 7;
+            int2      hi, lo;
+            split8(pb[u], hi, lo);
             *(int2*)(&sb[0][row * LD + ch * 8]) = hi;
             *(int2*)(&sb[1][row * LD + ch * 8]) = lo;
         }
         __syncthreads();
+        if (k0 + 64 < st)
+            fetch(k0 + 64);
         v4i_ft ah[NI], al[NI], ah16[NI];
 #pragma unroll
         for (int i = 0; i < NI; ++i)
@@ -826,6 +845,23 @@ __device__ inline void ft_mfma_tile_t(const FtCorrTile& tl, const FtGroup* __res
             }
         }
     }
+    // the epilogue's sums of squares (rows and columns outside the tile read
+    // row / column 0), all loaded before any is used
+    float csq[NI][4], rsq[4];
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = i * 16 + (lane >> 4) * 4 + e;
+            csq[i][e]   = cell_sq[g.cell_item0 + tl.cell0 + (m < ncell ? m : 0)];
+        }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+        const int n = wid * 64 + j * 16 + r16;
+        rsq[j]      = ref_sq[g.ref_item0 + tl.ref0 + (n < nref ? n : 0)];
+    }
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
@@ -834,8 +870,7 @@ __device__ inline void ft_mfma_tile_t(const FtCorrTile& tl, const FtGroup* __res
             const int m = i * 16 + (lane >> 4) * 4 + e;
             if (m >= ncell)
                 continue;
-            const float sq  = cell_sq[g.cell_item0 + tl.cell0 + m];
-            float*      out = scores + g.score0 + (uint64_t)(tl.cell0 + m) * g.nref + tl.ref0;
+            float* out = scores + g.score0 + (uint64_t)(tl.cell0 + m) * g.nref + tl.ref0;
 #pragma unroll
             for (int j = 0; j < 4; ++j)
             {
@@ -843,7 +878,7 @@ __device__ inline void ft_mfma_tile_t(const FtCorrTile& tl, const FtGroup* __res
                 if (n >= nref)
                     continue;
                 const long long s4   = ((long long)acc_x[i][j][e] << 8) + (long long)acc_ll[i][j][e];
-                const float     prod = sq * ref_sq[g.ref_item0 + tl.ref0 + n];
+                const float     prod = csq[i][e] * rsq[j];
                 const float     d    = sqrtf(prod);
                 out[n] = d != 0.f ? (float)((double)s4 / (double)d) : __builtin_nanf("");
             }

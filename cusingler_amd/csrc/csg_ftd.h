@@ -1117,7 +1117,7 @@ __global__ __launch_bounds__(256) void k_ftd_mfma(
 }
 
 // Exact tiles of FTD_BIG cells x 256 rows (groups of at least t64 cells).
-__global__ __launch_bounds__(256) void k_ftd_mfma64(
+__global__ __launch_bounds__(256, 2) void k_ftd_mfma64(
     const FtdCounts* __restrict__ cnt, const FtCorrTile* __restrict__ tiles,
     const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
     const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
