@@ -524,20 +524,34 @@ __device__ inline void plain_block(const int16_t* __restrict__ Xq, const int16_t
         }
     }
 
+    // the thread's row and column sums of squares, all loaded before any is
+    // used (a row / column outside the tile reads row / column 0)
+    float rsq[MI], csq[8];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+    {
+        const int m = i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4);
+        rsq[i]      = map.row_sq(map.row_ok(m) ? m : 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+        const int n = j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4);
+        csq[j]      = map.col_sq(map.col_ok(n) ? n : 0);
+    }
 #pragma unroll
     for (int i = 0; i < MI; ++i)
     {
         const int m = i < 4 ? ty * 4 + i : 64 + ty * 4 + (i - 4);
         if (!map.row_ok(m))
             continue;
-        const float sq  = map.row_sq(m);
-        float*      out = map.row_out(m);
+        float* out = map.row_out(m);
 #pragma unroll
         for (int j = 0; j < 8; ++j)
         {
             const int n = j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4);
             if (map.col_ok(n))
-                out[n] = corr_from_sums(acc[i][j], sq, map.col_sq(n));
+                out[n] = corr_from_sums(acc[i][j], rsq[i], csq[j]);
         }
     }
 }
@@ -1001,6 +1015,13 @@ __device__ inline void select_segment_reg(const Segment& sg, const float* __rest
             hi = fmaxf(hi, x[u]);
         }
     }
+    if (pp.kind == 0)
+    {
+        // len <= 1: the reference returns arr.front() unsorted
+        if (lane == 0)
+            out[item] = sg.n > 0 ? x[0] : __builtin_nanf("");
+        return;
+    }
     if (__ballot(nan))
     {
         if (lane == 0)
@@ -1035,17 +1056,11 @@ __device__ inline void select_item(const Src& src, uint64_t item,
     const Segment  sg = src.get(item);
     const float*   v  = scores + sg.off;
     const PctParam pp = params[sg.param];
-    if (pp.kind == 0)
-    {
-        // len <= 1: the reference returns arr.front() unsorted
-        if (lane == 0)
-            out[item] = sg.n > 0 ? v[0] : __builtin_nanf("");
-        return;
-    }
     if (sg.n <= 1024)
     {
         // the segment stays in registers for every pass, in the fewest slots
-        // (a power of two) that hold it
+        // (a power of two) that hold it; they are loaded before the
+        // parameters are looked at, so both loads are in flight together
         if (sg.n <= 64)
             select_segment_reg<1>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
         else if (sg.n <= 128)
@@ -1056,6 +1071,12 @@ __device__ inline void select_item(const Src& src, uint64_t item,
             select_segment_reg<8>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
         else
             select_segment_reg<16>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
+        return;
+    }
+    if (pp.kind == 0)
+    {
+        if (lane == 0)
+            out[item] = v[0];
         return;
     }
     bool nan = false;

@@ -216,6 +216,15 @@ __global__ __launch_bounds__(256) void k_bgemm_nt(const __bf16* __restrict__ A, 
             buf ^= 1;
         }
     }
+    // the lane's four column biases, loaded once (a column past N reads
+    // column N - 1 and stores nothing)
+    float bj[4] = { 0.f, 0.f, 0.f, 0.f };
+    if constexpr (MODE != 0)
+    {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bj[j] = bias[min(n0 + wn * 64 + j * 16 + r16, N - 1)];
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -235,7 +244,7 @@ __global__ __launch_bounds__(256) void k_bgemm_nt(const __bf16* __restrict__ A, 
                     C[o] = acc[i][j][e];
                 else
                 {
-                    const float v = acc[i][j][e] + bias[n];
+                    const float v = acc[i][j][e] + bj[j];
                     if (z)
                         z[o] = v;
                     H[o] = (__bf16)gelu_f(v);
@@ -341,6 +350,14 @@ __global__ __launch_bounds__(512) void k_bgemm_nt_w(const __bf16* __restrict__ A
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     }
+    // the lane's four column biases, loaded once (N is a multiple of BW_BN)
+    float bj[4] = { 0.f, 0.f, 0.f, 0.f };
+    if constexpr (MODE != 0)
+    {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bj[j] = bias[n0 + wn * 64 + j * 16 + r16];
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -358,7 +375,7 @@ __global__ __launch_bounds__(512) void k_bgemm_nt_w(const __bf16* __restrict__ A
                     C[o] = acc[i][j][e];
                 else
                 {
-                    const float v = acc[i][j][e] + bias[n];
+                    const float v = acc[i][j][e] + bj[j];
                     if (z)
                         z[o] = v;
                     H[o] = (__bf16)gelu_f(v);
