@@ -867,7 +867,7 @@ __global__ __launch_bounds__(256) void k_ftd_fused(
 }
 
 // Most consecutive reference rows one wave of k_ftd_fused_r takes at a time.
-constexpr int FTD_FUSED_RUN = 16;
+constexpr int FTD_FUSED_RUN = 64;
 
 // Two cells' rank values of one element, in half precision: a doubled
 // centred rank over at most 1024 genes has |x| <= 1025, which FP16 holds
@@ -876,9 +876,10 @@ typedef _Float16 ftd_h2 __attribute__((ext_vector_type(2)));
 
 // Chunk sums of one row for a group of NC cells (lane l: chunks 2l + h of
 // `step` elements, element (2l + h) step + k in slot [h][k], bit h STEP + k
-// of `ok` set for the slots inside G_sub).  A slot outside adds y = 0: its
-// fma leaves the sums' bits as they are (no sum here is ever -0), so the
-// sums are those of the sequential loops over the valid elements.
+// of `ok` set for the slots inside G_sub).  A slot outside has y = 0 (and
+// some finite rank value x): its fmas add +-0, which leaves the sums' bits
+// as they are (no sum here is ever -0), so the sums are those of the
+// sequential loops over the valid elements.
 template <int STEP, int NC>
 __device__ inline void ftd_fused_row(const int* bins, const uint32_t (&code)[2][STEP],
                                      const ftd_h2 (&x)[2][STEP][FTD_FUSE_MAX / 2], uint32_t ok,
@@ -914,8 +915,9 @@ __device__ inline void ftd_fused_row(const int* bins, const uint32_t (&code)[2][
 // sums and tree, so the same bits) for the fused groups whose chunk length
 // ceil(gs / 128) lies in (STEP_LO, STEP].  A wave takes runs of consecutive
 // reference items (one vector load of their groups and flags, a ballot of
-// the fused ones): up to FTD_FUSED_RUN items, fewer when the items would
-// leave waves idle (a round of a few cells has a few thousand rows).
+// the fused ones): ceil(items / waves) of them, at most FTD_FUSED_RUN, so
+// that a wave's rows are consecutive rows of one group as far as possible
+// (blocks outnumber the resident slots, so uneven runs even out).
 // Consecutive rows of a group share its gene list and cells, so a lane
 // loads its chunk elements' genes and the cells' rank values once per group
 // and keeps them in registers (the ranks as FP16 pairs, exact for the at
@@ -946,7 +948,8 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
     uint32_t       gl[2][STEP];
     ftd_h2         x[2][STEP][FTD_FUSE_MAX / 2];
     const uint64_t waves = (uint64_t)gridDim.x * 4;
-    const int      rl    = (int)min((uint64_t)FTD_FUSED_RUN, max((uint64_t)1, n / (2 * waves)));
+    const int      rl    = (int)min((uint64_t)FTD_FUSED_RUN,
+                                    max((uint64_t)1, (n + waves - 1) / waves));
     for (uint64_t run = (uint64_t)blockIdx.x * 4 + wid; run * rl < n; run += waves)
     {
         const uint64_t base = run * rl;
@@ -974,6 +977,11 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
                 mine           = step > STEP_LO && step <= STEP;
                 if (mine)
                 {
+                    // every load unconditional, so that they are all in
+                    // flight together: a slot outside G_sub takes element 0
+                    // (its code is gathered but never counted, its rank
+                    // value multiplies y = 0), a cell past nc takes cell 0
+                    // (never read)
                     const int16_t* xc = cell_rank + g.cell_dst0;
                     const int      st = (int)g.stride;
                     ok                = 0;
@@ -982,14 +990,15 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
 #pragma unroll
                         for (int j = 0; j < STEP; ++j)
                         {
-                            const int  i = (2 * lane + h) * step + j;
-                            const bool v = j < step && i < gs;
+                            const int  i  = (2 * lane + h) * step + j;
+                            const bool v  = j < step && i < gs;
+                            const int  ic = v ? i : 0;
                             ok |= (uint32_t)v << (h * STEP + j);
-                            gl[h][j] = v ? genes[g.gene_off + i] : 0u;
+                            gl[h][j] = genes[g.gene_off + ic];
 #pragma unroll
                             for (int c = 0; c < FTD_FUSE_MAX; ++c)
                                 x[h][j][c >> 1][c & 1] =
-                                    (_Float16)(v && c < nc ? xc[c * st + i] : (int16_t)0);
+                                    (_Float16)xc[(c < nc ? c : 0) * st + ic];
                         }
                 }
             }
@@ -1002,7 +1011,7 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int j = 0; j < STEP; ++j)
-                    code[h][j] = src[gl[h][j]];  // gene 0 (in the row) for slots outside
+                    code[h][j] = src[gl[h][j]];
             __builtin_amdgcn_wave_barrier();
             int zeros = 0;
 #pragma unroll
