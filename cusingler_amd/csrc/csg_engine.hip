@@ -2636,6 +2636,9 @@ int ftd_tune(csg_engine* eng, float quantile)
         FtdCounts hc;
         HIPCHK(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, eng->stream));
         HIPCHK(hipStreamSynchronize(eng->stream));
+        // the tile kinds (16-cell / big exact, FP32) follow the threshold:
+        // the launch mask must, or a candidate would time skipped tiles
+        eng->ftd_mask = ftd_mask_of(eng, hc);
         int rc2;
         if ((rc2 = ensure(eng, &eng->d_fl_tiles, &eng->fl_cap.tiles, std::max<uint64_t>(hc.n_tiles, 1))))
             return rc2;
