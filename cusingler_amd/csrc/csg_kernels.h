@@ -1056,21 +1056,22 @@ __device__ inline void select_item(const Src& src, uint64_t item,
     const Segment  sg = src.get(item);
     const float*   v  = scores + sg.off;
     const PctParam pp = params[sg.param];
-    if (sg.n <= 1024)
+    if (sg.n <= 512)
     {
         // the segment stays in registers for every pass, in the fewest slots
         // (a power of two) that hold it; they are loaded before the
-        // parameters are looked at, so both loads are in flight together
+        // parameters are looked at, so both loads are in flight together.
+        // (Longer segments take the streaming path below: a register path
+        // for them would cost every wave of the kernel the VGPRs of 16
+        // slots, and occupancy with them.)
         if (sg.n <= 64)
             select_segment_reg<1>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
         else if (sg.n <= 128)
             select_segment_reg<2>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
         else if (sg.n <= 256)
             select_segment_reg<4>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
-        else if (sg.n <= 512)
-            select_segment_reg<8>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
         else
-            select_segment_reg<16>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
+            select_segment_reg<8>(sg, v, pp, p, item, out, nan_list, nan_count, wsm, lane);
         return;
     }
     if (pp.kind == 0)
