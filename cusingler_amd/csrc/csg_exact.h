@@ -57,14 +57,13 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
     float*       mk = mask + (size_t)q * L;
     bool         nan = false;
     double       mlo = -1e300, mhi = -1e300;
+    // (no early exit at a NaN: the row's loads stay independent, so they
+    // are in flight together; a NaN row discards mlo / mhi)
+#pragma unroll 8
     for (uint32_t j = 0; j < L; ++j)
     {
         const float v = s[j];
-        if (__builtin_isnan(v))
-        {
-            nan = true;
-            break;
-        }
+        nan |= __builtin_isnan(v);
         mlo = fmax(mlo, (double)v - wp);
         mhi = fmax(mhi, (double)v + wp);
     }
@@ -90,20 +89,33 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
     const double thi = (double)(fhi - thre);
     int          ncand = 0, cand = -1;
     bool         unsure = false;
-    for (uint32_t j = 0; j < L; ++j)
+    // eight scores loaded together, then decided (the mask stores would
+    // otherwise wait behind each load)
+    for (uint32_t j0 = 0; j0 < L; j0 += 8)
     {
-        const double v = (double)s[j];
-        if (v + wp >= mlo)
+        float sv[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+            sv[u] = s[min(j0 + u, L - 1)];  // (past the row: its last score, not used)
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
         {
-            ++ncand;
-            cand = (int)j;
+            const uint32_t j = j0 + u;
+            if (j >= L)
+                break;
+            const double v = (double)sv[u];
+            if (v + wp >= mlo)
+            {
+                ++ncand;
+                cand = (int)j;
+            }
+            if (v - wp >= thi)
+                mk[j] = 1.f;
+            else if (v + wp < tlo)
+                mk[j] = 0.f;
+            else
+                unsure = true;
         }
-        if (v - wp >= thi)
-            mk[j] = 1.f;
-        else if (v + wp < tlo)
-            mk[j] = 0.f;
-        else
-            unsure = true;
     }
     if (ncand == 1 && !unsure)
     {

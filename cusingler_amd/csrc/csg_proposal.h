@@ -29,15 +29,14 @@ __global__ void k_fill_f32(float* __restrict__ p, size_t n, float v)
 __device__ inline bool prop_confident(const int32_t* top, const float* margin, uint32_t c, int KP,
                                       uint32_t L, float thr)
 {
-    if (!(margin[c] >= thr))  // NaN margins fall back
-        return false;
+    bool ok = margin[c] >= thr;  // NaN margins fall back
+#pragma unroll 8
     for (int t = 0; t < KP; ++t)
     {
         const int32_t l = top[(size_t)c * KP + t];
-        if (l < 0 || (uint32_t)l >= L)
-            return false;
+        ok &= l >= 0 && (uint32_t)l < L;
     }
-    return true;
+    return ok;
 }
 
 // First stage: every confident cell takes a slot in the lists of its first
@@ -166,15 +165,19 @@ __global__ void k_prop_check(const float* __restrict__ P, uint32_t n, uint32_t L
     const uint32_t c = in ? in[i] : i;
     if (!in && !prop_confident(top, margin, c, KP, L, thr))
         return;
-    const float* s  = P + (size_t)c * L;
-    float        mx = -__builtin_inff();
+    const float* s   = P + (size_t)c * L;
+    float        mx  = -__builtin_inff();
+    bool         nan = false;
+    // (no early exit at a NaN, so that the row's loads are in flight together)
+#pragma unroll 8
     for (uint32_t j = 0; j < L; ++j)
     {
         const float v = s[j];
-        if (__builtin_isnan(v))
-            return;
+        nan |= __builtin_isnan(v);
         mx = fmaxf(mx, v);
     }
+    if (nan)
+        return;
     const float last = s[top[(size_t)c * KP + edge]];
     if ((double)last + 2.0 * wp < (double)mx - (double)thre)
         return;
