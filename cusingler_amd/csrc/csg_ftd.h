@@ -709,13 +709,30 @@ __global__ void k_ftd_prep(const FtdCounts* __restrict__ cnt, const uint32_t* __
 
 // The last step of a wave's rank table: bins[0] = the zero count (the zero
 // code is counted by ballots, never by the LDS atomics), then the code
-// counts become bins[c] = 2 (#less) + (#equal) - gs by a wave scan (lane l
-// sums a contiguous run of ceil(nbins / 64) bins).
+// counts become bins[c] = 2 (#less) + (#equal) - gs by wave scans: up to
+// 512 bins window by window (lane l takes bin 64 w + l, so the LDS accesses
+// are free of bank conflicts, a running carry joins the windows), beyond
+// that one scan over contiguous runs of ceil(nbins / 64) bins per lane.
 __device__ inline void ftd_bins_to_ranks(int* bins, int nbins, int gs, int zeros, int lane)
 {
     if (lane == 0)
         bins[0] = zeros;
     __builtin_amdgcn_wave_barrier();
+    if (nbins <= 512)
+    {
+        int carry = 0;
+        for (int b0 = 0; b0 < nbins; b0 += 64)
+        {
+            const int i    = b0 + lane;
+            const int c    = i < nbins ? bins[i] : 0;
+            const int incl = wave_incl_scan(c);
+            if (i < nbins)
+                bins[i] = 2 * (carry + incl - c) + c - gs;
+            carry += __builtin_amdgcn_readlane(incl, 63);
+        }
+        __builtin_amdgcn_wave_barrier();
+        return;
+    }
     const int per = (nbins + 63) / 64;
     const int b0 = min(lane * per, nbins), b1 = min(b0 + per, nbins);
     int       s  = 0;
@@ -763,7 +780,8 @@ __device__ inline void ftd_row_table(const CT* __restrict__ src, const uint32_t*
 // rank table of ftd_row_table (per-wave LDS of `nbins` code bins and the
 // row's codes, `wave_ints` ints), the ranks stored, and the 128-chunk +
 // stride-tree sum of squares with lane l holding chunks 2l and 2l+1 (their
-// sum is the tree's first level) and shuffles for the remaining six levels.
+// sum is the tree's first level) and wave_tree_sum for the remaining six
+// levels.
 template <class CT>
 __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict__ groups,
                                         const uint32_t* __restrict__ refgroup,
@@ -798,9 +816,7 @@ __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict
             a1            = __builtin_fmaf(v, v, a1);
         }
     }
-    float p = a0 + a1;
-    for (int off = 1; off < 64; off <<= 1)
-        p = p + __shfl_down(p, off, 64);
+    const float p = wave_tree_sum(a0 + a1);
     if (lane == 0)
         out_sq[item] = p;
     __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
@@ -833,8 +849,8 @@ __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
 // FTD_FUSE_MAX cells: the row is ranked over G_sub as in ft_rank_ref_wave
 // (LDS code histogram, wave scan), and instead of storing its ranks the wave
 // forms, in the same pass and the same 128-chunk + stride-tree FP32 order as
-// the thin tile (lane l sums chunks 2l and 2l + 1 sequentially by fma, six
-// shuffle levels complete the tree), the row's sum of squares and its dot
+// the thin tile (lane l sums chunks 2l and 2l + 1 sequentially by fma,
+// wave_tree_sum completes the tree), the row's sum of squares and its dot
 // product with each of the group's cell rank rows, and stores the group's
 // scores corr_from_sums(sxy, sxx, syy) -- the same values the thin tile
 // computes from stored ranks.  Takes the fused groups whose chunk length
@@ -888,18 +904,13 @@ __global__ __launch_bounds__(256) void k_ftd_fused(
                         d[h][c] = __builtin_fmaf((float)xc[c * st + i], y, d[h][c]);
             }
         }
-        float syy = q[0] + q[1];
-        for (int off = 1; off < 64; off <<= 1)
-            syy = syy + __shfl_down(syy, off, 64);
-        syy = __shfl(syy, 0, 64);
+        const float syy = wave_tree_sum(q[0] + q[1]);
 #pragma unroll
         for (int c = 0; c < FTD_FUSE_MAX; ++c)
         {
             if (c >= nc)
                 break;
-            float sxy = d[0][c] + d[1][c];
-            for (int off = 1; off < 64; off <<= 1)
-                sxy = sxy + __shfl_down(sxy, off, 64);
+            const float sxy = wave_tree_sum(d[0][c] + d[1][c]);
             if (lane == 0)
                 scores[g.score0 + (uint64_t)c * g.nref + local] =
                     corr_from_sums(sxy, cell_sq[g.cell_item0 + c], syy);

@@ -498,6 +498,52 @@ __global__ void k_rank_features(const int16_t* __restrict__ X, size_t n, float i
 // lower label first) and the softmax margin p(top1) - p(top2); one wave per
 // row, L <= 256 logits (four per lane).  A row with a NaN logit gets margin
 // NaN and labels -1.
+// Largest (value, label) pair of the wave -- larger value first, lower
+// label among equal values -- returned to every lane: rows of 16 by DPP
+// row shifts (a lane whose source lies outside its row compares with
+// itself), then the four rows' winners.  The order is total on the pairs
+// taken here (no NaN values, distinct labels), so the winner does not
+// depend on the reduction order.  Every lane of the wave must be active.
+__device__ inline void wave_argmax(float& bv, int& bj)
+{
+    auto take = [&](float ov, int oj) {
+        if (ov > bv || (ov == bv && oj < bj))
+        {
+            bv = ov;
+            bj = oj;
+        }
+    };
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+    {
+        const int b = __builtin_bit_cast(int, bv);
+        int       tv, tj;
+        switch (s)
+        {
+        case 0: tv = CSG_DPP_ROW_SHR_KEEP(b, 1); tj = CSG_DPP_ROW_SHR_KEEP(bj, 1); break;
+        case 1: tv = CSG_DPP_ROW_SHR_KEEP(b, 2); tj = CSG_DPP_ROW_SHR_KEEP(bj, 2); break;
+        case 2: tv = CSG_DPP_ROW_SHR_KEEP(b, 4); tj = CSG_DPP_ROW_SHR_KEEP(bj, 4); break;
+        default: tv = CSG_DPP_ROW_SHR_KEEP(b, 8); tj = CSG_DPP_ROW_SHR_KEEP(bj, 8); break;
+        }
+        take(__builtin_bit_cast(float, tv), tj);
+    }
+    const int   b  = __builtin_bit_cast(int, bv);
+    const int   jj = bj;
+    const float v0 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 15));
+    const int   j0 = __builtin_amdgcn_readlane(jj, 15);
+    const float v1 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 31));
+    const int   j1 = __builtin_amdgcn_readlane(jj, 31);
+    const float v2 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 47));
+    const int   j2 = __builtin_amdgcn_readlane(jj, 47);
+    const float v3 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(b, 63));
+    const int   j3 = __builtin_amdgcn_readlane(jj, 63);
+    bv = v0;
+    bj = j0;
+    take(v1, j1);
+    take(v2, j2);
+    take(v3, j3);
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ acc,
                                               const float* __restrict__ bias, int rows, int L,
@@ -527,9 +573,7 @@ __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ acc,
             margin[r] = __builtin_nanf("");
         return;
     }
-    float mx = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
-    for (int o = 32; o > 0; o >>= 1)
-        mx = fmaxf(mx, __shfl_xor(mx, o));
+    const float mx = wave_max(fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])));
     float s = 0.f;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -552,16 +596,7 @@ __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ acc,
                 bj = j;
             }
         }
-        for (int o = 32; o > 0; o >>= 1)
-        {
-            const float ov = __shfl_xor(bv, o);
-            const int   oj = __shfl_xor(bj, o);
-            if (ov > bv || (ov == bv && oj < bj))
-            {
-                bv = ov;
-                bj = oj;
-            }
-        }
+        wave_argmax(bv, bj);
         if (bj >= L)
             bj = -1;  // fewer than KK labels
         if (t < K && lane == 0)
