@@ -266,117 +266,75 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
     uint32_t* __restrict__ glpos, uint32_t* __restrict__ ggenes)
 {
     extern __shared__ uint32_t sbits[];  // ceil(Gf / 32) words
-    __shared__ uint32_t           slab[256];
-    __shared__ uint32_t           spos[FTD_SETUP_NT], soff[FTD_SETUP_NT + 1];
-    __shared__ int                stmp[FTD_SETUP_NT / 64];
-    __shared__ unsigned long long skey[4];
-    __shared__ uint32_t           sk, snref;
+    __shared__ uint32_t slab[256];
+    __shared__ uint32_t spart[FTD_SETUP_NT];
+    __shared__ uint32_t sk, snref;
     const uint32_t ng  = cnt->n_groups;
     const uint32_t W   = (Gf + 31) / 32;
     const uint32_t tid = threadIdx.x;
     for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x)
     {
         __syncthreads();  // the previous group's LDS is no longer read
-        if (tid < 64)
+        if (tid == 0)
         {
-            // wave 0: the owner's candidate set as a 256-bit mask, then its
-            // labels in ascending order with their row offsets (lane t
-            // takes label 64 w + t; ranks by prefix popcounts, offsets by a
-            // wave scan of the row counts)
-            if (tid < 4)
-                skey[tid] = 0ull;
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t o  = gowner[g];
-            const uint32_t no = counts[o];
-            for (uint32_t t = tid; t < no; t += 64)
-            {
-                const uint32_t l = lists[(size_t)o * L + t];
-                atomicOr(&skey[l >> 6], 1ull << (l & 63));
-            }
-            __builtin_amdgcn_wave_barrier();
-            uint32_t kb = 0, rb = 0;
-#pragma unroll
+            const uint32_t o = gowner[g];
+            uint64_t       key[4];
+            ftd_key(lists + (size_t)o * L, counts[o], key);
+            uint32_t k = 0, nref = 0;
             for (uint32_t w = 0; w < 4; ++w)
             {
-                const uint64_t v    = skey[w];
-                const bool     set  = (v >> tid) & 1ull;
-                const uint32_t l    = w * 64 + tid;
-                const uint32_t len  = set ? llen[l] : 0u;
-                const uint32_t incl = (uint32_t)wave_incl_scan((int)len);
-                if (set)
+                uint64_t v = key[w];
+                while (v)
                 {
-                    const uint32_t kk =
-                        kb + (uint32_t)__builtin_popcountll(v & ((1ull << tid) - 1ull));
-                    slab[kk]                    = l;
-                    glabels[(size_t)g * L + kk] = l;
-                    glpos[(size_t)g * L + kk]   = rb + incl - len;
+                    const uint32_t l = w * 64 + (uint32_t)__builtin_ctzll(v);
+                    v &= v - 1;
+                    slab[k]                       = l;
+                    glabels[(size_t)g * L + k]    = l;
+                    glpos[(size_t)g * L + k]      = nref;
+                    nref += llen[l];
+                    ++k;
                 }
-                kb += (uint32_t)__builtin_popcountll(v);
-                rb += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
             }
-            if (tid == 0)
-            {
-                sk    = kb;
-                snref = rb;
-            }
+            sk    = k;
+            snref = nref;
         }
         for (uint32_t w = tid; w < W; w += FTD_SETUP_NT)
             sbits[w] = 0u;
         __syncthreads();
         const uint32_t k   = sk;
         const uint32_t cut = k ? cut_tab[k] : 0u;
-        // the ordered pairs' gene lists, FTD_SETUP_NT pairs at a time: one
-        // thread per pair reads its list's position and length, a block scan
-        // turns the lengths into offsets, and the pairs' genes are then
-        // spread evenly over the threads (each finds its pair by binary
-        // search over the offsets), so that their loads are in flight
-        // together
-        for (uint32_t p0 = 0; p0 < k * k; p0 += FTD_SETUP_NT)
+        // ordered pairs spread over the waves, genes over the lanes
+        for (uint32_t pi = tid >> 6; pi < k * k; pi += FTD_SETUP_NT / 64)
         {
-            const uint32_t p   = p0 + tid;
-            uint32_t       pos = 0, len = 0;
-            if (p < k * k)
+            const uint32_t i = slab[pi / k], j = slab[pi % k];
+            if (i == j)
+                continue;
+            const uint32_t pos = ctdidx[((size_t)i * L + j) * 2];
+            const uint32_t len = min(ctdidx[((size_t)i * L + j) * 2 + 1], cut);
+            for (uint32_t t = tid & 63; t < len; t += 64)
             {
-                const uint32_t i = slab[p / k], j = slab[p % k];
-                if (i != j)
-                {
-                    pos = ctdidx[((size_t)i * L + j) * 2];
-                    len = min(ctdidx[((size_t)i * L + j) * 2 + 1], cut);
-                }
-            }
-            int       total = 0;
-            const int off   = block_excl_scan<FTD_SETUP_NT>((int)len, stmp, &total);
-            spos[tid]       = pos;
-            soff[tid]       = (uint32_t)off;
-            if (tid == 0)
-                soff[FTD_SETUP_NT] = (uint32_t)total;
-            __syncthreads();
-#pragma unroll 4
-            for (uint32_t e = tid; e < (uint32_t)total; e += FTD_SETUP_NT)
-            {
-                uint32_t lo = 0, hi = FTD_SETUP_NT;  // soff[lo] <= e < soff[hi]
-                while (hi - lo > 1)
-                {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (soff[mid] <= e)
-                        lo = mid;
-                    else
-                        hi = mid;
-                }
-                const uint32_t gene = ctdiff[spos[lo] + (e - soff[lo])];
+                const uint32_t gene = ctdiff[pos + t];
                 atomicOr(&sbits[gene >> 5], 1u << (gene & 31));
             }
-            __syncthreads();
         }
+        __syncthreads();
         // ascending gene list: contiguous word ranges per thread, block scan
         const uint32_t per = (W + FTD_SETUP_NT - 1) / FTD_SETUP_NT;
         const uint32_t w0 = min(tid * per, W), w1 = min(w0 + per, W);
         uint32_t       own = 0;
         for (uint32_t w = w0; w < w1; ++w)
             own += __builtin_popcount(sbits[w]);
-        int            gsi = 0;
-        uint32_t       at  = (uint32_t)block_excl_scan<FTD_SETUP_NT>((int)own, stmp, &gsi);
-        const uint32_t gs  = (uint32_t)gsi;
+        spart[tid] = own;
+        __syncthreads();
+        for (uint32_t off = 1; off < FTD_SETUP_NT; off <<= 1)
+        {
+            const uint32_t v = tid >= off ? spart[tid - off] : 0u;
+            __syncthreads();
+            spart[tid] += v;
+            __syncthreads();
+        }
+        uint32_t       at  = spart[tid] - own;
+        const uint32_t gs  = spart[FTD_SETUP_NT - 1];
         uint32_t*      dst = ggenes + (size_t)g * gstride;
         for (uint32_t w = w0; w < w1; ++w)
         {
