@@ -2377,27 +2377,53 @@ int ftd_launch_a(csg_engine* eng)
     return CSG_OK;
 }
 
-// Reference-row ranks: one wave per row when the per-wave LDS fits, else
-// one block per row.
+// Reference-row ranks of the groups that are not fused: with byte codes the
+// register-resident kernel (k_ftd_fused_r in its rank mode) for G_sub up to
+// 1024 genes; longer gene sets and 16-bit codes one wave per row when the
+// per-wave LDS fits, else one block per row.
 int ftd_launch_rank_ref(csg_engine* eng, int grid)
 {
     const FtdLds   d   = ftd_lds(eng);
     const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1);
     FtdCounts*     cnt = eng->d_fl_cnt;
+    int            lo  = 0;  // rows of chunk length <= lo already ranked
+    if (eng->d_fref_codes8)
+    {
+        // byte codes: G_sub up to 1024 genes in the register-resident kernel
+        k_ftd_fused_r<4, 0, true><<<(unsigned)grid, 256, 0, eng->stream>>>(
+            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+            eng->d_fl_ggenes, eng->d_fref_codes8, Gf, nullptr, nullptr, nullptr, eng->d_ft_rr,
+            eng->d_ft_rsq);
+        lo = 4;
+        if (Gf > 512)
+        {
+            k_ftd_fused_r<8, 4, true><<<(unsigned)grid, 256, 0, eng->stream>>>(
+                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
+                eng->d_fl_ggenes, eng->d_fref_codes8, Gf, nullptr, nullptr, nullptr,
+                eng->d_ft_rr, eng->d_ft_rsq);
+            lo = 8;
+        }
+        if (Gf <= 1024)
+        {
+            LAUNCHCHK();
+            return CSG_OK;
+        }
+    }
     if (d.rank_ref_w <= 65536)
     {
         if (eng->d_fref_codes8)
             k_ftd_rank_ref_w<uint8_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
                 cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
                 eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
-                eng->d_ft_rsq);
+                eng->d_ft_rsq, lo);
         else
             k_ftd_rank_ref_w<uint16_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
                 cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
                 eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, d.wave_ints, eng->d_ft_rr,
-                eng->d_ft_rsq);
+                eng->d_ft_rsq, lo);
     }
     else
+        // (rows already ranked above are ranked again to the same values)
         k_ftd_rank_ref<<<(unsigned)grid, 128, d.rank_ref, eng->stream>>>(
             cnt, eng->d_fl_groups, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
             eng->d_fl_ggenes, eng->d_fref_codes, Gf, eng->fnbins, eng->d_ft_rr, eng->d_ft_rsq);

@@ -780,14 +780,15 @@ __device__ inline void ft_rank_ref_wave(uint32_t item, const FtGroup* __restrict
     __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
 }
 
-// Reference rows of every group except the fused ones.
+// Reference rows of every group except the fused ones whose chunk length
+// ceil(gs / 128) exceeds step_lo (the others are k_ftd_fused_r's).
 template <class CT>
 __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
     const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
     const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
     const CT* __restrict__ ref_codes, uint32_t Gf, int nbins, int wave_ints,
-    int16_t* __restrict__ out, float* __restrict__ out_sq)
+    int16_t* __restrict__ out, float* __restrict__ out_sq, int step_lo)
 {
     extern __shared__ int smem[];
     const int      wid   = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -796,7 +797,8 @@ __global__ __launch_bounds__(256) void k_ftd_rank_ref_w(
     const uint32_t n     = (uint32_t)cnt->n_ref_items;
     for (uint32_t item = blockIdx.x * 4 + wid; item < n; item += gridDim.x * 4)
     {
-        if (gflag[refgroup[item]] & FTD_FUSED)
+        const uint32_t gi = refgroup[item];
+        if ((gflag[gi] & FTD_FUSED) || ((int)groups[gi].gs + 127) / 128 <= step_lo)
             continue;  // uniform per wave
         ft_rank_ref_wave(item, groups, refgroup, refrow, genes, ref_codes, Gf, nbins, out,
                          out_sq, bins, codes, lane);
@@ -938,15 +940,23 @@ __device__ inline void ftd_fused_row(const int* bins, const uint32_t (&code)[2][
 // a DPP scan (lane l holds bins 4l .. 4l + 3: one 16-byte read and write),
 // the chunk sums accumulated from the codes the lane still holds, and the
 // trees completed by DPP (wave_tree_sum) -- no LDS traffic beyond the bins.
-template <int STEP, int STEP_LO>
+//
+// RANKS = true: the same rows of the groups that are not fused, whose ranks
+// are stored for the tiles (k_ftd_rank_ref_w's output): the rank values go
+// through an LDS row (chunk layout in, lane-strided out) so that the row is
+// stored coalesced, zero-padded to the group's stride, with the row's
+// chunk-tree sum of squares.
+template <int STEP, int STEP_LO, bool RANKS = false>
 __global__ __launch_bounds__(256) void k_ftd_fused_r(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
     const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
     const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
     const uint8_t* __restrict__ ref_codes, uint32_t Gf, const int16_t* __restrict__ cell_rank,
-    const float* __restrict__ cell_sq, float* __restrict__ scores)
+    const float* __restrict__ cell_sq, float* __restrict__ scores,
+    int16_t* __restrict__ out_rank = nullptr, float* __restrict__ out_sq = nullptr)
 {
     __shared__ __attribute__((aligned(16))) int sbins[4][256];
+    __shared__ __attribute__((aligned(16))) int16_t srow[4][RANKS ? 128 * STEP : 2];
     const int      wid   = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int*           bins  = sbins[wid];
     int4*          bins4 = (int4*)bins;
@@ -969,7 +979,7 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
         if (lane < rl && base + lane < n)
         {
             rg = refgroup[base + lane];
-            fz = (gflag[rg] & FTD_FUSED) != 0;
+            fz = ((gflag[rg] & FTD_FUSED) != 0) != RANKS;
         }
         uint64_t m = __ballot(fz);
         while (m)
@@ -1006,10 +1016,13 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
                             const int  ic = v ? i : 0;
                             ok |= (uint32_t)v << (h * STEP + j);
                             gl[h][j] = genes[g.gene_off + ic];
+                            if constexpr (!RANKS)
+                            {
 #pragma unroll
-                            for (int c = 0; c < FTD_FUSE_MAX; ++c)
-                                x[h][j][c >> 1][c & 1] =
-                                    (_Float16)xc[(c < nc ? c : 0) * st + ic];
+                                for (int c = 0; c < FTD_FUSE_MAX; ++c)
+                                    x[h][j][c >> 1][c & 1] =
+                                        (_Float16)xc[(c < nc ? c : 0) * st + ic];
+                            }
                         }
                 }
             }
@@ -1051,6 +1064,35 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
             r.w         = 2 * ex + b.w - gs;
             bins4[lane] = r;
             __builtin_amdgcn_wave_barrier();
+            const uint64_t local = item - g.ref_item0;
+            if constexpr (RANKS)
+            {
+                const int step = (gs + 127) / 128;
+                int16_t*  row  = srow[wid];
+                float     q[2] = { 0.f, 0.f };
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int j = 0; j < STEP; ++j)
+                    {
+                        if (j >= step)
+                            break;
+                        // a slot past G_sub stores the padding zero
+                        const int y = (ok >> (h * STEP + j)) & 1u ? bins[code[h][j]] : 0;
+                        row[(2 * lane + h) * step + j] = (int16_t)y;
+                        const float yf = (float)y;
+                        q[h]           = __builtin_fmaf(yf, yf, q[h]);
+                    }
+                __builtin_amdgcn_wave_barrier();
+                int16_t* dst = out_rank + g.ref_dst0 + local * g.stride;
+                for (int i = lane; i < (int)g.stride; i += 64)
+                    dst[i] = row[i];
+                const float p = wave_tree_sum(q[0] + q[1]);
+                if (lane == 0)
+                    out_sq[item] = p;
+                __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
+                continue;
+            }
             float syy, sxy[FTD_FUSE_MAX];
             switch (nc)
             {
@@ -1059,8 +1101,7 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
             case 3: ftd_fused_row<STEP, 3>(bins, code, x, ok, syy, sxy); break;
             default: ftd_fused_row<STEP, 4>(bins, code, x, ok, syy, sxy); break;
             }
-            syy                  = wave_tree_sum(syy);
-            const uint64_t local = item - g.ref_item0;
+            syy = wave_tree_sum(syy);
 #pragma unroll
             for (int c = 0; c < FTD_FUSE_MAX; ++c)
             {
