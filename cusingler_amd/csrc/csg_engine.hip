@@ -2431,6 +2431,30 @@ int ftd_launch_rank_ref(csg_engine* eng, int grid)
     return CSG_OK;
 }
 
+// Query-cell ranks: with codes below 256 one wave per cell for G_sub up to
+// 1024 genes (when its LDS fits), one block per cell for the rest.
+int ftd_launch_rank_qry(csg_engine* eng, int grid)
+{
+    const FtdLds   d   = ftd_lds(eng);
+    const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1);
+    FtdCounts*     cnt = eng->d_fl_cnt;
+    const size_t   lw  = (size_t)4 * ftd_qry_wave_ints(8, Gf) * 4;
+    int            lo  = 0;
+    if (eng->d_fref_codes8 && lw <= 65536)
+    {
+        k_ftd_rank_qry_w<8><<<(unsigned)grid, 256, lw, eng->stream>>>(
+            cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata,
+            eng->d_fqind, eng->d_fqptr, Gf, eng->d_ft_cr, eng->d_ft_csq);
+        lo = 8;
+    }
+    if (lo == 0 || Gf > 1024)
+        k_ftd_rank_qry<<<(unsigned)grid, 128, d.rank_qry, eng->stream>>>(
+            cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata,
+            eng->d_fqind, eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq, lo);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
 int ftd_launch_corr(csg_engine* eng, int grid)
 {
     const FtdLds   d   = ftd_lds(eng);
@@ -2524,9 +2548,8 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
         eng->d_fl_tiles, eng->d_fl_thin);
     if ((eng->ftd_mask & FTDM_RANK) && ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
         return CSG_ERR_DEVICE;
-    k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
-        cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
-        eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
+    if (ftd_launch_rank_qry(eng, eng->ftd_grid_rank))
+        return CSG_ERR_DEVICE;
     if (ftd_launch_corr(eng, eng->ftd_grid_corr) || ftd_launch_fused(eng, eng->ftd_grid_fused) ||
         ftd_launch_select(eng, eng->ftd_grid_select, quantile))
         return CSG_ERR_DEVICE;
@@ -2624,9 +2647,8 @@ int ftd_tune(csg_engine* eng, float quantile)
     k_ftd_refmap<<<(unsigned)eng->ftd_grid_setup, 256, 0, eng->stream>>>(
         cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_flstart, eng->d_fllen,
         eng->d_fl_refgroup, eng->d_fl_refrow);
-    k_ftd_rank_qry<<<(unsigned)eng->ftd_grid_rank, 128, d.rank_qry, eng->stream>>>(
-        cnt, eng->d_fl_items, eng->d_fl_groups, eng->d_fl_ggenes, eng->d_fqdata, eng->d_fqind,
-        eng->d_fqptr, Gf, eng->fnbins, eng->d_ft_cr, eng->d_ft_csq);
+    if (ftd_launch_rank_qry(eng, eng->ftd_grid_rank))
+        return CSG_ERR_DEVICE;
     k_ftd_tiles<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
                                                 (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0,
                                                 eng->d_fl_gthin0, eng->d_fl_tiles, eng->d_fl_thin);

@@ -1137,12 +1137,14 @@ __global__ __launch_bounds__(128) void k_ftd_rank_ref(
     }
 }
 
+// Query rows, one block per cell item; items of groups whose chunk length
+// ceil(gs / 128) is at most step_lo are k_ftd_rank_qry_w's.
 __global__ __launch_bounds__(128) void k_ftd_rank_qry(
     const FtdCounts* __restrict__ cnt, const FtRankItem* __restrict__ items,
     const FtGroup* __restrict__ groups, const uint32_t* __restrict__ genes,
     const uint16_t* __restrict__ qdata, const int* __restrict__ qind,
     const int* __restrict__ qptr, uint32_t Gf, int nbins, int16_t* __restrict__ out,
-    float* __restrict__ out_sq)
+    float* __restrict__ out_sq, int step_lo)
 {
     extern __shared__ int smem[];
     __shared__ float part[128];
@@ -1150,9 +1152,148 @@ __global__ __launch_bounds__(128) void k_ftd_rank_qry(
     const uint32_t   n = cnt->n_cells;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x)
     {
+        const FtRankItem it = items[i];
+        if (((int)groups[it.group].gs + 127) / 128 <= step_lo)
+            continue;  // uniform per block
         __syncthreads();
-        ft_rank_qry_item(items[i], i, groups, genes, qdata, qind, qptr, Gf, nbins, out, out_sq,
-                         smem, part, tmp);
+        ft_rank_qry_item(it, i, groups, genes, qdata, qind, qptr, Gf, nbins, out, out_sq, smem,
+                         part, tmp);
+    }
+}
+
+// Per-wave LDS (ints) of k_ftd_rank_qry_w: 256 code bins, the staged rank
+// row of 128 STEP int16, the dense row of Gf 16-bit codes (16-byte rounded).
+__host__ __device__ constexpr int ftd_qry_wave_ints(int step, uint32_t Gf)
+{
+    return 256 + 64 * step + (int)(((Gf + 1) / 2 + 3) & ~3u);
+}
+
+// Query rows of the fine-tune, one wave per cell item (four per block), for
+// codes below 256 and G_sub of at most 128 STEP genes: the cell's CSR row
+// densified over the fine-tune genes in the wave's LDS, gathered at G_sub in
+// the chunk layout of k_ftd_fused_r (the gene indices kept in registers
+// across a wave's run of consecutive items, which mostly share a group),
+// counted into 256 LDS bins (ballots for code 0, LDS atomics), turned into
+// rank values by a DPP scan over 16-byte bin quads, stored through an LDS
+// row (coalesced, zero-padded to the group's stride), with the chunk-tree
+// sum of squares (wave_tree_sum: the same additions as the block tree of
+// ft_rank_qry_item).
+template <int STEP>
+__global__ __launch_bounds__(256) void k_ftd_rank_qry_w(
+    const FtdCounts* __restrict__ cnt, const FtRankItem* __restrict__ items,
+    const FtGroup* __restrict__ groups, const uint32_t* __restrict__ genes,
+    const uint16_t* __restrict__ qdata, const int* __restrict__ qind,
+    const int* __restrict__ qptr, uint32_t Gf, int16_t* __restrict__ out,
+    float* __restrict__ out_sq)
+{
+    extern __shared__ __attribute__((aligned(16))) int qsm[];
+    const int      wid   = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int      wints = ftd_qry_wave_ints(STEP, Gf);
+    const int      dints = wints - 256 - 64 * STEP;
+    int*           bins  = qsm + wid * wints;
+    int4*          bins4 = (int4*)bins;
+    int16_t*       row   = (int16_t*)(bins + 256);
+    int*           dw    = bins + 256 + 64 * STEP;
+    uint16_t*      dense = (uint16_t*)dw;
+    const uint32_t n     = cnt->n_cells;
+    const uint32_t waves = gridDim.x * 4;
+    const uint32_t rl    = max(1u, (n + waves - 1) / waves);
+    uint32_t       cur   = 0xFFFFFFFFu;
+    bool           mine  = false;
+    int            gs = 0, step = 0, stride = 0;
+    uint32_t       ok = 0;
+    uint32_t       gl[2][STEP];
+    for (uint64_t run = (uint64_t)blockIdx.x * 4 + wid; run * rl < n; run += waves)
+    {
+        const uint32_t i1 = (uint32_t)min((uint64_t)n, run * rl + rl);
+        for (uint32_t idx = (uint32_t)(run * rl); idx < i1; ++idx)
+        {
+            const FtRankItem it = items[idx];
+            if (it.group != cur)
+            {
+                cur               = it.group;
+                const FtGroup g   = groups[cur];
+                gs                = (int)g.gs;
+                step              = (gs + 127) / 128;
+                stride            = (int)g.stride;
+                mine              = step <= STEP;
+                ok                = 0;
+                if (mine)
+                {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int j = 0; j < STEP; ++j)
+                        {
+                            const int  i = (2 * lane + h) * step + j;
+                            const bool v = j < step && i < gs;
+                            ok |= (uint32_t)v << (h * STEP + j);
+                            gl[h][j] = genes[g.gene_off + (v ? i : 0)];
+                        }
+                }
+            }
+            if (!mine)
+                continue;
+            for (int w = lane; w < dints; w += 64)
+                dw[w] = 0;
+            bins4[lane] = make_int4(0, 0, 0, 0);
+            __builtin_amdgcn_wave_barrier();
+            const int beg = qptr[it.src_row], end = qptr[it.src_row + 1];
+            for (int t = beg + lane; t < end; t += 64)
+                dense[qind[t]] = qdata[t];
+            __builtin_amdgcn_wave_barrier();
+            uint32_t code[2][STEP];
+            int      zeros = 0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int j = 0; j < STEP; ++j)
+                {
+                    code[h][j]   = dense[gl[h][j]];
+                    const bool v = (ok >> (h * STEP + j)) & 1u;
+                    zeros += __builtin_popcountll(__ballot(v && code[h][j] == 0));
+                    if (v && code[h][j] != 0)
+                        atomicAdd(&bins[code[h][j]], 1);
+                }
+            __builtin_amdgcn_wave_barrier();
+            int4 b = bins4[lane];
+            if (lane == 0)
+                b.x = zeros;  // no atomic touched bin 0
+            const int s  = b.x + b.y + b.z + b.w;
+            int       ex = wave_incl_scan(s) - s;
+            int4      r;
+            r.x = 2 * ex + b.x - gs;
+            ex += b.x;
+            r.y = 2 * ex + b.y - gs;
+            ex += b.y;
+            r.z = 2 * ex + b.z - gs;
+            ex += b.z;
+            r.w         = 2 * ex + b.w - gs;
+            bins4[lane] = r;
+            __builtin_amdgcn_wave_barrier();
+            float q[2] = { 0.f, 0.f };
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int j = 0; j < STEP; ++j)
+                {
+                    if (j >= step)
+                        break;
+                    // a slot past G_sub stores the padding zero
+                    const int y = (ok >> (h * STEP + j)) & 1u ? bins[code[h][j]] : 0;
+                    row[(2 * lane + h) * step + j] = (int16_t)y;
+                    const float yf = (float)y;
+                    q[h]           = __builtin_fmaf(yf, yf, q[h]);
+                }
+            __builtin_amdgcn_wave_barrier();
+            int16_t* dst = out + it.dst;
+            for (int i = lane; i < stride; i += 64)
+                dst[i] = row[i];
+            const float p = wave_tree_sum(q[0] + q[1]);
+            if (lane == 0)
+                out_sq[idx] = p;
+            __builtin_amdgcn_wave_barrier();  // LDS reads done before the next item
+        }
     }
 }
 
