@@ -974,14 +974,19 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
     for (uint64_t run = (uint64_t)blockIdx.x * 4 + wid; run * rl < n; run += waves)
     {
         const uint64_t base = run * rl;
-        uint32_t       rg   = 0;
-        bool           fz   = false;
+        uint32_t       rg = 0, rr = 0;
+        bool           fz = false;
         if (lane < rl && base + lane < n)
         {
             rg = refgroup[base + lane];
+            rr = refrow[base + lane];
             fz = ((gflag[rg] & FTD_FUSED) != 0) != RANKS;
         }
         uint64_t m = __ballot(fz);
+        // the next row's codes, gathered while this row is ranked (only
+        // when it is a row of the same group: the gene registers hold)
+        bool     pf = false;
+        uint32_t pcode[2][STEP];
         while (m)
         {
             const int k = __builtin_ctzll(m);
@@ -990,6 +995,7 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
             const uint64_t item = base + (uint64_t)k;
             if (gi != cur)
             {
+                pf = false;
                 cur            = gi;
                 g              = groups[gi];
                 gs             = (int)g.gs;
@@ -1028,14 +1034,42 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
             }
             if (!mine)
                 continue;
-            const uint8_t* src = ref_codes + (size_t)refrow[item] * Gf;
-            bins4[lane]        = make_int4(0, 0, 0, 0);
             uint32_t code[2][STEP];
+            if (pf)
+            {
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+                for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int j = 0; j < STEP; ++j)
-                    code[h][j] = src[gl[h][j]];
+                    for (int j = 0; j < STEP; ++j)
+                        code[h][j] = pcode[h][j];
+            }
+            else
+            {
+                const uint8_t* src =
+                    ref_codes + (size_t)__builtin_amdgcn_readlane(rr, k) * Gf;
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int j = 0; j < STEP; ++j)
+                        code[h][j] = src[gl[h][j]];
+            }
+            pf = false;
+            if (m)
+            {
+                const int k2 = __builtin_ctzll(m);
+                if ((uint32_t)__builtin_amdgcn_readlane(rg, k2) == cur)
+                {
+                    const uint8_t* src2 =
+                        ref_codes + (size_t)__builtin_amdgcn_readlane(rr, k2) * Gf;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int j = 0; j < STEP; ++j)
+                            pcode[h][j] = src2[gl[h][j]];
+                    pf = true;
+                }
+            }
+            bins4[lane] = make_int4(0, 0, 0, 0);
             __builtin_amdgcn_wave_barrier();
             int zeros = 0;
 #pragma unroll
