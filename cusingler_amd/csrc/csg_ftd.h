@@ -955,11 +955,15 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
     const float* __restrict__ cell_sq, float* __restrict__ scores,
     int16_t* __restrict__ out_rank = nullptr, float* __restrict__ out_sq = nullptr)
 {
-    __shared__ __attribute__((aligned(16))) int sbins[4][256];
+    // two copies of the 256 bins per wave (lanes 0-31 and 32-63 count into
+    // their own, halving the same-address conflicts of the atomics; the
+    // second starts 4 banks on), summed into the first for the rank values
+    __shared__ __attribute__((aligned(16))) int sbins[4][2 * 256 + 4];
     __shared__ __attribute__((aligned(16))) int16_t srow[4][RANKS ? 128 * STEP : 2];
     const int      wid   = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int*           bins  = sbins[wid];
     int4*          bins4 = (int4*)bins;
+    int*           mybin = bins + (lane >> 5) * (256 + 4);
     const uint64_t n     = cnt->n_ref_items;
     uint32_t       cur   = 0xFFFFFFFFu;
     bool           mine  = false;
@@ -1069,7 +1073,8 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
                     pf = true;
                 }
             }
-            bins4[lane] = make_int4(0, 0, 0, 0);
+            bins4[lane]      = make_int4(0, 0, 0, 0);
+            bins4[65 + lane] = make_int4(0, 0, 0, 0);
             __builtin_amdgcn_wave_barrier();
             int zeros = 0;
 #pragma unroll
@@ -1080,10 +1085,15 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
                     const bool v = (ok >> (h * STEP + j)) & 1u;
                     zeros += __builtin_popcountll(__ballot(v && code[h][j] == 0));
                     if (v && code[h][j] != 0)
-                        atomicAdd(&bins[code[h][j]], 1);
+                        atomicAdd(&mybin[code[h][j]], 1);
                 }
             __builtin_amdgcn_wave_barrier();
-            int4 b = bins4[lane];
+            int4       b  = bins4[lane];
+            const int4 b2 = bins4[65 + lane];
+            b.x += b2.x;
+            b.y += b2.y;
+            b.z += b2.z;
+            b.w += b2.w;
             if (lane == 0)
                 b.x = zeros;  // no atomic touched bin 0
             const int s  = b.x + b.y + b.z + b.w;
