@@ -239,6 +239,7 @@ struct csg_engine
     // ---- timing / counters ----
     double  phase_ms[CSG_N_PHASES]  = {};
     int64_t phase_cnt[CSG_N_PHASES] = {};
+    int     phase_mode = 2;  // 0 no phase events, 1 the correlation phase only, 2 all
     std::vector<PhaseEvents> pending;
     std::vector<hipEvent_t>  event_pool;
     int64_t counters[CSG_N_COUNTERS] = {};
@@ -343,8 +344,12 @@ struct PhaseScope
 {
     csg_engine* eng;
     PhaseEvents pe;
-    PhaseScope(csg_engine* e, int phase, int launches) : eng(e)
+    bool        on;
+    PhaseScope(csg_engine* e, int phase, int launches)
+        : eng(e), on(e->phase_mode == 2 || (e->phase_mode == 1 && phase == 1))
     {
+        if (!on)
+            return;
         pe.phase    = phase;
         pe.a        = take_event(e);
         pe.b        = take_event(e);
@@ -353,6 +358,8 @@ struct PhaseScope
     }
     ~PhaseScope()
     {
+        if (!on)
+            return;
         (void)hipEventRecord(pe.b, eng->stream);
         eng->pending.push_back(pe);
     }
@@ -3421,6 +3428,14 @@ int csg_set_finetune_path(csg_engine* eng, int path)
     if (!eng || path < CSG_PATH_PLAIN || path > CSG_PATH_FP16_MFMA)
         return CSG_ERR_ARGUMENT;
     eng->ft_exact = path == CSG_PATH_EXACT_INT8;
+    return CSG_OK;
+}
+
+int csg_set_phase_timing(csg_engine* eng, int mode)
+{
+    if (!eng || mode < 0 || mode > 2)
+        return CSG_ERR_ARGUMENT;
+    eng->phase_mode = mode;
     return CSG_OK;
 }
 

@@ -79,6 +79,7 @@ def lib():
     L.csg_set_finetune_budget.argtypes = [_V, _U64, _V]
     L.csg_set_finetune_fuse.argtypes = [_V, C.c_int]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
+    L.csg_set_phase_timing.argtypes = [_V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
     L.csg_selftest.argtypes = [_V, C.c_int, P(C.c_double)]
     L.csg_selftest_ft.argtypes = [_V, C.c_int, P(_I64)]
@@ -347,6 +348,10 @@ class Engine:
     def set_proposal(self, k: int, min_margin: float = 0.2):
         """Proposal-restricted exact scoring (k labels per cell; 0 turns it off)."""
         self._check(lib().csg_set_proposal(self._h, k, min_margin), "set_proposal")
+
+    def set_phase_timing(self, mode: int) -> None:
+        """0: no phase events, 1: the correlation phase only, 2: every phase."""
+        self._check(lib().csg_set_phase_timing(self._h, int(mode)), "set_phase_timing")
 
     def phase_times(self, reset: bool = False) -> Dict[str, tuple]:
         ms = np.zeros(N_PHASES, np.float64)

@@ -158,6 +158,14 @@ int csg_destroy(csg_engine* eng);
 #define CSG_N_PHASES 10
 int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
 
+/* Which phases csg_phase_times times (no counterpart in the reference: an
+ * instrumentation switch of this engine).  Every timed phase records two HIP
+ * events on the stream, which costs about 1% of a proposal-path step at the
+ * atlas size.  mode 0: none; 1: the score-step correlation phase only (the
+ * contraction's live launch time, as bench.py's roofline uses it); 2: every
+ * phase (the default).  CSG_ERR_ARGUMENT for any other mode. */
+int csg_set_phase_timing(csg_engine* eng, int mode);
+
 /* Counters of the last csg_get_label / csg_finetune call:
  * [0] query cells, [1] score-step gene count G_s, [2] reference cells,
  * [3] (cell, label) pairs re-scored in FP32 by the exact path,
