@@ -72,8 +72,8 @@ def cpu_baseline(ds, n_sample: int, cores: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="learned")
     ap.add_argument("--nq", type=int, default=None, help="override the query cell count")
     ap.add_argument("--form", "--path", dest="form", default="proposal",
