@@ -84,6 +84,8 @@ def main():
                     help="training cells for the proposal network (default: the config's, "
                          "else 500,000)")
     ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--hidden", type=int, default=256,
+                    help="width of the proposal network's two hidden layers (a multiple of 256)")
     ap.add_argument("--proposal-k", type=int, default=4)
     ap.add_argument("--min-margin", type=float, default=0.2)
     ap.add_argument("--net-cache", default=None,
@@ -142,12 +144,13 @@ def main():
         ylab, _ = eng.finetune(0.8, 0.05, 0)
         _hip_sync()
         t2 = time.perf_counter()
-        eng.mlp_init(512, seed=1)
+        eng.mlp_init(args.hidden, seed=1)
         losses = eng.mlp_train(ylab.astype(np.int32), epochs=args.epochs, batch=1024, lr=1e-3,
                                seed=2)
         t3 = time.perf_counter()
         training = {"cells": int(ylab.shape[0]), "labels_from": "plain finalLabel",
-                    "network": f"G_s -> 512 -> 512 -> {len(ds.codes)} (GELU), BF16 MFMA, FP32 Adam",
+                    "network": f"G_s -> {args.hidden} -> {args.hidden} -> {len(ds.codes)} (GELU), BF16 MFMA, "
+                               "FP32 Adam",
                     "epochs": args.epochs, "batch": 1024, "epoch_loss": [round(float(v), 5) for v in losses],
                     "prep_s": round(t1 - t0, 2), "plain_labelling_s": round(t2 - t1, 3),
                     "train_s": round(t3 - t2, 3)}
@@ -169,7 +172,7 @@ def main():
     eng.copyin(c1)  # fine-tune inputs; the candidate mask comes from the device
     Nq, Nr, Gs = ds.qry_height, ds.ref_height, c0.width
     if cached:
-        eng.mlp_init(512, seed=1)
+        eng.mlp_init(args.hidden, seed=1)
         eng.set_mlp_params(net)
 
     def measure(form):
