@@ -270,3 +270,62 @@ def test_device_finetune_fused_form(eng):
         eng.set_finetune_path("plain")
     eng.destroy()
     eng.destroy_score()
+
+
+def _jittered(ds, seed):
+    """`ds` with every stored value scaled by its own factor in (1, 1.25):
+    almost every value of a row is distinct, so rows hold hundreds of value
+    codes (16-bit codes)."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+
+    def j(d):
+        return (d * (1.0 + rng.integers(1, 1000, size=d.shape[0]) / 4096.0)).astype(np.float32)
+
+    return dataclasses.replace(ds, ref_data=j(ds.ref_data), qry_data=j(ds.qry_data))
+
+
+@pytest.mark.parametrize("case", ["long_gene_sets", "wide_codes"])
+def test_device_finetune_kernel_variants(eng, case):
+    """The device fine-tune's kernel variants beyond the byte-code,
+    1024-gene form give the host-grouped schedule's labels and round counts:
+    fine-tune gene sets above 1024 genes (the LDS-table rank and fused
+    kernels beside the register-resident ones) and more than 256 value codes
+    (16-bit codes), in FP32 and exact form, fused form on and off."""
+    from cusingler_amd import engine, synth
+    if case == "long_gene_sets":
+        # G_f = 1120 (the device fine-tune's LDS plan takes G_f up to about 1250)
+        ds = synth.tiny(n_ref_per_label=40, n_labels=10, n_genes=4000, n_qry=300, markers=6,
+                        seed=31)
+    else:
+        ds = _jittered(synth.tiny(n_ref_per_label=40, n_labels=8, n_genes=1500, n_qry=300,
+                                  markers=8, nonzero=0.5, seed=33), 5)
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    codes = max(int(c1.ref_data.max()), int(c1.qry_data.max()))
+    _say("kernel variants", case, "G_f", c1.width, "codes", codes)
+    if case == "long_gene_sets":
+        assert c1.width > 1024
+    else:
+        assert codes > 255
+    eng.copyin_score(c0)
+    eng.get_label(0.8, 0.3, "plain", outputs=False)  # wide threshold: many labels per cell
+    eng.copyin(c1)
+    try:
+        for path in ("plain", "exact_int8"):
+            eng.set_finetune_path(path)
+            eng.set_finetune_schedule("host")
+            ref_final, ref_rounds = eng.finetune(0.8, 0.3, 0)
+            eng.set_finetune_schedule("device")
+            for fuse in (0, 4):
+                eng.set_finetune_fuse(fuse)
+                final, rounds = eng.finetune(0.8, 0.3, 0)
+                _say("kernel variants", case, path, "fuse", fuse)
+                assert np.array_equal(final, ref_final), (case, path, fuse)
+                assert np.array_equal(rounds, ref_rounds), (case, path, fuse)
+    finally:
+        eng.set_finetune_fuse(4)
+        eng.set_finetune_schedule("device")
+        eng.set_finetune_path("plain")
+    eng.destroy()
+    eng.destroy_score()
