@@ -113,9 +113,19 @@ __global__ void k_ftd_init(const float* __restrict__ mask, uint32_t Nq, uint32_t
     const float* m = mask + (size_t)c * L;
     uint8_t*     o = lists + (size_t)c * L;
     uint32_t     k = 0;
-    for (uint32_t l = 0; l < L; ++l)
-        if (m[l] != 0.f)
-            o[k++] = (uint8_t)l;
+    // eight mask values loaded together, then listed (the list stores would
+    // otherwise wait behind each load)
+    for (uint32_t l0 = 0; l0 < L; l0 += 8)
+    {
+        float mv[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+            mv[u] = m[min(l0 + u, L - 1)];  // (past the row: its last value, not used)
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+            if (l0 + u < L && mv[u] != 0.f)
+                o[k++] = (uint8_t)(l0 + u);
+    }
     counts[c] = k;
     rounds[c] = 0;
     if (k > 1 && (times == 0 || 0 < times))
