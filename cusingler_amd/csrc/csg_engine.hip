@@ -178,8 +178,11 @@ struct csg_engine
     } fl_cap{};
     uint32_t               ftd_gcap = 0, ftd_T = 0;
     size_t                 ftd_gstride = 0;
-    hipGraphExec_t         ftd_gA = nullptr;
+    hipGraphExec_t         ftd_gA = nullptr;   // part A, many cells
     std::vector<uintptr_t> ftd_sigA;
+    hipGraphExec_t         ftd_gAs = nullptr;  // part A, a round of few cells
+    std::vector<uintptr_t> ftd_sigAs;
+    bool                   ftd_small_a = false;  // the next part A is the few-cell one
     // part B graphs, one per launch mask (which work-queue kernels a round
     // needs, from its counts)
     struct FtdGraphB
@@ -2149,6 +2152,10 @@ constexpr int FTD_TPB = 256;  // threads of the thread-level work-queue kernels
 
 int ftd_free(csg_engine* eng)
 {
+    if (eng->ftd_gAs)
+        (void)hipGraphExecDestroy(eng->ftd_gAs);
+    eng->ftd_gAs = nullptr;
+    eng->ftd_sigAs.clear();
     if (eng->ftd_gA)
         (void)hipGraphExecDestroy(eng->ftd_gA);
     for (auto& b : eng->ftd_gB)
@@ -2243,6 +2250,10 @@ int ftd_groups(csg_engine* eng, uint32_t n)
     if (n <= eng->ftd_gcap && eng->d_fl_groups)
         return CSG_OK;
     // the captured rounds name the tables about to move
+    if (eng->ftd_gAs)
+        (void)hipGraphExecDestroy(eng->ftd_gAs);
+    eng->ftd_gAs = nullptr;
+    eng->ftd_sigAs.clear();
     if (eng->ftd_gA)
         (void)hipGraphExecDestroy(eng->ftd_gA);
     eng->ftd_gA = nullptr;
@@ -2350,27 +2361,37 @@ uint32_t ftd_mask_of(const csg_engine* eng, const FtdCounts& h)
 }
 
 // Round part A: grouping, group set-up and the prefix sums.
-int ftd_launch_a(csg_engine* eng)
+int ftd_launch_a(csg_engine* eng, bool small)
 {
     const uint32_t L = eng->fL, T = eng->ftd_T;
     const unsigned g = (unsigned)eng->ftd_grid_threads;
     const FtdLds   d = ftd_lds(eng);
     FtdCounts*     cnt = eng->d_fl_cnt;
-    k_ftd_counters<<<1, 64, 0, eng->stream>>>(cnt, T);
-    k_ftd_begin<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_next,
-                                                (unsigned long long*)eng->d_fl_htab,
-                                                eng->d_fl_gcount);
-    k_ftd_hash<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_lists,
-                                               eng->d_fl_counts, L,
-                                               (unsigned long long*)eng->d_fl_htab,
-                                               eng->d_fl_howner, eng->d_fl_cslot);
-    k_ftd_gid<<<g, FTD_TPB, 0, eng->stream>>>(cnt, (const unsigned long long*)eng->d_fl_htab,
-                                              eng->d_fl_howner, eng->d_fl_hgid,
-                                              eng->d_fl_gowner);
-    k_ftd_member<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_lists,
-                                                 eng->d_fl_counts, L, eng->d_fl_cslot,
-                                                 eng->d_fl_hgid, eng->d_fl_gowner,
-                                                 eng->d_fl_gcount, eng->d_fl_cgid, eng->d_fl_cpos);
+    if (small)
+        // at most FTD_SMALL_NT active cells (this round's can only be fewer
+        // than the last round's): the grouping in one block
+        k_ftd_group_small<<<1, FTD_SMALL_NT, 0, eng->stream>>>(
+            cnt, T, eng->d_fl_act, eng->d_fl_next, (unsigned long long*)eng->d_fl_htab,
+            eng->d_fl_gcount, eng->d_fl_lists, eng->d_fl_counts, L, eng->d_fl_howner,
+            eng->d_fl_cslot, eng->d_fl_hgid, eng->d_fl_gowner, eng->d_fl_cgid, eng->d_fl_cpos);
+    else
+    {
+        k_ftd_counters<<<1, 64, 0, eng->stream>>>(cnt, T);
+        k_ftd_begin<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_next,
+                                                    (unsigned long long*)eng->d_fl_htab,
+                                                    eng->d_fl_gcount);
+        k_ftd_hash<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_lists,
+                                                   eng->d_fl_counts, L,
+                                                   (unsigned long long*)eng->d_fl_htab,
+                                                   eng->d_fl_howner, eng->d_fl_cslot);
+        k_ftd_gid<<<g, FTD_TPB, 0, eng->stream>>>(cnt,
+                                                  (const unsigned long long*)eng->d_fl_htab,
+                                                  eng->d_fl_howner, eng->d_fl_hgid,
+                                                  eng->d_fl_gowner);
+        k_ftd_member<<<g, FTD_TPB, 0, eng->stream>>>(
+            cnt, eng->d_fl_act, eng->d_fl_lists, eng->d_fl_counts, L, eng->d_fl_cslot,
+            eng->d_fl_hgid, eng->d_fl_gowner, eng->d_fl_gcount, eng->d_fl_cgid, eng->d_fl_cpos);
+    }
     k_ftd_setup<<<(unsigned)eng->ftd_grid_setup, FTD_SETUP_NT, d.setup, eng->stream>>>(
         cnt, eng->d_fl_gowner, eng->d_fl_gcount, eng->d_fl_lists, eng->d_fl_counts, L,
         eng->d_fl_ctdidx, eng->d_fl_ctdiff, eng->d_fl_cut, eng->d_fllen,
@@ -2952,13 +2973,18 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
     HIPCHK(hipStreamSynchronize(eng->stream));
     if ((rc = ftd_groups(eng, std::max<uint32_t>(h.n_next, 1))))
         return rc;
+    // the grouping of a round of few cells runs in one block; a round's
+    // active cells are at most the previous round's
+    eng->ftd_small_a = h.n_next <= (uint32_t)FTD_SMALL_NT;
     for (int round = 0; h.n_next > 0; ++round)
     {
         const int64_t t0 = now_us();
-        if ((rc = ftd_graph(eng, &eng->ftd_gA, &eng->ftd_sigA, ftd_sig_a(eng),
-                            [&]() { return ftd_launch_a(eng); })))
+        const bool      small = eng->ftd_small_a;
+        hipGraphExec_t* gA    = small ? &eng->ftd_gAs : &eng->ftd_gA;
+        if ((rc = ftd_graph(eng, gA, small ? &eng->ftd_sigAs : &eng->ftd_sigA, ftd_sig_a(eng),
+                            [&]() { return ftd_launch_a(eng, small); })))
             return rc;
-        HIPCHK(hipGraphLaunch(eng->ftd_gA, eng->stream));
+        HIPCHK(hipGraphLaunch(*gA, eng->stream));
         HIPCHK(hipMemcpyAsync(&h, eng->d_fl_cnt, sizeof(FtdCounts), hipMemcpyDeviceToHost,
                               eng->stream));
         HIPCHK(hipStreamSynchronize(eng->stream));
@@ -2982,7 +3008,8 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
         eng->counters[8] += h.n_groups;
         eng->counters[9] += (int64_t)h.n_scores;
         eng->counters[22] += h.n_fused;
-        eng->ftd_mask = ftd_mask_of(eng, h);
+        eng->ftd_small_a = h.n_active <= (uint32_t)FTD_SMALL_NT;
+        eng->ftd_mask    = ftd_mask_of(eng, h);
         if (ftd_bytes(h, L) <= eng->ftd_budget)
         {
             if ((rc = ftd_work(eng, h)))

@@ -132,33 +132,41 @@ __global__ void k_ftd_init(const float* __restrict__ mask, uint32_t Nq, uint32_t
         act_next[atomicAdd(&cnt->n_next, 1u)] = c;
 }
 
-// Round start, part 2 (after k_ftd_counters): the previous round's next list
-// becomes this round's active list; the group member counters and the hash
-// slots this round uses are cleared.
+// Round start, part 2 (after ftd_counters_body): the previous round's next
+// list becomes this round's active list; the group member counters and the
+// hash slots this round uses are cleared.  (i0, stride: this thread's first
+// index and the threads' count.)
+__device__ inline void ftd_begin_body(const FtdCounts* __restrict__ cnt,
+                                      uint32_t* __restrict__ act,
+                                      const uint32_t* __restrict__ act_next,
+                                      unsigned long long* __restrict__ htab,
+                                      uint32_t* __restrict__ gcount, uint32_t i0, uint32_t stride)
+{
+    const uint32_t n = cnt->n_active;
+    const uint32_t T = cnt->hT;
+    for (uint32_t i = i0; i < n; i += stride)
+    {
+        act[i]    = act_next[i];
+        gcount[i] = 0;
+    }
+    for (uint32_t s = i0; s < T; s += stride)
+        htab[s] = 0ull;
+}
+
 __global__ void k_ftd_begin(const FtdCounts* __restrict__ cnt, uint32_t* __restrict__ act,
                             const uint32_t* __restrict__ act_next,
                             unsigned long long* __restrict__ htab,
                             uint32_t* __restrict__ gcount)
 {
-    const uint32_t n      = cnt->n_active;
-    const uint32_t T      = cnt->hT;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    {
-        act[i]    = act_next[i];
-        gcount[i] = 0;
-    }
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < T; s += stride)
-        htab[s] = 0ull;
+    ftd_begin_body(cnt, act, act_next, htab, gcount, blockIdx.x * blockDim.x + threadIdx.x,
+                   gridDim.x * blockDim.x);
 }
 
 // Round start, part 1 (one thread): counters, and the hash slots of the
 // round: the smallest power of two of at least 2 n_active (and 1024), at most
 // T, so that an insert always finds a free slot.
-__global__ void k_ftd_counters(FtdCounts* __restrict__ cnt, uint32_t T)
+__device__ inline void ftd_counters_body(FtdCounts* __restrict__ cnt, uint32_t T)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0)
-        return;
     const uint32_t na = cnt->n_next;
     uint32_t       hT = 1024;
     while (hT < 2 * na && hT < T)
@@ -189,14 +197,24 @@ __global__ void k_ftd_counters(FtdCounts* __restrict__ cnt, uint32_t T)
     cnt->rthin_bound = 0;
 }
 
-__global__ void k_ftd_hash(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
-                           const uint8_t* __restrict__ lists, const uint32_t* __restrict__ counts,
-                           uint32_t L, unsigned long long* __restrict__ htab,
-                           uint32_t* __restrict__ howner, uint32_t* __restrict__ cslot)
+__global__ void k_ftd_counters(FtdCounts* __restrict__ cnt, uint32_t T)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    ftd_counters_body(cnt, T);
+}
+
+__device__ inline void ftd_hash_body(const FtdCounts* __restrict__ cnt,
+                                     const uint32_t* __restrict__ act,
+                                     const uint8_t* __restrict__ lists,
+                                     const uint32_t* __restrict__ counts, uint32_t L,
+                                     unsigned long long* __restrict__ htab,
+                                     uint32_t* __restrict__ howner, uint32_t* __restrict__ cslot,
+                                     uint32_t i0, uint32_t stride)
 {
     const uint32_t n = cnt->n_active;
     const uint32_t T = cnt->hT;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (uint32_t i = i0; i < n; i += stride)
     {
         const uint32_t c = act[i];
         uint64_t       key[4];
@@ -220,12 +238,23 @@ __global__ void k_ftd_hash(const FtdCounts* __restrict__ cnt, const uint32_t* __
     }
 }
 
-__global__ void k_ftd_gid(FtdCounts* __restrict__ cnt, const unsigned long long* __restrict__ htab,
-                          const uint32_t* __restrict__ howner, uint32_t* __restrict__ hgid,
-                          uint32_t* __restrict__ gowner)
+__global__ void k_ftd_hash(const FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+                           const uint8_t* __restrict__ lists, const uint32_t* __restrict__ counts,
+                           uint32_t L, unsigned long long* __restrict__ htab,
+                           uint32_t* __restrict__ howner, uint32_t* __restrict__ cslot)
+{
+    ftd_hash_body(cnt, act, lists, counts, L, htab, howner, cslot,
+                  blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__device__ inline void ftd_gid_body(FtdCounts* __restrict__ cnt,
+                                    const unsigned long long* __restrict__ htab,
+                                    const uint32_t* __restrict__ howner,
+                                    uint32_t* __restrict__ hgid, uint32_t* __restrict__ gowner,
+                                    uint32_t i0, uint32_t stride)
 {
     const uint32_t T = cnt->hT;
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < T; s += gridDim.x * blockDim.x)
+    for (uint32_t s = i0; s < T; s += stride)
         if (htab[s])
         {
             const uint32_t g = atomicAdd(&cnt->n_groups, 1u);
@@ -234,15 +263,26 @@ __global__ void k_ftd_gid(FtdCounts* __restrict__ cnt, const unsigned long long*
         }
 }
 
-__global__ void k_ftd_member(FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
-                             const uint8_t* __restrict__ lists,
-                             const uint32_t* __restrict__ counts, uint32_t L,
-                             const uint32_t* __restrict__ cslot, const uint32_t* __restrict__ hgid,
-                             const uint32_t* __restrict__ gowner, uint32_t* __restrict__ gcount,
-                             uint32_t* __restrict__ cgid, uint32_t* __restrict__ cpos)
+__global__ void k_ftd_gid(FtdCounts* __restrict__ cnt, const unsigned long long* __restrict__ htab,
+                          const uint32_t* __restrict__ howner, uint32_t* __restrict__ hgid,
+                          uint32_t* __restrict__ gowner)
+{
+    ftd_gid_body(cnt, htab, howner, hgid, gowner, blockIdx.x * blockDim.x + threadIdx.x,
+                 gridDim.x * blockDim.x);
+}
+
+__device__ inline void ftd_member_body(FtdCounts* __restrict__ cnt,
+                                       const uint32_t* __restrict__ act,
+                                       const uint8_t* __restrict__ lists,
+                                       const uint32_t* __restrict__ counts, uint32_t L,
+                                       const uint32_t* __restrict__ cslot,
+                                       const uint32_t* __restrict__ hgid,
+                                       const uint32_t* __restrict__ gowner,
+                                       uint32_t* __restrict__ gcount, uint32_t* __restrict__ cgid,
+                                       uint32_t* __restrict__ cpos, uint32_t i0, uint32_t stride)
 {
     const uint32_t n = cnt->n_active;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (uint32_t i = i0; i < n; i += stride)
     {
         const uint32_t c = act[i];
         const uint32_t g = hgid[cslot[i]];
@@ -258,6 +298,44 @@ __global__ void k_ftd_member(FtdCounts* __restrict__ cnt, const uint32_t* __rest
         cgid[i] = g;
         cpos[i] = atomicAdd(&gcount[g], 1u);
     }
+}
+
+__global__ void k_ftd_member(FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
+                             const uint8_t* __restrict__ lists,
+                             const uint32_t* __restrict__ counts, uint32_t L,
+                             const uint32_t* __restrict__ cslot, const uint32_t* __restrict__ hgid,
+                             const uint32_t* __restrict__ gowner, uint32_t* __restrict__ gcount,
+                             uint32_t* __restrict__ cgid, uint32_t* __restrict__ cpos)
+{
+    ftd_member_body(cnt, act, lists, counts, L, cslot, hgid, gowner, gcount, cgid, cpos,
+                    blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+// The grouping of a round of at most FTD_SMALL_NT active cells in one block
+// (the five launches above as barrier-separated phases of one launch, the
+// same bodies): a round of a few cells pays one launch instead of five.
+constexpr int FTD_SMALL_NT = 1024;
+
+__global__ __launch_bounds__(FTD_SMALL_NT) void k_ftd_group_small(
+    FtdCounts* __restrict__ cnt, uint32_t T, uint32_t* __restrict__ act,
+    const uint32_t* __restrict__ act_next, unsigned long long* __restrict__ htab,
+    uint32_t* __restrict__ gcount, const uint8_t* __restrict__ lists,
+    const uint32_t* __restrict__ counts, uint32_t L, uint32_t* __restrict__ howner,
+    uint32_t* __restrict__ cslot, uint32_t* __restrict__ hgid, uint32_t* __restrict__ gowner,
+    uint32_t* __restrict__ cgid, uint32_t* __restrict__ cpos)
+{
+    const uint32_t t = threadIdx.x;
+    if (t == 0)
+        ftd_counters_body(cnt, T);
+    __syncthreads();
+    ftd_begin_body(cnt, act, act_next, htab, gcount, t, FTD_SMALL_NT);
+    __syncthreads();
+    ftd_hash_body(cnt, act, lists, counts, L, htab, howner, cslot, t, FTD_SMALL_NT);
+    __syncthreads();
+    ftd_gid_body(cnt, htab, howner, hgid, gowner, t, FTD_SMALL_NT);
+    __syncthreads();
+    ftd_member_body(cnt, act, lists, counts, L, cslot, hgid, gowner, gcount, cgid, cpos, t,
+                    FTD_SMALL_NT);
 }
 
 // One block per group: ascending labels, their row offsets, G_sub.
