@@ -85,7 +85,7 @@ def main():
                          "else 500,000)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--hidden", type=int, default=256,
-                    help="width of the proposal network's two hidden layers (a multiple of 256)")
+                    help="width of the proposal network's two hidden layers (multiples of 256 take the pipelined layer kernel)")
     ap.add_argument("--proposal-k", type=int, default=4)
     ap.add_argument("--min-margin", type=float, default=0.2)
     ap.add_argument("--net-cache", default=None,
