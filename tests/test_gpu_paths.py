@@ -329,3 +329,37 @@ def test_device_finetune_kernel_variants(eng, case):
         eng.set_finetune_path("plain")
     eng.destroy()
     eng.destroy_score()
+
+
+def test_phase_timing_modes(eng):
+    """csg_set_phase_timing: mode 0 records no phase, mode 1 only the score
+    step's correlation, mode 2 every phase; other modes are rejected.  The
+    labels do not depend on the mode."""
+    from cusingler_amd import engine, synth
+    ds = synth.generate("small")
+    prep = engine.Prep(ds, 8)
+    c0 = prep.generate(0)
+    eng.copyin_score(c0)
+    labels = {}
+    try:
+        for mode in (0, 1, 2):
+            eng.set_phase_timing(mode)
+            eng.phase_times(reset=True)
+            first, _, _ = eng.get_label(0.8, 0.05, "exact_int8", outputs=True)
+            ph = eng.phase_times(reset=True)
+            labels[mode] = first
+            _say("phase timing mode", mode, {k: v[1] for k, v in ph.items()})
+            timed = {k for k, v in ph.items() if v[1] > 0}
+            if mode == 0:
+                assert not timed
+            elif mode == 1:
+                assert timed == {"corr"}
+            else:
+                assert {"rank", "corr", "select", "label"} <= timed
+        for bad in (-1, 3):
+            with pytest.raises(engine.CsgError):
+                eng.set_phase_timing(bad)
+    finally:
+        eng.set_phase_timing(2)
+    assert np.array_equal(labels[0], labels[1]) and np.array_equal(labels[0], labels[2])
+    eng.destroy_score()
