@@ -2407,7 +2407,7 @@ int ftd_launch_a(csg_engine* eng, bool small)
 
 // Reference-row ranks of the groups that are not fused: with byte codes the
 // register-resident kernel (k_ftd_fused_r in its rank mode) for G_sub up to
-// 1024 genes; longer gene sets and 16-bit codes one wave per row when the
+// 1280 genes; longer gene sets and 16-bit codes one wave per row when the
 // per-wave LDS fits, else one block per row.
 int ftd_launch_rank_ref(csg_engine* eng, int grid)
 {
@@ -2417,21 +2417,27 @@ int ftd_launch_rank_ref(csg_engine* eng, int grid)
     int            lo  = 0;  // rows of chunk length <= lo already ranked
     if (eng->d_fref_codes8)
     {
-        // byte codes: G_sub up to 1024 genes in the register-resident kernel
-        k_ftd_fused_r<4, 0, true><<<(unsigned)grid, 256, 0, eng->stream>>>(
-            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-            eng->d_fl_ggenes, eng->d_fref_codes8, Gf, nullptr, nullptr, nullptr, eng->d_ft_rr,
-            eng->d_ft_rsq);
+        // byte codes: G_sub up to 1280 genes in the register-resident
+        // kernel, by chunk length classes (0, 4], (4, 8], (8, 10]
+#define CSG_FTD_RANK_R(S, LO)                                                               \
+    k_ftd_fused_r<S, LO, true><<<(unsigned)grid, 256, 0, eng->stream>>>(                    \
+        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,       \
+        eng->d_fl_ggenes, eng->d_fref_codes8, Gf, nullptr, nullptr, nullptr, eng->d_ft_rr, \
+        eng->d_ft_rsq)
+        CSG_FTD_RANK_R(4, 0);
         lo = 4;
         if (Gf > 512)
         {
-            k_ftd_fused_r<8, 4, true><<<(unsigned)grid, 256, 0, eng->stream>>>(
-                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-                eng->d_fl_ggenes, eng->d_fref_codes8, Gf, nullptr, nullptr, nullptr,
-                eng->d_ft_rr, eng->d_ft_rsq);
+            CSG_FTD_RANK_R(8, 4);
             lo = 8;
         }
-        if (Gf <= 1024)
+        if (Gf > 1024)
+        {
+            CSG_FTD_RANK_R(10, 8);
+            lo = 10;
+        }
+#undef CSG_FTD_RANK_R
+        if (Gf <= 1280)
         {
             LAUNCHCHK();
             return CSG_OK;
@@ -2518,23 +2524,25 @@ int ftd_launch_fused(csg_engine* eng, int grid)
     const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1);
     if (eng->d_fref_codes8)
     {
-        // byte codes: chunk lengths (0, 4] and (4, 8] with the cells in
-        // registers, longer ones (G_sub above 1024 genes) through the
-        // LDS-table kernel
-        k_ftd_fused_r<4, 0><<<(unsigned)grid, 256, 0, eng->stream>>>(
-            cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-            eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->d_ft_cr, eng->d_ft_csq,
-            eng->d_ft_scores);
+        // byte codes: chunk lengths (0, 4], (4, 8] and (8, 10] with the
+        // cells in registers, longer ones (G_sub above 1280 genes, beyond
+        // the FP32 tiles' LDS plan today) through the LDS-table kernel
+#define CSG_FTD_FUSED_R(S, LO)                                                          \
+    k_ftd_fused_r<S, LO><<<(unsigned)grid, 256, 0, eng->stream>>>(                      \
+        cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,   \
+        eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->d_ft_cr, eng->d_ft_csq,          \
+        eng->d_ft_scores)
+        CSG_FTD_FUSED_R(4, 0);
         if (Gf > 512)
-            k_ftd_fused_r<8, 4><<<(unsigned)grid, 256, 0, eng->stream>>>(
-                cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
-                eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->d_ft_cr, eng->d_ft_csq,
-                eng->d_ft_scores);
+            CSG_FTD_FUSED_R(8, 4);
         if (Gf > 1024)
+            CSG_FTD_FUSED_R(10, 8);
+#undef CSG_FTD_FUSED_R
+        if (Gf > 1280)
             k_ftd_fused<uint8_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(
                 cnt, eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_refgroup, eng->d_fl_refrow,
                 eng->d_fl_ggenes, eng->d_fref_codes8, Gf, eng->fnbins, d.wave_ints, eng->d_ft_cr,
-                eng->d_ft_csq, eng->d_ft_scores, 8);
+                eng->d_ft_csq, eng->d_ft_scores, 10);
     }
     else
         k_ftd_fused<uint16_t><<<(unsigned)grid, 256, d.rank_ref_w, eng->stream>>>(

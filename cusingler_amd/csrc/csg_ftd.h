@@ -971,8 +971,8 @@ __global__ __launch_bounds__(256) void k_ftd_fused(
 constexpr int FTD_FUSED_RUN = 64;
 
 // Two cells' rank values of one element, in half precision: a doubled
-// centred rank over at most 1024 genes has |x| <= 1025, which FP16 holds
-// exactly.
+// centred rank 2 r - (gs + 1) over gs <= 2048 genes is an integer with
+// |x| <= gs - 1 <= 2047, which FP16 (11 significant bits) holds exactly.
 typedef _Float16 ftd_h2 __attribute__((ext_vector_type(2)));
 
 // Chunk sums of one row for a group of NC cells (lane l: chunks 2l + h of
@@ -1021,8 +1021,8 @@ __device__ inline void ftd_fused_row(const int* bins, const uint32_t (&code)[2][
 // (blocks outnumber the resident slots, so uneven runs even out).
 // Consecutive rows of a group share its gene list and cells, so a lane
 // loads its chunk elements' genes and the cells' rank values once per group
-// and keeps them in registers (the ranks as FP16 pairs, exact for the at
-// most 1024 genes these groups have).  Per row: the row's codes gathered at the
+// and keeps them in registers (the ranks as FP16 pairs, exact up to 2048
+// genes, chunk length 16).  Per row: the row's codes gathered at the
 // lane's chunk elements, counted into the wave's 256 LDS bins (ballots for
 // code 0, LDS atomics for the rest), the counts turned into rank values by
 // a DPP scan (lane l holds bins 4l .. 4l + 3: one 16-byte read and write),
@@ -1043,6 +1043,7 @@ __global__ __launch_bounds__(256) void k_ftd_fused_r(
     const float* __restrict__ cell_sq, float* __restrict__ scores,
     int16_t* __restrict__ out_rank = nullptr, float* __restrict__ out_sq = nullptr)
 {
+    static_assert(STEP <= 16, "the slot mask holds 2 STEP bits; FP16 ranks are exact to 2048 genes");
     // two copies of the 256 bins per wave (lanes 0-31 and 32-63 count into
     // their own, halving the same-address conflicts of the atomics; the
     // second starts 4 banks on), summed into the first for the rank values

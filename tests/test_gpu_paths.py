@@ -287,11 +287,11 @@ def _jittered(ds, seed):
 
 @pytest.mark.parametrize("case", ["long_gene_sets", "wide_codes"])
 def test_device_finetune_kernel_variants(eng, case):
-    """The device fine-tune's kernel variants beyond the byte-code,
-    1024-gene form give the host-grouped schedule's labels and round counts:
-    fine-tune gene sets above 1024 genes (the LDS-table rank and fused
-    kernels beside the register-resident ones) and more than 256 value codes
-    (16-bit codes), in FP32 and exact form, fused form on and off."""
+    """The device fine-tune's kernel variants give the host-grouped
+    schedule's labels and round counts: fine-tune gene sets above 1024 genes
+    (the register-resident kernels' chunk lengths 9 and 10) and more than 256
+    value codes (16-bit codes: the LDS-table rank and fused kernels), in FP32
+    and exact form, fused form on and off."""
     from cusingler_amd import engine, synth
     if case == "long_gene_sets":
         # G_f = 1120 (the device fine-tune's LDS plan takes G_f up to about 1250)
