@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--train-cells", type=int, default=None,
                     help="training cells for the proposal network (default: the config's, "
                          "else 500,000)")
+    ap.add_argument("--finetune-fuse", type=int, default=None,
+                    help="largest fine-tune group (cells) of the fused FP32 form, 0 to 4 "
+                         "(default: the engine's, 4)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--hidden", type=int, default=256,
                     help="width of the proposal network's two hidden layers (multiples of 256 take the pipelined layer kernel)")
@@ -127,6 +130,8 @@ def main():
           f"cells (+{cfg.n_train} training cells) in {t_gen:.1f} s", file=sys.stderr, flush=True)
 
     eng = engine.Engine(0)
+    if args.finetune_fuse is not None:
+        eng.set_finetune_fuse(args.finetune_fuse)
     training = None
     if cached:
         net = np.load(args.net_cache, allow_pickle=False)
