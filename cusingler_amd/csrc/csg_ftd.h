@@ -1029,13 +1029,16 @@ __device__ inline void ftd_fused_row(const int* bins, const uint32_t (&code)[2][
 // the chunk sums accumulated from the codes the lane still holds, and the
 // trees completed by DPP (wave_tree_sum) -- no LDS traffic beyond the bins.
 //
+// The (4, 8] instance is held to 128 VGPRs (4 waves per SIMD instead of 3,
+// two values spilled outside the row loop).
+//
 // RANKS = true: the same rows of the groups that are not fused, whose ranks
 // are stored for the tiles (k_ftd_rank_ref_w's output): the rank values go
 // through an LDS row (chunk layout in, lane-strided out) so that the row is
 // stored coalesced, zero-padded to the group's stride, with the row's
 // chunk-tree sum of squares.
 template <int STEP, int STEP_LO, bool RANKS = false>
-__global__ __launch_bounds__(256) void k_ftd_fused_r(
+__global__ __launch_bounds__(256, STEP == 8 && !RANKS ? 4 : 1) void k_ftd_fused_r(
     const FtdCounts* __restrict__ cnt, const FtGroup* __restrict__ groups,
     const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ refgroup,
     const uint32_t* __restrict__ refrow, const uint32_t* __restrict__ genes,
