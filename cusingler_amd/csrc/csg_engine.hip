@@ -3006,6 +3006,9 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
         {
             // state at the start of this round back to the host
             *fallback = true;
+            lists.resize((size_t)Nq * L);
+            counts.resize(Nq);
+            nrounds.resize(Nq);
             HIPCHK(hipMemcpy(lists.data(), eng->d_fl_lists, (size_t)Nq * L, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(counts.data(), eng->d_fl_counts, (size_t)Nq * 4, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(nrounds.data(), eng->d_fl_rounds, (size_t)Nq * 4, hipMemcpyDeviceToHost));
@@ -3274,9 +3277,12 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         eng->counters[i] = 0;
     PhaseScope    total(eng, 5, 0);
     const int64_t t_start = now_us();
-    std::vector<uint8_t>  lists((size_t)Nq * L);
-    std::vector<uint32_t> counts(Nq, 0);
-    std::vector<int32_t>  nrounds(Nq, 0);
+    // candidate lists, counts and round counts on the host: filled by the
+    // device schedule only when it hands over to the host-grouped one (sized
+    // there), so that a device fine-tune allocates none of them
+    std::vector<uint8_t>  lists;
+    std::vector<uint32_t> counts;
+    std::vector<int32_t>  nrounds;
     if (eng->ft_schedule == 0 && L <= 256)
     {
         bool fallback = false;
@@ -3295,6 +3301,9 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         }
         return CSG_OK;
     }
+    lists.assign((size_t)Nq * L, 0);
+    counts.assign(Nq, 0);
+    nrounds.assign(Nq, 0);
     // candidate lists from the mask, ascending label order (src/cusingler.cu:1307-1312)
     if (!eng->host_mask.empty())
     {
