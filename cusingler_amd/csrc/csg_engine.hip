@@ -182,6 +182,10 @@ struct csg_engine
     std::vector<uintptr_t> ftd_sigA;
     hipGraphExec_t         ftd_gAs = nullptr;  // part A, a round of few cells
     std::vector<uintptr_t> ftd_sigAs;
+    // part A of the rounds after the first, led by the previous round's
+    // compaction ([0] many cells, [1] few)
+    hipGraphExec_t         ftd_gAc[2] = { nullptr, nullptr };
+    std::vector<uintptr_t> ftd_sigAc[2];
     bool                   ftd_small_a = false;  // the next part A is the few-cell one
     // part B graphs, one per launch mask (which work-queue kernels a round
     // needs, from its counts)
@@ -2156,6 +2160,13 @@ int ftd_free(csg_engine* eng)
         (void)hipGraphExecDestroy(eng->ftd_gAs);
     eng->ftd_gAs = nullptr;
     eng->ftd_sigAs.clear();
+    for (int i = 0; i < 2; ++i)
+    {
+        if (eng->ftd_gAc[i])
+            (void)hipGraphExecDestroy(eng->ftd_gAc[i]);
+        eng->ftd_gAc[i] = nullptr;
+        eng->ftd_sigAc[i].clear();
+    }
     if (eng->ftd_gA)
         (void)hipGraphExecDestroy(eng->ftd_gA);
     for (auto& b : eng->ftd_gB)
@@ -2254,6 +2265,13 @@ int ftd_groups(csg_engine* eng, uint32_t n)
         (void)hipGraphExecDestroy(eng->ftd_gAs);
     eng->ftd_gAs = nullptr;
     eng->ftd_sigAs.clear();
+    for (int i = 0; i < 2; ++i)
+    {
+        if (eng->ftd_gAc[i])
+            (void)hipGraphExecDestroy(eng->ftd_gAc[i]);
+        eng->ftd_gAc[i] = nullptr;
+        eng->ftd_sigAc[i].clear();
+    }
     if (eng->ftd_gA)
         (void)hipGraphExecDestroy(eng->ftd_gA);
     eng->ftd_gA = nullptr;
@@ -2361,12 +2379,17 @@ uint32_t ftd_mask_of(const csg_engine* eng, const FtdCounts& h)
 }
 
 // Round part A: grouping, group set-up and the prefix sums.
-int ftd_launch_a(csg_engine* eng, bool small)
+int ftd_launch_a(csg_engine* eng, bool small, int compact_times)
 {
     const uint32_t L = eng->fL, T = eng->ftd_T;
     const unsigned g = (unsigned)eng->ftd_grid_threads;
     const FtdLds   d = ftd_lds(eng);
     FtdCounts*     cnt = eng->d_fl_cnt;
+    if (compact_times >= 0)
+        // the previous round's compaction into this round's active list
+        k_ftd_compact<<<g, FTD_TPB, 0, eng->stream>>>(cnt, eng->d_fl_act, eng->d_fl_counts,
+                                                      eng->d_fl_rounds, compact_times,
+                                                      eng->d_fl_next);
     if (small)
         // at most FTD_SMALL_NT active cells (this round's can only be fewer
         // than the last round's): the grouping in one block
@@ -2622,14 +2645,6 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
 }
 
 // Round end: the round is counted and the next round's active list formed.
-int ftd_launch_compact(csg_engine* eng, int times)
-{
-    k_ftd_compact<<<(unsigned)eng->ftd_grid_threads, FTD_TPB, 0, eng->stream>>>(
-        eng->d_fl_cnt, eng->d_fl_act, eng->d_fl_counts, eng->d_fl_rounds, times, eng->d_fl_next);
-    LAUNCHCHK();
-    return CSG_OK;
-}
-
 // Device bytes of one round's work tables for the given totals.
 uint64_t ftd_bytes(const FtdCounts& h, uint32_t L)
 {
@@ -2988,9 +3003,16 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
     {
         const int64_t t0 = now_us();
         const bool      small = eng->ftd_small_a;
-        hipGraphExec_t* gA    = small ? &eng->ftd_gAs : &eng->ftd_gA;
-        if ((rc = ftd_graph(eng, gA, small ? &eng->ftd_sigAs : &eng->ftd_sigA, ftd_sig_a(eng),
-                            [&]() { return ftd_launch_a(eng, small); })))
+        // every round after the first starts with the previous round's
+        // compaction, in the same graph
+        const bool              comp = round > 0;
+        hipGraphExec_t*         gA   = comp ? &eng->ftd_gAc[small] : small ? &eng->ftd_gAs : &eng->ftd_gA;
+        std::vector<uintptr_t>* sA   = comp ? &eng->ftd_sigAc[small] : small ? &eng->ftd_sigAs : &eng->ftd_sigA;
+        std::vector<uintptr_t>  sig  = ftd_sig_a(eng);
+        if (comp)
+            sig.push_back((uintptr_t)(uint32_t)times);
+        if ((rc = ftd_graph(eng, gA, sA, sig,
+                            [&]() { return ftd_launch_a(eng, small, comp ? times : -1); })))
             return rc;
         HIPCHK(hipGraphLaunch(*gA, eng->stream));
         HIPCHK(hipMemcpyAsync(&h, eng->d_fl_cnt, sizeof(FtdCounts), hipMemcpyDeviceToHost,
@@ -3035,8 +3057,8 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
         }
         else if ((rc = ftd_batches(eng, h, quantile, thre, times)))
             return rc;
-        if ((rc = ftd_launch_compact(eng, times)))
-            return rc;
+        // (the compaction into the next round's active list leads that
+        // round's part A)
         // part B leaves the next round's count in n_next (read with part A)
         h.n_next = 1;
         eng->counters[15] += now_us() - t0;
