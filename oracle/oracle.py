@@ -156,6 +156,16 @@ class Oracle:
                           _p(rounds))
         return final, rounds
 
+    def finetune_round(self, q: int, top, quantile=0.8, thre=0.05, variant: int = VAR_SEQ):
+        """One fine-tune round of query cell `q` on the candidate list `top`:
+        returns (new candidate list, per-label scores of the round)."""
+        t = np.ascontiguousarray(top, np.uint32)
+        out = np.zeros(max(len(t), 1), np.uint32)
+        sc = np.full(max(len(t), 1), np.nan, np.float32)
+        n = lib().or_finetune_round(self._h, q, _p(t), len(t), quantile, thre, variant, _p(out),
+                                    _p(sc))
+        return [int(v) for v in out[:n]], sc[:len(t)]
+
 
 def percentile(arr: np.ndarray, quantile: float) -> float:
     a = np.ascontiguousarray(arr, np.float32)
