@@ -31,7 +31,7 @@ CXX = os.environ.get("CXX", "g++")
 
 HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-ffp-contract=fast", "-Wno-unused-result"]
+             "-ffp-contract=off", "-Wno-unused-result"]
 CXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-sign-compare",
              "-Wno-unused-function", "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__"]
 

@@ -83,6 +83,7 @@ struct csg_engine
     double*    d_irr   = nullptr;  // inverse roots of the exact sums of X^2 (reference rows)
     bool      ref_ranked = false;
     uint32_t  Qc = 0;
+    uint32_t  qc_cap = 0;  // caller's cap on the query chunk (rows), 0: none
     int16_t*  d_Xq = nullptr;
     float*    d_sqq_f = nullptr;
     double*    d_irq   = nullptr;  // the same for the query chunk
@@ -773,6 +774,8 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
     size_t qc = budget / (size_t)ref_height;
     qc                  = std::max<size_t>(128, qc / 128 * 128);
     qc                  = std::min<size_t>(qc, round_up(qry_height, 128));
+    if (eng->qc_cap)
+        qc = std::min<size_t>(qc, eng->qc_cap);
     eng->Qc             = (uint32_t)qc;
     if ((rc = dalloc(eng, &eng->d_Xq, qc * eng->Kp)) || (rc = dalloc(eng, &eng->d_sqq_f, qc)) ||
         (rc = dalloc(eng, &eng->d_irq, qc)) || (rc = dalloc(eng, &eng->d_S, qc * ref_height)) ||
@@ -3452,6 +3455,18 @@ int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches)
         eng->ftd_budget = bytes;
     if (batches)
         *batches = eng->ftd_batches;
+    return CSG_OK;
+}
+
+int csg_set_query_chunk(csg_engine* eng, uint32_t rows, uint32_t* chunk)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (rows % 128)
+        return eng->fail(CSG_ERR_ARGUMENT, "the query chunk cap must be a multiple of 128 rows");
+    eng->qc_cap = rows;
+    if (chunk)
+        *chunk = eng->Qc;
     return CSG_OK;
 }
 

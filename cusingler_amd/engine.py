@@ -78,6 +78,7 @@ def lib():
     L.csg_finetune_tuning.argtypes = [_V, C.c_int, _V, _V]
     L.csg_set_finetune_budget.argtypes = [_V, _U64, _V]
     L.csg_set_finetune_fuse.argtypes = [_V, C.c_int]
+    L.csg_set_query_chunk.argtypes = [_V, _U32, _V]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_set_phase_timing.argtypes = [_V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
@@ -271,6 +272,13 @@ class Engine:
         b = C.c_int64()
         self._check(lib().csg_set_finetune_budget(self._h, nbytes, C.byref(b)), "set_finetune_budget")
         return b.value
+
+    def set_query_chunk(self, rows: int = 0) -> int:
+        """Caps the score step's query chunk (a multiple of 128 rows, 0 = no
+        cap) from the next copyin_score; returns the current chunk."""
+        c = C.c_uint32()
+        self._check(lib().csg_set_query_chunk(self._h, int(rows), C.byref(c)), "set_query_chunk")
+        return c.value
 
     def set_finetune_fuse(self, max_cells: int):
         """Largest fine-tune group (cells) of the fused FP32 form (0 = off)."""

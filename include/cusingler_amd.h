@@ -145,6 +145,14 @@ int csg_finetune_tuning(csg_engine* eng, int autotune, int32_t* grids, float* be
  * reference. */
 int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches);
 
+/* Caps the query chunk of the score step (src/cusingler.cu:335-409 slices the
+ * query to the free device memory; here a chunk is at most 8 GiB of scores
+ * and 1/16 of the free memory).  rows: a multiple of 128, 0 removes the cap;
+ * it applies from the next csg_copyin_score.  *chunk (optional) receives the
+ * chunk of the current inputs.  Lets a caller bound the score buffer, and
+ * tests run several chunks on small inputs. */
+int csg_set_query_chunk(csg_engine* eng, uint32_t rows, uint32_t* chunk);
+
 /* destroy (src/cusingler.cuh:25): frees the fine-tune buffers. */
 int csg_destroy(csg_engine* eng);
 

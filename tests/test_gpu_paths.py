@@ -79,6 +79,13 @@ def test_exact_path_labels_match_reference_order(eng, name):
     err = np.abs(scores.astype(np.float64) - osc.astype(np.float64))
     err = err[~np.isnan(err)] if np.isnan(err).any() else err
     assert err.size == 0 or np.max(err) <= w, f"score error {np.nanmax(err)} above the certified width {w}"
+    ok = ~np.isnan(osc)
+    rel = np.abs(scores[ok].astype(np.float64) - osc[ok]) / np.abs(osc[ok].astype(np.float64))
+    big = np.abs(osc[ok]) >= 0.1
+    print(f"[exact-rel] {name}: max rel {rel.max() if rel.size else 0:.3g}, over |P|>=0.1 "
+          f"{rel[big].max() if big.any() else 0:.3g}, pairs > 1e-5 rel {(rel > 1e-5).sum()} of "
+          f"{rel.size} (|P| of those: max {np.abs(osc[ok])[rel > 1e-5].max() if (rel > 1e-5).any() else 0:.3g})",
+          flush=True)
     print(f"[exact] {name}: rechecked pairs {cnt['rechecked_pairs']}, uncertified cells "
           f"{cnt['uncertified_cells']} of {ds.qry_height}, max |dP| {np.max(err) if err.size else 0:.3g} "
           f"(width {w:.3g})", flush=True)
