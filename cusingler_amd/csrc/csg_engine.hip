@@ -548,7 +548,7 @@ int csg_init_gpu(int gpuid, csg_engine** out)
                                               want) == hipSuccess;
         bool       ok3  = true;
         for (const void* f : { (const void*)k_ftd_rank_ref, (const void*)k_ftd_rank_qry,
-                               (const void*)k_ftd_corr })
+                               (const void*)k_ftd_corr, (const void*)k_ft_rank_ref })
             ok3 = ok3 && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              want) == hipSuccess;
         if (ok1 && ok2 && ok3)
@@ -1817,10 +1817,11 @@ int redo_ft_cells(csg_engine* eng, const FtBatch& b, const std::vector<uint32_t>
         max_st = std::max(max_st, g.stride);
     {
         PhaseScope ps(eng, 6, 1);
-        k_ft_corr_thin<FT_THIN><<<(unsigned)tiles.size(), 256, (size_t)FT_THIN * max_st * 2,
+        const int  ktt = std::min<int>((int)max_st, FT_THIN_KT);
+        k_ft_corr_thin<FT_THIN><<<(unsigned)tiles.size(), 256, (size_t)FT_THIN * ktt * 2,
                                   eng->stream>>>((const FtCorrTile*)(d + o_tiles), d_groups,
                                                  eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
-                                                 eng->d_ft_rsq, eng->d_ft_scores);
+                                                 eng->d_ft_rsq, eng->d_ft_scores, ktt);
         LAUNCHCHK();
     }
     FtSegments src{ d_cells, n, d_groups, d_labels, d_lpos, eng->d_fllen };
@@ -1964,17 +1965,21 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
 
     const size_t stride_max = round_up(max_gs, 8);
     const size_t lds_rank_q = (size_t)((eng->fnbins + 3) & ~3) * 4 +
-                              round_up(std::max<uint32_t>(eng->Gf, 1), 8) * 2 + stride_max * 4;
-    const size_t lds_corr   = (size_t)(FT_TC + FT_TR) * (stride_max + 2) * 2;
-    const size_t lds_thin   = (size_t)FT_THIN * stride_max * 2;
-    if (lds_rank_q > eng->lds_limit || lds_corr > eng->lds_limit || lds_thin > 65536 ||
+                              round_up(std::max<uint32_t>(eng->Gf, 1), 8) * 2 + stride_max * 2;
+    const size_t lds_rank_r = (size_t)((eng->fnbins + 3) & ~3) * 4 + stride_max * 4;
+    // the correlation tiles walk K in LDS stages, so any gene subset fits
+    const int    kt_corr    = ft_corr_stage((int)stride_max, eng->lds_limit, FT_TC + FT_TR);
+    const int    kt_thin    = std::min<int>((int)stride_max, FT_THIN_KT);
+    const size_t lds_corr   = (size_t)(FT_TC + FT_TR) * (kt_corr / 2 + 1) * 4;
+    const size_t lds_thin   = (size_t)FT_THIN * kt_thin * 2;
+    if (lds_rank_q > eng->lds_limit || lds_rank_r > eng->lds_limit ||
         (b.tiles.empty() && b.thin_tiles.empty()) || b.n_ref_items == 0 || b.cell_items.empty())
         return eng->fail(CSG_ERR_ARGUMENT, "fine-tune gene subset of " + std::to_string(max_gs) +
-                                               " genes exceeds the kernels' LDS plan");
+                                               " genes with " + std::to_string(eng->fnbins) +
+                                               " value codes exceeds the rank kernels' LDS");
     {
         PhaseScope   ps(eng, 7, 2);
-        const size_t smem_r = (size_t)((eng->fnbins + 3) & ~3) * 4 + stride_max * 4;
-        k_ft_rank_ref<<<(unsigned)b.n_ref_items, 128, smem_r, eng->stream>>>(
+        k_ft_rank_ref<<<(unsigned)b.n_ref_items, 128, lds_rank_r, eng->stream>>>(
             d_groups, (uint32_t)b.groups.size(), d_labels, eng->d_flstart, eng->d_fllen, d_genes,
             eng->d_fref_codes, std::max<uint32_t>(eng->Gf, 1), eng->fnbins, eng->d_ft_rr,
             eng->d_ft_rsq);
@@ -2000,14 +2005,14 @@ int run_ft_batch(csg_engine* eng, FtBatch& b, float quantile, float thre,
             k_ft_corr<FT_TC, FT_TR, FT_RC, FT_RR><<<(unsigned)b.tiles.size(), FT_NT, lds_corr,
                                                     eng->stream>>>(
                 d_tiles, d_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr, eng->d_ft_rsq,
-                eng->d_ft_scores);
+                eng->d_ft_scores, kt_corr);
             LAUNCHCHK();
         }
         if (!b.thin_tiles.empty())
         {
             k_ft_corr_thin<FT_THIN><<<(unsigned)b.thin_tiles.size(), 256, lds_thin, eng->stream>>>(
                 d_thin, d_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr, eng->d_ft_rsq,
-                eng->d_ft_scores);
+                eng->d_ft_scores, kt_thin);
             LAUNCHCHK();
         }
     }
@@ -2339,6 +2344,7 @@ int ftd_work(csg_engine* eng, const FtdCounts& h)
 struct FtdLds
 {
     size_t setup, rank_ref, rank_qry, corr, thin;
+    int    kt_corr, kt_thin;  // LDS stage widths (genes) of the FP32 correlation tiles
     int    wave_ints;  // per-wave LDS of the wave-per-row reference rank kernel
     size_t rank_ref_w;
 };
@@ -2350,9 +2356,12 @@ FtdLds ftd_lds(const csg_engine* eng)
     FtdLds       d;
     d.setup    = ((size_t)std::max<uint32_t>(eng->Gf, 1) + 31) / 32 * 4;
     d.rank_ref = nb + st * 4;
-    d.rank_qry = nb + st * 2 + st * 2 + st * 2;
-    d.corr     = (size_t)(FT_TC + FT_TR) * (st + 2) * 2;
-    d.thin     = (size_t)FT_THIN * st * 2;
+    d.rank_qry = nb + st * 2 + st * 2;
+    // the correlation tiles walk K in LDS stages, so any gene subset fits
+    d.kt_corr  = ft_corr_stage((int)st, eng->lds_limit, FT_TC + FT_TR);
+    d.corr     = (size_t)(FT_TC + FT_TR) * (d.kt_corr / 2 + 1) * 4;
+    d.kt_thin  = std::min<int>((int)st, FT_THIN_KT);
+    d.thin     = (size_t)FT_THIN * d.kt_thin * 2;
     d.wave_ints  = (int)(((eng->fnbins + 3) & ~3) + st / 2);
     d.rank_ref_w = (size_t)4 * d.wave_ints * 4;
     return d;
@@ -2531,11 +2540,11 @@ int ftd_launch_corr(csg_engine* eng, int grid)
     if (m & FTDM_CORR)
         k_ftd_corr<<<(unsigned)grid, FT_NT, d.corr, eng->stream>>>(
             cnt, eng->d_fl_tiles, eng->d_fl_groups, eng->d_fl_gflag, eng->d_ft_cr, eng->d_ft_csq,
-            eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores);
+            eng->d_ft_rr, eng->d_ft_rsq, eng->d_ft_scores, d.kt_corr);
     if (m & FTDM_THIN)
         k_ftd_thin<<<(unsigned)grid, 256, d.thin, eng->stream>>>(
             cnt, 0, eng->d_fl_thin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
-            eng->d_ft_rsq, eng->d_ft_scores);
+            eng->d_ft_rsq, eng->d_ft_scores, d.kt_thin);
     LAUNCHCHK();
     return CSG_OK;
 }
@@ -2630,7 +2639,7 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
     // FP32 redo of the undecided exact-form cells
     k_ftd_thin<<<(unsigned)eng->ftd_grid_corr, 256, d.thin, eng->stream>>>(
         cnt, 1, eng->d_fl_rthin, eng->d_fl_groups, eng->d_ft_cr, eng->d_ft_csq, eng->d_ft_rr,
-        eng->d_ft_rsq, eng->d_ft_scores);
+        eng->d_ft_rsq, eng->d_ft_scores, d.kt_thin);
     k_ftd_select<FtdListSegments><<<(unsigned)eng->ftd_grid_select, 256, 0, eng->stream>>>(
         cnt, 1, rseg, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_fl_pred, eng->d_fl_nan2);
     k_ftd_select_nan<FtdListSegments><<<g, FTD_TPB, 0, eng->stream>>>(
@@ -2969,9 +2978,10 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
     const uint32_t Nq = eng->fNq, L = eng->fL;
     int            rc;
     const FtdLds   d = ftd_lds(eng);
-    if (d.rank_qry > eng->lds_limit || d.corr > eng->lds_limit || d.thin > 65536 ||
-        d.setup > 65536 || d.rank_ref > eng->lds_limit)
-        return eng->fail(CSG_ERR_ARGUMENT, "fine-tune gene set exceeds the kernels' LDS plan");
+    if (d.rank_qry > eng->lds_limit || d.setup > 65536 || d.rank_ref > eng->lds_limit)
+        return eng->fail(CSG_ERR_ARGUMENT, "fine-tune gene set of " + std::to_string(eng->Gf) +
+                                               " genes with " + std::to_string(eng->fnbins) +
+                                               " value codes exceeds the rank kernels' LDS");
     if ((rc = ftd_static(eng)))
         return rc;
     const float* mask = nullptr;
@@ -3656,7 +3666,7 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
 
 int csg_selftest_ft(csg_engine* eng, int gs, int64_t* mismatches)
 {
-    if (!eng || !mismatches || gs < 1 || gs > 4096)
+    if (!eng || !mismatches || gs < 1 || gs > 16384)
         return CSG_ERR_ARGUMENT;
     HIPCHK(hipSetDevice(eng->device));
     const int      ncell = 40, nref = 70;
@@ -3701,42 +3711,45 @@ int csg_selftest_ft(csg_engine* eng, int gs, int64_t* mismatches)
         cleanup();
         return rc;
     }
-    const size_t lds = (size_t)(FT_TC + FT_TR) * (st + 2) * 2;
-    if (lds > eng->lds_limit)
-    {
-        cleanup();
-        return eng->fail(CSG_ERR_ARGUMENT, "gene subset too large for the LDS plan");
-    }
-    k_ft_corr<FT_TC, FT_TR, FT_RC, FT_RR><<<(unsigned)tiles.size(), FT_NT, lds, eng->stream>>>(
-        dt, dg, dc, d1, dr, d1, ds);
-    hipError_t         le = hipGetLastError();
-    std::vector<float> out((size_t)ncell * nref), out_thin((size_t)ncell * nref);
-    hipError_t ce = hipMemcpyAsync(out.data(), ds, out.size() * 4, hipMemcpyDeviceToHost, eng->stream);
-    hipError_t se = hipStreamSynchronize(eng->stream);
-    // the thin-tile kernel on the same pairs
+    // both tile kernels, each with its widest LDS stage and with 64-gene
+    // stages (several stages per row beyond 64 genes)
     std::vector<FtCorrTile> thin;
     for (uint32_t c0 = 0; c0 < (uint32_t)ncell; c0 += FT_THIN)
         for (uint32_t r0 = 0; r0 < (uint32_t)nref; r0 += 256)
             thin.push_back(FtCorrTile{ 0, c0, r0, 0 });
     FtCorrTile* dth = nullptr;
-    hipError_t  te  = hipSuccess;
-    if (le == hipSuccess && se == hipSuccess && upload(eng, &dth, thin.data(), thin.size()) == CSG_OK)
+    if ((rc = upload(eng, &dth, thin.data(), thin.size())))
     {
-        k_ft_corr_thin<FT_THIN><<<(unsigned)thin.size(), 256, (size_t)FT_THIN * st * 2, eng->stream>>>(
-            dth, dg, dc, d1, dr, d1, ds);
-        te = hipGetLastError();
-        if (te == hipSuccess)
-            te = hipMemcpyAsync(out_thin.data(), ds, out_thin.size() * 4, hipMemcpyDeviceToHost,
-                                eng->stream);
-        if (te == hipSuccess)
-            te = hipStreamSynchronize(eng->stream);
+        cleanup();
+        return rc;
     }
-    else
-        te = hipErrorUnknown;
+    std::vector<std::vector<float>> outs;
+    hipError_t                      err = hipSuccess;
+    for (int v = 0; v < 4 && err == hipSuccess; ++v)
+    {
+        const bool wide = v < 2;
+        const int  kt   = wide ? ft_corr_stage((int)st, eng->lds_limit, FT_TC + FT_TR) : 64;
+        const int  ktt  = wide ? std::min<int>((int)st, FT_THIN_KT) : 64;
+        if (v % 2 == 0)
+            k_ft_corr<FT_TC, FT_TR, FT_RC, FT_RR><<<(unsigned)tiles.size(), FT_NT,
+                                                    (size_t)(FT_TC + FT_TR) * (kt / 2 + 1) * 4,
+                                                    eng->stream>>>(dt, dg, dc, d1, dr, d1, ds, kt);
+        else
+            k_ft_corr_thin<FT_THIN><<<(unsigned)thin.size(), 256, (size_t)FT_THIN * ktt * 2,
+                                      eng->stream>>>(dth, dg, dc, d1, dr, d1, ds, ktt);
+        err = hipGetLastError();
+        outs.emplace_back((size_t)ncell * nref);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(outs.back().data(), ds, outs.back().size() * 4,
+                                 hipMemcpyDeviceToHost, eng->stream);
+        if (err == hipSuccess)
+            err = hipStreamSynchronize(eng->stream);
+    }
     dfree(dth);
     cleanup();
-    if (le != hipSuccess || ce != hipSuccess || se != hipSuccess || te != hipSuccess)
-        return eng->fail(CSG_ERR_DEVICE, "ft selftest kernel failed");
+    if (err != hipSuccess)
+        return eng->fail(CSG_ERR_DEVICE, std::string("ft selftest kernel failed: ") +
+                                             hipGetErrorString(err));
     // host: 128 chunks of ceil(gs/128) genes summed by fma, stride 1..64 tree
     int64_t   bad  = 0;
     const int step = (gs + 127) / 128;
@@ -3754,10 +3767,9 @@ int csg_selftest_ft(csg_engine* eng, int gs, int64_t* mismatches)
             for (int stride = 1; stride < 128; stride *= 2)
                 for (int t = 0; t < 128; t += 2 * stride)
                     part[t] = part[t] + part[t + stride];
-            if (!(out[(size_t)c * nref + r] == part[0]))
-                ++bad;
-            if (!(out_thin[(size_t)c * nref + r] == part[0]))
-                ++bad;
+            for (auto& o : outs)
+                if (!(o[(size_t)c * nref + r] == part[0]))
+                    ++bad;
         }
     *mismatches = bad;
     return CSG_OK;

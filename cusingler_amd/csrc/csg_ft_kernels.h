@@ -48,6 +48,9 @@ template <class F> __device__ inline float chunk_tree_sum128(int n, F f, float* 
 constexpr int FT_TC = 32, FT_TR = 32, FT_RC = 2, FT_RR = 2;
 constexpr int FT_NT   = (FT_TC / FT_RC) * (FT_TR / FT_RR);
 constexpr int FT_THIN = 8;
+// LDS stage width (genes) of the thin tiles: FT_THIN cell rows of int16 in
+// the 64 KiB a block gets by default
+constexpr int FT_THIN_KT = (65536 / (FT_THIN * 2)) & ~7;
 
 // A row to rank over a group's gene subset.
 struct FtRankItem
@@ -185,10 +188,13 @@ __device__ inline void ft_rank_qry_item(const FtRankItem& it, uint32_t idx,
                                         float* __restrict__ out_sq, int* smem, float* part, int* tmp)
 {
     const FtGroup g     = groups[it.group];
+    // LDS: code bins | the dense row over the fine-tune genes | the codes at
+    // G_sub; the rank row reuses the dense row's space (dead after the
+    // gather, and G_sub is a subset of the fine-tune genes)
     int*          bins  = smem;
     uint16_t*     dense = (uint16_t*)(smem + ((nbins + 3) & ~3));
     uint16_t*     codes = dense + ((Gf + 7) & ~7u);
-    int16_t*      row   = (int16_t*)(codes + ((g.gs + 7) & ~7u));
+    int16_t*      row   = (int16_t*)dense;
     for (uint32_t i = threadIdx.x; i < Gf; i += 128)
         dense[i] = 0;
     __syncthreads();
@@ -239,52 +245,88 @@ struct FtCorrTile
     uint32_t lim;    // at most this many cells from cell0 (0: no limit)
 };
 
+// Merges chunk sum `v` of chunk `ch` into the left-to-right tree: while the
+// low bits of ch are ones the node closes a pair with its left sibling; the
+// last chunk (127) closes the whole tree into `full`.
+__device__ inline void ft_tree_push(float (&stk)[7], float& full, float v, int ch)
+{
+    int  cc   = ch;
+    bool open = true;
+#pragma unroll
+    for (int l = 0; l < 7; ++l)
+    {
+        if (open)
+        {
+            if (cc & 1)
+            {
+                v = stk[l] + v;
+                cc >>= 1;
+            }
+            else
+            {
+                stk[l] = v;
+                open   = false;
+            }
+        }
+    }
+    if (open)
+        full = v;
+}
+
+// Root of the tree after the nz chunks that hold genes: the remaining
+// chunks are zero and adding a zero subtree leaves a sum unchanged, so the
+// root is the pending left subtrees (set bits of nz) folded from the lowest
+// level up.
+__device__ inline float ft_tree_root(const float (&stk)[7], float full, int nz)
+{
+    float v    = 0.f;
+    bool  have = false;
+#pragma unroll
+    for (int l = 0; l < 7; ++l)
+        if ((nz >> l) & 1)
+        {
+            v    = have ? stk[l] + v : stk[l];
+            have = true;
+        }
+    return nz == 128 ? full : v;
+}
+
+// LDS stage width (genes, a multiple of 8) of the FP32 correlation tiles for
+// rows of stride st when a block may use `lds` bytes: the whole row when it
+// fits, else the widest stage that does (the tile then walks K in stages).
+__host__ __device__ inline int ft_corr_stage(int st, size_t lds, int rows)
+{
+    const int words = (int)(lds / (4 * (size_t)rows)) - 1;  // pitch = kt/2 + 1 words
+    const int kt    = (words * 2) & ~7;
+    return st <= kt ? st : kt;
+}
+
 template <int TC, int TR, int RC, int RR>
 __device__ inline void ft_corr_tile(const FtCorrTile& tl, const FtGroup* __restrict__ groups,
                                     const int16_t* __restrict__ cell_rank,
                                     const float* __restrict__ cell_sq,
                                     const int16_t* __restrict__ ref_rank,
                                     const float* __restrict__ ref_sq, float* __restrict__ scores,
-                                    int* srow32)
+                                    int* srow32, int kt_max)
 {
     constexpr int    NT    = (TC / RC) * (TR / RR);
     const FtGroup    g     = groups[tl.group];
     const int        gs    = (int)g.gs;
     const int        st    = (int)g.stride;  // multiple of 8
-    // LDS row pitch st + 2 int16 = an odd number of 32-bit words, so lanes
-    // reading one gene of different rows hit different banks
-    const int        pw    = st / 2 + 1;     // pitch in words
+    // genes are staged in LDS kt at a time (all of them when the row fits);
+    // the LDS row pitch kt + 2 int16 is an odd number of 32-bit words, so
+    // lanes reading one gene of different rows hit different banks
+    const int        kt    = st < kt_max ? st : kt_max;  // multiple of 8
+    const int        pw    = kt / 2 + 1;     // pitch in words
     const int        ncell = min(TC, (int)g.ncell - (int)tl.cell0);
     const int        nref  = min(TR, (int)g.nref - (int)tl.ref0);
     int*             cs32  = srow32;             // TC rows
     int*             rs32  = srow32 + TC * pw;   // TR rows
-    const int        w4    = st / 8;             // int4 per row
     const int16_t*   cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
     const int16_t*   rbase = ref_rank + g.ref_dst0 + (uint64_t)tl.ref0 * st;
-    for (int v = threadIdx.x; v < ncell * w4; v += NT)
-    {
-        const int  row = v / w4, c = v % w4;
-        const int4 val = *(const int4*)(cbase + (uint64_t)row * st + c * 8);
-        int*       d   = cs32 + row * pw + c * 4;
-        d[0] = val.x;
-        d[1] = val.y;
-        d[2] = val.z;
-        d[3] = val.w;
-    }
-    for (int v = threadIdx.x; v < nref * w4; v += NT)
-    {
-        const int  row = v / w4, c = v % w4;
-        const int4 val = *(const int4*)(rbase + (uint64_t)row * st + c * 8);
-        int*       d   = rs32 + row * pw + c * 4;
-        d[0] = val.x;
-        d[1] = val.y;
-        d[2] = val.z;
-        d[3] = val.w;
-    }
-    __syncthreads();
-    const int16_t* cs = (const int16_t*)cs32;
-    const int16_t* rs = (const int16_t*)rs32;
-    const int      p16 = 2 * pw;  // pitch in int16
+    const int16_t*   cs    = (const int16_t*)cs32;
+    const int16_t*   rs    = (const int16_t*)rs32;
+    const int        p16   = 2 * pw;  // pitch in int16
     // thread -> RC cells x RR refs; the RR refs of a thread are TR/RR apart
     const int tc = threadIdx.x / (TR / RR);
     const int tr = threadIdx.x % (TR / RR);
@@ -299,92 +341,81 @@ __device__ inline void ft_corr_tile(const FtCorrTile& tl, const FtGroup* __restr
     const int nz   = (gs + step - 1) / step;  // chunks holding genes; later ones are 0
     float     stk[RC][RR][7];
     float     full[RC][RR];
+    float     acc[RC][RR];
 #pragma unroll
     for (int a = 0; a < RC; ++a)
 #pragma unroll
         for (int b = 0; b < RR; ++b)
         {
             full[a][b] = 0.f;
+            acc[a][b]  = 0.f;
 #pragma unroll
             for (int l = 0; l < 7; ++l)
                 stk[a][b][l] = 0.f;
         }
-    for (int ch = 0; ch < nz; ++ch)
+    int ch     = 0;                  // chunk being summed
+    int ch_end = step < gs ? step : gs;
+    for (int k0 = 0; k0 < gs; k0 += kt)
     {
-        const int lo = ch * step;
-        const int hi = min(lo + step, gs);
-        float     acc[RC][RR];
-#pragma unroll
-        for (int a = 0; a < RC; ++a)
-#pragma unroll
-            for (int b = 0; b < RR; ++b)
-                acc[a][b] = 0.f;
-        for (int i = lo; i < hi; ++i)
+        const int w4 = min(kt, st - k0) / 8;  // int4 per staged row
+        if (k0)
+            __syncthreads();  // the previous stage is no longer read
+        for (int v = threadIdx.x; v < ncell * w4; v += NT)
         {
-            float x[RC], y[RR];
+            const int  row = v / w4, c = v % w4;
+            const int4 val = *(const int4*)(cbase + (uint64_t)row * st + k0 + c * 8);
+            int*       d   = cs32 + row * pw + c * 4;
+            d[0] = val.x;
+            d[1] = val.y;
+            d[2] = val.z;
+            d[3] = val.w;
+        }
+        for (int v = threadIdx.x; v < nref * w4; v += NT)
+        {
+            const int  row = v / w4, c = v % w4;
+            const int4 val = *(const int4*)(rbase + (uint64_t)row * st + k0 + c * 8);
+            int*       d   = rs32 + row * pw + c * 4;
+            d[0] = val.x;
+            d[1] = val.y;
+            d[2] = val.z;
+            d[3] = val.w;
+        }
+        __syncthreads();
+        const int k1 = min(k0 + kt, gs);
+        int       i  = k0;
+        while (i < k1)
+        {
+            const int e = min(ch_end, k1);
+            for (; i < e; ++i)
+            {
+                float x[RC], y[RR];
 #pragma unroll
-            for (int a = 0; a < RC; ++a)
-                x[a] = (float)cs[ci[a] * p16 + i];
-#pragma unroll
-            for (int b = 0; b < RR; ++b)
-                y[b] = (float)rs[ri[b] * p16 + i];
-#pragma unroll
-            for (int a = 0; a < RC; ++a)
+                for (int a = 0; a < RC; ++a)
+                    x[a] = (float)cs[ci[a] * p16 + i - k0];
 #pragma unroll
                 for (int b = 0; b < RR; ++b)
-                    acc[a][b] = __builtin_fmaf(x[a], y[b], acc[a][b]);
-        }
-        // merge the chunk into the left-to-right tree: while the low bits of
-        // ch are ones the node closes a pair with its left sibling
+                    y[b] = (float)rs[ri[b] * p16 + i - k0];
 #pragma unroll
-        for (int a = 0; a < RC; ++a)
+                for (int a = 0; a < RC; ++a)
 #pragma unroll
-            for (int b = 0; b < RR; ++b)
-            {
-                float v    = acc[a][b];
-                int   cc   = ch;
-                bool  open = true;
-#pragma unroll
-                for (int l = 0; l < 7; ++l)
-                {
-                    if (open)
-                    {
-                        if (cc & 1)
-                        {
-                            v = stk[a][b][l] + v;
-                            cc >>= 1;
-                        }
-                        else
-                        {
-                            stk[a][b][l] = v;
-                            open         = false;
-                        }
-                    }
-                }
-                if (open)
-                    full[a][b] = v;  // ch == 127 closes the whole tree
+                    for (int b = 0; b < RR; ++b)
+                        acc[a][b] = __builtin_fmaf(x[a], y[b], acc[a][b]);
             }
-    }
-    // The remaining chunks are zero; adding a zero subtree leaves a sum
-    // unchanged, so the root is the pending left subtrees (set bits of nz)
-    // folded from the lowest level up.
-    float root[RC][RR];
+            if (i == ch_end)
+            {
 #pragma unroll
-    for (int a = 0; a < RC; ++a)
+                for (int a = 0; a < RC; ++a)
 #pragma unroll
-        for (int b = 0; b < RR; ++b)
-        {
-            float v    = 0.f;
-            bool  have = false;
-#pragma unroll
-            for (int l = 0; l < 7; ++l)
-                if ((nz >> l) & 1)
-                {
-                    v    = have ? stk[a][b][l] + v : stk[a][b][l];
-                    have = true;
-                }
-            root[a][b] = nz == 128 ? full[a][b] : v;
+                    for (int b = 0; b < RR; ++b)
+                    {
+                        ft_tree_push(stk[a][b], full[a][b], acc[a][b], ch);
+                        acc[a][b] = 0.f;
+                    }
+                ++ch;
+                ch_end = min(ch_end + step, gs);
+            }
         }
+    }
 #pragma unroll
     for (int a = 0; a < RC; ++a)
     {
@@ -398,7 +429,8 @@ __device__ inline void ft_corr_tile(const FtCorrTile& tl, const FtGroup* __restr
         {
             const int r = tr + b * (TR / RR);
             if (r < nref)
-                out[r] = corr_from_sums(root[a][b], sq, ref_sq[g.ref_item0 + tl.ref0 + r]);
+                out[r] = corr_from_sums(ft_tree_root(stk[a][b], full[a][b], nz), sq,
+                                        ref_sq[g.ref_item0 + tl.ref0 + r]);
         }
     }
 }
@@ -408,11 +440,11 @@ __global__ __launch_bounds__((TC / RC) * (TR / RR)) void k_ft_corr(
     const FtCorrTile* __restrict__ tiles, const FtGroup* __restrict__ groups,
     const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
     const int16_t* __restrict__ ref_rank, const float* __restrict__ ref_sq,
-    float* __restrict__ scores)
+    float* __restrict__ scores, int kt_max)
 {
     extern __shared__ int srow32[];
     ft_corr_tile<TC, TR, RC, RR>(tiles[blockIdx.x], groups, cell_rank, cell_sq, ref_rank, ref_sq,
-                                 scores, srow32);
+                                 scores, srow32, kt_max);
 }
 
 // Thin tile for groups with few cells: up to TC cells (rows in LDS) x 256
@@ -425,94 +457,86 @@ __device__ inline void ft_corr_thin_tile(const FtCorrTile& tl, const FtGroup* __
                                          const float* __restrict__ cell_sq,
                                          const int16_t* __restrict__ ref_rank,
                                          const float* __restrict__ ref_sq,
-                                         float* __restrict__ scores, int16_t* cthin)
+                                         float* __restrict__ scores, int16_t* cthin, int kt_max)
 {
     const FtGroup    g     = groups[tl.group];
     const int        gs    = (int)g.gs;
     const int        st    = (int)g.stride;
+    // cell rows staged kt genes at a time (all of them when they fit)
+    const int        kt    = st < kt_max ? st : kt_max;  // multiple of 8
     const int        ncell = min(tl.lim ? (int)tl.lim : TC, min(TC, (int)g.ncell - (int)tl.cell0));
     const int        nref  = min(256, (int)g.nref - (int)tl.ref0);
     const int16_t*   cbase = cell_rank + g.cell_dst0 + (uint64_t)tl.cell0 * st;
-    for (int v = threadIdx.x; v < ncell * st / 8; v += 256)
-        *(int4*)(cthin + v * 8) = *(const int4*)(cbase + (uint64_t)v * 8);
-    __syncthreads();
-    const int r = threadIdx.x;
-    if (r >= nref)
-        return;  // no barrier follows in this tile
-    const int16_t* y    = ref_rank + g.ref_dst0 + (uint64_t)(tl.ref0 + r) * st;
-    const int      step = (gs + 127) / 128;
-    const int      nz   = (gs + step - 1) / step;
-    float          stk[TC][7];
-    float          full[TC];
+    const int        r     = threadIdx.x;
+    const bool       live  = r < nref;  // idle lanes still take part in the stage barriers
+    const int16_t*   y     = ref_rank + g.ref_dst0 + (uint64_t)(tl.ref0 + (live ? r : 0)) * st;
+    const int        step  = (gs + 127) / 128;
+    const int        nz    = (gs + step - 1) / step;
+    float            stk[TC][7];
+    float            full[TC];
+    float            acc[TC];
 #pragma unroll
     for (int a = 0; a < TC; ++a)
     {
         full[a] = 0.f;
+        acc[a]  = 0.f;
 #pragma unroll
         for (int l = 0; l < 7; ++l)
             stk[a][l] = 0.f;
     }
-    for (int ch = 0; ch < nz; ++ch)
+    int ch     = 0;
+    int ch_end = step < gs ? step : gs;
+    for (int k0 = 0; k0 < gs; k0 += kt)
     {
-        const int lo = ch * step;
-        const int hi = min(lo + step, gs);
-        float     acc[TC];
-#pragma unroll
-        for (int a = 0; a < TC; ++a)
-            acc[a] = 0.f;
-        for (int i = lo; i < hi; ++i)
+        const int w8 = min(kt, st - k0) / 8;  // int4 per staged row
+        if (k0)
+            __syncthreads();  // the previous stage is no longer read
+        for (int v = threadIdx.x; v < ncell * w8; v += 256)
         {
-            const float yv = (float)y[i];
-#pragma unroll
-            for (int a = 0; a < TC; ++a)
-                if (a < ncell)
-                    acc[a] = __builtin_fmaf((float)cthin[a * st + i], yv, acc[a]);
+            const int row = v / w8, c = v % w8;
+            *(int4*)(cthin + row * kt + c * 8) =
+                *(const int4*)(cbase + (uint64_t)row * st + k0 + c * 8);
         }
-#pragma unroll
-        for (int a = 0; a < TC; ++a)
+        __syncthreads();
+        if (!live)
+            continue;
+        const int k1 = min(k0 + kt, gs);
+        int       i  = k0;
+        while (i < k1)
         {
-            float v    = acc[a];
-            int   cc   = ch;
-            bool  open = true;
-#pragma unroll
-            for (int l = 0; l < 7; ++l)
+            const int e = min(ch_end, k1);
+            for (; i < e; ++i)
             {
-                if (open)
-                {
-                    if (cc & 1)
-                    {
-                        v = stk[a][l] + v;
-                        cc >>= 1;
-                    }
-                    else
-                    {
-                        stk[a][l] = v;
-                        open      = false;
-                    }
-                }
+                const float yv = (float)y[i];
+#pragma unroll
+                for (int a = 0; a < TC; ++a)
+                    if (a < ncell)
+                        acc[a] = __builtin_fmaf((float)cthin[a * kt + i - k0], yv, acc[a]);
             }
-            if (open)
-                full[a] = v;
+            if (i == ch_end)
+            {
+#pragma unroll
+                for (int a = 0; a < TC; ++a)
+                {
+                    ft_tree_push(stk[a], full[a], acc[a], ch);
+                    acc[a] = 0.f;
+                }
+                ++ch;
+                ch_end = min(ch_end + step, gs);
+            }
         }
     }
+    if (!live)
+        return;  // no barrier follows in this tile
     const float syy = ref_sq[g.ref_item0 + tl.ref0 + r];
 #pragma unroll
     for (int a = 0; a < TC; ++a)
     {
         if (a >= ncell)
             continue;
-        float v    = 0.f;
-        bool  have = false;
-#pragma unroll
-        for (int l = 0; l < 7; ++l)
-            if ((nz >> l) & 1)
-            {
-                v    = have ? stk[a][l] + v : stk[a][l];
-                have = true;
-            }
-        const float root = nz == 128 ? full[a] : v;
         scores[g.score0 + (uint64_t)(tl.cell0 + a) * g.nref + tl.ref0 + r] =
-            corr_from_sums(root, cell_sq[g.cell_item0 + tl.cell0 + a], syy);
+            corr_from_sums(ft_tree_root(stk[a], full[a], nz), cell_sq[g.cell_item0 + tl.cell0 + a],
+                           syy);
     }
 }
 
@@ -523,11 +547,12 @@ __global__ __launch_bounds__(256) void k_ft_corr_thin(const FtCorrTile* __restri
                                                       const float* __restrict__ cell_sq,
                                                       const int16_t* __restrict__ ref_rank,
                                                       const float* __restrict__ ref_sq,
-                                                      float* __restrict__ scores)
+                                                      float* __restrict__ scores,
+                                                      int kt_max)
 {
     extern __shared__ int16_t cthin[];
     ft_corr_thin_tile<TC>(tiles[blockIdx.x], groups, cell_rank, cell_sq, ref_rank, ref_sq, scores,
-                          cthin);
+                          cthin, kt_max);
 }
 
 // New candidate list of one cell from its per-label scores, following

@@ -1481,7 +1481,7 @@ __global__ __launch_bounds__(FT_NT) void k_ftd_corr(
     const FtGroup* __restrict__ groups, const uint8_t* __restrict__ gflag,
     const int16_t* __restrict__ cell_rank, const float* __restrict__ cell_sq,
     const int16_t* __restrict__ ref_rank, const float* __restrict__ ref_sq,
-    float* __restrict__ scores)
+    float* __restrict__ scores, int kt_max)
 {
     extern __shared__ int srow32[];
     const uint32_t n = cnt->n_tiles;
@@ -1492,7 +1492,7 @@ __global__ __launch_bounds__(FT_NT) void k_ftd_corr(
             continue;
         __syncthreads();
         ft_corr_tile<FT_TC, FT_TR, FT_RC, FT_RR>(tl, groups, cell_rank, cell_sq, ref_rank, ref_sq,
-                                                 scores, srow32);
+                                                 scores, srow32, kt_max);
     }
 }
 
@@ -1501,7 +1501,7 @@ __global__ __launch_bounds__(256) void k_ftd_thin(
     const FtdCounts* __restrict__ cnt, int redo, const FtCorrTile* __restrict__ tiles,
     const FtGroup* __restrict__ groups, const int16_t* __restrict__ cell_rank,
     const float* __restrict__ cell_sq, const int16_t* __restrict__ ref_rank,
-    const float* __restrict__ ref_sq, float* __restrict__ scores)
+    const float* __restrict__ ref_sq, float* __restrict__ scores, int kt_max)
 {
     extern __shared__ int16_t cthin[];
     const uint32_t n = redo ? cnt->n_rthin : cnt->n_thin;
@@ -1509,7 +1509,7 @@ __global__ __launch_bounds__(256) void k_ftd_thin(
     {
         __syncthreads();
         ft_corr_thin_tile<FT_THIN>(tiles[t], groups, cell_rank, cell_sq, ref_rank, ref_sq, scores,
-                                   cthin);
+                                   cthin, kt_max);
     }
 }
 

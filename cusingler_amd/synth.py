@@ -59,6 +59,8 @@ class SynthConfig:
     seed: int = 1234
     n_train: int = 0         # extra query-like cells (learned proposal training set)
     marker_pool: float = 0.05  # markers are drawn from this top fraction of genes by baseline
+    base_sigma: float = 1.0    # baseline expression ~ LogNormal(0, base_sigma)
+    label_sigma: float = 0.0   # every gene's per-label fold ~ LogNormal(0, label_sigma) (0: none)
 
 
 CONFIGS: Dict[str, SynthConfig] = {
@@ -239,8 +241,12 @@ def generate(cfg: SynthConfig | str, n_qry: Optional[int] = None, n_ref: Optiona
     ref_order = interleave(ref_only)
     qry_order = interleave(qry_only)
 
-    base = rng.lognormal(0.0, 1.0, size=n_uni).astype(np.float32)
+    base = rng.lognormal(0.0, cfg.base_sigma, size=n_uni).astype(np.float32)
     fold = _fold_matrix(cfg, rng, common, n_uni, base)
+    if cfg.label_sigma > 0:
+        # label programmes beyond the markers: every gene's mean differs a
+        # little between labels, so label pairs differ in many genes
+        fold *= rng.lognormal(0.0, cfg.label_sigma, size=fold.shape).astype(np.float32)
     mu_uni = base[None, :] * fold  # [L, universe]
 
     # reference labels (grouped sizes), shuffled cell order

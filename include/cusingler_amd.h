@@ -266,11 +266,11 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
 int csg_selftest(csg_engine* eng, int which, double* max_err);
 
 /* Diagnostic of the fine-tune correlation kernels (wide and thin tiles): 40 x
- * 70 pairs of seeded random ranks over `gs` genes, compared bit for bit with
- * the 128-chunk + tree FP32 order of spearman_reduce
- * (src/cusingler.cu:804-856) computed on the host; *mismatches receives the
- * number of differing sums over both kernels.  No counterpart in
- * the reference. */
+ * 70 pairs of seeded random ranks over `gs` genes (1..16384), compared bit
+ * for bit with the 128-chunk + tree FP32 order of spearman_reduce
+ * (src/cusingler.cu:804-856) computed on the host; each kernel runs with its
+ * widest LDS stage and with 64-gene stages; *mismatches receives the number
+ * of differing sums over the four runs.  No counterpart in the reference. */
 int csg_selftest_ft(csg_engine* eng, int gs, int64_t* mismatches);
 
 /* Host-side evaluation of the reference's `percentile` (src/cusingler.cu:

@@ -79,12 +79,25 @@ def test_exact_path_labels_match_reference_order(eng, name):
     err = np.abs(scores.astype(np.float64) - osc.astype(np.float64))
     err = err[~np.isnan(err)] if np.isnan(err).any() else err
     assert err.size == 0 or np.max(err) <= w, f"score error {np.nanmax(err)} above the certified width {w}"
+    # every exact-path quantile is the exact correlation's (normalised in
+    # FP64: within one float ulp of the oracle's exact variant) or, where the
+    # certification re-scored the pair, the reference's FP32 value
+    _, _, oex = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_EXACT)
     ok = ~np.isnan(osc)
+    ulp = np.spacing(np.abs(oex[ok]).astype(np.float32))
+    near_exact = np.abs(scores[ok].astype(np.float64) - oex[ok]) <= ulp
+    is_fp32 = scores[ok].view(np.uint32) == osc[ok].view(np.uint32)
+    assert np.all(near_exact | is_fp32), f"{int((~(near_exact | is_fp32)).sum())} quantiles"
+    # against the reference's FP32 quantiles: within 1e-5 relative, except
+    # where the reference's own FP32 error (about 3.5e-6 absolute at 900
+    # genes) is above 1e-5 of a small quantile
     rel = np.abs(scores[ok].astype(np.float64) - osc[ok]) / np.abs(osc[ok].astype(np.float64))
-    big = np.abs(osc[ok]) >= 0.1
-    print(f"[exact-rel] {name}: max rel {rel.max() if rel.size else 0:.3g}, over |P|>=0.1 "
-          f"{rel[big].max() if big.any() else 0:.3g}, pairs > 1e-5 rel {(rel > 1e-5).sum()} of "
-          f"{rel.size} (|P| of those: max {np.abs(osc[ok])[rel > 1e-5].max() if (rel > 1e-5).any() else 0:.3g})",
+    big = np.abs(osc[ok]) >= 0.25
+    assert not big.any() or rel[big].max() <= 1e-5, rel[big].max()
+    print(f"[exact-rel] {name}: re-scored (FP32) quantiles {int((is_fp32 & ~near_exact).sum())}, "
+          f"max rel vs FP32 reference {rel.max() if rel.size else 0:.3g}, at |P|>=0.25 "
+          f"{rel[big].max() if big.any() else 0:.3g}; {(rel > 1e-5).sum()} of {rel.size} above "
+          f"1e-5 (|P| <= {np.abs(osc[ok])[rel > 1e-5].max() if (rel > 1e-5).any() else 0:.3g})",
           flush=True)
     print(f"[exact] {name}: rechecked pairs {cnt['rechecked_pairs']}, uncertified cells "
           f"{cnt['uncertified_cells']} of {ds.qry_height}, max |dP| {np.max(err) if err.size else 0:.3g} "
