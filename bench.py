@@ -54,6 +54,25 @@ def calibrate_copy(nbytes: int = 256 << 20):
     hip.hipFree(b)
 
 
+def host_cpus():
+    """Threads the CPU baseline uses and what the host is: the CPUs this
+    process may run on (nproc), capped by OMP_NUM_THREADS where the box sets
+    it (its CPU share per GPU), and the CPU model."""
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cap = os.environ.get("OMP_NUM_THREADS")
+    cores = min(nproc, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else nproc
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, nproc, model
+
+
 def cpu_baseline(ds, n_sample: int, cores: int):
     """The oracle (a CPU restatement of the reference's score + fine-tune steps)
     on the first n_sample query cells."""
@@ -87,7 +106,7 @@ def main():
                     help="largest fine-tune group (cells) of the fused FP32 form, 0 to 4 "
                          "(default: the engine's, 4)")
     ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--hidden", type=int, default=256,
+    ap.add_argument("--hidden", type=int, default=512,
                     help="width of the proposal network's two hidden layers (multiples of 256 take the pipelined layer kernel)")
     ap.add_argument("--proposal-k", type=int, default=4)
     ap.add_argument("--min-margin", type=float, default=0.2)
@@ -96,7 +115,8 @@ def main():
                          "exists (skipping training), else save them there after training")
     ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cores", type=int, default=None,
+                    help="host threads (default: nproc, capped by OMP_NUM_THREADS)")
     ap.add_argument("--only-chosen", action="store_true",
                     help="measure only the chosen form (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -111,6 +131,9 @@ def main():
     import dataclasses
     import numpy as np
     from cusingler_amd import engine, synth
+    auto_cores, nproc, cpu_model = host_cpus()
+    if args.cores is None:
+        args.cores = auto_cores
     if args.calibrate_copy:
         calibrate_copy()
 
@@ -204,12 +227,14 @@ def main():
         ph = eng.phase_times(reset=True)
         # labels (and counters) of one untimed pass for the agreement check;
         # the per-phase breakdown is taken from this pass, every phase timed
-        eng.set_phase_timing(2)
+        # (the device fine-tune's kernel families too: mode 3)
+        eng.set_phase_timing(3)
         first, _, _ = eng.get_label(0.8, 0.05, path, outputs=True)
         cnt = eng.counters()
         final, _ = eng.finetune(0.8, 0.05, 0)
         cnt.update({k: v for k, v in eng.counters().items() if k.startswith("ft_")})
         ph1 = eng.phase_times(reset=True)
+        eng.set_phase_timing(2)
         return dict(value=Nq * args.steps / dt, ms=dt * 1e3 / args.steps, phases=ph, phases1=ph1,
                     counters=cnt, first=first, final=final)
 
@@ -258,6 +283,35 @@ def main():
                 roofline["traffic_source"] = t.get("source")
     except (OSError, ValueError, KeyError):
         pass
+    # achieved HBM GB/s of the bandwidth-bound score-step stages (algorithmic
+    # bytes over the phase time of the labelled pass) and the fine-tune's work
+    Kp = (Gs + 63) // 64 * 64
+    nnz0 = int(c0.qry_indptr[-1])
+    c = r["counters"]
+    feat = 2 * Kp if chosen == "proposal" else 0
+    limbs = 4 * Kp if chosen in ("exact_int8", "proposal") else 0
+    # rank: CSR codes + indices read (6 B / nonzero) and the row pointers,
+    # the int16 rank row and its inverse root written, the limb planes (read
+    # the ranks, write two int8 planes) and the network features
+    rank_bytes = 6 * nnz0 + Nq * (8 + 2 * Kp + 8 + limbs + feat)
+    scores_read = c["prop_correlations"] if chosen == "proposal" else Nq * Nr
+    select_bytes = 4 * scores_read + 4 * Nq * len(ds.codes)
+
+    def stage(bytes_, ms):
+        gbs = bytes_ / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"bytes": int(bytes_), "ms": round(ms, 4), "GB/s": round(gbs, 1),
+                "frac_of_hbm_peak": round(gbs / MI355X_PEAK["hbm_GBps"], 4)}
+
+    stages = {"rank": stage(rank_bytes, r["phases1"]["rank"][0]),
+              "select": stage(select_bytes, r["phases1"]["select"][0])}
+    pg = c.get("ft_pair_genes", 0)
+    ft_ms = r["phases1"]["finetune"][0]
+    ft_line = {"pair_genes": pg, "ops": 2 * pg, "bytes": 6 * pg, "ms": round(ft_ms, 3),
+               "achieved_TOPs": round(2 * pg / (ft_ms * 1e-3) / 1e12, 3) if ft_ms > 0 else 0.0,
+               "families_ms": {k: round(r["phases1"][k][0], 3)
+                               for k in ("ft_rank", "ft_corr", "ft_select")},
+               "definition": "ops = 2 x sum over rounds of cells x reference rows x |G_sub|; "
+                             "bytes = 6 x the same (int16 ranks of both rows + FP32 score)"}
     plain = res.get("plain", r)
     match = {"first": float((r["first"] == plain["first"]).mean()),
              "final": float((r["final"] == plain["final"]).mean())}
@@ -271,7 +325,11 @@ def main():
                   "proposal": "int8"}[chosen],
         "data": "synthetic (seeded NB counts, log2(1+x), float32 CSR)",
         "config": {"workload": workload, "form": chosen, "query_cells": Nq, "reference_cells": Nr,
-                   "labels": len(ds.codes), "score_genes": Gs, "finetune_genes": c1.width},
+                   "labels": len(ds.codes), "score_genes": Gs, "finetune_genes": c1.width,
+                   "network": (f"input: doubled ranks over the score gene set ({Gs} genes), "
+                               f"{args.hidden} -> {args.hidden} hidden (GELU), {len(ds.codes)} "
+                               f"logits, top-{args.proposal_k}")
+                   if chosen == "proposal" else None},
         "plain": {"value": round(plain["value"], 1), "ms_per_step": round(plain["ms"], 3),
                   "phases_ms": {k: round(v[0], 3) for k, v in plain["phases1"].items()}},
         "speedup_vs_plain": round(r["value"] / plain["value"], 3),
@@ -281,6 +339,8 @@ def main():
         "phases_source": "one untimed pass with every phase timed (the timed steps time the "
                          "contraction phase only)",
         "counters": r["counters"],
+        "stages": stages,
+        "finetune": ft_line,
     }
     if "exact_int8" in res and chosen != "exact_int8":
         e = res["exact_int8"]
@@ -304,6 +364,8 @@ def main():
         print(f"[bench] CPU baseline on {n} cells, {args.cores} threads", file=sys.stderr,
               flush=True)
         out["cpu_baseline"] = cpu_baseline(ds, n, args.cores)
+        out["cpu_baseline"]["nproc"] = nproc
+        out["cpu_baseline"]["cpu_model"] = cpu_model
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

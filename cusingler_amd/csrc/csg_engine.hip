@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <random>
 #include <string>
 #include <unordered_map>
@@ -203,6 +204,7 @@ struct csg_engine
         ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_grid_fused = 4096, ftd_thin_max = 8,
         ftd_t64 = 48;
     int ftd_fuse_max = FTD_FUSE_MAX;  // groups of at most this many cells take the fused form
+    bool ftd_eager = false;  // part B is being launched eagerly (not captured): phase events allowed
     bool     ftd_autotune = true, ftd_tuned = false;
     uint64_t ftd_budget   = (uint64_t)8 << 30;  // device bytes of one work batch
     int64_t  ftd_batches  = 0;                   // work batches of oversized rounds
@@ -247,7 +249,9 @@ struct csg_engine
     // ---- timing / counters ----
     double  phase_ms[CSG_N_PHASES]  = {};
     int64_t phase_cnt[CSG_N_PHASES] = {};
-    int     phase_mode = 2;  // 0 no phase events, 1 the correlation phase only, 2 all
+    // 0 no phase events, 1 the correlation phase only, 2 all, 3 all and the
+    // device fine-tune's kernel families (its part B launched eagerly)
+    int     phase_mode = 2;
     std::vector<PhaseEvents> pending;
     std::vector<hipEvent_t>  event_pool;
     int64_t counters[CSG_N_COUNTERS] = {};
@@ -354,7 +358,7 @@ struct PhaseScope
     PhaseEvents pe;
     bool        on;
     PhaseScope(csg_engine* e, int phase, int launches)
-        : eng(e), on(e->phase_mode == 2 || (e->phase_mode == 1 && phase == 1))
+        : eng(e), on(e->phase_mode >= 2 || (e->phase_mode == 1 && phase == 1))
     {
         if (!on)
             return;
@@ -1212,6 +1216,9 @@ struct PropRound
 int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
 {
     const uint32_t L = eng->L;
+    // wide 128 x 256 blocks with folded limbs up to 1920 genes, else the
+    // three-partial 64 x 128 blocks
+    const bool wide = eng->G <= 1920 && eng->w8_ok;
     std::vector<uint32_t>  abase(L), acnt(L);
     std::vector<uint64_t>  coff(L);
     std::vector<GroupTile> tiles;
@@ -1225,8 +1232,8 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
         coff[l]           = n_scores;
         n_items += na;
         n_scores += (uint64_t)na * nl;
-        for (uint32_t m0 = 0; m0 < na; m0 += W8_BM)
-            for (uint32_t n0 = 0; n0 < nl; n0 += W8_BN)
+        for (uint32_t m0 = 0; m0 < na; m0 += wide ? W8_BM : I8_BM)
+            for (uint32_t n0 = 0; n0 < nl; n0 += wide ? W8_BN : I8_BN)
                 tiles.push_back(GroupTile{ l, m0, n0, 0u });
     }
     if (n_items == 0)
@@ -1282,10 +1289,16 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
     {
         PhaseScope ps(eng, 1, 1);
         const uint32_t b_max = (uint32_t)round_up(eng->Nr, 256) - 1;
-        k_gemm_int8w_group<<<(unsigned)tiles.size(), 512, W8_LDS, eng->stream>>>(
-            eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart, eng->d_llen,
-            eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl, (int)eng->Kp, b_max,
-            eng->d_irq, eng->d_irr, eng->d_S);
+        if (wide)
+            k_gemm_int8w_group<<<(unsigned)tiles.size(), 512, W8_LDS, eng->stream>>>(
+                eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
+                eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S);
+        else
+            k_gemm_int8_group<<<(unsigned)tiles.size(), 256, 0, eng->stream>>>(
+                eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
+                eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S);
         LAUNCHCHK();
     }
     {
@@ -1462,12 +1475,6 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     if (prop && ((uint32_t)eng->prop_k > eng->L || eng->L > 256))
         return eng->fail(CSG_ERR_ARGUMENT, "proposal size exceeds the label count, or more than "
                                            "256 labels");
-    if (prop && eng->G > 1920)
-        return eng->fail(CSG_ERR_ARGUMENT, "the learned proposal's grouped contraction needs a "
-                                           "score gene set of at most 1920 genes");
-    if (prop && !eng->w8_ok)
-        return eng->fail(CSG_ERR_DEVICE, "the device does not grant the grouped contraction's "
-                                         "LDS");
     if ((rc = set_pct(eng, &eng->d_pct, &eng->pct_quantile, eng->ctidx, eng->L, quantile)))
         return rc;
     if (prop && !eng->d_pcells && (rc = alloc_proposal(eng)))
@@ -2617,12 +2624,25 @@ int ftd_launch_b(csg_engine* eng, float quantile, float thre, int times)
         eng->d_fl_glabels, eng->d_flstart, eng->d_fllen, eng->d_fl_refgroup, eng->d_fl_refrow,
         (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0, eng->d_fl_gthin0,
         eng->d_fl_tiles, eng->d_fl_thin);
-    if ((eng->ftd_mask & FTDM_RANK) && ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
-        return CSG_ERR_DEVICE;
-    if (ftd_launch_rank_qry(eng, eng->ftd_grid_rank))
-        return CSG_ERR_DEVICE;
-    if (ftd_launch_corr(eng, eng->ftd_grid_corr) || ftd_launch_fused(eng, eng->ftd_grid_fused) ||
-        ftd_launch_select(eng, eng->ftd_grid_select, quantile))
+    // kernel families timed only when launched eagerly (phase events cannot
+    // be recorded into a captured graph)
+    auto fam = [&](int phase) {
+        return std::unique_ptr<PhaseScope>(eng->ftd_eager ? new PhaseScope(eng, phase, 1) : nullptr);
+    };
+    {
+        auto ps = fam(7);
+        if ((eng->ftd_mask & FTDM_RANK) && ftd_launch_rank_ref(eng, eng->ftd_grid_rank))
+            return CSG_ERR_DEVICE;
+        if (ftd_launch_rank_qry(eng, eng->ftd_grid_rank))
+            return CSG_ERR_DEVICE;
+    }
+    {
+        auto ps = fam(6);
+        if (ftd_launch_corr(eng, eng->ftd_grid_corr) || ftd_launch_fused(eng, eng->ftd_grid_fused))
+            return CSG_ERR_DEVICE;
+    }
+    auto ps_sel = fam(8);
+    if (ftd_launch_select(eng, eng->ftd_grid_select, quantile))
         return CSG_ERR_DEVICE;
     k_ftd_select_nan<FtdSegments><<<g, FTD_TPB, 0, eng->stream>>>(
         cnt, 0, seg, eng->d_fl_nan1, eng->d_ft_scores, eng->d_fpct, quantile, eng->d_ft_P,
@@ -3063,10 +3083,22 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
             if (!eng->ftd_tuned && eng->ftd_autotune && h.n_cells >= 4096 &&
                 (rc = ftd_tune(eng, quantile)))
                 return rc;
-            hipGraphExec_t gB = nullptr;
-            if ((rc = ftd_graph_b(eng, quantile, thre, times, &gB)))
-                return rc;
-            HIPCHK(hipGraphLaunch(gB, eng->stream));
+            if (eng->phase_mode == 3)
+            {
+                // part B launched eagerly, its kernel families timed
+                eng->ftd_eager = true;
+                rc             = ftd_launch_b(eng, quantile, thre, times);
+                eng->ftd_eager = false;
+                if (rc)
+                    return rc;
+            }
+            else
+            {
+                hipGraphExec_t gB = nullptr;
+                if ((rc = ftd_graph_b(eng, quantile, thre, times, &gB)))
+                    return rc;
+                HIPCHK(hipGraphLaunch(gB, eng->stream));
+            }
         }
         else if ((rc = ftd_batches(eng, h, quantile, thre, times)))
             return rc;
@@ -3088,6 +3120,7 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
                           eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
     eng->counters[11] += h.tot_redo;
+    eng->counters[10] += (int64_t)h.pair_genes;
     if (h.undefined)
         return eng->fail(CSG_ERR_UNDEFINED,
                          "a NaN-carrying fine-tune segment makes the reference's sort leave its "
@@ -3524,7 +3557,7 @@ int csg_set_finetune_path(csg_engine* eng, int path)
 
 int csg_set_phase_timing(csg_engine* eng, int mode)
 {
-    if (!eng || mode < 0 || mode > 2)
+    if (!eng || mode < 0 || mode > 3)
         return CSG_ERR_ARGUMENT;
     eng->phase_mode = mode;
     return CSG_OK;

@@ -61,6 +61,7 @@ struct FtdCounts
     uint64_t n_segs;
     uint64_t n_redo_segs;
     uint64_t rthin_bound;  // tiles a redo of every exact-form cell would take
+    uint64_t pair_genes;   // sum over rounds of cells x reference rows x |G_sub| (work measure)
 };
 
 // Height of the big exact fine-tune tile (cells).
@@ -451,6 +452,9 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
                                  (!fused && exact_ok && gs <= 1919 ? FTD_EXACT : 0));
             if (fused && gs >= 20)
                 atomicAdd(&cnt->n_fused, 1u);
+            if (gs >= 20)
+                atomicAdd((unsigned long long*)&cnt->pair_genes,
+                          (unsigned long long)G.ncell * G.nref * gs);
         }
     }
 }

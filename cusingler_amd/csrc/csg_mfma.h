@@ -76,10 +76,54 @@ __global__ void k_to_half(const int16_t* __restrict__ X, size_t n, _Float16* __r
 // is exact, and the score is exact_corr(S4, ir_q[m], ir_r[n]).
 constexpr int I8_BM = 64, I8_BN = 128, I8_BK = 64, I8_LD = 80;
 
-__global__ __launch_bounds__(256) void k_gemm_int8(
-    const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
-    const int8_t* __restrict__ rl, int Kp, const double* __restrict__ ir_q,
-    const double* __restrict__ ir_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+// Row / column maps of one 64 x 128 block of k_gemm_int8 (the same roles as
+// the wide kernel's maps below): a_row / b_row give the operand rows staged
+// for block rows / columns r (inside the operand buffers); row_ok / col_ok
+// which results are stored; row_out the output row of block row m (column 0
+// of the block); row_ir / col_ir the inverse roots of block row m / column n.
+struct DenseI8Map
+{
+    int           m0, n0, nq, nr;
+    const double* ir_q;
+    const double* ir_r;
+    float*        S;
+    size_t        ldS;
+    __device__ size_t a_row(int r) const { return (size_t)(m0 + r); }
+    __device__ size_t b_row(int r) const { return (size_t)(n0 + r); }
+    __device__ bool   row_ok(int m) const { return m0 + m < nq; }
+    __device__ double row_ir(int m) const { return ir_q[m0 + m]; }
+    __device__ float* row_out(int m) const { return S + (size_t)(m0 + m) * ldS + n0; }
+    __device__ bool   col_ok(int n) const { return n0 + n < nr; }
+    __device__ double col_ir(int n) const { return ir_r[n0 + n]; }
+};
+
+// One label of a grouped contraction in 64 x 128 blocks: the query rows
+// listed in cells[0, mcount) against reference rows b0 .. b0 + ncount - 1;
+// row m of the result is stored contiguously at out + m * ld.  Rows past the
+// list repeat its last row, reference rows past b_max read row b_max.
+struct GroupI8Map
+{
+    const uint32_t* cells;
+    int             mcount, ncount;
+    uint32_t        b0, b_max;
+    const double*   ir_q;
+    const double*   ir_r;
+    float*          out;
+    size_t          ld;
+    __device__ size_t a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
+    __device__ size_t b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
+    __device__ bool   row_ok(int m) const { return m < mcount; }
+    __device__ double row_ir(int m) const { return ir_q[a_row(m)]; }
+    __device__ float* row_out(int m) const { return out + (size_t)m * ld; }
+    __device__ bool   col_ok(int n) const { return n < ncount; }
+    __device__ double col_ir(int n) const { return ir_r[b_row(n)]; }
+};
+
+template <class Map>
+__device__ inline void gemm_int8_block(const int8_t* __restrict__ qh,
+                                       const int8_t* __restrict__ ql,
+                                       const int8_t* __restrict__ rh,
+                                       const int8_t* __restrict__ rl, int Kp, const Map& map)
 {
     __shared__ __attribute__((aligned(16))) int8_t sm[2][(2 * I8_BM + 2 * I8_BN) * I8_LD];
     const int tid  = threadIdx.x;
@@ -87,32 +131,32 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
     const int wid  = tid >> 6;
     const int wm   = wid >> 1;  // 0..1: 32-row half
     const int wn   = wid & 1;   // 0..1: 64-col half
-    const int m0   = blockIdx.y * I8_BM;
-    const int n0   = blockIdx.x * I8_BN;
 
     // global -> LDS staging: A planes 64 rows x 64 B (2 x 256 x 16 B), B planes
     // 128 rows x 64 B (2 x 512 x 16 B); thread t moves 16-byte chunks
-    auto load_stage = [&](int buf, int k0, int4 (&ra)[2], int4 (&rb)[4]) {
-        (void)buf;
+    size_t asrc[2], bsrc[4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+        asrc[u] = map.a_row(((tid + u * 256) >> 2) & 63) * (size_t)Kp;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        bsrc[u] = map.b_row(((tid + u * 256) >> 2) & 127) * (size_t)Kp;
+    auto load_stage = [&](int k0, int4 (&ra)[2], int4 (&rb)[4]) {
 #pragma unroll
         for (int u = 0; u < 2; ++u)
         {
             const int c   = tid + u * 256;   // 0..511 over (plane, row, chunk)
             const int pl  = c >> 8;          // 0: hi, 1: lo
-            const int row = (c >> 2) & 63;
             const int ch  = c & 3;
-            const int8_t* src = (pl ? ql : qh) + (size_t)(m0 + row) * Kp + k0 + ch * 16;
-            ra[u] = *(const int4*)src;
+            ra[u] = *(const int4*)((pl ? ql : qh) + asrc[u] + k0 + ch * 16);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
         {
             const int c   = tid + u * 256;   // 0..1023
             const int pl  = c >> 9;
-            const int row = (c >> 2) & 127;
             const int ch  = c & 3;
-            const int8_t* src = (pl ? rl : rh) + (size_t)(n0 + row) * Kp + k0 + ch * 16;
-            rb[u] = *(const int4*)src;
+            rb[u] = *(const int4*)((pl ? rl : rh) + bsrc[u] + k0 + ch * 16);
         }
     };
     auto store_stage = [&](int buf, const int4 (&ra)[2], const int4 (&rb)[4]) {
@@ -143,7 +187,7 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
         }
 
     int4 ra[2], rb[4];
-    load_stage(0, 0, ra, rb);
+    load_stage(0, ra, rb);
     store_stage(0, ra, rb);
     __syncthreads();
     const int r16  = lane & 15;
@@ -153,7 +197,7 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
     {
         const bool more = k0 + I8_BK < Kp;
         if (more)
-            load_stage(buf ^ 1, k0 + I8_BK, ra, rb);
+            load_stage(k0 + I8_BK, ra, rb);
         const int8_t* base = sm[buf];
         v4i ah[2], al[2], bh[4], bl[4];
 #pragma unroll
@@ -195,22 +239,32 @@ __global__ __launch_bounds__(256) void k_gemm_int8(
 #pragma unroll
         for (int e = 0; e < 4; ++e)
         {
-            const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + e;
-            if (m >= nq)
+            const int m = wm * 32 + i * 16 + (lane >> 4) * 4 + e;
+            if (!map.row_ok(m))
                 continue;
-            const double ir  = ir_q[m];
-            float*       out = S + (size_t)m * ldS;
+            const double ir  = map.row_ir(m);
+            float*       out = map.row_out(m);
 #pragma unroll
             for (int j = 0; j < 4; ++j)
             {
-                const int n = n0 + wn * 64 + j * 16 + r16;
-                if (n >= nr)
+                const int n = wn * 64 + j * 16 + r16;
+                if (!map.col_ok(n))
                     continue;
                 const double s4 = (double)acc_hh[i][j][e] * 65536.0 +
                                   (double)acc_x[i][j][e] * 256.0 + (double)acc_ll[i][j][e];
-                out[n] = exact_corr(s4, ir, ir_r[n]);
+                out[n] = exact_corr(s4, ir, map.col_ir(n));
             }
         }
+}
+
+__global__ __launch_bounds__(256) void k_gemm_int8(
+    const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
+    const int8_t* __restrict__ rl, int Kp, const double* __restrict__ ir_q,
+    const double* __restrict__ ir_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+{
+    const DenseI8Map map{ (int)blockIdx.y * I8_BM, (int)blockIdx.x * I8_BN, nq, nr, ir_q, ir_r, S,
+                          ldS };
+    gemm_int8_block(qh, ql, rh, rl, Kp, map);
 }
 
 // Wide-tile variant for |X| <= 1919 (gene sets of at most 1920 genes).  Then
@@ -496,6 +550,33 @@ __global__ __launch_bounds__(512) void k_gemm_int8w_group(
                           S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
                           nl };
     gemm_int8w_block(qh, ql, rh, rl, Kp, map, w8_sm);
+}
+
+// The grouped contraction in 64 x 128 blocks with three partials, for any
+// score gene set (the wide kernel's folded limbs need |X| <= 1919): tile t =
+// tiles[t] = (label l, m0, n0) with m0 a multiple of 64 and n0 of 128, the
+// same tables and output layout as k_gemm_int8w_group.
+__global__ __launch_bounds__(256) void k_gemm_int8_group(
+    const GroupTile* __restrict__ tiles, const uint32_t* __restrict__ cells,
+    const uint32_t* __restrict__ a_base, const uint32_t* __restrict__ a_cnt,
+    const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
+    const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
+    const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
+    int Kp, uint32_t b_max, const double* __restrict__ ir_q, const double* __restrict__ ir_r,
+    float* __restrict__ S)
+{
+    const GroupTile t  = tiles[xcd_order(blockIdx.x, gridDim.x)];
+    const uint32_t  nl = llen[t.label];
+    const GroupI8Map map{ cells + a_base[t.label] + t.m0,
+                          (int)(a_cnt[t.label] - t.m0),
+                          (int)(nl - t.n0),
+                          lstart[t.label] + t.n0,
+                          b_max,
+                          ir_q,
+                          ir_r,
+                          S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
+                          nl };
+    gemm_int8_block(qh, ql, rh, rl, Kp, map);
 }
 
 // ------------------------------------- approximate MFMA contractions ---

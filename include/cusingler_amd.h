@@ -171,7 +171,10 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
  * events on the stream, which costs about 1% of a proposal-path step at the
  * atlas size.  mode 0: none; 1: the score-step correlation phase only (the
  * contraction's live launch time, as bench.py's roofline uses it); 2: every
- * phase (the default).  CSG_ERR_ARGUMENT for any other mode. */
+ * phase (the default); 3: every phase, and the device-resident fine-tune
+ * launches each round's work kernels eagerly instead of replaying their
+ * captured graph, so that its rank [7], correlation [6] and select+filter [8]
+ * families are timed too.  CSG_ERR_ARGUMENT for any other mode. */
 int csg_set_phase_timing(csg_engine* eng, int mode);
 
 /* Counters of the last csg_get_label / csg_finetune call:
@@ -244,8 +247,9 @@ int csg_mlp_predict(csg_engine* eng, int k, int32_t* top, float* margin);
  * Unscored labels get quantile -inf, so they stay out of the mask; the labels
  * then differ from the full path's only where a label the network ranked
  * below the last scored one would have entered the mask although that one
- * did not.  Needs a score gene set of at most 1920 genes (the grouped
- * contraction folds the high limbs) and at most 256 labels. */
+ * did not.  Any score gene set (up to 1920 genes the grouped contraction
+ * folds the high limbs into two partials, beyond it keeps three); at most
+ * 256 labels. */
 int csg_set_proposal(csg_engine* eng, int k, float min_margin);
 
 /* Diagnostic: the per-pair correlation matrix [nq x ref_height] of query rows

@@ -376,10 +376,45 @@ def test_phase_timing_modes(eng):
                 assert timed == {"corr"}
             else:
                 assert {"rank", "corr", "select", "label"} <= timed
-        for bad in (-1, 3):
+        for bad in (-1, 4):
             with pytest.raises(engine.CsgError):
                 eng.set_phase_timing(bad)
     finally:
         eng.set_phase_timing(2)
     assert np.array_equal(labels[0], labels[1]) and np.array_equal(labels[0], labels[2])
     eng.destroy_score()
+
+
+def test_finetune_family_timing_mode(eng):
+    """Phase mode 3 launches the device fine-tune's round work eagerly and
+    times its kernel families (rank, correlation, select+filter); the labels
+    and round counts equal the graph replays', and the work counter (cells x
+    reference rows x |G_sub| over all rounds) is filled."""
+    from cusingler_amd import engine, synth
+    ds = synth.generate("small")
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    eng.copyin_score(c0)
+    eng.get_label(0.8, 0.3, "plain", outputs=False)  # wide threshold: several rounds
+    eng.copyin(c1)
+    try:
+        res = {}
+        for mode in (2, 3):
+            _say("fine-tune family timing, mode", mode)
+            eng.set_phase_timing(mode)
+            eng.phase_times(reset=True)
+            final, rounds = eng.finetune(0.8, 0.3, 0)
+            ph = eng.phase_times(reset=True)
+            # (the fine-tune's counters cover its last call)
+            res[mode] = (final, rounds, ph, eng.counters()["ft_pair_genes"])
+    finally:
+        eng.set_phase_timing(2)
+    eng.destroy()
+    eng.destroy_score()
+    assert np.array_equal(res[2][0], res[3][0]) and np.array_equal(res[2][1], res[3][1])
+    ph3 = res[3][2]
+    _say("families (ms):", {k: round(ph3[k][0], 3) for k in ("ft_rank", "ft_corr", "ft_select")},
+         "pair genes", res[3][3])
+    assert ph3["ft_rank"][1] > 0 and ph3["ft_corr"][1] > 0 and ph3["ft_select"][1] > 0
+    assert res[2][2]["ft_corr"][1] == 0
+    assert res[2][3] == res[3][3] > 0

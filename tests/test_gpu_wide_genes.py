@@ -118,3 +118,52 @@ def test_finetune_chunk_tree_staged(eng, gs):
     with 64-gene stages, bit-equal to the host's 128-chunk + tree order."""
     _say("ft chunk tree staged", gs)
     assert eng.selftest_ft(gs) == 0
+
+
+def test_wide_proposal(eng):
+    """The learned proposal on a score gene set above 1,920 genes (the
+    grouped contraction's three-partial 64 x 128 blocks): every scored
+    quantile equals the full exact path's, or one of the two re-scored the
+    pair in the reference's FP32 order, and the labels agree with the
+    oracle's."""
+    import dataclasses
+    from cusingler_amd import engine, synth
+    from oracle import oracle as O
+    cfg = dataclasses.replace(
+        synth.SynthConfig("wide3k", 32 * 6, 600, 16000, 0, 0, 32, 20, 2.0, 6.0, 0.3, seed=3,
+                          label_sigma=1.0, marker_pool=1.0))
+    ds = synth.generate(cfg)
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    orc = O.Oracle(ds, 8)
+    of, om, osc = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
+    ofin, _ = orc.finetune(0, ds.qry_height, om, 0.8, 0.05, 0, O.VAR_SEQ)
+    assert c0.width > 1920
+    _say("wide proposal: train, G_s", c0.width)
+    eng.copyin_score(c0)
+    eng.mlp_init(64, seed=5)
+    eng.mlp_train(ofin.astype(np.int32), epochs=20, batch=128, lr=2e-3, seed=1)
+    _say("wide proposal: full exact path")
+    fe, me, se = eng.get_label(0.8, 0.05, "exact_int8")
+    _say("wide proposal: proposal path")
+    eng.set_proposal(4, 0.2)
+    try:
+        fp, mp, sp = eng.get_label(0.8, 0.05, "exact_int8")
+        cnt = eng.counters()
+    finally:
+        eng.set_proposal(0)
+    eng.copyin(c1)
+    finp, _ = eng.finetune(0.8, 0.05, 0)
+    eng.destroy()
+    eng.destroy_score()
+    _say("wide proposal cells", cnt["prop_cells"], "scored pairs", cnt["prop_scored_pairs"],
+         "of", ds.qry_height * c0.ct_num)
+    assert cnt["prop_cells"] > 0
+    assert cnt["prop_scored_pairs"] < ds.qry_height * c0.ct_num
+    fin = np.isfinite(sp)
+    u, ue, uo = sp.view(np.uint32), se.view(np.uint32), osc.view(np.uint32)
+    odd = fin & (u != ue)
+    # where the two differ, one of them re-scored the pair in FP32 order
+    assert np.all((ue[odd] == uo[odd]) | (u[odd] == uo[odd])), int(odd.sum())
+    assert np.array_equal(fe, of) and np.array_equal(me, om)
+    assert float((fp == of).mean()) >= 0.99 and float((finp == ofin).mean()) >= 0.99
