@@ -73,6 +73,32 @@ def host_cpus():
     return cores, nproc, model
 
 
+class Pinned:
+    """Host arrays in pinned memory (hipHostMalloc), as a loader that reads
+    the query matrix for the GPU would hold them: uploads are direct DMA."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.ptrs = []
+
+    def copy(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        if self.hip.hipHostMalloc(ctypes.byref(p), ctypes.c_size_t(max(a.nbytes, 1)), 0) != 0:
+            raise RuntimeError("hipHostMalloc failed")
+        self.ptrs.append(p)
+        buf = (ctypes.c_char * max(a.nbytes, 1)).from_address(p.value)
+        out = np.frombuffer(buf, dtype=a.dtype, count=a.size).reshape(a.shape)
+        out[...] = a
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipHostFree(p)
+        self.ptrs = []
+
+
 def cpu_baseline(ds, n_sample: int, cores: int):
     """The oracle (a CPU restatement of the reference's score + fine-tune steps)
     on the first n_sample query cells."""
@@ -117,6 +143,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cores", type=int, default=None,
                     help="host threads (default: nproc, capped by OMP_NUM_THREADS)")
+    ap.add_argument("--no-e2e", action="store_true",
+                    help="skip the end-to-end leg (host query CSR -> host labels)")
     ap.add_argument("--only-chosen", action="store_true",
                     help="measure only the chosen form (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -238,9 +266,94 @@ def main():
         return dict(value=Nq * args.steps / dt, ms=dt * 1e3 / args.steps, phases=ph, phases1=ph1,
                     counters=cnt, first=first, final=final)
 
+    def measure_e2e(form, steps, warmup):
+        """Query CSR on the host (float values, pinned) -> first and final labels
+        on the host: the upload and device value coding of the query
+        (csg_copyin_query_raw, both gene sets), the score step, the fine-tune and
+        both label read-backs, per step.  The reference stays loaded (its
+        training, coding and upload are one-time work, outside the window)."""
+        path = "exact_int8" if form == "proposal" else form
+        eng.set_proposal(args.proposal_k if form == "proposal" else 0, args.min_margin)
+        pin = Pinned()
+        qd, qi = pin.copy(ds.qry_data), pin.copy(ds.qry_indices)
+        qp = pin.copy(np.asarray(ds.qry_indptr, np.int32))
+        m0, m1 = prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width)
+
+        def step():
+            eng.copyin_query_raw(qd, qi, qp, ds.qry_width, m0, m1)
+            first, _, _ = eng.get_label(0.8, 0.05, path, outputs="first")
+            final, _ = eng.finetune(0.8, 0.05, 0)
+            return first, final
+
+        for _ in range(warmup):
+            step()
+        eng.set_phase_timing(0)
+        _hip_sync()
+        t0 = time.perf_counter()
+        for i in range(steps):
+            first, final = step()
+        _hip_sync()
+        dt = time.perf_counter() - t0
+        eng.set_phase_timing(2)
+        eng.phase_times(reset=True)
+        step()
+        ph = eng.phase_times(reset=True)
+        pin.free()
+        nnz = int(ds.qry_indptr[-1])
+        up_ms = ph["query_coding"][0]
+        return {"value": round(Nq * steps / dt, 1), "unit": "cells/s", "form": form,
+                "ms_per_step": round(dt * 1e3 / steps, 3), "steps": steps,
+                "query_upload_and_coding_ms": round(up_ms, 3),
+                "query_bytes": 8 * nnz + 4 * (Nq + 1),
+                "upload_and_coding_GBps": round((8 * nnz + 4 * (Nq + 1)) / (up_ms * 1e-3) / 1e9, 1)
+                if up_ms > 0 else None,
+                "labels_equal_device_resident": bool(
+                    np.array_equal(first, res[form]["first"]) and
+                    np.array_equal(final, res[form]["final"])),
+                "window": "float query CSR in pinned host memory -> first and final labels "
+                          "on the host (query H2D + device value coding of both gene sets + "
+                          "score step + fine-tune + label D2H); reference resident"}
+
+    def measure_host_coding(form, steps):
+        """The reference's own structure (Pipeline::score then ::finetune,
+        src/pipeline.cpp:47-111): host value coding of both steps
+        (generateDenseMatrix, on args.cores threads), copyin_score / copyin
+        uploads, score step, fine-tune, labels to the host."""
+        path = "exact_int8" if form == "proposal" else form
+        eng.set_proposal(args.proposal_k if form == "proposal" else 0, args.min_margin)
+        t_code = 0.0
+        _hip_sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a = time.perf_counter()
+            k0 = prep.generate(0)
+            t_code += time.perf_counter() - a
+            eng.copyin_score(k0)
+            eng.get_label(0.8, 0.05, path, outputs="first")
+            a = time.perf_counter()
+            k1 = prep.generate(1)
+            t_code += time.perf_counter() - a
+            eng.copyin(k1)
+            eng.finetune(0.8, 0.05, 0)
+        _hip_sync()
+        dt = time.perf_counter() - t0
+        return {"value": round(Nq * steps / dt, 1), "unit": "cells/s", "form": form,
+                "ms_per_step": round(dt * 1e3 / steps, 3), "steps": steps,
+                "host_coding_ms_per_step": round(t_code * 1e3 / steps, 3),
+                "host_threads": args.cores,
+                "window": "reference structure: host value coding of reference and query for "
+                          "both steps, copyin_score / copyin (reference re-uploaded and "
+                          "re-ranked), score step, fine-tune, labels to the host"}
+
     forms = [chosen] if args.only_chosen else list(dict.fromkeys(
         [chosen, "plain"] + (["exact_int8"] if chosen == "proposal" else [])))
     res = {f: measure(f) for f in forms}
+    e2e = None
+    if not args.no_e2e:
+        print("[bench] end-to-end leg", file=sys.stderr, flush=True)
+        e2e = {f: measure_e2e(f, max(1, args.steps // 2), 1)
+               for f in ([chosen] if args.only_chosen else [chosen, "plain"])}
+        e2e["host_coding_" + chosen] = measure_host_coding(chosen, 2)
     r = res[chosen]
     steps = args.steps
     corr_ms, corr_n = r["phases"]["corr"]
@@ -341,6 +454,7 @@ def main():
         "counters": r["counters"],
         "stages": stages,
         "finetune": ft_line,
+        "end_to_end": e2e,
     }
     if "exact_int8" in res and chosen != "exact_int8":
         e = res["exact_int8"]

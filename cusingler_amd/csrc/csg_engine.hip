@@ -27,6 +27,7 @@
 
 #include "../../include/cusingler_amd.h"
 #include "csg_exact.h"
+#include "csg_code.h"
 #include "csg_ft_kernels.h"
 #include "csg_ftd.h"
 #include "csg_kernels.h"
@@ -85,6 +86,25 @@ struct csg_engine
     bool      ref_ranked = false;
     uint32_t  Qc = 0;
     uint32_t  qc_cap = 0;  // caller's cap on the query chunk (rows), 0: none
+    // query batches value-coded on the device (csg_copyin_query_raw)
+    uint32_t    q_cap = 0, fq_cap = 0;          // query rows the loaded inputs have room for
+    bool        q_ready = false, fq_ready = false;  // query data present
+    int         nbins_ref = 0, fnbins_ref = 0;  // the callers' max_uniq_gene (reference side)
+    size_t      q_nnz_cap = 0, fq_nnz_cap = 0;  // entries d_qdata / d_fqdata hold
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t  code_copied[2] = { nullptr, nullptr }, code_done[2] = { nullptr, nullptr };
+    float*      d_raw_data[2] = { nullptr, nullptr };
+    int32_t*    d_raw_idx[2]  = { nullptr, nullptr };
+    size_t      raw_cap = 0;  // entries of each slice buffer
+    int32_t*    d_raw_ptr = nullptr;
+    int32_t *   d_map_s = nullptr, *d_map_f = nullptr;
+    size_t      map_cap = 0;
+    uint32_t*   d_code_cnt = nullptr;   // kept entries per row, both sets
+    uint32_t*   d_code_over = nullptr;  // rows for the block coding kernel
+    size_t      code_rows_cap = 0;
+    uint8_t*    d_code_misc = nullptr;  // flags, n_over, max_uniq[2] (u32), bases[2] (i64)
+    uint8_t*    h_code_misc = nullptr;  // pinned
+    uint64_t    code_slice = (uint64_t)1 << 26;  // entries per upload slice
     int16_t*  d_Xq = nullptr;
     float*    d_sqq_f = nullptr;
     double*    d_irq   = nullptr;  // the same for the query chunk
@@ -552,7 +572,8 @@ int csg_init_gpu(int gpuid, csg_engine** out)
                                               want) == hipSuccess;
         bool       ok3  = true;
         for (const void* f : { (const void*)k_ftd_rank_ref, (const void*)k_ftd_rank_qry,
-                               (const void*)k_ftd_corr, (const void*)k_ft_rank_ref })
+                               (const void*)k_ftd_corr, (const void*)k_ft_rank_ref,
+                               (const void*)k_code_rows_big })
             ok3 = ok3 && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              want) == hipSuccess;
         if (ok1 && ok2 && ok3)
@@ -697,6 +718,27 @@ void csg_release(csg_engine* eng)
     csg_destroy_score(eng);
     csg_destroy(eng);
     free_mlp(eng);
+    if (eng->copy_stream)
+        (void)hipStreamSynchronize(eng->copy_stream);
+    for (int b = 0; b < 2; ++b)
+    {
+        dfree(eng->d_raw_data[b]);
+        dfree(eng->d_raw_idx[b]);
+        if (eng->code_copied[b])
+            (void)hipEventDestroy(eng->code_copied[b]);
+        if (eng->code_done[b])
+            (void)hipEventDestroy(eng->code_done[b]);
+    }
+    dfree(eng->d_raw_ptr);
+    dfree(eng->d_map_s);
+    dfree(eng->d_map_f);
+    dfree(eng->d_code_cnt);
+    dfree(eng->d_code_over);
+    dfree(eng->d_code_misc);
+    if (eng->h_code_misc)
+        (void)hipHostFree(eng->h_code_misc);
+    if (eng->copy_stream)
+        (void)hipStreamDestroy(eng->copy_stream);
     dfree(eng->d_P);
     dfree(eng->d_first);
     dfree(eng->d_mask);
@@ -729,10 +771,13 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
     int rc = check_ctidx(eng, ref_height, ct_num, ctidx);
     if (rc)
         return rc;
+    // no query data: reserve room for query batches of up to qry_height rows
+    // (csg_copyin_query_raw supplies them)
+    const bool reserve = !qry_data && !qry_indices && !qry_indptr;
     if ((rc = validate_csr(eng, "reference", ref_height, width, ref_data, ref_indices, ref_indptr,
                            max_uniq_gene)) ||
-        (rc = validate_csr(eng, "query", qry_height, width, qry_data, qry_indices, qry_indptr,
-                           max_uniq_gene)))
+        (!reserve && (rc = validate_csr(eng, "query", qry_height, width, qry_data, qry_indices,
+                                        qry_indptr, max_uniq_gene))))
         return rc;
     eng->Nr    = ref_height;
     eng->Nq    = qry_height;
@@ -743,13 +788,18 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
     if (rank_smem(eng->nbins, eng->Kp) > 65536)
         return eng->fail(CSG_ERR_ARGUMENT, "gene set and code range exceed the rank kernel's LDS");
     eng->ctidx.assign(ctidx, ctidx + 2 * ct_num);
-    const size_t rnnz = (size_t)ref_indptr[ref_height], qnnz = (size_t)qry_indptr[qry_height];
+    eng->nbins_ref = (int)max_uniq_gene;
+    eng->q_cap     = qry_height;
+    eng->q_ready   = !reserve;
+    const size_t rnnz = (size_t)ref_indptr[ref_height];
+    const size_t qnnz = reserve ? 0 : (size_t)qry_indptr[qry_height];
+    eng->q_nnz_cap    = qnnz;
     if ((rc = upload(eng, &eng->d_rdata, ref_data, rnnz)) ||
         (rc = upload(eng, &eng->d_rind, ref_indices, rnnz)) ||
         (rc = upload(eng, &eng->d_rptr, ref_indptr, (size_t)ref_height + 1)) ||
-        (rc = upload(eng, &eng->d_qdata, qry_data, qnnz)) ||
-        (rc = upload(eng, &eng->d_qind, qry_indices, qnnz)) ||
-        (rc = upload(eng, &eng->d_qptr, qry_indptr, (size_t)qry_height + 1)))
+        (!reserve && ((rc = upload(eng, &eng->d_qdata, qry_data, qnnz)) ||
+                      (rc = upload(eng, &eng->d_qind, qry_indices, qnnz)) ||
+                      (rc = upload(eng, &eng->d_qptr, qry_indptr, (size_t)qry_height + 1)))))
         return rc;
     std::vector<uint32_t> ls(ct_num), ll(ct_num);
     for (uint32_t l = 0; l < ct_num; ++l)
@@ -1463,6 +1513,9 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
         return CSG_ERR_ARGUMENT;
     if (!eng->score_loaded)
         return eng->fail(CSG_ERR_STATE, "csg_get_label before csg_copyin_score");
+    if (!eng->q_ready)
+        return eng->fail(CSG_ERR_STATE, "no query data: csg_copyin_score reserved room only; "
+                                        "load a batch with csg_copyin_query_raw");
     if (path < CSG_PATH_PLAIN || path > CSG_PATH_FP16_MFMA)
         return eng->fail(CSG_ERR_ARGUMENT, "unknown correlation path");
     const bool prop = eng->prop_k > 0;
@@ -1639,10 +1692,15 @@ int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint1
     if (rc)
         return rc;
     const uint32_t w = std::max<uint32_t>(width, 1);
+    // no query data: reserve room for query batches of up to qry_height rows
+    // (csg_copyin_query_raw supplies them)
+    const bool reserve = !qry_data && !qry_indices && !qry_indptr;
+    if (reserve && labels_mask)
+        return eng->fail(CSG_ERR_ARGUMENT, "a candidate mask needs the query it belongs to");
     if ((rc = validate_csr(eng, "reference", ref_height, w, ref_data, ref_indices, ref_indptr,
                            max_uniq_gene)) ||
-        (rc = validate_csr(eng, "query", qry_height, w, qry_data, qry_indices, qry_indptr,
-                           max_uniq_gene)))
+        (!reserve && (rc = validate_csr(eng, "query", qry_height, w, qry_data, qry_indices,
+                                        qry_indptr, max_uniq_gene))))
         return rc;
     for (uint64_t i = 0; i < (uint64_t)ct_num * ct_num; ++i)
         if ((uint64_t)ctdidx[2 * i] + ctdidx[2 * i + 1] > n_ctdiff)
@@ -1650,11 +1708,14 @@ int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint1
     for (uint64_t i = 0; i < n_ctdiff; ++i)
         if (ctdiff[i] >= w)
             return eng->fail(CSG_ERR_ARGUMENT, "ctdiff gene index outside the fine-tune gene set");
-    eng->fNr    = ref_height;
-    eng->fNq    = qry_height;
-    eng->Gf     = width;
-    eng->fL     = ct_num;
-    eng->fnbins = (int)max_uniq_gene;
+    eng->fNr        = ref_height;
+    eng->fNq        = qry_height;
+    eng->fq_cap     = qry_height;
+    eng->fq_ready   = !reserve;
+    eng->Gf         = width;
+    eng->fL         = ct_num;
+    eng->fnbins     = (int)max_uniq_gene;
+    eng->fnbins_ref = (int)max_uniq_gene;
     eng->fctidx.assign(ctidx, ctidx + 2 * ct_num);
     eng->ctdidx.assign(ctdidx, ctdidx + 2 * (size_t)ct_num * ct_num);
     eng->ctdiff.assign(ctdiff, ctdiff + n_ctdiff);
@@ -1671,15 +1732,17 @@ int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint1
             (rc = upload(eng, &eng->d_fllen, ll.data(), ll.size())))
             return rc;
     }
-    const size_t rnnz = (size_t)ref_indptr[ref_height], qnnz = (size_t)qry_indptr[qry_height];
+    const size_t rnnz = (size_t)ref_indptr[ref_height];
+    const size_t qnnz = reserve ? 0 : (size_t)qry_indptr[qry_height];
+    eng->fq_nnz_cap   = qnnz;
     uint16_t* d_rd = nullptr;
     int *     d_ri = nullptr, *d_rp = nullptr;
     if ((rc = upload(eng, &d_rd, ref_data, rnnz)) || (rc = upload(eng, &d_ri, ref_indices, rnnz)) ||
         (rc = upload(eng, &d_rp, ref_indptr, (size_t)ref_height + 1)) ||
         (rc = dalloc(eng, &eng->d_fref_codes, (size_t)ref_height * w)) ||
-        (rc = upload(eng, &eng->d_fqdata, qry_data, qnnz)) ||
-        (rc = upload(eng, &eng->d_fqind, qry_indices, qnnz)) ||
-        (rc = upload(eng, &eng->d_fqptr, qry_indptr, (size_t)qry_height + 1)))
+        (!reserve && ((rc = upload(eng, &eng->d_fqdata, qry_data, qnnz)) ||
+                      (rc = upload(eng, &eng->d_fqind, qry_indices, qnnz)) ||
+                      (rc = upload(eng, &eng->d_fqptr, qry_indptr, (size_t)qry_height + 1)))))
     {
         dfree(d_rd);
         dfree(d_ri);
@@ -2248,7 +2311,8 @@ int ftd_static(csg_engine* eng)
 {
     if (eng->ftd_static)
         return CSG_OK;
-    const uint32_t Nq = eng->fNq, L = eng->fL;
+    // per-cell state for the largest batch the inputs have room for
+    const uint32_t Nq = std::max(eng->fNq, eng->fq_cap), L = eng->fL;
     int            rc;
     std::vector<uint32_t> cut(L + 1, 0u);
     for (uint32_t k = 1; k <= L; ++k)
@@ -2875,7 +2939,8 @@ std::vector<uintptr_t> ftd_sig_b(const csg_engine* eng, float quantile, float th
              (uintptr_t)eng->d_fl_nan1, (uintptr_t)eng->d_fl_nan2, (uintptr_t)eng->d_fpct,
              (uintptr_t)eng->d_fl_segcell, (uintptr_t)eng->d_fl_refgroup,
              (uintptr_t)eng->d_fl_refrow, (uintptr_t)eng->d_fref_codes8,
-             (uintptr_t)eng->d_fref_codes,
+             (uintptr_t)eng->d_fref_codes, (uintptr_t)eng->d_fqdata, (uintptr_t)eng->d_fqind,
+             (uintptr_t)eng->d_fqptr, (uintptr_t)eng->fnbins,
              (uintptr_t)eng->d_fl_groups, (uintptr_t)eng->d_fl_ggenes, (uintptr_t)ftd_fuse(eng),
              (uintptr_t)eng->ftd_mask, (uintptr_t)q,
              (uintptr_t)t, (uintptr_t)(int64_t)times, (uintptr_t)eng->ftd_grid_rank,
@@ -3335,6 +3400,9 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         return CSG_ERR_ARGUMENT;
     if (!eng->ft_loaded)
         return eng->fail(CSG_ERR_STATE, "csg_finetune before csg_copyin");
+    if (!eng->fq_ready)
+        return eng->fail(CSG_ERR_STATE, "no query data: csg_copyin reserved room only; load a "
+                                        "batch with csg_copyin_query_raw");
     HIPCHK(hipSetDevice(eng->device));
     const uint32_t L  = eng->fL;
     const uint32_t Nq = eng->fNq;
@@ -3498,6 +3566,251 @@ int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches)
         eng->ftd_budget = bytes;
     if (batches)
         *batches = eng->ftd_batches;
+    return CSG_OK;
+}
+
+int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
+                         const float* qry_data, const int32_t* qry_indices,
+                         const int32_t* qry_indptr, const int32_t* map_score,
+                         const int32_t* map_finetune)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (!eng->score_loaded || !eng->ft_loaded)
+        return eng->fail(CSG_ERR_STATE, "csg_copyin_query_raw needs the score and fine-tune "
+                                        "inputs (csg_copyin_score, csg_copyin) loaded first");
+    if (qry_height == 0 || qry_width == 0 || !qry_indptr || !map_score || !map_finetune)
+        return eng->fail(CSG_ERR_ARGUMENT, "empty query batch or missing column maps");
+    if (qry_height > eng->q_cap || qry_height > eng->fq_cap)
+        return eng->fail(CSG_ERR_ARGUMENT, "query batch of " + std::to_string(qry_height) +
+                                               " rows exceeds the " +
+                                               std::to_string(std::min(eng->q_cap, eng->fq_cap)) +
+                                               " rows the loaded inputs have room for");
+    HIPCHK(hipSetDevice(eng->device));
+    // the row pointers and the maps, checked on the host
+    const int64_t base0 = qry_indptr[0];
+    for (uint32_t r = 0; r < qry_height; ++r)
+        if (qry_indptr[r + 1] < qry_indptr[r])
+            return eng->fail(CSG_ERR_ARGUMENT, "query row pointers decrease");
+    const size_t nnz = (size_t)(qry_indptr[qry_height] - base0);
+    if (nnz && (!qry_data || !qry_indices))
+        return eng->fail(CSG_ERR_ARGUMENT, "missing query values or columns");
+    for (uint32_t c = 0; c < qry_width; ++c)
+        if (map_score[c] >= (int32_t)eng->G || map_finetune[c] >= (int32_t)eng->Gf ||
+            map_score[c] < -1 || map_finetune[c] < -1)
+            return eng->fail(CSG_ERR_ARGUMENT, "column map entry outside its gene set");
+    int rc;
+    if (!eng->copy_stream)
+    {
+        HIPCHK(hipStreamCreateWithFlags(&eng->copy_stream, hipStreamNonBlocking));
+        for (int b = 0; b < 2; ++b)
+        {
+            HIPCHK(hipEventCreateWithFlags(&eng->code_copied[b], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&eng->code_done[b], hipEventDisableTiming));
+        }
+        HIPCHK(hipHostMalloc((void**)&eng->h_code_misc, 32, hipHostMallocDefault));
+        if ((rc = dalloc(eng, &eng->d_code_misc, 32)))
+            return rc;
+    }
+    // slices of whole rows of at most code_slice entries (a row longer than
+    // that is a slice of its own)
+    std::vector<uint32_t> cut{ 0 };
+    size_t                biggest = 0;
+    for (uint32_t r = 0; r < qry_height;)
+    {
+        uint32_t e = r + 1;
+        while (e < qry_height &&
+               (uint64_t)(qry_indptr[e + 1] - qry_indptr[r]) <= eng->code_slice)
+            ++e;
+        biggest = std::max<size_t>(biggest, (size_t)(qry_indptr[e] - qry_indptr[r]));
+        cut.push_back(e);
+        r = e;
+    }
+    // work buffers (grow-only; a buffer pair shares its capacity)
+    if (eng->map_cap < qry_width)
+    {
+        dfree(eng->d_map_s);
+        dfree(eng->d_map_f);
+        eng->map_cap = 0;
+        if ((rc = dalloc(eng, &eng->d_map_s, qry_width)) ||
+            (rc = dalloc(eng, &eng->d_map_f, qry_width)))
+            return rc;
+        eng->map_cap = qry_width;
+    }
+    if (eng->code_rows_cap < qry_height)
+    {
+        dfree(eng->d_raw_ptr);
+        dfree(eng->d_code_cnt);
+        dfree(eng->d_code_over);
+        eng->code_rows_cap = 0;
+        if ((rc = dalloc(eng, &eng->d_raw_ptr, (size_t)qry_height + 1)) ||
+            (rc = dalloc(eng, &eng->d_code_cnt, 2 * (size_t)qry_height)) ||
+            (rc = dalloc(eng, &eng->d_code_over, 2 * (size_t)qry_height)))
+            return rc;
+        eng->code_rows_cap = qry_height;
+    }
+    if (eng->raw_cap < biggest || !eng->d_raw_data[0])
+    {
+        for (int b = 0; b < 2; ++b)
+        {
+            dfree(eng->d_raw_data[b]);
+            dfree(eng->d_raw_idx[b]);
+        }
+        eng->raw_cap = 0;
+        const size_t n = std::max<size_t>(biggest, 1);
+        for (int b = 0; b < 2; ++b)
+            if ((rc = dalloc(eng, &eng->d_raw_data[b], n)) ||
+                (rc = dalloc(eng, &eng->d_raw_idx[b], n)))
+                return rc;
+        eng->raw_cap = n;
+    }
+    // coded outputs: at most every entry kept; row pointers for the largest
+    // batch the inputs have room for
+    if (eng->q_nnz_cap < nnz || !eng->d_qdata || !eng->d_qptr)
+    {
+        dfree(eng->d_qdata);
+        dfree(eng->d_qind);
+        dfree(eng->d_qptr);
+        eng->q_nnz_cap = 0;
+        const size_t n = std::max<size_t>(nnz, 1);
+        if ((rc = dalloc(eng, &eng->d_qdata, n)) || (rc = dalloc(eng, &eng->d_qind, n)) ||
+            (rc = dalloc(eng, &eng->d_qptr, (size_t)eng->q_cap + 1)))
+            return rc;
+        eng->q_nnz_cap = n;
+    }
+    if (eng->fq_nnz_cap < nnz || !eng->d_fqdata || !eng->d_fqptr)
+    {
+        dfree(eng->d_fqdata);
+        dfree(eng->d_fqind);
+        dfree(eng->d_fqptr);
+        eng->fq_nnz_cap = 0;
+        const size_t n = std::max<size_t>(nnz, 1);
+        if ((rc = dalloc(eng, &eng->d_fqdata, n)) || (rc = dalloc(eng, &eng->d_fqind, n)) ||
+            (rc = dalloc(eng, &eng->d_fqptr, (size_t)eng->fq_cap + 1)))
+            return rc;
+        eng->fq_nnz_cap = n;
+    }
+    HIPCHK(hipMemcpyAsync(eng->d_map_s, map_score, (size_t)qry_width * 4, hipMemcpyHostToDevice,
+                          eng->stream));
+    HIPCHK(hipMemcpyAsync(eng->d_map_f, map_finetune, (size_t)qry_width * 4,
+                          hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemcpyAsync(eng->d_raw_ptr, qry_indptr, ((size_t)qry_height + 1) * 4,
+                          hipMemcpyHostToDevice, eng->stream));
+    HIPCHK(hipMemsetAsync(eng->d_code_misc, 0, 32, eng->stream));
+    uint32_t* d_flags = (uint32_t*)eng->d_code_misc;
+    uint32_t* d_nover = d_flags + 1;
+    uint32_t* d_maxu  = d_flags + 2;
+    int64_t*  d_base  = (int64_t*)(eng->d_code_misc + 16);
+    uint32_t* cnt_s   = eng->d_code_cnt;
+    uint32_t* cnt_f   = eng->d_code_cnt + qry_height;
+    const int grid    = std::max(1, eng->prop.multiProcessorCount) * 8;
+    const int big_cap = (int)std::min<size_t>(eng->lds_limit / 8, (size_t)1 << 20);
+    // the copy stream waits for the maps and pointers, then the slices
+    // alternate between two buffers: slice i's upload waits for slice i - 2's
+    // coding, its coding for its upload
+    hipEvent_t start = take_event(eng);
+    HIPCHK(hipEventRecord(start, eng->stream));
+    HIPCHK(hipStreamWaitEvent(eng->copy_stream, start, 0));
+    eng->event_pool.push_back(start);
+    {
+        PhaseScope ps(eng, 10, (int)(cut.size() - 1) * 5);
+        for (size_t i = 0; i + 1 < cut.size(); ++i)
+        {
+            const int      b    = (int)(i & 1);
+            const uint32_t r0   = cut[i], r1 = cut[i + 1], nr = r1 - r0;
+            const int64_t  e0   = qry_indptr[r0];
+            const size_t   n    = (size_t)(qry_indptr[r1] - e0);
+            if (i >= 2)
+                HIPCHK(hipStreamWaitEvent(eng->copy_stream, eng->code_done[b], 0));
+            if (n)
+            {
+                HIPCHK(hipMemcpyAsync(eng->d_raw_data[b], qry_data + (e0 - base0), n * 4,
+                                      hipMemcpyHostToDevice, eng->copy_stream));
+                HIPCHK(hipMemcpyAsync(eng->d_raw_idx[b], qry_indices + (e0 - base0), n * 4,
+                                      hipMemcpyHostToDevice, eng->copy_stream));
+            }
+            HIPCHK(hipEventRecord(eng->code_copied[b], eng->copy_stream));
+            HIPCHK(hipStreamWaitEvent(eng->stream, eng->code_copied[b], 0));
+            const CodeSet ss{ eng->d_map_s, eng->d_qptr + r0, eng->d_qdata, eng->d_qind, cnt_s + r0 };
+            const CodeSet sf{ eng->d_map_f, eng->d_fqptr + r0, eng->d_fqdata, eng->d_fqind,
+                              cnt_f + r0 };
+            const unsigned g = (unsigned)std::min<uint32_t>((nr + 3) / 4, (uint32_t)grid);
+            k_code_count<<<g, CODE_NT, 0, eng->stream>>>(eng->d_raw_data[b], eng->d_raw_idx[b],
+                                                         eng->d_raw_ptr + r0, e0, nr, qry_width,
+                                                         eng->d_map_s, eng->d_map_f, cnt_s + r0,
+                                                         cnt_f + r0, d_flags);
+            k_code_scan<<<1, 1024, 0, eng->stream>>>(cnt_s + r0, cnt_f + r0, nr,
+                                                     eng->d_qptr + r0, eng->d_fqptr + r0, d_base);
+            HIPCHK(hipMemsetAsync(d_nover, 0, 4, eng->stream));
+            k_code_rows<<<g, CODE_NT, 0, eng->stream>>>(eng->d_raw_data[b], eng->d_raw_idx[b],
+                                                        eng->d_raw_ptr + r0, e0, nr, qry_width, ss,
+                                                        sf, d_maxu, eng->d_code_over, d_nover,
+                                                        d_flags);
+            k_code_rows_big<<<256, CODE_NT, (size_t)big_cap * 8, eng->stream>>>(
+                eng->d_raw_data[b], eng->d_raw_idx[b], eng->d_raw_ptr + r0, e0, qry_width, ss, sf,
+                eng->d_code_over, d_nover, big_cap, d_maxu, d_flags);
+            LAUNCHCHK();
+            HIPCHK(hipEventRecord(eng->code_done[b], eng->stream));
+        }
+    }
+    HIPCHK(hipMemcpyAsync(eng->h_code_misc, eng->d_code_misc, 32, hipMemcpyDeviceToHost,
+                          eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    collect_phases(eng);
+    uint32_t misc[4];
+    std::memcpy(misc, eng->h_code_misc, 16);
+    // the query is replaced whatever follows: a failed batch leaves none
+    eng->q_ready = eng->fq_ready = false;
+    if (misc[0] & CODE_BADCOL)
+        return eng->fail(CSG_ERR_ARGUMENT, "a query column index lies outside the query width");
+    if (misc[0] & CODE_NAN)
+        return eng->fail(CSG_ERR_UNDEFINED, "NaN query value (the reference's std::set<float> "
+                                            "ordering is undefined for it)");
+    if (misc[0] & CODE_OVERFLOW)
+        return eng->fail(CSG_ERR_UNDEFINED, "a query row holds more distinct values than uint16 "
+                                            "codes (65535) or the block coder's LDS");
+    const int nb = std::max(eng->nbins_ref, (int)misc[2]), fnb = std::max(eng->fnbins_ref, (int)misc[3]);
+    if (nb > 16384 || fnb > 16384 || rank_smem(nb, eng->Kp) > 65536)
+        return eng->fail(CSG_ERR_ARGUMENT, "a query row holds more distinct values than the rank "
+                                           "kernels' code bins");
+    eng->Nq     = qry_height;
+    eng->fNq    = qry_height;
+    eng->nbins  = nb;
+    eng->fnbins = fnb;
+    if (fnb > 256 && eng->d_fref_codes8)
+        dfree(eng->d_fref_codes8);  // byte-wide fine-tune codes no longer hold every code
+    eng->mask_valid = false;
+    eng->host_mask.clear();
+    eng->q_ready = eng->fq_ready = true;
+    return CSG_OK;
+}
+
+int csg_query_coded(csg_engine* eng, int step, uint16_t* data, int32_t* indices, int32_t* indptr)
+{
+    if (!eng || (step != 0 && step != 1) || !indptr)
+        return CSG_ERR_ARGUMENT;
+    const bool ok = step == 0 ? eng->score_loaded && eng->q_ready : eng->ft_loaded && eng->fq_ready;
+    if (!ok)
+        return eng->fail(CSG_ERR_STATE, "no query data of that step on the device");
+    HIPCHK(hipSetDevice(eng->device));
+    const uint32_t n   = step == 0 ? eng->Nq : eng->fNq;
+    const int32_t* ptr = step == 0 ? eng->d_qptr : eng->d_fqptr;
+    HIPCHK(hipMemcpy(indptr, ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+    const size_t nnz = (size_t)(indptr[n] - indptr[0]);
+    if (data && nnz)
+        HIPCHK(hipMemcpy(data, (step == 0 ? eng->d_qdata : eng->d_fqdata) + indptr[0], nnz * 2,
+                         hipMemcpyDeviceToHost));
+    if (indices && nnz)
+        HIPCHK(hipMemcpy(indices, (step == 0 ? eng->d_qind : eng->d_fqind) + indptr[0], nnz * 4,
+                         hipMemcpyDeviceToHost));
+    return CSG_OK;
+}
+
+int csg_set_upload_slice(csg_engine* eng, uint64_t entries)
+{
+    if (!eng || entries == 0)
+        return CSG_ERR_ARGUMENT;
+    eng->code_slice = entries;
     return CSG_OK;
 }
 

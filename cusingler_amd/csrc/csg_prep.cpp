@@ -77,6 +77,7 @@ struct csg_prep
     std::vector<uint32_t> train_values;  // positive-difference genes per pair
     std::vector<uint32_t> common_genes, thre_genes;  // ascending
     bool filtered = false;
+    std::vector<int> qcol;  // query column -> shared column (removeCols), -1: dropped
     // step results
     Coded                 cref, cqry;
     std::vector<uint32_t> ctdidx, ctdiff;
@@ -289,6 +290,14 @@ int csg_prep_train(int cores, uint32_t ref_height, uint32_t ref_width, const flo
         for (uint32_t i = 0; i < n_qry_genes; ++i)
             if (cs.count(qry_genes[i]))
                 qmap[i] = k++;
+    }
+    if (p->filtered)
+        p->qcol = qmap;
+    else
+    {
+        p->qcol.resize(qry_width);
+        for (uint32_t i = 0; i < qry_width; ++i)
+            p->qcol[i] = (int)i;
     }
 
     // ---- loadRefData: groupbyCelltypes + resort (src/io.cpp:616-684)
@@ -537,6 +546,23 @@ uint64_t csg_prep_train_table(const csg_prep* p, int which, uint32_t* out)
 }
 
 int csg_prep_filtered(const csg_prep* p) { return p && p->filtered ? 1 : 0; }
+
+int csg_prep_query_map(const csg_prep* p, int step, int32_t* out)
+{
+    if (!p || !out || (step != 0 && step != 1))
+        return CSG_ERR_ARGUMENT;
+    const std::vector<uint32_t>& gs = step == 0 ? p->common_genes : p->thre_genes;
+    std::vector<int> pos(std::max<size_t>(p->ref_width, p->qry_width) + 1, -1);
+    for (size_t k = 0; k < gs.size(); ++k)
+        if (gs[k] < pos.size())
+            pos[gs[k]] = (int)k;
+    for (uint32_t c = 0; c < p->qry_width; ++c)
+    {
+        const int s = c < p->qcol.size() ? p->qcol[c] : -1;
+        out[c]      = s >= 0 && (size_t)s < pos.size() ? pos[s] : -1;
+    }
+    return CSG_OK;
+}
 
 void csg_prep_release(csg_prep* p) { delete p; }
 

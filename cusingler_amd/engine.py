@@ -21,13 +21,13 @@ from . import _lib
 
 PATHS = {"plain": 0, "exact_int8": 1, "fp32_mfma": 2, "fp16": 3}
 PHASES = ["rank", "corr", "select", "label", "recheck", "finetune", "ft_corr", "ft_rank",
-          "ft_select", "propose"]
+          "ft_select", "propose", "query_coding"]
 COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells", "nan_segments",
             "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes", "ft_redone_cells",
             "ft_host_lists_us", "ft_host_group_us", "ft_host_build_us", "ft_batch_us",
             "prop_cells", "prop_fallback_margin", "prop_fallback_check", "prop_scored_pairs",
             "prop_widened", "prop_correlations", "ft_fused_groups"]
-N_PHASES = 10
+N_PHASES = 11
 N_COUNTERS = 23
 
 _V = C.c_void_p
@@ -79,6 +79,10 @@ def lib():
     L.csg_set_finetune_budget.argtypes = [_V, _U64, _V]
     L.csg_set_finetune_fuse.argtypes = [_V, C.c_int]
     L.csg_set_query_chunk.argtypes = [_V, _U32, _V]
+    L.csg_copyin_query_raw.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _V]
+    L.csg_prep_query_map.argtypes = [_V, C.c_int, _V]
+    L.csg_query_coded.argtypes = [_V, C.c_int, _V, _V, _V]
+    L.csg_set_upload_slice.argtypes = [_V, _U64]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_set_phase_timing.argtypes = [_V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
@@ -198,6 +202,15 @@ class Prep:
     def filtered(self) -> bool:
         return bool(lib().csg_prep_filtered(self._h))
 
+    def query_map(self, step: int, qry_width: int) -> np.ndarray:
+        """Per query column (as given to the training call), its column in
+        step `step`'s gene set or -1 (the device value coder's map)."""
+        out = np.zeros(qry_width, np.int32)
+        rc = lib().csg_prep_query_map(self._h, step, _p(out))
+        if rc != 0:
+            raise CsgError(rc, "query_map")
+        return out
+
 
 class Engine:
     """Device engine (include/cusingler_amd.h)."""
@@ -228,14 +241,62 @@ class Engine:
             _p(c.ctidx), c.max_uniq_gene), "copyin_score")
         self.Nq, self.L, self.Nr = c.qry_height, c.ct_num, c.ref_height
 
+    def reserve(self, c0: Coded, c1: Coded, qry_height: int):
+        """Loads the reference sides of both steps (copyin_score and copyin
+        without query data) with room for query batches of qry_height rows."""
+        self._check(lib().csg_copyin_score(
+            self._h, c0.ref_height, c0.width, _p(c0.ref_data), _p(c0.ref_indices),
+            _p(c0.ref_indptr), qry_height, None, None, None, c0.ct_num, _p(c0.ctidx),
+            c0.max_uniq_gene), "copyin_score (reserve)")
+        self.Nq, self.L, self.Nr = qry_height, c0.ct_num, c0.ref_height
+        self._check(lib().csg_copyin(
+            self._h, c1.ref_height, c1.width, _p(c1.ref_data), _p(c1.ref_indices),
+            _p(c1.ref_indptr), qry_height, None, None, None, c1.ct_num, _p(c1.ctidx),
+            _p(c1.ctdidx), _p(c1.ctdiff), len(c1.ctdiff), c1.max_uniq_gene, None),
+            "copyin (reserve)")
+        self.fNq = qry_height
+
+    def copyin_query_raw(self, data: np.ndarray, indices: np.ndarray, indptr: np.ndarray,
+                         qry_width: int, map_score: np.ndarray, map_finetune: np.ndarray):
+        """A batch of float CSR query rows, value-coded on the device for both
+        steps (replaces the loaded query)."""
+        d = np.ascontiguousarray(data, np.float32)
+        i = np.ascontiguousarray(indices, np.int32)
+        p = np.ascontiguousarray(indptr, np.int32)
+        ms = np.ascontiguousarray(map_score, np.int32)
+        mf = np.ascontiguousarray(map_finetune, np.int32)
+        n = p.shape[0] - 1
+        self._check(lib().csg_copyin_query_raw(self._h, n, qry_width, _p(d), _p(i), _p(p), _p(ms),
+                                               _p(mf)), "copyin_query_raw")
+        self.Nq = self.fNq = n
+
+    def query_coded(self, step: int):
+        """The query's value-coded CSR of a step as it sits on the device:
+        (data uint16, indices int32, indptr int32)."""
+        n = self.Nq if step == 0 else self.fNq
+        ptr = np.zeros(n + 1, np.int32)
+        self._check(lib().csg_query_coded(self._h, step, None, None, _p(ptr)), "query_coded")
+        nnz = int(ptr[-1] - ptr[0])
+        data = np.zeros(max(nnz, 1), np.uint16)
+        ind = np.zeros(max(nnz, 1), np.int32)
+        self._check(lib().csg_query_coded(self._h, step, _p(data), _p(ind), _p(ptr)),
+                    "query_coded")
+        return data[:nnz], ind[:nnz], ptr
+
+    def set_upload_slice(self, entries: int):
+        self._check(lib().csg_set_upload_slice(self._h, int(entries)), "set_upload_slice")
+
     def get_label(self, quantile: float = 0.8, thre: float = 0.05, path: str = "plain",
-                  outputs: bool = True):
+                  outputs=True):
+        """outputs: True (first labels, mask and scores to the host), "first"
+        (first labels only) or False (results stay on the device)."""
         first = np.zeros(self.Nq, np.int32) if outputs else None
-        mask = np.zeros((self.Nq, self.L), np.float32) if outputs else None
-        scores = np.zeros((self.Nq, self.L), np.float32) if outputs else None
+        full = outputs is True
+        mask = np.zeros((self.Nq, self.L), np.float32) if full else None
+        scores = np.zeros((self.Nq, self.L), np.float32) if full else None
         self._check(lib().csg_get_label(
             self._h, quantile, thre, PATHS[path], _p(first) if outputs else None,
-            _p(mask) if outputs else None, _p(scores) if outputs else None), "get_label")
+            _p(mask) if full else None, _p(scores) if full else None), "get_label")
         return first, mask, scores
 
     def destroy_score(self):

@@ -61,7 +61,10 @@ void csg_release(csg_engine* eng);
 /* copyin_score (src/cusingler.cuh:18, src/cusingler.cu:283-413): uploads the
  * score-step inputs, coded over the score gene set (width = |common_genes|).
  * max_uniq_gene is the preprocessing's bound on codes + 1
- * (src/io.cpp:502,520).  ctidx holds 2*ct_num entries. */
+ * (src/io.cpp:502,520).  ctidx holds 2*ct_num entries.  With the three query
+ * pointers NULL only the reference is uploaded and room is reserved for
+ * query batches of up to qry_height rows, which csg_copyin_query_raw
+ * supplies (max_uniq_gene then bounds the reference's codes). */
 int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width,
                      const uint16_t* ref_data, const int32_t* ref_indices,
                      const int32_t* ref_indptr, uint32_t qry_height, const uint16_t* qry_data,
@@ -87,7 +90,9 @@ int csg_destroy_score(csg_engine* eng);
  * and the per ordered label pair gene windows: ctdidx holds [pos, len] at
  * (i*ct_num + j)*2 into ctdiff (gene indices in the fine-tune space).
  * labels_mask (qry_height*ct_num floats) may be NULL to use the mask the last
- * csg_get_label left on the device. */
+ * csg_get_label left on the device.  With the three query pointers NULL
+ * (and no mask) room is reserved for query batches of up to qry_height rows,
+ * as in csg_copyin_score. */
 int csg_copyin(csg_engine* eng, uint32_t ref_height, uint32_t width, const uint16_t* ref_data,
                const int32_t* ref_indices, const int32_t* ref_indptr, uint32_t qry_height,
                const uint16_t* qry_data, const int32_t* qry_indices, const int32_t* qry_indptr,
@@ -145,6 +150,42 @@ int csg_finetune_tuning(csg_engine* eng, int autotune, int32_t* grids, float* be
  * reference. */
 int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches);
 
+/* Loads a batch of query cells as the reference's pipeline receives them --
+ * float CSR values over the query's own gene list (after log-normalising,
+ * src/pipeline.cpp:128-152) -- in place of the loaded query, for both the
+ * score step and the fine-tune, and value-codes it on the device: csrFilter
+ * (src/io.cpp:470-523) over the score gene set and over the fine-tune gene
+ * set at once, bit-identical to the host's codes.  The values and columns
+ * travel in slices on a second stream (pinned host memory makes that a
+ * direct DMA), each slice coded while the next one is in flight.
+ *   qry_indptr    : qry_height + 1 row pointers into qry_data / qry_indices
+ *   map_score     : qry_width entries, csg_prep_query_map(step 0)
+ *   map_finetune  : qry_width entries, csg_prep_query_map(step 1)
+ * Needs csg_copyin_score and csg_copyin loaded (with data, or reserving
+ * qry_height rows or more).  The batch's codes raise max_uniq_gene of each
+ * step above the reference's where a cell needs it.  Then csg_get_label and
+ * csg_finetune annotate the batch; the candidate mask comes from
+ * csg_get_label.  CSG_ERR_UNDEFINED for a NaN value or more than 65535
+ * distinct values in a row (the reference's uint16 codes).  The host arrays
+ * are read before the call returns.  No counterpart in the reference: its
+ * DataParser codes every query row on the host before copyin_score. */
+int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
+                         const float* qry_data, const int32_t* qry_indices,
+                         const int32_t* qry_indptr, const int32_t* map_score,
+                         const int32_t* map_finetune);
+
+/* Diagnostic: copies the query's value-coded CSR of step 0 (score gene set)
+ * or 1 (fine-tune gene set) from the device -- what csg_copyin_query_raw
+ * produced, or what csg_copyin_score / csg_copyin received.  indptr receives
+ * rows + 1 row pointers; data / indices (may be NULL) indptr[rows] -
+ * indptr[0] entries.  No counterpart in the reference. */
+int csg_query_coded(csg_engine* eng, int step, uint16_t* data, int32_t* indices, int32_t* indptr);
+
+/* Entries per upload slice of csg_copyin_query_raw (default 2^26; whole rows
+ * per slice, a longer row is a slice of its own).  No counterpart in the
+ * reference. */
+int csg_set_upload_slice(csg_engine* eng, uint64_t entries);
+
 /* Caps the query chunk of the score step (src/cusingler.cu:335-409 slices the
  * query to the free device memory; here a chunk is at most 8 GiB of scores
  * and 1/16 of the free memory).  rows: a multiple of 128, 0 removes the cap;
@@ -161,9 +202,10 @@ int csg_destroy(csg_engine* eng);
  * [2] quantile select, [3] label/mask, [4] certification recheck,
  * [5] fine-tune total, [6] fine-tune correlation, [7] fine-tune rank,
  * [8] fine-tune select+filter, [9] learned proposal (network inference and
- * work lists).  counts[i] is the number of launches timed in phase i.  Either
- * pointer may be NULL. */
-#define CSG_N_PHASES 10
+ * work lists), [10] query upload and value coding (csg_copyin_query_raw).
+ * counts[i] is the number of launches timed in phase i.  Either pointer may
+ * be NULL. */
+#define CSG_N_PHASES 11
 int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset);
 
 /* Which phases csg_phase_times times (no counterpart in the reference: an

@@ -66,6 +66,17 @@ uint64_t csg_prep_train_table(const csg_prep* p, int which, uint32_t* out);
 /* 1 if the gene lists differ and columns were remapped (filter_genes). */
 int csg_prep_filtered(const csg_prep* p);
 
+/* Column map of the query for value coding on the device
+ * (csg_copyin_query_raw): for every column c of the query as given to
+ * csg_prep_train (qry_width entries), the column of step `step`'s gene set
+ * (0: common_genes, 1: thre_genes) that csrFilter (src/io.cpp:470-523)
+ * writes for it, or -1 where the column is dropped (not shared with the
+ * reference, src/io.cpp:306-349,525-538, or outside the gene set).  The
+ * composition of removeCols' renumbering and generateDenseMatrix's index map
+ * (src/io.cpp:714-721); it depends only on the gene lists, so it codes any
+ * query batch over the same genes.  Returns 0 or CSG_ERR_ARGUMENT. */
+int csg_prep_query_map(const csg_prep* p, int step, int32_t* out);
+
 void csg_prep_release(csg_prep* p);
 
 #ifdef __cplusplus

@@ -1,0 +1,191 @@
+"""Value coding of the query on the device (csg_copyin_query_raw): the coded
+CSR of both steps bit-identical to the reference's own io.cpp outputs (golden
+fixtures) and to the host preprocessing on larger data (filtered genes,
+several upload slices, 16-bit codes, rows coded by the block kernel), the
+labels of the annotation that follows against the oracle, batches loaded into
+reserved room, and the error cases.
+
+Every case prints its name before touching the GPU."""
+
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from cusingler_amd import engine
+    e = engine.Engine(0)
+    yield e
+    e.close()
+
+
+def _fixture_dataset(g):
+    from cusingler_amd.synth import Dataset
+    return Dataset(cfg=None, ref_height=int(g["ref_height"]), ref_width=int(g["ref_width"]),
+                   ref_data=g["ref_data"], ref_indices=g["ref_indices"], ref_indptr=g["ref_indptr"],
+                   qry_height=int(g["qry_height"]), qry_width=int(g["qry_width"]),
+                   qry_data=g["qry_data"], qry_indices=g["qry_indices"], qry_indptr=g["qry_indptr"],
+                   codes=list(g["codes"]), celltype=g["celltype"], cellnames=list(g["cellnames"]),
+                   ref_genes=list(g["ref_genes"]), qry_genes=list(g["qry_genes"]),
+                   qry_truth=np.zeros(0, np.int32))
+
+
+def _load_raw(eng, prep, ds, rows=None):
+    """The query rows [0, rows) of ds as a raw float CSR batch."""
+    n = ds.qry_height if rows is None else rows
+    ptr = np.asarray(ds.qry_indptr[:n + 1], np.int64)
+    eng.copyin_query_raw(ds.qry_data[ptr[0]:ptr[-1]], ds.qry_indices[ptr[0]:ptr[-1]],
+                         (ptr - ptr[0]).astype(np.int32), ds.qry_width,
+                         prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width))
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "*.npz"))))
+def test_device_coding_matches_reference_io(eng, path):
+    from cusingler_amd import engine
+    g = np.load(path)
+    ds = _fixture_dataset(g)
+    name = os.path.basename(path)
+    prep = engine.Prep(ds, 4)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    _say("device coding vs io.cpp", name)
+    eng.copyin_score(c0)
+    eng.copyin(c1)
+    _load_raw(eng, prep, ds)
+    for step in (0, 1):
+        d, i, p = eng.query_coded(step)
+        assert np.array_equal(d, g[f"step{step}_qry_data"]), (name, step)
+        assert np.array_equal(i, g[f"step{step}_qry_indices"]), (name, step)
+        assert np.array_equal(p, g[f"step{step}_qry_indptr"]), (name, step)
+    first, mask, scores = eng.get_label(0.8, 0.05, "plain")
+    final, rounds = eng.finetune(0.8, 0.05, 0)
+    eng.destroy()
+    eng.destroy_score()
+    assert np.array_equal(first, g["first"]) and np.array_equal(mask, g["mask"])
+    assert_bits_equal(scores, g["scores"], f"{name} scores")
+    assert np.array_equal(final, g["final"]) and np.array_equal(rounds, g["rounds"])
+
+
+def _data(name):
+    from cusingler_amd import synth
+    if name == "small":
+        return synth.generate("small", n_qry=1000)
+    if name == "pancreas_1k":
+        return synth.generate("pancreas", n_qry=1000)
+    # dense rows, 16-bit codes: over 1,024 kept fine-tune entries per row
+    return synth.generate(synth.SynthConfig("wide4k", 40 * 5, 60, 20000, 0, 0, 40, 20, 2.0, 6.0,
+                                            0.4, seed=3, label_sigma=2.0, marker_pool=1.0))
+
+
+@pytest.mark.parametrize("name,slice_", [("small", 1 << 26), ("small", 20000),
+                                         ("pancreas_1k", 50000), ("wide4k", 1 << 26)])
+def test_device_coding_matches_host_and_oracle(eng, name, slice_):
+    from cusingler_amd import engine
+    from oracle import oracle as O
+    ds = _data(name)
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    _say("device coding", name, "slice", slice_, "rows", ds.qry_height, "nnz",
+         int(ds.qry_indptr[-1]))
+    eng.copyin_score(c0)
+    eng.copyin(c1)
+    eng.set_upload_slice(slice_)
+    try:
+        _load_raw(eng, prep, ds)
+    finally:
+        eng.set_upload_slice(1 << 26)
+    for step, c in ((0, c0), (1, c1)):
+        d, i, p = eng.query_coded(step)
+        assert np.array_equal(p, c.qry_indptr), (name, step)
+        assert np.array_equal(i, c.qry_indices), (name, step)
+        assert np.array_equal(d, c.qry_data), (name, step)
+    kept = np.diff(c1.qry_indptr).max()
+    first, mask, scores = eng.get_label(0.8, 0.05, "exact_int8")
+    final, rounds = eng.finetune(0.8, 0.05, 0)
+    eng.destroy()
+    eng.destroy_score()
+    orc = O.Oracle(ds, 8)
+    of, om, _ = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
+    ofin, ornd = orc.finetune(0, ds.qry_height, om, 0.8, 0.05, 0, O.VAR_SEQ)
+    _say("  max kept fine-tune entries per row", int(kept))
+    assert np.array_equal(first, of) and np.array_equal(mask, om)
+    assert np.array_equal(final, ofin) and np.array_equal(rounds, ornd)
+
+
+def test_batches_into_reserved_room(eng):
+    """The reference loaded once with room for 1,000 query rows; two batches
+    (rows 0-699, then 700-999) each give the oracle's labels of their rows."""
+    from cusingler_amd import engine
+    from oracle import oracle as O
+    ds = _data("small")
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    orc = O.Oracle(ds, 8)
+    of, om, osc = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
+    ofin, _ = orc.finetune(0, ds.qry_height, om, 0.8, 0.05, 0, O.VAR_SEQ)
+    _say("reserved room: reference only")
+    eng.reserve(c0, c1, ds.qry_height)
+    with pytest.raises(engine.CsgError):
+        eng.get_label(0.8, 0.05, "plain")  # no query data yet
+    ptr = np.asarray(ds.qry_indptr, np.int64)
+    for q0, q1 in ((0, 700), (700, 1000)):
+        _say("reserved room: batch", q0, q1)
+        sub = ptr[q0:q1 + 1]
+        eng.copyin_query_raw(ds.qry_data[sub[0]:sub[-1]], ds.qry_indices[sub[0]:sub[-1]],
+                             (sub - sub[0]).astype(np.int32), ds.qry_width,
+                             prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width))
+        first, mask, scores = eng.get_label(0.8, 0.05, "plain")
+        final, _ = eng.finetune(0.8, 0.05, 0)
+        assert np.array_equal(first, of[q0:q1]) and np.array_equal(mask, om[q0:q1])
+        assert_bits_equal(scores, osc[q0:q1], f"batch {q0}-{q1}")
+        assert np.array_equal(final, ofin[q0:q1])
+    eng.destroy()
+    eng.destroy_score()
+
+
+def test_raw_batch_errors(eng):
+    from cusingler_amd import engine
+    ds = _data("small")
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    eng.reserve(c0, c1, 500)
+    m0, m1 = prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width)
+    ptr = np.asarray(ds.qry_indptr[:301], np.int64)
+    data = ds.qry_data[:ptr[-1]].copy()
+    ind = ds.qry_indices[:ptr[-1]].copy()
+    p32 = ptr.astype(np.int32)
+    _say("raw batch errors")
+    with pytest.raises(engine.CsgError) as e:      # more rows than reserved
+        eng.copyin_query_raw(ds.qry_data, ds.qry_indices, ds.qry_indptr.astype(np.int32),
+                             ds.qry_width, m0, m1)
+    assert e.value.status == 1
+    bad = data.copy()
+    bad[17] = np.nan
+    with pytest.raises(engine.CsgError) as e:      # NaN: undefined in the reference
+        eng.copyin_query_raw(bad, ind, p32, ds.qry_width, m0, m1)
+    assert e.value.status == 5
+    with pytest.raises(engine.CsgError):           # the failed batch leaves no query
+        eng.get_label(0.8, 0.05, "plain")
+    badi = ind.copy()
+    badi[5] = ds.qry_width + 3
+    with pytest.raises(engine.CsgError) as e:      # column outside the query width
+        eng.copyin_query_raw(data, badi, p32, ds.qry_width, m0, m1)
+    assert e.value.status == 1
+    eng.copyin_query_raw(data, ind, p32, ds.qry_width, m0, m1)  # a good batch after them
+    first, _, _ = eng.get_label(0.8, 0.05, "plain")
+    assert first.shape == (300,)
+    eng.destroy()
+    eng.destroy_score()
