@@ -345,4 +345,30 @@ __global__ __launch_bounds__(CODE_NT) void k_code_rows_big(
     }
 }
 
+// Clears the coding passes' counters: misc = [flags, n_over, max_uniq[2]
+// (u32), bases[2] (i64)], 32 bytes.
+__global__ void k_code_init(uint32_t* __restrict__ misc)
+{
+    if (threadIdx.x < 8)
+        misc[threadIdx.x] = 0u;
+}
+
+__global__ void k_code_zero(uint32_t* __restrict__ v)
+{
+    if (threadIdx.x == 0)
+        *v = 0u;
+}
+
+// Copies the flags and both max_uniq values out of misc (status[3] = 0).
+__global__ void k_code_status(const uint32_t* __restrict__ misc, uint32_t* __restrict__ status)
+{
+    if (threadIdx.x == 0)
+    {
+        status[0] = misc[0];
+        status[1] = misc[2];
+        status[2] = misc[3];
+        status[3] = 0u;
+    }
+}
+
 }  // namespace csg

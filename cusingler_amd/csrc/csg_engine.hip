@@ -3569,6 +3569,52 @@ int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches)
     return CSG_OK;
 }
 
+namespace {
+
+// The coding passes of rows [0, nr) of a device CSR (row r's entries at
+// data/idx [ptr[r] - base, ptr[r + 1] - base)) into sets a and b, on `st`.
+// Scratch: cnt_a / cnt_b (nr each), over (2 nr), misc (32 bytes, cleared by
+// the caller; its running bases continue from earlier slices).  No
+// allocation, no synchronisation.
+void launch_coding(const csg_engine* eng, hipStream_t st, const float* data, const int32_t* idx,
+                   const int32_t* ptr, int64_t base, uint32_t nr, uint32_t width,
+                   const CodeSet& sa, const CodeSet& sb, uint32_t* over, uint8_t* misc)
+{
+    uint32_t* d_flags = (uint32_t*)misc;
+    uint32_t* d_nover = d_flags + 1;
+    uint32_t* d_maxu  = d_flags + 2;
+    int64_t*  d_base  = (int64_t*)(misc + 16);
+    const int grid    = std::max(1, eng->prop.multiProcessorCount) * 8;
+    const int big_cap = (int)std::min<size_t>(eng->lds_limit / 8, (size_t)1 << 20);
+    const unsigned g  = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nr + 3) / 4, (uint32_t)grid));
+    k_code_count<<<g, CODE_NT, 0, st>>>(data, idx, ptr, base, nr, width, sa.map, sb.map, sa.count,
+                                        sb.count, d_flags);
+    k_code_scan<<<1, 1024, 0, st>>>(sa.count, sb.count, nr, (int32_t*)sa.ptr, (int32_t*)sb.ptr,
+                                    d_base);
+    k_code_zero<<<1, 64, 0, st>>>(d_nover);
+    k_code_rows<<<g, CODE_NT, 0, st>>>(data, idx, ptr, base, nr, width, sa, sb, d_maxu, over,
+                                       d_nover, d_flags);
+    k_code_rows_big<<<256, CODE_NT, (size_t)big_cap * 8, st>>>(data, idx, ptr, base, width, sa, sb,
+                                                               over, d_nover, big_cap, d_maxu,
+                                                               d_flags);
+}
+
+// Status of a finished coding (misc read back): CSG_OK or the error.
+int coding_status(csg_engine* eng, const uint32_t* misc)
+{
+    if (misc[0] & CODE_BADCOL)
+        return eng->fail(CSG_ERR_ARGUMENT, "a query column index lies outside the query width");
+    if (misc[0] & CODE_NAN)
+        return eng->fail(CSG_ERR_UNDEFINED, "NaN query value (the reference's std::set<float> "
+                                            "ordering is undefined for it)");
+    if (misc[0] & CODE_OVERFLOW)
+        return eng->fail(CSG_ERR_UNDEFINED, "a query row holds more distinct values than uint16 "
+                                            "codes (65535) or the block coder's LDS");
+    return CSG_OK;
+}
+
+}  // namespace
+
 int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
                          const float* qry_data, const int32_t* qry_indices,
                          const int32_t* qry_indptr, const int32_t* map_score,
@@ -3696,15 +3742,9 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
                           hipMemcpyHostToDevice, eng->stream));
     HIPCHK(hipMemcpyAsync(eng->d_raw_ptr, qry_indptr, ((size_t)qry_height + 1) * 4,
                           hipMemcpyHostToDevice, eng->stream));
-    HIPCHK(hipMemsetAsync(eng->d_code_misc, 0, 32, eng->stream));
-    uint32_t* d_flags = (uint32_t*)eng->d_code_misc;
-    uint32_t* d_nover = d_flags + 1;
-    uint32_t* d_maxu  = d_flags + 2;
-    int64_t*  d_base  = (int64_t*)(eng->d_code_misc + 16);
+    k_code_init<<<1, 64, 0, eng->stream>>>((uint32_t*)eng->d_code_misc);
     uint32_t* cnt_s   = eng->d_code_cnt;
     uint32_t* cnt_f   = eng->d_code_cnt + qry_height;
-    const int grid    = std::max(1, eng->prop.multiProcessorCount) * 8;
-    const int big_cap = (int)std::min<size_t>(eng->lds_limit / 8, (size_t)1 << 20);
     // the copy stream waits for the maps and pointers, then the slices
     // alternate between two buffers: slice i's upload waits for slice i - 2's
     // coding, its coding for its upload
@@ -3734,21 +3774,9 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
             const CodeSet ss{ eng->d_map_s, eng->d_qptr + r0, eng->d_qdata, eng->d_qind, cnt_s + r0 };
             const CodeSet sf{ eng->d_map_f, eng->d_fqptr + r0, eng->d_fqdata, eng->d_fqind,
                               cnt_f + r0 };
-            const unsigned g = (unsigned)std::min<uint32_t>((nr + 3) / 4, (uint32_t)grid);
-            k_code_count<<<g, CODE_NT, 0, eng->stream>>>(eng->d_raw_data[b], eng->d_raw_idx[b],
-                                                         eng->d_raw_ptr + r0, e0, nr, qry_width,
-                                                         eng->d_map_s, eng->d_map_f, cnt_s + r0,
-                                                         cnt_f + r0, d_flags);
-            k_code_scan<<<1, 1024, 0, eng->stream>>>(cnt_s + r0, cnt_f + r0, nr,
-                                                     eng->d_qptr + r0, eng->d_fqptr + r0, d_base);
-            HIPCHK(hipMemsetAsync(d_nover, 0, 4, eng->stream));
-            k_code_rows<<<g, CODE_NT, 0, eng->stream>>>(eng->d_raw_data[b], eng->d_raw_idx[b],
-                                                        eng->d_raw_ptr + r0, e0, nr, qry_width, ss,
-                                                        sf, d_maxu, eng->d_code_over, d_nover,
-                                                        d_flags);
-            k_code_rows_big<<<256, CODE_NT, (size_t)big_cap * 8, eng->stream>>>(
-                eng->d_raw_data[b], eng->d_raw_idx[b], eng->d_raw_ptr + r0, e0, qry_width, ss, sf,
-                eng->d_code_over, d_nover, big_cap, d_maxu, d_flags);
+            launch_coding(eng, eng->stream, eng->d_raw_data[b], eng->d_raw_idx[b],
+                          eng->d_raw_ptr + r0, e0, nr, qry_width, ss, sf, eng->d_code_over,
+                          eng->d_code_misc);
             LAUNCHCHK();
             HIPCHK(hipEventRecord(eng->code_done[b], eng->stream));
         }
@@ -3761,14 +3789,8 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
     std::memcpy(misc, eng->h_code_misc, 16);
     // the query is replaced whatever follows: a failed batch leaves none
     eng->q_ready = eng->fq_ready = false;
-    if (misc[0] & CODE_BADCOL)
-        return eng->fail(CSG_ERR_ARGUMENT, "a query column index lies outside the query width");
-    if (misc[0] & CODE_NAN)
-        return eng->fail(CSG_ERR_UNDEFINED, "NaN query value (the reference's std::set<float> "
-                                            "ordering is undefined for it)");
-    if (misc[0] & CODE_OVERFLOW)
-        return eng->fail(CSG_ERR_UNDEFINED, "a query row holds more distinct values than uint16 "
-                                            "codes (65535) or the block coder's LDS");
+    if ((rc = coding_status(eng, misc)))
+        return rc;
     const int nb = std::max(eng->nbins_ref, (int)misc[2]), fnb = std::max(eng->fnbins_ref, (int)misc[3]);
     if (nb > 16384 || fnb > 16384 || rank_smem(nb, eng->Kp) > 65536)
         return eng->fail(CSG_ERR_ARGUMENT, "a query row holds more distinct values than the rank "
@@ -3782,6 +3804,51 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
     eng->mask_valid = false;
     eng->host_mask.clear();
     eng->q_ready = eng->fq_ready = true;
+    return CSG_OK;
+}
+
+int csg_code_csr(csg_engine* eng, uint32_t rows, uint32_t width, const float* data,
+                 const int32_t* indices, const int32_t* indptr, const int32_t* map_a,
+                 const int32_t* map_b, uint16_t* data_a, int32_t* indices_a, int32_t* indptr_a,
+                 uint16_t* data_b, int32_t* indices_b, int32_t* indptr_b, uint32_t* max_uniq)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (rows == 0 || width == 0 || !data || !indices || !indptr || !map_a || !map_b || !data_a ||
+        !indices_a || !indptr_a || !data_b || !indices_b || !indptr_b)
+        return eng->fail(CSG_ERR_ARGUMENT, "empty CSR or missing device pointer");
+    HIPCHK(hipSetDevice(eng->device));
+    int rc;
+    if (eng->code_rows_cap < rows || !eng->d_code_cnt)
+    {
+        dfree(eng->d_raw_ptr);
+        dfree(eng->d_code_cnt);
+        dfree(eng->d_code_over);
+        eng->code_rows_cap = 0;
+        if ((rc = dalloc(eng, &eng->d_raw_ptr, (size_t)rows + 1)) ||
+            (rc = dalloc(eng, &eng->d_code_cnt, 2 * (size_t)rows)) ||
+            (rc = dalloc(eng, &eng->d_code_over, 2 * (size_t)rows)))
+            return rc;
+        eng->code_rows_cap = rows;
+    }
+    if (!eng->d_code_misc && (rc = dalloc(eng, &eng->d_code_misc, 32)))
+        return rc;
+    const CodeSet sa{ map_a, indptr_a, data_a, indices_a, eng->d_code_cnt };
+    const CodeSet sb{ map_b, indptr_b, data_b, indices_b, eng->d_code_cnt + rows };
+    k_code_init<<<1, 64, 0, eng->stream>>>((uint32_t*)eng->d_code_misc);
+    launch_coding(eng, eng->stream, data, indices, indptr, 0, rows, width, sa, sb,
+                  eng->d_code_over, eng->d_code_misc);
+    LAUNCHCHK();
+    uint32_t misc[4];
+    HIPCHK(hipMemcpyAsync(misc, eng->d_code_misc, 16, hipMemcpyDeviceToHost, eng->stream));
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    if ((rc = coding_status(eng, misc)))
+        return rc;
+    if (max_uniq)
+    {
+        max_uniq[0] = misc[2];
+        max_uniq[1] = misc[3];
+    }
     return CSG_OK;
 }
 

@@ -83,6 +83,7 @@ def lib():
     L.csg_prep_query_map.argtypes = [_V, C.c_int, _V]
     L.csg_query_coded.argtypes = [_V, C.c_int, _V, _V, _V]
     L.csg_set_upload_slice.argtypes = [_V, _U64]
+    L.csg_code_csr.argtypes = [_V, _U32, _U32] + [_V] * 12
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_set_phase_timing.argtypes = [_V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
@@ -282,6 +283,16 @@ class Engine:
         self._check(lib().csg_query_coded(self._h, step, _p(data), _p(ind), _p(ptr)),
                     "query_coded")
         return data[:nnz], ind[:nnz], ptr
+
+    def code_csr(self, rows: int, width: int, data, indices, indptr, map_a, map_b, out_a,
+                 out_b):
+        """Value coding of a device CSR (device pointers as ints): out_x =
+        (data, indices, indptr) device pointers of set x; returns max_uniq."""
+        mu = np.zeros(2, np.uint32)
+        self._check(lib().csg_code_csr(self._h, rows, width, data, indices, indptr, map_a, map_b,
+                                       out_a[0], out_a[1], out_a[2], out_b[0], out_b[1],
+                                       out_b[2], _p(mu)), "code_csr")
+        return int(mu[0]), int(mu[1])
 
     def set_upload_slice(self, entries: int):
         self._check(lib().csg_set_upload_slice(self._h, int(entries)), "set_upload_slice")

@@ -174,6 +174,28 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
                          const int32_t* qry_indptr, const int32_t* map_score,
                          const int32_t* map_finetune);
 
+/* Value coding of a float CSR that is already on the device (csrFilter,
+ * src/io.cpp:470-523), for two gene sets a and b at once -- the device half of
+ * csg_copyin_query_raw, for callers whose cells arrive on the GPU.  All
+ * pointers but max_uniq are device pointers:
+ *   data, indices, indptr : rows x width float CSR (row r's entries at
+ *                           [indptr[r], indptr[r + 1]))
+ *   map_a, map_b          : width entries each, column -> gene-set column or
+ *                           -1 (csg_prep_query_map)
+ *   data_x, indices_x     : the coded CSR of set x (room for indptr[rows]
+ *                           entries); indptr_x : rows + 1 row pointers
+ *   max_uniq (host, 2)    : max over rows of (#distinct kept values + 1) per
+ *                           set (may be NULL)
+ * Runs on the engine's stream and returns when the codes are written;
+ * CSG_ERR_UNDEFINED for a NaN value or a row of more than 65535 distinct
+ * values.  Work buffers (2 x rows counts and row lists) are the engine's,
+ * grown on demand.  No counterpart in the reference (its coding runs on the
+ * host, src/io.cpp:714-784). */
+int csg_code_csr(csg_engine* eng, uint32_t rows, uint32_t width, const float* data,
+                 const int32_t* indices, const int32_t* indptr, const int32_t* map_a,
+                 const int32_t* map_b, uint16_t* data_a, int32_t* indices_a, int32_t* indptr_a,
+                 uint16_t* data_b, int32_t* indices_b, int32_t* indptr_b, uint32_t* max_uniq);
+
 /* Diagnostic: copies the query's value-coded CSR of step 0 (score gene set)
  * or 1 (fine-tune gene set) from the device -- what csg_copyin_query_raw
  * produced, or what csg_copyin_score / csg_copyin received.  indptr receives

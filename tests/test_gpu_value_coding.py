@@ -189,3 +189,51 @@ def test_raw_batch_errors(eng):
     assert first.shape == (300,)
     eng.destroy()
     eng.destroy_score()
+
+
+def _device_coding_inputs(ds, prep):
+    import torch
+    dev = torch.device("cuda:0")
+    t = {
+        "data": torch.from_numpy(np.ascontiguousarray(ds.qry_data, np.float32)).to(dev),
+        "indices": torch.from_numpy(np.ascontiguousarray(ds.qry_indices, np.int32)).to(dev),
+        "indptr": torch.from_numpy(np.ascontiguousarray(ds.qry_indptr, np.int32)).to(dev),
+        "map_a": torch.from_numpy(prep.query_map(0, ds.qry_width)).to(dev),
+        "map_b": torch.from_numpy(prep.query_map(1, ds.qry_width)).to(dev),
+    }
+    nnz = int(ds.qry_indptr[-1])
+    for x in ("a", "b"):
+        t["data_" + x] = torch.zeros(max(nnz, 1), dtype=torch.int16, device=dev)
+        t["indices_" + x] = torch.zeros(max(nnz, 1), dtype=torch.int32, device=dev)
+        t["indptr_" + x] = torch.zeros(ds.qry_height + 1, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    return t
+
+
+def _coded_from_device(t, x):
+    ptr = t["indptr_" + x].cpu().numpy()
+    n = int(ptr[-1])
+    return (t["data_" + x][:n].cpu().numpy().view(np.uint16), t["indices_" + x][:n].cpu().numpy(),
+            ptr)
+
+
+@pytest.mark.parametrize("name", ["small", "pancreas_1k", "wide4k"])
+def test_code_csr_device_pointers(eng, name):
+    """csg_code_csr on a CSR already on the device: both sets' codes equal the
+    host preprocessing's, max_uniq_gene covers them."""
+    from cusingler_amd import engine
+    ds = _data(name)
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    t = _device_coding_inputs(ds, prep)
+    _say("code_csr device pointers", name)
+    mu = eng.code_csr(ds.qry_height, ds.qry_width, t["data"].data_ptr(), t["indices"].data_ptr(),
+                      t["indptr"].data_ptr(), t["map_a"].data_ptr(), t["map_b"].data_ptr(),
+                      (t["data_a"].data_ptr(), t["indices_a"].data_ptr(), t["indptr_a"].data_ptr()),
+                      (t["data_b"].data_ptr(), t["indices_b"].data_ptr(), t["indptr_b"].data_ptr()))
+    for x, c, m in (("a", c0, mu[0]), ("b", c1, mu[1])):
+        d, i, p = _coded_from_device(t, x)
+        assert np.array_equal(p, c.qry_indptr) and np.array_equal(i, c.qry_indices), (name, x)
+        assert np.array_equal(d, c.qry_data), (name, x)
+        assert m <= c.max_uniq_gene and m == max(
+            len(np.unique(d[p[r]:p[r + 1]])) + 1 for r in range(len(p) - 1))
