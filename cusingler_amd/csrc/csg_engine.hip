@@ -57,6 +57,15 @@ int64_t now_us()
 
 // fine-tune correlation tile: 32 cells x 32 reference rows, 2 x 2 per thread
 
+// Work memory of csg_code_csr_noalloc, one copy per device for the life of
+// the process: [misc 256 B | row counts 2 x CSG_CODE_NOALLOC_MAX_ROWS u32 |
+// row lists 2 x CSG_CODE_NOALLOC_MAX_ROWS u32].  Host code never uses this
+// name as an address: each engine resolves the device address once at
+// csg_init_gpu (hipGetSymbolAddress) and passes that.
+constexpr size_t NOALLOC_MISC  = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_MISC + (size_t)4 * 4 * CSG_CODE_NOALLOC_MAX_ROWS;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+
 }  // namespace
 
 struct csg_engine
@@ -105,6 +114,7 @@ struct csg_engine
     uint8_t*    d_code_misc = nullptr;  // flags, n_over, max_uniq[2] (u32), bases[2] (i64)
     uint8_t*    h_code_misc = nullptr;  // pinned
     uint64_t    code_slice = (uint64_t)1 << 26;  // entries per upload slice
+    uint8_t*    noalloc_scratch = nullptr;  // device address of `scratch` on this engine's device
     int16_t*  d_Xq = nullptr;
     float*    d_sqq_f = nullptr;
     double*    d_irq   = nullptr;  // the same for the query chunk
@@ -555,6 +565,15 @@ int csg_init_gpu(int gpuid, csg_engine** out)
     if (hipSetDevice(gpuid) != hipSuccess || hipGetDeviceProperties(&eng->prop, gpuid) != hipSuccess ||
         hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
     {
+        delete eng;
+        return CSG_ERR_DEVICE;
+    }
+    // this device's copy of the allocation-free entry point's work memory
+    if (hipGetSymbolAddress((void**)&eng->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess ||
+        !eng->noalloc_scratch)
+    {
+        (void)hipGetLastError();
+        (void)hipStreamDestroy(eng->stream);
         delete eng;
         return CSG_ERR_DEVICE;
     }
@@ -3849,6 +3868,45 @@ int csg_code_csr(csg_engine* eng, uint32_t rows, uint32_t width, const float* da
         max_uniq[0] = misc[2];
         max_uniq[1] = misc[3];
     }
+    return CSG_OK;
+}
+
+int csg_code_csr_noalloc(csg_engine* eng, uint32_t rows, uint32_t width, const float* data,
+                         const int32_t* indices, const int32_t* indptr, const int32_t* map_a,
+                         const int32_t* map_b, uint16_t* data_a, int32_t* indices_a,
+                         int32_t* indptr_a, uint16_t* data_b, int32_t* indices_b,
+                         int32_t* indptr_b, uint32_t* status)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (rows == 0 || width == 0 || !data || !indices || !indptr || !map_a || !map_b || !data_a ||
+        !indices_a || !indptr_a || !data_b || !indices_b || !indptr_b || !status)
+        return eng->fail(CSG_ERR_ARGUMENT, "empty CSR or missing device pointer");
+    if (rows > (uint32_t)CSG_CODE_NOALLOC_MAX_ROWS)
+        return eng->fail(CSG_ERR_ARGUMENT,
+                         "csg_code_csr_noalloc codes at most " +
+                             std::to_string(CSG_CODE_NOALLOC_MAX_ROWS) + " rows per call (asked " +
+                             std::to_string(rows) + "); use csg_code_csr or split the rows");
+    HIPCHK(hipSetDevice(eng->device));
+    // the work memory's device address (never the array's name in host code)
+    uint8_t*      misc  = eng->noalloc_scratch;
+    uint32_t*     cnt   = (uint32_t*)(misc + NOALLOC_MISC);
+    uint32_t*     over  = cnt + 2 * (size_t)CSG_CODE_NOALLOC_MAX_ROWS;
+    const CodeSet sa{ map_a, indptr_a, data_a, indices_a, cnt };
+    const CodeSet sb{ map_b, indptr_b, data_b, indices_b, cnt + rows };
+    k_code_init<<<1, 64, 0, eng->stream>>>((uint32_t*)misc);
+    launch_coding(eng, eng->stream, data, indices, indptr, 0, rows, width, sa, sb, over, misc);
+    k_code_status<<<1, 64, 0, eng->stream>>>((const uint32_t*)misc, status);
+    LAUNCHCHK();
+    return CSG_OK;
+}
+
+int csg_synchronize(csg_engine* eng)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    HIPCHK(hipSetDevice(eng->device));
+    HIPCHK(hipStreamSynchronize(eng->stream));
     return CSG_OK;
 }
 

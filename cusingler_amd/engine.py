@@ -84,6 +84,8 @@ def lib():
     L.csg_query_coded.argtypes = [_V, C.c_int, _V, _V, _V]
     L.csg_set_upload_slice.argtypes = [_V, _U64]
     L.csg_code_csr.argtypes = [_V, _U32, _U32] + [_V] * 12
+    L.csg_code_csr_noalloc.argtypes = [_V, _U32, _U32] + [_V] * 12
+    L.csg_synchronize.argtypes = [_V]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_set_phase_timing.argtypes = [_V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
@@ -293,6 +295,19 @@ class Engine:
                                        out_a[0], out_a[1], out_a[2], out_b[0], out_b[1],
                                        out_b[2], _p(mu)), "code_csr")
         return int(mu[0]), int(mu[1])
+
+    def code_csr_noalloc(self, rows: int, width: int, data, indices, indptr, map_a, map_b,
+                         out_a, out_b, status):
+        """csg_code_csr_noalloc: the same coding with the library's fixed work
+        memory; `status` is a device pointer to 4 uint32 (flags, max_uniq a,
+        max_uniq b, 0), valid after the engine's stream passes the call."""
+        self._check(lib().csg_code_csr_noalloc(self._h, rows, width, data, indices, indptr, map_a,
+                                               map_b, out_a[0], out_a[1], out_a[2], out_b[0],
+                                               out_b[1], out_b[2], status), "code_csr_noalloc")
+
+    def synchronize(self):
+        """Waits for every call made on the engine's stream."""
+        self._check(lib().csg_synchronize(self._h), "synchronize")
 
     def set_upload_slice(self, entries: int):
         self._check(lib().csg_set_upload_slice(self._h, int(entries)), "set_upload_slice")

@@ -196,6 +196,33 @@ int csg_code_csr(csg_engine* eng, uint32_t rows, uint32_t width, const float* da
                  const int32_t* map_b, uint16_t* data_a, int32_t* indices_a, int32_t* indptr_a,
                  uint16_t* data_b, int32_t* indices_b, int32_t* indptr_b, uint32_t* max_uniq);
 
+/* csg_code_csr for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same inputs and outputs as csg_code_csr
+ * (device pointers), but the work memory (row counts and row lists) lives in
+ * a fixed per-device array of the library, 16 MiB + 256 B, held from the
+ * first csg_init_gpu on that device until the process ends; the call
+ * allocates nothing and does not synchronise.  At most
+ * CSG_CODE_NOALLOC_MAX_ROWS rows per call (CSG_ERR_ARGUMENT beyond).  Every
+ * call on one device shares that array, so calls that overlap on one GPU
+ * (two engines, or two host threads, on the same device) race; calls on one
+ * engine are ordered by its stream.  Results are valid once the engine's
+ * stream has passed the call: `status` (device, 4 words) then holds
+ * [flags, max_uniq of set a, max_uniq of set b, 0], flags bit 0 a NaN value,
+ * bit 1 a column outside `width`, bit 2 a row whose distinct values exceed
+ * uint16 codes or the block coder's LDS (codes are then not valid). */
+#define CSG_CODE_NOALLOC_MAX_ROWS (1 << 20)
+
+/* Waits until the engine's stream has passed every call made on it (for the
+ * entry points that do not synchronise).  No counterpart in the reference. */
+int csg_synchronize(csg_engine* eng);
+
+int csg_code_csr_noalloc(csg_engine* eng, uint32_t rows, uint32_t width, const float* data,
+                         const int32_t* indices, const int32_t* indptr, const int32_t* map_a,
+                         const int32_t* map_b, uint16_t* data_a, int32_t* indices_a,
+                         int32_t* indptr_a, uint16_t* data_b, int32_t* indices_b,
+                         int32_t* indptr_b, uint32_t* status);
+
 /* Diagnostic: copies the query's value-coded CSR of step 0 (score gene set)
  * or 1 (fine-tune gene set) from the device -- what csg_copyin_query_raw
  * produced, or what csg_copyin_score / csg_copyin received.  indptr receives
