@@ -266,7 +266,7 @@ def main():
         return dict(value=Nq * args.steps / dt, ms=dt * 1e3 / args.steps, phases=ph, phases1=ph1,
                     counters=cnt, first=first, final=final)
 
-    def measure_e2e(form, steps, warmup):
+    def measure_e2e(form, steps, warmup, pipelined):
         """Query CSR on the host (float values, pinned) -> first and final labels
         on the host: the upload and device value coding of the query
         (csg_copyin_query_raw, both gene sets), the score step, the fine-tune and
@@ -280,10 +280,17 @@ def main():
         m0, m1 = prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width)
 
         def step():
+            if pipelined:
+                # the next batch (here: the same arrays again) uploads while
+                # this one is coded and annotated
+                eng.prefetch_query_raw(qd, qi, qp)
             eng.copyin_query_raw(qd, qi, qp, ds.qry_width, m0, m1)
             first, _, _ = eng.get_label(0.8, 0.05, path, outputs="first")
             final, _ = eng.finetune(0.8, 0.05, 0)
             return first, final
+
+        if pipelined:
+            eng.prefetch_query_raw(qd, qi, qp)
 
         for _ in range(warmup):
             step()
@@ -298,11 +305,15 @@ def main():
         eng.phase_times(reset=True)
         step()
         ph = eng.phase_times(reset=True)
+        eng.synchronize()
+        if pipelined:
+            eng.copyin_query_raw(qd, qi, qp, ds.qry_width, m0, m1)  # consume the last prefetch
         pin.free()
         nnz = int(ds.qry_indptr[-1])
         up_ms = ph["query_coding"][0]
         return {"value": round(Nq * steps / dt, 1), "unit": "cells/s", "form": form,
                 "ms_per_step": round(dt * 1e3 / steps, 3), "steps": steps,
+                "pipelined": pipelined,
                 "query_upload_and_coding_ms": round(up_ms, 3),
                 "query_bytes": 8 * nnz + 4 * (Nq + 1),
                 "upload_and_coding_GBps": round((8 * nnz + 4 * (Nq + 1)) / (up_ms * 1e-3) / 1e9, 1)
@@ -312,7 +323,9 @@ def main():
                     np.array_equal(final, res[form]["final"])),
                 "window": "float query CSR in pinned host memory -> first and final labels "
                           "on the host (query H2D + device value coding of both gene sets + "
-                          "score step + fine-tune + label D2H); reference resident"}
+                          "score step + fine-tune + label D2H); reference resident" +
+                          ("; the next batch's H2D overlaps this batch's annotation"
+                           if pipelined else "; upload, then annotation")}
 
     def measure_host_coding(form, steps):
         """The reference's own structure (Pipeline::score then ::finetune,
@@ -351,8 +364,9 @@ def main():
     e2e = None
     if not args.no_e2e:
         print("[bench] end-to-end leg", file=sys.stderr, flush=True)
-        e2e = {f: measure_e2e(f, max(1, args.steps // 2), 1)
+        e2e = {f: measure_e2e(f, max(1, args.steps // 2), 1, True)
                for f in ([chosen] if args.only_chosen else [chosen, "plain"])}
+        e2e["serial_" + chosen] = measure_e2e(chosen, max(1, args.steps // 2), 1, False)
         e2e["host_coding_" + chosen] = measure_host_coding(chosen, 2)
     r = res[chosen]
     steps = args.steps

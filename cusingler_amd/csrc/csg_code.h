@@ -29,13 +29,14 @@ __device__ inline uint32_t code_key(float v)
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// Entries a wave codes in its LDS (8 bytes each); longer rows go to the
-// block kernel.
-constexpr int CODE_WAVE_CAP = 1024;
+// Kept entries per set a wave codes in its LDS (8 bytes each, both sets of a
+// row held at once); longer rows go to the block kernel.
+constexpr int CODE_WAVE_CAP = 512;
 constexpr int CODE_NT       = 256;  // threads of the coding kernels' blocks
 
-// Flags of a coding pass (bit 0: a NaN value, bit 1: a column outside the
-// query width, bit 2: a row with more distinct values than uint16 codes hold).
+// Flags of a coding pass (bit 0: a NaN among the kept values, bit 1: a column
+// outside the query width, bit 2: a row with more distinct values than
+// uint16 codes hold).
 constexpr uint32_t CODE_NAN = 1, CODE_BADCOL = 2, CODE_OVERFLOW = 4;
 
 struct CodeSet
@@ -51,7 +52,7 @@ struct CodeSet
 // (data, indices, indptr) with indptr relative to `base` (the first entry of
 // row 0 sits at data[0]).
 __global__ __launch_bounds__(CODE_NT) void k_code_count(
-    const float* __restrict__ data, const int32_t* __restrict__ indices,
+    const int32_t* __restrict__ indices,
     const int32_t* __restrict__ indptr, int64_t base, uint32_t nrows, uint32_t width,
     const int32_t* __restrict__ map_a, const int32_t* __restrict__ map_b,
     uint32_t* __restrict__ cnt_a, uint32_t* __restrict__ cnt_b, uint32_t* __restrict__ flags)
@@ -71,9 +72,6 @@ __global__ __launch_bounds__(CODE_NT) void k_code_count(
                 fl |= CODE_BADCOL;
                 continue;
             }
-            const float v = data[i];
-            if (v != v)
-                fl |= CODE_NAN;
             ca += map_a[c] >= 0;
             cb += map_b[c] >= 0;
         }
@@ -186,72 +184,110 @@ __device__ inline uint32_t code_ranks(const uint64_t* keys, int n, int t, int nt
     return (uint32_t)incl;  // the last thread's value is the distinct count
 }
 
-// One wave per row: both sets' codes of rows [0, nrows).  `over` lists the
-// rows (with their set in bit 31) whose kept entries exceed the wave's LDS.
+// Sorts keys[0, n) of one row and set in the wave's LDS and writes the
+// codes; returns the distinct count (every lane).
+__device__ inline uint32_t code_wave_row(uint64_t* keys, int n, int lane, uint16_t* out)
+{
+    int n2 = 64;
+    while (n2 < n)
+        n2 <<= 1;
+    for (int k = n + lane; k < n2; k += 64)
+        keys[k] = ~0ull;
+    __builtin_amdgcn_wave_barrier();
+    code_bitonic(keys, n2, lane, 64, [] { __builtin_amdgcn_wave_barrier(); });
+    const uint32_t d = code_ranks(keys, n, lane, 64, out, [](int v) { return wave_incl_scan(v); });
+    __builtin_amdgcn_wave_barrier();  // LDS reads done before the keys are rewritten
+    return __builtin_amdgcn_readlane(d, 63);
+}
+
+// One wave per row: both sets' codes of rows [0, nrows), the row's entries
+// read once for both.  `over` lists the (row, set) pairs -- set in bit 31 --
+// whose kept entries exceed the wave's LDS; a NaN among kept values raises
+// CODE_NAN (the reference orders only the kept values).
 __global__ __launch_bounds__(CODE_NT) void k_code_rows(
     const float* __restrict__ data, const int32_t* __restrict__ indices,
     const int32_t* __restrict__ indptr, int64_t base, uint32_t nrows, uint32_t width, CodeSet sa,
     CodeSet sb, uint32_t* __restrict__ max_uniq, uint32_t* __restrict__ over,
     uint32_t* __restrict__ n_over, uint32_t* __restrict__ flags)
 {
-    __shared__ uint64_t skeys[CODE_NT / 64][CODE_WAVE_CAP];
+    __shared__ uint64_t skeys[CODE_NT / 64][2][CODE_WAVE_CAP];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint64_t* keys = skeys[wid];
+    uint64_t* ka = skeys[wid][0];
+    uint64_t* kb = skeys[wid][1];
     const uint32_t nw = gridDim.x * (CODE_NT / 64);
+    const uint64_t below = (1ull << lane) - 1;
     uint32_t       mu[2] = { 1u, 1u };
+    uint32_t       fl    = 0;
     for (uint32_t r = blockIdx.x * (CODE_NT / 64) + wid; r < nrows; r += nw)
     {
-        const int64_t beg = indptr[r] - base, end = indptr[r + 1] - base;
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
+        const int  na = (int)sa.count[r], nb = (int)sb.count[r];
+        const bool wa = na > 0 && na <= CODE_WAVE_CAP, wb = nb > 0 && nb <= CODE_WAVE_CAP;
+        if (lane == 0)
         {
-            const CodeSet& cs = s ? sb : sa;
-            const int      n  = (int)cs.count[r];
-            if (n == 0)
-                continue;
-            if (n > CODE_WAVE_CAP)
+            if (na > CODE_WAVE_CAP)
+                over[atomicAdd(n_over, 1u)] = r;
+            if (nb > CODE_WAVE_CAP)
+                over[atomicAdd(n_over, 1u)] = r | 0x80000000u;
+        }
+        if (!wa && !wb)
+            continue;  // uniform per wave
+        const int64_t beg = indptr[r] - base, end = indptr[r + 1] - base;
+        const int64_t oa = sa.ptr[r], ob = sb.ptr[r];
+        // kept entries in row order: columns out, (key, slot) pairs to LDS
+        int slot_a = 0, slot_b = 0;
+        for (int64_t i0 = beg; i0 < end; i0 += 64)
+        {
+            const int64_t i  = i0 + lane;
+            int32_t       ma = -1, mb = -1;
+            if (i < end)
             {
-                if (lane == 0)
-                    over[atomicAdd(n_over, 1u)] = r | ((uint32_t)s << 31);
-                continue;
+                const int32_t c = indices[i];
+                if (c >= 0 && (uint32_t)c < width)
+                {
+                    ma = wa ? sa.map[c] : -1;
+                    mb = wb ? sb.map[c] : -1;
+                }
             }
-            const int64_t o = cs.ptr[r];
-            // kept entries in row order: columns out, (key, slot) pairs to LDS
-            int slot = 0;
-            for (int64_t i0 = beg; i0 < end; i0 += 64)
+            const uint64_t bala = __ballot(ma >= 0), balb = __ballot(mb >= 0);
+            if (ma >= 0 || mb >= 0)
             {
-                const int64_t i    = i0 + lane;
-                int32_t       m    = -1;
-                if (i < end)
+                const float    v   = data[i];
+                const uint64_t key = (uint64_t)code_key(v) << 32;
+                if (v != v)
+                    fl |= CODE_NAN;
+                if (ma >= 0)
                 {
-                    const int32_t c = indices[i];
-                    m               = c >= 0 && (uint32_t)c < width ? cs.map[c] : -1;
+                    const int k    = slot_a + __popcll(bala & below);
+                    sa.indices[oa + k] = ma;
+                    ka[k]          = key | (uint32_t)k;
                 }
-                const uint64_t bal = __ballot(m >= 0);
-                if (m >= 0)
+                if (mb >= 0)
                 {
-                    const int k = slot + __popcll(bal & ((1ull << lane) - 1));
-                    cs.indices[o + k] = m;
-                    keys[k]           = ((uint64_t)code_key(data[i]) << 32) | (uint32_t)k;
+                    const int k    = slot_b + __popcll(balb & below);
+                    sb.indices[ob + k] = mb;
+                    kb[k]          = key | (uint32_t)k;
                 }
-                slot += __popcll(bal);
             }
-            int n2 = 64;
-            while (n2 < n)
-                n2 <<= 1;
-            for (int k = n + lane; k < n2; k += 64)
-                keys[k] = ~0ull;
-            __builtin_amdgcn_wave_barrier();
-            code_bitonic(keys, n2, lane, 64, [] { __builtin_amdgcn_wave_barrier(); });
-            const uint32_t d = code_ranks(keys, n, lane, 64, cs.data + o,
-                                          [](int v) { return wave_incl_scan(v); });
-            const uint32_t distinct = __builtin_amdgcn_readlane(d, 63);
-            mu[s]                   = max(mu[s], distinct + 1);
-            if (distinct > 65535u && lane == 0)
-                atomicOr(flags, CODE_OVERFLOW);
-            __builtin_amdgcn_wave_barrier();  // LDS reads done before the next row
+            slot_a += __popcll(bala);
+            slot_b += __popcll(balb);
+        }
+        if (wa)
+        {
+            const uint32_t d = code_wave_row(ka, na, lane, sa.data + oa);
+            mu[0]            = max(mu[0], d + 1);
+            if (d > 65535u)
+                fl |= CODE_OVERFLOW;
+        }
+        if (wb)
+        {
+            const uint32_t d = code_wave_row(kb, nb, lane, sb.data + ob);
+            mu[1]            = max(mu[1], d + 1);
+            if (d > 65535u)
+                fl |= CODE_OVERFLOW;
         }
     }
+    if (fl)
+        atomicOr(flags, fl);
     if (lane == 0)
     {
         atomicMax(&max_uniq[0], mu[0]);
@@ -321,9 +357,12 @@ __global__ __launch_bounds__(CODE_NT) void k_code_rows_big(
             const int at   = sslot;
             if (kept)
             {
-                const int k       = at + incl - 1;
+                const int   k     = at + incl - 1;
+                const float v     = data[i];
                 cs.indices[o + k] = m;
-                bkeys[k]          = ((uint64_t)code_key(data[i]) << 32) | (uint32_t)k;
+                bkeys[k]          = ((uint64_t)code_key(v) << 32) | (uint32_t)k;
+                if (v != v)
+                    atomicOr(flags, CODE_NAN);
             }
             __syncthreads();
             if (threadIdx.x == CODE_NT - 1)

@@ -115,6 +115,20 @@ struct csg_engine
     uint8_t*    h_code_misc = nullptr;  // pinned
     uint64_t    code_slice = (uint64_t)1 << 26;  // entries per upload slice
     uint8_t*    noalloc_scratch = nullptr;  // device address of `scratch` on this engine's device
+    // query batches uploaded ahead (csg_prefetch_query_raw): two slots,
+    // consumed oldest first
+    struct PreSlot
+    {
+        float*      d_data = nullptr;
+        int32_t*    d_idx  = nullptr;
+        size_t      cap    = 0;
+        hipEvent_t  copied = nullptr;
+        bool        valid  = false;
+        uint64_t    seq    = 0;
+        const void *h_data = nullptr, *h_idx = nullptr, *h_ptr = nullptr;
+        uint32_t    rows   = 0;
+    } pre[2];
+    uint64_t pre_seq = 0;
     int16_t*  d_Xq = nullptr;
     float*    d_sqq_f = nullptr;
     double*    d_irq   = nullptr;  // the same for the query chunk
@@ -749,6 +763,13 @@ void csg_release(csg_engine* eng)
             (void)hipEventDestroy(eng->code_done[b]);
     }
     dfree(eng->d_raw_ptr);
+    for (auto& ps : eng->pre)
+    {
+        dfree(ps.d_data);
+        dfree(ps.d_idx);
+        if (ps.copied)
+            (void)hipEventDestroy(ps.copied);
+    }
     dfree(eng->d_map_s);
     dfree(eng->d_map_f);
     dfree(eng->d_code_cnt);
@@ -3590,6 +3611,27 @@ int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches)
 
 namespace {
 
+// The second stream of the query uploads, its events and the coding counters
+// (created on first use).
+int ensure_copy_stream(csg_engine* eng)
+{
+    int rc;
+    if (!eng->d_code_misc && (rc = dalloc(eng, &eng->d_code_misc, 32)))
+        return rc;
+    if (eng->copy_stream)
+        return CSG_OK;
+    HIPCHK(hipStreamCreateWithFlags(&eng->copy_stream, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b)
+    {
+        HIPCHK(hipEventCreateWithFlags(&eng->code_copied[b], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&eng->code_done[b], hipEventDisableTiming));
+    }
+    for (auto& ps : eng->pre)
+        HIPCHK(hipEventCreateWithFlags(&ps.copied, hipEventDisableTiming));
+    HIPCHK(hipHostMalloc((void**)&eng->h_code_misc, 32, hipHostMallocDefault));
+    return CSG_OK;
+}
+
 // The coding passes of rows [0, nr) of a device CSR (row r's entries at
 // data/idx [ptr[r] - base, ptr[r + 1] - base)) into sets a and b, on `st`.
 // Scratch: cnt_a / cnt_b (nr each), over (2 nr), misc (32 bytes, cleared by
@@ -3603,10 +3645,10 @@ void launch_coding(const csg_engine* eng, hipStream_t st, const float* data, con
     uint32_t* d_nover = d_flags + 1;
     uint32_t* d_maxu  = d_flags + 2;
     int64_t*  d_base  = (int64_t*)(misc + 16);
-    const int grid    = std::max(1, eng->prop.multiProcessorCount) * 8;
     const int big_cap = (int)std::min<size_t>(eng->lds_limit / 8, (size_t)1 << 20);
-    const unsigned g  = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nr + 3) / 4, (uint32_t)grid));
-    k_code_count<<<g, CODE_NT, 0, st>>>(data, idx, ptr, base, nr, width, sa.map, sb.map, sa.count,
+    // one wave per row (rows strided over the grid beyond 2^20 blocks)
+    const unsigned g  = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nr + 3) / 4, 1u << 20));
+    k_code_count<<<g, CODE_NT, 0, st>>>(idx, ptr, base, nr, width, sa.map, sb.map, sa.count,
                                         sb.count, d_flags);
     k_code_scan<<<1, 1024, 0, st>>>(sa.count, sb.count, nr, (int32_t*)sa.ptr, (int32_t*)sb.ptr,
                                     d_base);
@@ -3633,6 +3675,57 @@ int coding_status(csg_engine* eng, const uint32_t* misc)
 }
 
 }  // namespace
+
+int csg_prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qry_data,
+                           const int32_t* qry_indices, const int32_t* qry_indptr)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (qry_height == 0 || !qry_indptr)
+        return eng->fail(CSG_ERR_ARGUMENT, "empty query batch");
+    const size_t nnz = (size_t)(qry_indptr[qry_height] - qry_indptr[0]);
+    if (nnz && (!qry_data || !qry_indices))
+        return eng->fail(CSG_ERR_ARGUMENT, "missing query values or columns");
+    HIPCHK(hipSetDevice(eng->device));
+    int rc;
+    if ((rc = ensure_copy_stream(eng)))
+        return rc;
+    // a free slot, else the older one (whose batch is then dropped)
+    csg_engine::PreSlot* ps = !eng->pre[0].valid ? &eng->pre[0]
+                              : !eng->pre[1].valid ? &eng->pre[1]
+                              : eng->pre[0].seq < eng->pre[1].seq ? &eng->pre[0] : &eng->pre[1];
+    ps->valid = false;
+    if (ps->cap < nnz || !ps->d_data)
+    {
+        // an upload into the old buffers may still be in flight
+        HIPCHK(hipStreamSynchronize(eng->copy_stream));
+        dfree(ps->d_data);
+        dfree(ps->d_idx);
+        ps->cap = 0;
+        const size_t n = std::max<size_t>(nnz, 1);
+        if ((rc = dalloc(eng, &ps->d_data, n)) || (rc = dalloc(eng, &ps->d_idx, n)))
+            return rc;
+        ps->cap = n;
+    }
+    // the copy stream is in order: this upload follows the slot's previous
+    // upload, and the slot's previous batch has been coded (csg_copyin_query_raw
+    // returns after its coding)
+    if (nnz)
+    {
+        HIPCHK(hipMemcpyAsync(ps->d_data, qry_data, nnz * 4, hipMemcpyHostToDevice,
+                              eng->copy_stream));
+        HIPCHK(hipMemcpyAsync(ps->d_idx, qry_indices, nnz * 4, hipMemcpyHostToDevice,
+                              eng->copy_stream));
+    }
+    HIPCHK(hipEventRecord(ps->copied, eng->copy_stream));
+    ps->valid  = true;
+    ps->seq    = ++eng->pre_seq;
+    ps->h_data = qry_data;
+    ps->h_idx  = qry_indices;
+    ps->h_ptr  = qry_indptr;
+    ps->rows   = qry_height;
+    return CSG_OK;
+}
 
 int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
                          const float* qry_data, const int32_t* qry_indices,
@@ -3665,23 +3758,26 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
             map_score[c] < -1 || map_finetune[c] < -1)
             return eng->fail(CSG_ERR_ARGUMENT, "column map entry outside its gene set");
     int rc;
-    if (!eng->copy_stream)
-    {
-        HIPCHK(hipStreamCreateWithFlags(&eng->copy_stream, hipStreamNonBlocking));
-        for (int b = 0; b < 2; ++b)
-        {
-            HIPCHK(hipEventCreateWithFlags(&eng->code_copied[b], hipEventDisableTiming));
-            HIPCHK(hipEventCreateWithFlags(&eng->code_done[b], hipEventDisableTiming));
-        }
-        HIPCHK(hipHostMalloc((void**)&eng->h_code_misc, 32, hipHostMallocDefault));
-        if ((rc = dalloc(eng, &eng->d_code_misc, 32)))
-            return rc;
-    }
+    if ((rc = ensure_copy_stream(eng)))
+        return rc;
+    // a batch uploaded ahead by csg_prefetch_query_raw from these arrays (the
+    // oldest such) is coded from its device copy in one slice
+    csg_engine::PreSlot* slot = nullptr;
+    for (auto& ps : eng->pre)
+        if (ps.valid && ps.h_data == (const void*)qry_data && ps.h_idx == (const void*)qry_indices &&
+            ps.h_ptr == (const void*)qry_indptr && ps.rows == qry_height &&
+            (!slot || ps.seq < slot->seq))
+            slot = &ps;
+    const bool staged = slot != nullptr;
+    if (slot)
+        slot->valid = false;
     // slices of whole rows of at most code_slice entries (a row longer than
     // that is a slice of its own)
     std::vector<uint32_t> cut{ 0 };
     size_t                biggest = 0;
-    for (uint32_t r = 0; r < qry_height;)
+    if (staged)
+        cut.push_back(qry_height);
+    for (uint32_t r = 0; r < qry_height && !staged;)
     {
         uint32_t e = r + 1;
         while (e < qry_height &&
@@ -3779,22 +3875,33 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
             const uint32_t r0   = cut[i], r1 = cut[i + 1], nr = r1 - r0;
             const int64_t  e0   = qry_indptr[r0];
             const size_t   n    = (size_t)(qry_indptr[r1] - e0);
-            if (i >= 2)
-                HIPCHK(hipStreamWaitEvent(eng->copy_stream, eng->code_done[b], 0));
-            if (n)
+            const float*   src_data = eng->d_raw_data[b];
+            const int32_t* src_idx  = eng->d_raw_idx[b];
+            if (staged)
             {
-                HIPCHK(hipMemcpyAsync(eng->d_raw_data[b], qry_data + (e0 - base0), n * 4,
-                                      hipMemcpyHostToDevice, eng->copy_stream));
-                HIPCHK(hipMemcpyAsync(eng->d_raw_idx[b], qry_indices + (e0 - base0), n * 4,
-                                      hipMemcpyHostToDevice, eng->copy_stream));
+                HIPCHK(hipStreamWaitEvent(eng->stream, slot->copied, 0));
+                src_data = slot->d_data;
+                src_idx  = slot->d_idx;
             }
-            HIPCHK(hipEventRecord(eng->code_copied[b], eng->copy_stream));
-            HIPCHK(hipStreamWaitEvent(eng->stream, eng->code_copied[b], 0));
+            else
+            {
+                if (i >= 2)
+                    HIPCHK(hipStreamWaitEvent(eng->copy_stream, eng->code_done[b], 0));
+                if (n)
+                {
+                    HIPCHK(hipMemcpyAsync(eng->d_raw_data[b], qry_data + (e0 - base0), n * 4,
+                                          hipMemcpyHostToDevice, eng->copy_stream));
+                    HIPCHK(hipMemcpyAsync(eng->d_raw_idx[b], qry_indices + (e0 - base0), n * 4,
+                                          hipMemcpyHostToDevice, eng->copy_stream));
+                }
+                HIPCHK(hipEventRecord(eng->code_copied[b], eng->copy_stream));
+                HIPCHK(hipStreamWaitEvent(eng->stream, eng->code_copied[b], 0));
+            }
             const CodeSet ss{ eng->d_map_s, eng->d_qptr + r0, eng->d_qdata, eng->d_qind, cnt_s + r0 };
             const CodeSet sf{ eng->d_map_f, eng->d_fqptr + r0, eng->d_fqdata, eng->d_fqind,
                               cnt_f + r0 };
-            launch_coding(eng, eng->stream, eng->d_raw_data[b], eng->d_raw_idx[b],
-                          eng->d_raw_ptr + r0, e0, nr, qry_width, ss, sf, eng->d_code_over,
+            launch_coding(eng, eng->stream, src_data, src_idx, eng->d_raw_ptr + r0,
+                          staged ? base0 : e0, nr, qry_width, ss, sf, eng->d_code_over,
                           eng->d_code_misc);
             LAUNCHCHK();
             HIPCHK(hipEventRecord(eng->code_done[b], eng->stream));
@@ -3850,7 +3957,7 @@ int csg_code_csr(csg_engine* eng, uint32_t rows, uint32_t width, const float* da
             return rc;
         eng->code_rows_cap = rows;
     }
-    if (!eng->d_code_misc && (rc = dalloc(eng, &eng->d_code_misc, 32)))
+    if ((rc = ensure_copy_stream(eng)))
         return rc;
     const CodeSet sa{ map_a, indptr_a, data_a, indices_a, eng->d_code_cnt };
     const CodeSet sb{ map_b, indptr_b, data_b, indices_b, eng->d_code_cnt + rows };

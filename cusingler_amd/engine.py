@@ -86,6 +86,7 @@ def lib():
     L.csg_code_csr.argtypes = [_V, _U32, _U32] + [_V] * 12
     L.csg_code_csr_noalloc.argtypes = [_V, _U32, _U32] + [_V] * 12
     L.csg_synchronize.argtypes = [_V]
+    L.csg_prefetch_query_raw.argtypes = [_V, _U32, _V, _V, _V]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_set_phase_timing.argtypes = [_V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
@@ -272,6 +273,16 @@ class Engine:
         self._check(lib().csg_copyin_query_raw(self._h, n, qry_width, _p(d), _p(i), _p(p), _p(ms),
                                                _p(mf)), "copyin_query_raw")
         self.Nq = self.fNq = n
+
+    def prefetch_query_raw(self, data: np.ndarray, indices: np.ndarray, indptr: np.ndarray):
+        """Starts the upload of the next batch (the arrays must stay alive and
+        unchanged until copyin_query_raw consumes them; pass the same arrays)."""
+        for a, dt in ((data, np.float32), (indices, np.int32), (indptr, np.int32)):
+            if a.dtype != dt or not a.flags.c_contiguous:
+                raise ValueError("prefetch needs contiguous float32 / int32 arrays (the same "
+                                 "objects copyin_query_raw will get)")
+        self._check(lib().csg_prefetch_query_raw(self._h, indptr.shape[0] - 1, _p(data),
+                                                 _p(indices), _p(indptr)), "prefetch_query_raw")
 
     def query_coded(self, step: int):
         """The query's value-coded CSR of a step as it sits on the device:

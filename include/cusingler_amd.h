@@ -165,8 +165,8 @@ int csg_set_finetune_budget(csg_engine* eng, uint64_t bytes, int64_t* batches);
  * qry_height rows or more).  The batch's codes raise max_uniq_gene of each
  * step above the reference's where a cell needs it.  Then csg_get_label and
  * csg_finetune annotate the batch; the candidate mask comes from
- * csg_get_label.  CSG_ERR_UNDEFINED for a NaN value or more than 65535
- * distinct values in a row (the reference's uint16 codes).  The host arrays
+ * csg_get_label.  CSG_ERR_UNDEFINED for a NaN among a row's kept values or
+ * more than 65535 distinct values in a row (the reference's uint16 codes).  The host arrays
  * are read before the call returns.  No counterpart in the reference: its
  * DataParser codes every query row on the host before copyin_score. */
 int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
@@ -187,7 +187,7 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
  *   max_uniq (host, 2)    : max over rows of (#distinct kept values + 1) per
  *                           set (may be NULL)
  * Runs on the engine's stream and returns when the codes are written;
- * CSG_ERR_UNDEFINED for a NaN value or a row of more than 65535 distinct
+ * CSG_ERR_UNDEFINED for a NaN among a row's kept values or a row of more than 65535 distinct
  * values.  Work buffers (2 x rows counts and row lists) are the engine's,
  * grown on demand.  No counterpart in the reference (its coding runs on the
  * host, src/io.cpp:714-784). */
@@ -208,7 +208,7 @@ int csg_code_csr(csg_engine* eng, uint32_t rows, uint32_t width, const float* da
  * (two engines, or two host threads, on the same device) race; calls on one
  * engine are ordered by its stream.  Results are valid once the engine's
  * stream has passed the call: `status` (device, 4 words) then holds
- * [flags, max_uniq of set a, max_uniq of set b, 0], flags bit 0 a NaN value,
+ * [flags, max_uniq of set a, max_uniq of set b, 0], flags bit 0 a NaN kept value,
  * bit 1 a column outside `width`, bit 2 a row whose distinct values exceed
  * uint16 codes or the block coder's LDS (codes are then not valid). */
 #define CSG_CODE_NOALLOC_MAX_ROWS (1 << 20)
@@ -234,6 +234,19 @@ int csg_query_coded(csg_engine* eng, int step, uint16_t* data, int32_t* indices,
  * per slice, a longer row is a slice of its own).  No counterpart in the
  * reference. */
 int csg_set_upload_slice(csg_engine* eng, uint64_t entries);
+
+/* Starts the upload of the next query batch's values and columns on the
+ * engine's second stream and returns at once, so that the upload overlaps the
+ * score step and fine-tune of the batch before it (SURVEY's streaming
+ * overlap: src/cusingler.cu:335-409 slices and uploads synchronously).  The
+ * next csg_copyin_query_raw with the same three pointers and row count codes
+ * that batch from its device copy instead of uploading it; the host arrays
+ * must stay valid and unchanged until that call.  Two batches can be in
+ * flight (two device copies, consumed oldest first), so the upload of batch
+ * i + 1 can be started before batch i is coded; a third prefetch drops the
+ * older unconsumed one.  No counterpart in the reference. */
+int csg_prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qry_data,
+                           const int32_t* qry_indices, const int32_t* qry_indptr);
 
 /* Caps the query chunk of the score step (src/cusingler.cu:335-409 slices the
  * query to the free device memory; here a chunk is at most 8 GiB of scores

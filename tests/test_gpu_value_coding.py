@@ -173,7 +173,7 @@ def test_raw_batch_errors(eng):
                              ds.qry_width, m0, m1)
     assert e.value.status == 1
     bad = data.copy()
-    bad[17] = np.nan
+    bad[int(np.nonzero(m0[ind] >= 0)[0][0])] = np.nan  # a kept value (unkept ones are ignored)
     with pytest.raises(engine.CsgError) as e:      # NaN: undefined in the reference
         eng.copyin_query_raw(bad, ind, p32, ds.qry_width, m0, m1)
     assert e.value.status == 5
@@ -237,3 +237,46 @@ def test_code_csr_device_pointers(eng, name):
         assert np.array_equal(d, c.qry_data), (name, x)
         assert m <= c.max_uniq_gene and m == max(
             len(np.unique(d[p[r]:p[r + 1]])) + 1 for r in range(len(p) - 1))
+
+
+def test_prefetched_batches(eng):
+    """csg_prefetch_query_raw: batch B uploads while batch A is annotated; the
+    next copyin_query_raw of B codes it from the device copy, with the
+    oracle's labels; a copyin of other arrays than the prefetched ones
+    ignores the prefetch."""
+    from cusingler_amd import engine
+    from oracle import oracle as O
+    ds = _data("small")
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    orc = O.Oracle(ds, 8)
+    of, om, _ = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
+    ofin, _ = orc.finetune(0, ds.qry_height, om, 0.8, 0.05, 0, O.VAR_SEQ)
+    m0, m1 = prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width)
+    ptr = np.asarray(ds.qry_indptr, np.int64)
+
+    def batch(q0, q1):
+        sub = ptr[q0:q1 + 1]
+        return (np.ascontiguousarray(ds.qry_data[sub[0]:sub[-1]], np.float32),
+                np.ascontiguousarray(ds.qry_indices[sub[0]:sub[-1]], np.int32),
+                (sub - sub[0]).astype(np.int32))
+
+    eng.reserve(c0, c1, ds.qry_height)
+    A, B, C = batch(0, 400), batch(400, 1000), batch(100, 300)
+    _say("prefetch: A loaded, B prefetched during A")
+    eng.copyin_query_raw(*A, ds.qry_width, m0, m1)
+    eng.prefetch_query_raw(*B)
+    fa, _, _ = eng.get_label(0.8, 0.05, "exact_int8", outputs="first")
+    fina, _ = eng.finetune(0.8, 0.05, 0)
+    eng.copyin_query_raw(*B, ds.qry_width, m0, m1)
+    fb, _, _ = eng.get_label(0.8, 0.05, "exact_int8", outputs="first")
+    finb, _ = eng.finetune(0.8, 0.05, 0)
+    assert np.array_equal(fa, of[:400]) and np.array_equal(fina, ofin[:400])
+    assert np.array_equal(fb, of[400:]) and np.array_equal(finb, ofin[400:])
+    _say("prefetch: B prefetched, C loaded instead")
+    eng.prefetch_query_raw(*B)
+    eng.copyin_query_raw(*C, ds.qry_width, m0, m1)
+    fc, _, _ = eng.get_label(0.8, 0.05, "plain", outputs="first")
+    assert np.array_equal(fc, of[100:300])
+    eng.destroy()
+    eng.destroy_score()

@@ -84,7 +84,9 @@ def test_noalloc_coder_matches_allocating_coder():
                 assert np.array_equal(dy, c.qry_data[:py[-1]]), (rows, s)
         # a NaN value raises the status flag (the call itself returns)
         t = _case(ds, prep, 50)
-        t["data"][3] = float("nan")
+        ind = ds.qry_indices[:int(ds.qry_indptr[50])]
+        kept = int(np.nonzero(prep.query_map(0, ds.qry_width)[ind] >= 0)[0][0])
+        t["data"][kept] = float("nan")  # a kept value (unkept ones are ignored)
         import torch
         torch.cuda.synchronize()  # the write is on torch's stream
         ya, yb = _out(t, "y")
