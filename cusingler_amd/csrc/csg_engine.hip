@@ -28,6 +28,7 @@
 #include "../../include/cusingler_amd.h"
 #include "csg_exact.h"
 #include "csg_code.h"
+#include "csg_train.h"
 #include "csg_ft_kernels.h"
 #include "csg_ftd.h"
 #include "csg_kernels.h"
@@ -129,6 +130,8 @@ struct csg_engine
         uint32_t    rows   = 0;
     } pre[2];
     uint64_t pre_seq = 0;
+    // results of the last csg_train
+    std::vector<uint32_t> tr_idxs, tr_values, tr_common, tr_thre;
     int16_t*  d_Xq = nullptr;
     float*    d_sqq_f = nullptr;
     double*    d_irq   = nullptr;  // the same for the query chunk
@@ -3675,6 +3678,211 @@ int coding_status(csg_engine* eng, const uint32_t* misc)
 }
 
 }  // namespace
+
+int csg_train(csg_engine* eng, uint32_t ref_height, uint32_t ref_width, const float* ref_data,
+              const int32_t* ref_indices, const int32_t* ref_indptr, const int32_t* ref_map,
+              uint32_t width, uint32_t ct_num, const int32_t* celltype, uint64_t* sizes)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (ref_height == 0 || ref_width == 0 || width == 0 || !ref_indptr || !celltype)
+        return eng->fail(CSG_ERR_ARGUMENT, "empty reference matrix");
+    if (ct_num == 0 || ct_num > 256)
+        return eng->fail(CSG_ERR_ARGUMENT, "1 to 256 labels (the reference's uint8 codes)");
+    HIPCHK(hipSetDevice(eng->device));
+    const uint32_t L = ct_num, W = ref_width;
+    // label codes narrowed to uint8 (src/io.cpp:813); each label needs a cell
+    std::vector<uint8_t>  lab(ref_height);
+    std::vector<uint32_t> nlab(L, 0);
+    for (uint32_t r = 0; r < ref_height; ++r)
+    {
+        lab[r] = (uint8_t)celltype[r];
+        if (lab[r] >= L)
+            return eng->fail(CSG_ERR_ARGUMENT, "cell type code without a label name");
+        nlab[lab[r]]++;
+        if (ref_indptr[r + 1] < ref_indptr[r])
+            return eng->fail(CSG_ERR_ARGUMENT, "reference row pointers decrease");
+    }
+    for (uint32_t l = 0; l < L; ++l)
+        if (nlab[l] == 0)
+            return eng->fail(CSG_ERR_UNDEFINED, "label " + std::to_string(l) +
+                                                    " has no reference cell (its median is "
+                                                    "undefined in the reference)");
+    const size_t k_pair = (size_t)std::round(500 * std::pow((2 / 3.0), std::log2((double)L)));
+    const size_t thre_n = (size_t)std::round(500 * std::pow((2 / 3.0), std::log2(2)));
+    if (k_pair > W)
+        return eng->fail(CSG_ERR_UNDEFINED, "fewer reference genes than the score gene count per "
+                                            "label pair");
+    const size_t nnz  = (size_t)(ref_indptr[ref_height] - ref_indptr[0]);
+    const size_t nseg = (size_t)L * W;
+    if (ref_indptr[0] != 0)
+        return eng->fail(CSG_ERR_ARGUMENT, "reference row pointers must start at 0");
+    // device work memory, freed on return
+    float *    d_data = nullptr, *d_vals = nullptr, *d_med = nullptr;
+    int32_t *  d_ind = nullptr, *d_ptr = nullptr, *d_map = nullptr;
+    uint8_t*   d_lab = nullptr;
+    uint32_t * d_cnt = nullptr, *d_fill = nullptr, *d_seg = nullptr, *d_big = nullptr,
+             *d_misc = nullptr, *d_nlab = nullptr, *d_npos = nullptr, *d_nzero = nullptr,
+             *d_vout = nullptr, *d_bits = nullptr;
+    uint64_t * d_off = nullptr, *d_tiles = nullptr, *d_scr = nullptr, *d_voff = nullptr;
+    auto cleanup = [&]() {
+        dfree(d_data); dfree(d_vals); dfree(d_med); dfree(d_ind); dfree(d_ptr); dfree(d_map);
+        dfree(d_lab); dfree(d_cnt); dfree(d_fill); dfree(d_seg); dfree(d_big); dfree(d_misc);
+        dfree(d_nlab); dfree(d_npos); dfree(d_nzero); dfree(d_vout); dfree(d_bits); dfree(d_off);
+        dfree(d_tiles); dfree(d_scr); dfree(d_voff);
+    };
+    int rc = CSG_OK;
+    auto run = [&]() -> int {
+        const size_t ntile = (nseg + TRAIN_TILE - 1) / TRAIN_TILE;
+        if ((rc = upload(eng, &d_data, ref_data, std::max<size_t>(nnz, 1))) ||
+            (rc = upload(eng, &d_ind, ref_indices, std::max<size_t>(nnz, 1))) ||
+            (rc = upload(eng, &d_ptr, ref_indptr, (size_t)ref_height + 1)) ||
+            (rc = upload(eng, &d_lab, lab.data(), lab.size())) ||
+            (rc = upload(eng, &d_nlab, nlab.data(), nlab.size())) ||
+            (ref_map && (rc = upload(eng, &d_map, ref_map, width))) ||
+            (rc = dalloc(eng, &d_cnt, nseg)) || (rc = dalloc(eng, &d_fill, nseg)) ||
+            (rc = dalloc(eng, &d_off, nseg)) || (rc = dalloc(eng, &d_seg, std::max<size_t>(nnz, 1))) ||
+            (rc = dalloc(eng, &d_vals, std::max<size_t>(nnz, 1))) || (rc = dalloc(eng, &d_med, nseg)) ||
+            (rc = dalloc(eng, &d_big, nseg)) || (rc = dalloc(eng, &d_misc, 4)) ||
+            (rc = dalloc(eng, &d_tiles, ntile)))
+            return rc;
+        const unsigned fg = 2048;
+        k_train_fill<<<fg, 256, 0, eng->stream>>>(d_cnt, nseg, 0u);
+        k_train_fill<<<fg, 256, 0, eng->stream>>>(d_fill, nseg, 0u);
+        k_train_fill<<<1, 64, 0, eng->stream>>>(d_misc, 4, 0u);
+        const unsigned rg = (unsigned)std::min<uint32_t>((ref_height + 3) / 4, 1u << 20);
+        k_train_count<<<rg, TRAIN_NT, 0, eng->stream>>>(d_ind, d_ptr, ref_height, d_lab, d_map,
+                                                        width, W, d_cnt, d_seg, d_misc);
+        k_train_tile_sum<<<(unsigned)ntile, 1024, 0, eng->stream>>>(d_cnt, nseg, d_tiles);
+        LAUNCHCHK();
+        std::vector<uint64_t> tiles(ntile);
+        HIPCHK(hipMemcpyAsync(tiles.data(), d_tiles, ntile * 8, hipMemcpyDeviceToHost, eng->stream));
+        uint32_t misc[4];
+        HIPCHK(hipMemcpyAsync(misc, d_misc, 16, hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        if (misc[0])
+            return eng->fail(CSG_ERR_ARGUMENT, "a reference column index lies outside its width");
+        uint64_t run_off = 0;
+        for (auto& t : tiles)
+        {
+            const uint64_t v = t;
+            t                = run_off;
+            run_off += v;
+        }
+        HIPCHK(hipMemcpyAsync(d_tiles, tiles.data(), ntile * 8, hipMemcpyHostToDevice, eng->stream));
+        k_train_tile_scan<<<(unsigned)ntile, 1024, 0, eng->stream>>>(d_cnt, nseg, d_tiles, d_off);
+        k_train_scatter<<<4096, 256, 0, eng->stream>>>(d_data, d_seg, nnz, d_off, d_fill, d_vals);
+        k_train_median<<<8192, TRAIN_NT, 0, eng->stream>>>(d_cnt, d_off, d_vals, d_nlab, L, W,
+                                                          d_med, d_big, d_misc + 1);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(misc, d_misc, 16, hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        if (misc[1])
+        {
+            // segments longer than a wave's LDS: sorted in global scratch
+            size_t slice = 64;
+            const uint32_t nmax = *std::max_element(nlab.begin(), nlab.end());
+            while (slice < nmax)
+                slice <<= 1;
+            const unsigned bg = std::min<uint32_t>(misc[1], 256u);
+            if ((rc = dalloc(eng, &d_scr, (size_t)bg * slice)))
+                return rc;
+            k_train_median_big<<<bg, TRAIN_NT, 0, eng->stream>>>(d_cnt, d_off, d_vals, d_nlab, W,
+                                                                d_big, d_misc + 1, d_scr, slice,
+                                                                d_med);
+            LAUNCHCHK();
+            dfree(d_scr);
+        }
+        // positive / zero differences per ordered pair, then their windows
+        const size_t npair = (size_t)L * L;
+        if ((rc = dalloc(eng, &d_npos, npair)) || (rc = dalloc(eng, &d_nzero, npair)))
+            return rc;
+        const unsigned pg = (unsigned)std::min<size_t>(npair, 4096);
+        k_train_pair_count<<<pg, TRAIN_NT, 0, eng->stream>>>(d_med, L, W, d_npos, d_nzero);
+        LAUNCHCHK();
+        std::vector<uint32_t> npos(npair), nzero(npair);
+        HIPCHK(hipMemcpyAsync(npos.data(), d_npos, npair * 4, hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipMemcpyAsync(nzero.data(), d_nzero, npair * 4, hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        std::vector<uint64_t> voff(npair, 0);
+        eng->tr_idxs.assign(2 * npair, 0);
+        uint64_t total = 0;
+        size_t   kmax  = 64;
+        for (size_t pr = 0; pr < npair; ++pr)
+        {
+            if (pr / L == pr % L)
+                continue;  // (0, 0) for a label with itself
+            const uint64_t P = npos[pr], Z = nzero[pr];
+            voff[pr]              = total;
+            eng->tr_idxs[2 * pr]     = (uint32_t)total;
+            eng->tr_idxs[2 * pr + 1] = (uint32_t)P;
+            total += P;
+            size_t need = P;
+            if (P < k_pair && Z < k_pair - P)
+                need = std::max<size_t>(need, W - P - Z);
+            while (kmax < need)
+                kmax <<= 1;
+        }
+        const unsigned sg    = (unsigned)std::min<size_t>(npair, 512);
+        const size_t   words = ((size_t)W + 31) / 32;
+        if ((rc = upload(eng, &d_voff, voff.data(), npair)) ||
+            (rc = dalloc(eng, &d_vout, std::max<uint64_t>(total, 1))) ||
+            (rc = dalloc(eng, &d_bits, 2 * words)) || (rc = dalloc(eng, &d_scr, (size_t)sg * kmax)))
+            return rc;
+        k_train_fill<<<64, 256, 0, eng->stream>>>(d_bits, 2 * words, 0u);
+        k_train_pair_sort<<<sg, TRAIN_NT, 0, eng->stream>>>(
+            d_med, L, W, d_npos, d_nzero, d_voff, (uint32_t)thre_n, (uint32_t)k_pair, d_scr, kmax,
+            d_vout, d_bits, d_bits + words);
+        LAUNCHCHK();
+        eng->tr_values.resize(total);
+        std::vector<uint32_t> bits(2 * words);
+        if (total)
+            HIPCHK(hipMemcpyAsync(eng->tr_values.data(), d_vout, total * 4, hipMemcpyDeviceToHost,
+                                  eng->stream));
+        HIPCHK(hipMemcpyAsync(bits.data(), d_bits, 2 * words * 4, hipMemcpyDeviceToHost,
+                              eng->stream));
+        HIPCHK(hipStreamSynchronize(eng->stream));
+        eng->tr_thre.clear();
+        eng->tr_common.clear();
+        for (uint32_t g = 0; g < W; ++g)
+        {
+            if ((bits[g >> 5] >> (g & 31)) & 1u)
+                eng->tr_thre.push_back(g);
+            if ((bits[words + (g >> 5)] >> (g & 31)) & 1u)
+                eng->tr_common.push_back(g);
+        }
+        return CSG_OK;
+    };
+    rc = run();
+    cleanup();
+    if (rc)
+        return rc;
+    if (sizes)
+    {
+        sizes[0] = eng->tr_values.size();
+        sizes[1] = eng->tr_common.size();
+        sizes[2] = eng->tr_thre.size();
+    }
+    return CSG_OK;
+}
+
+uint64_t csg_train_table(const csg_engine* eng, int which, uint32_t* out)
+{
+    if (!eng)
+        return 0;
+    const std::vector<uint32_t>* v = nullptr;
+    switch (which)
+    {
+    case 0: v = &eng->tr_idxs; break;
+    case 1: v = &eng->tr_values; break;
+    case 2: v = &eng->tr_common; break;
+    case 3: v = &eng->tr_thre; break;
+    default: return 0;
+    }
+    if (out && !v->empty())
+        std::memcpy(out, v->data(), v->size() * sizeof(uint32_t));
+    return v->size();
+}
 
 int csg_prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qry_data,
                            const int32_t* qry_indices, const int32_t* qry_indptr)

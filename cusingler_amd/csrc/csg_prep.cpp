@@ -78,6 +78,7 @@ struct csg_prep
     std::vector<uint32_t> common_genes, thre_genes;  // ascending
     bool filtered = false;
     std::vector<int> qcol;  // query column -> shared column (removeCols), -1: dropped
+    std::vector<int> rcol;  // the same for the reference's columns
     // step results
     Coded                 cref, cqry;
     std::vector<uint32_t> ctdidx, ctdiff;
@@ -292,12 +293,18 @@ int csg_prep_train(int cores, uint32_t ref_height, uint32_t ref_width, const flo
                 qmap[i] = k++;
     }
     if (p->filtered)
+    {
         p->qcol = qmap;
+        p->rcol = rmap;
+    }
     else
     {
         p->qcol.resize(qry_width);
         for (uint32_t i = 0; i < qry_width; ++i)
             p->qcol[i] = (int)i;
+        p->rcol.resize(ref_width);
+        for (uint32_t i = 0; i < ref_width; ++i)
+            p->rcol[i] = (int)i;
     }
 
     // ---- loadRefData: groupbyCelltypes + resort (src/io.cpp:616-684)
@@ -546,6 +553,15 @@ uint64_t csg_prep_train_table(const csg_prep* p, int which, uint32_t* out)
 }
 
 int csg_prep_filtered(const csg_prep* p) { return p && p->filtered ? 1 : 0; }
+
+int csg_prep_ref_map(const csg_prep* p, int32_t* out)
+{
+    if (!p || !out)
+        return CSG_ERR_ARGUMENT;
+    for (uint32_t c = 0; c < p->ref_width; ++c)
+        out[c] = c < p->rcol.size() ? p->rcol[c] : -1;
+    return CSG_OK;
+}
 
 int csg_prep_query_map(const csg_prep* p, int step, int32_t* out)
 {

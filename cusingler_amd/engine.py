@@ -87,6 +87,10 @@ def lib():
     L.csg_code_csr_noalloc.argtypes = [_V, _U32, _U32] + [_V] * 12
     L.csg_synchronize.argtypes = [_V]
     L.csg_prefetch_query_raw.argtypes = [_V, _U32, _V, _V, _V]
+    L.csg_train.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _U32, _U32, _V, _V]
+    L.csg_train_table.restype = _U64
+    L.csg_train_table.argtypes = [_V, C.c_int, _V]
+    L.csg_prep_ref_map.argtypes = [_V, _V]
     L.csg_phase_times.argtypes = [_V, _V, _V, C.c_int]
     L.csg_set_phase_timing.argtypes = [_V, C.c_int]
     L.csg_counters.argtypes = [_V, _V]
@@ -206,6 +210,14 @@ class Prep:
     def filtered(self) -> bool:
         return bool(lib().csg_prep_filtered(self._h))
 
+    def ref_map(self, ref_width: int) -> np.ndarray:
+        """Per reference column, its shared column after removeCols or -1."""
+        out = np.zeros(ref_width, np.int32)
+        rc = lib().csg_prep_ref_map(self._h, _p(out))
+        if rc != 0:
+            raise CsgError(rc, "ref_map")
+        return out
+
     def query_map(self, step: int, qry_width: int) -> np.ndarray:
         """Per query column (as given to the training call), its column in
         step `step`'s gene set or -1 (the device value coder's map)."""
@@ -273,6 +285,26 @@ class Engine:
         self._check(lib().csg_copyin_query_raw(self._h, n, qry_width, _p(d), _p(i), _p(p), _p(ms),
                                                _p(mf)), "copyin_query_raw")
         self.Nq = self.fNq = n
+
+    def train(self, ds, ref_map: Optional[np.ndarray] = None) -> Dict[str, np.ndarray]:
+        """trainData on the device for the reference of dataset `ds`: returns
+        train_idxs, train_values, common_genes, thre_genes."""
+        d = np.ascontiguousarray(ds.ref_data, np.float32)
+        i = np.ascontiguousarray(ds.ref_indices, np.int32)
+        p = np.ascontiguousarray(ds.ref_indptr, np.int32)
+        ct = np.ascontiguousarray(ds.celltype, np.int32)
+        m = None if ref_map is None else np.ascontiguousarray(ref_map, np.int32)
+        sizes = np.zeros(3, np.uint64)
+        self._check(lib().csg_train(self._h, ds.ref_height, ds.ref_width, _p(d), _p(i), _p(p),
+                                    _p(m) if m is not None else None, ds.ref_width,
+                                    len(ds.codes), _p(ct), _p(sizes)), "train")
+        out = {}
+        for k, name in enumerate(["train_idxs", "train_values", "common_genes", "thre_genes"]):
+            n = int(lib().csg_train_table(self._h, k, None))
+            a = np.zeros(max(n, 1), np.uint32)
+            lib().csg_train_table(self._h, k, _p(a))
+            out[name] = a[:n]
+        return out
 
     def prefetch_query_raw(self, data: np.ndarray, indices: np.ndarray, indptr: np.ndarray):
         """Starts the upload of the next batch (the arrays must stay alive and

@@ -235,6 +235,29 @@ int csg_query_coded(csg_engine* eng, int step, uint16_t* data, int32_t* indices,
  * reference. */
 int csg_set_upload_slice(csg_engine* eng, uint64_t entries);
 
+/* trainData (src/io.cpp:141-262) on the device: per label and gene the
+ * median with implicit zeros, per ordered label pair the genes by descending
+ * rounded median difference (ties to the lower gene), the pair's positive
+ * genes (ref_train_idxs / ref_train_values), the top 333 positives of every
+ * pair (thre_genes) and the top k_pair(L) genes of every pair regardless of
+ * sign (common_genes) -- bit-identical to the reference's.  Input: the
+ * reference's float CSR as given to the pipeline (rows in any label order:
+ * the medians do not depend on it), its per-row label codes (narrowed to
+ * uint8), and ref_map (width entries: the column after removeCols, or -1;
+ * csg_prep_ref_map; NULL = identity).  ref_width is the reference's
+ * pre-intersection width, as in the reference (genes past the shared ones
+ * have median 0).  sizes (optional) receives [|ref_train_values|,
+ * |common_genes|, |thre_genes|]; csg_train_table copies them out. */
+int csg_train(csg_engine* eng, uint32_t ref_height, uint32_t ref_width, const float* ref_data,
+              const int32_t* ref_indices, const int32_t* ref_indptr, const int32_t* ref_map,
+              uint32_t width, uint32_t ct_num, const int32_t* celltype, uint64_t* sizes);
+
+/* Results of the last csg_train: which = 0 ref_train_idxs ([pos, len] per
+ * ordered pair, 2 ct_num^2), 1 ref_train_values, 2 common_genes, 3
+ * thre_genes (ascending).  Returns the element count; copies when out !=
+ * NULL. */
+uint64_t csg_train_table(const csg_engine* eng, int which, uint32_t* out);
+
 /* Starts the upload of the next query batch's values and columns on the
  * engine's second stream and returns at once, so that the upload overlaps the
  * score step and fine-tune of the batch before it (SURVEY's streaming

@@ -77,6 +77,13 @@ int csg_prep_filtered(const csg_prep* p);
  * query batch over the same genes.  Returns 0 or CSG_ERR_ARGUMENT. */
 int csg_prep_query_map(const csg_prep* p, int step, int32_t* out);
 
+/* Column map of the reference for training on the device (csg_train): for
+ * every reference column (ref_width entries) its shared column after
+ * removeCols (src/io.cpp:306-349,687-711), or -1 where the gene is not
+ * shared; the identity when both gene lists are equal.  Returns 0 or
+ * CSG_ERR_ARGUMENT. */
+int csg_prep_ref_map(const csg_prep* p, int32_t* out);
+
 void csg_prep_release(csg_prep* p);
 
 #ifdef __cplusplus

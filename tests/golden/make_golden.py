@@ -9,7 +9,7 @@ For a few small seeded datasets it stores the inputs and:
     candidate masks, per-label quantile scores, round counts) in the
     reference's FP32 order.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [name ...]
 """
 
 from __future__ import annotations
@@ -36,14 +36,30 @@ def datasets():
         "tiny_nan": synth.with_empty_rows(
             synth.tiny(n_ref_per_label=10, n_labels=3, n_genes=320, n_qry=30, seed=21),
             [3, 17], [5, 11]),
+        # every stored value shifted: explicit stored zeros (-1.0) and values
+        # of both signs (-1.5), which the medians and codes order around the
+        # implicit zeros
+        "tiny_zeros": shifted(synth.tiny(n_ref_per_label=8, n_labels=3, n_genes=320, n_qry=24,
+                                         seed=13), 1.0),
+        "tiny_signed": shifted(synth.tiny(n_ref_per_label=8, n_labels=4, n_genes=360, n_qry=24,
+                                          seed=17), 1.5),
     }
+
+
+def shifted(ds, by):
+    import dataclasses
+    return dataclasses.replace(ds, ref_data=(ds.ref_data - np.float32(by)).astype(np.float32),
+                               qry_data=(ds.qry_data - np.float32(by)).astype(np.float32))
 
 
 def main():
     if not O.ref_driver_available():
         raise SystemExit("oracle/_ref/ref_driver is missing: run `make -C oracle ref` where "
                          "/root/reference is present")
+    only = set(sys.argv[1:])  # dataset names to (re)write; all when none given
     for name, ds in datasets().items():
+        if only and name not in only:
+            continue
         ref = O.run_ref_driver(ds, f"/tmp/golden_{name}")
         orc = O.Oracle(ds, 4)
         first, mask, scores = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
