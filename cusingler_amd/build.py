@@ -99,8 +99,21 @@ def build_synth():
     return out
 
 
+def build_shim():
+    """The reference's device API over the C-ABI (shim/cusingler_shim.cpp)
+    and a driver that runs the reference's pipeline order through it."""
+    shim = os.path.join(PKG, "shim")
+    out = os.path.join(shim, "shim_driver")
+    tmp = out + ".tmp"
+    _run([CXX, "-std=c++17", "-O2", "-Wall", os.path.join(shim, "cusingler_shim.cpp"),
+          os.path.join(shim, "shim_driver.cpp"), "-o", tmp, "-L" + PKG, "-lcusingler_amd",
+          "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(force=True):
-    outs = [build_engine(force), build_pymodule(), build_synth()]
+    outs = [build_engine(force), build_pymodule(), build_synth(), build_shim()]
     return outs
 
 
