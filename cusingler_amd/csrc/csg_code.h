@@ -33,6 +33,7 @@ __device__ inline uint32_t code_key(float v)
 // row held at once); longer rows go to the block kernel.
 constexpr int CODE_WAVE_CAP = 512;
 constexpr int CODE_NT       = 256;  // threads of the coding kernels' blocks
+constexpr int CODE_UNROLL   = 4;    // groups of 64 entries a wave loads at once
 
 // Flags of a coding pass (bit 0: a NaN among the kept values, bit 1: a column
 // outside the query width, bit 2: a row with more distinct values than
@@ -64,16 +65,29 @@ __global__ __launch_bounds__(CODE_NT) void k_code_count(
         const int64_t beg = indptr[r] - base, end = indptr[r + 1] - base;
         int           ca = 0, cb = 0;
         uint32_t      fl = 0;
-        for (int64_t i = beg + lane; i < end; i += 64)
+        for (int64_t i0 = beg; i0 < end; i0 += 64 * CODE_UNROLL)
         {
-            const int32_t c = indices[i];
-            if (c < 0 || (uint32_t)c >= width)
+            int32_t c[CODE_UNROLL];
+#pragma unroll
+            for (int u = 0; u < CODE_UNROLL; ++u)
             {
-                fl |= CODE_BADCOL;
-                continue;
+                const int64_t i = i0 + u * 64 + lane;
+                c[u]            = i < end ? indices[i] : 0;
+                if (i < end && (c[u] < 0 || (uint32_t)c[u] >= width))
+                {
+                    fl |= CODE_BADCOL;
+                    c[u] = -1;
+                }
+                else if (i >= end)
+                    c[u] = -1;
             }
-            ca += map_a[c] >= 0;
-            cb += map_b[c] >= 0;
+#pragma unroll
+            for (int u = 0; u < CODE_UNROLL; ++u)
+                if (c[u] >= 0)
+                {
+                    ca += map_a[c[u]] >= 0;
+                    cb += map_b[c[u]] >= 0;
+                }
         }
         ca = wave_sum(ca);
         cb = wave_sum(cb);
@@ -185,8 +199,10 @@ __device__ inline uint32_t code_ranks(const uint64_t* keys, int n, int t, int nt
 }
 
 // Sorts keys[0, n) of one row and set in the wave's LDS and writes the
-// codes; returns the distinct count (every lane).
-__device__ inline uint32_t code_wave_row(uint64_t* keys, int n, int lane, uint16_t* out)
+// codes: scattered by slot into the wave's LDS row `codes`, then stored to
+// `out` in order (coalesced); returns the distinct count (every lane).
+__device__ inline uint32_t code_wave_row(uint64_t* keys, uint16_t* codes, int n, int lane,
+                                         uint16_t* out)
 {
     int n2 = 64;
     while (n2 < n)
@@ -195,8 +211,12 @@ __device__ inline uint32_t code_wave_row(uint64_t* keys, int n, int lane, uint16
         keys[k] = ~0ull;
     __builtin_amdgcn_wave_barrier();
     code_bitonic(keys, n2, lane, 64, [] { __builtin_amdgcn_wave_barrier(); });
-    const uint32_t d = code_ranks(keys, n, lane, 64, out, [](int v) { return wave_incl_scan(v); });
-    __builtin_amdgcn_wave_barrier();  // LDS reads done before the keys are rewritten
+    const uint32_t d = code_ranks(keys, n, lane, 64, codes,
+                                  [](int v) { return wave_incl_scan(v); });
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < n; k += 64)
+        out[k] = codes[k];
+    __builtin_amdgcn_wave_barrier();  // LDS reads done before keys and codes are rewritten
     return __builtin_amdgcn_readlane(d, 63);
 }
 
@@ -211,9 +231,11 @@ __global__ __launch_bounds__(CODE_NT) void k_code_rows(
     uint32_t* __restrict__ n_over, uint32_t* __restrict__ flags)
 {
     __shared__ uint64_t skeys[CODE_NT / 64][2][CODE_WAVE_CAP];
+    __shared__ uint16_t scodes[CODE_NT / 64][CODE_WAVE_CAP];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     uint64_t* ka = skeys[wid][0];
     uint64_t* kb = skeys[wid][1];
+    uint16_t* cw = scodes[wid];
     const uint32_t nw = gridDim.x * (CODE_NT / 64);
     const uint64_t below = (1ull << lane) - 1;
     uint32_t       mu[2] = { 1u, 1u };
@@ -233,54 +255,64 @@ __global__ __launch_bounds__(CODE_NT) void k_code_rows(
             continue;  // uniform per wave
         const int64_t beg = indptr[r] - base, end = indptr[r + 1] - base;
         const int64_t oa = sa.ptr[r], ob = sb.ptr[r];
-        // kept entries in row order: columns out, (key, slot) pairs to LDS
+        // kept entries in row order: columns out, (key, slot) pairs to LDS;
+        // CODE_UNROLL groups of 64 entries per pass, their column and map
+        // loads issued before any is used
         int slot_a = 0, slot_b = 0;
-        for (int64_t i0 = beg; i0 < end; i0 += 64)
+        for (int64_t i0 = beg; i0 < end; i0 += 64 * CODE_UNROLL)
         {
-            const int64_t i  = i0 + lane;
-            int32_t       ma = -1, mb = -1;
-            if (i < end)
+            int32_t c[CODE_UNROLL], ma[CODE_UNROLL], mb[CODE_UNROLL];
+#pragma unroll
+            for (int u = 0; u < CODE_UNROLL; ++u)
             {
-                const int32_t c = indices[i];
-                if (c >= 0 && (uint32_t)c < width)
-                {
-                    ma = wa ? sa.map[c] : -1;
-                    mb = wb ? sb.map[c] : -1;
-                }
+                const int64_t i = i0 + u * 64 + lane;
+                c[u]            = i < end ? indices[i] : -1;
             }
-            const uint64_t bala = __ballot(ma >= 0), balb = __ballot(mb >= 0);
-            if (ma >= 0 || mb >= 0)
+#pragma unroll
+            for (int u = 0; u < CODE_UNROLL; ++u)
             {
-                const float    v   = data[i];
-                const uint64_t key = (uint64_t)code_key(v) << 32;
-                if (v != v)
-                    fl |= CODE_NAN;
-                if (ma >= 0)
-                {
-                    const int k    = slot_a + __popcll(bala & below);
-                    sa.indices[oa + k] = ma;
-                    ka[k]          = key | (uint32_t)k;
-                }
-                if (mb >= 0)
-                {
-                    const int k    = slot_b + __popcll(balb & below);
-                    sb.indices[ob + k] = mb;
-                    kb[k]          = key | (uint32_t)k;
-                }
+                const bool ok = c[u] >= 0 && (uint32_t)c[u] < width;
+                ma[u]         = ok && wa ? sa.map[c[u]] : -1;
+                mb[u]         = ok && wb ? sb.map[c[u]] : -1;
             }
-            slot_a += __popcll(bala);
-            slot_b += __popcll(balb);
+#pragma unroll
+            for (int u = 0; u < CODE_UNROLL; ++u)
+            {
+                const int64_t  i    = i0 + u * 64 + lane;
+                const uint64_t bala = __ballot(ma[u] >= 0), balb = __ballot(mb[u] >= 0);
+                if (ma[u] >= 0 || mb[u] >= 0)
+                {
+                    const float    v   = data[i];
+                    const uint64_t key = (uint64_t)code_key(v) << 32;
+                    if (v != v)
+                        fl |= CODE_NAN;
+                    if (ma[u] >= 0)
+                    {
+                        const int k        = slot_a + __popcll(bala & below);
+                        sa.indices[oa + k] = ma[u];
+                        ka[k]              = key | (uint32_t)k;
+                    }
+                    if (mb[u] >= 0)
+                    {
+                        const int k        = slot_b + __popcll(balb & below);
+                        sb.indices[ob + k] = mb[u];
+                        kb[k]              = key | (uint32_t)k;
+                    }
+                }
+                slot_a += __popcll(bala);
+                slot_b += __popcll(balb);
+            }
         }
         if (wa)
         {
-            const uint32_t d = code_wave_row(ka, na, lane, sa.data + oa);
+            const uint32_t d = code_wave_row(ka, cw, na, lane, sa.data + oa);
             mu[0]            = max(mu[0], d + 1);
             if (d > 65535u)
                 fl |= CODE_OVERFLOW;
         }
         if (wb)
         {
-            const uint32_t d = code_wave_row(kb, nb, lane, sb.data + ob);
+            const uint32_t d = code_wave_row(kb, cw, nb, lane, sb.data + ob);
             mu[1]            = max(mu[1], d + 1);
             if (d > 65535u)
                 fl |= CODE_OVERFLOW;
@@ -288,10 +320,15 @@ __global__ __launch_bounds__(CODE_NT) void k_code_rows(
     }
     if (fl)
         atomicOr(flags, fl);
+    // one address takes every wave's maximum: read it first, so that waves
+    // whose maximum is not larger (nearly all, once it has grown) skip the
+    // serialising atomic
     if (lane == 0)
     {
-        atomicMax(&max_uniq[0], mu[0]);
-        atomicMax(&max_uniq[1], mu[1]);
+        if (mu[0] > __atomic_load_n(&max_uniq[0], __ATOMIC_RELAXED))
+            atomicMax(&max_uniq[0], mu[0]);
+        if (mu[1] > __atomic_load_n(&max_uniq[1], __ATOMIC_RELAXED))
+            atomicMax(&max_uniq[1], mu[1]);
     }
 }
 

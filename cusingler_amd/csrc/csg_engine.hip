@@ -3656,8 +3656,12 @@ void launch_coding(const csg_engine* eng, hipStream_t st, const float* data, con
     k_code_scan<<<1, 1024, 0, st>>>(sa.count, sb.count, nr, (int32_t*)sa.ptr, (int32_t*)sb.ptr,
                                     d_base);
     k_code_zero<<<1, 64, 0, st>>>(d_nover);
-    k_code_rows<<<g, CODE_NT, 0, st>>>(data, idx, ptr, base, nr, width, sa, sb, d_maxu, over,
-                                       d_nover, d_flags);
+    // rows strided over a grid of 8 blocks per CU: each wave codes many rows
+    // and reports its maximum once
+    const unsigned gr = (unsigned)std::max<uint32_t>(
+        1, std::min<uint32_t>((nr + 3) / 4, (uint32_t)std::max(1, eng->prop.multiProcessorCount) * 8));
+    k_code_rows<<<gr, CODE_NT, 0, st>>>(data, idx, ptr, base, nr, width, sa, sb, d_maxu, over,
+                                        d_nover, d_flags);
     k_code_rows_big<<<256, CODE_NT, (size_t)big_cap * 8, st>>>(data, idx, ptr, base, width, sa, sb,
                                                                over, d_nover, big_cap, d_maxu,
                                                                d_flags);
