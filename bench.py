@@ -73,32 +73,6 @@ def host_cpus():
     return cores, nproc, model
 
 
-class Pinned:
-    """Host arrays in pinned memory (hipHostMalloc), as a loader that reads
-    the query matrix for the GPU would hold them: uploads are direct DMA."""
-
-    def __init__(self):
-        self.hip = ctypes.CDLL("libamdhip64.so")
-        self.ptrs = []
-
-    def copy(self, a):
-        import numpy as np
-        a = np.ascontiguousarray(a)
-        p = ctypes.c_void_p()
-        if self.hip.hipHostMalloc(ctypes.byref(p), ctypes.c_size_t(max(a.nbytes, 1)), 0) != 0:
-            raise RuntimeError("hipHostMalloc failed")
-        self.ptrs.append(p)
-        buf = (ctypes.c_char * max(a.nbytes, 1)).from_address(p.value)
-        out = np.frombuffer(buf, dtype=a.dtype, count=a.size).reshape(a.shape)
-        out[...] = a
-        return out
-
-    def free(self):
-        for p in self.ptrs:
-            self.hip.hipHostFree(p)
-        self.ptrs = []
-
-
 def cpu_baseline(ds, n_sample: int, cores: int):
     """The oracle (a CPU restatement of the reference's score + fine-tune steps)
     on the first n_sample query cells."""
@@ -288,7 +262,7 @@ def main():
         training, coding and upload are one-time work, outside the window)."""
         path = "exact_int8" if form == "proposal" else form
         eng.set_proposal(args.proposal_k if form == "proposal" else 0, args.min_margin)
-        pin = Pinned()
+        pin = engine.Pinned()
         qd, qi = pin.copy(ds.qry_data), pin.copy(ds.qry_indices)
         qp = pin.copy(np.asarray(ds.qry_indptr, np.int32))
         m0, m1 = prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width)

@@ -4250,6 +4250,27 @@ int csg_query_coded(csg_engine* eng, int step, uint16_t* data, int32_t* indices,
     return CSG_OK;
 }
 
+int csg_host_alloc(uint64_t bytes, void** ptr)
+{
+    if (!ptr)
+        return CSG_ERR_ARGUMENT;
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return CSG_ERR_MEMORY;
+    }
+    return CSG_OK;
+}
+
+int csg_host_free(void* ptr)
+{
+    if (!ptr)
+        return CSG_OK;
+    return hipHostFree(ptr) == hipSuccess ? CSG_OK : CSG_ERR_DEVICE;
+}
+
 int csg_set_upload_slice(csg_engine* eng, uint64_t entries)
 {
     if (!eng || entries == 0)

@@ -86,6 +86,8 @@ def lib():
     L.csg_code_csr.argtypes = [_V, _U32, _U32] + [_V] * 12
     L.csg_code_csr_noalloc.argtypes = [_V, _U32, _U32] + [_V] * 12
     L.csg_synchronize.argtypes = [_V]
+    L.csg_host_alloc.argtypes = [_U64, P(_V)]
+    L.csg_host_free.argtypes = [_V]
     L.csg_prefetch_query_raw.argtypes = [_V, _U32, _V, _V, _V]
     L.csg_train.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _U32, _U32, _V, _V]
     L.csg_train_table.restype = _U64
@@ -226,6 +228,43 @@ class Prep:
         if rc != 0:
             raise CsgError(rc, "query_map")
         return out
+
+
+class Pinned:
+    """Page-locked host arrays from the library's HIP runtime (csg_host_alloc):
+    query batches read into them upload by DMA without a staging copy.  The
+    arrays are views of that memory; do not use them after free()."""
+
+    def __init__(self):
+        self._ptrs: List[_V] = []
+
+    def empty(self, n: int, dtype) -> np.ndarray:
+        dt = np.dtype(dtype)
+        nbytes = max(int(n) * dt.itemsize, 1)
+        p = _V()
+        rc = lib().csg_host_alloc(nbytes, C.byref(p))
+        if rc != 0 or not p.value:
+            raise CsgError(rc or 3, f"csg_host_alloc({nbytes}) failed")
+        self._ptrs.append(p)
+        buf = (C.c_char * nbytes).from_address(p.value)
+        return np.frombuffer(buf, dtype=dt, count=int(n))
+
+    def copy(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a)
+        out = self.empty(a.size, a.dtype).reshape(a.shape)
+        out[...] = a
+        return out
+
+    def free(self):
+        for p in self._ptrs:
+            lib().csg_host_free(p)
+        self._ptrs = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
 
 
 class Engine:

@@ -230,6 +230,14 @@ int csg_code_csr_noalloc(csg_engine* eng, uint32_t rows, uint32_t width, const f
  * indptr[0] entries.  No counterpart in the reference. */
 int csg_query_coded(csg_engine* eng, int step, uint16_t* data, int32_t* indices, int32_t* indptr);
 
+/* Page-locked host memory from the library's HIP runtime, for loaders that
+ * read query batches straight into the buffers csg_prefetch_query_raw /
+ * csg_copyin_query_raw upload (DMA without a staging copy).  csg_host_free
+ * takes only pointers csg_host_alloc returned.  No counterpart in the
+ * reference. */
+int csg_host_alloc(uint64_t bytes, void** ptr);
+int csg_host_free(void* ptr);
+
 /* Entries per upload slice of csg_copyin_query_raw (default 2^26; whole rows
  * per slice, a longer row is a slice of its own).  No counterpart in the
  * reference. */
