@@ -439,7 +439,7 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             w8_issue(qh, ql, rh, rl, src, sm + ((t + 2) % W8_NS) * W8_STAGE, wid,
                      (t + 2) * W8_BK);
         const int8_t* s = sm + (t % W8_NS) * W8_STAGE;
-        v4i ah[4], al[4], ah16[4];
+        v4i ah[4], al[4], ah16[4], bh[4], bl[4], bh16[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
         {
@@ -447,25 +447,40 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             const int o   = row * W8_BK + ((kc ^ w8_swz(row)) << 4);
             ah[i]         = *(const v4i*)(s + o);
             al[i]         = *(const v4i*)(s + W8_BM * W8_BK + o);
-            ah16[i]       = times16(ah[i]);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
-            const int row  = wn * 64 + j * 16 + r16;
-            const int o    = row * W8_BK + ((kc ^ w8_swz(row)) << 4);
-            const v4i bh   = *(const v4i*)(s + 2 * W8_BM * W8_BK + o);
-            const v4i bl   = *(const v4i*)(s + (2 * W8_BM + W8_BN) * W8_BK + o);
-            const v4i bh16 = times16(bh);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-            {
-                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bl, acc_x[i][j], 0, 0, 0);
-                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bh, acc_x[i][j], 0, 0, 0);
-                acc_x[i][j]  = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah16[i], bh16, acc_x[i][j], 0, 0, 0);
-                acc_ll[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bl, acc_ll[i][j], 0, 0, 0);
-            }
+            const int row = wn * 64 + j * 16 + r16;
+            const int o   = row * W8_BK + ((kc ^ w8_swz(row)) << 4);
+            bh[j]         = *(const v4i*)(s + 2 * W8_BM * W8_BK + o);
+            bl[j]         = *(const v4i*)(s + (2 * W8_BM + W8_BN) * W8_BK + o);
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+            ah16[i] = times16(ah[i]);
+            bh16[i] = times16(bh[i]);
+        }
+        // product by product over all 16 output tiles: an accumulator's
+        // successive MFMAs are 16 issues apart, so none waits for the last
+        // (integer sums: the order does not change the result)
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                {
+                    if (p == 0)
+                        acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bl[j], acc_x[i][j], 0, 0, 0);
+                    else if (p == 1)
+                        acc_ll[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bl[j], acc_ll[i][j], 0, 0, 0);
+                    else if (p == 2)
+                        acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bh[j], acc_x[i][j], 0, 0, 0);
+                    else
+                        acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah16[i], bh16[j], acc_x[i][j], 0, 0, 0);
+                }
     }
     double* s_ir = (double*)(sm + W8_IR_OFF);
     if ((lane & 3) == 0)
