@@ -609,7 +609,7 @@ int csg_init_gpu(int gpuid, csg_engine** out)
         bool       ok3  = true;
         for (const void* f : { (const void*)k_ftd_rank_ref, (const void*)k_ftd_rank_qry,
                                (const void*)k_ftd_corr, (const void*)k_ft_rank_ref,
-                               (const void*)k_code_rows_big })
+                               (const void*)k_code_rows_big, (const void*)k_ftd_setup })
             ok3 = ok3 && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              want) == hipSuccess;
         if (ok1 && ok2 && ok3)
@@ -2468,7 +2468,8 @@ FtdLds ftd_lds(const csg_engine* eng)
     const size_t st = round_up(std::max<uint32_t>(eng->Gf, 1), 8);
     const size_t nb = (size_t)((eng->fnbins + 3) & ~3) * 4;
     FtdLds       d;
-    d.setup    = ((size_t)std::max<uint32_t>(eng->Gf, 1) + 31) / 32 * 4;
+    // bit words and a flag byte per gene (the flags padded to whole words)
+    d.setup    = ((size_t)std::max<uint32_t>(eng->Gf, 1) + 31) / 32 * (4 + 32);
     d.rank_ref = nb + st * 4;
     d.rank_qry = nb + st * 2 + st * 2;
     // the correlation tiles walk K in LDS stages, so any gene subset fits
@@ -3106,7 +3107,9 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
     const uint32_t Nq = eng->fNq, L = eng->fL;
     int            rc;
     const FtdLds   d = ftd_lds(eng);
-    if (d.rank_qry > eng->lds_limit || d.setup > 65536 || d.rank_ref > eng->lds_limit)
+    // (the set-up kernel's static LDS is about 5.5 KB)
+    if (d.rank_qry > eng->lds_limit || d.setup + 8192 > std::max<size_t>(eng->lds_limit, 65536) ||
+        d.rank_ref > eng->lds_limit)
         return eng->fail(CSG_ERR_ARGUMENT, "fine-tune gene set of " + std::to_string(eng->Gf) +
                                                " genes with " + std::to_string(eng->fnbins) +
                                                " value codes exceeds the rank kernels' LDS");

@@ -354,9 +354,11 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
     uint32_t* __restrict__ glabels,
     uint32_t* __restrict__ glpos, uint32_t* __restrict__ ggenes)
 {
-    extern __shared__ uint32_t sbits[];  // ceil(Gf / 32) words
+    // dynamic LDS: ceil(Gf / 32) bit words, then one flag byte per gene
+    extern __shared__ uint32_t sbits[];
     __shared__ uint32_t slab[256];
     __shared__ uint32_t spart[FTD_SETUP_NT];
+    __shared__ uint8_t  sflag[256];
     __shared__ uint32_t sk, snref;
     const uint32_t ng  = cnt->n_groups;
     const uint32_t W   = (Gf + 31) / 32;
@@ -364,35 +366,59 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
     for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x)
     {
         __syncthreads();  // the previous group's LDS is no longer read
-        if (tid == 0)
+        if (tid < 64)
         {
-            const uint32_t o = gowner[g];
-            uint64_t       key[4];
-            ftd_key(lists + (size_t)o * L, counts[o], key);
-            uint32_t k = 0, nref = 0;
+            // the owner's candidate labels in ascending order, by the first
+            // wave: membership flags, one ballot per 64 labels, ranks from
+            // the ballot's lower bits, row offsets from a wave scan of the
+            // labels' sizes (the loads of list entries and sizes in parallel)
+            const uint32_t o   = gowner[g];
+            const uint32_t c   = counts[o];
+            const uint8_t* lst = lists + (size_t)o * L;
+#pragma unroll
+            for (uint32_t w = 0; w < 4; ++w)
+                sflag[w * 64 + tid] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t t = tid; t < c; t += 64)
+                sflag[lst[t]] = 1;
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const uint64_t below = (1ull << tid) - 1ull;
+            uint32_t       k = 0, nref = 0;
+#pragma unroll
             for (uint32_t w = 0; w < 4; ++w)
             {
-                uint64_t v = key[w];
-                while (v)
+                const uint32_t l  = w * 64 + tid;
+                const bool     in = l < L && sflag[l];
+                const uint64_t m  = __ballot(in);
+                const uint32_t n  = in ? llen[l] : 0u;
+                const int      ex = wave_incl_scan((int)n) - (int)n;
+                if (in)
                 {
-                    const uint32_t l = w * 64 + (uint32_t)__builtin_ctzll(v);
-                    v &= v - 1;
-                    slab[k]                       = l;
-                    glabels[(size_t)g * L + k]    = l;
-                    glpos[(size_t)g * L + k]      = nref;
-                    nref += llen[l];
-                    ++k;
+                    const uint32_t r            = k + (uint32_t)__builtin_popcountll(m & below);
+                    slab[r]                     = l;
+                    glabels[(size_t)g * L + r]  = l;
+                    glpos[(size_t)g * L + r]    = nref + (uint32_t)ex;
                 }
+                k += (uint32_t)__builtin_popcountll(m);
+                nref += (uint32_t)__builtin_amdgcn_readlane(ex + (int)n, 63);
             }
-            sk    = k;
-            snref = nref;
+            if (tid == 0)
+            {
+                sk    = k;
+                snref = nref;
+            }
         }
-        for (uint32_t w = tid; w < W; w += FTD_SETUP_NT)
-            sbits[w] = 0u;
+        uint8_t* gflags = (uint8_t*)(sbits + W);
+        for (uint32_t w = tid; w < W * 8; w += FTD_SETUP_NT)
+            ((uint32_t*)gflags)[w] = 0u;
         __syncthreads();
         const uint32_t k   = sk;
         const uint32_t cut = k ? cut_tab[k] : 0u;
-        // ordered pairs spread over the waves, genes over the lanes
+        // ordered pairs spread over the waves, genes over the lanes: a flag
+        // byte per gene (plain stores of the same value, no atomics on the
+        // few words a small gene set has)
         for (uint32_t pi = tid >> 6; pi < k * k; pi += FTD_SETUP_NT / 64)
         {
             const uint32_t i = slab[pi / k], j = slab[pi % k];
@@ -401,10 +427,20 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
             const uint32_t pos = ctdidx[((size_t)i * L + j) * 2];
             const uint32_t len = min(ctdidx[((size_t)i * L + j) * 2 + 1], cut);
             for (uint32_t t = tid & 63; t < len; t += 64)
+                gflags[ctdiff[pos + t]] = 1;
+        }
+        __syncthreads();
+        for (uint32_t w = tid; w < W; w += FTD_SETUP_NT)
+        {
+            const uint32_t* f4 = (const uint32_t*)(gflags + w * 32);
+            uint32_t        v  = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
             {
-                const uint32_t gene = ctdiff[pos + t];
-                atomicOr(&sbits[gene >> 5], 1u << (gene & 31));
+                const uint32_t f = f4[q];  // flags of genes 32 w + 4 q .. + 3, one per byte
+                v |= ((f & 1u) | ((f >> 7) & 2u) | ((f >> 14) & 4u) | ((f >> 21) & 8u)) << (4 * q);
             }
+            sbits[w] = v;
         }
         __syncthreads();
         // ascending gene list: contiguous word ranges per thread, block scan
