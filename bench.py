@@ -254,16 +254,20 @@ def main():
         return dict(value=Nq * args.steps / dt, ms=dt * 1e3 / args.steps, phases=ph, phases1=ph1,
                     counters=cnt, first=first, final=final)
 
-    def measure_e2e(form, steps, warmup, pipelined):
+    def measure_e2e(form, steps, warmup, pipelined, narrow=False):
         """Query CSR on the host (float values, pinned) -> first and final labels
         on the host: the upload and device value coding of the query
         (csg_copyin_query_raw, both gene sets), the score step, the fine-tune and
         both label read-backs, per step.  The reference stays loaded (its
-        training, coding and upload are one-time work, outside the window)."""
+        training, coding and upload are one-time work, outside the window).
+        `narrow`: the query's column indices held as uint16, as the h5ad
+        reader produces them for widths up to 65536 (6 instead of 8 bytes per
+        stored value cross PCIe)."""
         path = "exact_int8" if form == "proposal" else form
         eng.set_proposal(args.proposal_k if form == "proposal" else 0, args.min_margin)
         pin = engine.Pinned()
-        qd, qi = pin.copy(ds.qry_data), pin.copy(ds.qry_indices)
+        qd = pin.copy(ds.qry_data)
+        qi = pin.copy(np.asarray(ds.qry_indices, np.uint16 if narrow else np.int32))
         qp = pin.copy(np.asarray(ds.qry_indptr, np.int32))
         m0, m1 = prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width)
 
@@ -299,12 +303,14 @@ def main():
         pin.free()
         nnz = int(ds.qry_indptr[-1])
         up_ms = ph["query_coding"][0]
+        qbytes = (6 if narrow else 8) * nnz + 4 * (Nq + 1)
         return {"value": round(Nq * steps / dt, 1), "unit": "cells/s", "form": form,
                 "ms_per_step": round(dt * 1e3 / steps, 3), "steps": steps,
                 "pipelined": pipelined,
+                "column_indices": "uint16" if narrow else "int32",
                 "query_upload_and_coding_ms": round(up_ms, 3),
-                "query_bytes": 8 * nnz + 4 * (Nq + 1),
-                "upload_and_coding_GBps": round((8 * nnz + 4 * (Nq + 1)) / (up_ms * 1e-3) / 1e9, 1)
+                "query_bytes": qbytes,
+                "upload_and_coding_GBps": round(qbytes / (up_ms * 1e-3) / 1e9, 1)
                 if up_ms > 0 else None,
                 "labels_equal_device_resident": bool(
                     np.array_equal(first, res[form]["first"]) and
@@ -355,6 +361,9 @@ def main():
         e2e = {f: measure_e2e(f, max(1, args.steps // 2), 1, True)
                for f in ([chosen] if args.only_chosen else [chosen, "plain"])}
         e2e["serial_" + chosen] = measure_e2e(chosen, max(1, args.steps // 2), 1, False)
+        if ds.qry_width <= 65536:
+            e2e["uint16_columns_" + chosen] = measure_e2e(chosen, max(1, args.steps // 2), 1, True,
+                                                          narrow=True)
         e2e["host_coding_" + chosen] = measure_host_coding(chosen, 2)
     r = res[chosen]
     steps = args.steps

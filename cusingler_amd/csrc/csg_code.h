@@ -51,9 +51,11 @@ struct CodeSet
 
 // Kept entries per row and set.  Rows [0, nrows) of the CSR given by
 // (data, indices, indptr) with indptr relative to `base` (the first entry of
-// row 0 sits at data[0]).
+// row 0 sits at data[0]).  Column indices are int32, or uint16 for batches
+// uploaded in that narrower form (query widths up to 65536).
+template <class IT>
 __global__ __launch_bounds__(CODE_NT) void k_code_count(
-    const int32_t* __restrict__ indices,
+    const IT* __restrict__ indices,
     const int32_t* __restrict__ indptr, int64_t base, uint32_t nrows, uint32_t width,
     const int32_t* __restrict__ map_a, const int32_t* __restrict__ map_b,
     uint32_t* __restrict__ cnt_a, uint32_t* __restrict__ cnt_b, uint32_t* __restrict__ flags)
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(CODE_NT) void k_code_count(
             for (int u = 0; u < CODE_UNROLL; ++u)
             {
                 const int64_t i = i0 + u * 64 + lane;
-                c[u]            = i < end ? indices[i] : 0;
+                c[u]            = i < end ? (int32_t)indices[i] : 0;
                 if (i < end && (c[u] < 0 || (uint32_t)c[u] >= width))
                 {
                     fl |= CODE_BADCOL;
@@ -224,8 +226,9 @@ __device__ inline uint32_t code_wave_row(uint64_t* keys, uint16_t* codes, int n,
 // read once for both.  `over` lists the (row, set) pairs -- set in bit 31 --
 // whose kept entries exceed the wave's LDS; a NaN among kept values raises
 // CODE_NAN (the reference orders only the kept values).
+template <class IT>
 __global__ __launch_bounds__(CODE_NT) void k_code_rows(
-    const float* __restrict__ data, const int32_t* __restrict__ indices,
+    const float* __restrict__ data, const IT* __restrict__ indices,
     const int32_t* __restrict__ indptr, int64_t base, uint32_t nrows, uint32_t width, CodeSet sa,
     CodeSet sb, uint32_t* __restrict__ max_uniq, uint32_t* __restrict__ over,
     uint32_t* __restrict__ n_over, uint32_t* __restrict__ flags)
@@ -266,7 +269,7 @@ __global__ __launch_bounds__(CODE_NT) void k_code_rows(
             for (int u = 0; u < CODE_UNROLL; ++u)
             {
                 const int64_t i = i0 + u * 64 + lane;
-                c[u]            = i < end ? indices[i] : -1;
+                c[u]            = i < end ? (int32_t)indices[i] : -1;
             }
 #pragma unroll
             for (int u = 0; u < CODE_UNROLL; ++u)
@@ -349,8 +352,9 @@ __device__ inline int code_block_scan(int v, int* red)
 
 // One block per listed row (set in bit 31) with more kept entries than a
 // wave holds; `cap` keys of dynamic LDS.
+template <class IT>
 __global__ __launch_bounds__(CODE_NT) void k_code_rows_big(
-    const float* __restrict__ data, const int32_t* __restrict__ indices,
+    const float* __restrict__ data, const IT* __restrict__ indices,
     const int32_t* __restrict__ indptr, int64_t base, uint32_t width, CodeSet sa, CodeSet sb,
     const uint32_t* __restrict__ over, const uint32_t* __restrict__ n_over, int cap,
     uint32_t* __restrict__ max_uniq, uint32_t* __restrict__ flags)
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(CODE_NT) void k_code_rows_big(
             int32_t       m = -1;
             if (i < end)
             {
-                const int32_t c = indices[i];
+                const int32_t c = (int32_t)indices[i];
                 m               = c >= 0 && (uint32_t)c < width ? cs.map[c] : -1;
             }
             const int kept = m >= 0;

@@ -128,6 +128,7 @@ struct csg_engine
         uint64_t    seq    = 0;
         const void *h_data = nullptr, *h_idx = nullptr, *h_ptr = nullptr;
         uint32_t    rows   = 0;
+        int         idx_bytes = 4;  // column index width of the staged batch
     } pre[2];
     uint64_t pre_seq = 0;
     // results of the last csg_train
@@ -609,7 +610,8 @@ int csg_init_gpu(int gpuid, csg_engine** out)
         bool       ok3  = true;
         for (const void* f : { (const void*)k_ftd_rank_ref, (const void*)k_ftd_rank_qry,
                                (const void*)k_ftd_corr, (const void*)k_ft_rank_ref,
-                               (const void*)k_code_rows_big, (const void*)k_ftd_setup })
+                               (const void*)k_code_rows_big<int32_t>,
+                               (const void*)k_code_rows_big<uint16_t>, (const void*)k_ftd_setup })
             ok3 = ok3 && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              want) == hipSuccess;
         if (ok1 && ok2 && ok3)
@@ -3643,7 +3645,9 @@ int ensure_copy_stream(csg_engine* eng)
 // Scratch: cnt_a / cnt_b (nr each), over (2 nr), misc (32 bytes, cleared by
 // the caller; its running bases continue from earlier slices).  No
 // allocation, no synchronisation.
-void launch_coding(const csg_engine* eng, hipStream_t st, const float* data, const int32_t* idx,
+extern "C++" {
+template <class IT>
+void launch_coding(const csg_engine* eng, hipStream_t st, const float* data, const IT* idx,
                    const int32_t* ptr, int64_t base, uint32_t nr, uint32_t width,
                    const CodeSet& sa, const CodeSet& sb, uint32_t* over, uint8_t* misc)
 {
@@ -3654,7 +3658,7 @@ void launch_coding(const csg_engine* eng, hipStream_t st, const float* data, con
     const int big_cap = (int)std::min<size_t>(eng->lds_limit / 8, (size_t)1 << 20);
     // one wave per row (rows strided over the grid beyond 2^20 blocks)
     const unsigned g  = (unsigned)std::max<uint32_t>(1, std::min<uint32_t>((nr + 3) / 4, 1u << 20));
-    k_code_count<<<g, CODE_NT, 0, st>>>(idx, ptr, base, nr, width, sa.map, sb.map, sa.count,
+    k_code_count<IT><<<g, CODE_NT, 0, st>>>(idx, ptr, base, nr, width, sa.map, sb.map, sa.count,
                                         sb.count, d_flags);
     k_code_scan<<<1, 1024, 0, st>>>(sa.count, sb.count, nr, (int32_t*)sa.ptr, (int32_t*)sb.ptr,
                                     d_base);
@@ -3663,12 +3667,13 @@ void launch_coding(const csg_engine* eng, hipStream_t st, const float* data, con
     // and reports its maximum once
     const unsigned gr = (unsigned)std::max<uint32_t>(
         1, std::min<uint32_t>((nr + 3) / 4, (uint32_t)std::max(1, eng->prop.multiProcessorCount) * 8));
-    k_code_rows<<<gr, CODE_NT, 0, st>>>(data, idx, ptr, base, nr, width, sa, sb, d_maxu, over,
+    k_code_rows<IT><<<gr, CODE_NT, 0, st>>>(data, idx, ptr, base, nr, width, sa, sb, d_maxu, over,
                                         d_nover, d_flags);
-    k_code_rows_big<<<256, CODE_NT, (size_t)big_cap * 8, st>>>(data, idx, ptr, base, width, sa, sb,
+    k_code_rows_big<IT><<<256, CODE_NT, (size_t)big_cap * 8, st>>>(data, idx, ptr, base, width, sa, sb,
                                                                over, d_nover, big_cap, d_maxu,
                                                                d_flags);
 }
+}  // extern "C++"
 
 // Status of a finished coding (misc read back): CSG_OK or the error.
 int coding_status(csg_engine* eng, const uint32_t* misc)
@@ -3891,8 +3896,10 @@ uint64_t csg_train_table(const csg_engine* eng, int which, uint32_t* out)
     return v->size();
 }
 
-int csg_prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qry_data,
-                           const int32_t* qry_indices, const int32_t* qry_indptr)
+extern "C++" {
+template <class IT>
+int prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qry_data,
+                       const IT* qry_indices, const int32_t* qry_indptr)
 {
     if (!eng)
         return CSG_ERR_ARGUMENT;
@@ -3929,10 +3936,11 @@ int csg_prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qr
     {
         HIPCHK(hipMemcpyAsync(ps->d_data, qry_data, nnz * 4, hipMemcpyHostToDevice,
                               eng->copy_stream));
-        HIPCHK(hipMemcpyAsync(ps->d_idx, qry_indices, nnz * 4, hipMemcpyHostToDevice,
+        HIPCHK(hipMemcpyAsync(ps->d_idx, qry_indices, nnz * sizeof(IT), hipMemcpyHostToDevice,
                               eng->copy_stream));
     }
     HIPCHK(hipEventRecord(ps->copied, eng->copy_stream));
+    ps->idx_bytes = (int)sizeof(IT);
     ps->valid  = true;
     ps->seq    = ++eng->pre_seq;
     ps->h_data = qry_data;
@@ -3941,11 +3949,13 @@ int csg_prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qr
     ps->rows   = qry_height;
     return CSG_OK;
 }
+}  // extern "C++"
 
-int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
-                         const float* qry_data, const int32_t* qry_indices,
-                         const int32_t* qry_indptr, const int32_t* map_score,
-                         const int32_t* map_finetune)
+extern "C++" {
+template <class IT>
+int copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
+                     const float* qry_data, const IT* qry_indices, const int32_t* qry_indptr,
+                     const int32_t* map_score, const int32_t* map_finetune)
 {
     if (!eng)
         return CSG_ERR_ARGUMENT;
@@ -3981,6 +3991,7 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
     for (auto& ps : eng->pre)
         if (ps.valid && ps.h_data == (const void*)qry_data && ps.h_idx == (const void*)qry_indices &&
             ps.h_ptr == (const void*)qry_indptr && ps.rows == qry_height &&
+            ps.idx_bytes == (int)sizeof(IT) &&
             (!slot || ps.seq < slot->seq))
             slot = &ps;
     const bool staged = slot != nullptr;
@@ -4090,13 +4101,13 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
             const uint32_t r0   = cut[i], r1 = cut[i + 1], nr = r1 - r0;
             const int64_t  e0   = qry_indptr[r0];
             const size_t   n    = (size_t)(qry_indptr[r1] - e0);
-            const float*   src_data = eng->d_raw_data[b];
-            const int32_t* src_idx  = eng->d_raw_idx[b];
+            const float* src_data = eng->d_raw_data[b];
+            const IT*    src_idx  = (const IT*)eng->d_raw_idx[b];
             if (staged)
             {
                 HIPCHK(hipStreamWaitEvent(eng->stream, slot->copied, 0));
                 src_data = slot->d_data;
-                src_idx  = slot->d_idx;
+                src_idx  = (const IT*)slot->d_idx;
             }
             else
             {
@@ -4106,8 +4117,9 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
                 {
                     HIPCHK(hipMemcpyAsync(eng->d_raw_data[b], qry_data + (e0 - base0), n * 4,
                                           hipMemcpyHostToDevice, eng->copy_stream));
-                    HIPCHK(hipMemcpyAsync(eng->d_raw_idx[b], qry_indices + (e0 - base0), n * 4,
-                                          hipMemcpyHostToDevice, eng->copy_stream));
+                    HIPCHK(hipMemcpyAsync(eng->d_raw_idx[b], qry_indices + (e0 - base0),
+                                          n * sizeof(IT), hipMemcpyHostToDevice,
+                                          eng->copy_stream));
                 }
                 HIPCHK(hipEventRecord(eng->code_copied[b], eng->copy_stream));
                 HIPCHK(hipStreamWaitEvent(eng->stream, eng->code_copied[b], 0));
@@ -4146,6 +4158,40 @@ int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_widt
     eng->host_mask.clear();
     eng->q_ready = eng->fq_ready = true;
     return CSG_OK;
+}
+}  // extern "C++"
+
+int csg_prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qry_data,
+                           const int32_t* qry_indices, const int32_t* qry_indptr)
+{
+    return prefetch_query_raw(eng, qry_height, qry_data, qry_indices, qry_indptr);
+}
+
+int csg_prefetch_query_raw_u16(csg_engine* eng, uint32_t qry_height, const float* qry_data,
+                               const uint16_t* qry_indices, const int32_t* qry_indptr)
+{
+    return prefetch_query_raw(eng, qry_height, qry_data, qry_indices, qry_indptr);
+}
+
+int csg_copyin_query_raw(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
+                         const float* qry_data, const int32_t* qry_indices,
+                         const int32_t* qry_indptr, const int32_t* map_score,
+                         const int32_t* map_finetune)
+{
+    return copyin_query_raw(eng, qry_height, qry_width, qry_data, qry_indices, qry_indptr,
+                            map_score, map_finetune);
+}
+
+int csg_copyin_query_raw_u16(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
+                             const float* qry_data, const uint16_t* qry_indices,
+                             const int32_t* qry_indptr, const int32_t* map_score,
+                             const int32_t* map_finetune)
+{
+    if (eng && qry_width > 65536)
+        return eng->fail(CSG_ERR_ARGUMENT, "16-bit column indices need a query width of at "
+                                           "most 65536");
+    return copyin_query_raw(eng, qry_height, qry_width, qry_data, qry_indices, qry_indptr,
+                            map_score, map_finetune);
 }
 
 int csg_code_csr(csg_engine* eng, uint32_t rows, uint32_t width, const float* data,

@@ -80,6 +80,8 @@ def lib():
     L.csg_set_finetune_fuse.argtypes = [_V, C.c_int]
     L.csg_set_query_chunk.argtypes = [_V, _U32, _V]
     L.csg_copyin_query_raw.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _V]
+    L.csg_copyin_query_raw_u16.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _V]
+    L.csg_prefetch_query_raw_u16.argtypes = [_V, _U32, _V, _V, _V]
     L.csg_prep_query_map.argtypes = [_V, C.c_int, _V]
     L.csg_query_coded.argtypes = [_V, C.c_int, _V, _V, _V]
     L.csg_set_upload_slice.argtypes = [_V, _U64]
@@ -314,15 +316,18 @@ class Engine:
     def copyin_query_raw(self, data: np.ndarray, indices: np.ndarray, indptr: np.ndarray,
                          qry_width: int, map_score: np.ndarray, map_finetune: np.ndarray):
         """A batch of float CSR query rows, value-coded on the device for both
-        steps (replaces the loaded query)."""
+        steps (replaces the loaded query).  Column indices in uint16 (query
+        widths up to 65536) take the 16-bit entry point; others are int32."""
         d = np.ascontiguousarray(data, np.float32)
-        i = np.ascontiguousarray(indices, np.int32)
+        u16 = isinstance(indices, np.ndarray) and indices.dtype == np.uint16
+        i = np.ascontiguousarray(indices, np.uint16 if u16 else np.int32)
         p = np.ascontiguousarray(indptr, np.int32)
         ms = np.ascontiguousarray(map_score, np.int32)
         mf = np.ascontiguousarray(map_finetune, np.int32)
         n = p.shape[0] - 1
-        self._check(lib().csg_copyin_query_raw(self._h, n, qry_width, _p(d), _p(i), _p(p), _p(ms),
-                                               _p(mf)), "copyin_query_raw")
+        f = lib().csg_copyin_query_raw_u16 if u16 else lib().csg_copyin_query_raw
+        self._check(f(self._h, n, qry_width, _p(d), _p(i), _p(p), _p(ms), _p(mf)),
+                    "copyin_query_raw")
         self.Nq = self.fNq = n
 
     def train(self, ds, ref_map: Optional[np.ndarray] = None) -> Dict[str, np.ndarray]:
@@ -348,12 +353,15 @@ class Engine:
     def prefetch_query_raw(self, data: np.ndarray, indices: np.ndarray, indptr: np.ndarray):
         """Starts the upload of the next batch (the arrays must stay alive and
         unchanged until copyin_query_raw consumes them; pass the same arrays)."""
-        for a, dt in ((data, np.float32), (indices, np.int32), (indptr, np.int32)):
+        u16 = indices.dtype == np.uint16
+        for a, dt in ((data, np.float32), (indices, np.uint16 if u16 else np.int32),
+                      (indptr, np.int32)):
             if a.dtype != dt or not a.flags.c_contiguous:
-                raise ValueError("prefetch needs contiguous float32 / int32 arrays (the same "
-                                 "objects copyin_query_raw will get)")
-        self._check(lib().csg_prefetch_query_raw(self._h, indptr.shape[0] - 1, _p(data),
-                                                 _p(indices), _p(indptr)), "prefetch_query_raw")
+                raise ValueError("prefetch needs contiguous float32 / int32 (or uint16 column) "
+                                 "arrays (the same objects copyin_query_raw will get)")
+        f = lib().csg_prefetch_query_raw_u16 if u16 else lib().csg_prefetch_query_raw
+        self._check(f(self._h, indptr.shape[0] - 1, _p(data), _p(indices), _p(indptr)),
+                    "prefetch_query_raw")
 
     def query_coded(self, step: int):
         """The query's value-coded CSR of a step as it sits on the device:

@@ -67,6 +67,11 @@ class H5Error(RuntimeError):
     pass
 
 
+# H5T_conv_except_func_t returning H5T_CONV_ABORT (-1) for every exception
+_CONV_CB = C.CFUNCTYPE(C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p)
+_ABORT_CB = _CONV_CB(lambda *args: -1)
+
+
 def _candidates() -> List[str]:
     out = []
     env = os.environ.get("CSG_HDF5_LIB")
@@ -151,6 +156,7 @@ def hdf5():
         "H5Pcreate": (_hid, [_hid]),
         "H5Pset_chunk": (_herr, [_hid, C.c_int, _P(_hsize)]),
         "H5Pset_deflate": (_herr, [_hid, C.c_uint]),
+        "H5Pset_type_conv_cb": (_herr, [_hid, C.c_void_p, C.c_void_p]),
         "H5Pclose": (_herr, [_hid]),
     }
     for name, (res, args) in proto.items():
@@ -170,6 +176,11 @@ def hdf5():
     }
     lib.c_s1 = g("H5T_C_S1_g")
     lib.dcpl_class = g("H5P_CLS_DATASET_CREATE_ID_g")
+    # a transfer property list whose type conversions fail on any value
+    # outside the destination type (the library clamps them by default)
+    lib.strict_dxpl = lib.H5Pcreate(g("H5P_CLS_DATASET_XFER_ID_g"))
+    _ok(lib.H5Pset_type_conv_cb(lib.strict_dxpl, C.cast(_ABORT_CB, C.c_void_p), None),
+        "H5Pset_type_conv_cb")
     _H5 = lib
     return lib
 
@@ -297,11 +308,13 @@ class H5File:
             self.h.H5Dclose(did)
 
     def read(self, path: str, start: int = 0, stop: Optional[int] = None, dtype=None,
-             out: Optional[np.ndarray] = None) -> np.ndarray:
+             out: Optional[np.ndarray] = None, strict: bool = False) -> np.ndarray:
         """Elements [start, stop) of a 1-D numeric dataset (the whole of an n-D
         one when start = 0 and stop is None), converted by the library to
         `dtype` (default: the file's type without loss) and written into `out`
-        when given (a contiguous array of that dtype and length)."""
+        when given (a contiguous array of that dtype and length).  `strict`:
+        a value outside `dtype`'s range fails the read (the library would
+        clamp it)."""
         h = self.h
         did = self._open_dataset(path)
         fsp = h.H5Dget_space(did)
@@ -333,19 +346,22 @@ class H5File:
                 raise ValueError(f"{path}: `out` must be a contiguous {dt} array of {n} elements")
             if n == 0:
                 return out
+            xfer = h.strict_dxpl if strict else _P_DEFAULT
             if msp is None:
                 st, cnt = (_hsize * 1)(start), (_hsize * 1)(n)
                 _ok(h.H5Sselect_hyperslab(fsp, _S_SELECT_SET, st, None, cnt, None),
                     "H5Sselect_hyperslab")
                 msp = h.H5Screate_simple(1, cnt, None)
                 try:
-                    _ok(h.H5Dread(did, h.native[dt], msp, fsel, _P_DEFAULT,
-                                  out.ctypes.data_as(C.c_void_p)), f"read {path}")
+                    _ok(h.H5Dread(did, h.native[dt], msp, fsel, xfer,
+                                  out.ctypes.data_as(C.c_void_p)),
+                        f"read {path}" + (f" (a value outside {dt})" if strict else ""))
                 finally:
                     h.H5Sclose(msp)
             else:
-                _ok(h.H5Dread(did, h.native[dt], _S_ALL, _S_ALL, _P_DEFAULT,
-                              out.ctypes.data_as(C.c_void_p)), f"read {path}")
+                _ok(h.H5Dread(did, h.native[dt], _S_ALL, _S_ALL, xfer,
+                              out.ctypes.data_as(C.c_void_p)),
+                    f"read {path}" + (f" (a value outside {dt})" if strict else ""))
             return out
         finally:
             h.H5Sclose(fsp)
@@ -721,8 +737,9 @@ class CsrRows:
     def read(self, r0: int, r1: int, data: np.ndarray, indices: np.ndarray,
              indptr: np.ndarray) -> int:
         """Rows [r0, r1) into the first nnz entries of `data` (float32) and
-        `indices` (int32) and the first r1 - r0 + 1 of `indptr` (int32, from 0).
-        Returns nnz."""
+        `indices` (int32, or uint16 for widths up to 65536: the library
+        converts while reading and fails on a column outside uint16) and the
+        first r1 - r0 + 1 of `indptr` (int32, from 0).  Returns nnz."""
         b, e = int(self.indptr[r0]), int(self.indptr[r1])
         n = e - b
         if n > data.shape[0] or n > indices.shape[0] or r1 - r0 + 1 > indptr.shape[0]:
@@ -733,7 +750,11 @@ class CsrRows:
             self.f.read("/X/data", b, e, np.float32, out=data[:n])
         else:
             data[:n] = _as_float32(self.f.read("/X/data", b, e))
-        if self._ind_t == np.int32:
+        if indices.dtype == np.uint16:
+            if self.width > 65536:
+                raise ValueError(f"16-bit columns need a width of at most 65536 ({self.width})")
+            self.f.read("/X/indices", b, e, np.uint16, out=indices[:n], strict=True)
+        elif self._ind_t == np.int32:
             self.f.read("/X/indices", b, e, np.int32, out=indices[:n])
         else:
             indices[:n] = _as_int32(self.f.read("/X/indices", b, e), "/X/indices")
@@ -745,7 +766,7 @@ def annotate_h5ad(ref_file: str, qry_file: str, out_file: Optional[str] = None,
                   path: str = "plain", batch_rows: int = 262144, quantile: float = 80,
                   finetune_thre: float = 0.05, finetune_times: int = 0, cores: int = 16,
                   gpuid: int = 0, proposal: Optional[Tuple[np.ndarray, int, int, float]] = None,
-                  verbose: bool = True) -> Dict[str, object]:
+                  narrow_columns: bool = True, verbose: bool = True) -> Dict[str, object]:
     """Annotates a query .h5ad of any height in batches of `batch_rows` cells.
 
     The reference is trained on the host (DataParser's steps, one-time) and
@@ -756,7 +777,8 @@ def annotate_h5ad(ref_file: str, qry_file: str, out_file: Optional[str] = None,
     device for both gene sets (csg_copyin_query_raw), scored and fine-tuned.
     `quantile` is in percent, as ``run`` takes it.  `proposal` =
     (network parameters, hidden width, k, min_margin) turns on the learned
-    proposal on the exact path.
+    proposal on the exact path.  `narrow_columns`: batches hold 16-bit
+    column indices when the query width allows.
 
     Returns first / final label indices (int32), the label names, the cell
     names and timings; writes the harness's TSV when `out_file` is given."""
@@ -810,7 +832,10 @@ def annotate_h5ad(ref_file: str, qry_file: str, out_file: Optional[str] = None,
             dev_path = "exact_int8"
         del c0, c1
         t_setup = time.perf_counter() - t0
-        slots = [(pin.empty(cap_nnz, np.float32), pin.empty(cap_nnz, np.int32),
+        # columns cross the bus as uint16 when the width allows (6 instead of
+        # 8 bytes per stored value)
+        it = np.uint16 if (narrow_columns and rows.width <= 65536) else np.int32
+        slots = [(pin.empty(cap_nnz, np.float32), pin.empty(cap_nnz, it),
                   pin.empty(cap_rows + 1, np.int32)) for _ in range(2)]
         first = np.empty(rows.height, np.int32)
         final = np.empty(rows.height, np.int32)

@@ -268,8 +268,8 @@ uint64_t csg_train_table(const csg_engine* eng, int which, uint32_t* out);
 
 /* Starts the upload of the next query batch's values and columns on the
  * engine's second stream and returns at once, so that the upload overlaps the
- * score step and fine-tune of the batch before it (SURVEY's streaming
- * overlap: src/cusingler.cu:335-409 slices and uploads synchronously).  The
+ * score step and fine-tune of the batch before it (the reference slices and
+ * uploads synchronously, src/cusingler.cu:335-409).  The
  * next csg_copyin_query_raw with the same three pointers and row count codes
  * that batch from its device copy instead of uploading it; the host arrays
  * must stay valid and unchanged until that call.  Two batches can be in
@@ -278,6 +278,19 @@ uint64_t csg_train_table(const csg_engine* eng, int which, uint32_t* out);
  * older unconsumed one.  No counterpart in the reference. */
 int csg_prefetch_query_raw(csg_engine* eng, uint32_t qry_height, const float* qry_data,
                            const int32_t* qry_indices, const int32_t* qry_indptr);
+
+/* csg_prefetch_query_raw and csg_copyin_query_raw with 16-bit column indices
+ * (query widths up to 65536; CSG_ERR_ARGUMENT beyond): 6 instead of 8 bytes
+ * per stored value cross the bus, for loaders that read the columns in that
+ * form (cusingler_amd/h5ad.py does when the width allows).  A prefetched
+ * batch is consumed only by the copy-in of the same index width.  Results are
+ * those of the 32-bit forms.  No counterpart in the reference. */
+int csg_prefetch_query_raw_u16(csg_engine* eng, uint32_t qry_height, const float* qry_data,
+                               const uint16_t* qry_indices, const int32_t* qry_indptr);
+int csg_copyin_query_raw_u16(csg_engine* eng, uint32_t qry_height, uint32_t qry_width,
+                             const float* qry_data, const uint16_t* qry_indices,
+                             const int32_t* qry_indptr, const int32_t* map_score,
+                             const int32_t* map_finetune);
 
 /* Caps the query chunk of the score step (src/cusingler.cu:335-409 slices the
  * query to the free device memory; here a chunk is at most 8 GiB of scores

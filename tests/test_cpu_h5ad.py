@@ -274,3 +274,39 @@ def test_tsv_matches_numpy_savetxt(tmp_path):
     np.savetxt(b, np.array(res), delimiter="\t", header="cell\tfirstLabel\tfinalLabel",
                comments="", fmt="%s")
     assert open(a, "rb").read() == open(b, "rb").read()
+
+
+def test_uint16_columns(tmp_path, rng):
+    """Row batches read with 16-bit columns (the library converting while
+    reading) equal the int32 ones; a column outside uint16 fails the read
+    instead of being clamped."""
+    _hdf5_or_skip()
+    data, ind, ptr = _csr(rng, 30, 50, 0.3)
+    ind = ind * 1000 + 7  # columns up to 49007, over int16
+    f = str(tmp_path / "u16.h5ad")
+    h5ad.write_h5ad(f, 30, 65536, data, ind.astype(np.int64), ptr, [str(i) for i in range(30)],
+                    [f"g{i}" for i in range(65536)], compression=2, chunk=64)
+    r = h5ad.CsrRows(f)
+    n = int(ptr[-1])
+    d16, i16, p16 = np.empty(n, np.float32), np.empty(n, np.uint16), np.empty(31, np.int32)
+    d32, i32, p32 = np.empty(n, np.float32), np.empty(n, np.int32), np.empty(31, np.int32)
+    assert r.read(0, 30, d16, i16, p16) == r.read(0, 30, d32, i32, p32) == n
+    assert np.array_equal(i16.astype(np.int32), i32) and np.array_equal(d16, d32)
+    assert np.array_equal(i32, ind) and np.array_equal(p16, p32)
+    r.close()
+    bad = ind.copy()
+    bad[5] = 65536 + 5
+    fb = str(tmp_path / "u16_bad.h5ad")
+    h5ad.write_h5ad(fb, 30, 65536, data, bad.astype(np.int64), ptr, [str(i) for i in range(30)],
+                    [f"g{i}" for i in range(65536)])
+    r = h5ad.CsrRows(fb)
+    with pytest.raises(h5ad.H5Error, match="outside"):
+        r.read(0, 30, d16, i16, p16)
+    r.close()
+    fw = str(tmp_path / "wide.h5ad")
+    h5ad.write_h5ad(fw, 30, 70000, data, ind, ptr, [str(i) for i in range(30)],
+                    [f"g{i}" for i in range(70000)])
+    r = h5ad.CsrRows(fw)
+    with pytest.raises(ValueError, match="65536"):
+        r.read(0, 30, d16, i16, p16)
+    r.close()

@@ -280,3 +280,56 @@ def test_prefetched_batches(eng):
     assert np.array_equal(fc, of[100:300])
     eng.destroy()
     eng.destroy_score()
+
+
+@pytest.mark.parametrize("name,slice_", [("small", 20000), ("wide4k", 1 << 26)])
+def test_uint16_columns(eng, name, slice_):
+    """csg_copyin_query_raw_u16 / csg_prefetch_query_raw_u16: the batch with
+    16-bit column indices codes to the same CSRs as the int32 form (several
+    upload slices, 16-bit codes), with the oracle's labels; a batch
+    prefetched in one index width is not consumed by a copy-in of the
+    other."""
+    from cusingler_amd import engine
+    from oracle import oracle as O
+    ds = _data(name)
+    prep = engine.Prep(ds, 8)
+    c0, c1 = prep.generate(0), prep.generate(1)
+    m0, m1 = prep.query_map(0, ds.qry_width), prep.query_map(1, ds.qry_width)
+    ptr = np.asarray(ds.qry_indptr, np.int64)
+    d = np.ascontiguousarray(ds.qry_data[ptr[0]:ptr[-1]], np.float32)
+    i32 = np.ascontiguousarray(ds.qry_indices[ptr[0]:ptr[-1]], np.int32)
+    i16 = i32.astype(np.uint16)
+    p = (ptr - ptr[0]).astype(np.int32)
+    assert ds.qry_width <= 65536 and np.array_equal(i16.astype(np.int32), i32)
+    _say("uint16 columns", name, "slice", slice_, "nnz", int(p[-1]))
+    eng.reserve(c0, c1, ds.qry_height)
+    eng.set_upload_slice(slice_)
+    try:
+        eng.copyin_query_raw(d, i16, p, ds.qry_width, m0, m1)
+    finally:
+        eng.set_upload_slice(1 << 26)
+    for step, c in ((0, c0), (1, c1)):
+        dd, ii, pp = eng.query_coded(step)
+        assert np.array_equal(pp, c.qry_indptr) and np.array_equal(ii, c.qry_indices)
+        assert np.array_equal(dd, c.qry_data), (name, step)
+    first, _, _ = eng.get_label(0.8, 0.05, "exact_int8", outputs="first")
+    final, _ = eng.finetune(0.8, 0.05, 0)
+    orc = O.Oracle(ds, 8)
+    of, om, _ = orc.score(0, ds.qry_height, 0.8, 0.05, O.VAR_SEQ)
+    ofin, _ = orc.finetune(0, ds.qry_height, om, 0.8, 0.05, 0, O.VAR_SEQ)
+    assert np.array_equal(first, of) and np.array_equal(final, ofin)
+    _say("uint16 columns: prefetched 16-bit batch consumed by its copy-in only")
+    eng.prefetch_query_raw(d, i16, p)
+    eng.copyin_query_raw(d, i16, p, ds.qry_width, m0, m1)
+    f2, _, _ = eng.get_label(0.8, 0.05, "plain", outputs="first")
+    assert np.array_equal(f2, of)
+    # a 16-bit prefetch left unconsumed by an int32 copy-in of the same values
+    eng.prefetch_query_raw(d, i16, p)
+    eng.copyin_query_raw(d, i32, p, ds.qry_width, m0, m1)
+    f3, _, _ = eng.get_label(0.8, 0.05, "plain", outputs="first")
+    assert np.array_equal(f3, of)
+    eng.copyin_query_raw(d, i16, p, ds.qry_width, m0, m1)  # consumes the staged batch
+    f4, _, _ = eng.get_label(0.8, 0.05, "plain", outputs="first")
+    assert np.array_equal(f4, of)
+    eng.destroy()
+    eng.destroy_score()
