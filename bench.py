@@ -125,8 +125,8 @@ def main():
                     help="run one 256 MiB device copy first (scales HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: the annotation path shards query cells across GPUs "
-              f"by design but this build measures one GPU's share (asked for --gpus {args.gpus})",
+        print(f"bench.py runs on one GPU only: query cells shard across GPUs by design, and "
+              f"this benchmark measures one GPU's share (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
 
