@@ -81,6 +81,7 @@ struct csg_engine
     bool      score_loaded = false;
     uint32_t  Nr = 0, Nq = 0, G = 0, Kp = 0, L = 0;
     bool      w8_ok = false;  // the wide INT8 kernels' LDS could be granted
+    bool      x3_ok = false;  // the three-partial wide INT8 kernels' LDS could be granted
     bool      bw_ok = false;  // the wide BF16 network kernel's LDS could be granted
     int       nbins = 0;
     std::vector<uint32_t> ctidx;
@@ -626,6 +627,13 @@ int csg_init_gpu(int gpuid, csg_engine** out)
                  hipFuncSetAttribute((const void*)k_gemm_int8w_group,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      W8_LDS) == hipSuccess;
+    eng->x3_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)X3_LDS &&
+                 hipFuncSetAttribute((const void*)k_gemm_int8x,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     X3_LDS) == hipSuccess &&
+                 hipFuncSetAttribute((const void*)k_gemm_int8x_group,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     X3_LDS) == hipSuccess;
     eng->bw_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)BW_LDS &&
                  hipFuncSetAttribute((const void*)k_bgemm_nt_w<1>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -933,6 +941,15 @@ int score_chunk(csg_engine* eng, int path, uint32_t nq)
     {
         dim3 grid((eng->Nr + W8_BN - 1) / W8_BN, (nq + W8_BM - 1) / W8_BM);
         k_gemm_int8w<<<grid, 512, W8_LDS, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                                                    (int)eng->Kp, eng->d_irq, eng->d_irr,
+                                                    (int)nq, (int)eng->Nr, eng->d_S,
+                                                    (size_t)eng->Nr);
+    }
+    else if (path == CSG_PATH_EXACT_INT8 && eng->x3_ok)
+    {
+        // gene sets beyond the folded limbs: three partials, 128 x 128 blocks
+        dim3 grid((eng->Nr + X3_BN - 1) / X3_BN, (nq + X3_BM - 1) / X3_BM);
+        k_gemm_int8x<<<grid, 512, X3_LDS, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
                                                     (int)eng->Kp, eng->d_irq, eng->d_irr,
                                                     (int)nq, (int)eng->Nr, eng->d_S,
                                                     (size_t)eng->Nr);
@@ -1312,8 +1329,10 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
 {
     const uint32_t L = eng->L;
     // wide 128 x 256 blocks with folded limbs up to 1920 genes, else the
-    // three-partial 64 x 128 blocks
+    // three-partial 128 x 128 blocks (64 x 128 where their LDS is refused)
     const bool wide = eng->G <= 1920 && eng->w8_ok;
+    const bool x3   = !wide && eng->x3_ok;
+    const uint32_t tbm = wide ? W8_BM : x3 ? X3_BM : I8_BM, tbn = wide ? W8_BN : x3 ? X3_BN : I8_BN;
     std::vector<uint32_t>  abase(L), acnt(L);
     std::vector<uint64_t>  coff(L);
     std::vector<GroupTile> tiles;
@@ -1327,8 +1346,8 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
         coff[l]           = n_scores;
         n_items += na;
         n_scores += (uint64_t)na * nl;
-        for (uint32_t m0 = 0; m0 < na; m0 += wide ? W8_BM : I8_BM)
-            for (uint32_t n0 = 0; n0 < nl; n0 += wide ? W8_BN : I8_BN)
+        for (uint32_t m0 = 0; m0 < na; m0 += tbm)
+            for (uint32_t n0 = 0; n0 < nl; n0 += tbn)
                 tiles.push_back(GroupTile{ l, m0, n0, 0u });
     }
     if (n_items == 0)
@@ -1386,6 +1405,11 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
         const uint32_t b_max = (uint32_t)round_up(eng->Nr, 256) - 1;
         if (wide)
             k_gemm_int8w_group<<<(unsigned)tiles.size(), 512, W8_LDS, eng->stream>>>(
+                eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
+                eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S);
+        else if (x3)
+            k_gemm_int8x_group<<<(unsigned)tiles.size(), 512, X3_LDS, eng->stream>>>(
                 eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
                 eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
                 (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S);
@@ -4421,12 +4445,15 @@ int csg_counters(csg_engine* eng, int64_t* out)
 
 int csg_selftest(csg_engine* eng, int which, double* max_err)
 {
-    if (!eng || !max_err || which < 0 || which > 4)
+    if (!eng || !max_err || which < 0 || which > 5)
         return CSG_ERR_ARGUMENT;
     if (which == 4 && !eng->w8_ok)
         return eng->fail(CSG_ERR_DEVICE, "the device does not grant the wide kernel's LDS");
+    if (which == 5 && !eng->x3_ok)
+        return eng->fail(CSG_ERR_DEVICE, "the device does not grant the three-partial kernel's LDS");
     HIPCHK(hipSetDevice(eng->device));
-    const int nq = 256, nr = 256, Kp = 192, G = 1500;
+    // the three-partial kernel over the whole int8 range of the high limb
+    const int nq = 256, nr = 256, Kp = 192, G = which == 5 ? 32000 : 1500;
     std::vector<int16_t> xq((size_t)nq * Kp), xr((size_t)nr * Kp);
     uint64_t st = 0x9E3779B97F4A7C15ull;
     auto rnd = [&]() {
@@ -4483,6 +4510,13 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
         k_gemm_int8w<<<dim3(nr / W8_BN, nq / W8_BM), 512, W8_LDS, eng->stream>>>(qh, ql, rh, rl, Kp, d1i,
                                                                             d1i, nq, nr, dS, nr);
     }
+    else if (which == 5)
+    {
+        k_split_limbs<<<bq, 256, 0, eng->stream>>>(dq, xq.size(), qh, ql);
+        k_split_limbs<<<br, 256, 0, eng->stream>>>(dr, xr.size(), rh, rl);
+        k_gemm_int8x<<<dim3(nr / X3_BN, nq / X3_BM), 512, X3_LDS, eng->stream>>>(
+            qh, ql, rh, rl, Kp, d1i, d1i, nq, nr, dS, nr);
+    }
     else if (which == 2)
         k_gemm_f32mfma<<<dim3(nr / F_BN, nq / F_BM), 256, 0, eng->stream>>>(dq, dr, Kp, d1, d1, nq,
                                                                             nr, dS, nr);
@@ -4512,7 +4546,7 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
             }
             const float  dev = out[(size_t)m * nr + n];
             double       err;
-            if (which == 0 || which == 4)
+            if (which == 0 || which == 4 || which == 5)
                 err = dev == (float)(double)s4 ? 0.0 : std::fabs((double)dev - (double)s4) + 1.0;
             else if (which == 3)
                 err = dev == seq ? 0.0 : std::fabs((double)dev - (double)seq) + 1.0;

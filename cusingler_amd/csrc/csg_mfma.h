@@ -567,6 +567,214 @@ __global__ __launch_bounds__(512) void k_gemm_int8w_group(
     gemm_int8w_block(qh, ql, rh, rl, Kp, map, w8_sm);
 }
 
+// ------------------------------------------------ three-partial wide tile ---
+//
+// For score gene sets beyond the folded kernel's 1920 genes (|X| up to 32639,
+// h in [-128, 127]): X*Y = 65536 h*h' + 256 (h*l' + l*h') + l*l' in three int32
+// partials.  A 128 x 128 block of 8 waves of 32 x 64 (2 x 4 MFMA tiles, 96
+// accumulator registers); K advances 64 at a time through four 32 KB LDS
+// stages [A hi 128 rows][A lo][B hi][B lo] of 64 bytes, filled by
+// direct-to-LDS loads with three stages in flight, the same swizzled 16-byte
+// slots as the folded kernel.  A wave instruction fills one group of 16 rows;
+// lane i of every wave loads row (wave * 16 + i / 4) of each plane.
+constexpr int X3_BM = 128, X3_BN = 128, X3_NS = 4;
+constexpr int X3_STAGE  = (2 * X3_BM + 2 * X3_BN) * W8_BK;  // 32 KB
+constexpr int X3_IR_OFF = X3_NS * X3_STAGE;
+constexpr int X3_LDS    = X3_IR_OFF + (X3_BM + X3_BN) * 8;  // dynamic LDS per block
+
+// Wave `wid`'s four direct-to-LDS loads of one stage: group wid of each plane.
+__device__ __forceinline__ void x3_issue(const int8_t* __restrict__ qh,
+                                         const int8_t* __restrict__ ql,
+                                         const int8_t* __restrict__ rh,
+                                         const int8_t* __restrict__ rl, size_t asrc, size_t bsrc,
+                                         int8_t* stage, int wid, int k0)
+{
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+    {
+        const int8_t* base = u == 0 ? qh : u == 1 ? ql : u == 2 ? rh : rl;
+        __builtin_amdgcn_global_load_lds(
+            base + (u < 2 ? asrc : bsrc) + k0,
+            (__attribute__((address_space(3))) void*)(stage + (wid + 8 * u) * 1024), 16, 0, 0);
+    }
+}
+
+// `sm` is the block's X3_LDS bytes of dynamic LDS; Kp is a multiple of 64.  The
+// maps are the folded kernel's (block columns 0 .. 127).
+template <class Map>
+__device__ inline void gemm_int8x_block(const int8_t* __restrict__ qh,
+                                        const int8_t* __restrict__ ql,
+                                        const int8_t* __restrict__ rh,
+                                        const int8_t* __restrict__ rl, int Kp, const Map& map,
+                                        int8_t* sm)
+{
+    const int tid  = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid  = tid >> 6;
+    const int wm   = wid >> 1;  // 0..3: 32-row quarter of the block
+    const int wn   = wid & 1;   // 0..1: 64-column half
+    const int rq   = wid * 16 + (lane >> 2);  // this lane's staged row of every plane
+    const size_t arow  = map.a_row(rq), brow = map.b_row(rq);
+    const size_t chunk = (size_t)(((lane & 3) ^ w8_swz(rq)) * 16);
+    const size_t asrc = arow * (size_t)Kp + chunk, bsrc = brow * (size_t)Kp + chunk;
+    v4i acc_hh[2][4], acc_x[2][4], acc_ll[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            acc_hh[i][j] = v4i{ 0, 0, 0, 0 };
+            acc_x[i][j]  = v4i{ 0, 0, 0, 0 };
+            acc_ll[i][j] = v4i{ 0, 0, 0, 0 };
+        }
+    const int T = Kp / W8_BK;
+    x3_issue(qh, ql, rh, rl, asrc, bsrc, sm, wid, 0);
+    // the inverse roots of the lane's rows, between the first two stages
+    // (complete by the first stage's wait, which leaves only later stages
+    // outstanding)
+    asm volatile("" ::: "memory");
+    const double ir_a = map.row_ir_at(arow);
+    const double ir_b = map.col_ir_at(brow);
+    asm volatile("" ::: "memory");
+    if (T > 1)
+        x3_issue(qh, ql, rh, rl, asrc, bsrc, sm + X3_STAGE, wid, W8_BK);
+    if (T > 2)
+        x3_issue(qh, ql, rh, rl, asrc, bsrc, sm + 2 * X3_STAGE, wid, 2 * W8_BK);
+    const int r16 = lane & 15;
+    const int kc  = lane >> 4;
+    for (int t = 0; t < T; ++t)
+    {
+        // stage t is complete when at most the loads of the stages issued
+        // after it (t + 1, t + 2: four each) are outstanding; the barrier
+        // then makes every wave's loads visible and ends every wave's reads of
+        // stage t - 1, whose buffer stage t + 3 fills
+        if (t + 2 < T)
+            __builtin_amdgcn_s_waitcnt(0x0F78);  // vmcnt <= 8
+        else if (t + 1 < T)
+            __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt <= 4
+        else
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt 0
+        asm volatile("s_barrier" ::: "memory");
+        if (t + 3 < T)
+            x3_issue(qh, ql, rh, rl, asrc, bsrc, sm + ((t + 3) % X3_NS) * X3_STAGE, wid,
+                     (t + 3) * W8_BK);
+        const int8_t* st = sm + (t % X3_NS) * X3_STAGE;
+        v4i ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+        {
+            const int row = wm * 32 + i * 16 + r16;
+            const int o   = row * W8_BK + ((kc ^ w8_swz(row)) << 4);
+            ah[i]         = *(const v4i*)(st + o);
+            al[i]         = *(const v4i*)(st + X3_BM * W8_BK + o);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const int row = wn * 64 + j * 16 + r16;
+            const int o   = row * W8_BK + ((kc ^ w8_swz(row)) << 4);
+            bh[j]         = *(const v4i*)(st + 2 * X3_BM * W8_BK + o);
+            bl[j]         = *(const v4i*)(st + (2 * X3_BM + X3_BN) * W8_BK + o);
+        }
+        // product by product over the wave's 8 output tiles (integer sums:
+        // the order does not change the result)
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                {
+                    if (p == 0)
+                        acc_hh[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bh[j], acc_hh[i][j], 0, 0, 0);
+                    else if (p == 1)
+                        acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bl[j], acc_x[i][j], 0, 0, 0);
+                    else if (p == 2)
+                        acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bh[j], acc_x[i][j], 0, 0, 0);
+                    else
+                        acc_ll[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bl[j], acc_ll[i][j], 0, 0, 0);
+                }
+    }
+    double* s_ir = (double*)(sm + X3_IR_OFF);
+    if ((lane & 3) == 0)
+    {
+        s_ir[rq]         = ir_a;
+        s_ir[X3_BM + rq] = ir_b;
+    }
+    __syncthreads();
+    double rir[2][4], cir[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        cir[j] = s_ir[X3_BM + wn * 64 + j * 16 + r16];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            rir[i][e] = s_ir[wm * 32 + i * 16 + (lane >> 4) * 4 + e];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int m = wm * 32 + i * 16 + (lane >> 4) * 4 + e;
+            if (!map.row_ok(m))
+                continue;
+            float* out = map.row_out(m);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int n = wn * 64 + j * 16 + r16;
+                if (!map.col_ok(n))
+                    continue;
+                const double s4 = (double)acc_hh[i][j][e] * 65536.0 +
+                                  (double)acc_x[i][j][e] * 256.0 + (double)acc_ll[i][j][e];
+                __builtin_nontemporal_store(exact_corr(s4, rir[i][e], cir[j]), out + n);
+            }
+        }
+}
+
+__global__ __launch_bounds__(512) void k_gemm_int8x(
+    const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
+    const int8_t* __restrict__ rl, int Kp, const double* __restrict__ ir_q,
+    const double* __restrict__ ir_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+{
+    extern __shared__ __attribute__((aligned(16))) int8_t x3_sm[];
+    const uint32_t gx = gridDim.x;
+    const uint32_t t  = xcd_order(blockIdx.x + gx * blockIdx.y, gx * gridDim.y);
+    // XCD ranges walked in groups of 8 row tiles, column-major inside a group
+    const uint32_t gsz = 8 * gx, grp = t / gsz, in = t % gsz;
+    const uint32_t rows = min(8u, gridDim.y - grp * 8);
+    const DenseW8Map map{ (int)(grp * 8 + in % rows) * X3_BM, (int)(in / rows) * X3_BN, nq, nr,
+                          ir_q, ir_r, S, ldS };
+    gemm_int8x_block(qh, ql, rh, rl, Kp, map, x3_sm);
+}
+
+// The grouped contraction in these blocks: the tables and output layout of
+// k_gemm_int8w_group, tiles of X3_BM cells x X3_BN reference rows.
+__global__ __launch_bounds__(512) void k_gemm_int8x_group(
+    const GroupTile* __restrict__ tiles, const uint32_t* __restrict__ cells,
+    const uint32_t* __restrict__ a_base, const uint32_t* __restrict__ a_cnt,
+    const uint32_t* __restrict__ lstart, const uint32_t* __restrict__ llen,
+    const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
+    const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
+    int Kp, uint32_t b_max, const double* __restrict__ ir_q, const double* __restrict__ ir_r,
+    float* __restrict__ S)
+{
+    extern __shared__ __attribute__((aligned(16))) int8_t x3_sm[];
+    const GroupTile t  = tiles[xcd_order(blockIdx.x, gridDim.x)];
+    const uint32_t  nl = llen[t.label];
+    const GroupW8Map map{ cells + a_base[t.label] + t.m0,
+                          (int)(a_cnt[t.label] - t.m0),
+                          (int)(nl - t.n0),
+                          lstart[t.label] + t.n0,
+                          b_max,
+                          ir_q,
+                          ir_r,
+                          S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
+                          nl };
+    gemm_int8x_block(qh, ql, rh, rl, Kp, map, x3_sm);
+}
+
 // The grouped contraction in 64 x 128 blocks with three partials, for any
 // score gene set (the wide kernel's folded limbs need |X| <= 1919): tile t =
 // tiles[t] = (label l, m0, n0) with m0 a multiple of 64 and n0 of 128, the

@@ -78,6 +78,11 @@ CONFIGS: Dict[str, SynthConfig] = {
     # The atlas plus 500,000 query-like training cells for the learned proposal.
     "learned": SynthConfig("learned", 20000, 200000, 20000, 0, 0, 50, 100, 2.0, 8.0, 0.05,
                            family="poisson", share=0.6, n_train=500000),
+    # The learned-proposal atlas with every gene's level varying between labels
+    # (fold ~ LogNormal(0, 2)): label pairs differ in thousands of genes, so
+    # G_s = 2,097 (over the folded-limb kernel's 1,920) and G_f = 8,183.
+    "atlas_wide": SynthConfig("atlas_wide", 20000, 200000, 20000, 0, 0, 50, 100, 2.0, 8.0, 0.05,
+                              family="poisson", share=0.6, n_train=500000, label_sigma=2.0),
     # 50,000 ref x 30,000 genes, 100 labels = 10 parents x 10 children sharing
     # 70% of markers; 1,000,000 query cells at 4% nonzero.
     "streaming": SynthConfig("streaming", 50000, 1000000, 30000, 0, 0, 100, 100, 2.0, 8.0, 0.04,

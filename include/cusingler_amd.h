@@ -410,8 +410,10 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
 
 /* Diagnostic of the contraction kernels on seeded random ranks (|X| <= 1500,
  * 256 x 256 rows, K = 192): which = 0 INT8 limbs, 1 FP16, 2 FP32 MFMA,
- * 3 plain FP32, 4 INT8 limbs wide-tile kernel (gene sets up to 1920).  *max_err receives 0 when the device results equal the host's
- * exact (0, 4) / sequential-fma (3) values, else the largest difference (+1); for
+ * 3 plain FP32, 4 INT8 limbs wide-tile kernel (gene sets up to 1920), 5 INT8
+ * limbs three-partial wide-tile kernel (|X| <= 32000: every high-limb value).
+ * *max_err receives 0 when the device results equal the host's
+ * exact (0, 4, 5) / sequential-fma (3) values, else the largest difference (+1); for
  * 1 and 2 the largest error of the dot product relative to sqrt(sxx * syy),
  * i.e. in correlation units.  No
  * counterpart in the reference: it checks the MFMA operand layouts. */

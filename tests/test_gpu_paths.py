@@ -25,12 +25,12 @@ def eng():
     e.close()
 
 
-@pytest.mark.parametrize("which,name", [(0, "int8"), (4, "int8_wide"), (3, "plain"), (1, "fp16"),
-                                        (2, "fp32_mfma")])
+@pytest.mark.parametrize("which,name", [(0, "int8"), (4, "int8_wide"), (5, "int8_x3"), (3, "plain"),
+                                        (1, "fp16"), (2, "fp32_mfma")])
 def test_contraction_selftest(eng, which, name):
     _say("selftest", name)
     err = eng.selftest(which)
-    if which in (0, 3, 4):
+    if which in (0, 3, 4, 5):
         assert err == 0.0, f"{name} contraction differs from the host ({err})"
     else:
         assert err < 1e-5, f"{name} relative error {err}"
