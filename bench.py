@@ -378,8 +378,9 @@ def main():
     logical_ops = 2.0 * corr_per_step * Gs * steps          # one logical MAC = 2 ops
     achieved = logical_ops / (corr_ms * 1e-3) / 1e12 if corr_ms > 0 else 0.0
     if chosen in ("exact_int8", "proposal"):
-        kname = "k_gemm_int8w_group" if chosen == "proposal" else (
-            "k_gemm_int8w" if Gs <= 1920 else "k_gemm_int8")
+        # folded limbs up to 1,920 genes, three partials beyond
+        kname = ("k_gemm_int8w" if Gs <= 1920 else "k_gemm_int8x") + (
+            "_group" if chosen == "proposal" else "")
         peak, unit = MI355X_PEAK["int8_tops"], "TOP/s"
     elif chosen == "fp16":
         peak, unit, kname = MI355X_PEAK["fp16_tflops"], "TFLOP/s", "k_gemm_f16"
