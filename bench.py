@@ -105,6 +105,9 @@ def main():
     ap.add_argument("--finetune-fuse", type=int, default=None,
                     help="largest fine-tune group (cells) of the fused FP32 form, 0 to 4 "
                          "(default: the engine's, 4)")
+    ap.add_argument("--finetune-chain", type=int, default=None,
+                    help="longest chain of fine-tune rounds launched without a host read-back "
+                         "(0 = none; default: the engine's, 0)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--hidden", type=int, default=512,
                     help="width of the proposal network's two hidden layers (multiples of 256 take the pipelined layer kernel)")
@@ -157,6 +160,8 @@ def main():
     eng = engine.Engine(0)
     if args.finetune_fuse is not None:
         eng.set_finetune_fuse(args.finetune_fuse)
+    if args.finetune_chain is not None:
+        eng.set_finetune_chain(args.finetune_chain)
     training = None
     if cached:
         net = np.load(args.net_cache, allow_pickle=False)
@@ -241,6 +246,10 @@ def main():
         _hip_sync()
         dt = time.perf_counter() - t0
         ph = eng.phase_times(reset=True)
+        # fine-tune rounds of the last timed step that ran in chains (the
+        # labelled pass below launches its rounds one by one to time them)
+        chained = {f"timed_step_{k}": v for k, v in eng.counters().items()
+                   if k in ("ft_chained_rounds", "ft_chain_halts")}
         # labels (and counters) of one untimed pass for the agreement check;
         # the per-phase breakdown is taken from this pass, every phase timed
         # (the device fine-tune's kernel families too: mode 3)
@@ -249,6 +258,7 @@ def main():
         cnt = eng.counters()
         final, _ = eng.finetune(0.8, 0.05, 0)
         cnt.update({k: v for k, v in eng.counters().items() if k.startswith("ft_")})
+        cnt.update(chained)
         ph1 = eng.phase_times(reset=True)
         eng.set_phase_timing(2)
         return dict(value=Nq * args.steps / dt, ms=dt * 1e3 / args.steps, phases=ph, phases1=ph1,

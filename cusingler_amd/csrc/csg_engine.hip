@@ -248,6 +248,18 @@ struct csg_engine
     };
     std::vector<FtdGraphB> ftd_gB;
     uint32_t               ftd_mask = 0xFFFFFFFFu;  // launch mask of the current round
+    // chains of rounds launched together without a host read-back (few-cell
+    // rounds; one graph per chain length, launch mask and tables)
+    struct FtdChainGraph
+    {
+        int                    rounds = 0;
+        uint32_t               mask   = 0;
+        hipGraphExec_t         exec   = nullptr;
+        std::vector<uintptr_t> sig;
+    };
+    std::vector<FtdChainGraph> ftd_gC;
+    int                        ftd_chain_max = 0;  // longest chain (rounds); 0: no chains
+    uint64_t                   ftd_chain_scores = 0;  // most scores of a chained round (0: any)
     // launch settings of the work-queue kernels (blocks), tunable
     int   ftd_grid_threads = 1024, ftd_grid_setup = 512, ftd_grid_rank = 2048,
         ftd_grid_corr = 512, ftd_grid_select = 2048, ftd_grid_fused = 4096, ftd_thin_max = 8,
@@ -2301,6 +2313,14 @@ namespace {
 
 constexpr int FTD_TPB = 256;  // threads of the thread-level work-queue kernels
 
+void ftd_free_chains(csg_engine* eng)
+{
+    for (auto& c : eng->ftd_gC)
+        if (c.exec)
+            (void)hipGraphExecDestroy(c.exec);
+    eng->ftd_gC.clear();
+}
+
 int ftd_free(csg_engine* eng)
 {
     if (eng->ftd_gAs)
@@ -2320,6 +2340,7 @@ int ftd_free(csg_engine* eng)
         if (b.exec)
             (void)hipGraphExecDestroy(b.exec);
     eng->ftd_gB.clear();
+    ftd_free_chains(eng);
     eng->ftd_gA = nullptr;
     eng->ftd_sigA.clear();
     dfree(eng->d_fl_lists);
@@ -2428,6 +2449,7 @@ int ftd_groups(csg_engine* eng, uint32_t n)
         if (b.exec)
             (void)hipGraphExecDestroy(b.exec);
     eng->ftd_gB.clear();
+    ftd_free_chains(eng);
     const uint32_t cap = std::max(n, eng->ftd_gcap + eng->ftd_gcap / 2);
     uint32_t       T   = 1024;
     while (T < 2 * cap)
@@ -2519,20 +2541,41 @@ uint32_t ftd_fuse(const csg_engine* eng)
 // non-empty, from the round's totals (a superset for every group batch of
 // the round).  Kernels outside the mask are not launched; each mask has its
 // own part B graph.
-constexpr uint32_t FTDM_RANK = 1, FTDM_MFMA = 2, FTDM_MFMA64 = 4, FTDM_CORR = 8, FTDM_THIN = 16,
-                   FTDM_FUSED = 32, FTDM_REDO = 64, FTDM_ALL = 127;
-
 uint32_t ftd_mask_of(const csg_engine* eng, const FtdCounts& h)
 {
-    const uint32_t ftiles = h.n_tiles - h.n_etiles - h.n_btiles;
-    return (h.ref_elems ? FTDM_RANK : 0u) | (h.n_etiles ? FTDM_MFMA : 0u) |
-           (h.n_btiles ? FTDM_MFMA64 : 0u) | (ftiles ? FTDM_CORR : 0u) |
-           (h.n_thin ? FTDM_THIN : 0u) | (h.n_fused && ftd_fuse(eng) ? FTDM_FUSED : 0u) |
-           (h.n_etiles || h.n_btiles ? FTDM_REDO : 0u);
+    return ftd_mask_need(h.ref_elems, h.n_tiles, h.n_etiles, h.n_btiles, h.n_thin,
+                         ftd_fuse(eng) ? h.n_fused : 0u);
 }
 
-// Round part A: grouping, group set-up and the prefix sums.
-int ftd_launch_a(csg_engine* eng, bool small, int compact_times)
+// Capacities of the current work tables, in the units of the round totals
+// they hold (a chained round whose totals exceed them halts).
+FtdCaps ftd_caps(const csg_engine* eng)
+{
+    const auto&    c = eng->fl_cap;
+    const uint64_t L = std::max<uint32_t>(eng->fL, 1);
+    FtdCaps        k;
+    k.ref_items  = std::min({ (uint64_t)eng->d_ft_rsq_cap, (uint64_t)c.refgroup, (uint64_t)c.refrow });
+    k.cells      = std::min({ (uint64_t)eng->d_ft_csq_cap, (uint64_t)c.items, (uint64_t)c.cells,
+                              (uint64_t)c.redo, (uint64_t)c.stage / L });
+    k.ref_elems  = eng->d_ft_rr_cap;
+    k.cell_elems = eng->d_ft_cr_cap;
+    k.scores     = eng->d_ft_scores_cap;
+    if (eng->ftd_chain_scores)
+        k.scores = std::min<uint64_t>(k.scores, eng->ftd_chain_scores);
+    k.segs       = std::min({ (uint64_t)eng->d_ft_P_cap, (uint64_t)c.rseg, (uint64_t)c.pred,
+                              (uint64_t)c.nan1, (uint64_t)c.nan2, (uint64_t)c.segcell });
+    k.tiles      = c.tiles;
+    k.thin       = c.thin;
+    k.rthin      = c.rthin;
+    return k;
+}
+
+// Round part A: grouping, group set-up and the prefix sums.  `chain`: the
+// round is one of a chain launched without a host read-back, with the work
+// tables of these capacities and part B's families of `chain_mask` (a round
+// needing more halts the chain, see k_ftd_scan).
+int ftd_launch_a(csg_engine* eng, bool small, int compact_times, const FtdCaps* chain = nullptr,
+                 uint32_t chain_mask = 0)
 {
     const uint32_t L = eng->fL, T = eng->ftd_T;
     const unsigned g = (unsigned)eng->ftd_grid_threads;
@@ -2574,9 +2617,10 @@ int ftd_launch_a(csg_engine* eng, bool small, int compact_times)
         std::max<uint32_t>(eng->Gf, 1), (uint32_t)eng->ftd_gstride, eng->ft_exact ? 1 : 0,
         ftd_fuse(eng), eng->d_fl_groups, eng->d_fl_gflag, eng->d_fl_glabels, eng->d_fl_glpos,
         eng->d_fl_ggenes);
-    k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
-                                                   (uint32_t)eng->ftd_thin_max, (uint32_t)eng->ftd_t64, eng->d_fl_gtile0,
-                                                   eng->d_fl_gthin0, eng->d_fl_gseg0);
+    k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(
+        cnt, eng->d_fl_groups, eng->d_fl_gflag, (uint32_t)eng->ftd_thin_max,
+        (uint32_t)eng->ftd_t64, eng->d_fl_gtile0, eng->d_fl_gthin0, eng->d_fl_gseg0, 1,
+        chain ? 1 : 0, chain_mask, chain ? *chain : FtdCaps{});
     LAUNCHCHK();
     return CSG_OK;
 }
@@ -2896,7 +2940,7 @@ int ftd_tune(csg_engine* eng, float quantile)
         k_ftd_scan<<<1, FTD_SCAN_NT, 0, eng->stream>>>(cnt, eng->d_fl_groups, eng->d_fl_gflag,
                                                        (uint32_t)eng->ftd_thin_max, t64,
                                                        eng->d_fl_gtile0, eng->d_fl_gthin0,
-                                                       eng->d_fl_gseg0);
+                                                       eng->d_fl_gseg0, 0, 0, 0u, FtdCaps{});
         LAUNCHCHK();
         FtdCounts hc;
         HIPCHK(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, eng->stream));
@@ -3040,6 +3084,57 @@ int ftd_graph_b(csg_engine* eng, float quantile, float thre, int times, hipGraph
     return rc;
 }
 
+// A chain of `rounds` few-cell rounds in one graph: each round's part A (led
+// by the previous round's compaction) and its part B with the launch mask
+// `mask`, replayed without a host read-back between the rounds.  A round
+// that needs larger work tables or a family outside the mask halts the chain
+// (k_ftd_scan).  One graph per chain length and mask, re-captured when a
+// table or setting in its signature changes.
+int ftd_graph_chain(csg_engine* eng, int rounds, uint32_t mask, float quantile, float thre,
+                    int times, hipGraphExec_t* out)
+{
+    const FtdCaps  caps = ftd_caps(eng);
+    const uint32_t keep = eng->ftd_mask;
+    eng->ftd_mask       = mask;
+    std::vector<uintptr_t>       sig = ftd_sig_a(eng);
+    const std::vector<uintptr_t> sb  = ftd_sig_b(eng, quantile, thre, times);
+    sig.insert(sig.end(), sb.begin(), sb.end());
+    for (uint64_t v : { caps.ref_items, caps.cells, caps.ref_elems, caps.cell_elems, caps.scores,
+                        caps.segs, caps.tiles, caps.thin, caps.rthin })
+        sig.push_back((uintptr_t)v);
+    sig.push_back((uintptr_t)rounds);
+    csg_engine::FtdChainGraph* c = nullptr;
+    for (auto& e : eng->ftd_gC)
+        if (e.rounds == rounds && e.mask == mask)
+            c = &e;
+    if (!c)
+    {
+        if (eng->ftd_gC.size() >= 8)
+        {
+            if (eng->ftd_gC.front().exec)
+                (void)hipGraphExecDestroy(eng->ftd_gC.front().exec);
+            eng->ftd_gC.erase(eng->ftd_gC.begin());
+        }
+        eng->ftd_gC.emplace_back();
+        c         = &eng->ftd_gC.back();
+        c->rounds = rounds;
+        c->mask   = mask;
+    }
+    const int rc = ftd_graph(eng, &c->exec, &c->sig, sig, [&]() -> int {
+        for (int r = 0; r < rounds; ++r)
+        {
+            int e;
+            if ((e = ftd_launch_a(eng, true, times, &caps, mask)) ||
+                (e = ftd_launch_b(eng, quantile, thre, times)))
+                return e;
+        }
+        return CSG_OK;
+    });
+    eng->ftd_mask = keep;
+    *out          = rc ? nullptr : c->exec;
+    return rc;
+}
+
 // A round whose work tables exceed the budget: its groups are processed in
 // ranges whose tables fit (each range re-scanned on the device), every range
 // through the same round graph.
@@ -3111,7 +3206,7 @@ int ftd_batches(csg_engine* eng, const FtdCounts& h, float quantile, float thre,
                                                        (uint32_t)eng->ftd_thin_max,
                                                        (uint32_t)eng->ftd_t64,
                                                        eng->d_fl_gtile0, eng->d_fl_gthin0,
-                                                       eng->d_fl_gseg0);
+                                                       eng->d_fl_gseg0, 0, 0, 0u, FtdCaps{});
         LAUNCHCHK();
         HIPCHK(hipGraphLaunch(gB, eng->stream));
         // the range values above are read before the next copy replaces them
@@ -3169,13 +3264,72 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
     // the grouping of a round of few cells runs in one block; a round's
     // active cells are at most the previous round's
     eng->ftd_small_a = h.n_next <= (uint32_t)FTD_SMALL_NT;
-    for (int round = 0; h.n_next > 0; ++round)
+    // a round removes at least one label from every active cell (its
+    // lowest-scoring ones), so rounds after one whose largest list held m
+    // labels number at most m - 2, and the compaction of the last of them
+    // opens round m - 1 (which then has no active cells)
+    uint32_t max_count = L;
+    bool     comp      = false;  // part A opens with the previous round's compaction
+    auto     state_to_host = [&]() -> int {
+        // state at the start of the current round, for the host schedule
+        *fallback = true;
+        lists.resize((size_t)Nq * L);
+        counts.resize(Nq);
+        nrounds.resize(Nq);
+        HIPCHK(hipMemcpy(lists.data(), eng->d_fl_lists, (size_t)Nq * L, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(counts.data(), eng->d_fl_counts, (size_t)Nq * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(nrounds.data(), eng->d_fl_rounds, (size_t)Nq * 4, hipMemcpyDeviceToHost));
+        return CSG_OK;
+    };
+    auto add_totals = [&]() {
+        eng->counters[6] += h.tot_rounds;
+        eng->counters[7] += (int64_t)h.tot_active;
+        eng->counters[8] += (int64_t)h.tot_groups;
+        eng->counters[9] += (int64_t)h.tot_scores;
+        eng->counters[22] += (int64_t)h.tot_fused;
+        eng->counters[10] += (int64_t)h.pair_genes;
+    };
+    for (;;)
     {
         const int64_t t0 = now_us();
+        // few-cell rounds run in chains: the rounds a list of max_count
+        // labels can still take, launched together with the last round's
+        // launch mask and work tables, one read-back per chain
+        const int chain = comp && eng->ftd_small_a && eng->ftd_chain_max > 0 &&
+                                  eng->phase_mode != 3
+                              ? (int)std::min<uint32_t>(max_count > 2 ? max_count - 1 : 1u,
+                                                        (uint32_t)eng->ftd_chain_max)
+                              : 0;
+        if (chain > 0)
+        {
+            hipGraphExec_t gC = nullptr;
+            if ((rc = ftd_graph_chain(eng, chain, eng->ftd_mask, quantile, thre, times, &gC)))
+                return rc;
+            HIPCHK(hipGraphLaunch(gC, eng->stream));
+            HIPCHK(hipMemcpyAsync(&h, eng->d_fl_cnt, sizeof(FtdCounts), hipMemcpyDeviceToHost,
+                                  eng->stream));
+            HIPCHK(hipStreamSynchronize(eng->stream));
+            eng->counters[15] += now_us() - t0;
+            eng->counters[23] += chain;
+            if (h.undefined)
+                return eng->fail(CSG_ERR_UNDEFINED,
+                                 "a NaN-carrying fine-tune segment makes the reference's sort "
+                                 "leave its array (undefined behaviour)");
+            if (!h.halt)
+            {
+                if (h.n_active == 0)
+                    break;  // the chain ran past the last round
+                max_count = h.max_count;
+                continue;   // (the next part A opens with this chain's last compaction)
+            }
+            // a halted round: its compaction ran, the rest runs below from
+            // its part A (tables grown, its own launch mask)
+            eng->counters[24]++;
+            k_ftd_resume<<<1, 64, 0, eng->stream>>>(eng->d_fl_cnt);
+            LAUNCHCHK();
+            comp = false;
+        }
         const bool      small = eng->ftd_small_a;
-        // every round after the first starts with the previous round's
-        // compaction, in the same graph
-        const bool              comp = round > 0;
         hipGraphExec_t*         gA   = comp ? &eng->ftd_gAc[small] : small ? &eng->ftd_gAs : &eng->ftd_gA;
         std::vector<uintptr_t>* sA   = comp ? &eng->ftd_sigAc[small] : small ? &eng->ftd_sigAs : &eng->ftd_sigA;
         std::vector<uintptr_t>  sig  = ftd_sig_a(eng);
@@ -3196,21 +3350,10 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
             break;
         if (h.collision)
         {
-            // state at the start of this round back to the host
-            *fallback = true;
-            lists.resize((size_t)Nq * L);
-            counts.resize(Nq);
-            nrounds.resize(Nq);
-            HIPCHK(hipMemcpy(lists.data(), eng->d_fl_lists, (size_t)Nq * L, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(counts.data(), eng->d_fl_counts, (size_t)Nq * 4, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(nrounds.data(), eng->d_fl_rounds, (size_t)Nq * 4, hipMemcpyDeviceToHost));
-            return CSG_OK;
+            add_totals();
+            return state_to_host();
         }
-        eng->counters[6]++;
-        eng->counters[7] += h.n_active;
-        eng->counters[8] += h.n_groups;
-        eng->counters[9] += (int64_t)h.n_scores;
-        eng->counters[22] += h.n_fused;
+        max_count        = h.max_count;
         eng->ftd_small_a = h.n_active <= (uint32_t)FTD_SMALL_NT;
         eng->ftd_mask    = ftd_mask_of(eng, h);
         if (ftd_bytes(h, L) <= eng->ftd_budget)
@@ -3241,8 +3384,7 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
             return rc;
         // (the compaction into the next round's active list leads that
         // round's part A)
-        // part B leaves the next round's count in n_next (read with part A)
-        h.n_next = 1;
+        comp = true;
         eng->counters[15] += now_us() - t0;
     }
     k_ftd_final<<<nblk(Nq), 256, 0, eng->stream>>>(eng->d_fl_lists, eng->d_fl_counts, Nq, L,
@@ -3257,7 +3399,7 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
                           eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
     eng->counters[11] += h.tot_redo;
-    eng->counters[10] += (int64_t)h.pair_genes;
+    add_totals();
     if (h.undefined)
         return eng->fail(CSG_ERR_UNDEFINED,
                          "a NaN-carrying fine-tune segment makes the reference's sort leave its "
@@ -3482,6 +3624,8 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
     if (rc)
         return rc;
     for (int i = 6; i < 16; ++i)  // the score step's own counters stay
+        eng->counters[i] = 0;
+    for (int i : { 22, 23, 24 })
         eng->counters[i] = 0;
     PhaseScope    total(eng, 5, 0);
     const int64_t t_start = now_us();
@@ -4371,6 +4515,17 @@ int csg_set_finetune_fuse(csg_engine* eng, int max_cells)
     if (max_cells < 0 || max_cells > FTD_FUSE_MAX)
         return eng->fail(CSG_ERR_ARGUMENT, "fused fine-tune groups hold 0 to 4 cells");
     eng->ftd_fuse_max = max_cells;
+    return CSG_OK;
+}
+
+int csg_set_finetune_chain(csg_engine* eng, int max_rounds, uint64_t max_scores)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (max_rounds < 0 || max_rounds > 64)
+        return eng->fail(CSG_ERR_ARGUMENT, "fine-tune chains hold 0 to 64 rounds");
+    eng->ftd_chain_max    = max_rounds;
+    eng->ftd_chain_scores = max_scores;
     return CSG_OK;
 }
 

@@ -62,7 +62,42 @@ struct FtdCounts
     uint64_t n_redo_segs;
     uint64_t rthin_bound;  // tiles a redo of every exact-form cell would take
     uint64_t pair_genes;   // sum over rounds of cells x reference rows x |G_sub| (work measure)
+    // Chained rounds (several rounds launched without a host read-back): a
+    // round whose work exceeds the tables the chain was launched with, or
+    // whose grouping met a hash collision, sets `halt` in its scan and leaves
+    // the state at its start (its compaction done); every later kernel of the
+    // chain then returns at once.  Sticky until the host clears it.
+    uint32_t halt;
+    uint32_t max_count;   // largest candidate list of the round's active cells
+    uint64_t tot_active;  // active cells summed over rounds
+    uint64_t tot_groups;
+    uint64_t tot_scores;
+    uint64_t tot_fused;
+    uint32_t tot_rounds;  // rounds with active cells (all rounds of the call)
+    uint32_t pad_;
 };
+
+// Capacities of the work tables a chain of rounds was launched with (in the
+// units of the matching FtdCounts totals); a round needing more halts.
+struct FtdCaps
+{
+    uint64_t ref_items, cells, ref_elems, cell_elems, scores, segs, tiles, thin, rthin;
+};
+
+// Launch mask of a round: the work-queue kernel families whose work lists
+// are not empty.  A round's part B launches the families of its mask only.
+constexpr uint32_t FTDM_RANK = 1, FTDM_MFMA = 2, FTDM_MFMA64 = 4, FTDM_CORR = 8, FTDM_THIN = 16,
+                   FTDM_FUSED = 32, FTDM_REDO = 64, FTDM_ALL = 127;
+
+__host__ __device__ inline uint32_t ftd_mask_need(uint64_t ref_elems, uint32_t n_tiles,
+                                                  uint32_t n_etiles, uint32_t n_btiles,
+                                                  uint32_t n_thin, uint32_t n_fused)
+{
+    const uint32_t ftiles = n_tiles - n_etiles - n_btiles;
+    return (ref_elems ? FTDM_RANK : 0u) | (n_etiles ? FTDM_MFMA : 0u) |
+           (n_btiles ? FTDM_MFMA64 : 0u) | (ftiles ? FTDM_CORR : 0u) | (n_thin ? FTDM_THIN : 0u) |
+           (n_fused ? FTDM_FUSED : 0u) | (n_etiles || n_btiles ? FTDM_REDO : 0u);
+}
 
 // Height of the big exact fine-tune tile (cells).
 constexpr int FTD_BIG = 32;
@@ -159,6 +194,8 @@ __global__ void k_ftd_begin(const FtdCounts* __restrict__ cnt, uint32_t* __restr
                             unsigned long long* __restrict__ htab,
                             uint32_t* __restrict__ gcount)
 {
+    if (cnt->halt)
+        return;
     ftd_begin_body(cnt, act, act_next, htab, gcount, blockIdx.x * blockDim.x + threadIdx.x,
                    gridDim.x * blockDim.x);
 }
@@ -184,6 +221,7 @@ __device__ inline void ftd_counters_body(FtdCounts* __restrict__ cnt, uint32_t T
     cnt->n_nan       = 0;
     cnt->n_nan2      = 0;
     cnt->collision   = 0;
+    cnt->max_count   = 0;
     cnt->max_stride  = 0;
     cnt->round      += 1;
     cnt->g_lo        = 0u;
@@ -200,7 +238,7 @@ __device__ inline void ftd_counters_body(FtdCounts* __restrict__ cnt, uint32_t T
 
 __global__ void k_ftd_counters(FtdCounts* __restrict__ cnt, uint32_t T)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0)
+    if (blockIdx.x != 0 || threadIdx.x != 0 || cnt->halt)
         return;
     ftd_counters_body(cnt, T);
 }
@@ -244,6 +282,8 @@ __global__ void k_ftd_hash(const FtdCounts* __restrict__ cnt, const uint32_t* __
                            uint32_t L, unsigned long long* __restrict__ htab,
                            uint32_t* __restrict__ howner, uint32_t* __restrict__ cslot)
 {
+    if (cnt->halt)
+        return;
     ftd_hash_body(cnt, act, lists, counts, L, htab, howner, cslot,
                   blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
@@ -268,6 +308,8 @@ __global__ void k_ftd_gid(FtdCounts* __restrict__ cnt, const unsigned long long*
                           const uint32_t* __restrict__ howner, uint32_t* __restrict__ hgid,
                           uint32_t* __restrict__ gowner)
 {
+    if (cnt->halt)
+        return;
     ftd_gid_body(cnt, htab, howner, hgid, gowner, blockIdx.x * blockDim.x + threadIdx.x,
                  gridDim.x * blockDim.x);
 }
@@ -282,16 +324,19 @@ __device__ inline void ftd_member_body(FtdCounts* __restrict__ cnt,
                                        uint32_t* __restrict__ gcount, uint32_t* __restrict__ cgid,
                                        uint32_t* __restrict__ cpos, uint32_t i0, uint32_t stride)
 {
-    const uint32_t n = cnt->n_active;
+    const uint32_t n  = cnt->n_active;
+    uint32_t       mc = 0;
     for (uint32_t i = i0; i < n; i += stride)
     {
-        const uint32_t c = act[i];
-        const uint32_t g = hgid[cslot[i]];
-        const uint32_t o = gowner[g];
+        const uint32_t c  = act[i];
+        const uint32_t g  = hgid[cslot[i]];
+        const uint32_t o  = gowner[g];
+        const uint32_t kc = counts[c];
+        mc                = max(mc, kc);
         if (o != c)
         {
             uint64_t a[4], b[4];
-            ftd_key(lists + (size_t)c * L, counts[c], a);
+            ftd_key(lists + (size_t)c * L, kc, a);
             ftd_key(lists + (size_t)o * L, counts[o], b);
             if (a[0] != b[0] || a[1] != b[1] || a[2] != b[2] || a[3] != b[3])
                 atomicOr(&cnt->collision, 1u);
@@ -299,6 +344,10 @@ __device__ inline void ftd_member_body(FtdCounts* __restrict__ cnt,
         cgid[i] = g;
         cpos[i] = atomicAdd(&gcount[g], 1u);
     }
+    // (every lane of the wave reaches this point; counts are below 2^24)
+    const float wm = wave_max((float)mc);
+    if ((threadIdx.x & 63) == 0 && wm > 0.f)
+        atomicMax(&cnt->max_count, (uint32_t)wm);
 }
 
 __global__ void k_ftd_member(FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ act,
@@ -308,6 +357,8 @@ __global__ void k_ftd_member(FtdCounts* __restrict__ cnt, const uint32_t* __rest
                              const uint32_t* __restrict__ gowner, uint32_t* __restrict__ gcount,
                              uint32_t* __restrict__ cgid, uint32_t* __restrict__ cpos)
 {
+    if (cnt->halt)
+        return;
     ftd_member_body(cnt, act, lists, counts, L, cslot, hgid, gowner, gcount, cgid, cpos,
                     blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
@@ -325,6 +376,8 @@ __global__ __launch_bounds__(FTD_SMALL_NT) void k_ftd_group_small(
     uint32_t* __restrict__ cslot, uint32_t* __restrict__ hgid, uint32_t* __restrict__ gowner,
     uint32_t* __restrict__ cgid, uint32_t* __restrict__ cpos)
 {
+    if (cnt->halt)
+        return;
     const uint32_t t = threadIdx.x;
     if (t == 0)
         ftd_counters_body(cnt, T);
@@ -360,6 +413,8 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
     __shared__ uint32_t spart[FTD_SETUP_NT];
     __shared__ uint8_t  sflag[256];
     __shared__ uint32_t sk, snref;
+    if (cnt->halt)
+        return;
     const uint32_t ng  = cnt->n_groups;
     const uint32_t W   = (Gf + 31) / 32;
     const uint32_t tid = threadIdx.x;
@@ -488,9 +543,6 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
                                  (!fused && exact_ok && gs <= 1919 ? FTD_EXACT : 0));
             if (fused && gs >= 20)
                 atomicAdd(&cnt->n_fused, 1u);
-            if (gs >= 20)
-                atomicAdd((unsigned long long*)&cnt->pair_genes,
-                          (unsigned long long)G.ncell * G.nref * gs);
         }
     }
 }
@@ -537,16 +589,29 @@ __host__ __device__ inline FtdSizes ftd_sizes(const FtGroup& G, uint8_t flag, ui
     return z;
 }
 
+//
+// `acc` (the round's own scan, not a batch range's): the round's totals are
+// added to the call's tot_* counters and pair_genes.  `chain` (a round of a
+// chain launched without a host read-back): a round whose totals exceed
+// `caps`, which needs a kernel family outside `chain_mask` (the families the
+// chain launches), or whose grouping met a collision, halts the chain
+// instead -- its active list goes back to n_next and its work counts to zero,
+// so that the rest of the round does nothing and the host can run it again
+// from part A.
 __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict__ cnt,
                                                           FtGroup* __restrict__ groups,
                                                           const uint8_t* __restrict__ gflag,
                                                           uint32_t thin_max, uint32_t t64,
                                                           uint32_t* __restrict__ gtile0,
                                                           uint32_t* __restrict__ gthin0,
-                                                          uint64_t* __restrict__ gseg0)
+                                                          uint64_t* __restrict__ gseg0,
+                                                          int acc, int chain,
+                                                          uint32_t chain_mask, FtdCaps caps)
 {
-    constexpr int NQ = 11;
+    constexpr int NQ = 12;
     __shared__ uint64_t part[NQ][FTD_SCAN_NT];
+    if (cnt->halt)
+        return;
     // groups [lo, hi) of the current batch (all groups unless the host set a
     // range); the batch's work counters start at zero
     const uint32_t      lo  = cnt->g_lo;
@@ -555,27 +620,31 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
     const uint32_t      tid = threadIdx.x;
     const uint32_t      per = (ng + FTD_SCAN_NT - 1) / FTD_SCAN_NT;
     const uint32_t      g0 = lo + min(tid * per, ng), g1 = lo + min(tid * per + per, ng);
-    uint64_t            acc[NQ] = {};
+    uint64_t            sum[NQ] = {};
     uint32_t            smax    = 0;
     for (uint32_t g = g0; g < g1; ++g)
     {
         const FtdSizes z = ftd_sizes(groups[g], gflag[g], thin_max, t64);
-        acc[0] += z.ref_items;
-        acc[1] += z.cell_items;
-        acc[2] += z.ref_elems;
-        acc[3] += z.cell_elems;
-        acc[4] += z.scores;
-        acc[5] += z.tiles;
-        acc[6] += z.thin;
-        acc[7] += z.segs;
-        acc[8] += z.rthin;
-        acc[9] += z.etiles;
-        acc[10] += z.btiles;
+        sum[0] += z.ref_items;
+        sum[1] += z.cell_items;
+        sum[2] += z.ref_elems;
+        sum[3] += z.cell_elems;
+        sum[4] += z.scores;
+        sum[5] += z.tiles;
+        sum[6] += z.thin;
+        sum[7] += z.segs;
+        sum[8] += z.rthin;
+        sum[9] += z.etiles;
+        sum[10] += z.btiles;
         if (!(gflag[g] & FTD_SHORT))
-            smax = max(smax, groups[g].stride);
+        {
+            const FtGroup& G = groups[g];
+            smax = max(smax, G.stride);
+            sum[11] += (uint64_t)G.ncell * G.nref * G.gs;
+        }
     }
     for (int q = 0; q < NQ; ++q)
-        part[q][tid] = acc[q];
+        part[q][tid] = sum[q];
     __syncthreads();
     for (uint32_t off = 1; off < FTD_SCAN_NT; off <<= 1)
     {
@@ -589,7 +658,7 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
     }
     uint64_t run[NQ];
     for (int q = 0; q < NQ; ++q)
-        run[q] = part[q][tid] - acc[q];
+        run[q] = part[q][tid] - sum[q];
     for (uint32_t g = g0; g < g1; ++g)
     {
         FtGroup&       G = groups[g];
@@ -626,17 +695,48 @@ __global__ __launch_bounds__(FTD_SCAN_NT) void k_ftd_scan(FtdCounts* __restrict_
     }
     if (tid == FTD_SCAN_NT - 1)
     {
-        cnt->n_ref_items = part[0][tid];
-        cnt->n_cells     = (uint32_t)part[1][tid];
-        cnt->ref_elems   = part[2][tid];
-        cnt->cell_elems  = part[3][tid];
-        cnt->n_scores    = part[4][tid];
-        cnt->n_tiles     = (uint32_t)part[5][tid];
-        cnt->n_thin      = (uint32_t)part[6][tid];
-        cnt->n_segs      = part[7][tid];
-        cnt->rthin_bound = part[8][tid];
-        cnt->n_etiles    = (uint32_t)part[9][tid];
-        cnt->n_btiles    = (uint32_t)part[10][tid];
+        uint64_t t[NQ];
+        for (int q = 0; q < NQ; ++q)
+            t[q] = part[q][tid];
+        // (every table of a count is at least as large as the cap given)
+        const bool over = t[0] > caps.ref_items || t[1] > caps.cells || t[2] > caps.ref_elems ||
+                          t[3] > caps.cell_elems || t[4] > caps.scores || t[7] > caps.segs ||
+                          t[5] > caps.tiles || t[6] > caps.thin || t[8] > caps.rthin;
+        const uint32_t need = ftd_mask_need(t[2], (uint32_t)t[5], (uint32_t)t[9],
+                                            (uint32_t)t[10], (uint32_t)t[6], cnt->n_fused);
+        if (chain && (over || (need & ~chain_mask) || cnt->collision))
+        {
+            // the round is run again from its part A by the host: its
+            // active list (left in act_next by its compaction) is listed
+            // again and nothing of the round's work runs
+            cnt->halt     = 1;
+            cnt->n_next   = cnt->n_active;
+            cnt->round   -= 1;
+            cnt->g_lo     = 1;
+            cnt->g_hi     = 0;
+            for (int q = 0; q < NQ; ++q)
+                t[q] = 0;
+        }
+        else if (acc && !cnt->collision)
+        {
+            cnt->tot_rounds += cnt->n_active ? 1u : 0u;
+            cnt->tot_active += cnt->n_active;
+            cnt->tot_groups += cnt->n_groups;
+            cnt->tot_scores += t[4];
+            cnt->tot_fused += cnt->n_fused;
+            cnt->pair_genes += t[11];
+        }
+        cnt->n_ref_items = t[0];
+        cnt->n_cells     = (uint32_t)t[1];
+        cnt->ref_elems   = t[2];
+        cnt->cell_elems  = t[3];
+        cnt->n_scores    = t[4];
+        cnt->n_tiles     = (uint32_t)t[5];
+        cnt->n_thin      = (uint32_t)t[6];
+        cnt->n_segs      = t[7];
+        cnt->rthin_bound = t[8];
+        cnt->n_etiles    = (uint32_t)t[9];
+        cnt->n_btiles    = (uint32_t)t[10];
     }
 }
 
@@ -1768,6 +1868,8 @@ __global__ void k_ftd_compact(FtdCounts* __restrict__ cnt, const uint32_t* __res
                               const uint32_t* __restrict__ counts, int32_t* __restrict__ rounds,
                               int times, uint32_t* __restrict__ act_next)
 {
+    if (cnt->halt)
+        return;
     const uint32_t n = cnt->n_active;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
@@ -1777,6 +1879,13 @@ __global__ void k_ftd_compact(FtdCounts* __restrict__ cnt, const uint32_t* __res
         if (counts[c] > 1 && (times == 0 || r < times))
             act_next[atomicAdd(&cnt->n_next, 1u)] = c;
     }
+}
+
+// A halted chain's round is resumed from its part A by the host.
+__global__ void k_ftd_resume(FtdCounts* __restrict__ cnt)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        cnt->halt = 0;
 }
 
 // Counter reset for the autotuner's repeated select launches.

@@ -26,9 +26,10 @@ COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells",
             "ft_rounds", "ft_items", "ft_groups", "ft_pairs", "ft_pair_genes", "ft_redone_cells",
             "ft_host_lists_us", "ft_host_group_us", "ft_host_build_us", "ft_batch_us",
             "prop_cells", "prop_fallback_margin", "prop_fallback_check", "prop_scored_pairs",
-            "prop_widened", "prop_correlations", "ft_fused_groups"]
+            "prop_widened", "prop_correlations", "ft_fused_groups", "ft_chained_rounds",
+            "ft_chain_halts"]
 N_PHASES = 11
-N_COUNTERS = 23
+N_COUNTERS = 25
 
 _V = C.c_void_p
 _U32 = C.c_uint32
@@ -78,6 +79,7 @@ def lib():
     L.csg_finetune_tuning.argtypes = [_V, C.c_int, _V, _V]
     L.csg_set_finetune_budget.argtypes = [_V, _U64, _V]
     L.csg_set_finetune_fuse.argtypes = [_V, C.c_int]
+    L.csg_set_finetune_chain.argtypes = [_V, C.c_int, _U64]
     L.csg_set_query_chunk.argtypes = [_V, _U32, _V]
     L.csg_copyin_query_raw.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _V]
     L.csg_copyin_query_raw_u16.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _V]
@@ -460,6 +462,13 @@ class Engine:
     def set_finetune_fuse(self, max_cells: int):
         """Largest fine-tune group (cells) of the fused FP32 form (0 = off)."""
         self._check(lib().csg_set_finetune_fuse(self._h, int(max_cells)), "set_finetune_fuse")
+
+    def set_finetune_chain(self, max_rounds: int, max_scores: int = 0):
+        """Longest chain of fine-tune rounds launched without a host read-back
+        (0 = every round reads its counts back); a chained round with more than
+        max_scores correlations (0 = no limit) halts the chain and runs unchained."""
+        self._check(lib().csg_set_finetune_chain(self._h, int(max_rounds), int(max_scores)),
+                    "set_finetune_chain")
 
     def finetune_tuning(self, autotune: Optional[bool] = None) -> Dict[str, object]:
         """Autotuner state of the device-resident fine-tune (optionally switching

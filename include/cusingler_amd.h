@@ -131,6 +131,19 @@ int csg_set_finetune_schedule(csg_engine* eng, int schedule);
  * and round counts do not depend on it.  No counterpart in the reference. */
 int csg_set_finetune_fuse(csg_engine* eng, int max_cells);
 
+/* Longest chain of fine-tune rounds that the device-resident schedule
+ * launches together without reading anything back (default 0: chains off;
+ * at most 64).  Once a round has at most 1,024 active cells, the
+ * following rounds run as chains of captured rounds: a round removes at least
+ * one label from every cell, so a chain covers the rounds the longest
+ * candidate list can still take.  Each chained round checks on the device
+ * that its work fits the tables and the launch mask the chain was captured
+ * with, and that its correlations (cells x reference rows) are at most
+ * max_scores (0: no limit); one that does not halts the chain with its state
+ * intact, and the host runs it again unchained.  Labels and round counts do
+ * not depend on these settings.  No counterpart in the reference. */
+int csg_set_finetune_chain(csg_engine* eng, int max_rounds, uint64_t max_scores);
+
 /* Autotuner of the device-resident fine-tune: on the first round with enough
  * work after the fine-tune inputs change, the block counts of the reference
  * rank, correlation and select work-queue kernels and the exact correlation
@@ -341,8 +354,12 @@ int csg_set_phase_timing(csg_engine* eng, int mode);
  * scored, [20] cells widened to their second group of ranked labels,
  * [21] correlations the proposal path computed (sum over its pairs of the
  * label's reference cells), [22] fine-tune groups of the device-resident
- * schedule that took the fused FP32 form (csg_set_finetune_fuse). */
-#define CSG_N_COUNTERS 23
+ * schedule that took the fused FP32 form (csg_set_finetune_fuse), [23]
+ * fine-tune rounds launched in chains without a host read-back (including
+ * rounds past the last one, which do nothing), [24] chains halted by a
+ * round that needed larger work tables, another launch mask or the host
+ * schedule (csg_set_finetune_chain). */
+#define CSG_N_COUNTERS 25
 int csg_counters(csg_engine* eng, int64_t* out);
 
 /* ---- learned label proposal (no counterpart in the reference) ----
