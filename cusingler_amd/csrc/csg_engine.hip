@@ -31,6 +31,7 @@
 #include "csg_train.h"
 #include "csg_ft_kernels.h"
 #include "csg_ftd.h"
+#include "csg_ftd_tail.h"
 #include "csg_kernels.h"
 #include "csg_mfma.h"
 #include "csg_mlp.h"
@@ -259,6 +260,23 @@ struct csg_engine
     };
     std::vector<FtdChainGraph> ftd_gC;
     int                        ftd_chain_max = 0;  // longest chain (rounds); 0: no chains
+    // the last rounds in one cooperative launch (csg_ftd_tail.h)
+    uint32_t      ftd_tail_max = 64;  // most active cells handed to it (0: never)
+    bool          tt_ran = false;       // the last fine-tune ended in it
+    int           tt_refused = 0;  // why it was last refused: 1 LDS, 2 no cooperative
+                                   // launch, 3 no resident block, 4 launch refused
+    int           tt_blocks_per_cu = 4;
+    size_t        tt_lds_limit = 65536;
+    FtdTailState* d_tt_state = nullptr;
+    FtdTailState* h_tt_state = nullptr;  // pinned
+    FtdTailSlot*  d_tt_slots = nullptr;
+    FtdTailChunk* d_tt_chunks = nullptr;
+    uint32_t*     d_tt_genes = nullptr;
+    int16_t*      d_tt_crank = nullptr;
+    float *       d_tt_scores = nullptr, *d_tt_P = nullptr;
+    uint64_t*     d_tt_nanl = nullptr;
+    size_t tt_cap_state = 0, tt_cap_slots = 0, tt_cap_chunks = 0, tt_cap_genes = 0,
+           tt_cap_crank = 0, tt_cap_scores = 0, tt_cap_P = 0, tt_cap_nanl = 0;
     uint64_t                   ftd_chain_scores = 0;  // most scores of a chained round (0: any)
     // launch settings of the work-queue kernels (blocks), tunable
     int   ftd_grid_threads = 1024, ftd_grid_setup = 512, ftd_grid_rank = 2048,
@@ -630,6 +648,30 @@ int csg_init_gpu(int gpuid, csg_engine** out)
         if (ok1 && ok2 && ok3)
             eng->lds_limit = (size_t)want;
         (void)hipGetLastError();  // a refused attribute must not surface at the next launch
+    }
+    // the fine-tune tail kernel: as much dynamic LDS as the device gives a
+    // block beside the kernel's static LDS (its sort stack: about 3.4 KB)
+    eng->tt_lds_limit = 65536;
+    {
+        hipFuncAttributes fa8{}, fa16{};
+        if (hipFuncGetAttributes(&fa8, (const void*)k_ftd_tail<uint8_t>) == hipSuccess &&
+            hipFuncGetAttributes(&fa16, (const void*)k_ftd_tail<uint16_t>) == hipSuccess)
+        {
+            const size_t st   = std::max(fa8.sharedSizeBytes, fa16.sharedSizeBytes);
+            const size_t have = (size_t)eng->prop.sharedMemPerBlock;
+            if (have > 65536 + st)
+            {
+                const int want = (int)(have - st);
+                if (hipFuncSetAttribute((const void*)k_ftd_tail<uint8_t>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        want) == hipSuccess &&
+                    hipFuncSetAttribute((const void*)k_ftd_tail<uint16_t>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        want) == hipSuccess)
+                    eng->tt_lds_limit = (size_t)want;
+            }
+        }
+        (void)hipGetLastError();
     }
     // the wide INT8 kernels stage three K steps of operands in dynamic LDS
     eng->w8_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)W8_LDS &&
@@ -2385,6 +2427,19 @@ int ftd_free(csg_engine* eng)
     dfree(eng->d_fl_refgroup);
     dfree(eng->d_fl_refrow);
     eng->fl_cap = {};
+    dfree(eng->d_tt_state);
+    dfree(eng->d_tt_slots);
+    dfree(eng->d_tt_chunks);
+    dfree(eng->d_tt_genes);
+    dfree(eng->d_tt_crank);
+    dfree(eng->d_tt_scores);
+    dfree(eng->d_tt_P);
+    dfree(eng->d_tt_nanl);
+    eng->tt_cap_state = eng->tt_cap_slots = eng->tt_cap_chunks = eng->tt_cap_genes = 0;
+    eng->tt_cap_crank = eng->tt_cap_scores = eng->tt_cap_P = eng->tt_cap_nanl = 0;
+    if (eng->h_tt_state)
+        (void)hipHostFree(eng->h_tt_state);
+    eng->h_tt_state = nullptr;
     eng->ftd_gcap = 0;
     eng->ftd_T    = 0;
     if (eng->h_fl_cnt)
@@ -3216,6 +3271,105 @@ int ftd_batches(csg_engine* eng, const FtdCounts& h, float quantile, float thre,
     return CSG_OK;
 }
 
+// The rounds from the current one on, for its n active cells (part A's
+// list, counts and lists at the round's start), as one cooperative launch of
+// k_ftd_tail.  *ran stays false (nothing launched) when the kernel's LDS or a
+// co-resident grid does not fit.
+int ftd_tail(csg_engine* eng, uint32_t n, uint32_t max_count, float quantile, float thre,
+             int times, bool* ran)
+{
+    *ran               = false;
+    const uint32_t Gf  = std::max<uint32_t>(eng->Gf, 1), L = eng->fL;
+    const FtdLds   d   = ftd_lds(eng);
+    const size_t   lds = ftd_tail_lds(Gf, eng->fnbins, d.wave_ints);
+    eng->tt_refused    = lds > eng->tt_lds_limit ? 1 : !eng->prop.cooperativeLaunch ? 2 : 0;
+    if (eng->tt_refused || n == 0)
+        return CSG_OK;
+    const void* fn = eng->d_fref_codes8 ? (const void*)k_ftd_tail<uint8_t>
+                                        : (const void*)k_ftd_tail<uint16_t>;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, FTT_NT, lds) != hipSuccess ||
+        per < 1)
+    {
+        (void)hipGetLastError();
+        eng->tt_refused = 3;
+        return CSG_OK;
+    }
+    const int cu   = std::max(1, eng->prop.multiProcessorCount);
+    const int grid = std::min(per, std::max(1, eng->tt_blocks_per_cu)) * cu;
+    // score rows: a cell's labels are at most the max_count largest ones
+    std::vector<uint32_t> sz(L);
+    for (uint32_t l = 0; l < L; ++l)
+        sz[l] = eng->fctidx[2 * l + 1];
+    std::sort(sz.begin(), sz.end(), std::greater<uint32_t>());
+    uint64_t top = 0;
+    for (uint32_t i = 0; i < std::min(max_count, L); ++i)
+        top += sz[i];
+    const uint64_t rows   = std::max<uint64_t>((uint64_t)n * top, 1);
+    const uint64_t chunks = rows / FTT_CH + n;
+    const uint32_t gst    = round_up(Gf, 8);
+    int            rc;
+    if ((rc = ensure(eng, &eng->d_tt_state, &eng->tt_cap_state, 1)) ||
+        (rc = ensure(eng, &eng->d_tt_slots, &eng->tt_cap_slots, n)) ||
+        (rc = ensure(eng, &eng->d_tt_chunks, &eng->tt_cap_chunks, chunks)) ||
+        (rc = ensure(eng, &eng->d_tt_genes, &eng->tt_cap_genes, (size_t)n * gst)) ||
+        (rc = ensure(eng, &eng->d_tt_crank, &eng->tt_cap_crank, (size_t)n * gst)) ||
+        (rc = ensure(eng, &eng->d_tt_scores, &eng->tt_cap_scores, rows)) ||
+        (rc = ensure(eng, &eng->d_tt_P, &eng->tt_cap_P, (size_t)n * L)) ||
+        (rc = ensure(eng, &eng->d_tt_nanl, &eng->tt_cap_nanl, (size_t)n * L)))
+        return rc;
+    if (!eng->h_tt_state)
+        HIPCHK(hipHostMalloc((void**)&eng->h_tt_state, sizeof(FtdTailState), hipHostMallocDefault));
+    HIPCHK(hipMemsetAsync(eng->d_tt_state, 0, sizeof(FtdTailState), eng->stream));
+    FtdTailArgs a{};
+    a.cnt       = eng->d_fl_cnt;
+    a.st        = eng->d_tt_state;
+    a.slots     = eng->d_tt_slots;
+    a.chunks    = eng->d_tt_chunks;
+    a.act       = eng->d_fl_act;
+    a.n         = n;
+    a.lists     = eng->d_fl_lists;
+    a.counts    = eng->d_fl_counts;
+    a.rounds    = eng->d_fl_rounds;
+    a.L         = L;
+    a.times     = times;
+    a.ctdidx    = eng->d_fl_ctdidx;
+    a.ctdiff    = eng->d_fl_ctdiff;
+    a.cut_tab   = eng->d_fl_cut;
+    a.lstart    = eng->d_flstart;
+    a.llen      = eng->d_fllen;
+    a.Gf        = Gf;
+    a.gst       = gst;
+    a.nbins     = eng->fnbins;
+    a.wave_ints = d.wave_ints;
+    a.qdata     = eng->d_fqdata;
+    a.qind      = eng->d_fqind;
+    a.qptr      = eng->d_fqptr;
+    a.genes     = eng->d_tt_genes;
+    a.crank     = eng->d_tt_crank;
+    a.scores    = eng->d_tt_scores;
+    a.P         = eng->d_tt_P;
+    a.nanl      = eng->d_tt_nanl;
+    a.pct       = eng->d_fpct;
+    a.quantile  = quantile;
+    a.thre      = thre;
+    const void* codes = eng->d_fref_codes8 ? (const void*)eng->d_fref_codes8
+                                           : (const void*)eng->d_fref_codes;
+    void* params[] = { (void*)&a, (void*)&codes };
+    if (hipLaunchCooperativeKernel(fn, dim3((unsigned)grid), dim3(FTT_NT), params,
+                                   (unsigned)lds, eng->stream) != hipSuccess)
+    {
+        // (refused: the rounds go on one launch sequence at a time)
+        (void)hipGetLastError();
+        eng->tt_refused = 4;
+        return CSG_OK;
+    }
+    HIPCHK(hipMemcpyAsync(eng->h_tt_state, eng->d_tt_state, sizeof(FtdTailState),
+                          hipMemcpyDeviceToHost, eng->stream));
+    *ran = true;
+    return CSG_OK;
+}
+
 // The whole fine-tune on the device.  Returns CSG_OK with *fallback set when
 // a hash collision was met: lists / counts / rounds then hold the state at
 // the start of that round for the host-grouped schedule to continue from.
@@ -3270,6 +3424,8 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
     // opens round m - 1 (which then has no active cells)
     uint32_t max_count = L;
     bool     comp      = false;  // part A opens with the previous round's compaction
+    bool     tail      = false;  // the last rounds ran in the tail kernel
+    eng->tt_ran        = false;
     auto     state_to_host = [&]() -> int {
         // state at the start of the current round, for the host schedule
         *fallback = true;
@@ -3288,6 +3444,18 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
         eng->counters[9] += (int64_t)h.tot_scores;
         eng->counters[22] += (int64_t)h.tot_fused;
         eng->counters[10] += (int64_t)h.pair_genes;
+        if (tail)
+        {
+            // rounds after the tail's first (which part A counted); each
+            // cell a group of its own
+            const FtdTailState& t = *eng->h_tt_state;
+            eng->counters[6] += t.rounds;
+            eng->counters[7] += (int64_t)t.tot_active;
+            eng->counters[8] += (int64_t)t.tot_active;
+            eng->counters[9] += (int64_t)t.tot_scores;
+            eng->counters[10] += (int64_t)t.pair_genes;
+            eng->counters[25] += t.rounds + 1;
+        }
     };
     for (;;)
     {
@@ -3353,7 +3521,20 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
             add_totals();
             return state_to_host();
         }
-        max_count        = h.max_count;
+        max_count = h.max_count;
+        if (eng->ftd_tail_max && h.n_active <= eng->ftd_tail_max)
+        {
+            // this round and the rest in one launch (part A's grouping unused)
+            bool ran = false;
+            if ((rc = ftd_tail(eng, h.n_active, h.max_count, quantile, thre, times, &ran)))
+                return rc;
+            if (ran)
+            {
+                tail = eng->tt_ran = true;
+                eng->counters[15] += now_us() - t0;
+                break;
+            }
+        }
         eng->ftd_small_a = h.n_active <= (uint32_t)FTD_SMALL_NT;
         eng->ftd_mask    = ftd_mask_of(eng, h);
         if (ftd_bytes(h, L) <= eng->ftd_budget)
@@ -3400,6 +3581,8 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
     HIPCHK(hipStreamSynchronize(eng->stream));
     eng->counters[11] += h.tot_redo;
     add_totals();
+    if (tail && eng->h_tt_state->timeout)
+        return eng->fail(CSG_ERR_DEVICE, "the fine-tune tail kernel's grid barrier timed out");
     if (h.undefined)
         return eng->fail(CSG_ERR_UNDEFINED,
                          "a NaN-carrying fine-tune segment makes the reference's sort leave its "
@@ -3625,7 +3808,7 @@ int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finet
         return rc;
     for (int i = 6; i < 16; ++i)  // the score step's own counters stay
         eng->counters[i] = 0;
-    for (int i : { 22, 23, 24 })
+    for (int i : { 22, 23, 24, 25 })
         eng->counters[i] = 0;
     PhaseScope    total(eng, 5, 0);
     const int64_t t_start = now_us();
@@ -4515,6 +4698,35 @@ int csg_set_finetune_fuse(csg_engine* eng, int max_cells)
     if (max_cells < 0 || max_cells > FTD_FUSE_MAX)
         return eng->fail(CSG_ERR_ARGUMENT, "fused fine-tune groups hold 0 to 4 cells");
     eng->ftd_fuse_max = max_cells;
+    return CSG_OK;
+}
+
+int csg_finetune_tail_profile(csg_engine* eng, double* us, int max, int* count)
+{
+    if (!eng || !us || !count || max < 0)
+        return CSG_ERR_ARGUMENT;
+    *count = -eng->tt_refused;
+    if (!eng->h_tt_state || !eng->tt_ran)
+        return CSG_OK;
+    const FtdTailState& t = *eng->h_tt_state;
+    // barriers passed: the first, then two per round with cells and the
+    // last one (the round that found no cell)
+    const int nb = std::min<int>(FTT_STAMPS, 1 + 2 * ((int)t.rounds + 1));
+    int       n  = 0;
+    for (int g = 1; g < nb && n < max; ++g, ++n)
+        us[n] = (double)(t.stamp[g] - t.stamp[g - 1]) / 100.0;  // 100 MHz clock
+    *count = n;
+    return CSG_OK;
+}
+
+int csg_set_finetune_tail(csg_engine* eng, uint32_t max_cells, int blocks_per_cu)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (blocks_per_cu < 1 || blocks_per_cu > 16)
+        return eng->fail(CSG_ERR_ARGUMENT, "the tail kernel takes 1 to 16 blocks per CU");
+    eng->ftd_tail_max     = max_cells;
+    eng->tt_blocks_per_cu = blocks_per_cu;
     return CSG_OK;
 }
 

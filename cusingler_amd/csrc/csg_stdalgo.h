@@ -216,23 +216,29 @@ CSG_HD inline void unguarded_insertion_sort(float* first, float* last, const flo
     }
 }
 
-// std::sort(a, a + n) with operator<.  Returns false if the library's
-// unguarded scans would have left the array (then the reference itself has
-// undefined behaviour and no result is claimed).
-CSG_HD inline bool std_sort(float* a, long n)
+// Pending ranges of std_sort (the library's recursion, made explicit): at
+// most 2 floor(log2 n) + 1 are open at once.
+struct StdSortRange
+{
+    float* first;
+    float* last;
+    int    depth;
+};
+constexpr int kStdSortStack = 130;
+
+// std::sort(a, a + n) with operator<, its pending ranges in `stack`
+// (kStdSortStack entries: LDS for a device caller that must not spill them
+// to scratch).  Returns false if the library's unguarded scans would have
+// left the array (then the reference itself has undefined behaviour and no
+// result is claimed).
+CSG_HD inline bool std_sort(float* a, long n, StdSortRange* stack)
 {
     if (n <= 1)
         return true;
     bool        oob = false;
     const float* lo = a;
     const float* hi = a + n;
-    struct Range
-    {
-        float* first;
-        float* last;
-        int    depth;
-    };
-    Range stack[130];
+    using Range = StdSortRange;
     int   sp   = 0;
     stack[sp++] = Range{ a, a + n, 2 * floor_log2(n) };
     while (sp > 0 && !oob)
@@ -271,6 +277,13 @@ CSG_HD inline bool std_sort(float* a, long n)
     else
         insertion_sort(a, a + n);
     return !oob;
+}
+
+// std::sort(a, a + n) with its pending ranges on the caller's stack.
+CSG_HD inline bool std_sort(float* a, long n)
+{
+    StdSortRange stack[kStdSortStack];
+    return std_sort(a, n, stack);
 }
 
 // std::minmax_element(a, a + n) with operator<: writes the indices of the

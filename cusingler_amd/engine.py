@@ -27,9 +27,9 @@ COUNTERS = ["cells", "G_s", "ref_cells", "rechecked_pairs", "uncertified_cells",
             "ft_host_lists_us", "ft_host_group_us", "ft_host_build_us", "ft_batch_us",
             "prop_cells", "prop_fallback_margin", "prop_fallback_check", "prop_scored_pairs",
             "prop_widened", "prop_correlations", "ft_fused_groups", "ft_chained_rounds",
-            "ft_chain_halts"]
+            "ft_chain_halts", "ft_tail_rounds"]
 N_PHASES = 11
-N_COUNTERS = 25
+N_COUNTERS = 26
 
 _V = C.c_void_p
 _U32 = C.c_uint32
@@ -80,6 +80,8 @@ def lib():
     L.csg_set_finetune_budget.argtypes = [_V, _U64, _V]
     L.csg_set_finetune_fuse.argtypes = [_V, C.c_int]
     L.csg_set_finetune_chain.argtypes = [_V, C.c_int, _U64]
+    L.csg_set_finetune_tail.argtypes = [_V, _U32, C.c_int]
+    L.csg_finetune_tail_profile.argtypes = [_V, _V, C.c_int, _V]
     L.csg_set_query_chunk.argtypes = [_V, _U32, _V]
     L.csg_copyin_query_raw.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _V]
     L.csg_copyin_query_raw_u16.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _V]
@@ -469,6 +471,23 @@ class Engine:
         max_scores correlations (0 = no limit) halts the chain and runs unchained."""
         self._check(lib().csg_set_finetune_chain(self._h, int(max_rounds), int(max_scores)),
                     "set_finetune_chain")
+
+    def set_finetune_tail(self, max_cells: int, blocks_per_cu: int = 4):
+        """Rounds with at most max_cells active cells (and all later ones) run in
+        one cooperative launch (0 = never)."""
+        self._check(lib().csg_set_finetune_tail(self._h, int(max_cells), int(blocks_per_cu)),
+                    "set_finetune_tail")
+
+    def finetune_tail_profile(self) -> np.ndarray:
+        """Microseconds between the last tail kernel's grid barriers (the first
+        round's begin, then per round: rows, end + next begin)."""
+        out = np.zeros(64, np.float64)
+        n = C.c_int()
+        self._check(lib().csg_finetune_tail_profile(self._h, _p(out), 64, C.byref(n)),
+                    "finetune_tail_profile")
+        if n.value < 0:
+            raise CsgError(0, f"the tail kernel was refused (reason {-n.value})")
+        return out[:n.value]
 
     def finetune_tuning(self, autotune: Optional[bool] = None) -> Dict[str, object]:
         """Autotuner state of the device-resident fine-tune (optionally switching

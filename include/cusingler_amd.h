@@ -144,6 +144,27 @@ int csg_set_finetune_fuse(csg_engine* eng, int max_cells);
  * not depend on these settings.  No counterpart in the reference. */
 int csg_set_finetune_chain(csg_engine* eng, int max_rounds, uint64_t max_scores);
 
+/* The device-resident schedule's last rounds in one cooperative launch: once
+ * a round has at most max_cells active cells (default 64; 0 never), that
+ * round and every later one run in a persistent kernel of blocks_per_cu
+ * blocks of 512 threads per CU (default 4; 1 to 16, fewer if they cannot be
+ * co-resident).
+ * Each cell is a group of its own; per round the cells' owner blocks build
+ * G_sub and rank the cell, every wave ranks and correlates reference rows,
+ * and the owner blocks select the quantiles and filter, with two grid
+ * barriers per round and no host involvement.  The scores are the fused
+ * FP32 form's, so labels and round counts do not depend on these settings.
+ * No counterpart in the reference. */
+int csg_set_finetune_tail(csg_engine* eng, uint32_t max_cells, int blocks_per_cu);
+
+/* Phase durations of the last fine-tune's tail kernel, in microseconds of
+ * the device clock between successive grid barriers: [0] the first round's
+ * begin, then per round the rows phase and the end + next begin phase.
+ * *count receives how many were written (0 when the tail did not run, or
+ * minus the reason it was last refused: 1 its LDS, 2 no cooperative launch,
+ * 3 no co-resident block, 4 the launch refused). */
+int csg_finetune_tail_profile(csg_engine* eng, double* us, int max, int* count);
+
 /* Autotuner of the device-resident fine-tune: on the first round with enough
  * work after the fine-tune inputs change, the block counts of the reference
  * rank, correlation and select work-queue kernels and the exact correlation
@@ -358,8 +379,9 @@ int csg_set_phase_timing(csg_engine* eng, int mode);
  * fine-tune rounds launched in chains without a host read-back (including
  * rounds past the last one, which do nothing), [24] chains halted by a
  * round that needed larger work tables, another launch mask or the host
- * schedule (csg_set_finetune_chain). */
-#define CSG_N_COUNTERS 25
+ * schedule (csg_set_finetune_chain), [25] fine-tune rounds run by the tail
+ * kernel (csg_set_finetune_tail). */
+#define CSG_N_COUNTERS 26
 int csg_counters(csg_engine* eng, int64_t* out);
 
 /* ---- learned label proposal (no counterpart in the reference) ----
