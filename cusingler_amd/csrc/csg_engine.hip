@@ -2572,7 +2572,8 @@ FtdLds ftd_lds(const csg_engine* eng)
     const size_t nb = (size_t)((eng->fnbins + 3) & ~3) * 4;
     FtdLds       d;
     // bit words and a flag byte per gene (the flags padded to whole words)
-    d.setup    = ((size_t)std::max<uint32_t>(eng->Gf, 1) + 31) / 32 * (4 + 32);
+    d.setup    = ((size_t)std::max<uint32_t>(eng->Gf, 1) + 31) / 32 * (4 + 32) +
+              (size_t)FTD_PAIRS * 8;
     d.rank_ref = nb + st * 4;
     d.rank_qry = nb + st * 2 + st * 2;
     // the correlation tiles walk K in LDS stages, so any gene subset fits

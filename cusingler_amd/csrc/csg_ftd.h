@@ -397,6 +397,66 @@ __global__ __launch_bounds__(FTD_SMALL_NT) void k_ftd_group_small(
 // groups of at most `fuse_max` cells take the fused FP32 form instead.
 constexpr int FTD_SETUP_NT = 1024;
 
+// Ordered pair windows staged in LDS at a time by ftd_flag_windows.
+constexpr int FTD_PAIRS = 1024;
+
+// The gene flags of a candidate set's G_sub: for every ordered pair (i, j),
+// i != j, of lst[0..k), flag[g] = 1 for the first min(len_ij, cut) genes g
+// of the pair's window (src/cusingler.cu:1144-1156).  A whole block of NT
+// threads: the windows' (pos, len) are staged in `pw` (FTD_PAIRS entries of
+// LDS) by parallel loads, then every thread flags genes over the flattened
+// (pair, gene) range with eight independent loads in flight -- a set of k
+// labels has k (k - 1) windows, and walking them one wave per window left
+// the loads of a large set serialised.  Plain stores of the same value; the
+// caller clears `flag` before and synchronises after.
+template <int NT, class LT>
+__device__ inline void ftd_flag_windows(const LT* lst, uint32_t k, uint32_t L, uint32_t cut,
+                                        const uint32_t* __restrict__ ctdidx,
+                                        const uint32_t* __restrict__ ctdiff, uint8_t* flag,
+                                        uint2* pw)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t np  = k * k;
+    for (uint32_t p0 = 0; p0 < np; p0 += FTD_PAIRS)
+    {
+        const uint32_t pn = min(np - p0, (uint32_t)FTD_PAIRS);
+        __syncthreads();  // (flags cleared; the previous batch's windows read)
+        for (uint32_t q = tid; q < pn; q += NT)
+        {
+            const uint32_t pi = p0 + q;
+            const uint32_t i = lst[pi / k], j = lst[pi % k];
+            uint2          w{ 0u, 0u };
+            if (i != j)
+            {
+                w.x = ctdidx[((size_t)i * L + j) * 2];
+                w.y = min(ctdidx[((size_t)i * L + j) * 2 + 1], cut);
+            }
+            pw[q] = w;
+        }
+        __syncthreads();
+        // (a window holds at most `cut` genes: pair q, gene t at flattened
+        // index q * cut + t)
+        const uint32_t tot = pn * cut;
+        for (uint32_t e0 = tid; e0 < tot; e0 += 8 * NT)
+        {
+            uint32_t g[8];
+            bool     v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+            {
+                const uint32_t e = e0 + u * NT;
+                const uint32_t q = e / cut, t = e - q * cut;
+                v[u]             = e < tot && t < pw[min(q, pn - 1)].y;
+                g[u]             = v[u] ? ctdiff[pw[q].x + t] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (v[u])
+                    flag[g[u]] = 1;
+        }
+    }
+}
+
 __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
     FtdCounts* __restrict__ cnt, const uint32_t* __restrict__ gowner,
     const uint32_t* __restrict__ gcount, const uint8_t* __restrict__ lists,
@@ -407,7 +467,8 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
     uint32_t* __restrict__ glabels,
     uint32_t* __restrict__ glpos, uint32_t* __restrict__ ggenes)
 {
-    // dynamic LDS: ceil(Gf / 32) bit words, then one flag byte per gene
+    // dynamic LDS: ceil(Gf / 32) bit words, one flag byte per gene (whole
+    // words), FTD_PAIRS staged pair windows
     extern __shared__ uint32_t sbits[];
     __shared__ uint32_t slab[256];
     __shared__ uint32_t spart[FTD_SETUP_NT];
@@ -471,19 +532,11 @@ __global__ __launch_bounds__(FTD_SETUP_NT) void k_ftd_setup(
         __syncthreads();
         const uint32_t k   = sk;
         const uint32_t cut = k ? cut_tab[k] : 0u;
-        // ordered pairs spread over the waves, genes over the lanes: a flag
-        // byte per gene (plain stores of the same value, no atomics on the
-        // few words a small gene set has)
-        for (uint32_t pi = tid >> 6; pi < k * k; pi += FTD_SETUP_NT / 64)
-        {
-            const uint32_t i = slab[pi / k], j = slab[pi % k];
-            if (i == j)
-                continue;
-            const uint32_t pos = ctdidx[((size_t)i * L + j) * 2];
-            const uint32_t len = min(ctdidx[((size_t)i * L + j) * 2 + 1], cut);
-            for (uint32_t t = tid & 63; t < len; t += 64)
-                gflags[ctdiff[pos + t]] = 1;
-        }
+        // a flag byte per gene of every ordered pair's window (plain stores
+        // of the same value, no atomics on the few words a small gene set has)
+        if (k > 1)
+            ftd_flag_windows<FTD_SETUP_NT>(slab, k, L, cut, ctdidx, ctdiff, gflags,
+                                           (uint2*)(gflags + (size_t)W * 32));
         __syncthreads();
         for (uint32_t w = tid; w < W; w += FTD_SETUP_NT)
         {

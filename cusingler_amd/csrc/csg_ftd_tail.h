@@ -105,9 +105,6 @@ struct FtdTailArgs
     float            thre;
 };
 
-// Pair windows the begin phase stages in LDS at a time (pos, len each).
-constexpr int FTT_PAIRS = 1024;
-
 // Per-block LDS of the tail kernel: the largest of the begin phase's, the
 // end phase's and the row phase's per-wave tables.
 __host__ __device__ inline size_t ftd_tail_lds(uint32_t Gf, int nbins, int wave_ints)
@@ -116,7 +113,7 @@ __host__ __device__ inline size_t ftd_tail_lds(uint32_t Gf, int nbins, int wave_
     const size_t nb    = ((size_t)nbins + 3) & ~(size_t)3;
     // begin: gene flags (bytes), gene list (u16), dense query row (u16),
     // codes (u16), bins, then the staged pair windows
-    const size_t begin = 4 * (gw + 3 * ((Gf + 1) / 2 + 1) + nb + 2 * FTT_PAIRS) + 64;
+    const size_t begin = 4 * (gw + 3 * ((Gf + 1) / 2 + 1) + nb + 2 * FTD_PAIRS) + 64;
     // end: list offsets, the new list, a wave select table per wave
     const size_t endp  = 4 * 256 + 256 + FTT_NW * sizeof(WaveSelectSmem) + 64;
     const size_t rows  = (size_t)FTT_NW * wave_ints * 4;
@@ -183,55 +180,14 @@ __device__ inline void ftd_tail_begin(const FtdTailArgs& a, uint32_t s, const ui
     uint16_t*     codes = dense + 2 * hw;                      // codes at G_sub
     int*          bins  = (int*)(codes + 2 * hw);              // nbins
     uint2*        pw    = (uint2*)(((uintptr_t)(bins + (((size_t)a.nbins + 3) & ~(size_t)3)) + 7) &
-                              ~(uintptr_t)7);  // FTT_PAIRS windows, 8-byte aligned
+                              ~(uintptr_t)7);  // FTD_PAIRS windows, 8-byte aligned
     __shared__ int s_tmp[FTT_NW];
     __shared__ int s_gs;
     for (size_t w = tid; w < gw; w += FTT_NT)
         ((uint32_t*)flag)[w] = 0u;
-    // the pair windows: FTT_PAIRS ordered pairs' (pos, len) staged in LDS by
-    // parallel loads, then their genes flagged by every thread over the
-    // flattened (pair, gene) range, eight independent loads in flight per
-    // thread (plain stores of the same value)
+    // the ordered pairs' windows (k > 1 for a cell taking part in a round)
     const uint32_t cut = a.cut_tab[k];
-    const uint32_t np  = k * k;
-    for (uint32_t p0 = 0; p0 < np; p0 += FTT_PAIRS)
-    {
-        const uint32_t pn = min(np - p0, (uint32_t)FTT_PAIRS);
-        __syncthreads();  // (flags cleared; the previous batch's windows read)
-        for (uint32_t q = tid; q < pn; q += FTT_NT)
-        {
-            const uint32_t pi = p0 + q;
-            const uint32_t i = lst[pi / k], j = lst[pi % k];
-            uint2          w{ 0u, 0u };
-            if (i != j)
-            {
-                w.x = a.ctdidx[((size_t)i * a.L + j) * 2];
-                w.y = min(a.ctdidx[((size_t)i * a.L + j) * 2 + 1], cut);
-            }
-            pw[q] = w;
-        }
-        __syncthreads();
-        // (the windows hold at most `cut` genes each: pair q, gene t at
-        // flattened index q * cut + t)
-        const uint32_t tot = pn * cut;
-        for (uint32_t e0 = tid; e0 < tot; e0 += 8 * FTT_NT)
-        {
-            uint32_t g[8];
-            bool     v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-            {
-                const uint32_t e = e0 + u * FTT_NT;
-                const uint32_t q = e / cut, t = e - q * cut;
-                v[u]             = e < tot && t < pw[min(q, pn - 1)].y;
-                g[u]             = v[u] ? a.ctdiff[pw[q].x + t] : 0u;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (v[u])
-                    flag[g[u]] = 1;
-        }
-    }
+    ftd_flag_windows<FTT_NT>(lst, k, a.L, cut, a.ctdidx, a.ctdiff, flag, pw);
     __syncthreads();
     // ascending gene list: contiguous gene ranges per thread, block scan
     const uint32_t per = (Gf + FTT_NT - 1) / FTT_NT;
