@@ -60,9 +60,12 @@ struct FtdTailState
     uint32_t nact[2];    // cells taking part in the round of parity p
     uint64_t rows;       // score rows reserved by the slots
     uint32_t rounds;     // rounds with at least one cell
-    uint32_t bar;        // grid barrier: blocks arrived, over all barriers
+    uint32_t bar0;       // the first barrier: blocks arrived
+    uint32_t bar;        // later barriers: XCD groups arrived, over all barriers
     uint32_t timeout;    // a barrier wait gave up (the results are void)
     uint32_t pad_;
+    uint32_t xn[16];     // blocks per XCD (counted at the start)
+    uint32_t xarr[16];   // blocks of each XCD arrived, over all later barriers
     uint64_t tot_active, tot_scores, pair_genes;
     // device clock (s_memrealtime, 100 MHz) when block 0 left barrier g, for
     // g < FTT_STAMPS: per-phase durations for the host's phase report
@@ -120,40 +123,89 @@ __host__ __device__ inline size_t ftd_tail_lds(uint32_t Gf, int nbins, int wave_
     return std::max(begin, std::max(endp, rows));
 }
 
-// Grid barrier number `gen` (0, 1, ...) of the cooperative launch (which
-// makes every block co-resident): the block's threads meet, then one thread
-// publishes the block's writes device-wide (release), arrives on the counter
-// and waits until every block has arrived, and takes the other blocks'
-// writes (acquire) for the whole block -- one fence pair per block rather
-// than per thread.  The wait is bounded (about a second): a barrier that
-// never fills sets `timeout`; every block that sees it returns (so all of
-// them end within about a second of the first) and the host reports the
-// launch as failed.  Returns false (in every thread) after a timeout.
-__device__ inline bool ftd_tail_sync(FtdTailState* st, uint32_t gen)
+// The XCD (accelerator complex die) a wave runs on: blocks of one XCD share
+// its L2, the device's coherence point for them.
+__device__ inline uint32_t ftd_xcc_id()
+{
+    // hardware register XCC_ID (20), bits 3:0
+    return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15u;
+}
+
+// Bounded wait until *ctr >= target (about a second at most): false after a
+// timeout, which is recorded for every block to see.
+__device__ inline bool ftd_tail_wait(FtdTailState* st, const uint32_t* ctr, uint32_t target)
+{
+    for (uint32_t spin = 0;; ++spin)
+    {
+        if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target)
+            return true;
+        if (spin > (1u << 24))
+        {
+            __hip_atomic_fetch_or(&st->timeout, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Grid barriers of the cooperative launch (which makes every block
+// co-resident).  A block's writes reach its XCD's L2 once its waves have
+// waited for their stores; an agent-scope release (an L2 write-back) then
+// publishes everything that L2 holds, and an acquire (an L1/L2
+// invalidation) takes in what the other XCDs published.
+//   * The first barrier: every block releases, arrives on `bar0` and waits
+//     for all (it also counts the blocks of each XCD, `xn`).
+//   * Later barriers (`gen` = 0, 1, ...): a block waits for its stores and
+//     arrives on its XCD's counter; the last block of an XCD to arrive
+//     releases that XCD's L2 once for all of them and arrives on `bar`; every
+//     block waits for all XCDs and acquires.  One write-back per XCD per
+//     barrier instead of one per block (each write-back walks the L2).
+// A wait that does not end within about a second sets `timeout`; every block
+// that sees it returns, so all of them end and the host reports the launch
+// as failed.  Both return false (in every thread) after a timeout.
+__device__ inline bool ftd_tail_sync_first(FtdTailState* st)
 {
     __shared__ int s_ok;
     __syncthreads();
     if (threadIdx.x == 0)
     {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(&st->bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t target = (gen + 1) * gridDim.x;
-        for (uint32_t spin = 0;; ++spin)
-        {
-            if (__hip_atomic_load(&st->bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target)
-                break;
-            if (spin > (1u << 24))
-            {
-                __hip_atomic_fetch_or(&st->timeout, 1u, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
+        __hip_atomic_fetch_add(&st->bar0, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = ftd_tail_wait(st, &st->bar0, gridDim.x);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        s_ok = __hip_atomic_load(&st->timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
-        if (blockIdx.x == 0 && gen < (uint32_t)FTT_STAMPS)
-            st->stamp[gen] = __builtin_amdgcn_s_memrealtime();
+        s_ok = s_ok && __hip_atomic_load(&st->timeout, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) == 0;
+        if (blockIdx.x == 0)
+            st->stamp[0] = __builtin_amdgcn_s_memrealtime();
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+// `x` / `gx`: this block's XCD and its number of blocks; `nx`: XCDs with
+// blocks.  Barrier `gen` stamps stamp[gen + 1] (block 0).
+__device__ inline bool ftd_tail_sync(FtdTailState* st, uint32_t gen, uint32_t x, uint32_t gx,
+                                     uint32_t nx)
+{
+    __shared__ int s_ok;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are in the L2
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        const uint32_t prev =
+            __hip_atomic_fetch_add(&st->xarr[x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev + 1 == (gen + 1) * gx)
+        {
+            // the XCD's last block: every block of it has arrived
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(&st->bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_ok = ftd_tail_wait(st, &st->bar, (gen + 1) * nx);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ok = s_ok && __hip_atomic_load(&st->timeout, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) == 0;
+        if (blockIdx.x == 0 && gen + 1 < (uint32_t)FTT_STAMPS)
+            st->stamp[gen + 1] = __builtin_amdgcn_s_memrealtime();
     }
     __syncthreads();
     return s_ok != 0;
@@ -442,6 +494,12 @@ __global__ __launch_bounds__(FTT_NT) void k_ftd_tail(FtdTailArgs a, const CT* __
     __shared__ uint8_t  s_list[256];
     __shared__ uint32_t s_k;
     FtdTailState*       st = a.st;
+    __shared__ uint32_t s_x, s_gx, s_nx;
+    if (threadIdx.x == 0)
+    {
+        s_x = ftd_xcc_id();
+        __hip_atomic_fetch_add(&st->xn[s_x], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     // first round: every slot's cell, its score region, its begin
     for (uint32_t s = blockIdx.x; s < a.n; s += gridDim.x)
     {
@@ -464,9 +522,19 @@ __global__ __launch_bounds__(FTT_NT) void k_ftd_tail(FtdTailArgs a, const CT* __
         ftd_tail_begin(a, s, s_list, s_k, 0, false, tsm);
         __syncthreads();
     }
-    uint32_t gen = 0;
-    if (!ftd_tail_sync(st, gen++))
+    if (!ftd_tail_sync_first(st))
         return;
+    if (threadIdx.x == 0)
+    {
+        uint32_t nx = 0;
+        for (int i = 0; i < 16; ++i)
+            nx += st->xn[i] ? 1u : 0u;
+        s_gx = st->xn[s_x];
+        s_nx = nx;
+    }
+    __syncthreads();
+    const uint32_t x = s_x, gx = s_gx, nx = s_nx;
+    uint32_t       gen = 0;
     // (the first round was counted by the part A that handed it over: the
     // totals cover the rounds after it)
     for (int p = 0, round = 0;; p ^= 1, ++round)
@@ -480,7 +548,7 @@ __global__ __launch_bounds__(FTT_NT) void k_ftd_tail(FtdTailArgs a, const CT* __
             st->tot_active += nact;
         }
         ftd_tail_rows<CT>(a, ref_codes, p, (int*)tsm);
-        if (!ftd_tail_sync(st, gen++))
+        if (!ftd_tail_sync(st, gen++, x, gx, nx))
             return;
         if (blockIdx.x == 0 && threadIdx.x == 0)
         {
@@ -513,7 +581,7 @@ __global__ __launch_bounds__(FTT_NT) void k_ftd_tail(FtdTailArgs a, const CT* __
             else if (threadIdx.x == 0)
                 a.slots[s].gs = 0;
         }
-        if (!ftd_tail_sync(st, gen++))
+        if (!ftd_tail_sync(st, gen++, x, gx, nx))
             return;
     }
 }
