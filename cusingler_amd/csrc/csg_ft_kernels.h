@@ -775,18 +775,22 @@ __device__ inline void ft_mfma_tile_t(const FtCorrTile& tl, const FtGroup* __res
             acc_x[i][j]  = v4i_ft{ 0, 0, 0, 0 };
             acc_ll[i][j] = v4i_ft{ 0, 0, 0, 0 };
         }
+    // X = 256 h + l, l the signed low byte and h = (X + 128) >> 8 (|X| <=
+    // 1919 here): two int16 per packed add and arithmetic shift, the bytes
+    // gathered by permutes (12 instructions for 8 values)
     auto split8 = [](int4 v, int2& hi, int2& lo) {
-        const int16_t* x = (const int16_t*)&v;
-        int8_t         h[8], l[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-        {
-            const int lv = (int)(int8_t)(x[u] & 0xff);
-            l[u]         = (int8_t)lv;
-            h[u]         = (int8_t)((x[u] - lv) >> 8);
-        }
-        hi = *(const int2*)h;
-        lo = *(const int2*)l;
+        typedef short v2s __attribute__((ext_vector_type(2)));
+        auto h2 = [](int d) {
+            v2s t = __builtin_bit_cast(v2s, d);
+            t     = (t + (v2s){ 128, 128 }) >> (v2s){ 8, 8 };
+            return __builtin_bit_cast(uint32_t, t);
+        };
+        // bytes 0 and 2 of each of two dwords (low dword first)
+        constexpr uint32_t even = 0x06040200u;
+        lo = int2{ (int)__builtin_amdgcn_perm((uint32_t)v.y, (uint32_t)v.x, even),
+                   (int)__builtin_amdgcn_perm((uint32_t)v.w, (uint32_t)v.z, even) };
+        hi = int2{ (int)__builtin_amdgcn_perm(h2(v.y), h2(v.x), even),
+                   (int)__builtin_amdgcn_perm(h2(v.w), h2(v.z), even) };
     };
     // the K loop keeps the next step's rank chunks in registers while the
     // current step's MFMAs run (loads of step k0 + 64 overlap them)
