@@ -1077,7 +1077,7 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
     HIPCHK(hipMemsetAsync(eng->d_exact, 0, (size_t)nq * L, eng->stream));
     {
         PhaseScope ps(eng, 3, 1);
-        k_label_certify<<<(nq + 255) / 256, 256, 0, eng->stream>>>(
+        k_label_certify<<<(nq + 3) / 4, 256, 0, eng->stream>>>(
             P, nq, L, wp, thre, F, M, eng->d_cstatus, eng->d_items, eng->d_nitems);
         LAUNCHCHK();
     }

@@ -44,40 +44,63 @@ __device__ inline float next_down(float x) { return -next_up(-x); }
 // (half-width wp), or list the labels that need the reference's FP32 value.
 // status[q]: 0 decided, 1 partial recheck, 2 full recheck (a NaN quantile:
 // the libstdc++ minmax semantics then need every exact value).
-__global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32_t L, double wp,
-                                float thre, int32_t* __restrict__ first, float* __restrict__ mask,
-                                uint8_t* __restrict__ status, RecheckItem* __restrict__ items,
-                                unsigned int* __restrict__ n_items)
+// One wave per cell, the lanes over its labels (four per lane: L <= 256),
+// so that a row's quantiles are read and its mask written coalesced.  The
+// bounds of the reference's maximum, max_j (v_j -+ wp) in double, are
+// (max_j v_j) -+ wp rounded once (rounding is monotone).
+__global__ __launch_bounds__(256) void k_label_certify(
+    const float* __restrict__ P, uint32_t nq, uint32_t L, double wp, float thre,
+    int32_t* __restrict__ first, float* __restrict__ mask, uint8_t* __restrict__ status,
+    RecheckItem* __restrict__ items, unsigned int* __restrict__ n_items)
 {
     // n_items[0]: (cell, label) pairs listed; n_items[1]: cells left uncertified
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t q    = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int      lane = threadIdx.x & 63;
     if (q >= nq)
-        return;
+        return;  // (the whole wave)
     const float* s  = P + (size_t)q * L;
     float*       mk = mask + (size_t)q * L;
+    float        v[4];
     bool         nan = false;
-    double       mlo = -1e300, mhi = -1e300;
-    // (no early exit at a NaN: the row's loads stay independent, so they
-    // are in flight together; a NaN row discards mlo / mhi)
-#pragma unroll 8
-    for (uint32_t j = 0; j < L; ++j)
+    float        vmax = -__builtin_inff();
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
     {
-        const float v = s[j];
-        nan |= __builtin_isnan(v);
-        mlo = fmax(mlo, (double)v - wp);
-        mhi = fmax(mhi, (double)v + wp);
-    }
-    if (nan)
-    {
-        status[q] = 2;
-        atomicAdd(n_items + 1, 1u);
-        for (uint32_t j = 0; j < L; ++j)
+        const uint32_t j = lane + 64 * u;
+        v[u]             = j < L ? s[j] : 0.f;
+        if (j < L)
         {
-            const unsigned int t = atomicAdd(n_items, 1u);
-            items[t]             = RecheckItem{ q, j };
+            nan |= __builtin_isnan(v[u]);
+            vmax = fmaxf(vmax, v[u]);
+        }
+    }
+    // lane 0 reserves `count` list slots; every lane gets the base
+    auto reserve = [&](unsigned int count) {
+        unsigned int base = 0;
+        if (lane == 0)
+            base = atomicAdd(n_items, count);
+        return (unsigned int)__builtin_amdgcn_readfirstlane((int)base);
+    };
+    if (__ballot(nan))
+    {
+        if (lane == 0)
+        {
+            status[q] = 2;
+            atomicAdd(n_items + 1, 1u);
+        }
+        const unsigned int base = reserve(L);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+            const uint32_t j = lane + 64 * u;
+            if (j < L)
+                items[base + j] = RecheckItem{ q, j };
         }
         return;
     }
+    vmax              = wave_max(vmax);
+    const double mlo  = (double)vmax - wp;
+    const double mhi  = (double)vmax + wp;
     // float bounds of the reference's maximum and threshold
     float flo = (float)mlo;
     if ((double)flo > mlo)
@@ -87,54 +110,61 @@ __global__ void k_label_certify(const float* __restrict__ P, uint32_t nq, uint32
         fhi = next_up(fhi);
     const double tlo = (double)(flo - thre);
     const double thi = (double)(fhi - thre);
-    int          ncand = 0, cand = -1;
-    bool         unsure = false;
-    // eight scores loaded together, then decided (the mask stores would
-    // otherwise wait behind each load)
-    for (uint32_t j0 = 0; j0 < L; j0 += 8)
+    uint32_t ncand = 0;
+    int      cand  = -1;
+    bool     unsure = false;
+    bool     list[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
     {
-        float sv[8];
-#pragma unroll
-        for (uint32_t u = 0; u < 8; ++u)
-            sv[u] = s[min(j0 + u, L - 1)];  // (past the row: its last score, not used)
-#pragma unroll
-        for (uint32_t u = 0; u < 8; ++u)
-        {
-            const uint32_t j = j0 + u;
-            if (j >= L)
-                break;
-            const double v = (double)sv[u];
-            if (v + wp >= mlo)
-            {
-                ++ncand;
-                cand = (int)j;
-            }
-            if (v - wp >= thi)
-                mk[j] = 1.f;
-            else if (v + wp < tlo)
-                mk[j] = 0.f;
-            else
-                unsure = true;
-        }
+        const uint32_t j   = lane + 64 * u;
+        const double   dv  = (double)v[u];
+        const bool     ok  = j < L;
+        const bool     inC = ok && dv + wp >= mlo;
+        const bool     hi1 = dv - wp >= thi, lo0 = dv + wp < tlo;
+        if (ok && hi1)
+            mk[j] = 1.f;
+        else if (ok && lo0)
+            mk[j] = 0.f;
+        const bool unk = ok && !hi1 && !lo0;
+        unsure |= unk;
+        list[u] = inC || unk;
+        const unsigned long long m = __ballot(inC);
+        ncand += (uint32_t)__builtin_popcountll(m);
+        if (m)
+            cand = 64 * u + 63 - __builtin_clzll(m);  // the last candidate label
     }
-    if (ncand == 1 && !unsure)
+    if (ncand == 1 && !__ballot(unsure))
     {
-        first[q]  = cand;
-        status[q] = 0;
+        if (lane == 0)
+        {
+            first[q]  = cand;
+            status[q] = 0;
+        }
         return;
     }
-    status[q] = 1;
-    atomicAdd(n_items + 1, 1u);
-    for (uint32_t j = 0; j < L; ++j)
+    if (lane == 0)
     {
-        const double v   = (double)s[j];
-        const bool   inC = v + wp >= mlo;
-        const bool   unk = !(v - wp >= thi) && !(v + wp < tlo);
-        if (inC || unk)
-        {
-            const unsigned int t = atomicAdd(n_items, 1u);
-            items[t]             = RecheckItem{ q, j };
-        }
+        status[q] = 1;
+        atomicAdd(n_items + 1, 1u);
+    }
+    unsigned long long m[4];
+    unsigned int       total = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+    {
+        m[u] = __ballot(list[u]);
+        total += (unsigned int)__builtin_popcountll(m[u]);
+    }
+    unsigned int at = reserve(total);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+    {
+        if (list[u])
+            items[at + __builtin_amdgcn_mbcnt_hi((uint32_t)(m[u] >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m[u], 0u))] =
+                RecheckItem{ q, lane + 64u * u };
+        at += (unsigned int)__builtin_popcountll(m[u]);
     }
 }
 

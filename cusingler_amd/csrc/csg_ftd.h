@@ -1739,7 +1739,7 @@ struct FtdListSegments
 // redo = 0: all segments (count n_segs, NaN list counted in n_nan);
 // redo = 1: the listed segments (n_redo_segs, n_nan2).
 template <class Src>
-__global__ __launch_bounds__(256) void k_ftd_select(FtdCounts* __restrict__ cnt, int redo, Src src,
+__global__ __launch_bounds__(256, 8) void k_ftd_select(FtdCounts* __restrict__ cnt, int redo, Src src,
                                                     const float* __restrict__ scores,
                                                     const PctParam* __restrict__ params, float p,
                                                     float* __restrict__ out,

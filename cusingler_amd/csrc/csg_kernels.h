@@ -1112,7 +1112,7 @@ __device__ inline void select_item(const Src& src, uint64_t item,
 }
 
 template <class Src>
-__global__ __launch_bounds__(256) void k_select(Src src, uint64_t n_items,
+__global__ __launch_bounds__(256, 8) void k_select(Src src, uint64_t n_items,
                                                 const float* __restrict__ scores,
                                                 const PctParam* __restrict__ params, float p,
                                                 float* __restrict__ out,
