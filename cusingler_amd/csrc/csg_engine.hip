@@ -82,6 +82,8 @@ struct csg_engine
     bool      score_loaded = false;
     uint32_t  Nr = 0, Nq = 0, G = 0, Kp = 0, L = 0;
     bool      w8_ok = false;  // the wide INT8 kernels' LDS could be granted
+    bool      kf_ok = false;  // ... and the three-product form's
+    int       limb_form = 0;  // 0: three-product form where it applies, 1: folded four products
     bool      x3_ok = false;  // the three-partial wide INT8 kernels' LDS could be granted
     bool      bw_ok = false;  // the wide BF16 network kernel's LDS could be granted
     int       nbins = 0;
@@ -148,8 +150,10 @@ struct csg_engine
     int*      d_status = nullptr;
     // exact / approximate MFMA paths
     int8_t *  d_rh = nullptr, *d_rl = nullptr, *d_qh = nullptr, *d_ql = nullptr;
+    double *  d_dr = nullptr, *d_dq = nullptr;  // rows' corrections, three-product form
     _Float16 *d_rf16 = nullptr, *d_qf16 = nullptr;
     int       ref_prepared_path = -1;
+    bool      ref_kf = false;  // the prepared limb planes hold the three-product form
     uint8_t*  d_cstatus = nullptr;
     uint8_t*  d_exact = nullptr;
     RecheckItem* d_items = nullptr;
@@ -675,12 +679,19 @@ int csg_init_gpu(int gpuid, csg_engine** out)
     }
     // the wide INT8 kernels stage three K steps of operands in dynamic LDS
     eng->w8_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)W8_LDS &&
-                 hipFuncSetAttribute((const void*)k_gemm_int8w,
+                 hipFuncSetAttribute((const void*)k_gemm_int8w<false>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      W8_LDS) == hipSuccess &&
-                 hipFuncSetAttribute((const void*)k_gemm_int8w_group,
+                 hipFuncSetAttribute((const void*)k_gemm_int8w_group<false>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      W8_LDS) == hipSuccess;
+    eng->kf_ok = eng->w8_ok && (size_t)eng->prop.sharedMemPerBlock >= (size_t)W8_LDS_KF &&
+                 hipFuncSetAttribute((const void*)k_gemm_int8w<true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     W8_LDS_KF) == hipSuccess &&
+                 hipFuncSetAttribute((const void*)k_gemm_int8w_group<true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     W8_LDS_KF) == hipSuccess;
     eng->x3_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)X3_LDS &&
                  hipFuncSetAttribute((const void*)k_gemm_int8x,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -740,6 +751,8 @@ int csg_destroy_score(csg_engine* eng)
     dfree(eng->d_rl);
     dfree(eng->d_qh);
     dfree(eng->d_ql);
+    dfree(eng->d_dr);
+    dfree(eng->d_dq);
     dfree(eng->d_rf16);
     dfree(eng->d_qf16);
     dfree(eng->d_cstatus);
@@ -991,13 +1004,19 @@ int score_chunk(csg_engine* eng, int path, uint32_t nq)
                                                     eng->d_sqq_f, eng->d_sqr_f, (int)nq,
                                                     (int)eng->Nr, eng->d_S, (size_t)eng->Nr);
     }
+    else if (path == CSG_PATH_EXACT_INT8 && eng->ref_kf)
+    {
+        dim3 grid((eng->Nr + W8_BN - 1) / W8_BN, (nq + W8_BM - 1) / W8_BM);
+        k_gemm_int8w<true><<<grid, 512, W8_LDS_KF, eng->stream>>>(
+            eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl, (int)eng->Kp, eng->d_irq, eng->d_irr,
+            (int)nq, (int)eng->Nr, eng->d_S, (size_t)eng->Nr, eng->d_dq, eng->d_dr);
+    }
     else if (path == CSG_PATH_EXACT_INT8 && eng->G <= 1920 && eng->w8_ok)
     {
         dim3 grid((eng->Nr + W8_BN - 1) / W8_BN, (nq + W8_BM - 1) / W8_BM);
-        k_gemm_int8w<<<grid, 512, W8_LDS, eng->stream>>>(eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                                                    (int)eng->Kp, eng->d_irq, eng->d_irr,
-                                                    (int)nq, (int)eng->Nr, eng->d_S,
-                                                    (size_t)eng->Nr);
+        k_gemm_int8w<false><<<grid, 512, W8_LDS, eng->stream>>>(
+            eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl, (int)eng->Kp, eng->d_irq, eng->d_irr,
+            (int)nq, (int)eng->Nr, eng->d_S, (size_t)eng->Nr, nullptr, nullptr);
     }
     else if (path == CSG_PATH_EXACT_INT8 && eng->x3_ok)
     {
@@ -1034,13 +1053,24 @@ int score_chunk(csg_engine* eng, int path, uint32_t nq)
     return CSG_OK;
 }
 
-// Operand planes of `rows` rows of X for the MFMA paths.
+// The exact path's limb form for the loaded gene set: the three-product form
+// (k_split_kf) needs |X| <= 1007, i.e. G <= 1008, and the wide kernels.
+bool use_kf(const csg_engine* eng)
+{
+    return eng->limb_form == 0 && eng->kf_ok && eng->G <= 1008;
+}
+
+// Operand planes of `rows` rows of X for the MFMA paths; the exact path's in
+// the three-product form (with the rows' corrections in d) when kf.
 int prepare_operands(csg_engine* eng, int path, const int16_t* X, size_t rows, int8_t* h,
-                     int8_t* l, _Float16* f)
+                     int8_t* l, _Float16* f, bool kf = false, double* d = nullptr)
 {
     const size_t n = rows * eng->Kp;
     const unsigned blocks = (unsigned)((n / 8 + 255) / 256);
-    if (path == CSG_PATH_EXACT_INT8)
+    if (path == CSG_PATH_EXACT_INT8 && kf)
+        k_split_kf<<<(unsigned)((rows + 3) / 4), 256, 0, eng->stream>>>(X, rows, (int)eng->Kp, h,
+                                                                       l, d);
+    else if (path == CSG_PATH_EXACT_INT8)
         k_split_limbs<<<blocks, 256, 0, eng->stream>>>(X, n, h, l);
     else if (path == CSG_PATH_FP16_MFMA)
         k_to_half<<<blocks, 256, 0, eng->stream>>>(X, n, f);
@@ -1457,11 +1487,16 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
     {
         PhaseScope ps(eng, 1, 1);
         const uint32_t b_max = (uint32_t)round_up(eng->Nr, 256) - 1;
-        if (wide)
-            k_gemm_int8w_group<<<(unsigned)tiles.size(), 512, W8_LDS, eng->stream>>>(
+        if (eng->ref_kf)
+            k_gemm_int8w_group<true><<<(unsigned)tiles.size(), 512, W8_LDS_KF, eng->stream>>>(
                 eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
                 eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S);
+                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, eng->d_dq, eng->d_dr);
+        else if (wide)
+            k_gemm_int8w_group<false><<<(unsigned)tiles.size(), 512, W8_LDS, eng->stream>>>(
+                eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
+                eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, nullptr, nullptr);
         else if (x3)
             k_gemm_int8x_group<<<(unsigned)tiles.size(), 512, X3_LDS, eng->stream>>>(
                 eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
@@ -1668,6 +1703,7 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
             (rc = dalloc(eng, &eng->d_rl, nr_pad * eng->Kp)) ||
             (rc = dalloc(eng, &eng->d_qh, qc * eng->Kp)) ||
             (rc = dalloc(eng, &eng->d_ql, qc * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_dr, nr_pad)) || (rc = dalloc(eng, &eng->d_dq, qc)) ||
             (rc = dalloc(eng, &eng->d_cstatus, qc)) ||
             (rc = dalloc(eng, &eng->d_exact, qc * eng->L)) ||
             (rc = dalloc(eng, &eng->d_items, qc * eng->L)) ||
@@ -1689,12 +1725,17 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
         eng->ref_ranked        = true;
         eng->ref_prepared_path = -1;
     }
-    if ((path == CSG_PATH_EXACT_INT8 || path == CSG_PATH_FP16_MFMA) && eng->ref_prepared_path != path)
+    if ((path == CSG_PATH_EXACT_INT8 || path == CSG_PATH_FP16_MFMA) &&
+        (eng->ref_prepared_path != path ||
+         (path == CSG_PATH_EXACT_INT8 && eng->ref_kf != use_kf(eng))))
     {
         PhaseScope ps(eng, 0, 1);
-        if ((rc = prepare_operands(eng, path, eng->d_Xr, nr_pad, eng->d_rh, eng->d_rl, eng->d_rf16)))
+        const bool kf = path == CSG_PATH_EXACT_INT8 && use_kf(eng);
+        if ((rc = prepare_operands(eng, path, eng->d_Xr, nr_pad, eng->d_rh, eng->d_rl, eng->d_rf16,
+                                   kf, eng->d_dr)))
             return rc;
         eng->ref_prepared_path = path;
+        eng->ref_kf            = kf;
     }
     for (uint32_t q0 = 0; q0 < eng->Nq; q0 += eng->Qc)
     {
@@ -1710,7 +1751,8 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
                                 prop ? eng->d_mx : nullptr)))
                 return rc;
             if ((rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh,
-                                       eng->d_ql, eng->d_qf16)))
+                                       eng->d_ql, eng->d_qf16,
+                                       path == CSG_PATH_EXACT_INT8 && eng->ref_kf, eng->d_dq)))
                 return rc;
         }
         if (prop)
@@ -3912,6 +3954,7 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
             (rc = dalloc(eng, &eng->d_rl, nr_pad * eng->Kp)) ||
             (rc = dalloc(eng, &eng->d_qh, qc * eng->Kp)) ||
             (rc = dalloc(eng, &eng->d_ql, qc * eng->Kp)) ||
+            (rc = dalloc(eng, &eng->d_dr, nr_pad)) || (rc = dalloc(eng, &eng->d_dq, qc)) ||
             (rc = dalloc(eng, &eng->d_cstatus, qc)) ||
             (rc = dalloc(eng, &eng->d_exact, qc * eng->L)) ||
             (rc = dalloc(eng, &eng->d_items, qc * eng->L)) ||
@@ -3932,16 +3975,22 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
         eng->ref_ranked        = true;
         eng->ref_prepared_path = -1;
     }
-    if ((path == CSG_PATH_EXACT_INT8 || path == CSG_PATH_FP16_MFMA) && eng->ref_prepared_path != path)
+    if ((path == CSG_PATH_EXACT_INT8 || path == CSG_PATH_FP16_MFMA) &&
+        (eng->ref_prepared_path != path ||
+         (path == CSG_PATH_EXACT_INT8 && eng->ref_kf != use_kf(eng))))
     {
-        if ((rc = prepare_operands(eng, path, eng->d_Xr, nr_pad, eng->d_rh, eng->d_rl, eng->d_rf16)))
+        const bool kf = path == CSG_PATH_EXACT_INT8 && use_kf(eng);
+        if ((rc = prepare_operands(eng, path, eng->d_Xr, nr_pad, eng->d_rh, eng->d_rl, eng->d_rf16,
+                                   kf, eng->d_dr)))
             return rc;
         eng->ref_prepared_path = path;
+        eng->ref_kf            = kf;
     }
     if ((rc = rank_rows(eng, eng->d_qdata, eng->d_qind, eng->d_qptr, q0, nq, eng->d_Xq,
                         path == CSG_PATH_EXACT_INT8 ? nullptr : eng->d_sqq_f, eng->d_irq)) ||
         (rc = prepare_operands(eng, path, eng->d_Xq, round_up(nq, 128), eng->d_qh, eng->d_ql,
-                               eng->d_qf16)) ||
+                               eng->d_qf16, path == CSG_PATH_EXACT_INT8 && eng->ref_kf,
+                               eng->d_dq)) ||
         (rc = score_chunk(eng, path, nq)))
         return rc;
     HIPCHK(hipMemcpyAsync(out, eng->d_S, (size_t)nq * eng->Nr * sizeof(float),
@@ -4731,6 +4780,18 @@ int csg_set_finetune_tail(csg_engine* eng, uint32_t max_cells, int blocks_per_cu
     return CSG_OK;
 }
 
+int csg_set_limb_form(csg_engine* eng, int form, int* prev)
+{
+    if (!eng)
+        return CSG_ERR_ARGUMENT;
+    if (form < 0 || form > 1)
+        return eng->fail(CSG_ERR_ARGUMENT, "limb form 0 (automatic) or 1 (folded)");
+    if (prev)
+        *prev = eng->limb_form;
+    eng->limb_form = form;
+    return CSG_OK;
+}
+
 int csg_set_finetune_chain(csg_engine* eng, int max_rounds, uint64_t max_scores)
 {
     if (!eng)
@@ -4813,15 +4874,19 @@ int csg_counters(csg_engine* eng, int64_t* out)
 
 int csg_selftest(csg_engine* eng, int which, double* max_err)
 {
-    if (!eng || !max_err || which < 0 || which > 5)
+    if (!eng || !max_err || which < 0 || which > 6)
         return CSG_ERR_ARGUMENT;
     if (which == 4 && !eng->w8_ok)
         return eng->fail(CSG_ERR_DEVICE, "the device does not grant the wide kernel's LDS");
+    if (which == 6 && !eng->kf_ok)
+        return eng->fail(CSG_ERR_DEVICE,
+                         "the device does not grant the three-product wide kernel's LDS");
     if (which == 5 && !eng->x3_ok)
         return eng->fail(CSG_ERR_DEVICE, "the device does not grant the three-partial kernel's LDS");
     HIPCHK(hipSetDevice(eng->device));
     // the three-partial kernel over the whole int8 range of the high limb
-    const int nq = 256, nr = 256, Kp = 192, G = which == 5 ? 32000 : 1500;
+    // the three-product form over its whole range, |X| <= 1007
+    const int nq = 256, nr = 256, Kp = 192, G = which == 5 ? 32000 : which == 6 ? 1007 : 1500;
     std::vector<int16_t> xq((size_t)nq * Kp), xr((size_t)nr * Kp);
     uint64_t st = 0x9E3779B97F4A7C15ull;
     auto rnd = [&]() {
@@ -4840,18 +4905,19 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
     int8_t *qh = nullptr, *ql = nullptr, *rh = nullptr, *rl = nullptr;
     _Float16 *qf = nullptr, *rf = nullptr;
     float *d1 = nullptr, *dS = nullptr;
-    double* d1i = nullptr;
+    double *d1i = nullptr, *cq = nullptr, *cr = nullptr;
     int    rc = CSG_OK;
     auto   cleanup = [&]() {
         dfree(dq); dfree(dr); dfree(qh); dfree(ql); dfree(rh); dfree(rl); dfree(qf); dfree(rf);
-        dfree(d1); dfree(dS); dfree(d1i);
+        dfree(d1); dfree(dS); dfree(d1i); dfree(cq); dfree(cr);
     };
     if ((rc = upload(eng, &dq, xq.data(), xq.size())) || (rc = upload(eng, &dr, xr.data(), xr.size())) ||
         (rc = upload(eng, &d1, ones.data(), ones.size())) ||
         (rc = upload(eng, &d1i, ones64.data(), ones64.size())) || (rc = dalloc(eng, &dS, (size_t)nq * nr)) ||
         (rc = dalloc(eng, &qh, xq.size())) || (rc = dalloc(eng, &ql, xq.size())) ||
         (rc = dalloc(eng, &rh, xr.size())) || (rc = dalloc(eng, &rl, xr.size())) ||
-        (rc = dalloc(eng, &qf, xq.size())) || (rc = dalloc(eng, &rf, xr.size())))
+        (rc = dalloc(eng, &qf, xq.size())) || (rc = dalloc(eng, &rf, xr.size())) ||
+        (rc = dalloc(eng, &cq, (size_t)nq)) || (rc = dalloc(eng, &cr, (size_t)nr)))
     {
         cleanup();
         return rc;
@@ -4875,8 +4941,15 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
     {
         k_split_limbs<<<bq, 256, 0, eng->stream>>>(dq, xq.size(), qh, ql);
         k_split_limbs<<<br, 256, 0, eng->stream>>>(dr, xr.size(), rh, rl);
-        k_gemm_int8w<<<dim3(nr / W8_BN, nq / W8_BM), 512, W8_LDS, eng->stream>>>(qh, ql, rh, rl, Kp, d1i,
-                                                                            d1i, nq, nr, dS, nr);
+        k_gemm_int8w<false><<<dim3(nr / W8_BN, nq / W8_BM), 512, W8_LDS, eng->stream>>>(
+            qh, ql, rh, rl, Kp, d1i, d1i, nq, nr, dS, nr, nullptr, nullptr);
+    }
+    else if (which == 6)
+    {
+        k_split_kf<<<(unsigned)(nq / 4), 256, 0, eng->stream>>>(dq, (size_t)nq, Kp, qh, ql, cq);
+        k_split_kf<<<(unsigned)(nr / 4), 256, 0, eng->stream>>>(dr, (size_t)nr, Kp, rh, rl, cr);
+        k_gemm_int8w<true><<<dim3(nr / W8_BN, nq / W8_BM), 512, W8_LDS_KF, eng->stream>>>(
+            qh, ql, rh, rl, Kp, d1i, d1i, nq, nr, dS, nr, cq, cr);
     }
     else if (which == 5)
     {
@@ -4914,7 +4987,7 @@ int csg_selftest(csg_engine* eng, int which, double* max_err)
             }
             const float  dev = out[(size_t)m * nr + n];
             double       err;
-            if (which == 0 || which == 4 || which == 5)
+            if (which == 0 || which == 4 || which == 5 || which == 6)
                 err = dev == (float)(double)s4 ? 0.0 : std::fabs((double)dev - (double)s4) + 1.0;
             else if (which == 3)
                 err = dev == seq ? 0.0 : std::fabs((double)dev - (double)seq) + 1.0;

@@ -51,6 +51,59 @@ __global__ void k_split_limbs(const int16_t* __restrict__ X, size_t n, int8_t* _
     *(int2*)(lo + i) = *(const int2*)ll;
 }
 
+// Three-product limb form, for |X| <= 1007 (gene sets of up to 1008 genes):
+// X = 65 h + l with h = floor((X + 32) / 65) in [-15, 15] and l in [-32, 32].
+// Then X Y = 65 [(h + l)(h' + l') + (8h)(8h')] - 64 l l', since
+// 65 (h l' + l h') + 65 l l' + (65 + 65 * 64) h h' - 64 l l' expands to
+// 65^2 h h' + 65 (h l' + l h') + l l'.  The planes hold u = 8 (h + 16) in
+// [8, 248] (bytes whose low three bits are zero) and v = l + 32 in [0, 64];
+// a 32-bit word of four such bytes gives the three MFMA operands with no
+// carry between bytes:
+//     (u >> 3) + v     = (h + l) + 48     (the sum of two bytes below 96)
+//     u ^ 0x80808080   = 8h                (a signed byte)
+//     v                = l + 32
+// and the offsets are taken out per row: with s_p = sum_k ((h + l) + 48) and
+// s_v = sum_k (l + 32) over the row's Kp columns,
+//     X.Y = 65 A1 - 64 A2 - d(X) - d(Y),   d = 3120 s_p - 2048 s_v - 42112 Kp
+//         = 3120 sum h + 1072 sum l + 42112 Kp,
+// where A1 = sum (u>>3 + v)(u'>>3 + v') + (8h)(8h') and A2 = sum v v' are
+// the two int32 partials.  One wave per row; `d` is exact in a double.
+__global__ __launch_bounds__(256) void k_split_kf(const int16_t* __restrict__ X, size_t rows,
+                                                  int Kp, int8_t* __restrict__ hi,
+                                                  int8_t* __restrict__ lo, double* __restrict__ d)
+{
+    const size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int    lane = threadIdx.x & 63;
+    if (row >= rows)
+        return;
+    const int16_t* x  = X + row * Kp;
+    int8_t*        uh = hi + row * Kp;
+    int8_t*        vl = lo + row * Kp;
+    int            sh = 0, sl = 0;
+    for (int k = lane * 8; k < Kp; k += 512)
+    {
+        const int4     v  = *(const int4*)(x + k);
+        const int16_t* xv = (const int16_t*)&v;
+        uint8_t        uu[8], ww[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            const int h = (xv[u] + 32 + 65 * 16) / 65 - 16;  // numerator > 0
+            const int l = xv[u] - 65 * h;
+            sh += h;
+            sl += l;
+            uu[u] = (uint8_t)(8 * (h + 16));
+            ww[u] = (uint8_t)(l + 32);
+        }
+        *(int2*)(uh + k) = *(const int2*)uu;
+        *(int2*)(vl + k) = *(const int2*)ww;
+    }
+    sh = wave_sum(sh);
+    sl = wave_sum(sl);
+    if (lane == 0)
+        d[row] = 3120.0 * sh + 1072.0 * sl + 42112.0 * Kp;
+}
+
 // FP16 copy of a [rows x Kp] int16 matrix (exact for |X| <= 2048).
 __global__ void k_to_half(const int16_t* __restrict__ X, size_t n, _Float16* __restrict__ out)
 {
@@ -288,6 +341,8 @@ constexpr int W8_STAGE = (2 * W8_BM + 2 * W8_BN) * W8_BK;  // 48 KB
 // the tile's row and column inverse roots (doubles) follow the stages
 constexpr int W8_IR_OFF = W8_NS * W8_STAGE;
 constexpr int W8_LDS    = W8_IR_OFF + (W8_BM + W8_BN) * 8;  // dynamic LDS per block
+// the three-product form also stages the rows' offset corrections d
+constexpr int W8_LDS_KF = W8_LDS + (W8_BM + W8_BN) * 8;
 
 __device__ inline v4i times16(v4i h)
 {
@@ -322,6 +377,8 @@ struct DenseW8Map
     const double* ir_r;
     float*        S;
     size_t        ldS;
+    const double* d_q = nullptr;  // three-product form: row corrections
+    const double* d_r = nullptr;
     __device__ size_t a_row(int r) const { return (size_t)(m0 + r); }
     __device__ size_t b_row(int r) const { return (size_t)(n0 + r); }
     __device__ bool   row_ok(int m) const { return m0 + m < nq; }
@@ -344,6 +401,8 @@ struct GroupW8Map
     const double*   ir_r;
     float*          out;
     size_t          ld;
+    const double*   d_q = nullptr;  // three-product form: row corrections
+    const double*   d_r = nullptr;
     __device__ size_t a_row(int r) const { return cells[r < mcount ? r : mcount - 1]; }
     __device__ size_t b_row(int r) const { return b0 + r < b_max ? b0 + r : b_max; }
     __device__ bool   row_ok(int m) const { return m < mcount; }
@@ -373,8 +432,10 @@ __device__ __forceinline__ void w8_issue(const int8_t* __restrict__ qh,
     }
 }
 
-// `sm` is the block's W8_LDS bytes of dynamic LDS; Kp is a multiple of 64.
-template <class Map>
+// `sm` is the block's W8_LDS (W8_LDS_KF) bytes of dynamic LDS; Kp is a
+// multiple of 64.  KF: the planes hold the three-product form (k_split_kf)
+// and map.d_q / map.d_r the rows' corrections.
+template <bool KF, class Map>
 __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
                                         const int8_t* __restrict__ ql,
                                         const int8_t* __restrict__ rh,
@@ -419,6 +480,13 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
     const double ir_a  = map.row_ir_at(arow);
     const double ir_b0 = map.col_ir_at(map.b_row(rq));
     const double ir_b1 = map.col_ir_at(map.b_row(rq + 128));
+    double       d_a = 0.0, d_b0 = 0.0, d_b1 = 0.0;
+    if constexpr (KF)
+    {
+        d_a  = map.d_q[arow];
+        d_b0 = map.d_r[map.b_row(rq)];
+        d_b1 = map.d_r[map.b_row(rq + 128)];
+    }
     asm volatile("" ::: "memory");
     if (T > 1)
         w8_issue(qh, ql, rh, rl, src, sm + W8_STAGE, wid, W8_BK);
@@ -456,6 +524,38 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
             bh[j]         = *(const v4i*)(s + 2 * W8_BM * W8_BK + o);
             bl[j]         = *(const v4i*)(s + (2 * W8_BM + W8_BN) * W8_BK + o);
         }
+        if constexpr (KF)
+        {
+            // ah / bh: u = 8 (h + 16), al / bl: v = l + 32 -> operands
+            // (u >> 3) + v and u ^ 0x80808080 (no carry between bytes)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                {
+                    ah16[i][c] = (int)((unsigned)ah[i][c] ^ 0x80808080u);
+                    ah[i][c]   = (int)(((unsigned)ah[i][c] >> 3) + (unsigned)al[i][c]);
+                    bh16[i][c] = (int)((unsigned)bh[i][c] ^ 0x80808080u);
+                    bh[i][c]   = (int)(((unsigned)bh[i][c] >> 3) + (unsigned)bl[i][c]);
+                }
+            // acc_x: A1 (two products), acc_ll: A2; an accumulator's
+            // successive MFMAs are at least 16 issues apart
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                    {
+                        if (p == 0)
+                            acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah[i], bh[j], acc_x[i][j], 0, 0, 0);
+                        else if (p == 1)
+                            acc_ll[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al[i], bl[j], acc_ll[i][j], 0, 0, 0);
+                        else
+                            acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah16[i], bh16[j], acc_x[i][j], 0, 0, 0);
+                    }
+            continue;
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
         {
@@ -483,22 +583,35 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
                 }
     }
     double* s_ir = (double*)(sm + W8_IR_OFF);
+    double* s_d  = s_ir + W8_BM + W8_BN;
     if ((lane & 3) == 0)
     {
         s_ir[rq]               = ir_a;
         s_ir[W8_BM + rq]       = ir_b0;
         s_ir[W8_BM + 128 + rq] = ir_b1;
+        if constexpr (KF)
+        {
+            s_d[rq]               = d_a;
+            s_d[W8_BM + rq]       = d_b0;
+            s_d[W8_BM + 128 + rq] = d_b1;
+        }
     }
     __syncthreads();
-    double rir[4][4], cir[4];
+    double rir[4][4], cir[4], rd[4][4], cd[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)
+    {
         cir[j] = s_ir[W8_BM + wn * 64 + j * 16 + r16];
+        cd[j]  = KF ? s_d[W8_BM + wn * 64 + j * 16 + r16] : 0.0;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
+        {
             rir[i][e] = s_ir[wm * 64 + i * 16 + (lane >> 4) * 4 + e];
+            rd[i][e]  = KF ? s_d[wm * 64 + i * 16 + (lane >> 4) * 4 + e] : 0.0;
+        }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -514,16 +627,22 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
                 const int n = wn * 64 + j * 16 + r16;
                 if (!map.col_ok(n))
                     continue;
-                const double s4 = (double)acc_x[i][j][e] * 256.0 + (double)acc_ll[i][j][e];
+                // integers below 2^53: exact in any order
+                const double s4 =
+                    KF ? ((double)acc_x[i][j][e] * 65.0 - (double)acc_ll[i][j][e] * 64.0) -
+                             (rd[i][e] + cd[j])
+                       : (double)acc_x[i][j][e] * 256.0 + (double)acc_ll[i][j][e];
                 __builtin_nontemporal_store(exact_corr(s4, rir[i][e], cir[j]), out + n);
             }
         }
 }
 
+template <bool KF>
 __global__ __launch_bounds__(512) void k_gemm_int8w(
     const int8_t* __restrict__ qh, const int8_t* __restrict__ ql, const int8_t* __restrict__ rh,
     const int8_t* __restrict__ rl, int Kp, const double* __restrict__ ir_q,
-    const double* __restrict__ ir_r, int nq, int nr, float* __restrict__ S, size_t ldS)
+    const double* __restrict__ ir_r, int nq, int nr, float* __restrict__ S, size_t ldS,
+    const double* __restrict__ d_q, const double* __restrict__ d_r)
 {
     extern __shared__ __attribute__((aligned(16))) int8_t w8_sm[];
     const uint32_t gx = gridDim.x;
@@ -533,8 +652,8 @@ __global__ __launch_bounds__(512) void k_gemm_int8w(
     const uint32_t gsz = 8 * gx, grp = t / gsz, in = t % gsz;
     const uint32_t rows = min(8u, gridDim.y - grp * 8);
     const DenseW8Map map{ (int)(grp * 8 + in % rows) * W8_BM, (int)(in / rows) * W8_BN, nq, nr,
-                          ir_q, ir_r, S, ldS };
-    gemm_int8w_block(qh, ql, rh, rl, Kp, map, w8_sm);
+                          ir_q, ir_r, S, ldS, d_q, d_r };
+    gemm_int8w_block<KF>(qh, ql, rh, rl, Kp, map, w8_sm);
 }
 
 // Grouped contraction: block b takes tile tiles[xcd_order(b)] = (label l, m0,
@@ -543,6 +662,7 @@ __global__ __launch_bounds__(512) void k_gemm_int8w(
 // against the label's rows are stored contiguously, label l's block starting
 // at S + c_off[l].  Tiles are listed label by label, so a label's tiles run
 // on one XCD.
+template <bool KF>
 __global__ __launch_bounds__(512) void k_gemm_int8w_group(
     const GroupTile* __restrict__ tiles, const uint32_t* __restrict__ cells,
     const uint32_t* __restrict__ a_base, const uint32_t* __restrict__ a_cnt,
@@ -550,7 +670,7 @@ __global__ __launch_bounds__(512) void k_gemm_int8w_group(
     const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
     const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
     int Kp, uint32_t b_max, const double* __restrict__ ir_q, const double* __restrict__ ir_r,
-    float* __restrict__ S)
+    float* __restrict__ S, const double* __restrict__ d_q, const double* __restrict__ d_r)
 {
     extern __shared__ __attribute__((aligned(16))) int8_t w8_sm[];
     const GroupTile t  = tiles[xcd_order(blockIdx.x, gridDim.x)];
@@ -563,8 +683,10 @@ __global__ __launch_bounds__(512) void k_gemm_int8w_group(
                           ir_q,
                           ir_r,
                           S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
-                          nl };
-    gemm_int8w_block(qh, ql, rh, rl, Kp, map, w8_sm);
+                          nl,
+                          d_q,
+                          d_r };
+    gemm_int8w_block<KF>(qh, ql, rh, rl, Kp, map, w8_sm);
 }
 
 // ------------------------------------------------ three-partial wide tile ---

@@ -81,6 +81,7 @@ def lib():
     L.csg_set_finetune_fuse.argtypes = [_V, C.c_int]
     L.csg_set_finetune_chain.argtypes = [_V, C.c_int, _U64]
     L.csg_set_finetune_tail.argtypes = [_V, _U32, C.c_int]
+    L.csg_set_limb_form.argtypes = [_V, C.c_int, _V]
     L.csg_finetune_tail_profile.argtypes = [_V, _V, C.c_int, _V]
     L.csg_set_query_chunk.argtypes = [_V, _U32, _V]
     L.csg_copyin_query_raw.argtypes = [_V, _U32, _U32, _V, _V, _V, _V, _V]
@@ -477,6 +478,14 @@ class Engine:
         one cooperative launch (0 = never)."""
         self._check(lib().csg_set_finetune_tail(self._h, int(max_cells), int(blocks_per_cu)),
                     "set_finetune_tail")
+
+    def set_limb_form(self, form: int) -> int:
+        """Exact INT8 contraction's limb form: 0 = the three-product form for
+        gene sets of up to 1008 genes (default), 1 = the folded four-product
+        form everywhere.  Returns the previous form."""
+        prev = C.c_int()
+        self._check(lib().csg_set_limb_form(self._h, int(form), C.byref(prev)), "set_limb_form")
+        return prev.value
 
     def finetune_tail_profile(self) -> np.ndarray:
         """Microseconds between the last tail kernel's grid barriers (the first

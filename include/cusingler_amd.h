@@ -447,12 +447,23 @@ int csg_set_proposal(csg_engine* eng, int k, float min_margin);
  * 1071-1090). */
 int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float* out);
 
+/* Limb form of the exact INT8 contraction.  form 0 (default): for gene sets
+ * of up to 1008 genes (|X| <= 1007) the three-product form, X = 65h + l with
+ * X Y = 65 [(h + l)(h' + l') + (8h)(8h')] - 64 l l' (three INT8 products per
+ * score, two int32 partials, per-row offset corrections); wider sets keep the
+ * folded four-product form.  form 1: the folded form for every gene set.
+ * Scores are exact in both, so results do not depend on this setting.  *prev
+ * (may be null) receives the previous form.  No counterpart in the
+ * reference. */
+int csg_set_limb_form(csg_engine* eng, int form, int* prev);
+
 /* Diagnostic of the contraction kernels on seeded random ranks (|X| <= 1500,
  * 256 x 256 rows, K = 192): which = 0 INT8 limbs, 1 FP16, 2 FP32 MFMA,
  * 3 plain FP32, 4 INT8 limbs wide-tile kernel (gene sets up to 1920), 5 INT8
- * limbs three-partial wide-tile kernel (|X| <= 32000: every high-limb value).
+ * limbs three-partial wide-tile kernel (|X| <= 32000: every high-limb value),
+ * 6 the wide-tile kernel's three-product form (|X| <= 1007).
  * *max_err receives 0 when the device results equal the host's
- * exact (0, 4, 5) / sequential-fma (3) values, else the largest difference (+1); for
+ * exact (0, 4, 5, 6) / sequential-fma (3) values, else the largest difference (+1); for
  * 1 and 2 the largest error of the dot product relative to sqrt(sxx * syy),
  * i.e. in correlation units.  No
  * counterpart in the reference: it checks the MFMA operand layouts. */

@@ -27,12 +27,13 @@ def eng():
     e.close()
 
 
-@pytest.mark.parametrize("which,name", [(0, "int8"), (4, "int8_wide"), (5, "int8_x3"), (3, "plain"),
+@pytest.mark.parametrize("which,name", [(0, "int8"), (4, "int8_wide"), (5, "int8_x3"),
+                                        (6, "int8_wide_three_product"), (3, "plain"),
                                         (1, "fp16"), (2, "fp32_mfma")])
 def test_contraction_selftest(eng, which, name):
     _say("selftest", name)
     err = eng.selftest(which)
-    if which in (0, 3, 4, 5):
+    if which in (0, 3, 4, 5, 6):
         assert err == 0.0, f"{name} contraction differs from the host ({err})"
     else:
         assert err < 1e-5, f"{name} relative error {err}"
@@ -224,6 +225,32 @@ def test_correlation_error_vs_fp64(eng, name):
     assert errs["plain"] <= 1e-5
     assert errs["fp32_mfma"] <= 1e-5
     assert errs["fp16"] <= 1e-4
+
+
+@pytest.mark.parametrize("name", ["tiny_nan", "pancreas_2k", "atlas_1k"])
+def test_exact_limb_forms_bit_equal(eng, name):
+    """The exact contraction's three-product form (gene sets of up to 1,008
+    genes) and the folded four-product form give the same score matrix bit for
+    bit, and the same labels, mask and quantiles."""
+    from cusingler_amd import engine
+    ds = _data(name)
+    c0 = engine.Prep(ds, 8).generate(0)
+    assert c0.width <= 1008
+    nq = min(512, ds.qry_height)
+    out = {}
+    for form in (1, 0):
+        _say("exact limb form", form, name, "G_s", c0.width)
+        eng.set_limb_form(form)
+        try:
+            eng.copyin_score(c0)
+            s = eng.score_matrix("exact_int8", 0, nq)
+            lab = eng.get_label(0.8, 0.05, "exact_int8")
+            eng.destroy_score()
+        finally:
+            eng.set_limb_form(0)
+        out[form] = (s, *lab)
+    for a, b, what in zip(out[0], out[1], ["scores", "first", "mask", "quantiles"]):
+        assert_bits_equal(np.asarray(a), np.asarray(b), f"{name} {what}")
 
 
 def test_device_finetune_batches_oversized_rounds(eng):

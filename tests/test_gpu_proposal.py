@@ -214,6 +214,26 @@ def test_proposal_scores_and_labels(eng, atlas_lite):
     assert agree_final >= 0.999 and agree_first >= 0.999
 
 
+def test_proposal_limb_forms_bit_equal(eng, atlas_lite):
+    """The grouped contraction in the three-product form and in the folded
+    four-product form: identical labels, mask and quantiles."""
+    eng.set_mlp_params(atlas_lite["params"])
+    eng.copyin_score(atlas_lite["c0"])
+    assert atlas_lite["c0"].width <= 1008
+    out = {}
+    for form in (1, 0):
+        _say("proposal, limb form", form)
+        eng.set_limb_form(form)
+        try:
+            eng.set_proposal(4, 0.2)
+            out[form] = eng.get_label(0.8, 0.05, "exact_int8")
+        finally:
+            eng.set_proposal(0)
+            eng.set_limb_form(0)
+    for a, b in zip(out[0], out[1]):
+        assert np.asarray(a).tobytes() == np.asarray(b).tobytes()
+
+
 def test_trained_network_gains_over_untrained(eng, atlas_lite):
     """What the network buys: the trained proposal scores a fraction of the
     (cell, label) pairs at >= 99.9% agreement; an untrained network of the
