@@ -700,7 +700,7 @@ int csg_init_gpu(int gpuid, csg_engine** out)
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      X3_LDS) == hipSuccess;
     eng->bw_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)BW_LDS &&
-                 hipFuncSetAttribute((const void*)k_bgemm_nt_w<1>,
+                 hipFuncSetAttribute((const void*)k_bgemm_nt_w,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      BW_LDS) == hipSuccess;
     (void)hipGetLastError();
@@ -1248,18 +1248,18 @@ int mlp_forward(csg_engine* eng, int rows, bool keep)
     const bool     wide = eng->bw_ok && H % BW_BN == 0 && Kp % BW_BK == 0 && H % BW_BK == 0;
     const unsigned gw   = (unsigned)(((rows + BW_BM - 1) / BW_BM) * (H / BW_BN));
     if (wide)
-        k_bgemm_nt_w<1><<<gw, 512, BW_LDS, eng->stream>>>(
-            eng->d_mx, Kp, wt + eng->m_off[0], Kp, rows, H, Kp, H, nullptr, w + eng->m_off[1],
-            keep ? eng->d_mz1 : nullptr, eng->d_mh1);
+        k_bgemm_nt_w<<<gw, 512, BW_LDS, eng->stream>>>(eng->d_mx, Kp, wt + eng->m_off[0], Kp, rows,
+                                                       H, Kp, H, w + eng->m_off[1],
+                                                       keep ? eng->d_mz1 : nullptr, eng->d_mh1);
     else
         k_bgemm_nt<1><<<gh, 256, 0, eng->stream>>>(eng->d_mx, Kp, wt + eng->m_off[0], Kp, rows, H,
                                                     Kp, H, nullptr, w + eng->m_off[1],
                                                     keep ? eng->d_mz1 : nullptr, eng->d_mh1);
     LAUNCHCHK();
     if (wide)
-        k_bgemm_nt_w<1><<<gw, 512, BW_LDS, eng->stream>>>(
-            eng->d_mh1, H, wt + eng->m_off[2], H, rows, H, H, H, nullptr, w + eng->m_off[3],
-            keep ? eng->d_mz2 : nullptr, eng->d_mh2);
+        k_bgemm_nt_w<<<gw, 512, BW_LDS, eng->stream>>>(eng->d_mh1, H, wt + eng->m_off[2], H, rows,
+                                                       H, H, H, w + eng->m_off[3],
+                                                       keep ? eng->d_mz2 : nullptr, eng->d_mh2);
     else
         k_bgemm_nt<1><<<gh, 256, 0, eng->stream>>>(eng->d_mh1, H, wt + eng->m_off[2], H, rows, H,
                                                     H, H, nullptr, w + eng->m_off[3],

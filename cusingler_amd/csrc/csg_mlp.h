@@ -16,16 +16,33 @@ namespace csg {
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float  v4f_m __attribute__((ext_vector_type(4)));
 
-__device__ inline float gelu_f(float z)
+// erfc(a) for a >= 0 by Abramowitz & Stegun 7.1.26 (absolute error below
+// 1.5e-7): t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-a^2), t = 1 / (1 + p a),
+// on the hardware reciprocal and base-2 exponential.
+__device__ inline float erfc_as(float a)
 {
-    return 0.5f * z * (1.f + erff(z * 0.70710678118654752f));
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, a, 1.f));
+    const float p =
+        t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f),
+                         -0.284496736f),
+                 0.254829592f);
+    return p * __builtin_amdgcn_exp2f(-(a * a) * 1.4426950408889634f);
 }
+
+// Phi(z) = 0.5 erfc(-z / sqrt 2): from erfc of |z| / sqrt 2 on either side, so
+// the lower tail keeps its small values (absolute error below 1e-6).
+__device__ inline float gelu_cdf(float z)
+{
+    const float ec = erfc_as(fabsf(z) * 0.70710678118654752f);
+    return z >= 0.f ? 1.f - 0.5f * ec : 0.5f * ec;
+}
+
+__device__ inline float gelu_f(float z) { return z * gelu_cdf(z); }
 
 __device__ inline float gelu_grad(float z)
 {
-    const float cdf = 0.5f * (1.f + erff(z * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
-    return cdf + z * pdf;
+    const float pdf = 0.39894228040143268f * __builtin_amdgcn_exp2f(-0.5f * z * z * 1.4426950408889634f);
+    return gelu_cdf(z) + z * pdf;
 }
 
 // C[M x N] (FP32, row-major, ldc) = sum_k A(m, k) * B(k, n) with
@@ -253,48 +270,52 @@ __global__ __launch_bounds__(256) void k_bgemm_nt(const __bf16* __restrict__ A, 
         }
 }
 
-// The same product for wide layers (N a multiple of 256, K of 64): block tile
-// 128 x 256 (eight waves of 64 x 64), K staged 64 at a time through three LDS
-// stages filled by direct-to-LDS loads, two stages in flight (the INT8
-// contraction's pipeline, csg_mfma.h).  A stage holds [A 128 rows][B 256
-// rows] of 64 BF16 (128 bytes); the 16-byte slot of row r holding K chunk c is
-// c ^ bw_swz(r).  Within a stage the two 32-wide K halves are accumulated in
-// order, so every sum equals k_bgemm_nt's bit for bit.  Tiles run in XCD
-// order, the two column tiles of a row tile next to each other.
-constexpr int BW_BM = 128, BW_BN = 256, BW_BK = 64, BW_NS = 3;
-constexpr int BW_STAGE = (BW_BM + BW_BN) * BW_BK * 2;  // 48 KB
-constexpr int BW_LDS   = BW_NS * BW_STAGE;
+// The hidden layers' product for wide layers (N a multiple of 256, K of 32),
+// with MODE 1's epilogue: block tile 256 x 256, eight waves of 128 x 64, each
+// in 4 x 2 tiles of v_mfma_f32_32x32x16_bf16 (the 32 x 32 MFMAs issue closer
+// to their peak rate than the 16 x 16 ones).  K is staged 32 at a time
+// through four LDS stages filled by direct-to-LDS loads, three in flight; a
+// stage holds [A 256 rows][B 256 rows] of 32 BF16 (64 bytes), the 16-byte
+// slot of row r holding K chunk c being c ^ bw_swz(r).  Epilogue: v = acc +
+// bias[n]; z = v (FP32, when given); the BF16 gelu(v) of a wave's 128 x 64
+// outputs is staged in LDS and written by 16-byte row stores.  Tiles run in
+// XCD order, the column tiles of a row tile next to each other.
+constexpr int BW_BM = 256, BW_BN = 256, BW_BK = 32, BW_NS = 4;
+constexpr int BW_STAGE = (BW_BM + BW_BN) * BW_BK * 2;  // 32 KB
+constexpr int BW_LDS   = BW_NS * BW_STAGE;             // also the epilogue's 8 x 16 KB
 
-__device__ __forceinline__ int bw_swz(int r) { return (r ^ (r >> 3)) & 7; }
+__device__ __forceinline__ int bw_swz(int r) { return (r >> 2) & 3; }
 
-template <int MODE>
+typedef float v16f_m __attribute__((ext_vector_type(16)));
+
 __global__ __launch_bounds__(512) void k_bgemm_nt_w(const __bf16* __restrict__ A, int lda,
                                                     const __bf16* __restrict__ B, int ldb, int M,
-                                                    int N, int K, int ldc, float* __restrict__ C,
+                                                    int N, int K, int ldh,
                                                     const float* __restrict__ bias,
                                                     float* __restrict__ z, __bf16* __restrict__ H)
 {
     extern __shared__ __attribute__((aligned(16))) int8_t bw_sm[];
     const int      tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int      wm = wid >> 2, wn = wid & 3;
+    const int      wm = wid >> 2, wn = wid & 3;  // 128-row half, 64-column quarter
     const uint32_t nct = (uint32_t)(N / BW_BN);
     const uint32_t t   = xcd_order(blockIdx.x, gridDim.x);
     const int      m0 = (int)(t / nct) * BW_BM, n0 = (int)(t % nct) * BW_BN;
-    // this lane's source (element offset, swizzled chunk) for its six row
-    // groups g = wid + 8 u of 8 rows: groups 0-15 A rows, 16-47 B rows
-    size_t src[6];
+    // this lane's source (element offset, swizzled chunk) for its four row
+    // groups g = wid + 8 u of 16 rows: groups 0-15 A rows, 16-31 B rows; a
+    // wave's load fills one group, lane l row l / 4, slot l % 4
+    size_t src[4];
 #pragma unroll
-    for (int u = 0; u < 6; ++u)
+    for (int u = 0; u < 4; ++u)
     {
         const int g = wid + 8 * u;
-        const int r = (g < 16 ? g : g - 16) * 8 + (lane >> 3);
-        const int c = (lane & 7) ^ bw_swz(r);
+        const int r = (g < 16 ? g : g - 16) * 16 + (lane >> 2);
+        const int c = (lane & 3) ^ bw_swz(r);
         src[u]      = g < 16 ? (size_t)min(m0 + r, M - 1) * lda + c * 8
                              : (size_t)min(n0 + r, N - 1) * ldb + c * 8;
     }
     auto issue = [&](int8_t* stage, int k0) {
 #pragma unroll
-        for (int u = 0; u < 6; ++u)
+        for (int u = 0; u < 4; ++u)
         {
             const int     g    = wid + 8 * u;
             const __bf16* base = g < 16 ? A : B;
@@ -303,85 +324,100 @@ __global__ __launch_bounds__(512) void k_bgemm_nt_w(const __bf16* __restrict__ A
                 0, 0);
         }
     };
-    v4f_m acc[4][4];
+    v16f_m acc[4][2];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            acc[i][j] = v4f_m{ 0.f, 0.f, 0.f, 0.f };
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                acc[i][j][e] = 0.f;
     const int T = K / BW_BK;
     issue(bw_sm, 0);
     if (T > 1)
         issue(bw_sm + BW_STAGE, BW_BK);
-    const int r16 = lane & 15, kq = lane >> 4;
+    if (T > 2)
+        issue(bw_sm + 2 * BW_STAGE, 2 * BW_BK);
+    // 32 x 32 x 16 operands: lane l holds row (l % 32), K elements 8 (l / 32) .. + 7
+    const int r32 = lane & 31, kh = lane >> 5;
     for (int s = 0; s < T; ++s)
     {
-        // stage s is complete when at most the six loads of stage s + 1 are
-        // outstanding; the barrier publishes every wave's loads and ends every
-        // wave's reads of stage s - 1, whose buffer stage s + 2 reuses
-        if (s + 1 < T)
-            __builtin_amdgcn_s_waitcnt(0x0F76);  // vmcnt <= 6
+        // stage s is complete when at most the loads of stages s + 1 and s + 2
+        // (four each) are outstanding; the barrier publishes every wave's
+        // loads and ends every wave's reads of stage s - 1, whose buffer
+        // stage s + 3 reuses
+        if (s + 2 < T)
+            __builtin_amdgcn_s_waitcnt(0x0F78);  // vmcnt <= 8
+        else if (s + 1 < T)
+            __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt <= 4
         else
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt 0
         asm volatile("s_barrier" ::: "memory");
-        if (s + 2 < T)
-            issue(bw_sm + ((s + 2) % BW_NS) * BW_STAGE, (s + 2) * BW_BK);
+        if (s + 3 < T)
+            issue(bw_sm + ((s + 3) % BW_NS) * BW_STAGE, (s + 3) * BW_BK);
         const int8_t* st = bw_sm + (s % BW_NS) * BW_STAGE;
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int ks = 0; ks < 2; ++ks)
         {
-            v8bf a[4], b[4];
+            const int ch = ks * 2 + kh;
+            v8bf      a[4], b[2];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
             {
-                const int row = wm * 64 + i * 16 + r16;
-                a[i] = *(const v8bf*)(st + row * 128 + (((kq + 4 * h) ^ bw_swz(row)) << 4));
+                const int row = wm * 128 + i * 32 + r32;
+                a[i] = *(const v8bf*)(st + row * 64 + ((ch ^ bw_swz(row)) << 4));
             }
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < 2; ++j)
             {
-                const int row = wn * 64 + j * 16 + r16;
-                b[j] = *(const v8bf*)(st + (BW_BM + row) * 128 + (((kq + 4 * h) ^ bw_swz(row)) << 4));
+                const int row = wn * 64 + j * 32 + r32;
+                b[j] = *(const v8bf*)(st + (BW_BM + row) * 64 + ((ch ^ bw_swz(row)) << 4));
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     }
-    // the lane's four column biases, loaded once (N is a multiple of BW_BN)
-    float bj[4] = { 0.f, 0.f, 0.f, 0.f };
-    if constexpr (MODE != 0)
-    {
+    // the lane's two column biases (N is a multiple of BW_BN)
+    float bj[2];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            bj[j] = bias[n0 + wn * 64 + j * 16 + r16];
-    }
+    for (int j = 0; j < 2; ++j)
+        bj[j] = bias[n0 + wn * 64 + j * 32 + r32];
+    __syncthreads();  // every wave is done with the stages
+    // 32 x 32 result: acc[e] of lane l is row 8 (e / 4) + 4 (l / 32) + e % 4,
+    // column l % 32.  The wave's 128 x 64 BF16 outputs go to its 16 KB of
+    // LDS (16-byte chunk c of row r at slot c ^ (r & 7)), then out by rows.
+    __bf16* w = (__bf16*)bw_sm + wid * (128 * 64);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-        {
-            const int m = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + e;
-            if (m >= M)
-                continue;
+        for (int e = 0; e < 16; ++e)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < 2; ++j)
             {
-                const int    n = n0 + wn * 64 + j * 16 + r16;
-                const size_t o = (size_t)m * ldc + n;
-                if constexpr (MODE == 0)
-                    C[o] = acc[i][j][e];
-                else
+                const float v   = acc[i][j][e] + bj[j];
+                const int   row = i * 32 + (e >> 2) * 8 + kh * 4 + (e & 3);
+                const int   col = j * 32 + r32;
+                if (z)
                 {
-                    const float v = acc[i][j][e] + bj[j];
-                    if (z)
-                        z[o] = v;
-                    H[o] = (__bf16)gelu_f(v);
+                    const int m = m0 + wm * 128 + row;
+                    if (m < M)
+                        z[(size_t)m * ldh + n0 + wn * 64 + col] = v;
                 }
+                w[row * 64 + (((col >> 3) ^ (row & 7)) << 3) + (col & 7)] = (__bf16)gelu_f(v);
             }
-        }
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");  // this wave's rows only
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+        const int r = q * 8 + (lane >> 3), c = lane & 7;
+        const int m = m0 + wm * 128 + r;
+        const int4 v = *(const int4*)(w + r * 64 + ((c ^ (r & 7)) << 3));
+        if (m < M)
+            *(int4*)(H + (size_t)m * ldh + n0 + wn * 64 + c * 8) = v;
+    }
 }
 
 // dst [C x R] (BF16) = transpose of src [R x C] (FP32).
