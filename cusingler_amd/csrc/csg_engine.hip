@@ -98,7 +98,8 @@ struct csg_engine
     float*    d_sqr_f = nullptr;
     double*    d_irr   = nullptr;  // inverse roots of the exact sums of X^2 (reference rows)
     bool      ref_ranked = false;
-    uint32_t  Qc = 0;
+    uint32_t  Qc = 0;        // query rows per chunk (the proposal path)
+    uint32_t  Qc_dense = 0;  // query rows per chunk of the dense score matrices
     uint32_t  qc_cap = 0;  // caller's cap on the query chunk (rows), 0: none
     // query batches value-coded on the device (csg_copyin_query_raw)
     uint32_t    q_cap = 0, fq_cap = 0;          // query rows the loaded inputs have room for
@@ -936,21 +937,30 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
         (rc = dalloc(eng, &eng->d_sqr_f, nr_pad)) || (rc = dalloc(eng, &eng->d_irr, nr_pad)))
         return rc;
     HIPCHK(hipMemsetAsync(eng->d_Xr, 0, nr_pad * eng->Kp * sizeof(int16_t), eng->stream));
-    // query chunk: the score chunk holds at most 2^31 - 2^20 scores (8 GiB, so
-    // that every chunk-local score index stays below 2^31) and at most a
-    // quarter of the free device memory
-    size_t budget = ((size_t)1 << 31) - ((size_t)1 << 20);
+    // query chunks: the score buffer takes at most a quarter of the free
+    // device memory.  The proposal path's scores are addressed by 64-bit
+    // segment offsets, so its chunk (Qc rows, every score of a chunk fitting
+    // the buffer) is bounded by memory only; a dense score matrix chunk
+    // (Qc_dense rows) also holds at most 2^31 - 2^20 scores, so that every
+    // chunk-local dense index stays below 2^31.
+    size_t budget = (size_t)1 << 36;
     {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b / 16 < budget)
             budget = std::max<size_t>(free_b / 16, (size_t)1 << 26);
     }
-    size_t qc = budget / (size_t)ref_height;
-    qc                  = std::max<size_t>(128, qc / 128 * 128);
-    qc                  = std::min<size_t>(qc, round_up(qry_height, 128));
-    if (eng->qc_cap)
-        qc = std::min<size_t>(qc, eng->qc_cap);
+    const size_t dense_budget = std::min(budget, ((size_t)1 << 31) - ((size_t)1 << 20));
+    auto rows_for = [&](size_t scores) {
+        size_t q = scores / (size_t)ref_height;
+        q        = std::max<size_t>(128, q / 128 * 128);
+        q        = std::min<size_t>(q, round_up(qry_height, 128));
+        if (eng->qc_cap)
+            q = std::min<size_t>(q, eng->qc_cap);
+        return q;
+    };
+    const size_t qc     = rows_for(budget);
     eng->Qc             = (uint32_t)qc;
+    eng->Qc_dense       = (uint32_t)std::min(qc, rows_for(dense_budget));
     if ((rc = dalloc(eng, &eng->d_Xq, qc * eng->Kp)) || (rc = dalloc(eng, &eng->d_sqq_f, qc)) ||
         (rc = dalloc(eng, &eng->d_irq, qc)) || (rc = dalloc(eng, &eng->d_S, qc * ref_height)) ||
         (rc = dalloc(eng, &eng->d_nan_list, qc * ct_num)) ||
@@ -1737,9 +1747,10 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
         eng->ref_prepared_path = path;
         eng->ref_kf            = kf;
     }
-    for (uint32_t q0 = 0; q0 < eng->Nq; q0 += eng->Qc)
+    const uint32_t chunk = prop ? eng->Qc : eng->Qc_dense;
+    for (uint32_t q0 = 0; q0 < eng->Nq; q0 += chunk)
     {
-        const uint32_t nq = std::min(eng->Qc, eng->Nq - q0);
+        const uint32_t nq = std::min(chunk, eng->Nq - q0);
         {
             PhaseScope ps(eng, 0, path == CSG_PATH_PLAIN || path == CSG_PATH_FP32_MFMA ? 1 : 2);
             // the exact path needs the FP32 sums only for re-scored cells
@@ -3942,7 +3953,7 @@ int csg_score_matrix(csg_engine* eng, int path, uint32_t q0, uint32_t nq, float*
         return eng->fail(CSG_ERR_STATE, "csg_score_matrix before csg_copyin_score");
     if (path < CSG_PATH_PLAIN || path > CSG_PATH_FP16_MFMA)
         return eng->fail(CSG_ERR_ARGUMENT, "unknown correlation path");
-    if (nq == 0 || nq > eng->Qc || q0 > eng->Nq || nq > eng->Nq - q0)
+    if (nq == 0 || nq > eng->Qc_dense || q0 > eng->Nq || nq > eng->Nq - q0)
         return eng->fail(CSG_ERR_ARGUMENT, "query rows outside the loaded cells or above one chunk");
     HIPCHK(hipSetDevice(eng->device));
     const size_t nr_pad = round_up(eng->Nr, 256);

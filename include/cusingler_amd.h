@@ -327,11 +327,14 @@ int csg_copyin_query_raw_u16(csg_engine* eng, uint32_t qry_height, uint32_t qry_
                              const int32_t* map_finetune);
 
 /* Caps the query chunk of the score step (src/cusingler.cu:335-409 slices the
- * query to the free device memory; here a chunk is at most 8 GiB of scores
- * and 1/16 of the free memory).  rows: a multiple of 128, 0 removes the cap;
- * it applies from the next csg_copyin_score.  *chunk (optional) receives the
- * chunk of the current inputs.  Lets a caller bound the score buffer, and
- * tests run several chunks on small inputs. */
+ * query to the free device memory; here the score buffer takes at most 1/16
+ * of the free memory, a chunk of the learned proposal path fills at most that
+ * buffer, and a chunk of a dense score matrix -- the paths without the
+ * proposal, csg_score_matrix -- holds at most 8 GiB of scores).  rows: a
+ * multiple of 128, 0 removes the cap; it applies from the next
+ * csg_copyin_score.  *chunk (optional) receives the proposal path's chunk of
+ * the current inputs.  Lets a caller bound the score buffer, and tests run
+ * several chunks on small inputs. */
 int csg_set_query_chunk(csg_engine* eng, uint32_t rows, uint32_t* chunk);
 
 /* destroy (src/cusingler.cuh:25): frees the fine-tune buffers. */
@@ -440,7 +443,7 @@ int csg_mlp_predict(csg_engine* eng, int k, int32_t* top, float* margin);
 int csg_set_proposal(csg_engine* eng, int k, float min_margin);
 
 /* Diagnostic: the per-pair correlation matrix [nq x ref_height] of query rows
- * [q0, q0 + nq) of the loaded score data (nq at most one query chunk) by the
+ * [q0, q0 + nq) of the loaded score data (nq at most one dense chunk) by the
  * given path's contraction (the exact path's values before certification),
  * for stating each path's error against FP64.  No counterpart in the
  * reference (the reference keeps this matrix internal, src/cusingler.cu:
