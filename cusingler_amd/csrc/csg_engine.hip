@@ -1384,7 +1384,12 @@ int mlp_propose(csg_engine* eng, uint32_t nq, int K, int32_t* top, float* margin
     {
 #define CSG_TOPK(k)                                                                                \
     case k:                                                                                        \
-        k_topk<k><<<(nq + 3) / 4, 256, 0, eng->stream>>>(eng->d_macc, b3, (int)nq, L, Lp, top, margin); \
+        if (L <= 64)                                                                               \
+            k_topk_rows<k><<<(nq + 127) / 128, 128, 0, eng->stream>>>(eng->d_macc, b3, (int)nq, L, \
+                                                                     Lp, top, margin);             \
+        else                                                                                       \
+            k_topk<k><<<(nq + 3) / 4, 256, 0, eng->stream>>>(eng->d_macc, b3, (int)nq, L, Lp, top, \
+                                                             margin);                              \
         break;
         CSG_TOPK(1)
         CSG_TOPK(2)
@@ -4883,10 +4888,103 @@ int csg_counters(csg_engine* eng, int64_t* out)
     return CSG_OK;
 }
 
+// The top-k kernels: k_topk_rows (one thread per row, L <= 64) against
+// k_topk (one wave per row) on seeded logits with ties, -inf entries and NaN
+// rows, for L = 1, 2, 17, 50, 64 and K = 1, 4, 8; returns the number of
+// differing outputs (labels, or margins by their bits).
+int selftest_topk(csg_engine* eng, double* mismatches)
+{
+    const int Lp = 64, rows = 1000;
+    std::vector<float> acc((size_t)rows * Lp), bias(Lp);
+    uint64_t           st = 0x2545F4914F6CDD1Dull;
+    auto               rnd = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    for (int j = 0; j < Lp; ++j)
+        bias[j] = (float)((int)(rnd() % 9) - 4) * 0.125f;
+    for (int r = 0; r < rows; ++r)
+        for (int j = 0; j < Lp; ++j)
+        {
+            const uint64_t u = rnd();
+            // coarse values (many ties), some -inf, a NaN row every 97 rows
+            float v = (float)((int)(u % 33) - 16) * 0.25f;
+            if (u % 61 == 0)
+                v = -__builtin_inff();
+            if (r % 97 == 5 && j == (int)(u % 50))
+                v = __builtin_nanf("");
+            acc[(size_t)r * Lp + j] = v;
+        }
+    float *d_acc = nullptr, *d_bias = nullptr, *d_m = nullptr;
+    int32_t* d_t = nullptr;
+    int      rc  = CSG_OK;
+    if ((rc = upload(eng, &d_acc, acc.data(), acc.size())) ||
+        (rc = upload(eng, &d_bias, bias.data(), bias.size())) ||
+        (rc = dalloc(eng, &d_t, (size_t)2 * rows * 8)) || (rc = dalloc(eng, &d_m, (size_t)2 * rows)))
+    {
+        dfree(d_acc); dfree(d_bias); dfree(d_t); dfree(d_m);
+        return rc;
+    }
+    double bad = 0;
+    for (int L : { 1, 2, 17, 50, 64 })
+        for (int K : { 1, 4, 8 })
+        {
+            if (K > L)
+                continue;
+            int32_t* t1 = d_t + (size_t)rows * 8;
+            float*   m1 = d_m + rows;
+#define CSG_TOPK_PAIR(k)                                                                           \
+    case k:                                                                                        \
+        k_topk<k><<<(rows + 3) / 4, 256, 0, eng->stream>>>(d_acc, d_bias, rows, L, Lp, d_t, d_m);  \
+        k_topk_rows<k><<<(rows + 127) / 128, 128, 0, eng->stream>>>(d_acc, d_bias, rows, L, Lp, t1, \
+                                                                    m1);                           \
+        break;
+            switch (K)
+            {
+                CSG_TOPK_PAIR(1)
+                CSG_TOPK_PAIR(4)
+                CSG_TOPK_PAIR(8)
+            }
+#undef CSG_TOPK_PAIR
+            std::vector<int32_t>  ta((size_t)rows * K), tb((size_t)rows * K);
+            std::vector<uint32_t> ma(rows), mb(rows);
+            hipError_t e0 = hipGetLastError();
+            hipError_t e1 = hipMemcpyAsync(ta.data(), d_t, ta.size() * 4, hipMemcpyDeviceToHost, eng->stream);
+            hipError_t e2 = hipMemcpyAsync(tb.data(), t1, tb.size() * 4, hipMemcpyDeviceToHost, eng->stream);
+            hipError_t e3 = hipMemcpyAsync(ma.data(), d_m, rows * 4, hipMemcpyDeviceToHost, eng->stream);
+            hipError_t e4 = hipMemcpyAsync(mb.data(), m1, rows * 4, hipMemcpyDeviceToHost, eng->stream);
+            hipError_t e5 = hipStreamSynchronize(eng->stream);
+            if (e0 != hipSuccess || e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess ||
+                e4 != hipSuccess || e5 != hipSuccess)
+            {
+                dfree(d_acc); dfree(d_bias); dfree(d_t); dfree(d_m);
+                return eng->fail(CSG_ERR_DEVICE, "top-k selftest failed");
+            }
+            for (size_t i = 0; i < ta.size(); ++i)
+                bad += ta[i] != tb[i];
+            for (int r = 0; r < rows; ++r)
+            {
+                const bool na = (ma[r] & 0x7F800000u) == 0x7F800000u && (ma[r] & 0x7FFFFFu);
+                const bool nb = (mb[r] & 0x7F800000u) == 0x7F800000u && (mb[r] & 0x7FFFFFu);
+                bad += na != nb || (!na && ma[r] != mb[r]);
+            }
+        }
+    dfree(d_acc); dfree(d_bias); dfree(d_t); dfree(d_m);
+    *mismatches = bad;
+    return CSG_OK;
+}
+
 int csg_selftest(csg_engine* eng, int which, double* max_err)
 {
-    if (!eng || !max_err || which < 0 || which > 6)
+    if (!eng || !max_err || which < 0 || which > 7)
         return CSG_ERR_ARGUMENT;
+    if (which == 7)
+    {
+        HIPCHK(hipSetDevice(eng->device));
+        return selftest_topk(eng, max_err);
+    }
     if (which == 4 && !eng->w8_ok)
         return eng->fail(CSG_ERR_DEVICE, "the device does not grant the wide kernel's LDS");
     if (which == 6 && !eng->kf_ok)

@@ -650,6 +650,90 @@ __global__ __launch_bounds__(256) void k_topk(const float* __restrict__ acc,
         margin[r] = L > 1 ? (expf(b0 - mx) - expf(b1 - mx)) / s : 1.f;
 }
 
+// The same for L <= 64 with one thread per row, bit for bit: 128 rows per
+// block staged through LDS (row stride 65, so a wave's rows fall in distinct
+// banks), each row's values in registers.  The selection scans the labels in
+// ascending order with k_topk's predicate, and the softmax sum adds the
+// per-label terms in the tree order of k_topk's butterfly (lane 0's sum:
+// pairs (j, j + o) for o = 32, 16, ..., 1).  The wave kernel's reductions are
+// chains of cross-lane shuffles, about a microsecond per row.
+template <int K>
+__global__ __launch_bounds__(128) void k_topk_rows(const float* __restrict__ acc,
+                                                   const float* __restrict__ bias, int rows,
+                                                   int L, int ldl, int32_t* __restrict__ top,
+                                                   float* __restrict__ margin)
+{
+    constexpr int KK = K < 2 ? 2 : K;
+    __shared__ float tile[128 * 65];
+    const int        r0 = blockIdx.x * 128;
+    const int        nr = min(128, rows - r0);
+    for (int idx = threadIdx.x; idx < nr * L; idx += 128)
+    {
+        const int rr = idx / L, j = idx - rr * L;
+        tile[rr * 65 + j] = acc[(size_t)(r0 + rr) * ldl + j] + bias[j];
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t >= nr)
+        return;
+    const int r = r0 + t;
+    float     v[64];
+    bool      nan = false;
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+    {
+        v[j] = j < L ? tile[t * 65 + j] : -__builtin_inff();
+        nan |= __builtin_isnan(v[j]);
+    }
+    if (nan)
+    {
+        for (int u = 0; u < K; ++u)
+            top[(size_t)r * K + u] = -1;
+        margin[r] = __builtin_nanf("");
+        return;
+    }
+    float mx = v[0];
+#pragma unroll
+    for (int j = 1; j < 64; ++j)
+        mx = fmaxf(mx, v[j]);
+    float e[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j)
+        e[j] = j < L ? expf(v[j] - mx) : 0.f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int j = 0; j < o; ++j)
+            e[j] = e[j] + e[j + o];
+    const float s     = e[0];
+    uint64_t    taken = 0;
+    float       b0 = 0.f, b1 = 0.f;
+#pragma unroll
+    for (int tt = 0; tt < KK; ++tt)
+    {
+        float bv = -__builtin_inff();
+        int   bj = 1 << 30;
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+            if (!((taken >> j) & 1) && j < L && (v[j] > bv || (v[j] == bv && j < bj)))
+            {
+                bv = v[j];
+                bj = j;
+            }
+        if (bj >= L)
+            bj = -1;  // fewer than KK labels
+        else
+            taken |= 1ull << bj;
+        if (tt < K)
+            top[(size_t)r * K + tt] = bj;
+        if (tt == 0)
+            b0 = bv;
+        if (tt == 1)
+            b1 = bv;
+    }
+    margin[r] = L > 1 ? (expf(b0 - mx) - expf(b1 - mx)) / s : 1.f;
+}
+
 // Rows perm[0, rows) of a [* x cols] BF16 matrix (cols a multiple of 8) and
 // their labels, gathered into a minibatch.
 __global__ void k_gather_batch(const __bf16* __restrict__ feat, const int32_t* __restrict__ labels,

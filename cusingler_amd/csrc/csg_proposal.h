@@ -152,39 +152,74 @@ __global__ void k_prop_scatter_p(const RecheckItem* __restrict__ items, uint32_t
 // listed cell also takes slots (counted in cnt) in the lists of its ranked
 // labels [edge + 1, KP).  Cells with a NaN score are left to the
 // certification, which re-scores all of their labels.
-__global__ void k_prop_check(const float* __restrict__ P, uint32_t n, uint32_t L,
-                             const int32_t* __restrict__ top, const float* __restrict__ margin,
-                             int KP, int edge, float thr, float thre, double wp,
-                             const uint32_t* __restrict__ in, int widen,
-                             uint32_t* __restrict__ cnt, uint32_t* __restrict__ slot,
-                             uint32_t* __restrict__ list, uint32_t* __restrict__ n_list)
+// One thread per cell.  A listed cell's list position and label slots are
+// taken in LDS first and each block adds its totals to the global counters
+// once (per-thread atomics on the few counters, which share a cache line,
+// had serialised in one L2 channel).
+__global__ __launch_bounds__(256) void k_prop_check(
+    const float* __restrict__ P, uint32_t n, uint32_t L, const int32_t* __restrict__ top,
+    const float* __restrict__ margin, int KP, int edge, float thr, float thre, double wp,
+    const uint32_t* __restrict__ in, int widen, uint32_t* __restrict__ cnt,
+    uint32_t* __restrict__ slot, uint32_t* __restrict__ list, uint32_t* __restrict__ n_list)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const uint32_t c = in ? in[i] : i;
-    if (!in && !prop_confident(top, margin, c, KP, L, thr))
-        return;
-    const float* s   = P + (size_t)c * L;
-    float        mx  = -__builtin_inff();
-    bool         nan = false;
-    // (no early exit at a NaN, so that the row's loads are in flight together)
-#pragma unroll 8
-    for (uint32_t j = 0; j < L; ++j)
+    __shared__ uint32_t hist[257], base[257];
+    for (uint32_t j = threadIdx.x; j <= L; j += blockDim.x)
+        hist[j] = 0;
+    __syncthreads();
+    const uint32_t i    = blockIdx.x * blockDim.x + threadIdx.x;
+    bool           pass = false;
+    uint32_t       c    = 0;
+    if (i < n)
     {
-        const float v = s[j];
-        nan |= __builtin_isnan(v);
-        mx = fmaxf(mx, v);
+        c = in ? in[i] : i;
+        if (in || prop_confident(top, margin, c, KP, L, thr))
+        {
+            const float* s   = P + (size_t)c * L;
+            float        mx  = -__builtin_inff();
+            bool         nan = false;
+            // (no early exit at a NaN, so that the row's loads are in flight together)
+#pragma unroll 8
+            for (uint32_t j = 0; j < L; ++j)
+            {
+                const float v = s[j];
+                nan |= __builtin_isnan(v);
+                mx = fmaxf(mx, v);
+            }
+            if (!nan)
+            {
+                const float last = s[top[(size_t)c * KP + edge]];
+                pass = !((double)last + 2.0 * wp < (double)mx - (double)thre);
+            }
+        }
     }
-    if (nan)
-        return;
-    const float last = s[top[(size_t)c * KP + edge]];
-    if ((double)last + 2.0 * wp < (double)mx - (double)thre)
-        return;
-    list[atomicAdd(n_list, 1u)] = c;
-    if (widen)
-        for (int t = edge + 1; t < KP; ++t)
-            slot[(size_t)c * KP + t] = atomicAdd(&cnt[top[(size_t)c * KP + t]], 1u);
+    uint32_t lpos = 0, ls[8];
+    int      lab[8];
+    if (pass)
+    {
+        lpos = atomicAdd(&hist[L], 1u);
+        if (widen)
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (t > edge && t < KP)
+                {
+                    lab[t] = top[(size_t)c * KP + t];
+                    ls[t]  = atomicAdd(&hist[lab[t]], 1u);
+                }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j <= L; j += blockDim.x)
+        if (hist[j])
+            base[j] = atomicAdd(j < L ? &cnt[j] : n_list, hist[j]);
+    __syncthreads();
+    if (pass)
+    {
+        list[base[L] + lpos] = c;
+        if (widen)
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (t > edge && t < KP)
+                    slot[(size_t)c * KP + t] = base[lab[t]] + ls[t];
+    }
 }
 
 }  // namespace csg

@@ -468,8 +468,11 @@ int csg_set_limb_form(csg_engine* eng, int form, int* prev);
  * *max_err receives 0 when the device results equal the host's
  * exact (0, 4, 5, 6) / sequential-fma (3) values, else the largest difference (+1); for
  * 1 and 2 the largest error of the dot product relative to sqrt(sxx * syy),
- * i.e. in correlation units.  No
- * counterpart in the reference: it checks the MFMA operand layouts. */
+ * i.e. in correlation units.  which = 7 compares the proposal's two top-k
+ * kernels (one thread per row for up to 64 labels, one wave per row) on
+ * seeded logits with ties, -inf and NaN rows; *max_err receives the number
+ * of differing labels and margins (bits).  No counterpart in the reference:
+ * it checks the MFMA operand layouts and kernel equivalences. */
 int csg_selftest(csg_engine* eng, int which, double* max_err);
 
 /* Diagnostic of the fine-tune correlation kernels (wide and thin tiles): 40 x

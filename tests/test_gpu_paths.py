@@ -39,6 +39,13 @@ def test_contraction_selftest(eng, which, name):
         assert err < 1e-5, f"{name} relative error {err}"
 
 
+def test_topk_kernels_bit_equal(eng):
+    """The proposal's top-k in one thread per row (up to 64 labels) equals the
+    wave-per-row kernel: labels and margin bits, with ties, -inf and NaN rows."""
+    _say("selftest top-k")
+    assert eng.selftest(7) == 0
+
+
 @pytest.mark.parametrize("gs", [20, 64, 127, 128, 129, 255, 256, 257, 300, 1000])
 def test_finetune_chunk_tree_bit_exact(eng, gs):
     _say("ft chunk tree", gs)
