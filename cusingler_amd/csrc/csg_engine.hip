@@ -149,6 +149,13 @@ struct csg_engine
     uint64_t* d_nan_list = nullptr;
     unsigned int* d_nan_count = nullptr;
     int*      d_status = nullptr;
+    // counters the score step's kernels add on the device (the indices of
+    // `counters`), [32] the undefined-sort flag of its NaN segments, [36, 42)
+    // the proposal rounds' plans (d_pplan: per round tiles, items, 2 unused,
+    // as uint32), [42] the certification's pairs and tiles (uint32); read
+    // back in one copy with the step's closing synchronisation
+    unsigned long long* d_dcnt = nullptr;
+    unsigned long long  h_dcnt[48] = {};
     // exact / approximate MFMA paths
     int8_t *  d_rh = nullptr, *d_rl = nullptr, *d_qh = nullptr, *d_ql = nullptr;
     double *  d_dr = nullptr, *d_dq = nullptr;  // rows' corrections, three-product form
@@ -318,14 +325,19 @@ struct csg_engine
     float*     d_pmargin = nullptr;
     uint32_t * d_pcnt = nullptr, *d_pslot = nullptr, *d_pfb = nullptr, *d_pcells = nullptr;
     uint32_t*  d_plist = nullptr;  // cells the edge tests list (two lists of Qc)
-    // per-round label tables of the grouped contraction, views into d_pmeta:
-    // [c_off L (u64) | a_base L | a_cnt L | tiles], uploaded in one copy from
-    // the pinned h_pmeta
-    uint32_t * d_pabase = nullptr, *d_pacnt = nullptr;
+    // per-round label tables of the grouped contraction, built on the device
+    // by k_prop_plan, views into d_pmeta: [c_off L (u64) | a_base L | a_cnt L
+    // | 48 bytes unused | tiles (bound: every label's list at Qc cells)]; the
+    // round plans (d_pplan) are part of d_dcnt
+    uint32_t * d_pabase = nullptr, *d_pacnt = nullptr, *d_pplan = nullptr;
+    // the previous score step's device counts, read back with its closing
+    // synchronisation, size the next one's grids (a kernel loops where a
+    // count outgrew its grid): per proposal round the tiles and items, the
+    // certification's pairs and tiles
+    uint32_t   est_tiles[3] = {}, est_items[3] = {}, est_pairs = 0, est_rc_tiles = 0;
     uint64_t*  d_pcoff = nullptr;
     uint8_t*   d_pmeta = nullptr;
-    uint8_t*   h_pmeta = nullptr;
-    size_t     pmeta_cap = 0, h_pmeta_cap = 0;
+    size_t     pmeta_cap = 0;
     RecheckItem* d_pitems = nullptr;
     Segment*   d_pseg = nullptr;
     GroupTile* d_ptiles = nullptr;
@@ -398,6 +410,24 @@ template <class T> int upload(csg_engine* eng, T** p, const T* src, size_t count
 }
 
 // Grow-only device allocation.
+// Grid of a persistent or grid-stride kernel over device-counted work:
+// per_cu blocks per CU, a multiple of 8 (XcdRange).
+unsigned persistent_grid(const csg_engine* eng, int per_cu)
+{
+    const unsigned cu = (unsigned)std::max(1, eng->prop.multiProcessorCount);
+    return (cu * per_cu + 7) / 8 * 8;
+}
+
+// Grid for `est` units of work (the previous step's count), `per_block`
+// units per block, at least the persistent grid of per_cu blocks per CU; a
+// multiple of 8 (XcdRange), with some room for growth.
+unsigned est_grid(const csg_engine* eng, uint32_t est, uint32_t per_block, int per_cu)
+{
+    const uint64_t need = ((uint64_t)est * 33 / 32 + per_block - 1) / per_block + 8;
+    return std::max<unsigned>(persistent_grid(eng, per_cu),
+                              (unsigned)std::min<uint64_t>(need / 8 * 8 + 8, 1u << 30));
+}
+
 template <class T> int ensure(csg_engine* eng, T** p, size_t* cap, size_t n)
 {
     if (n <= *cap && *p)
@@ -683,14 +713,20 @@ int csg_init_gpu(int gpuid, csg_engine** out)
                  hipFuncSetAttribute((const void*)k_gemm_int8w<false>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      W8_LDS) == hipSuccess &&
-                 hipFuncSetAttribute((const void*)k_gemm_int8w_group<false>,
+                 hipFuncSetAttribute((const void*)k_gemm_int8w_group<false, false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     W8_LDS) == hipSuccess &&
+                 hipFuncSetAttribute((const void*)k_gemm_int8w_group<false, true>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      W8_LDS) == hipSuccess;
     eng->kf_ok = eng->w8_ok && (size_t)eng->prop.sharedMemPerBlock >= (size_t)W8_LDS_KF &&
                  hipFuncSetAttribute((const void*)k_gemm_int8w<true>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      W8_LDS_KF) == hipSuccess &&
-                 hipFuncSetAttribute((const void*)k_gemm_int8w_group<true>,
+                 hipFuncSetAttribute((const void*)k_gemm_int8w_group<true, false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     W8_LDS_KF) == hipSuccess &&
+                 hipFuncSetAttribute((const void*)k_gemm_int8w_group<true, true>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      W8_LDS_KF) == hipSuccess;
     eng->x3_ok = (size_t)eng->prop.sharedMemPerBlock >= (size_t)X3_LDS &&
@@ -714,7 +750,8 @@ int csg_init_gpu(int gpuid, csg_engine** out)
         eng->ftd_grid_corr    = cu * 2;
         eng->ftd_grid_select  = cu * 8;
     }
-    if (dalloc(eng, &eng->d_nan_count, 1) || dalloc(eng, &eng->d_status, 1))
+    if (dalloc(eng, &eng->d_nan_count, 1) || dalloc(eng, &eng->d_status, 1) ||
+        dalloc(eng, &eng->d_dcnt, 48))
     {
         delete eng;
         return CSG_ERR_MEMORY;
@@ -864,6 +901,7 @@ void csg_release(csg_engine* eng)
     dfree(eng->d_first);
     dfree(eng->d_mask);
     dfree(eng->d_nan_count);
+    dfree(eng->d_dcnt);
     dfree(eng->d_status);
     collect_phases(eng);
     for (auto e : eng->event_pool)
@@ -958,7 +996,10 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width, const
             q = std::min<size_t>(q, eng->qc_cap);
         return q;
     };
-    const size_t qc     = rows_for(budget);
+    // (and a chunk's INT8 operand plane, Qc x Kp bytes, below 2^32: the wide
+    // kernels' 32-bit row offsets)
+    const size_t qc     = std::min(rows_for(budget),
+                                   std::max<size_t>(128, (((size_t)1 << 32) - 1) / eng->Kp / 128 * 128));
     eng->Qc             = (uint32_t)qc;
     eng->Qc_dense       = (uint32_t)std::min(qc, rows_for(dense_budget));
     if ((rc = dalloc(eng, &eng->d_Xq, qc * eng->Kp)) || (rc = dalloc(eng, &eng->d_sqq_f, qc)) ||
@@ -1113,6 +1154,21 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
     int32_t*       F  = eng->d_first + q0;
     float*         M  = eng->d_mask + (size_t)q0 * L;
     const double   wp = certify_width(eng->G);
+    // the re-score tables, for every (cell, label) pair of a chunk
+    uint32_t max_len = 1;
+    for (uint32_t l = 0; l < L; ++l)
+        max_len = std::max(max_len, eng->ctidx[2 * l + 1]);
+    const size_t pairs = (size_t)eng->Qc * L;
+    const size_t tile_bound = (pairs / PG_GBM + L) * ((max_len + PG_BN - 1) / PG_BN);
+    int rc;
+    if ((rc = ensure(eng, &eng->d_item_seg, &eng->rc_cap_seg, pairs)) ||
+        (rc = ensure(eng, &eng->d_rc_cells, &eng->rc_cap_cells, pairs)) ||
+        (rc = ensure(eng, &eng->d_rc_sorted, &eng->rc_cap_sorted, pairs)) ||
+        (rc = ensure(eng, &eng->d_rc_abase, &eng->rc_cap_l, 4 * (size_t)L + 2)) ||
+        (rc = ensure(eng, &eng->d_rc_coff, &eng->rc_cap_coff, L)) ||
+        (rc = ensure(eng, &eng->d_rc_tiles, &eng->rc_cap_tiles, tile_bound)) ||
+        (rc = ensure(eng, &eng->d_rc_ucells, &eng->rc_cap_ucells, eng->Qc)))
+        return rc;
     HIPCHK(hipMemsetAsync(eng->d_nitems, 0, 2 * sizeof(unsigned int), eng->stream));
     HIPCHK(hipMemsetAsync(eng->d_exact, 0, (size_t)nq * L, eng->stream));
     {
@@ -1121,34 +1177,15 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
             P, nq, L, wp, thre, F, M, eng->d_cstatus, eng->d_items, eng->d_nitems);
         LAUNCHCHK();
     }
-    unsigned int hn[2] = { 0, 0 };  // pairs to re-score, uncertified cells
-    HIPCHK(hipMemcpyAsync(hn, eng->d_nitems, sizeof(hn), hipMemcpyDeviceToHost, eng->stream));
-    HIPCHK(hipStreamSynchronize(eng->stream));
-    const unsigned int n = hn[0];
-    if (n == 0)
-        return CSG_OK;
-    eng->counters[3] += n;
-    eng->counters[4] += hn[1];
-    // the pairs grouped by label on the device: label l's cells listed from
-    // abase[l], a cell's scores against the label's rows contiguous from
-    // coff[l] on, so that every pair's scores are one contiguous segment
-    uint32_t max_len = 1;
-    for (uint32_t l = 0; l < L; ++l)
-        max_len = std::max(max_len, eng->ctidx[2 * l + 1]);
-    const size_t tile_bound = ((size_t)n / PG_GBM + L) * ((max_len + PG_BN - 1) / PG_BN);
-    // a cell's pairs take at most Nr score slots (one per reference row of its
-    // labels), so the grouped scores fit the score chunk
-    if ((uint64_t)hn[1] * eng->Nr > (uint64_t)eng->Qc * eng->Nr)
-        return eng->fail(CSG_ERR_MEMORY, "re-scored pairs exceed the score buffer");
-    int rc;
-    if ((rc = ensure(eng, &eng->d_item_seg, &eng->rc_cap_seg, n)) ||
-        (rc = ensure(eng, &eng->d_rc_cells, &eng->rc_cap_cells, n)) ||
-        (rc = ensure(eng, &eng->d_rc_sorted, &eng->rc_cap_sorted, n)) ||
-        (rc = ensure(eng, &eng->d_rc_abase, &eng->rc_cap_l, 4 * (size_t)L + 2)) ||
-        (rc = ensure(eng, &eng->d_rc_coff, &eng->rc_cap_coff, L)) ||
-        (rc = ensure(eng, &eng->d_rc_tiles, &eng->rc_cap_tiles, tile_bound)) ||
-        (rc = ensure(eng, &eng->d_rc_ucells, &eng->rc_cap_ucells, std::max<uint32_t>(hn[1], 1))))
-        return rc;
+    // The pairs to re-score (d_nitems[0], in d_items) grouped by label on the
+    // device: label l's cells listed from abase[l], a cell's scores against
+    // the label's rows contiguous from coff[l] on, so that every pair's
+    // scores are one contiguous segment.  The counts stay on the device: the
+    // tables are sized for every pair of the chunk (csg_get_label), the
+    // kernels loop over the counted items and tiles.  A cell's pairs take at
+    // most Nr score slots (one per reference row of its labels), so the
+    // grouped scores fit the score buffer.
+    uint32_t* n_pairs = eng->d_nitems;
     // one uint32 table: [cnt L | abase L | tbase L | fill L | ntiles | nu]
     uint32_t* cnt    = eng->d_rc_abase;
     uint32_t* abase  = cnt + L;
@@ -1156,45 +1193,51 @@ int certify_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
     uint32_t* fill   = tbase + L;
     uint32_t* ntiles = fill + L;
     uint32_t* nu     = ntiles + 1;
+    const unsigned gs = est_grid(eng, eng->est_pairs, 256, 4);
     {
         PhaseScope ps(eng, 4, 3);
         k_fill_u32<<<1, 256, 0, eng->stream>>>(cnt, (size_t)4 * L + 2, 0u);
-        k_rc_count<<<(n + 255) / 256, 256, 0, eng->stream>>>(eng->d_items, n, cnt);
+        k_rc_count<<<gs, 256, 0, eng->stream>>>(eng->d_items, n_pairs, cnt);
         k_rc_scan<<<1, 64, 0, eng->stream>>>(cnt, eng->d_llen, L, PG_GBM, PG_BN, abase,
                                              eng->d_rc_coff, tbase, fill, ntiles);
-        k_rc_scatter<<<(n + 255) / 256, 256, 0, eng->stream>>>(
-            eng->d_items, n, abase, eng->d_rc_coff, eng->d_llen, fill, eng->d_rc_cells,
-            eng->d_rc_sorted, eng->d_item_seg);
+        k_rc_scatter<<<gs, 256, 0, eng->stream>>>(eng->d_items, n_pairs, abase, eng->d_rc_coff,
+                                                  eng->d_llen, fill, eng->d_rc_cells,
+                                                  eng->d_rc_sorted, eng->d_item_seg);
         k_rc_tiles<<<(L + 63) / 64, 64, 0, eng->stream>>>(cnt, eng->d_llen, tbase, L, PG_GBM,
                                                           PG_BN, eng->d_rc_tiles);
         k_rc_cells<<<(nq + 255) / 256, 256, 0, eng->stream>>>(eng->d_cstatus, nq,
                                                               eng->d_rc_ucells, nu);
         // the reference's FP32 sums of squares of the re-scored cells (the
         // exact path's rank pass leaves them out)
-        k_sq_seq<<<std::max(1u, (hn[1] + 255) / 256), 256, 0, eng->stream>>>(eng->d_Xq, eng->Kp, eng->G,
-                                                               eng->d_rc_ucells, nu, eng->d_sqq_f);
-        k_gemm_plain_group<<<(unsigned)tile_bound, 256, 0, eng->stream>>>(
+        k_sq_seq<<<(nq + 255) / 256, 256, 0, eng->stream>>>(eng->d_Xq, eng->Kp, eng->G,
+                                                            eng->d_rc_ucells, nu, eng->d_sqq_f);
+        k_gemm_plain_group<<<est_grid(eng, eng->est_rc_tiles, 1, 4), 256, 0, eng->stream>>>(
             eng->d_rc_tiles, eng->d_rc_cells, abase, cnt, eng->d_lstart, eng->d_llen,
             eng->d_rc_coff, eng->d_Xq, eng->d_Xr, (int)eng->Kp,
             (uint32_t)round_up(eng->Nr, 256) - 1, eng->d_sqq_f, eng->d_sqr_f, eng->d_S, ntiles);
         LAUNCHCHK();
         HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
         TableSegments src{ eng->d_item_seg };
-        k_select<TableSegments><<<(n + 3) / 4, 256, 0, eng->stream>>>(
-            src, n, eng->d_S, eng->d_pct, quantile, eng->d_Pref, eng->d_nan_list,
+        k_select_n<TableSegments><<<est_grid(eng, eng->est_pairs, 4, 8), 256, 0, eng->stream>>>(
+            src, n_pairs, eng->d_S, eng->d_pct, quantile, eng->d_Pref, eng->d_nan_list,
             eng->d_nan_count);
         LAUNCHCHK();
+        // NaN-carrying segments: the libstdc++ sort emulation in place (an
+        // undefined sort sets the flag the step's closing read-back checks)
+        k_select_nan_inplace<TableSegments><<<64, 64, 0, eng->stream>>>(
+            src, eng->d_nan_list, eng->d_nan_count, eng->d_S, eng->d_pct, quantile, eng->d_Pref,
+            (int*)(eng->d_dcnt + 32));
+        LAUNCHCHK();
     }
-    if ((rc = resolve_nan_segments(eng, TableSegments{ eng->d_item_seg }, eng->d_S, eng->d_pct,
-                                   quantile, eng->d_Pref)))
-        return rc;
     {
         PhaseScope ps(eng, 4, 2);
-        k_apply_recheck<<<(n + 255) / 256, 256, 0, eng->stream>>>(eng->d_rc_sorted, n, eng->d_Pref, P,
-                                                                  L, eng->d_exact);
+        k_apply_recheck<<<gs, 256, 0, eng->stream>>>(eng->d_rc_sorted, n_pairs, eng->d_Pref, P, L,
+                                                     eng->d_exact);
         LAUNCHCHK();
         k_label_final<<<(nq + 255) / 256, 256, 0, eng->stream>>>(P, eng->d_cstatus, eng->d_exact,
                                                                  nq, L, thre, F, M);
+        LAUNCHCHK();
+        k_cert_tally<<<1, 64, 0, eng->stream>>>(eng->d_nitems, eng->d_nan_count, eng->d_dcnt);
         LAUNCHCHK();
     }
     return CSG_OK;
@@ -1407,24 +1450,40 @@ int mlp_propose(csg_engine* eng, uint32_t nq, int K, int32_t* top, float* margin
     return CSG_OK;
 }
 
-// One grouped scoring round of a chunk.  Label l's list holds cw[l] cells
+// One grouped scoring round of a chunk, planned on the device (no host
+// read-back).  Label l's list holds cw[l] cells (device counts; NULL: none)
 // scored on the window [t0, t1) of their ranked labels (`win` names them, or
-// NULL for every confident cell of the chunk with n_win = nq), followed by
-// the n_all cells of `all`, scored on every label (`base`: device copy of
-// cw, their slot offset).  Every (cell, label) item is contracted, its
-// quantile selected and stored in the chunk's quantile rows P.
+// NULL for every confident cell of the chunk; n_win bounds their number,
+// *n_win_dev counts them when given), followed by the *n_all cells of `all`
+// (NULL: none), scored on every label (`base`: cw, their slot offset).  Every
+// (cell, label) item is contracted, its quantile selected and stored in the
+// chunk's quantile rows P.  A label's list holds at most nq cells (a cell is
+// in each list once), so the items and scores fit the tables and the score
+// buffer, and the tiles the planned bound.
 struct PropRound
 {
-    std::vector<uint32_t> cw;
-    const uint32_t*       win   = nullptr;
-    uint32_t              n_win = 0;
-    int                   t0 = 0, t1 = 0;
-    const uint32_t*       all   = nullptr;
-    uint32_t              n_all = 0;
-    const uint32_t*       base  = nullptr;
+    const uint32_t* cw        = nullptr;
+    const uint32_t* win       = nullptr;
+    uint32_t        n_win     = 0;
+    const uint32_t* n_win_dev = nullptr;
+    int             t0 = 0, t1 = 0;
+    const uint32_t* all   = nullptr;
+    const uint32_t* n_all = nullptr;
+    const uint32_t* base  = nullptr;
 };
 
-int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
+// Bound on a round's tiles: every label's list at Qc cells, in the smallest
+// tiles of the three grouped kernels.
+size_t prop_tile_bound(const csg_engine* eng)
+{
+    size_t t = 0;
+    for (uint32_t l = 0; l < eng->L; ++l)
+        t += ((size_t)eng->ctidx[2 * l + 1] + I8_BN - 1) / I8_BN;
+    return t * (((size_t)eng->Qc + I8_BM - 1) / I8_BM);
+}
+
+int prop_round(csg_engine* eng, const PropRound& r, int round, uint32_t nq, float quantile,
+               float* P)
 {
     const uint32_t L = eng->L;
     // wide 128 x 256 blocks with folded limbs up to 1920 genes, else the
@@ -1432,120 +1491,93 @@ int prop_round(csg_engine* eng, const PropRound& r, float quantile, float* P)
     const bool wide = eng->G <= 1920 && eng->w8_ok;
     const bool x3   = !wide && eng->x3_ok;
     const uint32_t tbm = wide ? W8_BM : x3 ? X3_BM : I8_BM, tbn = wide ? W8_BN : x3 ? X3_BN : I8_BN;
-    std::vector<uint32_t>  abase(L), acnt(L);
-    std::vector<uint64_t>  coff(L);
-    std::vector<GroupTile> tiles;
-    uint64_t               n_items = 0, n_scores = 0;
-    for (uint32_t l = 0; l < L; ++l)
-    {
-        const uint32_t na = r.cw[l] + r.n_all;
-        const uint32_t nl = eng->ctidx[2 * l + 1];
-        abase[l]          = (uint32_t)n_items;
-        acnt[l]           = na;
-        coff[l]           = n_scores;
-        n_items += na;
-        n_scores += (uint64_t)na * nl;
-        for (uint32_t m0 = 0; m0 < na; m0 += tbm)
-            for (uint32_t n0 = 0; n0 < nl; n0 += tbn)
-                tiles.push_back(GroupTile{ l, m0, n0, 0u });
-    }
-    if (n_items == 0)
-        return CSG_OK;
-    if (n_scores > (uint64_t)eng->Qc * eng->Nr || n_items > (uint64_t)eng->Qc * L)
-        return eng->fail(CSG_ERR_MEMORY, "proposal round exceeds the score buffer");
-    int rc;
-    // one pinned upload of the round's tables (the previous round's copy has
-    // completed: a host read of counts separates the rounds)
-    const size_t o_ab = (size_t)L * 8, o_ac = o_ab + (size_t)L * 4;
-    const size_t o_t  = (o_ac + (size_t)L * 4 + 15) & ~(size_t)15;
-    const size_t bytes = o_t + tiles.size() * sizeof(GroupTile);
-    if ((rc = ensure(eng, &eng->d_pmeta, &eng->pmeta_cap, bytes)))
-        return rc;
-    if (bytes > eng->h_pmeta_cap)
-    {
-        if (eng->h_pmeta)
-            (void)hipHostFree(eng->h_pmeta);
-        eng->h_pmeta     = nullptr;
-        eng->h_pmeta_cap = 0;
-        const size_t want = std::max(bytes, eng->h_pmeta_cap * 3 / 2);
-        HIPCHK(hipHostMalloc((void**)&eng->h_pmeta, want, hipHostMallocDefault));
-        eng->h_pmeta_cap = want;
-    }
-    std::memcpy(eng->h_pmeta, coff.data(), (size_t)L * 8);
-    std::memcpy(eng->h_pmeta + o_ab, abase.data(), (size_t)L * 4);
-    std::memcpy(eng->h_pmeta + o_ac, acnt.data(), (size_t)L * 4);
-    std::memcpy(eng->h_pmeta + o_t, tiles.data(), tiles.size() * sizeof(GroupTile));
-    HIPCHK(hipMemcpyAsync(eng->d_pmeta, eng->h_pmeta, bytes, hipMemcpyHostToDevice, eng->stream));
-    eng->d_pcoff  = (uint64_t*)eng->d_pmeta;
-    eng->d_pabase = (uint32_t*)(eng->d_pmeta + o_ab);
-    eng->d_pacnt  = (uint32_t*)(eng->d_pmeta + o_ac);
-    eng->d_ptiles = (GroupTile*)(eng->d_pmeta + o_t);
+    uint32_t* ntiles = eng->d_pplan + 4 * round;
+    uint32_t* nitems = ntiles + 1;
+    const unsigned g_tiles = est_grid(eng, eng->est_tiles[round], 1, 1);
+    const unsigned g_items = est_grid(eng, eng->est_items[round], 4, 8);
+    k_prop_plan<<<32, 256, 0, eng->stream>>>(r.cw, r.n_all, eng->d_llen, L, tbm, tbn,
+                                            eng->d_pabase, eng->d_pacnt, eng->d_pcoff,
+                                            eng->d_ptiles, ntiles, eng->d_dcnt);
+    LAUNCHCHK();
     {
         PhaseScope ps(eng, 9, 1);
         if (r.t1 > r.t0 && r.n_win)
         {
             k_prop_scatter_win<<<nblk(r.n_win), 256, 0, eng->stream>>>(
-                eng->d_ptop, eng->d_pmargin, eng->prop_margin, r.win, r.n_win, eng->prop_kp, r.t0,
-                r.t1, L, eng->d_pslot, eng->d_pabase, eng->d_pcoff, eng->d_llen, eng->d_pcells,
-                eng->d_pitems, eng->d_pseg);
+                eng->d_ptop, eng->d_pmargin, eng->prop_margin, r.win, r.n_win, r.n_win_dev,
+                eng->prop_kp, r.t0, r.t1, L, eng->d_pslot, eng->d_pabase, eng->d_pcoff,
+                eng->d_llen, eng->d_pcells, eng->d_pitems, eng->d_pseg);
             LAUNCHCHK();
         }
-        if (r.n_all)
+        if (r.all)
         {
-            k_prop_scatter_all<<<nblk(r.n_all), 256, 0, eng->stream>>>(
-                r.all, r.n_all, L, r.base, eng->d_pabase, eng->d_pcoff, eng->d_llen, eng->d_pcells,
-                eng->d_pitems, eng->d_pseg);
+            k_prop_scatter_all<<<nblk(nq), 256, 0, eng->stream>>>(
+                r.all, nq, r.n_all, L, r.base, eng->d_pabase, eng->d_pcoff, eng->d_llen,
+                eng->d_pcells, eng->d_pitems, eng->d_pseg);
             LAUNCHCHK();
         }
     }
-    if (!tiles.empty())
     {
         PhaseScope ps(eng, 1, 1);
         const uint32_t b_max = (uint32_t)round_up(eng->Nr, 256) - 1;
-        if (eng->ref_kf)
-            k_gemm_int8w_group<true><<<(unsigned)tiles.size(), 512, W8_LDS_KF, eng->stream>>>(
-                eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
-                eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, eng->d_dq, eng->d_dr);
-        else if (wide)
-            k_gemm_int8w_group<false><<<(unsigned)tiles.size(), 512, W8_LDS, eng->stream>>>(
-                eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
-                eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, nullptr, nullptr);
+        if (wide)
+        {
+            // one tile per block on the expected count's grid, then the tiles
+            // past its capacity (none unless the count grew) in a looping form
+            const bool    kf  = eng->ref_kf;
+            const size_t  lds = kf ? W8_LDS_KF : W8_LDS;
+            const double* dq = kf ? eng->d_dq : nullptr;
+            const double* dr = kf ? eng->d_dr : nullptr;
+            const unsigned gp = persistent_grid(eng, 1);
+            if (kf)
+            {
+                k_gemm_int8w_group<true, false><<<g_tiles, 512, lds, eng->stream>>>(
+                    eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
+                    eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                    (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, dq, dr, ntiles, 0u);
+                k_gemm_int8w_group<true, true><<<gp, 512, lds, eng->stream>>>(
+                    eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
+                    eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                    (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, dq, dr, ntiles,
+                    g_tiles / 8);
+            }
+            else
+            {
+                k_gemm_int8w_group<false, false><<<g_tiles, 512, lds, eng->stream>>>(
+                    eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
+                    eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                    (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, dq, dr, ntiles, 0u);
+                k_gemm_int8w_group<false, true><<<gp, 512, lds, eng->stream>>>(
+                    eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
+                    eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
+                    (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, dq, dr, ntiles,
+                    g_tiles / 8);
+            }
+        }
         else if (x3)
-            k_gemm_int8x_group<<<(unsigned)tiles.size(), 512, X3_LDS, eng->stream>>>(
+            k_gemm_int8x_group<<<g_tiles, 512, X3_LDS, eng->stream>>>(
                 eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
                 eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S);
+                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, ntiles);
         else
-            k_gemm_int8_group<<<(unsigned)tiles.size(), 256, 0, eng->stream>>>(
+            k_gemm_int8_group<<<std::max(g_tiles, persistent_grid(eng, 4)), 256, 0, eng->stream>>>(
                 eng->d_ptiles, eng->d_pcells, eng->d_pabase, eng->d_pacnt, eng->d_lstart,
                 eng->d_llen, eng->d_pcoff, eng->d_qh, eng->d_ql, eng->d_rh, eng->d_rl,
-                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S);
+                (int)eng->Kp, b_max, eng->d_irq, eng->d_irr, eng->d_S, ntiles);
         LAUNCHCHK();
     }
     {
         PhaseScope ps(eng, 2, 2);
         HIPCHK(hipMemsetAsync(eng->d_nan_count, 0, sizeof(unsigned int), eng->stream));
         // NaN quantiles stay NaN: the certification re-scores those cells
-        k_select<TableSegments><<<(unsigned)((n_items + 3) / 4), 256, 0, eng->stream>>>(
-            TableSegments{ eng->d_pseg }, n_items, eng->d_S, eng->d_pct, quantile, eng->d_Pref,
+        k_select_n<TableSegments><<<g_items, 256, 0, eng->stream>>>(
+            TableSegments{ eng->d_pseg }, nitems, eng->d_S, eng->d_pct, quantile, eng->d_Pref,
             eng->d_nan_list, eng->d_nan_count);
         LAUNCHCHK();
-        k_prop_scatter_p<<<nblk(n_items), 256, 0, eng->stream>>>(eng->d_pitems, (uint32_t)n_items,
-                                                                eng->d_Pref, P, L);
+        k_prop_scatter_p<<<est_grid(eng, eng->est_items[round], 256, 4), 256, 0, eng->stream>>>(
+            eng->d_pitems, nitems, eng->d_Pref, P, L);
         LAUNCHCHK();
     }
-    eng->counters[19] += (int64_t)n_items;
-    eng->counters[21] += (int64_t)n_scores;
-    return CSG_OK;
-}
-
-int read_counts(csg_engine* eng, const uint32_t* d, std::vector<uint32_t>& h)
-{
-    HIPCHK(hipMemcpyAsync(h.data(), d, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                          eng->stream));
-    HIPCHK(hipStreamSynchronize(eng->stream));
     return CSG_OK;
 }
 
@@ -1577,68 +1609,60 @@ int propose_chunk(csg_engine* eng, uint32_t q0, uint32_t nq, float quantile, flo
                                                        eng->d_pfb);
         LAUNCHCHK();
     }
-    std::vector<uint32_t> cnt(L + 1);
-    if ((rc = read_counts(eng, cA, cnt)))
-        return rc;
-    const uint32_t nfb = cnt[L];
-    float*         P   = eng->d_P + (size_t)q0 * L;
+    float* P = eng->d_P + (size_t)q0 * L;
     k_fill_f32<<<nblk((size_t)nq * L), 256, 0, eng->stream>>>(P, (size_t)nq * L, -__builtin_inff());
     LAUNCHCHK();
+    // stage 1: the confident cells on their first K labels, the cells below
+    // the margin (cA[L] of them, listed in d_pfb) on every label
     PropRound r1;
-    r1.cw.assign(cnt.begin(), cnt.begin() + L);
+    r1.cw    = cA;
     r1.n_win = nq;
     r1.t1    = K;
     r1.all   = eng->d_pfb;
-    r1.n_all = nfb;
+    r1.n_all = cA + L;
     r1.base  = cA;
-    if ((rc = prop_round(eng, r1, quantile, P)))
+    if ((rc = prop_round(eng, r1, 0, nq, quantile, P)))
         return rc;
-    // stage 2: widen the edge cells to their next ranked labels
+    // stage 2: widen the edge cells (cB[L], listed in d_plist) to their next
+    // ranked labels
     const bool widen = KP > K;
-    k_prop_check<<<nblk(nq), 256, 0, eng->stream>>>(P, nq, L, eng->d_ptop, eng->d_pmargin, KP,
-                                                   K - 1, eng->prop_margin, thre, wp, nullptr,
-                                                   widen ? 1 : 0, cB, eng->d_pslot, eng->d_plist,
-                                                   cB + L);
+    k_prop_check<<<nblk(nq), 256, 0, eng->stream>>>(P, nq, nullptr, L, eng->d_ptop,
+                                                   eng->d_pmargin, KP, K - 1, eng->prop_margin,
+                                                   thre, wp, nullptr, widen ? 1 : 0, cB,
+                                                   eng->d_pslot, eng->d_plist, cB + L);
     LAUNCHCHK();
-    if ((rc = read_counts(eng, cB, cnt)))
-        return rc;
-    const uint32_t n1 = cnt[L];
-    uint32_t       n_full = n1;
-    const uint32_t* full  = eng->d_plist;
-    if (n1 && widen)
+    const uint32_t* full   = eng->d_plist;
+    const uint32_t* n_full = cB + L;
+    if (widen)
     {
         PropRound r2;
-        r2.cw.assign(cnt.begin(), cnt.begin() + L);
-        r2.win   = eng->d_plist;
-        r2.n_win = n1;
-        r2.t0    = K;
-        r2.t1    = KP;
-        if ((rc = prop_round(eng, r2, quantile, P)))
+        r2.cw        = cB;
+        r2.win       = eng->d_plist;
+        r2.n_win     = nq;
+        r2.n_win_dev = cB + L;
+        r2.t0        = K;
+        r2.t1        = KP;
+        if ((rc = prop_round(eng, r2, 1, nq, quantile, P)))
             return rc;
-        k_prop_check<<<nblk(n1), 256, 0, eng->stream>>>(P, n1, L, eng->d_ptop, eng->d_pmargin, KP,
-                                                       KP - 1, eng->prop_margin, thre, wp,
-                                                       eng->d_plist, 0, nullptr, nullptr,
-                                                       eng->d_plist + eng->Qc, cC + L);
+        k_prop_check<<<nblk(nq), 256, 0, eng->stream>>>(P, nq, cB + L, L, eng->d_ptop,
+                                                       eng->d_pmargin, KP, KP - 1,
+                                                       eng->prop_margin, thre, wp, eng->d_plist, 0,
+                                                       nullptr, nullptr, eng->d_plist + eng->Qc,
+                                                       cC + L);
         LAUNCHCHK();
-        std::vector<uint32_t> n2(1);
-        if ((rc = read_counts(eng, cC + L, n2)))
-            return rc;
-        n_full = n2[0];
         full   = eng->d_plist + eng->Qc;
-        eng->counters[20] += n1;
+        n_full = cC + L;
     }
-    if (n_full)
-    {
-        PropRound r3;
-        r3.cw.assign(L, 0u);
-        r3.all   = full;
-        r3.n_all = n_full;
-        if ((rc = prop_round(eng, r3, quantile, P)))
-            return rc;
-    }
-    eng->counters[16] += (int64_t)(nq - nfb - n_full);
-    eng->counters[17] += nfb;
-    eng->counters[18] += n_full;
+    // stage 3: the cells whose last ranked label is still at the edge, on
+    // every label
+    PropRound r3;
+    r3.all   = full;
+    r3.n_all = n_full;
+    if ((rc = prop_round(eng, r3, 2, nq, quantile, P)))
+        return rc;
+    k_prop_tally<<<1, 64, 0, eng->stream>>>(nq, cA + L, cB + L, n_full, widen ? 1 : 0,
+                                            eng->d_dcnt);
+    LAUNCHCHK();
     return CSG_OK;
 }
 
@@ -1650,8 +1674,14 @@ int alloc_proposal(csg_engine* eng)
         (rc = dalloc(eng, &eng->d_pcnt, 3 * (L + 1))) || (rc = dalloc(eng, &eng->d_pslot, qc * 8)) ||
         (rc = dalloc(eng, &eng->d_pfb, qc)) || (rc = dalloc(eng, &eng->d_plist, 2 * qc)) || (rc = dalloc(eng, &eng->d_pcells, qc * L)) ||
         (rc = dalloc(eng, &eng->d_pitems, qc * L)) || (rc = dalloc(eng, &eng->d_pseg, qc * L)) ||
-        (rc = ensure(eng, &eng->d_pmeta, &eng->pmeta_cap, (size_t)16 * L + 16)))
+        (rc = ensure(eng, &eng->d_pmeta, &eng->pmeta_cap,
+                     (size_t)16 * L + 48 + prop_tile_bound(eng) * sizeof(GroupTile))))
         return rc;
+    eng->d_pcoff  = (uint64_t*)eng->d_pmeta;
+    eng->d_pabase = (uint32_t*)(eng->d_pmeta + (size_t)8 * L);
+    eng->d_pacnt  = eng->d_pabase + L;
+    eng->d_pplan  = (uint32_t*)(eng->d_dcnt + 36);
+    eng->d_ptiles = (GroupTile*)(eng->d_pmeta + (size_t)16 * L + 48);
     return CSG_OK;
 }
 
@@ -1668,13 +1698,9 @@ void free_proposal(csg_engine* eng)
     dfree(eng->d_pseg);
     dfree(eng->d_pmeta);
     eng->pmeta_cap = 0;
-    eng->d_pabase = eng->d_pacnt = nullptr;
+    eng->d_pabase = eng->d_pacnt = eng->d_pplan = nullptr;
     eng->d_pcoff  = nullptr;
     eng->d_ptiles = nullptr;
-    if (eng->h_pmeta)
-        (void)hipHostFree(eng->h_pmeta);
-    eng->h_pmeta     = nullptr;
-    eng->h_pmeta_cap = 0;
 }
 
 }  // namespace
@@ -1706,6 +1732,7 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     if (prop && !eng->d_pcells && (rc = alloc_proposal(eng)))
         return rc;
     std::memset(eng->counters, 0, sizeof(eng->counters));
+    HIPCHK(hipMemsetAsync(eng->d_dcnt, 0, 48 * sizeof(unsigned long long), eng->stream));
     eng->ft_exact    = path == CSG_PATH_EXACT_INT8;
     eng->counters[0] = eng->Nq;
     eng->counters[1] = eng->G;
@@ -1822,8 +1849,36 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
     if (scores)
         HIPCHK(hipMemcpyAsync(scores, eng->d_P, nql * sizeof(float), hipMemcpyDeviceToHost,
                               eng->stream));
+    // the counters the kernels kept on the device, and the work counts that
+    // size the next step's grids, in one copy with the closing read-back
+    if (path == CSG_PATH_EXACT_INT8)
+    {
+        k_gather_u32<<<1, 64, 0, eng->stream>>>(eng->d_nitems, eng->d_rc_abase + 4 * eng->L,
+                                                (uint32_t*)(eng->d_dcnt + 42));
+        LAUNCHCHK();
+    }
+    HIPCHK(hipMemcpyAsync(eng->h_dcnt, eng->d_dcnt, sizeof(eng->h_dcnt), hipMemcpyDeviceToHost,
+                          eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
     collect_phases(eng);
+    const uint32_t* plan = (const uint32_t*)(eng->h_dcnt + 36);
+    if (prop)
+        for (int r = 0; r < 3; ++r)
+        {
+            eng->est_tiles[r] = plan[4 * r];
+            eng->est_items[r] = plan[4 * r + 1];
+        }
+    if (path == CSG_PATH_EXACT_INT8)
+    {
+        eng->est_pairs    = plan[12];
+        eng->est_rc_tiles = plan[13];
+    }
+    for (int i = 0; i < CSG_N_COUNTERS; ++i)
+        eng->counters[i] += (int64_t)eng->h_dcnt[i];
+    if ((int)(eng->h_dcnt[32] & 0xFFFFFFFFull))
+        return eng->fail(CSG_ERR_UNDEFINED,
+                         "a NaN-carrying score segment makes the reference's std::sort read out "
+                         "of bounds; its percentile is undefined");
     eng->mask_valid = true;
     return CSG_OK;
 }

@@ -178,11 +178,12 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v)
 }
 
 // Pairs per label.
-__global__ void k_rc_count(const RecheckItem* __restrict__ items, uint32_t n,
-                           uint32_t* __restrict__ cnt)
+// (Grid-stride over the *n_dev items the certification listed.)
+__global__ void k_rc_count(const RecheckItem* __restrict__ items,
+                           const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ cnt)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
+    const uint32_t n = *n_dev;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         atomicAdd(&cnt[items[i].l], 1u);
 }
 
@@ -215,22 +216,24 @@ __global__ void k_rc_scan(const uint32_t* __restrict__ cnt, const uint32_t* __re
 // pair in label order and its score segment (contiguous within the label's
 // block).  Places within a label follow atomics; every pair's values do not
 // depend on its place.
-__global__ void k_rc_scatter(const RecheckItem* __restrict__ items, uint32_t n,
+__global__ void k_rc_scatter(const RecheckItem* __restrict__ items,
+                             const uint32_t* __restrict__ n_dev,
                              const uint32_t* __restrict__ abase, const uint64_t* __restrict__ coff,
                              const uint32_t* __restrict__ llen, uint32_t* __restrict__ fill,
                              uint32_t* __restrict__ cells, RecheckItem* __restrict__ sorted,
                              Segment* __restrict__ segs)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const RecheckItem it  = items[i];
-    const uint32_t    len = llen[it.l];
-    const uint32_t    p   = atomicAdd(&fill[it.l], 1u);
-    const uint32_t    pos = abase[it.l] + p;
-    cells[pos]            = it.q;
-    sorted[pos]           = it;
-    segs[pos]             = Segment{ coff[it.l] + (uint64_t)p * len, len, it.l };
+    const uint32_t n = *n_dev;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const RecheckItem it  = items[i];
+        const uint32_t    len = llen[it.l];
+        const uint32_t    p   = atomicAdd(&fill[it.l], 1u);
+        const uint32_t    pos = abase[it.l] + p;
+        cells[pos]            = it.q;
+        sorted[pos]           = it;
+        segs[pos]             = Segment{ coff[it.l] + (uint64_t)p * len, len, it.l };
+    }
 }
 
 // Tile list of the grouped contraction, one thread per label.
@@ -258,16 +261,43 @@ __global__ void k_rc_cells(const uint8_t* __restrict__ status, uint32_t nq,
 
 // Writes the re-scored quantiles into P and redoes the decision of every cell
 // that needed them (status 1 or 2).
-__global__ void k_apply_recheck(const RecheckItem* __restrict__ items, unsigned int n,
+__global__ void k_apply_recheck(const RecheckItem* __restrict__ items,
+                                const uint32_t* __restrict__ n_dev,
                                 const float* __restrict__ Pref, float* __restrict__ P,
                                 uint32_t L, uint8_t* __restrict__ exact)
 {
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const RecheckItem it                = items[i];
-    P[(size_t)it.q * L + it.l]          = Pref[i];
-    exact[(size_t)it.q * L + it.l]      = 1;
+    const uint32_t n = *n_dev;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const RecheckItem it           = items[i];
+        P[(size_t)it.q * L + it.l]     = Pref[i];
+        exact[(size_t)it.q * L + it.l] = 1;
+    }
+}
+
+// out[0] = *a, out[1] = *b (two device counts gathered for one read-back).
+__global__ void k_gather_u32(const unsigned int* __restrict__ a, const unsigned int* __restrict__ b,
+                             uint32_t* __restrict__ out)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        out[0] = *a;
+        out[1] = *b;
+    }
+}
+
+// The certification's tallies of a chunk, added to the device counters:
+// [3] re-scored pairs, [4] uncertified cells, [5] NaN-carrying segments.
+__global__ void k_cert_tally(const unsigned int* __restrict__ nitems,
+                             const unsigned int* __restrict__ nan_count,
+                             unsigned long long* __restrict__ dcnt)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        dcnt[3] += nitems[0];
+        dcnt[4] += nitems[1];
+        dcnt[5] += *nan_count;
+    }
 }
 
 __global__ void k_label_final(const float* __restrict__ P, const uint8_t* __restrict__ status,

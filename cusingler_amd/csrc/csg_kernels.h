@@ -589,20 +589,25 @@ __global__ __launch_bounds__(256) void k_gemm_plain_group(
     const int16_t* __restrict__ Xr, int Kp, uint32_t b_max, const float* __restrict__ sq_q,
     const float* __restrict__ sq_r, float* __restrict__ S, const uint32_t* __restrict__ ntiles)
 {
-    if (blockIdx.x >= ntiles[0])
-        return;  // the grid is a bound on the device-built tile count
-    const GroupTile t  = tiles[blockIdx.x];
-    const uint32_t  nl = llen[t.label];
-    const GroupPMap map{ cells + a_base[t.label] + t.m0,
-                         (int)(a_cnt[t.label] - t.m0),
-                         (int)(nl - t.n0),
-                         lstart[t.label] + t.n0,
-                         b_max,
-                         sq_q,
-                         sq_r,
-                         S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
-                         nl };
-    plain_block<GroupPMap, PG_GBM>(Xq, Xr, Kp, map);
+    // any grid over the device-built tile list (one tile per block when the
+    // grid is large enough)
+    const uint32_t T = ntiles[0];
+    for (uint32_t k = blockIdx.x; k < T; k += gridDim.x)
+    {
+        const GroupTile t  = tiles[k];
+        const uint32_t  nl = llen[t.label];
+        const GroupPMap map{ cells + a_base[t.label] + t.m0,
+                             (int)(a_cnt[t.label] - t.m0),
+                             (int)(nl - t.n0),
+                             lstart[t.label] + t.n0,
+                             b_max,
+                             sq_q,
+                             sq_r,
+                             S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
+                             nl };
+        plain_block<GroupPMap, PG_GBM>(Xq, Xr, Kp, map);
+        __syncthreads();  // the next tile reuses the LDS buffers
+    }
 }
 
 // ------------------------------------------------ per-label quantile select ---
@@ -1126,6 +1131,61 @@ __global__ __launch_bounds__(256, 8) void k_select(Src src, uint64_t n_items,
     if (item >= n_items)
         return;
     select_item(src, item, scores, params, p, out, nan_list, nan_count, sm[wid], lane);
+}
+
+// The same over *n_dev items counted on the device: a grid-stride loop, one
+// wave per item at a time.
+template <class Src>
+__global__ __launch_bounds__(256, 8) void k_select_n(Src src, const uint32_t* __restrict__ n_dev,
+                                                  const float* __restrict__ scores,
+                                                  const PctParam* __restrict__ params, float p,
+                                                  float* __restrict__ out,
+                                                  uint64_t* __restrict__ nan_list,
+                                                  unsigned int* __restrict__ nan_count)
+{
+    __shared__ WaveSelectSmem sm[256 / kWave];
+    // (the wave index is uniform: in scalar registers, so is the item)
+    const int      wid  = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int      lane = threadIdx.x & (kWave - 1);
+    const uint32_t n    = *n_dev;
+    for (uint32_t item = blockIdx.x * (256 / kWave) + wid; item < n;
+         item += gridDim.x * (256 / kWave))
+        select_item(src, item, scores, params, p, out, nan_list, nan_count, sm[wid], lane);
+}
+
+// NaN-carrying segments listed by k_select / k_select_n, sorted in place (the
+// segments' scores are not read again), each by one lane with the libstdc++
+// introsort emulation; the reference's positions are read after it.  A
+// segment whose sort would read out of bounds in the reference sets *status.
+template <class Src>
+__global__ void k_select_nan_inplace(Src src, const uint64_t* __restrict__ nan_list,
+                                     const unsigned int* __restrict__ count,
+                                     float* __restrict__ scores,
+                                     const PctParam* __restrict__ params, float p,
+                                     float* __restrict__ out, int* __restrict__ status)
+{
+    const unsigned int n = *count;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint64_t item = nan_list[i];
+        const Segment  sg   = src.get(item);
+        const PctParam pp   = params[sg.param];
+        float*         a    = scores + sg.off;
+        if (!std_sort(a, (long)sg.n))
+        {
+            atomicExch(status, 1);
+            out[item] = __builtin_nanf("");
+            continue;
+        }
+        float r;
+        if (pp.kind == 1)
+            r = a[0];
+        else if (pp.kind == 2)
+            r = a[sg.n - 1];
+        else
+            r = pct_interp(p, pp.prev_index, pp.step, a[pp.k - 1], a[pp.k]);
+        out[item] = r;
+    }
 }
 
 // NaN-carrying segments: one lane copies the segment to `work` and runs the

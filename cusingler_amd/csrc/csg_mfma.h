@@ -365,6 +365,25 @@ __device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t n)
     return x * q + min(x, r) + local;
 }
 
+// Grids over a tile list built on the device (`T` tiles, not known when the
+// grid is sized): the blocks of XCD x (block b runs on XCD b % 8; the grid
+// is a multiple of 8) take the contiguous range [x T / 8, (x + 1) T / 8) in
+// dispatch order, so tiles that share operands (neighbours in the list)
+// share one XCD's L2.  A grid of at least T blocks gives each block one
+// tile (the blocks past the range return), a smaller one loops.
+struct XcdRange
+{
+    uint32_t k, end, step;
+    // skip: the range's first tiles that an earlier launch took
+    __device__ XcdRange(uint32_t T, uint32_t skip = 0)
+    {
+        const uint32_t x = blockIdx.x % 8;
+        k                = (uint32_t)((uint64_t)T * x / 8) + skip + blockIdx.x / 8;
+        end              = (uint32_t)((uint64_t)T * (x + 1) / 8);
+        step             = gridDim.x / 8;
+    }
+};
+
 // Row / column maps of one 128 x 256 block of the wide kernel.  a_row(r) and
 // b_row(r) give the operand rows staged for block rows / columns r (always
 // inside the operand buffers); row_ok / col_ok say which results are stored;
@@ -418,7 +437,7 @@ struct GroupW8Map
 __device__ __forceinline__ void w8_issue(const int8_t* __restrict__ qh,
                                          const int8_t* __restrict__ ql,
                                          const int8_t* __restrict__ rh,
-                                         const int8_t* __restrict__ rl, const size_t (&src)[6],
+                                         const int8_t* __restrict__ rl, const uint32_t (&src)[6],
                                          int8_t* stage, int wid, int k0)
 {
 #pragma unroll
@@ -427,8 +446,8 @@ __device__ __forceinline__ void w8_issue(const int8_t* __restrict__ qh,
         const int     g    = wid + 8 * u;
         const int8_t* base = g < 8 ? qh : g < 16 ? ql : g < 32 ? rh : rl;
         __builtin_amdgcn_global_load_lds(
-            base + src[u] + k0, (__attribute__((address_space(3))) void*)(stage + g * 1024), 16,
-            0, 0);
+            base + (size_t)src[u] + k0, (__attribute__((address_space(3))) void*)(stage + g * 1024),
+            16, 0, 0);
     }
 }
 
@@ -448,17 +467,18 @@ __device__ inline void gemm_int8w_block(const int8_t* __restrict__ qh,
     const int wm   = wid >> 2;  // 0..1: 64-row half of the block
     const int wn   = wid & 3;   // 0..3: 64-column quarter
 
-    // A rows: both planes take row rq; B rows: rq and rq + 128
+    // A rows: both planes take row rq; B rows: rq and rq + 128 (element
+    // offsets below 2^32: a plane holds fewer than 2^32 bytes)
     const int    rq   = wid * 16 + (lane >> 2);
     const size_t arow = map.a_row(rq);
-    size_t       src[6];
+    uint32_t     src[6];
 #pragma unroll
     for (int u = 0; u < 6; ++u)
     {
         const int g = wid + 8 * u;
         const int r = (g < 16 ? (g & 7) : ((g - 16) & 15)) * 16 + (lane >> 2);  // row in its plane
-        src[u]      = (g < 16 ? arow : map.b_row(r)) * (size_t)Kp +
-                 (size_t)(((lane & 3) ^ w8_swz(r)) * 16);
+        src[u]      = (uint32_t)((g < 16 ? arow : map.b_row(r)) * (size_t)Kp) +
+                 (uint32_t)(((lane & 3) ^ w8_swz(r)) * 16);
     }
     v4i acc_x[4][4], acc_ll[4][4];
 #pragma unroll
@@ -656,13 +676,19 @@ __global__ __launch_bounds__(512) void k_gemm_int8w(
     gemm_int8w_block<KF>(qh, ql, rh, rl, Kp, map, w8_sm);
 }
 
-// Grouped contraction: block b takes tile tiles[xcd_order(b)] = (label l, m0,
-// n0), the cells cells[a_base[l] + m0 ..] (a_cnt[l] in all) against reference
-// rows lstart[l] + n0 .. of label l (llen[l] in all); each cell's scores
-// against the label's rows are stored contiguously, label l's block starting
-// at S + c_off[l].  Tiles are listed label by label, so a label's tiles run
-// on one XCD.
-template <bool KF>
+// Grouped contraction over the *ntiles tiles of a device-built list
+// (XcdRange): without LOOP each block takes at most one tile, the `skip`
+// first of each XCD range being another launch's; with LOOP the blocks loop
+// over the rest of each range (a loop over tiles costs the tile body
+// registers, so the one-tile form, with a grid sized to the expected tile
+// count, does the work and the looping form takes what outgrew that grid).
+// Tile (label l, m0, n0) takes the cells
+// cells[a_base[l] + m0 ..] (a_cnt[l] in all) against reference rows
+// lstart[l] + n0 .. of label l (llen[l] in all); each cell's scores against
+// the label's rows are stored contiguously, label l's block starting at
+// S + c_off[l].  Tiles are listed label by label, so a label's tiles run on
+// one XCD.
+template <bool KF, bool LOOP>
 __global__ __launch_bounds__(512) void k_gemm_int8w_group(
     const GroupTile* __restrict__ tiles, const uint32_t* __restrict__ cells,
     const uint32_t* __restrict__ a_base, const uint32_t* __restrict__ a_cnt,
@@ -670,23 +696,30 @@ __global__ __launch_bounds__(512) void k_gemm_int8w_group(
     const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
     const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
     int Kp, uint32_t b_max, const double* __restrict__ ir_q, const double* __restrict__ ir_r,
-    float* __restrict__ S, const double* __restrict__ d_q, const double* __restrict__ d_r)
+    float* __restrict__ S, const double* __restrict__ d_q, const double* __restrict__ d_r,
+    const uint32_t* __restrict__ ntiles, uint32_t skip)
 {
     extern __shared__ __attribute__((aligned(16))) int8_t w8_sm[];
-    const GroupTile t  = tiles[xcd_order(blockIdx.x, gridDim.x)];
-    const uint32_t  nl = llen[t.label];
-    const GroupW8Map map{ cells + a_base[t.label] + t.m0,
-                          (int)(a_cnt[t.label] - t.m0),
-                          (int)(nl - t.n0),
-                          lstart[t.label] + t.n0,
-                          b_max,
-                          ir_q,
-                          ir_r,
-                          S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
-                          nl,
-                          d_q,
-                          d_r };
-    gemm_int8w_block<KF>(qh, ql, rh, rl, Kp, map, w8_sm);
+    for (XcdRange r(*ntiles, skip); r.k < r.end; r.k += r.step)
+    {
+        const GroupTile t  = tiles[r.k];
+        const uint32_t  nl = llen[t.label];
+        const GroupW8Map map{ cells + a_base[t.label] + t.m0,
+                              (int)(a_cnt[t.label] - t.m0),
+                              (int)(nl - t.n0),
+                              lstart[t.label] + t.n0,
+                              b_max,
+                              ir_q,
+                              ir_r,
+                              S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
+                              nl,
+                              d_q,
+                              d_r };
+        gemm_int8w_block<KF>(qh, ql, rh, rl, Kp, map, w8_sm);
+        if constexpr (!LOOP)
+            break;
+        __syncthreads();  // the next tile's loads overwrite the stages
+    }
 }
 
 // ------------------------------------------------ three-partial wide tile ---
@@ -880,27 +913,31 @@ __global__ __launch_bounds__(512) void k_gemm_int8x_group(
     const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
     const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
     int Kp, uint32_t b_max, const double* __restrict__ ir_q, const double* __restrict__ ir_r,
-    float* __restrict__ S)
+    float* __restrict__ S, const uint32_t* __restrict__ ntiles)
 {
     extern __shared__ __attribute__((aligned(16))) int8_t x3_sm[];
-    const GroupTile t  = tiles[xcd_order(blockIdx.x, gridDim.x)];
-    const uint32_t  nl = llen[t.label];
-    const GroupW8Map map{ cells + a_base[t.label] + t.m0,
-                          (int)(a_cnt[t.label] - t.m0),
-                          (int)(nl - t.n0),
-                          lstart[t.label] + t.n0,
-                          b_max,
-                          ir_q,
-                          ir_r,
-                          S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
-                          nl };
-    gemm_int8x_block(qh, ql, rh, rl, Kp, map, x3_sm);
+    for (XcdRange r(*ntiles); r.k < r.end; r.k += r.step)
+    {
+        const GroupTile t  = tiles[r.k];
+        const uint32_t  nl = llen[t.label];
+        const GroupW8Map map{ cells + a_base[t.label] + t.m0,
+                              (int)(a_cnt[t.label] - t.m0),
+                              (int)(nl - t.n0),
+                              lstart[t.label] + t.n0,
+                              b_max,
+                              ir_q,
+                              ir_r,
+                              S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
+                              nl };
+        gemm_int8x_block(qh, ql, rh, rl, Kp, map, x3_sm);
+        __syncthreads();  // the next tile's loads overwrite the stages
+    }
 }
 
 // The grouped contraction in 64 x 128 blocks with three partials, for any
-// score gene set (the wide kernel's folded limbs need |X| <= 1919): tile t =
-// tiles[t] = (label l, m0, n0) with m0 a multiple of 64 and n0 of 128, the
-// same tables and output layout as k_gemm_int8w_group.
+// score gene set (the wide kernel's folded limbs need |X| <= 1919): tiles
+// (label l, m0, n0) with m0 a multiple of 64 and n0 of 128, the same tables,
+// output layout and grid as k_gemm_int8w_group.
 __global__ __launch_bounds__(256) void k_gemm_int8_group(
     const GroupTile* __restrict__ tiles, const uint32_t* __restrict__ cells,
     const uint32_t* __restrict__ a_base, const uint32_t* __restrict__ a_cnt,
@@ -908,20 +945,24 @@ __global__ __launch_bounds__(256) void k_gemm_int8_group(
     const uint64_t* __restrict__ c_off, const int8_t* __restrict__ qh,
     const int8_t* __restrict__ ql, const int8_t* __restrict__ rh, const int8_t* __restrict__ rl,
     int Kp, uint32_t b_max, const double* __restrict__ ir_q, const double* __restrict__ ir_r,
-    float* __restrict__ S)
+    float* __restrict__ S, const uint32_t* __restrict__ ntiles)
 {
-    const GroupTile t  = tiles[xcd_order(blockIdx.x, gridDim.x)];
-    const uint32_t  nl = llen[t.label];
-    const GroupI8Map map{ cells + a_base[t.label] + t.m0,
-                          (int)(a_cnt[t.label] - t.m0),
-                          (int)(nl - t.n0),
-                          lstart[t.label] + t.n0,
-                          b_max,
-                          ir_q,
-                          ir_r,
-                          S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
-                          nl };
-    gemm_int8_block(qh, ql, rh, rl, Kp, map);
+    for (XcdRange r(*ntiles); r.k < r.end; r.k += r.step)
+    {
+        const GroupTile t  = tiles[r.k];
+        const uint32_t  nl = llen[t.label];
+        const GroupI8Map map{ cells + a_base[t.label] + t.m0,
+                              (int)(a_cnt[t.label] - t.m0),
+                              (int)(nl - t.n0),
+                              lstart[t.label] + t.n0,
+                              b_max,
+                              ir_q,
+                              ir_r,
+                              S + c_off[t.label] + (size_t)t.m0 * nl + t.n0,
+                              nl };
+        gemm_int8_block(qh, ql, rh, rl, Kp, map);
+        __syncthreads();  // the next tile reuses the LDS buffers
+    }
 }
 
 // ------------------------------------- approximate MFMA contractions ---
