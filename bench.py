@@ -403,7 +403,12 @@ def main():
                 "launches": corr_n, "avg_launch_ms": round(corr_ms / max(corr_n, 1), 4),
                 "ops_per_launch": ops_desc}
     if chosen in ("exact_int8", "proposal"):
-        roofline["limb_mfma_utilisation"] = round(4 * achieved / peak, 5)
+        # INT8 limb products per logical multiply-add: three in the
+        # three-product form (score gene sets of up to 1,008 genes), four in
+        # the folded form
+        products = 3 if Gs <= 1008 else 4
+        roofline["limb_products"] = products
+        roofline["limb_mfma_utilisation"] = round(products * achieved / peak, 5)
     workload = (f"{args.config}: {Nq} query x {Nr} reference cells, {len(ds.codes)} labels, "
                 f"G_s={Gs}, G_f={c1.width}")
     # HBM bytes per launch from the committed counter runs (FETCH_SIZE and
