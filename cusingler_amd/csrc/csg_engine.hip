@@ -155,7 +155,11 @@ struct csg_engine
     // as uint32), [42] the certification's pairs and tiles (uint32); read
     // back in one copy with the step's closing synchronisation
     unsigned long long* d_dcnt = nullptr;
-    unsigned long long  h_dcnt[48] = {};
+    unsigned long long* h_dcnt = nullptr;  // pinned, 48 entries
+    // csg_get_label without outputs leaves its read-back in flight: folded
+    // (counters, grid estimates, phase events, the NaN-sort flag) by the next
+    // call that synchronises (fold_score_step)
+    bool dcnt_pending = false, dcnt_prop = false, dcnt_exact = false;
     // exact / approximate MFMA paths
     int8_t *  d_rh = nullptr, *d_rl = nullptr, *d_qh = nullptr, *d_ql = nullptr;
     double *  d_dr = nullptr, *d_dq = nullptr;  // rows' corrections, three-product form
@@ -751,7 +755,9 @@ int csg_init_gpu(int gpuid, csg_engine** out)
         eng->ftd_grid_select  = cu * 8;
     }
     if (dalloc(eng, &eng->d_nan_count, 1) || dalloc(eng, &eng->d_status, 1) ||
-        dalloc(eng, &eng->d_dcnt, 48))
+        dalloc(eng, &eng->d_dcnt, 48) ||
+        hipHostMalloc((void**)&eng->h_dcnt, 48 * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess)
     {
         delete eng;
         return CSG_ERR_MEMORY;
@@ -902,6 +908,9 @@ void csg_release(csg_engine* eng)
     dfree(eng->d_mask);
     dfree(eng->d_nan_count);
     dfree(eng->d_dcnt);
+    if (eng->h_dcnt)
+        (void)hipHostFree(eng->h_dcnt);
+    eng->h_dcnt = nullptr;
     dfree(eng->d_status);
     collect_phases(eng);
     for (auto e : eng->event_pool)
@@ -1703,6 +1712,40 @@ void free_proposal(csg_engine* eng)
     eng->d_ptiles = nullptr;
 }
 
+// The score step's read-back, once the stream has reached it: the counters the
+// kernels kept on the device, the work counts that size the next step's
+// grids, the phase events, and the undefined-sort flag of its NaN segments.
+int fold_score_step(csg_engine* eng)
+{
+    if (!eng->dcnt_pending)
+        return CSG_OK;
+    HIPCHK(hipStreamSynchronize(eng->stream));
+    eng->dcnt_pending = false;
+    collect_phases(eng);
+    for (int i = 0; i < CSG_N_COUNTERS; ++i)
+        eng->counters[i] += (int64_t)eng->h_dcnt[i];
+    const uint32_t* plan = (const uint32_t*)(eng->h_dcnt + 36);
+    if (eng->dcnt_prop)
+        for (int r = 0; r < 3; ++r)
+        {
+            eng->est_tiles[r] = plan[4 * r];
+            eng->est_items[r] = plan[4 * r + 1];
+        }
+    if (eng->dcnt_exact)
+    {
+        eng->est_pairs    = plan[12];
+        eng->est_rc_tiles = plan[13];
+    }
+    if ((int)(eng->h_dcnt[32] & 0xFFFFFFFFull))
+    {
+        eng->mask_valid = false;
+        return eng->fail(CSG_ERR_UNDEFINED,
+                         "a NaN-carrying score segment makes the reference's std::sort read out "
+                         "of bounds; its percentile is undefined");
+    }
+    return CSG_OK;
+}
+
 }  // namespace
 
 int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path,
@@ -1722,6 +1765,8 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
         return eng->fail(CSG_ERR_ARGUMENT, "the learned proposal runs on the exact INT8 path");
     HIPCHK(hipSetDevice(eng->device));
     int rc;
+    if ((rc = fold_score_step(eng)))  // a previous step's deferred read-back
+        return rc;
     if (prop && (rc = mlp_check(eng)))
         return rc;
     if (prop && ((uint32_t)eng->prop_k > eng->L || eng->L > 256))
@@ -1857,29 +1902,17 @@ int csg_get_label(csg_engine* eng, float quantile, float finetune_thre, int path
                                                 (uint32_t*)(eng->d_dcnt + 42));
         LAUNCHCHK();
     }
-    HIPCHK(hipMemcpyAsync(eng->h_dcnt, eng->d_dcnt, sizeof(eng->h_dcnt), hipMemcpyDeviceToHost,
-                          eng->stream));
-    HIPCHK(hipStreamSynchronize(eng->stream));
-    collect_phases(eng);
-    const uint32_t* plan = (const uint32_t*)(eng->h_dcnt + 36);
-    if (prop)
-        for (int r = 0; r < 3; ++r)
-        {
-            eng->est_tiles[r] = plan[4 * r];
-            eng->est_items[r] = plan[4 * r + 1];
-        }
-    if (path == CSG_PATH_EXACT_INT8)
-    {
-        eng->est_pairs    = plan[12];
-        eng->est_rc_tiles = plan[13];
-    }
-    for (int i = 0; i < CSG_N_COUNTERS; ++i)
-        eng->counters[i] += (int64_t)eng->h_dcnt[i];
-    if ((int)(eng->h_dcnt[32] & 0xFFFFFFFFull))
-        return eng->fail(CSG_ERR_UNDEFINED,
-                         "a NaN-carrying score segment makes the reference's std::sort read out "
-                         "of bounds; its percentile is undefined");
-    eng->mask_valid = true;
+    HIPCHK(hipMemcpyAsync(eng->h_dcnt, eng->d_dcnt, 48 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, eng->stream));
+    eng->dcnt_pending = true;
+    eng->dcnt_prop    = prop;
+    eng->dcnt_exact   = path == CSG_PATH_EXACT_INT8;
+    eng->mask_valid   = true;
+    // with outputs the call returns them synchronised; without, the stream
+    // runs on (a following csg_finetune queues behind it) and the read-back
+    // is folded by the next call that synchronises
+    if (first_labels || labels_mask || scores)
+        return fold_score_step(eng);
     return CSG_OK;
 }
 
@@ -3904,11 +3937,25 @@ int finetune_host(csg_engine* eng, float quantile, float finetune_thre, int fine
 
 extern "C" {
 
+int finetune_entry(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
+                   uint32_t* final_labels, int32_t* rounds_out);
+
 int csg_finetune(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
                  uint32_t* final_labels, int32_t* rounds_out)
 {
     if (!eng)
         return CSG_ERR_ARGUMENT;
+    // a score step left in flight runs ahead of this call's work on the
+    // stream; its read-back is folded once the fine-tune has synchronised
+    int       rc  = finetune_entry(eng, quantile, finetune_thre, finetune_times, final_labels,
+                                   rounds_out);
+    const int rc2 = fold_score_step(eng);
+    return rc ? rc : rc2;
+}
+
+int finetune_entry(csg_engine* eng, float quantile, float finetune_thre, int finetune_times,
+                   uint32_t* final_labels, int32_t* rounds_out)
+{
     if (!eng->ft_loaded)
         return eng->fail(CSG_ERR_STATE, "csg_finetune before csg_copyin");
     if (!eng->fq_ready)
@@ -4919,6 +4966,7 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset)
     if (!eng)
         return CSG_ERR_ARGUMENT;
     (void)hipStreamSynchronize(eng->stream);
+    int rc = fold_score_step(eng);
     collect_phases(eng);
     for (int i = 0; i < CSG_N_PHASES; ++i)
     {
@@ -4932,15 +4980,16 @@ int csg_phase_times(csg_engine* eng, double* ms, int64_t* counts, int reset)
             eng->phase_cnt[i] = 0;
         }
     }
-    return CSG_OK;
+    return rc;
 }
 
 int csg_counters(csg_engine* eng, int64_t* out)
 {
     if (!eng || !out)
         return CSG_ERR_ARGUMENT;
+    const int rc = fold_score_step(eng);
     std::memcpy(out, eng->counters, sizeof(eng->counters));
-    return CSG_OK;
+    return rc;
 }
 
 // The top-k kernels: k_topk_rows (one thread per row, L <= 64) against
