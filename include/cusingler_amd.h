@@ -76,6 +76,10 @@ int csg_copyin_score(csg_engine* eng, uint32_t ref_height, uint32_t width,
  * `quantile` (a fraction, 0.8 by default) of those scores, the first label
  * (last maximum) and the candidate mask (score >= max - finetune_thre).
  * Outputs may be NULL; results stay on the device for csg_finetune either way.
+ * With all three NULL the call returns once the step is queued on the
+ * engine's stream; its counters and the CSG_ERR_UNDEFINED of a NaN-carrying
+ * score segment are then reported by the next csg_finetune, csg_counters,
+ * csg_phase_times or csg_get_label.
  *   first_labels : int32 [qry_height]
  *   labels_mask  : float [qry_height * ct_num] (0/1, the reference's `labels`)
  *   scores       : float [qry_height * ct_num] per-label quantile scores */
