@@ -3718,14 +3718,23 @@ int finetune_device(csg_engine* eng, float quantile, float thre, int times,
     k_ftd_final<<<nblk(Nq), 256, 0, eng->stream>>>(eng->d_fl_lists, eng->d_fl_counts, Nq, L,
                                                   eng->d_fl_final);
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(final_labels, eng->d_fl_final, (size_t)Nq * 4, hipMemcpyDeviceToHost,
+    // the labels and round counts come back through page-locked memory (one
+    // DMA each) and are copied out on the host: a copy into the caller's
+    // pageable memory is staged by the runtime piece by piece
+    const size_t nb = (size_t)Nq * 4;
+    if ((rc = ensure_host(eng, &eng->h_ft_out, &eng->h_ft_out_cap, 2 * nb)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(eng->h_ft_out, eng->d_fl_final, nb, hipMemcpyDeviceToHost,
                           eng->stream));
     if (rounds_out)
-        HIPCHK(hipMemcpyAsync(rounds_out, eng->d_fl_rounds, (size_t)Nq * 4,
-                              hipMemcpyDeviceToHost, eng->stream));
+        HIPCHK(hipMemcpyAsync(eng->h_ft_out + nb, eng->d_fl_rounds, nb, hipMemcpyDeviceToHost,
+                              eng->stream));
     HIPCHK(hipMemcpyAsync(&h, eng->d_fl_cnt, sizeof(FtdCounts), hipMemcpyDeviceToHost,
                           eng->stream));
     HIPCHK(hipStreamSynchronize(eng->stream));
+    std::memcpy(final_labels, eng->h_ft_out, nb);
+    if (rounds_out)
+        std::memcpy(rounds_out, eng->h_ft_out + nb, nb);
     eng->counters[11] += h.tot_redo;
     add_totals();
     if (tail && eng->h_tt_state->timeout)
