@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--finetune-chain", type=int, default=None,
                     help="longest chain of fine-tune rounds launched without a host read-back "
                          "(0 = none; default: the engine's, 0)")
+    ap.add_argument("--finetune-tail", type=int, default=None,
+                    help="most active cells of a fine-tune round handed to the persistent tail "
+                         "kernel (0 = never; default: the engine's, 64)")
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--hidden", type=int, default=512,
                     help="width of the proposal network's two hidden layers (multiples of 256 take the pipelined layer kernel)")
@@ -162,6 +165,8 @@ def main():
         eng.set_finetune_fuse(args.finetune_fuse)
     if args.finetune_chain is not None:
         eng.set_finetune_chain(args.finetune_chain)
+    if args.finetune_tail is not None:
+        eng.set_finetune_tail(args.finetune_tail)
     training = None
     if cached:
         net = np.load(args.net_cache, allow_pickle=False)
